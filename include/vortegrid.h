@@ -1,0 +1,314 @@
+/* vortegrid.h -- C-ABI of the MI355X-native velocity-from-vorticity path.
+ *
+ * Drop-in boundary for VorteGrid's VortonSim path (mijagourlay/VorteGrid,
+ * Samples/MjgIntelFluidDemo_Part20/): direct Biot-Savart, the NestedGrid
+ * influence treecode, the vector-Poisson multigrid solve with treecode
+ * Dirichlet boundaries, and trilinear grid interpolation for tracer
+ * advection.  Every entry point names the reference routine it replaces.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Particle/grid buffers passed to vg_*
+ *    compute calls are DEVICE pointers owned by the caller (allocate them
+ *    with vg_malloc or any HIP allocator); the library never keeps them
+ *    beyond the call except where noted (vg_sim_set_vortons mirrors
+ *    VortonSim::SetVortons, vortonSim.h:329, and keeps the view).
+ *  - Vec3 arrays are "strided float triples": element i starts at
+ *    ptr[i * stride] (stride in floats).  stride 3 = packed Vec3 array;
+ *    stride 18 = the 72-byte Particle record (particle.h:425-446) with
+ *    pos at float offset 0, velocity at 3, angular velocity at 9, size at 13.
+ *  - Work is enqueued on the given stream (vg_stream, a hipStream_t; NULL =
+ *    the legacy default stream) and is asynchronous unless the function says
+ *    it returns host data.
+ *  - Status codes instead of the reference's debug ASSERTs: empty inputs are
+ *    VG_OK with a zero field (vortonSim.cpp:2668-2675); zero-extent grids are
+ *    a no-op (pclOpAdvect.cpp:138-142); out-of-domain positions are clamped
+ *    to the boundary cell (the reference's behaviour there is undefined).
+ */
+#ifndef VORTEGRID_H
+#define VORTEGRID_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum vg_status {
+    VG_OK = 0,
+    VG_EINVAL = 1,   /* bad argument (null pointer, bad size, bad enum) */
+    VG_EMPTY = 2,    /* reserved: nothing to do is reported as VG_OK */
+    VG_EDEVICE = 3,  /* HIP runtime error; see vg_last_error() */
+    VG_ENOMEM = 4,   /* device allocation failed */
+    VG_ESTATE = 5    /* call order violated (e.g. no tree built yet) */
+} vg_status;
+
+typedef void *vg_stream; /* hipStream_t */
+
+/* Rsqrt policy of the pair kernels.  REFERENCE reproduces finvsqrtf
+ * (Core/Math/math.h:206-216, one Newton step, <=1.75e-3 rel error) for
+ * parity; ACCURATE uses the hardware reciprocal square root (~1 ulp). */
+enum { VG_RSQRT_REFERENCE = 0, VG_RSQRT_ACCURATE = 1 };
+
+/* Velocity-from-vorticity technique (vorton.h:37-47). */
+enum {
+    VG_TECH_DIRECT = 0,   /* VELOCITY_TECHNIQUE_DIRECT */
+    VG_TECH_TREE = 1,     /* VELOCITY_TECHNIQUE_TREE */
+    VG_TECH_POISSON = 2   /* VELOCITY_TECHNIQUE_POISSON_GAUSS_SEIDEL */
+};
+
+/* Grid geometry: UniformGridGeometry (uniformGrid.h:144-860).  Points are
+ * x-fastest: offset = ix + np[0]*(iy + np[1]*iz) (uniformGrid.h:756-759). */
+typedef struct vg_grid_desc {
+    float min[3];     /* mMinCorner */
+    float extent[3];  /* mGridExtent */
+    float cell[3];    /* mCellExtent (spacing) */
+    float cpe[3];     /* mCellsPerExtent */
+    uint32_t np[3];   /* mNumPoints */
+} vg_grid_desc;
+
+/* Device view of a vorton array (Vorton = Particle, vorton.h:198-387). */
+typedef struct vg_vortons {
+    const float *pos;     /* strided positions */
+    const float *angvel;  /* strided angular velocity = vorticity / 2 */
+    const float *size;    /* strided diameters (mSize) or NULL */
+    size_t stride;        /* floats between consecutive records */
+    size_t n;
+    float uniform_size;   /* diameter used when size == NULL */
+} vg_vortons;
+
+/* ------------------------------------------------------------ runtime */
+const char *vg_last_error(void);
+const char *vg_version(void);
+vg_status vg_device_count(int *count);
+vg_status vg_malloc(void **ptr, size_t bytes);
+vg_status vg_free(void *ptr);
+vg_status vg_memcpy_htod(void *dst, const void *src, size_t bytes);
+vg_status vg_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+vg_status vg_memset(void *dst, int value, size_t bytes);
+vg_status vg_stream_create(vg_stream *s);
+vg_status vg_stream_destroy(vg_stream s);
+vg_status vg_stream_synchronize(vg_stream s);
+/* Kernel timing: when enabled, every kernel the library launches is
+ * bracketed by HIP events on its own stream and its duration is tallied
+ * by kernel name (read with vg_profile_get after a synchronize). */
+vg_status vg_profile_enable(int on);
+vg_status vg_profile_reset(void);
+vg_status vg_profile_get(const char *kernel, double *total_ms, uint64_t *count);
+/* Writes "name count total_ms\n" lines into buf (NUL-terminated). */
+vg_status vg_profile_dump(char *buf, size_t cap);
+
+/* ------------------------------------------------------------ geometry */
+/* UniformGridGeometry::DefineShape (uniformGrid.h:259-322). */
+vg_status vg_define_shape(vg_grid_desc *g, size_t num_elements,
+                          const float vmin[3], const float vmax[3], int pow2);
+/* UniformGridGeometry::Decimate (uniformGrid.h:361-376). */
+vg_status vg_decimate(vg_grid_desc *dst, const vg_grid_desc *src, uint32_t dec);
+/* NestedGrid::Initialize + PrecomputeDecimations (nestedGrid.h:69-85,
+ * 312-391): writes up to max_layers layers and per-parent decimations
+ * (dec[3*l + a]; dec of layer 0 is 0). */
+vg_status vg_nested_layers(const vg_grid_desc *src, vg_grid_desc *layers,
+                           uint32_t *dec, uint32_t max_layers, uint32_t *depth);
+/* VortonSim::UpdateBoundingBox(min, max, bFinal=true) (vortonSim.cpp:
+ * 1027-1096): pads the box by 3 vorton radii and calls DefineShape(n, ...,
+ * pow2=true).  mn/mx are updated in place. */
+vg_status vg_finalize_bounding_box(float mn[3], float mx[3], float vorton_radius,
+                                   size_t num_vortons, vg_grid_desc *grid_template);
+/* Auto-mollification (vortonSim.cpp:2458-2477). */
+vg_status vg_auto_mollify(const vg_grid_desc *g, float vorton_radius,
+                          float *spread, float *circ);
+
+/* ------------------------------------------------------------ AABB */
+/* PclOpFindBoundingBox::FindBoundingBox (pclOpFindBoundingBox.cpp:20-148):
+ * grows mn/mx (host arrays, in/out) by the n strided positions.  Blocks
+ * until the result is on the host. */
+vg_status vg_find_bounding_box(const float *pos, size_t stride, size_t n,
+                               float mn[3], float mx[3], vg_stream s);
+
+/* ------------------------------------------------------------ direct sums */
+/* VortonSim::ComputeVelocity_Direct (vortonSim.cpp:1144-1158) evaluated at
+ * nq strided query points; out is packed xyz (3*nq floats).  FP32 with
+ * tile-compensated (Kahan) accumulation. */
+vg_status vg_direct_velocity(const vg_vortons *v, const float *q, size_t q_stride,
+                             size_t nq, float spread, float circ, int rsqrt_policy,
+                             float *out, vg_stream s);
+/* VortonSim::ComputeVectorPotential_Direct (vortonSim.cpp:1113-1127). */
+vg_status vg_direct_vecpot(const vg_vortons *v, const float *q, size_t q_stride,
+                           size_t nq, float spread, float circ, int rsqrt_policy,
+                           float *out, vg_stream s);
+/* Direct velocity at every grid point (ComputeVelocityAtGridpoints_Slice,
+ * DIRECT branch, vortonSim.cpp:2223-2293, with the 1-2eps spacing nudge). */
+vg_status vg_direct_velocity_grid(const vg_vortons *v, const vg_grid_desc *g,
+                                  float spread, float circ, int rsqrt_policy,
+                                  float *out, vg_stream s);
+/* FP64 accuracy reference on the GPU: same regularised law, exact
+ * reciprocal square root, double accumulation.  out: 3*nq doubles. */
+vg_status vg_direct_velocity_f64(const vg_vortons *v, const float *q,
+                                 size_t q_stride, size_t nq, float spread,
+                                 float circ, double *out, vg_stream s);
+
+/* ------------------------------------------------------------ influence tree */
+typedef struct vg_tree vg_tree;
+vg_status vg_tree_create(vg_tree **t);
+vg_status vg_tree_destroy(vg_tree *t);
+/* VortonSim::CreateInfluenceTree (vortonSim.cpp:1605-1624) =
+ * NestedGrid::Initialize + MakeBaseVortonGrid (:1192-1321) +
+ * AggregateClusters (:1337-1569).  Cluster sums are accumulated in vorton
+ * index order per cell, so layers match the reference bit for bit. */
+vg_status vg_tree_build(vg_tree *t, const vg_grid_desc *grid_template,
+                        const vg_vortons *v, vg_stream s);
+vg_status vg_tree_depth(const vg_tree *t, uint32_t *depth);
+/* Copies layer l to the host: grid descriptor, and (optional, may be NULL)
+ * cluster position[3*cap], angular velocity[3*cap], size[cap].  Blocks. */
+vg_status vg_tree_layer(const vg_tree *t, uint32_t l, vg_grid_desc *g,
+                        float *pos, float *angvel, float *size);
+/* ComputeVelocity_Tree from the root (vortonSim.cpp:1787-1905) at nq
+ * strided query points. */
+vg_status vg_tree_velocity(const vg_tree *t, const float *q, size_t q_stride,
+                           size_t nq, float spread, float circ,
+                           int rsqrt_policy, float *out, vg_stream s);
+/* ComputeVelocityAtGridpoints_Slice, TREE branch (vortonSim.cpp:2223-2293):
+ * every point of grid g (direct sum when the tree has one layer). */
+vg_status vg_tree_velocity_grid(const vg_tree *t, const vg_grid_desc *g,
+                                float spread, float circ, int rsqrt_policy,
+                                float *out, vg_stream s);
+/* Number of cluster interactions vg_tree_velocity_grid performs on grid g
+ * (same walk, no pair arithmetic): the work measure for pair-rate and
+ * FLOP-rate accounting.  Blocks until the count is on the host. */
+vg_status vg_tree_interaction_count(const vg_tree *t, const vg_grid_desc *g,
+                                    uint64_t *count, vg_stream s);
+/* ComputeVectorPotential_Tree (vortonSim.cpp:1650-1768) at query points. */
+vg_status vg_tree_vecpot(const vg_tree *t, const float *q, size_t q_stride,
+                         size_t nq, float spread, float circ, int rsqrt_policy,
+                         float *out, vg_stream s);
+/* ComputeVectorPotentialAtGridpoints_Slice(boundariesOnly=true)
+ * (vortonSim.cpp:2055-2138): writes the 6 faces of out (interior left
+ * untouched).  copy_odd=1 reproduces the reference (odd x copies x-1);
+ * copy_odd=0 computes every face point. */
+vg_status vg_tree_vecpot_boundary(const vg_tree *t, const vg_grid_desc *g,
+                                  float spread, float circ, int rsqrt_policy,
+                                  int copy_odd, float *out, vg_stream s);
+/* PopulateVorticityGridFromVortons(grid, scale) (vortonSim.cpp:2372-2436):
+ * trilinear splat of vorticity*scale*V_vorton/V_cell onto the tree's leaf
+ * grid geometry (= the grid template).  out: 3*capacity floats. */
+vg_status vg_tree_splat_vorticity(const vg_tree *t, float vorton_radius,
+                                  float scale, float *out, vg_stream s);
+/* Particles::PopulateVelocityGrid (pclOpPopulateVelocityGrid.cpp:33-87):
+ * velocities vel (strided, indexed like the vortons given to
+ * vg_tree_build) averaged onto the leaf grid. */
+vg_status vg_tree_populate_velocity_grid(const vg_tree *t, const float *vel,
+                                         size_t vel_stride, float *out,
+                                         vg_stream s);
+
+/* ------------------------------------------------------------ grid operators */
+/* UniformGrid::DownSample, FASTER_LESS_ACCURATE injection
+ * (uniformGrid.h:1453-1512). */
+vg_status vg_downsample_inject(const vg_grid_desc *lo, float *lo_field,
+                               const vg_grid_desc *hi, const float *hi_field,
+                               vg_stream s);
+/* UniformGrid::UpSample(INTERIOR_ONLY) (uniformGrid.h:1637-1700): overwrites
+ * interior fine points with trilinear samples of the coarse grid. */
+vg_status vg_upsample_interior(const vg_grid_desc *hi, float *hi_field,
+                               const vg_grid_desc *lo, const float *lo_field,
+                               vg_stream s);
+/* SolveVectorPoisson(soln, lap, steps, BC_DIRICHLET) (uniformGridMath.cpp:
+ * 1369-1405): steps x (red zebra pass, black zebra pass) of SOR 1.72 with
+ * x sequential in place (:1116-1188). */
+vg_status vg_sor_solve(const vg_grid_desc *g, float *soln, const float *lap,
+                       uint32_t steps, vg_stream s);
+/* ComputeJacobian + ComputeCurlFromJacobian (uniformGridMath.cpp:207-451),
+ * fused: no Mat33 grid is materialised. */
+vg_status vg_curl(const vg_grid_desc *g, const float *a, float *curl, vg_stream s);
+/* FP64 relative residual ||f - L u|| / ||f|| over interior points
+ * (7-point Laplacian).  Blocks until the result is on the host. */
+vg_status vg_residual_norm(const vg_grid_desc *g, const float *u, const float *f,
+                           double *rel, vg_stream s);
+
+/* ------------------------------------------------------------ multigrid */
+typedef struct vg_multigrid vg_multigrid;
+vg_status vg_mg_create(vg_multigrid **m);
+vg_status vg_mg_destroy(vg_multigrid *m);
+/* VortonSim::ComputeVectorPotential multigrid cascade (vortonSim.cpp:
+ * 2523-2571): a0 holds the Dirichlet boundary on entry (its interior is
+ * reset to 0, the cold start of :2531); negw0 = -vorticity.  Injection
+ * restriction, replace-prolongation, steps_per_level SOR steps per visit. */
+vg_status vg_mg_cascade(vg_multigrid *m, const vg_grid_desc *g, float *a0,
+                        const float *negw0, uint32_t steps_per_level,
+                        vg_stream s);
+/* Residual-correction V-cycles (red-black Gauss-Seidel smoothing, full
+ * weighting restriction, trilinear correction, FP64 coarsest solve and
+ * FP64 residual norms) until ||f-Au||/||f|| <= tol or max_cycles.
+ * Boundary values of a0 are kept (Dirichlet); the interior initial guess is
+ * whatever a0 holds.  Reports the final relative residual and cycles. */
+vg_status vg_mg_vcycle_solve(vg_multigrid *m, const vg_grid_desc *g, float *a0,
+                             const float *negw0, double tol, uint32_t max_cycles,
+                             double *rel_residual, uint32_t *cycles, vg_stream s);
+
+/* ------------------------------------------------------------ particles */
+/* AssignVelocityFromField (pclOpAdvect.cpp:132-152, slice :42-81): vel =
+ * interp (gain == 1) or vel += gain*(interp - vel). */
+vg_status vg_assign_velocity_from_field(const vg_grid_desc *g, const float *field,
+                                        const float *pos, float *vel,
+                                        size_t stride, size_t n, float gain,
+                                        vg_stream s);
+/* PclOpEvolve (pclOpEvolve.cpp:36-123): pos += vel * dt. */
+vg_status vg_evolve(float *pos, const float *vel, size_t stride, size_t n,
+                    float dt, vg_stream s);
+/* Fused AssignVelocityFromField + Evolve for tracers: one read and one write
+ * of each position.  vel may be NULL (velocity not stored; gain must then be
+ * 1), otherwise it is read when gain != 1 and written. */
+vg_status vg_advect_tracers(const vg_grid_desc *g, const float *field, float *pos,
+                            float *vel, size_t stride, size_t n, float dt,
+                            float gain, vg_stream s);
+
+/* ------------------------------------------------------------ simulation */
+/* vg_sim = VortonSim (vortonSim.h:137-623) with device-resident state:
+ * grid template, influence tree, multigrid pyramids and velocity grid. */
+typedef struct vg_sim vg_sim;
+typedef struct vg_sim_config {
+    int technique;       /* VG_TECH_*; reference default is POISSON */
+    int rsqrt_policy;    /* VG_RSQRT_* */
+    int auto_mollify;    /* 1: ENABLE_AUTO_MOLLIFICATION for TREE */
+    uint32_t mg_steps;   /* SOR steps per level (reference: 16) */
+    int boundary_copy_odd; /* 1: odd-x face points copy x-1 (reference) */
+    int poisson_solver;  /* 0: reference cascade, 1: V-cycles to mg_tol */
+    double mg_tol;
+    uint32_t mg_max_cycles;
+} vg_sim_config;
+vg_status vg_sim_config_default(vg_sim_config *cfg);
+vg_status vg_sim_create(vg_sim **sim, const vg_sim_config *cfg, vg_stream s);
+vg_status vg_sim_destroy(vg_sim *sim);
+/* VortonSim::SetVortons (vortonSim.h:329): keeps the device view. */
+vg_status vg_sim_set_vortons(vg_sim *sim, const vg_vortons *v);
+/* VortonSim::FindBoundingBox (vortonSim.cpp:1001-1008). */
+vg_status vg_sim_find_bounding_box(vg_sim *sim);
+/* VortonSim::UpdateBoundingBox (vortonSim.cpp:1027-1096). */
+vg_status vg_sim_update_bounding_box(vg_sim *sim, const float mn[3],
+                                     const float mx[3], int final_update);
+/* VortonSim::Update -> UpdateVortexParticleMethod (vortonSim.cpp:4117-4256),
+ * velocity-from-vorticity part: influence tree, vorticity splat (POISSON),
+ * velocity grid, then AssignVelocityFromField(vortons, grid, 1) into
+ * vel_out (strided like the vortons; NULL skips it). */
+vg_status vg_sim_update(vg_sim *sim, float dt, uint32_t frame, float *vel_out,
+                        size_t vel_stride);
+/* VortonSim::GetVelocityGrid (vortonSim.h:334): device pointer valid until
+ * the next update. */
+vg_status vg_sim_velocity_grid(const vg_sim *sim, vg_grid_desc *g,
+                               const float **field);
+/* GetVectorPotentialMultiGrid()[0] / GetNegativeVorticityMultiGrid()[0]. */
+vg_status vg_sim_vector_potential(const vg_sim *sim, vg_grid_desc *g,
+                                  const float **field);
+vg_status vg_sim_negative_vorticity(const vg_sim *sim, vg_grid_desc *g,
+                                    const float **field);
+vg_status vg_sim_grid_template(const vg_sim *sim, vg_grid_desc *g);
+vg_status vg_sim_bounds(const vg_sim *sim, float mn[3], float mx[3]);
+vg_status vg_sim_tree(const vg_sim *sim, const vg_tree **t);
+/* Mollification factors used by the last update. */
+vg_status vg_sim_spreading(const vg_sim *sim, float *spread, float *circ);
+/* FP64 true relative residual of the last Poisson solve (POISSON only). */
+vg_status vg_sim_poisson_residual(const vg_sim *sim, double *rel);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

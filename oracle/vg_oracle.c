@@ -1,0 +1,888 @@
+/* vg_oracle.c -- CPU restatement of VorteGrid's velocity-from-vorticity path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product).  Compiled with
+ * -ffp-contract=off so every float operation rounds as the reference source
+ * text orders it (x86-64 SSE, FLT_EVAL_METHOD 0).  Citations are relative to
+ * Samples/MjgIntelFluidDemo_Part20/.
+ */
+#include "vg_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+static const float kTwoThirds = 2.0f / 3.0f;           /* vorton.h:95 */
+static const float kPi = 3.1415926535897932384626433832795f; /* particle.h:67 */
+
+/* ------------------------------------------------------------------------ */
+/* parallel-for over [0,n) in contiguous chunks (stands in for TBB's
+ * parallel_for with grain n/threads; every item is written by one worker,
+ * so results do not depend on the thread count). */
+typedef void (*range_fn)(void *ctx, size_t b, size_t e);
+typedef struct { range_fn fn; void *ctx; size_t b, e; } range_job;
+static void *range_thread(void *a) {
+    range_job *j = (range_job *)a;
+    j->fn(j->ctx, j->b, j->e);
+    return NULL;
+}
+static void parallel_range(size_t n, int nthreads, range_fn fn, void *ctx) {
+    if (nthreads <= 1 || n < 2) { fn(ctx, 0, n); return; }
+    if ((size_t)nthreads > n) nthreads = (int)n;
+    pthread_t th[256];
+    range_job jobs[256];
+    if (nthreads > 256) nthreads = 256;
+    size_t chunk = (n + nthreads - 1) / nthreads;
+    int used = 0;
+    for (int t = 0; t < nthreads; ++t) {
+        size_t b = (size_t)t * chunk, e = b + chunk;
+        if (b >= n) break;
+        if (e > n) e = n;
+        jobs[t].fn = fn; jobs[t].ctx = ctx; jobs[t].b = b; jobs[t].e = e;
+        pthread_create(&th[t], NULL, range_thread, &jobs[t]);
+        ++used;
+    }
+    for (int t = 0; t < used; ++t) pthread_join(th[t], NULL);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Core/Math/math.h:206-216.  The reference reads the float through `long&`,
+ * which is 32-bit on its LLP64 target; restated with int32_t. */
+float vgo_finvsqrtf(float val) {
+    int32_t i;
+    memcpy(&i, &val, 4);
+    i = 0x5f3759df - (i >> 1);
+    float y;
+    memcpy(&y, &i, 4);
+    const float f = 1.5f;
+    const float x = val * 0.5f;
+    y = y * (f - ((x * y) * y));
+    return y;
+}
+
+/* Core/Math/math.h:149-175 */
+uint32_t vgo_nearest_pow2(uint32_t v) {
+    if (v == 0) return 1u;
+    --v;
+    unsigned shift = 0;
+    while ((v >> shift) != 0) ++shift;
+    return 1u << shift;
+}
+
+/* vorton.h:161-179 VORTON_ACCUMULATE_VELOCITY_private */
+void vgo_accumulate_velocity(float acc[3], const float q[3], const float p[3],
+                             const float om[3], float size, float spread,
+                             float circ, int rsqrt_policy) {
+    const float dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
+    const float radius = size * 0.5f * spread;
+    const float radius2 = radius * radius;
+    const float dist2 = dx * dx + dy * dy + dz * dz;
+    float law;
+    if (dist2 < radius2) {
+        law = 1.0f / (radius2 * radius);
+    } else {
+        const float rs = rsqrt_policy ? 1.0f / sqrtf(dist2) : vgo_finvsqrtf(dist2);
+        law = rs / dist2;
+    }
+    /* (TwoThirds * radius2 * radius * Omega) ^ (d * law) */
+    const float s = kTwoThirds * radius2 * radius;
+    const float ax = om[0] * s, ay = om[1] * s, az = om[2] * s;
+    const float bx = dx * law, by = dy * law, bz = dz * law;
+    const float cx = ay * bz - az * by;
+    const float cy = az * bx - ax * bz;
+    const float cz = ax * by - ay * bx;
+    acc[0] += cx * circ;
+    acc[1] += cy * circ;
+    acc[2] += cz * circ;
+}
+
+/* vorton.h:118-136 VORTON_ACCUMULATE_VECTOR_POTENTIAL_private */
+void vgo_accumulate_vecpot(float acc[3], const float q[3], const float p[3],
+                           const float om[3], float size, float spread,
+                           float circ, int rsqrt_policy) {
+    const float dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
+    const float radius = size * 0.5f * spread;
+    const float radius2 = radius * radius;
+    const float dist2 = dx * dx + dy * dy + dz * dz;
+    float law;
+    if (dist2 < radius2) {
+        law = dist2 / (radius2 * radius);
+    } else {
+        law = rsqrt_policy ? 1.0f / sqrtf(dist2) : vgo_finvsqrtf(dist2);
+    }
+    const float s = kTwoThirds * radius2 * radius;
+    acc[0] += ((om[0] * s) * law) * circ;
+    acc[1] += ((om[1] * s) * law) * circ;
+    acc[2] += ((om[2] * s) * law) * circ;
+}
+
+/* ------------------------------------------------------------------------ */
+/* vortonSim.cpp:1113-1158: serial sum over vortons 0..N-1 per query. */
+typedef struct {
+    const float *pos, *om, *size, *q;
+    size_t n;
+    float spread, circ;
+    int policy, vecpot;
+    float *out;
+    double *out64;
+} direct_ctx;
+
+static void direct_range(void *vc, size_t b, size_t e) {
+    direct_ctx *c = (direct_ctx *)vc;
+    for (size_t iq = b; iq < e; ++iq) {
+        float acc[3] = {0.f, 0.f, 0.f};
+        const float *q = c->q + 3 * iq;
+        for (size_t i = 0; i < c->n; ++i) {
+            if (c->vecpot)
+                vgo_accumulate_vecpot(acc, q, c->pos + 3 * i, c->om + 3 * i,
+                                      c->size[i], c->spread, c->circ, c->policy);
+            else
+                vgo_accumulate_velocity(acc, q, c->pos + 3 * i, c->om + 3 * i,
+                                        c->size[i], c->spread, c->circ, c->policy);
+        }
+        c->out[3 * iq] = acc[0];
+        c->out[3 * iq + 1] = acc[1];
+        c->out[3 * iq + 2] = acc[2];
+    }
+}
+
+void vgo_direct_velocity(const float *pos, const float *om, const float *size,
+                         size_t n, const float *q, size_t nq, float spread,
+                         float circ, int rsqrt_policy, float *out, int nthreads) {
+    direct_ctx c = {pos, om, size, q, n, spread, circ, rsqrt_policy, 0, out, NULL};
+    parallel_range(nq, nthreads, direct_range, &c);
+}
+
+void vgo_direct_vecpot(const float *pos, const float *om, const float *size,
+                       size_t n, const float *q, size_t nq, float spread,
+                       float circ, int rsqrt_policy, float *out, int nthreads) {
+    direct_ctx c = {pos, om, size, q, n, spread, circ, rsqrt_policy, 1, out, NULL};
+    parallel_range(nq, nthreads, direct_range, &c);
+}
+
+/* FP64 accuracy oracle: the same regularised law as vorton.h:161-179 evaluated
+ * in double with an exact reciprocal square root. */
+static void direct64_range(void *vc, size_t b, size_t e) {
+    direct_ctx *c = (direct_ctx *)vc;
+    for (size_t iq = b; iq < e; ++iq) {
+        double ax = 0, ay = 0, az = 0;
+        const double qx = c->q[3 * iq], qy = c->q[3 * iq + 1], qz = c->q[3 * iq + 2];
+        for (size_t i = 0; i < c->n; ++i) {
+            const double dx = qx - c->pos[3 * i], dy = qy - c->pos[3 * i + 1],
+                         dz = qz - c->pos[3 * i + 2];
+            const double r = (double)c->size[i] * 0.5 * (double)c->spread;
+            const double r2 = r * r;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            const double law = d2 < r2 ? 1.0 / (r2 * r) : 1.0 / (d2 * sqrt(d2));
+            const double s = (2.0 / 3.0) * r2 * r * (double)c->circ;
+            const double wx = c->om[3 * i] * s, wy = c->om[3 * i + 1] * s,
+                         wz = c->om[3 * i + 2] * s;
+            ax += (wy * dz - wz * dy) * law;
+            ay += (wz * dx - wx * dz) * law;
+            az += (wx * dy - wy * dx) * law;
+        }
+        c->out64[3 * iq] = ax;
+        c->out64[3 * iq + 1] = ay;
+        c->out64[3 * iq + 2] = az;
+    }
+}
+
+void vgo_direct_velocity_f64(const float *pos, const float *om, const float *size,
+                             size_t n, const float *q, size_t nq, float spread,
+                             float circ, double *out, int nthreads) {
+    direct_ctx c = {pos, om, size, q, n, spread, circ, 1, 0, NULL, out};
+    parallel_range(nq, nthreads, direct64_range, &c);
+}
+
+/* ------------------------------------------------------------------------ */
+/* uniformGrid.h:794-809 PrecomputeSpacing */
+static void precompute_spacing(vgo_grid *g) {
+    for (int a = 0; a < 3; ++a)
+        g->cell[a] = g->extent[a] / (float)(g->np[a] - 1);
+    g->cpe[0] = (float)(g->np[0] - 1) / g->extent[0];
+    g->cpe[1] = (float)(g->np[1] - 1) / g->extent[1];
+    if (0.0f == g->extent[2])
+        g->cpe[2] = 1.0f / FLT_MIN;
+    else
+        g->cpe[2] = (float)(g->np[2] - 1) / g->extent[2];
+}
+
+/* uniformGrid.h:259-322 DefineShape */
+void vgo_define_shape(vgo_grid *g, size_t n, const float vmin[3],
+                      const float vmax[3], int pow2) {
+    memset(g, 0, sizeof(*g));
+    if (n == 0) return;
+    const float nudge = 1.0f + FLT_EPSILON;
+    float eff[3];
+    int ndims = 3;
+    for (int a = 0; a < 3; ++a) {
+        g->min[a] = vmin[a];
+        g->extent[a] = (vmax[a] - vmin[a]) * nudge;
+        eff[a] = g->extent[a];
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (0.0f == eff[a]) { eff[a] = 1.0f; g->extent[a] = 0.0f; --ndims; }
+    }
+    const float volume = eff[0] * eff[1] * eff[2];
+    const float icvcr = powf(volume / (float)n, -1.0f / (float)ndims);
+    size_t nc[3];
+    for (int a = 0; a < 3; ++a) {
+        size_t v = (size_t)(g->extent[a] * icvcr + 0.5f);
+        nc[a] = v > 1 ? v : 1;
+    }
+    if (pow2)
+        for (int a = 0; a < 3; ++a) nc[a] = vgo_nearest_pow2((uint32_t)nc[a]);
+    while (nc[0] * nc[1] * nc[2] >= n * 8) {
+        for (int a = 0; a < 3; ++a) nc[a] = nc[a] / 2 > 1 ? nc[a] / 2 : 1;
+    }
+    for (int a = 0; a < 3; ++a) g->np[a] = (uint32_t)(nc[a] + 1);
+    precompute_spacing(g);
+}
+
+/* uniformGrid.h:361-376 Decimate */
+void vgo_decimate(vgo_grid *dst, const vgo_grid *src, uint32_t dec) {
+    vgo_grid d;
+    memset(&d, 0, sizeof(d));
+    for (int a = 0; a < 3; ++a) {
+        d.extent[a] = src->extent[a];
+        d.min[a] = src->min[a];
+        d.np[a] = (src->np[a] - 1) / dec + 1;
+        if (dec > 1 && d.np[a] < 2) d.np[a] = 2;
+    }
+    precompute_spacing(&d);
+    *dst = d;
+}
+
+size_t vgo_grid_capacity(const vgo_grid *g) {
+    return (size_t)g->np[0] * g->np[1] * g->np[2];
+}
+
+void vgo_grid_query_point(const vgo_grid *g, uint32_t ix, uint32_t iy,
+                          uint32_t iz, float out[3]) {
+    const float nudge = 1.0f - 2.0f * FLT_EPSILON;
+    const uint32_t idx[3] = {ix, iy, iz};
+    for (int a = 0; a < 3; ++a)
+        out[a] = g->min[a] + (float)idx[a] * (g->cell[a] * nudge);
+}
+
+/* ------------------------------------------------------------------------ */
+/* pclOpFindBoundingBox.cpp:20-148 (serial form; min/max are exact). */
+void vgo_find_bbox(const float *pos, size_t stride, size_t n, float mn[3],
+                   float mx[3]) {
+    for (size_t i = 0; i < n; ++i) {
+        const float *p = pos + i * stride;
+        for (int a = 0; a < 3; ++a) {
+            if (p[a] < mn[a]) mn[a] = p[a];
+            if (p[a] > mx[a]) mx[a] = p[a];
+        }
+    }
+}
+
+/* vortonSim.cpp:1027-1096 UpdateBoundingBox(..., bFinal=true) */
+void vgo_finalize_bbox(float mn[3], float mx[3], float vorton_radius,
+                       size_t n_vortons, vgo_grid *grid_template) {
+    if (n_vortons == 0) {
+        const float z[3] = {0.f, 0.f, 0.f};
+        vgo_define_shape(grid_template, 0, z, z, 1);
+        return;
+    }
+    const float margin = (1.0f + 2.0f) * vorton_radius;
+    const float nudge = margin * 1.0f;
+    for (int a = 0; a < 3; ++a) { mn[a] -= nudge; mx[a] += nudge; }
+    vgo_define_shape(grid_template, n_vortons, mn, mx, 1);
+}
+
+/* ------------------------------------------------------------------------ */
+/* nestedGrid.h:69-85 Initialize + PrecomputeDecimations (:371-391) */
+void vgo_nested_init(vgo_nested *ng, const vgo_grid *src) {
+    memset(ng, 0, sizeof(*ng));
+    vgo_decimate(&ng->layer[0], src, 1);
+    uint32_t idx = 1;
+    while (vgo_grid_capacity(&ng->layer[idx - 1]) > 8 && idx < VGO_MAX_LAYERS) {
+        vgo_decimate(&ng->layer[idx], &ng->layer[idx - 1], 2);
+        ++idx;
+    }
+    ng->depth = idx;
+    for (uint32_t l = 1; l < idx; ++l)
+        for (int a = 0; a < 3; ++a)
+            ng->dec[l][a] = (ng->layer[l - 1].np[a] - 1) / (ng->layer[l].np[a] - 1);
+}
+
+/* uniformGrid.h:536-574 IndicesOfPosition (truncation). */
+static void indices_of_position(const vgo_grid *g, const float p[3],
+                                uint32_t idx[3]) {
+    for (int a = 0; a < 3; ++a) {
+        const float rel = p[a] - g->min[a];
+        const float f = rel * g->cpe[a];
+        idx[a] = (uint32_t)(int32_t)f;
+    }
+}
+
+void vgo_indices_of_position(const vgo_grid *g, const float p[3], uint32_t idx[3]) {
+    indices_of_position(g, p, idx);
+}
+
+/* vortonSim.cpp:1192-1321 MakeBaseVortonGrid and :1337-1569 AggregateClusters */
+int vgo_tree_build(vgo_tree *t, const vgo_grid *grid_template,
+                   const float *pos, const float *om, const float *size,
+                   size_t n) {
+    memset(t, 0, sizeof(*t));
+    vgo_nested_init(&t->ng, grid_template);
+    for (uint32_t l = 0; l < t->ng.depth; ++l) {
+        const size_t cap = vgo_grid_capacity(&t->ng.layer[l]);
+        t->pos[l] = (float *)calloc(cap * 3, sizeof(float));
+        t->om[l] = (float *)calloc(cap * 3, sizeof(float));
+        t->size[l] = (float *)calloc(cap, sizeof(float));
+        if (!t->pos[l] || !t->om[l] || !t->size[l]) return -1;
+    }
+    const vgo_grid *g0 = &t->ng.layer[0];
+    const size_t cap0 = vgo_grid_capacity(g0);
+    float *aux = (float *)malloc(cap0 * sizeof(float));
+    for (size_t i = 0; i < cap0; ++i) aux[i] = FLT_MIN;
+    for (size_t v = 0; v < n; ++v) {
+        uint32_t idx[3];
+        indices_of_position(g0, pos + 3 * v, idx);
+        const size_t off = idx[0] + (size_t)g0->np[0] * (idx[1] + (size_t)g0->np[1] * idx[2]);
+        const float wx = om[3 * v] * 2.0f, wy = om[3 * v + 1] * 2.0f, wz = om[3 * v + 2] * 2.0f;
+        const float mag = sqrtf(wx * wx + wy * wy + wz * wz);
+        for (int a = 0; a < 3; ++a) t->pos[0][3 * off + a] += pos[3 * v + a] * mag;
+        for (int a = 0; a < 3; ++a) t->om[0][3 * off + a] += om[3 * v + a];
+        t->size[0][off] = size[v];
+        aux[off] += mag;
+    }
+    for (size_t off = 0; off < cap0; ++off) {
+        if (aux[off] != FLT_MIN) {
+            const float inv = 1.0f / aux[off];
+            for (int a = 0; a < 3; ++a) t->pos[0][3 * off + a] *= inv;
+        }
+    }
+    free(aux);
+    for (uint32_t pl = 1; pl < t->ng.depth; ++pl) {
+        const vgo_grid *gp = &t->ng.layer[pl], *gc = &t->ng.layer[pl - 1];
+        const uint32_t *dims = t->ng.dec[pl];
+        const uint32_t nc[3] = {gp->np[0] - 1, gp->np[1] - 1, gp->np[2] - 1};
+        for (uint32_t iz = 0; iz < nc[2]; ++iz)
+            for (uint32_t iy = 0; iy < nc[1]; ++iy)
+                for (uint32_t ix = 0; ix < nc[0]; ++ix) {
+                    const size_t po = ix + (size_t)gp->np[0] * (iy + (size_t)gp->np[1] * iz);
+                    float *pp = t->pos[pl] + 3 * po, *pw = t->om[pl] + 3 * po;
+                    float asum = FLT_MIN;
+                    for (uint32_t cz = 0; cz < dims[2]; ++cz)
+                        for (uint32_t cy = 0; cy < dims[1]; ++cy)
+                            for (uint32_t cx = 0; cx < dims[0]; ++cx) {
+                                const size_t co = (ix * dims[0] + cx) +
+                                    (size_t)gc->np[0] * ((iy * dims[1] + cy) +
+                                    (size_t)gc->np[1] * (iz * dims[2] + cz));
+                                const float *cw = t->om[pl - 1] + 3 * co;
+                                const float wx = cw[0] * 2.0f, wy = cw[1] * 2.0f, wz = cw[2] * 2.0f;
+                                const float mag = sqrtf(wx * wx + wy * wy + wz * wz);
+                                for (int a = 0; a < 3; ++a)
+                                    pp[a] += t->pos[pl - 1][3 * co + a] * mag;
+                                for (int a = 0; a < 3; ++a) pw[a] += cw[a];
+                                asum += mag;
+                                if (t->size[pl - 1][co] != 0.0f)
+                                    t->size[pl][po] = t->size[pl - 1][co];
+                            }
+                    const float inv = 1.0f / asum;
+                    for (int a = 0; a < 3; ++a) pp[a] *= inv;
+                }
+    }
+    return 0;
+}
+
+void vgo_tree_free(vgo_tree *t) {
+    for (uint32_t l = 0; l < VGO_MAX_LAYERS; ++l) {
+        free(t->pos[l]); free(t->om[l]); free(t->size[l]);
+        t->pos[l] = t->om[l] = t->size[l] = NULL;
+    }
+}
+
+/* vortonSim.cpp:1650-1768 / 1787-1905: recursive traversal.  A cluster of
+ * size 0 is empty (zero circulation); it is skipped, which leaves the sum
+ * unchanged except that a query exactly at the origin no longer produces
+ * 0/0 through the fast reciprocal square root. */
+static void tree_rec(const vgo_tree *t, const float q[3], const uint32_t ind[3],
+                     uint32_t ilayer, float spread, float circ, int policy,
+                     int vecpot, float out[3]) {
+    const vgo_grid *gc = &t->ng.layer[ilayer - 1];
+    const uint32_t *dims = t->ng.dec[ilayer];
+    const uint32_t cmin[3] = {ind[0] * dims[0], ind[1] * dims[1], ind[2] * dims[2]};
+    const float mf = 0.0001f;
+    float margin[3];
+    for (int a = 0; a < 3; ++a) margin[a] = mf * gc->cell[a];
+    if (0.0f == gc->cell[2]) margin[2] += FLT_MIN;
+    float acc[3] = {0.f, 0.f, 0.f};
+    for (uint32_t iz = 0; iz < dims[2]; ++iz) {
+        const uint32_t cz = cmin[2] + iz;
+        const float zmin = gc->min[2] + (float)cz * gc->cell[2];
+        const float zmax = gc->min[2] + (float)(cz + 1) * gc->cell[2];
+        for (uint32_t iy = 0; iy < dims[1]; ++iy) {
+            const uint32_t cy = cmin[1] + iy;
+            const float ymin = gc->min[1] + (float)cy * gc->cell[1];
+            const float ymax = gc->min[1] + (float)(cy + 1) * gc->cell[1];
+            for (uint32_t ix = 0; ix < dims[0]; ++ix) {
+                const uint32_t cx = cmin[0] + ix;
+                const float xmin = gc->min[0] + (float)cx * gc->cell[0];
+                const float xmax = gc->min[0] + (float)(cx + 1) * gc->cell[0];
+                if (ilayer > 1 && q[0] >= xmin - margin[0] && q[1] >= ymin - margin[1] &&
+                    q[2] >= zmin - margin[2] && q[0] < xmax + margin[0] &&
+                    q[1] < ymax + margin[1] && q[2] < zmax + margin[2]) {
+                    const uint32_t ci[3] = {cx, cy, cz};
+                    float sub[3];
+                    tree_rec(t, q, ci, ilayer - 1, spread, circ, policy, vecpot, sub);
+                    acc[0] += sub[0]; acc[1] += sub[1]; acc[2] += sub[2];
+                } else {
+                    const size_t off = cx + (size_t)gc->np[0] * (cy + (size_t)gc->np[1] * cz);
+                    const float sz = t->size[ilayer - 1][off];
+                    if (sz == 0.0f) continue;
+                    if (vecpot)
+                        vgo_accumulate_vecpot(acc, q, t->pos[ilayer - 1] + 3 * off,
+                                              t->om[ilayer - 1] + 3 * off, sz, spread,
+                                              circ, policy);
+                    else
+                        vgo_accumulate_velocity(acc, q, t->pos[ilayer - 1] + 3 * off,
+                                                t->om[ilayer - 1] + 3 * off, sz, spread,
+                                                circ, policy);
+                }
+            }
+        }
+    }
+    out[0] = acc[0]; out[1] = acc[1]; out[2] = acc[2];
+}
+
+void vgo_tree_velocity(const vgo_tree *t, const float q[3], float spread,
+                       float circ, int rsqrt_policy, float out[3]) {
+    const uint32_t zeros[3] = {0, 0, 0};
+    tree_rec(t, q, zeros, t->ng.depth - 1, spread, circ, rsqrt_policy, 0, out);
+}
+
+void vgo_tree_vecpot(const vgo_tree *t, const float q[3], float spread,
+                     float circ, int rsqrt_policy, float out[3]) {
+    const uint32_t zeros[3] = {0, 0, 0};
+    tree_rec(t, q, zeros, t->ng.depth - 1, spread, circ, rsqrt_policy, 1, out);
+}
+
+/* vortonSim.cpp:2458-2477 */
+void vgo_auto_mollify(const vgo_grid *g, float vorton_radius, float *spread,
+                      float *circ) {
+    const float one_third = 1.0f / 3.0f;
+    const float cell_volume = g->cell[0] * g->cell[1] * g->cell[2];
+    const float cgl = 1.0f * powf(cell_volume, one_third);
+    float s = cgl / vorton_radius;
+    if (s < 1.0f) s = 1.0f;
+    *spread = s;
+    *circ = 1.0f / (s * s * s);
+}
+
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    const vgo_tree *t;
+    const vgo_grid *g;
+    const float *pos, *om, *size;
+    size_t n;
+    int technique, policy, copy_odd;
+    float spread, circ;
+    float *out;
+} grid_ctx;
+
+static void velgrid_range(void *vc, size_t b, size_t e) {
+    grid_ctx *c = (grid_ctx *)vc;
+    const vgo_grid *g = c->g;
+    for (size_t iz = b; iz < e; ++iz)
+        for (uint32_t iy = 0; iy < g->np[1]; ++iy)
+            for (uint32_t ix = 0; ix < g->np[0]; ++ix) {
+                float q[3], v[3] = {0.f, 0.f, 0.f};
+                vgo_grid_query_point(g, ix, iy, (uint32_t)iz, q);
+                const size_t off = ix + (size_t)g->np[0] * (iy + (size_t)g->np[1] * iz);
+                if (c->technique == 1 && c->t && c->t->ng.depth > 1) {
+                    vgo_tree_velocity(c->t, q, c->spread, c->circ, c->policy, v);
+                } else {
+                    for (size_t i = 0; i < c->n; ++i)
+                        vgo_accumulate_velocity(v, q, c->pos + 3 * i, c->om + 3 * i,
+                                                c->size[i], c->spread, c->circ, c->policy);
+                }
+                c->out[3 * off] = v[0];
+                c->out[3 * off + 1] = v[1];
+                c->out[3 * off + 2] = v[2];
+            }
+}
+
+void vgo_velocity_grid(const vgo_tree *t, const vgo_grid *g, const float *pos,
+                       const float *om, const float *size, size_t n,
+                       int technique, float spread, float circ,
+                       int rsqrt_policy, float *out, int nthreads) {
+    grid_ctx c = {t, g, pos, om, size, n, technique, rsqrt_policy, 0, spread, circ, out};
+    parallel_range(g->np[2], nthreads, velgrid_range, &c);
+}
+
+/* vortonSim.cpp:2055-2138 (boundariesOnly) */
+static void vpb_point(grid_ctx *c, uint32_t ix, uint32_t iy, uint32_t iz) {
+    const vgo_grid *g = c->g;
+    const size_t off = ix + (size_t)g->np[0] * (iy + (size_t)g->np[1] * iz);
+    if (c->copy_odd && (ix & 1u)) {
+        for (int a = 0; a < 3; ++a) c->out[3 * off + a] = c->out[3 * (off - 1) + a];
+        return;
+    }
+    float q[3], v[3] = {0.f, 0.f, 0.f};
+    vgo_grid_query_point(g, ix, iy, iz, q);
+    if (c->t && c->t->ng.depth > 1) {
+        vgo_tree_vecpot(c->t, q, c->spread, c->circ, c->policy, v);
+    } else {
+        for (size_t i = 0; i < c->n; ++i)
+            vgo_accumulate_vecpot(v, q, c->pos + 3 * i, c->om + 3 * i, c->size[i],
+                                  c->spread, c->circ, c->policy);
+    }
+    for (int a = 0; a < 3; ++a) c->out[3 * off + a] = v[a];
+}
+
+static void vpb_range(void *vc, size_t b, size_t e) {
+    grid_ctx *c = (grid_ctx *)vc;
+    const vgo_grid *g = c->g;
+    const uint32_t *d = g->np;
+    for (size_t izs = b; izs < e; ++izs) {
+        const uint32_t iz = (uint32_t)izs;
+        const int tb = (iz == 0) || (iz == d[2] - 1);
+        for (uint32_t iy = 0; iy < d[1]; ++iy) {
+            const int fb = (iy == 0) || (iy == d[1] - 1);
+            const uint32_t inc = (tb || fb) ? 1u : (d[0] - 1);
+            for (uint32_t ix = 0; ix < d[0]; ix += inc) {
+                vpb_point(c, ix, iy, iz);
+                if (inc == 0) break;
+            }
+        }
+    }
+}
+
+void vgo_vecpot_boundary(const vgo_tree *t, const vgo_grid *g,
+                         const float *pos, const float *om, const float *size,
+                         size_t n, float spread, float circ, int rsqrt_policy,
+                         int copy_odd, float *out, int nthreads) {
+    grid_ctx c = {t, g, pos, om, size, n, 1, rsqrt_policy, copy_odd, spread, circ, out};
+    parallel_range(g->np[2], nthreads, vpb_range, &c);
+}
+
+/* ------------------------------------------------------------------------ */
+/* uniformGrid.h:1117-1148 Interpolate (weights not clamped). */
+void vgo_interpolate(const vgo_grid *g, const float *f, const float p[3],
+                     float out[3]) {
+    uint32_t idx[3];
+    indices_of_position(g, p, idx);
+    float t[3], o[3];
+    for (int a = 0; a < 3; ++a) {
+        const float cmin = g->min[a] + (float)idx[a] * g->cell[a];
+        const float diff = p[a] - cmin;
+        t[a] = diff * g->cpe[a];
+        o[a] = 1.0f - t[a];
+    }
+    const size_t nx = g->np[0], nxy = (size_t)g->np[0] * g->np[1];
+    const size_t o000 = idx[0] + nx * (idx[1] + (size_t)g->np[1] * idx[2]);
+    const size_t o100 = o000 + 1, o010 = o000 + nx, o110 = o000 + nx + 1;
+    const size_t o001 = o000 + nxy, o101 = o000 + nxy + 1, o011 = o000 + nxy + nx,
+                 o111 = o000 + nxy + nx + 1;
+    for (int c = 0; c < 3; ++c) {
+        out[c] = ((o[0] * f[3 * o000 + c] + t[0] * f[3 * o100 + c]) * o[1] +
+                  (o[0] * f[3 * o010 + c] + t[0] * f[3 * o110 + c]) * t[1]) * o[2] +
+                 ((o[0] * f[3 * o001 + c] + t[0] * f[3 * o101 + c]) * o[1] +
+                  (o[0] * f[3 * o011 + c] + t[0] * f[3 * o111 + c]) * t[1]) * t[2];
+    }
+}
+
+static float clamp01(float v) { return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); }
+
+/* uniformGrid.h:1278-1342 Accumulate (weights clamped to [0,1]). */
+void vgo_accumulate(const vgo_grid *g, float *f, int ncomp, const float p[3],
+                    const float *item) {
+    uint32_t idx[3];
+    indices_of_position(g, p, idx);
+    float t[3], o[3];
+    for (int a = 0; a < 3; ++a) {
+        const float cmin = g->min[a] + (float)idx[a] * g->cell[a];
+        t[a] = clamp01((p[a] - cmin) * g->cpe[a]);
+        o[a] = 1.0f - t[a];
+    }
+    const size_t nx = g->np[0], nxy = (size_t)g->np[0] * g->np[1];
+    const size_t o000 = idx[0] + nx * (idx[1] + (size_t)g->np[1] * idx[2]);
+    const size_t offs[8] = {o000, o000 + 1, o000 + nx, o000 + nx + 1,
+                            o000 + nxy, o000 + nxy + 1, o000 + nxy + nx,
+                            o000 + nxy + nx + 1};
+    const float w[8] = {o[0] * o[1] * o[2], t[0] * o[1] * o[2], o[0] * t[1] * o[2],
+                        t[0] * t[1] * o[2], o[0] * o[1] * t[2], t[0] * o[1] * t[2],
+                        o[0] * t[1] * t[2], t[0] * t[1] * t[2]};
+    for (int k = 0; k < 8; ++k)
+        for (int c = 0; c < ncomp; ++c) f[ncomp * offs[k] + c] += item[c] * w[k];
+}
+
+/* vortonSim.cpp:2372-2436 */
+void vgo_splat_vorticity(const vgo_grid *g, const float *pos, const float *om,
+                         size_t n, float vorton_radius, float scale, float *out) {
+    memset(out, 0, vgo_grid_capacity(g) * 3 * sizeof(float));
+    if (n == 0) return;
+    const float cell_volume = g->cell[0] * g->cell[1] * g->cell[2];
+    const float four_pi_over3 = (4.0f * kPi) / 3.0f;
+    const float vorton_volume = four_pi_over3 * (vorton_radius * vorton_radius * vorton_radius);
+    const float vc = scale * vorton_volume / cell_volume;
+    for (size_t i = 0; i < n; ++i) {
+        const float item[3] = {(om[3 * i] * 2.0f) * vc, (om[3 * i + 1] * 2.0f) * vc,
+                               (om[3 * i + 2] * 2.0f) * vc};
+        vgo_accumulate(g, out, 3, pos + 3 * i, item);
+    }
+}
+
+/* pclOpPopulateVelocityGrid.cpp:33-87 */
+void vgo_populate_velocity_grid(const vgo_grid *g, const float *pos,
+                                const float *vel, size_t n, float *out) {
+    const size_t cap = vgo_grid_capacity(g);
+    memset(out, 0, cap * 3 * sizeof(float));
+    float *wsum = (float *)calloc(cap, sizeof(float));
+    const float one = 1.0f;
+    for (size_t i = 0; i < n; ++i) {
+        vgo_accumulate(g, out, 3, pos + 3 * i, vel + 3 * i);
+        vgo_accumulate(g, wsum, 1, pos + 3 * i, &one);
+    }
+    for (size_t o = 0; o < cap; ++o) {
+        if (wsum[o] > 0.0f) {
+            const float inv = 1.0f / wsum[o];
+            for (int c = 0; c < 3; ++c) out[3 * o + c] *= inv;
+        }
+    }
+    free(wsum);
+}
+
+/* uniformGrid.h:1453-1585 DownSampleSlice */
+void vgo_downsample(const vgo_grid *lo, float *lf, const vgo_grid *hi,
+                    const float *hf, int accurate) {
+    static const float mult[4] = {8.0f, 4.0f, 2.0f, 1.0f};
+    uint32_t dims[3];
+    for (int a = 0; a < 3; ++a) dims[a] = (hi->np[a] - 1) / (lo->np[a] - 1);
+    const size_t hnx = hi->np[0], hnxy = (size_t)hi->np[0] * hi->np[1];
+    for (uint32_t iz = 0; iz < lo->np[2]; ++iz)
+        for (uint32_t iy = 0; iy < lo->np[1]; ++iy)
+            for (uint32_t ix = 0; ix < lo->np[0]; ++ix) {
+                const size_t lo_off = ix + (size_t)lo->np[0] * (iy + (size_t)lo->np[1] * iz);
+                float *r = lf + 3 * lo_off;
+                const uint32_t cm[3] = {ix * dims[0], iy * dims[1], iz * dims[2]};
+                if (!accurate) {
+                    const size_t ho = cm[0] + hnx * cm[1] + hnxy * cm[2];
+                    for (int c = 0; c < 3; ++c) r[c] = hf[3 * ho + c];
+                    continue;
+                }
+                float acc[3] = {0.f, 0.f, 0.f};
+                const uint32_t sh[3] = {dims[0] / 2, dims[1] / 2, dims[2] / 2};
+                for (uint32_t i2 = 0; i2 < dims[2] + sh[2]; ++i2) {
+                    const int s2 = (int)i2 - (int)sh[2];
+                    const uint32_t h2 = cm[2] + (uint32_t)s2;
+                    if (h2 >= hi->np[2]) continue;
+                    for (uint32_t i1 = 0; i1 < dims[1] + sh[1]; ++i1) {
+                        const int s1 = (int)i1 - (int)sh[1];
+                        const uint32_t h1 = cm[1] + (uint32_t)s1;
+                        if (h1 >= hi->np[1]) continue;
+                        for (uint32_t i0 = 0; i0 < dims[0] + sh[0]; ++i0) {
+                            const int s0 = (int)i0 - (int)sh[0];
+                            const uint32_t h0 = cm[0] + (uint32_t)s0;
+                            if (h0 >= hi->np[0]) continue;
+                            const unsigned md = (unsigned)(abs(s0) + abs(s1) + abs(s2));
+                            const size_t ho = h0 + hnx * h1 + hnxy * h2;
+                            for (int c = 0; c < 3; ++c) acc[c] += hf[3 * ho + c] * mult[md];
+                        }
+                    }
+                }
+                const float inv = 1.0f / 64.0f;
+                for (int c = 0; c < 3; ++c) r[c] = acc[c] * inv;
+            }
+}
+
+/* uniformGrid.h:1637-1700 UpSampleSlice, serial semantics (one slice
+ * covering all z). */
+void vgo_upsample(const vgo_grid *hi, float *hf, const vgo_grid *lo,
+                  const float *lf, int interior_only) {
+    float pmin[3], sp[3];
+    for (int a = 0; a < 3; ++a) {
+        const float adj = (fabsf(hi->min[a]) * 2.0f) * FLT_EPSILON;
+        pmin[a] = hi->min[a] + adj;
+        sp[a] = hi->cell[a] * (1.0f - 4.0f * FLT_EPSILON);
+    }
+    uint32_t lo_i[3] = {0, 0, 0}, hi_i[3] = {hi->np[0], hi->np[1], hi->np[2]};
+    if (interior_only) {
+        for (int a = 0; a < 3; ++a) {
+            lo_i[a] = 1;
+            hi_i[a] = (hi->np[a] > 1 ? hi->np[a] : 1) - 1;
+        }
+    }
+    for (uint32_t iz = lo_i[2]; iz < hi_i[2]; ++iz) {
+        float p[3];
+        p[2] = pmin[2] + (float)iz * sp[2];
+        for (uint32_t iy = lo_i[1]; iy < hi_i[1]; ++iy) {
+            p[1] = pmin[1] + (float)iy * sp[1];
+            for (uint32_t ix = lo_i[0]; ix < hi_i[0]; ++ix) {
+                p[0] = pmin[0] + (float)ix * sp[0];
+                const size_t off = ix + (size_t)hi->np[0] * (iy + (size_t)hi->np[1] * iz);
+                vgo_interpolate(lo, lf, p, hf + 3 * off);
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* uniformGridMath.cpp:1116-1188 StepTowardVectorPoissonSolution (Dirichlet):
+ * rows whose (iy+iz) parity matches the colour, x sequential in place. */
+void vgo_sor_pass(const vgo_grid *g, float *s, const float *lap, int color) {
+    const float rsx = 1.0f / g->cell[0], rsy = 1.0f / g->cell[1];
+    const float rsz = g->cell[2] > FLT_EPSILON ? 1.0f / g->cell[2] : 0.0f;
+    const float r2x = rsx * rsx, r2y = rsy * rsy, r2z = rsz * rsz;
+    const float h = 0.5f / (r2x + r2y + r2z);
+    const float relax = 1.72f, omr = 1.0f - relax;
+    const size_t nx = g->np[0], ny = g->np[1], nz = g->np[2], nxy = nx * ny;
+    for (size_t iz = 1; iz + 1 < nz; ++iz) {
+        const size_t yshift = (iz % 2 == (size_t)color) ? 1 : 0;
+        for (size_t iy = 1 + yshift; iy + 1 < ny; iy += 2) {
+            const size_t row = iy * nx + iz * nxy;
+            for (size_t ix = 1; ix + 1 < nx; ++ix) {
+                const size_t o = ix + row;
+                for (int c = 0; c < 3; ++c) {
+                    const float v = ((s[3 * (o + 1) + c] + s[3 * (o - 1) + c]) * r2x +
+                                     (s[3 * (o + nx) + c] + s[3 * (o - nx) + c]) * r2y +
+                                     (s[3 * (o + nxy) + c] + s[3 * (o - nxy) + c]) * r2z -
+                                     lap[3 * o + c]) * h;
+                    s[3 * o + c] = omr * s[3 * o + c] + relax * v;
+                }
+            }
+        }
+    }
+}
+
+/* uniformGridMath.cpp:1369-1405 */
+void vgo_solve_vector_poisson(const vgo_grid *g, float *soln, const float *lap,
+                              size_t num_steps) {
+    size_t maxdim = g->np[0];
+    if (g->np[1] > maxdim) maxdim = g->np[1];
+    if (g->np[2] > maxdim) maxdim = g->np[2];
+    const size_t iters = num_steps > 0 ? num_steps : 2 * maxdim;
+    for (size_t it = 0; it < iters; ++it) {
+        vgo_sor_pass(g, soln, lap, 0);
+        vgo_sor_pass(g, soln, lap, 1);
+    }
+}
+
+/* vortonSim.cpp:2523-2571 (VORTON_SIM_USE_MULTI_GRID branch). */
+void vgo_mg_cascade(const vgo_grid *g, float *a0, const float *negw0,
+                    size_t steps) {
+    vgo_nested ng;
+    vgo_nested_init(&ng, g);
+    float *A[VGO_MAX_LAYERS], *W[VGO_MAX_LAYERS];
+    A[0] = a0;
+    W[0] = (float *)negw0;
+    for (uint32_t l = 1; l < ng.depth; ++l) {
+        A[l] = (float *)calloc(vgo_grid_capacity(&ng.layer[l]) * 3, sizeof(float));
+        W[l] = (float *)calloc(vgo_grid_capacity(&ng.layer[l]) * 3, sizeof(float));
+    }
+    vgo_solve_vector_poisson(&ng.layer[0], A[0], W[0], steps);
+    uint32_t max_valid = 0;
+    for (uint32_t l = 1; l < ng.depth; ++l) {
+        const uint32_t *np = ng.layer[l].np;
+        uint32_t md = np[0];
+        if (np[1] < md) md = np[1];
+        if (np[2] < md) md = np[2];
+        if (md > 2) {
+            vgo_downsample(&ng.layer[l], W[l], &ng.layer[l - 1], W[l - 1], 0);
+            vgo_downsample(&ng.layer[l], A[l], &ng.layer[l - 1], A[l - 1], 0);
+            vgo_solve_vector_poisson(&ng.layer[l], A[l], W[l], steps);
+        } else {
+            max_valid = l - 1;
+            break;
+        }
+    }
+    for (uint32_t l = max_valid; l >= 1; --l) {
+        vgo_upsample(&ng.layer[l - 1], A[l - 1], &ng.layer[l], A[l], 1);
+        vgo_solve_vector_poisson(&ng.layer[l - 1], A[l - 1], W[l - 1], steps);
+    }
+    for (uint32_t l = 1; l < ng.depth; ++l) { free(A[l]); free(W[l]); }
+}
+
+/* uniformGridMath.cpp:304-451 ComputeJacobian fused with :207-232
+ * ComputeCurlFromJacobian. */
+void vgo_curl(const vgo_grid *g, const float *v, float *curl) {
+    const float rx = 1.0f / g->cell[0], ry = 1.0f / g->cell[1];
+    const float rz = g->cell[2] > FLT_EPSILON ? 1.0f / g->cell[2] : 0.0f;
+    const float r[3] = {rx, ry, rz};
+    const float hr[3] = {0.5f * rx, 0.5f * ry, 0.5f * rz};
+    const size_t n[3] = {g->np[0], g->np[1], g->np[2]};
+    const size_t stride[3] = {1, n[0], n[0] * n[1]};
+    for (size_t iz = 0; iz < n[2]; ++iz)
+        for (size_t iy = 0; iy < n[1]; ++iy)
+            for (size_t ix = 0; ix < n[0]; ++ix) {
+                const size_t idx[3] = {ix, iy, iz};
+                const size_t o = ix + n[0] * iy + n[0] * n[1] * iz;
+                float j[3][3]; /* j[a][b] = d v_b / d a */
+                for (int a = 0; a < 3; ++a) {
+                    const size_t s = stride[a];
+                    for (int b = 0; b < 3; ++b) {
+                        float d;
+                        if (idx[a] == 0)
+                            d = (v[3 * (o + s) + b] - v[3 * o + b]) * r[a];
+                        else if (idx[a] == n[a] - 1)
+                            d = (v[3 * o + b] - v[3 * (o - s) + b]) * r[a];
+                        else
+                            d = (v[3 * (o + s) + b] - v[3 * (o - s) + b]) * hr[a];
+                        j[a][b] = d;
+                    }
+                }
+                curl[3 * o] = j[1][2] - j[2][1];
+                curl[3 * o + 1] = j[2][0] - j[0][2];
+                curl[3 * o + 2] = j[0][1] - j[1][0];
+            }
+}
+
+double vgo_residual_rel(const vgo_grid *g, const float *u, const float *f) {
+    const double rx = 1.0 / g->cell[0], ry = 1.0 / g->cell[1], rz = 1.0 / g->cell[2];
+    const double r2[3] = {rx * rx, ry * ry, rz * rz};
+    const size_t nx = g->np[0], ny = g->np[1], nz = g->np[2], nxy = nx * ny;
+    double num = 0, den = 0;
+    for (size_t iz = 1; iz + 1 < nz; ++iz)
+        for (size_t iy = 1; iy + 1 < ny; ++iy)
+            for (size_t ix = 1; ix + 1 < nx; ++ix) {
+                const size_t o = ix + iy * nx + iz * nxy;
+                for (int c = 0; c < 3; ++c) {
+                    const double uc = u[3 * o + c];
+                    const double lap =
+                        ((double)u[3 * (o + 1) + c] + u[3 * (o - 1) + c] - 2 * uc) * r2[0] +
+                        ((double)u[3 * (o + nx) + c] + u[3 * (o - nx) + c] - 2 * uc) * r2[1] +
+                        ((double)u[3 * (o + nxy) + c] + u[3 * (o - nxy) + c] - 2 * uc) * r2[2];
+                    const double res = (double)f[3 * o + c] - lap;
+                    num += res * res;
+                    den += (double)f[3 * o + c] * f[3 * o + c];
+                }
+            }
+    return den > 0 ? sqrt(num / den) : sqrt(num);
+}
+
+/* ------------------------------------------------------------------------ */
+/* pclOpAdvect.cpp:42-81 */
+void vgo_assign_velocity_from_field(const vgo_grid *g, const float *field,
+                                    float *pos, float *vel, size_t stride,
+                                    size_t n, float gain) {
+    (void)pos;
+    if (g->extent[0] == 0.0f && g->extent[1] == 0.0f && g->extent[2] == 0.0f) return;
+    for (size_t i = 0; i < n; ++i) {
+        float v[3];
+        vgo_interpolate(g, field, pos + i * stride, v);
+        float *pv = vel + i * stride;
+        if (1.0f == gain) {
+            pv[0] = v[0]; pv[1] = v[1]; pv[2] = v[2];
+        } else {
+            for (int c = 0; c < 3; ++c) {
+                const float d = v[c] - pv[c];
+                pv[c] += gain * d;
+            }
+        }
+    }
+}
+
+/* pclOpEvolve.cpp:36-55 */
+void vgo_evolve(float *pos, const float *vel, size_t stride, size_t n, float dt) {
+    for (size_t i = 0; i < n; ++i) {
+        float *p = pos + i * stride;
+        const float *v = vel + i * stride;
+        for (int c = 0; c < 3; ++c) p[c] += v[c] * dt;
+    }
+}

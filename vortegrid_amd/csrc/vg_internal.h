@@ -1,0 +1,244 @@
+// vg_internal.h -- shared internals of libvortegrid (host + device).
+//
+// Numerical conventions
+//  * Index and position arithmetic that decides which cell a point falls in
+//    (grid binning, tree opening tests, interpolation weights, cluster
+//    reductions, SOR/curl stencils) is compiled with FP contraction OFF so it
+//    rounds exactly like the reference's source order.
+//  * The O(N^2) and treecode pair sums allow FMA contraction and accumulate
+//    with tile-level Kahan compensation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/vortegrid.h"
+
+#define VG_API extern "C" __attribute__((visibility("default")))
+
+namespace vg {
+
+// ------------------------------------------------------------------ errors
+void set_error(const std::string &msg);
+vg_status device_error(hipError_t e, const char *what, const char *file, int line);
+
+#define VG_HIP(call)                                                        \
+    do {                                                                    \
+        hipError_t _e = (call);                                             \
+        if (_e != hipSuccess) return ::vg::device_error(_e, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define VG_REQUIRE(cond, msg)                                               \
+    do {                                                                    \
+        if (!(cond)) { ::vg::set_error(msg); return VG_EINVAL; }            \
+    } while (0)
+
+#define VG_TRY(expr)                                                        \
+    do {                                                                    \
+        vg_status _s = (expr);                                              \
+        if (_s != VG_OK) return _s;                                         \
+    } while (0)
+
+// ------------------------------------------------------------------ profiling
+// Kernel launches go through launch(); with profiling enabled each launch is
+// bracketed by two events on its stream and tallied under `name`.
+bool profiling_enabled();
+void profile_begin(const char *name, hipStream_t s, void **token);
+void profile_end(void *token, hipStream_t s);
+
+template <typename Kernel, typename... Args>
+vg_status launch(const char *name, Kernel k, dim3 grid, dim3 block, size_t shmem,
+                 hipStream_t s, Args... args) {
+    if (grid.x == 0 || grid.y == 0 || grid.z == 0) return VG_OK;
+    void *tok = nullptr;
+    const bool prof = profiling_enabled();
+    if (prof) profile_begin(name, s, &tok);
+    hipLaunchKernelGGL(k, grid, block, shmem, s, args...);
+    hipError_t e = hipGetLastError();
+    if (prof) profile_end(tok, s);
+    if (e != hipSuccess) return device_error(e, name, __FILE__, __LINE__);
+    return VG_OK;
+}
+
+inline uint32_t ceil_div(size_t a, size_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// Transient device scratch, stream ordered (released with sfree on the same
+// stream).
+vg_status salloc(void **p, size_t bytes, hipStream_t s);
+vg_status sfree(void *p, hipStream_t s);
+// Zero device memory with a fill kernel (safe under graph capture, unlike
+// a memset node).
+vg_status zero_fill(void *p, size_t bytes, hipStream_t s);
+
+// ------------------------------------------------------------------ geometry (host)
+void define_shape(vg_grid_desc *g, size_t n, const float vmin[3], const float vmax[3], int pow2);
+void decimate(vg_grid_desc *dst, const vg_grid_desc *src, uint32_t dec);
+__host__ __device__ inline size_t capacity(const vg_grid_desc &g) {
+    return (size_t)g.np[0] * g.np[1] * g.np[2];
+}
+constexpr uint32_t kMaxLayers = 16;
+struct Nested {
+    uint32_t depth = 0;
+    vg_grid_desc layer[kMaxLayers];
+    uint32_t dec[kMaxLayers][3];
+};
+void nested_init(Nested *ng, const vg_grid_desc &src);
+
+// ------------------------------------------------------------------ device view of a tree
+struct TreeView {
+    uint32_t depth;
+    uint32_t dec[kMaxLayers][3];
+    vg_grid_desc layer[kMaxLayers];
+    const float4 *P[kMaxLayers];   // cluster (x, y, z, size)
+    const float4 *W[kMaxLayers];   // cluster (Omega_x, Omega_y, Omega_z, sum|w|)
+    // Leaf cell lists over the layer-0 points: vortons sorted by cell.
+    const uint32_t *cell_start;    // per layer-0 point
+    const uint32_t *cell_count;
+    const float4 *sorted_p;        // (x, y, z, size) in sorted order
+    const float4 *sorted_w;        // (Omega, 0)
+    const uint32_t *sorted_idx;    // original vorton index
+    uint32_t n;
+};
+
+// ------------------------------------------------------------------ device helpers
+#ifdef __HIPCC__
+
+// Core/Math/math.h:206-216: 0x5f3759df estimate + one Newton step.
+__device__ __forceinline__ float finvsqrtf(float v) {
+    int i = __float_as_int(v);
+    i = 0x5f3759df - (i >> 1);
+    float y = __int_as_float(i);
+    const float x = v * 0.5f;
+    y = y * (1.5f - x * y * y);
+    return y;
+}
+
+// a + b*c with both roundings (no FMA): matches `min + float(i) * spacing`.
+__device__ __forceinline__ float madd_exact(float a, float b, float c) {
+#pragma clang fp contract(off)
+    return a + b * c;
+}
+
+// IndicesOfPosition (uniformGrid.h:536-574) for one axis: truncation of
+// (p - min) * cellsPerExtent, clamped to a valid cell (the reference leaves
+// out-of-domain positions undefined).
+__device__ __forceinline__ int cell_of(float p, float gmin, float cpe, int ncells) {
+#pragma clang fp contract(off)
+    const float f = (p - gmin) * cpe;
+    int i = (f > -1.0f) ? (int)f : -1;   // trunc; values <= -1 clamp to 0 below
+    i = i < 0 ? 0 : i;
+    i = i > ncells - 1 ? ncells - 1 : i;
+    return i;
+}
+
+__device__ __forceinline__ size_t point_offset(const vg_grid_desc &g, uint32_t ix, uint32_t iy,
+                                               uint32_t iz) {
+    return (size_t)ix + (size_t)g.np[0] * ((size_t)iy + (size_t)g.np[1] * iz);
+}
+
+// Grid point position with the (1 - 2 eps) spacing nudge of the grid-point
+// query loops (vortonSim.cpp:2064-2065, 2229-2230).
+__device__ __forceinline__ float3 grid_query_point(const vg_grid_desc &g, uint32_t ix,
+                                                   uint32_t iy, uint32_t iz) {
+#pragma clang fp contract(off)
+    const float nudge = 1.0f - 2.0f * FLT_EPSILON;
+    float3 p;
+    p.x = g.min[0] + (float)ix * (g.cell[0] * nudge);
+    p.y = g.min[1] + (float)iy * (g.cell[1] * nudge);
+    p.z = g.min[2] + (float)iz * (g.cell[2] * nudge);
+    return p;
+}
+
+// Interpolate (uniformGrid.h:1117-1148, 1225-1258): nested lerp x, y, z.
+__device__ __forceinline__ float3 interpolate(const vg_grid_desc &g, const float *__restrict__ f,
+                                              float3 p) {
+#pragma clang fp contract(off)
+    const int ix = cell_of(p.x, g.min[0], g.cpe[0], (int)g.np[0] - 1);
+    const int iy = cell_of(p.y, g.min[1], g.cpe[1], (int)g.np[1] - 1);
+    const int iz = cell_of(p.z, g.min[2], g.cpe[2], (int)g.np[2] - 1);
+    const float tx = (p.x - (g.min[0] + (float)ix * g.cell[0])) * g.cpe[0];
+    const float ty = (p.y - (g.min[1] + (float)iy * g.cell[1])) * g.cpe[1];
+    const float tz = (p.z - (g.min[2] + (float)iz * g.cell[2])) * g.cpe[2];
+    const float ox = 1.0f - tx, oy = 1.0f - ty, oz = 1.0f - tz;
+    const size_t nx = g.np[0], nxy = (size_t)g.np[0] * g.np[1];
+    const size_t o000 = (size_t)ix + nx * ((size_t)iy + (size_t)g.np[1] * iz);
+    const float *a000 = f + 3 * o000, *a100 = a000 + 3, *a010 = a000 + 3 * nx,
+                *a110 = a010 + 3, *a001 = a000 + 3 * nxy, *a101 = a001 + 3,
+                *a011 = a001 + 3 * nx, *a111 = a011 + 3;
+    float r[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        r[c] = ((ox * a000[c] + tx * a100[c]) * oy + (ox * a010[c] + tx * a110[c]) * ty) * oz +
+               ((ox * a001[c] + tx * a101[c]) * oy + (ox * a011[c] + tx * a111[c]) * ty) * tz;
+    }
+    return make_float3(r[0], r[1], r[2]);
+}
+
+// Kahan compensated add of a partial sum into (sum, comp).
+__device__ __forceinline__ void kahan_add(float3 &sum, float3 &comp, float3 part) {
+    float y, t;
+    y = part.x - comp.x; t = sum.x + y; comp.x = (t - sum.x) - y; sum.x = t;
+    y = part.y - comp.y; t = sum.y + y; comp.y = (t - sum.y) - y; sum.y = t;
+    y = part.z - comp.z; t = sum.z + y; comp.z = (t - sum.z) - y; sum.z = t;
+}
+
+// Source record prepared for the pair kernels (vorton.h:118-179 with the
+// per-source factors hoisted):  A = (x, y, z, radius^2),
+// B = (Omega * (2/3) r^2 r, 1/(r^2 r)).  circ is applied by the caller.
+__device__ __forceinline__ void prepare_source(float4 ps, float4 w, float spread, float4 &A,
+                                               float4 &B) {
+#pragma clang fp contract(off)
+    const float radius = ps.w * 0.5f * spread;
+    const float r2 = radius * radius;
+    const float s = (2.0f / 3.0f) * r2 * radius;
+    A = make_float4(ps.x, ps.y, ps.z, r2);
+    B = make_float4(w.x * s, w.y * s, w.z * s, 1.0f / (r2 * radius));
+}
+
+// VORTON_ACCUMULATE_VELOCITY_private (vorton.h:161-179).
+template <int POLICY>
+__device__ __forceinline__ void pair_velocity(float3 &acc, float3 q, float4 A, float4 B) {
+    const float dx = q.x - A.x, dy = q.y - A.y, dz = q.z - A.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    float law;
+    if (POLICY == VG_RSQRT_REFERENCE) {
+        law = finvsqrtf(d2) * __builtin_amdgcn_rcpf(d2);
+    } else {
+        const float rs = __builtin_amdgcn_rsqf(d2);
+        law = rs * rs * rs;
+    }
+    law = d2 < A.w ? B.w : law;
+    // (a x d) * law == a x (d * law)
+    const float cx = B.y * dz - B.z * dy;
+    const float cy = B.z * dx - B.x * dz;
+    const float cz = B.x * dy - B.y * dx;
+    acc.x += cx * law;
+    acc.y += cy * law;
+    acc.z += cz * law;
+}
+
+// VORTON_ACCUMULATE_VECTOR_POTENTIAL_private (vorton.h:118-136).
+template <int POLICY>
+__device__ __forceinline__ void pair_vecpot(float3 &acc, float3 q, float4 A, float4 B) {
+    const float dx = q.x - A.x, dy = q.y - A.y, dz = q.z - A.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    float law = POLICY == VG_RSQRT_REFERENCE ? finvsqrtf(d2) : __builtin_amdgcn_rsqf(d2);
+    law = d2 < A.w ? d2 * B.w : law;
+    acc.x += B.x * law;
+    acc.y += B.y * law;
+    acc.z += B.z * law;
+}
+
+__device__ __forceinline__ float4 load_strided3(const float *p, size_t stride, size_t i,
+                                                float w) {
+    const float *r = p + i * stride;
+    return make_float4(r[0], r[1], r[2], w);
+}
+
+#endif  // __HIPCC__
+
+}  // namespace vg

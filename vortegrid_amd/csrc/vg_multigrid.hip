@@ -1,0 +1,472 @@
+// vg_multigrid.hip -- vector-Poisson solves for the vector potential,
+// nabla^2 A = -w, with Dirichlet values on the 6 faces.
+//
+// 1. Reference cascade (VortonSim::ComputeVectorPotential, vortonSim.cpp:
+//    2523-2571): SOR on the finest grid, injection-restrict both A and -w down
+//    the NestedGrid pyramid with SOR on each layer, then prolong-overwrite the
+//    interiors back up with SOR after each.  Kept for parity.
+// 2. Residual-correction V-cycles for a converged solve: red-black point
+//    Gauss-Seidel smoothing, full-weighting restriction, trilinear
+//    correction, FP64 Gauss-Seidel on the coarsest layer.  The finest-layer
+//    solution is carried as a compensated FP32 pair (hi + lo) and refined
+//    against FP64 residuals (mixed-precision iterative refinement): an FP32
+//    field alone cannot reach a 1e-6 relative residual on large grids, since
+//    its rounding noise amplified by the Laplacian sits near 1e-4.
+#include <cmath>
+#include <vector>
+
+#include "vg_internal.h"
+
+namespace vg {
+vg_status sor_solve(const vg_grid_desc &g, float *soln, const float *lap, uint32_t steps,
+                    hipStream_t s);
+vg_status inject(const vg_grid_desc &lo, float *lf, const vg_grid_desc &hi, const float *hf,
+                 hipStream_t s);
+vg_status prolong_interior(const vg_grid_desc &hi, float *hf, const vg_grid_desc &lo,
+                           const float *lf, hipStream_t s);
+}  // namespace vg
+
+struct vg_multigrid {
+    // cascade pyramid (NestedGrid layers)
+    vg::Nested ng;
+    float *A[vg::kMaxLayers] = {};
+    float *W[vg::kMaxLayers] = {};
+    size_t cap[vg::kMaxLayers] = {};
+    // V-cycle hierarchy
+    uint32_t vdepth = 0;
+    vg_grid_desc vg[vg::kMaxLayers];
+    float *u[vg::kMaxLayers] = {};
+    float *f[vg::kMaxLayers] = {};
+    float *r[vg::kMaxLayers] = {};
+    size_t vcap[vg::kMaxLayers] = {};
+    float *lo = nullptr;       // compensation term of the finest solution
+    size_t lo_cap = 0;
+    double *part = nullptr;    // FP64 norm partials
+};
+
+namespace vg {
+namespace {
+
+template <typename T>
+vg_status grow(T **p, size_t *cap, size_t need) {
+    if (need <= *cap && *p) return VG_OK;
+    if (*p) VG_HIP(hipFree(*p));
+    *p = nullptr;
+    *cap = 0;
+    VG_HIP(hipMalloc((void **)p, need * sizeof(T)));
+    *cap = need;
+    return VG_OK;
+}
+
+__global__ void k_zero_interior(vg_grid_desc g, float *__restrict__ a) {
+    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= capacity(g)) return;
+    const size_t nx = g.np[0], ny = g.np[1], nz = g.np[2];
+    const size_t ix = o % nx, iy = (o / nx) % ny, iz = o / (nx * ny);
+    if (ix == 0 || iy == 0 || iz == 0 || ix + 1 == nx || iy + 1 == ny || iz + 1 == nz) return;
+    a[3 * o] = 0.f;
+    a[3 * o + 1] = 0.f;
+    a[3 * o + 2] = 0.f;
+}
+
+__global__ void k_zero_all(float *__restrict__ a, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0.f;
+}
+
+struct Lap {
+    float r2x, r2y, r2z, diag;  // diag = 2 (r2x + r2y + r2z)
+};
+
+Lap lap_coef(const vg_grid_desc &g) {
+    Lap c;
+    const float rx = 1.0f / g.cell[0], ry = 1.0f / g.cell[1], rz = 1.0f / g.cell[2];
+    c.r2x = rx * rx;
+    c.r2y = ry * ry;
+    c.r2z = rz * rz;
+    c.diag = 2.0f * (c.r2x + c.r2y + c.r2z);
+    return c;
+}
+
+// Red-black point Gauss-Seidel: colour = (ix + iy + iz) & 1.  Threads map to
+// the colour's points of one x row: ix = 1 + ((iy + iz + color + 1) & 1) + 2k.
+__global__ void k_rbgs(vg_grid_desc g, float *__restrict__ u, const float *__restrict__ f,
+                       int color, Lap c) {
+    const uint32_t iy = 1 + blockIdx.y, iz = 1 + blockIdx.z;
+    const uint32_t first = 1 + ((iy + iz + 1 + (uint32_t)color) & 1u);
+    const uint32_t ix = first + 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+    const size_t nx = g.np[0], nxy = (size_t)g.np[0] * g.np[1];
+    if (ix + 1 >= nx) return;
+    const size_t o = ix + (size_t)iy * nx + (size_t)iz * nxy;
+    const float inv = 1.0f / c.diag;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float s = (u[3 * (o + 1) + k] + u[3 * (o - 1) + k]) * c.r2x +
+                        (u[3 * (o + nx) + k] + u[3 * (o - nx) + k]) * c.r2y +
+                        (u[3 * (o + nxy) + k] + u[3 * (o - nxy) + k]) * c.r2z;
+        u[3 * o + k] = (s - f[3 * o + k]) * inv;
+    }
+}
+
+// r = f - L u on interior points, 0 on the boundary.
+__global__ void k_residual32(vg_grid_desc g, const float *__restrict__ u,
+                             const float *__restrict__ f, float *__restrict__ r, Lap c) {
+    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= capacity(g)) return;
+    const size_t nx = g.np[0], ny = g.np[1], nz = g.np[2], nxy = nx * ny;
+    const size_t ix = o % nx, iy = (o / nx) % ny, iz = o / nxy;
+    if (ix == 0 || iy == 0 || iz == 0 || ix + 1 == nx || iy + 1 == ny || iz + 1 == nz) {
+        r[3 * o] = r[3 * o + 1] = r[3 * o + 2] = 0.f;
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float uc = u[3 * o + k];
+        const float lap = (u[3 * (o + 1) + k] + u[3 * (o - 1) + k]) * c.r2x +
+                          (u[3 * (o + nx) + k] + u[3 * (o - nx) + k]) * c.r2y +
+                          (u[3 * (o + nxy) + k] + u[3 * (o - nxy) + k]) * c.r2z - c.diag * uc;
+        r[3 * o + k] = f[3 * o + k] - lap;
+    }
+}
+
+// Full weighting (weights 8/4/2/1 by Manhattan distance, /64) onto coarse
+// interior points; coarse boundary = 0 (homogeneous Dirichlet correction).
+__global__ void k_restrict_fw(vg_grid_desc lo, float *__restrict__ flo, vg_grid_desc hi,
+                              const float *__restrict__ rhi) {
+    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= capacity(lo)) return;
+    const size_t nx = lo.np[0], ny = lo.np[1], nz = lo.np[2];
+    const size_t ix = o % nx, iy = (o / nx) % ny, iz = o / (nx * ny);
+    if (ix == 0 || iy == 0 || iz == 0 || ix + 1 == nx || iy + 1 == ny || iz + 1 == nz) {
+        flo[3 * o] = flo[3 * o + 1] = flo[3 * o + 2] = 0.f;
+        return;
+    }
+    const size_t hx = hi.np[0], hxy = (size_t)hi.np[0] * hi.np[1];
+    const size_t c = 2 * ix + 2 * iy * hx + 2 * iz * hxy;
+    float acc[3] = {0.f, 0.f, 0.f};
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int md = abs(dx) + abs(dy) + abs(dz);
+                const float w = (float)(8 >> md);
+                const size_t h = c + dx + (long long)dy * (long long)hx + (long long)dz * (long long)hxy;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) acc[k] += w * rhi[3 * h + k];
+            }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) flo[3 * o + k] = acc[k] * (1.0f / 64.0f);
+}
+
+// u_hi(interior) += trilinear(u_lo).
+__global__ void k_prolong_add(vg_grid_desc hi, float *__restrict__ uhi, vg_grid_desc lo,
+                              const float *__restrict__ ulo) {
+    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= capacity(hi)) return;
+    const size_t nx = hi.np[0], ny = hi.np[1], nz = hi.np[2];
+    const size_t ix = o % nx, iy = (o / nx) % ny, iz = o / (nx * ny);
+    if (ix == 0 || iy == 0 || iz == 0 || ix + 1 == nx || iy + 1 == ny || iz + 1 == nz) return;
+    const size_t lx = lo.np[0], lxy = (size_t)lo.np[0] * lo.np[1];
+    const size_t x0 = ix / 2, y0 = iy / 2, z0 = iz / 2;
+    const size_t x1 = x0 + (ix & 1), y1 = y0 + (iy & 1), z1 = z0 + (iz & 1);
+    const float wx = (ix & 1) ? 0.5f : 1.0f, wy = (iy & 1) ? 0.5f : 1.0f, wz = (iz & 1) ? 0.5f : 1.0f;
+    float acc[3] = {0.f, 0.f, 0.f};
+    const size_t xs[2] = {x0, x1}, ys[2] = {y0, y1}, zs[2] = {z0, z1};
+    const int nxs = (ix & 1) ? 2 : 1, nys = (iy & 1) ? 2 : 1, nzs = (iz & 1) ? 2 : 1;
+    for (int a = 0; a < nzs; ++a)
+        for (int b = 0; b < nys; ++b)
+            for (int d = 0; d < nxs; ++d) {
+                const size_t l = xs[d] + ys[b] * lx + zs[a] * lxy;
+                const float w = wx * wy * wz;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) acc[k] += w * ulo[3 * l + k];
+            }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) uhi[3 * o + k] += acc[k];
+}
+
+// FP64 coarsest solve: one workgroup, red-black Gauss-Seidel in double.
+__global__ __launch_bounds__(256) void k_coarse_f64(vg_grid_desc g, float *__restrict__ u,
+                                                    const float *__restrict__ f, int sweeps,
+                                                    double *__restrict__ work) {
+    const size_t nx = g.np[0], ny = g.np[1], nz = g.np[2], nxy = nx * ny, n = nxy * nz;
+    const double rx = 1.0 / g.cell[0], ry = 1.0 / g.cell[1], rz = 1.0 / g.cell[2];
+    const double r2x = rx * rx, r2y = ry * ry, r2z = rz * rz;
+    const double inv = 1.0 / (2.0 * (r2x + r2y + r2z));
+    for (size_t i = threadIdx.x; i < 3 * n; i += blockDim.x) work[i] = u[i];
+    __syncthreads();
+    for (int it = 0; it < sweeps; ++it) {
+        for (int color = 0; color < 2; ++color) {
+            for (size_t o = threadIdx.x; o < n; o += blockDim.x) {
+                const size_t ix = o % nx, iy = (o / nx) % ny, iz = o / nxy;
+                if (ix == 0 || iy == 0 || iz == 0 || ix + 1 == nx || iy + 1 == ny || iz + 1 == nz) continue;
+                if (((ix + iy + iz) & 1) != (size_t)color) continue;
+                for (int k = 0; k < 3; ++k) {
+                    const double s = (work[3 * (o + 1) + k] + work[3 * (o - 1) + k]) * r2x +
+                                     (work[3 * (o + nx) + k] + work[3 * (o - nx) + k]) * r2y +
+                                     (work[3 * (o + nxy) + k] + work[3 * (o - nxy) + k]) * r2z;
+                    work[3 * o + k] = (s - (double)f[3 * o + k]) * inv;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (size_t i = threadIdx.x; i < 3 * n; i += blockDim.x) u[i] = (float)work[i];
+}
+
+// FP64 residual of the compensated solution (hi + lo): writes the FP32
+// residual for the next correction and block partials of ||r||^2, ||f||^2.
+__global__ __launch_bounds__(256) void k_residual_pair(vg_grid_desc g, const float *__restrict__ hi,
+                                                       const float *__restrict__ lo,
+                                                       const float *__restrict__ f,
+                                                       float *__restrict__ r32, double r2x,
+                                                       double r2y, double r2z,
+                                                       double *__restrict__ part) {
+    __shared__ double sn[256], sd[256];
+    const size_t nx = g.np[0], ny = g.np[1], nz = g.np[2], nxy = nx * ny;
+    const size_t total = capacity(g);
+    double num = 0.0, den = 0.0;
+    for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < total;
+         o += (size_t)gridDim.x * blockDim.x) {
+        const size_t ix = o % nx, iy = (o / nx) % ny, iz = o / nxy;
+        if (ix == 0 || iy == 0 || iz == 0 || ix + 1 == nx || iy + 1 == ny || iz + 1 == nz) {
+            r32[3 * o] = r32[3 * o + 1] = r32[3 * o + 2] = 0.f;
+            continue;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+#define U(off) ((double)hi[3 * (off) + k] + (double)lo[3 * (off) + k])
+            const double uc = U(o);
+            const double lap = (U(o + 1) + U(o - 1) - 2.0 * uc) * r2x +
+                               (U(o + nx) + U(o - nx) - 2.0 * uc) * r2y +
+                               (U(o + nxy) + U(o - nxy) - 2.0 * uc) * r2z;
+#undef U
+            const double fc = f[3 * o + k];
+            const double res = fc - lap;
+            r32[3 * o + k] = (float)res;
+            num += res * res;
+            den += fc * fc;
+        }
+    }
+    sn[threadIdx.x] = num;
+    sd[threadIdx.x] = den;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sn[threadIdx.x] += sn[threadIdx.x + w];
+            sd[threadIdx.x] += sd[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sn[0];
+        part[2 * blockIdx.x + 1] = sd[0];
+    }
+}
+
+// (hi, lo) += e with an error-free TwoSum per component.
+__global__ void k_twosum_add(float *__restrict__ hi, float *__restrict__ lo,
+                             const float *__restrict__ e, size_t n) {
+#pragma clang fp contract(off)
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float a = hi[i], b = e[i] + lo[i];
+    const float s = a + b;
+    const float bb = s - a;
+    const float err = (a - (s - bb)) + (b - bb);
+    hi[i] = s;
+    lo[i] = err;
+}
+
+constexpr uint32_t kNormBlocks = 512;
+
+vg_status fp64_residual(vg_multigrid *m, const vg_grid_desc &g, const float *hi, const float *lo,
+                        const float *f, float *r32, double *rel, hipStream_t s) {
+    const double rx = 1.0 / (double)g.cell[0], ry = 1.0 / (double)g.cell[1],
+                 rz = 1.0 / (double)g.cell[2];
+    VG_TRY(launch("vg_mg_residual_f64", k_residual_pair, dim3(kNormBlocks), dim3(256), 0, s, g, hi,
+                  lo, f, r32, rx * rx, ry * ry, rz * rz, m->part));
+    std::vector<double> h(2 * kNormBlocks);
+    VG_HIP(hipMemcpyAsync(h.data(), m->part, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    VG_HIP(hipStreamSynchronize(s));
+    double num = 0.0, den = 0.0;
+    for (uint32_t b = 0; b < kNormBlocks; ++b) {
+        num += h[2 * b];
+        den += h[2 * b + 1];
+    }
+    *rel = den > 0.0 ? sqrt(num / den) : sqrt(num);
+    return VG_OK;
+}
+
+vg_status smooth(const vg_grid_desc &g, float *u, const float *f, int sweeps, hipStream_t s) {
+    if (g.np[0] < 3 || g.np[1] < 3 || g.np[2] < 3) return VG_OK;
+    const Lap c = lap_coef(g);
+    const dim3 grid(ceil_div((g.np[0] - 1) / 2 + 1, 64), g.np[1] - 2, g.np[2] - 2);
+    for (int it = 0; it < sweeps; ++it) {
+        VG_TRY(launch("vg_mg_rbgs", k_rbgs, grid, dim3(64), 0, s, g, u, f, 0, c));
+        VG_TRY(launch("vg_mg_rbgs", k_rbgs, grid, dim3(64), 0, s, g, u, f, 1, c));
+    }
+    return VG_OK;
+}
+
+// One V-cycle on level l (u[l] holds the initial guess).
+vg_status vcycle(vg_multigrid *m, uint32_t l, hipStream_t s) {
+    const vg_grid_desc &g = m->vg[l];
+    const size_t cap = capacity(g);
+    if (l + 1 == m->vdepth) {
+        double *work = nullptr;
+        VG_TRY(salloc((void **)&work, 3 * cap * sizeof(double), s));
+        uint32_t dmax = g.np[0];
+        if (g.np[1] > dmax) dmax = g.np[1];
+        if (g.np[2] > dmax) dmax = g.np[2];
+        int sweeps = (int)(4 * dmax * dmax + 64);
+        if (sweeps > 20000) sweeps = 20000;
+        vg_status st = launch("vg_mg_coarse_f64", k_coarse_f64, dim3(1), dim3(256), 0, s, g, m->u[l],
+                              (const float *)m->f[l], sweeps, work);
+        sfree(work, s);
+        return st;
+    }
+    const int nu1 = 2, nu2 = 2;
+    VG_TRY(smooth(g, m->u[l], m->f[l], nu1, s));
+    VG_TRY(launch("vg_mg_residual", k_residual32, dim3(ceil_div(cap, 256)), dim3(256), 0, s, g,
+                  (const float *)m->u[l], (const float *)m->f[l], m->r[l], lap_coef(g)));
+    const vg_grid_desc &gc = m->vg[l + 1];
+    const size_t capc = capacity(gc);
+    VG_TRY(launch("vg_mg_restrict_fw", k_restrict_fw, dim3(ceil_div(capc, 256)), dim3(256), 0, s,
+                  gc, m->f[l + 1], g, (const float *)m->r[l]));
+    VG_TRY(launch("vg_mg_zero", k_zero_all, dim3(ceil_div(3 * capc, 256)), dim3(256), 0, s,
+                  m->u[l + 1], 3 * capc));
+    VG_TRY(vcycle(m, l + 1, s));
+    VG_TRY(launch("vg_mg_prolong_add", k_prolong_add, dim3(ceil_div(cap, 256)), dim3(256), 0, s, g,
+                  m->u[l], gc, (const float *)m->u[l + 1]));
+    return smooth(g, m->u[l], m->f[l], nu2, s);
+}
+
+}  // namespace
+}  // namespace vg
+
+using namespace vg;
+
+VG_API vg_status vg_mg_create(vg_multigrid **m) {
+    VG_REQUIRE(m, "out is null");
+    *m = new vg_multigrid();
+    return VG_OK;
+}
+
+VG_API vg_status vg_mg_destroy(vg_multigrid *m) {
+    if (!m) return VG_OK;
+    for (uint32_t l = 0; l < kMaxLayers; ++l) {
+        hipFree(m->A[l]);
+        hipFree(m->W[l]);
+        hipFree(m->u[l]);
+        hipFree(m->f[l]);
+        hipFree(m->r[l]);
+    }
+    hipFree(m->lo);
+    hipFree(m->part);
+    delete m;
+    return VG_OK;
+}
+
+VG_API vg_status vg_mg_cascade(vg_multigrid *m, const vg_grid_desc *g, float *a0,
+                               const float *negw0, uint32_t steps, vg_stream stream) {
+    VG_REQUIRE(m && g && a0 && negw0, "null argument");
+    VG_REQUIRE(g->np[0] >= 2 && g->np[1] >= 2 && g->np[2] >= 2, "grid is empty");
+    hipStream_t s = (hipStream_t)stream;
+    nested_init(&m->ng, *g);
+    const Nested &ng = m->ng;
+    for (uint32_t l = 1; l < ng.depth; ++l) {
+        size_t c1 = m->cap[l], c2 = m->cap[l];
+        VG_TRY(grow(&m->A[l], &c1, 3 * capacity(ng.layer[l])));
+        VG_TRY(grow(&m->W[l], &c2, 3 * capacity(ng.layer[l])));
+        m->cap[l] = c1 < c2 ? c1 : c2;
+    }
+    float *A[kMaxLayers];
+    const float *W[kMaxLayers];
+    A[0] = a0;
+    W[0] = negw0;
+    for (uint32_t l = 1; l < ng.depth; ++l) {
+        A[l] = m->A[l];
+        W[l] = m->W[l];
+    }
+    // Cold start: interior of the finest layer is zero (vortonSim.cpp:2531).
+    VG_TRY(launch("vg_mg_zero_interior", k_zero_interior, dim3(ceil_div(capacity(*g), 256)),
+                  dim3(256), 0, s, *g, a0));
+    VG_TRY(sor_solve(ng.layer[0], A[0], W[0], steps, s));
+    uint32_t max_valid = 0;
+    for (uint32_t l = 1; l < ng.depth; ++l) {
+        const uint32_t *np = ng.layer[l].np;
+        uint32_t md = np[0];
+        if (np[1] < md) md = np[1];
+        if (np[2] < md) md = np[2];
+        if (md > 2) {
+            VG_TRY(inject(ng.layer[l], m->W[l], ng.layer[l - 1], W[l - 1], s));
+            VG_TRY(inject(ng.layer[l], A[l], ng.layer[l - 1], A[l - 1], s));
+            VG_TRY(sor_solve(ng.layer[l], A[l], W[l], steps, s));
+        } else {
+            max_valid = l - 1;
+            break;
+        }
+    }
+    for (uint32_t l = max_valid; l >= 1; --l) {
+        VG_TRY(prolong_interior(ng.layer[l - 1], A[l - 1], ng.layer[l], A[l], s));
+        VG_TRY(sor_solve(ng.layer[l - 1], A[l - 1], W[l - 1], steps, s));
+    }
+    return VG_OK;
+}
+
+VG_API vg_status vg_mg_vcycle_solve(vg_multigrid *m, const vg_grid_desc *g, float *a0,
+                                    const float *negw0, double tol, uint32_t max_cycles,
+                                    double *rel_residual, uint32_t *cycles, vg_stream stream) {
+    VG_REQUIRE(m && g && a0 && negw0, "null argument");
+    VG_REQUIRE(g->np[0] >= 3 && g->np[1] >= 3 && g->np[2] >= 3, "grid has no interior");
+    VG_REQUIRE(g->cell[0] > 0.f && g->cell[1] > 0.f && g->cell[2] > 0.f, "degenerate spacing");
+    hipStream_t s = (hipStream_t)stream;
+    // Hierarchy: halve every axis while all have >= 4 even cells.
+    m->vg[0] = *g;
+    uint32_t d = 1;
+    while (d < kMaxLayers) {
+        const vg_grid_desc &p = m->vg[d - 1];
+        bool ok = true;
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t c = p.np[a] - 1;
+            if (c < 4 || (c & 1u)) ok = false;
+        }
+        if (!ok) break;
+        decimate(&m->vg[d], &p, 2);
+        ++d;
+    }
+    m->vdepth = d;
+    // The coarsest layer is solved by one workgroup; refuse hierarchies that
+    // cannot coarsen far enough for that to stay cheap.
+    VG_REQUIRE(capacity(m->vg[d - 1]) <= 32768,
+               "grid too anisotropic for the V-cycle hierarchy (coarsest layer > 32768 points)");
+    const size_t cap0 = capacity(*g);
+    for (uint32_t l = 0; l < d; ++l) {
+        const size_t c = 3 * capacity(m->vg[l]);
+        size_t a = m->vcap[l], b = m->vcap[l], e = m->vcap[l];
+        VG_TRY(grow(&m->u[l], &a, c));
+        VG_TRY(grow(&m->f[l], &b, c));
+        VG_TRY(grow(&m->r[l], &e, c));
+        m->vcap[l] = a < b ? (a < e ? a : e) : (b < e ? b : e);
+    }
+    VG_TRY(grow(&m->lo, &m->lo_cap, 3 * cap0));
+    if (!m->part) VG_HIP(hipMalloc((void **)&m->part, 2 * kNormBlocks * sizeof(double)));
+    VG_TRY(launch("vg_mg_zero", k_zero_all, dim3(ceil_div(3 * cap0, 256)), dim3(256), 0, s, m->lo,
+                  3 * cap0));
+    double rel = 0.0;
+    uint32_t it = 0;
+    // f[0] holds the FP32 residual r = -w - L(hi + lo); u[0] the correction.
+    VG_TRY(fp64_residual(m, *g, a0, m->lo, negw0, m->f[0], &rel, s));
+    while (rel > tol && it < max_cycles) {
+        VG_TRY(launch("vg_mg_zero", k_zero_all, dim3(ceil_div(3 * cap0, 256)), dim3(256), 0, s,
+                      m->u[0], 3 * cap0));
+        VG_TRY(vcycle(m, 0, s));
+        VG_TRY(launch("vg_mg_twosum", k_twosum_add, dim3(ceil_div(3 * cap0, 256)), dim3(256), 0, s,
+                      a0, m->lo, (const float *)m->u[0], 3 * cap0));
+        ++it;
+        VG_TRY(fp64_residual(m, *g, a0, m->lo, negw0, m->f[0], &rel, s));
+    }
+    if (rel_residual) *rel_residual = rel;
+    if (cycles) *cycles = it;
+    return VG_OK;
+}

@@ -1,0 +1,825 @@
+// vg_tree.hip -- the NestedGrid influence tree (VortonSim::CreateInfluenceTree,
+// vortonSim.cpp:1192-1624) and everything that walks it: treecode velocity
+// and vector potential (:1650-1905), boundary-face vector potential
+// (:2055-2138), and the vorton -> grid splats that reuse its leaf cell lists
+// (PopulateVorticityGridFromVortons :2372-2436, Particles::
+// PopulateVelocityGrid pclOpPopulateVelocityGrid.cpp:33-87).
+//
+// Build (all deterministic, no float atomics):
+//   bin     vorton -> leaf cell key (truncation, as OffsetOfPosition),
+//           slot = atomicAdd(count[key])
+//   scan    count -> cell start (two-level exclusive scan)
+//   scatter sorted_idx[start + slot] = i, then each cell's short index list
+//           is insertion-sorted so cells hold vortons in index order
+//   leaf    per leaf point: sequential |w|-weighted sums in index order
+//           (bit-identical to MakeBaseVortonGrid's serial loop)
+//   levels  per parent point: AggregateClusters' 2x2x2 sums in z,y,x order
+// Layers are dense over grid points (SoA float4 pairs), like the reference's
+// UniformGrid<Vorton> layers, so traversal is pure index arithmetic.
+#include <cstring>
+
+#include "vg_internal.h"
+
+struct vg_tree {
+    vg::Nested ng;
+    float4 *P[vg::kMaxLayers] = {};
+    float4 *W[vg::kMaxLayers] = {};
+    size_t layer_cap[vg::kMaxLayers] = {};
+    uint32_t *cell_start = nullptr, *cell_count = nullptr;
+    size_t cell_cap = 0;
+    uint32_t *keys = nullptr, *slots = nullptr, *sorted_idx = nullptr;
+    float4 *sorted_p = nullptr, *sorted_w = nullptr;
+    size_t n_cap = 0;
+    uint32_t *scan_tmp = nullptr;
+    size_t scan_cap = 0;
+    uint32_t n = 0;
+    bool built = false;
+};
+
+namespace vg {
+namespace {
+
+// ------------------------------------------------------------------ scan
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 4;  // elements per thread
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan of one tile per block; block totals to sums (if non-null).
+__global__ __launch_bounds__(kScanBlock) void k_scan_tile(const uint32_t *__restrict__ in,
+                                                          uint32_t *__restrict__ out, size_t n,
+                                                          uint32_t *__restrict__ sums) {
+    __shared__ uint32_t wsum[kScanBlock / 64];
+    const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t tot = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = base + k < n ? in[base + k] : 0u;
+        tot += v[k];
+    }
+    const uint32_t incl = wave_incl_scan(tot);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t woff = 0;
+    for (int w = 0; w < wave; ++w) woff += wsum[w];
+    uint32_t run = woff + incl - tot;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (sums && threadIdx.x == kScanBlock - 1) sums[blockIdx.x] = run;
+}
+
+__global__ void k_scan_add(uint32_t *__restrict__ out, size_t n,
+                           const uint32_t *__restrict__ offs) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += offs[i / kScanTile];
+}
+
+size_t scan_scratch_elems(size_t n) {
+    size_t total = 0;
+    while (n > (size_t)kScanTile) {
+        n = (n + kScanTile - 1) / kScanTile;
+        total += 2 * n;
+    }
+    return total + 1;
+}
+
+// Exclusive scan in -> out using tmp (scan_scratch_elems(n) elements).
+vg_status exclusive_scan(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp,
+                         hipStream_t s) {
+    if (n == 0) return VG_OK;
+    const uint32_t nb = ceil_div(n, kScanTile);
+    if (nb == 1)
+        return launch("vg_scan_tile", k_scan_tile, dim3(1), dim3(kScanBlock), 0, s, in, out, n,
+                      (uint32_t *)nullptr);
+    uint32_t *sums = tmp, *sums_scan = tmp + nb;
+    VG_TRY(launch("vg_scan_tile", k_scan_tile, dim3(nb), dim3(kScanBlock), 0, s, in, out, n, sums));
+    VG_TRY(exclusive_scan(sums, sums_scan, nb, tmp + 2 * nb, s));
+    return launch("vg_scan_add", k_scan_add, dim3(ceil_div(n, 256)), dim3(256), 0, s, out, n,
+                  (const uint32_t *)sums_scan);
+}
+
+// ------------------------------------------------------------------ build
+struct VortonIn {
+    const float *pos, *angvel, *size;
+    size_t stride;
+    float usize;
+};
+
+__global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// MakeBaseVortonGrid binning: OffsetOfPosition (uniformGrid.h:645-651).
+__global__ void k_bin(VortonIn vin, size_t n, vg_grid_desc g, uint32_t *__restrict__ count,
+                      uint32_t *__restrict__ keys, uint32_t *__restrict__ slots) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *p = vin.pos + i * vin.stride;
+    const int ix = cell_of(p[0], g.min[0], g.cpe[0], (int)g.np[0] - 1);
+    const int iy = cell_of(p[1], g.min[1], g.cpe[1], (int)g.np[1] - 1);
+    const int iz = cell_of(p[2], g.min[2], g.cpe[2], (int)g.np[2] - 1);
+    const uint32_t key = (uint32_t)point_offset(g, ix, iy, iz);
+    keys[i] = key;
+    slots[i] = atomicAdd(&count[key], 1u);
+}
+
+__global__ void k_scatter(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ slots,
+                          const uint32_t *__restrict__ start, size_t n,
+                          uint32_t *__restrict__ sorted_idx) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) sorted_idx[start[keys[i]] + slots[i]] = (uint32_t)i;
+}
+
+// Restore index order inside each cell (the atomic slots are arbitrary).
+__global__ void k_sort_cells(const uint32_t *__restrict__ start,
+                             const uint32_t *__restrict__ count, size_t ncells,
+                             uint32_t *__restrict__ sorted_idx) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    const uint32_t cnt = count[c];
+    if (cnt < 2) return;
+    uint32_t *seg = sorted_idx + start[c];
+    for (uint32_t a = 1; a < cnt; ++a) {
+        const uint32_t v = seg[a];
+        uint32_t b = a;
+        while (b > 0 && seg[b - 1] > v) {
+            seg[b] = seg[b - 1];
+            --b;
+        }
+        seg[b] = v;
+    }
+}
+
+__global__ void k_gather_sorted(VortonIn vin, const uint32_t *__restrict__ sorted_idx, size_t n,
+                                float4 *__restrict__ sp, float4 *__restrict__ sw) {
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const size_t i = sorted_idx[j];
+    const float *p = vin.pos + i * vin.stride;
+    const float *w = vin.angvel + i * vin.stride;
+    const float sz = vin.size ? vin.size[i * vin.stride] : vin.usize;
+    sp[j] = make_float4(p[0], p[1], p[2], sz);
+    sw[j] = make_float4(w[0], w[1], w[2], 0.f);
+}
+
+// MakeBaseVortonGrid accumulation + normalisation (vortonSim.cpp:1237-1310).
+__global__ void k_leaf(const uint32_t *__restrict__ start, const uint32_t *__restrict__ count,
+                       const float4 *__restrict__ sp, const float4 *__restrict__ sw, size_t cap,
+                       float4 *__restrict__ P, float4 *__restrict__ W) {
+#pragma clang fp contract(off)
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cap) return;
+    const uint32_t cnt = count[c];
+    float px = 0.f, py = 0.f, pz = 0.f, wx = 0.f, wy = 0.f, wz = 0.f, sz = 0.f;
+    float aux = FLT_MIN;
+    const uint32_t b = cnt ? start[c] : 0u;
+    for (uint32_t k = 0; k < cnt; ++k) {
+        const float4 p = sp[b + k], w = sw[b + k];
+        const float vx = w.x * 2.0f, vy = w.y * 2.0f, vz = w.z * 2.0f;
+        const float mag = sqrtf(vx * vx + vy * vy + vz * vz);
+        px += p.x * mag; py += p.y * mag; pz += p.z * mag;
+        wx += w.x; wy += w.y; wz += w.z;
+        sz = p.w;
+        aux += mag;
+    }
+    if (aux != FLT_MIN) {
+        const float inv = 1.0f / aux;
+        px *= inv; py *= inv; pz *= inv;
+    }
+    P[c] = make_float4(px, py, pz, sz);
+    W[c] = make_float4(wx, wy, wz, aux);
+}
+
+// AggregateClusters (vortonSim.cpp:1337-1569) for one parent layer.
+__global__ void k_aggregate(vg_grid_desc gp, vg_grid_desc gc, uint32_t dx, uint32_t dy,
+                            uint32_t dz, const float4 *__restrict__ Pc,
+                            const float4 *__restrict__ Wc, float4 *__restrict__ Pp,
+                            float4 *__restrict__ Wp) {
+#pragma clang fp contract(off)
+    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t cap = capacity(gp);
+    if (o >= cap) return;
+    const uint32_t ix = (uint32_t)(o % gp.np[0]);
+    const uint32_t iy = (uint32_t)((o / gp.np[0]) % gp.np[1]);
+    const uint32_t iz = (uint32_t)(o / ((size_t)gp.np[0] * gp.np[1]));
+    float px = 0.f, py = 0.f, pz = 0.f, wx = 0.f, wy = 0.f, wz = 0.f, sz = 0.f;
+    float aux = FLT_MIN;
+    if (ix < gp.np[0] - 1 && iy < gp.np[1] - 1 && iz < gp.np[2] - 1) {
+        for (uint32_t cz = 0; cz < dz; ++cz)
+            for (uint32_t cy = 0; cy < dy; ++cy)
+                for (uint32_t cx = 0; cx < dx; ++cx) {
+                    const size_t co = point_offset(gc, ix * dx + cx, iy * dy + cy, iz * dz + cz);
+                    const float4 p = Pc[co], w = Wc[co];
+                    const float vx = w.x * 2.0f, vy = w.y * 2.0f, vz = w.z * 2.0f;
+                    const float mag = sqrtf(vx * vx + vy * vy + vz * vz);
+                    px += p.x * mag; py += p.y * mag; pz += p.z * mag;
+                    wx += w.x; wy += w.y; wz += w.z;
+                    aux += mag;
+                    if (p.w != 0.0f) sz = p.w;
+                }
+        const float inv = 1.0f / aux;
+        px *= inv; py *= inv; pz *= inv;
+    } else {
+        aux = 0.f;
+    }
+    Pp[o] = make_float4(px, py, pz, sz);
+    Wp[o] = make_float4(wx, wy, wz, aux);
+}
+
+template <typename T>
+vg_status grow(T **p, size_t *cap, size_t need) {
+    if (need <= *cap && *p) return VG_OK;
+    if (*p) VG_HIP(hipFree(*p));
+    *p = nullptr;
+    *cap = 0;
+    size_t alloc = need + need / 8 + 64;
+    VG_HIP(hipMalloc((void **)p, alloc * sizeof(T)));
+    *cap = alloc;
+    return VG_OK;
+}
+
+// ------------------------------------------------------------------ traversal
+struct QuerySrc {
+    const float *q;
+    size_t stride;
+    size_t nq;
+    vg_grid_desc g;  // grid/face queries
+};
+
+// Reference opening rule (vortonSim.cpp:1819-1856): a child cell is opened
+// when the query lies in [min - m, max + m) on every axis, m = 1e-4 * child
+// spacing; all other children are applied as supervortons; the children of
+// opened layer-1 cells (the leaves) are always applied.  The opened set on a
+// layer is a box of at most 2 cells per axis, so the walk is a loop over
+// layers with no stack.  Sum order differs from the recursion (rounding
+// only); opening decisions are bit-identical.
+template <int POLICY, int VECPOT, int COUNT = 0>
+__device__ float3 tree_eval(const TreeView &tv, float3 q, float spread, uint32_t *count = nullptr) {
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    float3 sum = make_float3(0.f, 0.f, 0.f), comp = sum;
+    const float qa[3] = {q.x, q.y, q.z};
+    for (int il = (int)tv.depth - 1; il >= 1; --il) {
+        const vg_grid_desc &gc = tv.layer[il - 1];
+        const uint32_t *dims = tv.dec[il];
+        int c0[3], c1[3], olo[3], ohi[3];
+        bool open_any = il > 1;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            c0[a] = lo[a] * (int)dims[a];
+            c1[a] = hi[a] * (int)dims[a] + (int)dims[a] - 1;
+            olo[a] = 1 << 30;
+            ohi[a] = -1;
+        }
+        if (open_any) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float cell = gc.cell[a];
+                float m;
+                {
+#pragma clang fp contract(off)
+                    m = 0.0001f * cell;
+                    if (a == 2 && 0.0f == cell) m = m + FLT_MIN;
+                }
+                for (int c = c0[a]; c <= c1[a]; ++c) {
+                    const float mn = madd_exact(gc.min[a], (float)c, cell);
+                    const float mx = madd_exact(gc.min[a], (float)(c + 1), cell);
+                    bool in;
+                    {
+#pragma clang fp contract(off)
+                        in = qa[a] >= mn - m && qa[a] < mx + m;
+                    }
+                    if (in) {
+                        olo[a] = c < olo[a] ? c : olo[a];
+                        ohi[a] = c > ohi[a] ? c : ohi[a];
+                    }
+                }
+                if (ohi[a] < olo[a]) open_any = false;
+            }
+        }
+        const float4 *__restrict__ P = tv.P[il - 1];
+        const float4 *__restrict__ W = tv.W[il - 1];
+        float3 part = make_float3(0.f, 0.f, 0.f);
+        for (int cz = c0[2]; cz <= c1[2]; ++cz)
+            for (int cy = c0[1]; cy <= c1[1]; ++cy)
+                for (int cx = c0[0]; cx <= c1[0]; ++cx) {
+                    if (open_any && cx >= olo[0] && cx <= ohi[0] && cy >= olo[1] &&
+                        cy <= ohi[1] && cz >= olo[2] && cz <= ohi[2])
+                        continue;
+                    const size_t off = point_offset(gc, cx, cy, cz);
+                    const float4 p = P[off];
+                    if (p.w == 0.0f) continue;  // empty cluster: zero circulation
+                    if (COUNT) {
+                        ++*count;
+                        continue;
+                    }
+                    const float4 w = W[off];
+                    const float radius = p.w * 0.5f * spread;
+                    const float r2 = radius * radius;
+                    const float s = (2.0f / 3.0f) * r2 * radius;
+                    const float4 A = make_float4(p.x, p.y, p.z, r2);
+                    const float4 B =
+                        make_float4(w.x * s, w.y * s, w.z * s, __builtin_amdgcn_rcpf(r2 * radius));
+                    if (VECPOT) pair_vecpot<POLICY>(part, q, A, B);
+                    else pair_velocity<POLICY>(part, q, A, B);
+                }
+        kahan_add(sum, comp, part);
+        if (!open_any) break;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { lo[a] = olo[a]; hi[a] = ohi[a]; }
+    }
+    return sum;
+}
+
+// Direct fallback when the tree has a single layer (vortonSim.cpp:2276-2279).
+template <int POLICY, int VECPOT>
+__device__ float3 sorted_direct(const TreeView &tv, float3 q, float spread) {
+    float3 sum = make_float3(0.f, 0.f, 0.f), comp = sum;
+    float3 part = sum;
+    for (uint32_t j = 0; j < tv.n; ++j) {
+        float4 A, B;
+        prepare_source(tv.sorted_p[j], tv.sorted_w[j], spread, A, B);
+        if (VECPOT) pair_vecpot<POLICY>(part, q, A, B);
+        else pair_velocity<POLICY>(part, q, A, B);
+        if ((j & 255u) == 255u) {
+            kahan_add(sum, comp, part);
+            part = make_float3(0.f, 0.f, 0.f);
+        }
+    }
+    kahan_add(sum, comp, part);
+    return sum;
+}
+
+template <int POLICY, int VECPOT>
+__device__ __forceinline__ float3 eval_query(const TreeView &tv, float3 q, float spread) {
+    return tv.depth > 1 ? tree_eval<POLICY, VECPOT>(tv, q, spread)
+                        : sorted_direct<POLICY, VECPOT>(tv, q, spread);
+}
+
+template <int POLICY, int VECPOT>
+__global__ __launch_bounds__(256) void k_tree_points(const TreeView tv, QuerySrc qs, float spread,
+                                                     float circ, float *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= qs.nq) return;
+    const float *p = qs.q + i * qs.stride;
+    const float3 r = eval_query<POLICY, VECPOT>(tv, make_float3(p[0], p[1], p[2]), spread);
+    out[3 * i] = r.x * circ;
+    out[3 * i + 1] = r.y * circ;
+    out[3 * i + 2] = r.z * circ;
+}
+
+template <int POLICY>
+__global__ __launch_bounds__(256) void k_tree_grid(const TreeView tv, vg_grid_desc g, float spread,
+                                                   float circ, float *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= capacity(g)) return;
+    const uint32_t ix = (uint32_t)(i % g.np[0]);
+    const uint32_t iy = (uint32_t)((i / g.np[0]) % g.np[1]);
+    const uint32_t iz = (uint32_t)(i / ((size_t)g.np[0] * g.np[1]));
+    const float3 r = eval_query<POLICY, 0>(tv, grid_query_point(g, ix, iy, iz), spread);
+    out[3 * i] = r.x * circ;
+    out[3 * i + 1] = r.y * circ;
+    out[3 * i + 2] = r.z * circ;
+}
+
+__global__ __launch_bounds__(256) void k_tree_count(const TreeView tv, vg_grid_desc g,
+                                                    unsigned long long *__restrict__ total) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t c = 0;
+    if (i < capacity(g)) {
+        const uint32_t ix = (uint32_t)(i % g.np[0]);
+        const uint32_t iy = (uint32_t)((i / g.np[0]) % g.np[1]);
+        const uint32_t iz = (uint32_t)(i / ((size_t)g.np[0] * g.np[1]));
+        if (tv.depth > 1) tree_eval<0, 0, 1>(tv, grid_query_point(g, ix, iy, iz), 1.0f, &c);
+        else c = tv.n;
+    }
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(total, (unsigned long long)c);
+}
+
+// Face point enumeration: z=0 plane, z=nz-1 plane, then for each interior
+// z the y=0 row, the y=ny-1 row and the x=0 / x=nx-1 points of interior rows.
+__device__ __forceinline__ bool face_point(const vg_grid_desc &g, size_t t, uint32_t &ix,
+                                           uint32_t &iy, uint32_t &iz) {
+    const size_t nx = g.np[0], ny = g.np[1], nz = g.np[2];
+    const size_t plane = nx * ny;
+    if (t < plane) {
+        iz = 0; iy = (uint32_t)(t / nx); ix = (uint32_t)(t % nx);
+        return true;
+    }
+    t -= plane;
+    if (nz > 1) {
+        if (t < plane) {
+            iz = (uint32_t)(nz - 1); iy = (uint32_t)(t / nx); ix = (uint32_t)(t % nx);
+            return true;
+        }
+        t -= plane;
+    }
+    if (nz <= 2) return false;
+    const size_t side = ny > 2 ? 2 * (ny - 2) : 0;
+    const size_t slab = 2 * nx + side;
+    const size_t zz = t / slab;
+    if (zz >= nz - 2) return false;
+    size_t r = t % slab;
+    iz = (uint32_t)(1 + zz);
+    if (r < nx) { iy = 0; ix = (uint32_t)r; return true; }
+    r -= nx;
+    if (r < nx) { iy = (uint32_t)(ny - 1); ix = (uint32_t)r; return true; }
+    r -= nx;
+    iy = (uint32_t)(1 + r / 2);
+    ix = (r & 1) ? (uint32_t)(nx - 1) : 0u;
+    return true;
+}
+
+size_t face_count(const vg_grid_desc &g) {
+    const size_t nx = g.np[0], ny = g.np[1], nz = g.np[2];
+    size_t c = nx * ny * (nz > 1 ? 2 : 1);
+    if (nz > 2) c += (nz - 2) * (2 * nx + (ny > 2 ? 2 * (ny - 2) : 0));
+    return c;
+}
+
+template <int POLICY>
+__global__ __launch_bounds__(256) void k_tree_faces(const TreeView tv, vg_grid_desc g,
+                                                    size_t nface, int copy_odd, float spread,
+                                                    float circ, float *__restrict__ out) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nface) return;
+    uint32_t ix, iy, iz;
+    if (!face_point(g, t, ix, iy, iz)) return;
+    if (copy_odd && (ix & 1u)) return;
+    const float3 r = eval_query<POLICY, 1>(tv, grid_query_point(g, ix, iy, iz), spread);
+    const size_t o = point_offset(g, ix, iy, iz);
+    out[3 * o] = r.x * circ;
+    out[3 * o + 1] = r.y * circ;
+    out[3 * o + 2] = r.z * circ;
+}
+
+// Odd-x face points copy x-1 (vortonSim.cpp:2124-2129).
+__global__ void k_faces_copy_odd(vg_grid_desc g, size_t nface, float *__restrict__ out) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nface) return;
+    uint32_t ix, iy, iz;
+    if (!face_point(g, t, ix, iy, iz) || !(ix & 1u)) return;
+    const size_t o = point_offset(g, ix, iy, iz);
+    out[3 * o] = out[3 * (o - 1)];
+    out[3 * o + 1] = out[3 * (o - 1) + 1];
+    out[3 * o + 2] = out[3 * (o - 1) + 2];
+}
+
+// ------------------------------------------------------------------ splats
+// UniformGrid::Accumulate (uniformGrid.h:1278-1342) as a gather: grid point
+// (ix,iy,iz) receives, from each vorton in the <=8 cells it is a corner of,
+// weight ((wx*wy)*wz) with wx = tween.x when the point is the cell's +x
+// corner, 1-tween.x otherwise, tween clamped to [0,1].
+__device__ __forceinline__ float splat_weight(const vg_grid_desc &g, float4 p, int cx, int cy,
+                                              int cz, int px, int py, int pz) {
+#pragma clang fp contract(off)
+    float t[3];
+    const float pp[3] = {p.x, p.y, p.z};
+    const int c[3] = {cx, cy, cz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float v = (pp[a] - (g.min[a] + (float)c[a] * g.cell[a])) * g.cpe[a];
+        v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+        t[a] = v;
+    }
+    const float wx = px ? t[0] : 1.0f - t[0];
+    const float wy = py ? t[1] : 1.0f - t[1];
+    const float wz = pz ? t[2] : 1.0f - t[2];
+    return wx * wy * wz;
+}
+
+// mode 0: vorticity item (Omega*2)*vc; mode 1: velocity average.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_splat_gather(const TreeView tv, float vc,
+                                                      const float *__restrict__ vel,
+                                                      size_t vel_stride, float *__restrict__ out) {
+#pragma clang fp contract(off)
+    const vg_grid_desc &g = tv.layer[0];
+    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= capacity(g)) return;
+    const int ix = (int)(o % g.np[0]);
+    const int iy = (int)((o / g.np[0]) % g.np[1]);
+    const int iz = (int)(o / ((size_t)g.np[0] * g.np[1]));
+    const int ncx = (int)g.np[0] - 1, ncy = (int)g.np[1] - 1, ncz = (int)g.np[2] - 1;
+    float ax = 0.f, ay = 0.f, az = 0.f, wsum = 0.f;
+    for (int cz = iz - 1; cz <= iz; ++cz) {
+        if (cz < 0 || cz >= ncz) continue;
+        for (int cy = iy - 1; cy <= iy; ++cy) {
+            if (cy < 0 || cy >= ncy) continue;
+            for (int cx = ix - 1; cx <= ix; ++cx) {
+                if (cx < 0 || cx >= ncx) continue;
+                const size_t key = point_offset(g, cx, cy, cz);
+                const uint32_t cnt = tv.cell_count[key];
+                if (!cnt) continue;
+                const uint32_t b = tv.cell_start[key];
+                for (uint32_t k = 0; k < cnt; ++k) {
+                    const float4 p = tv.sorted_p[b + k];
+                    const float w = splat_weight(g, p, cx, cy, cz, cx != ix, cy != iy, cz != iz);
+                    if (MODE == 0) {
+                        const float4 om = tv.sorted_w[b + k];
+                        ax += ((om.x * 2.0f) * vc) * w;
+                        ay += ((om.y * 2.0f) * vc) * w;
+                        az += ((om.z * 2.0f) * vc) * w;
+                    } else {
+                        const float *v = vel + (size_t)tv.sorted_idx[b + k] * vel_stride;
+                        ax += v[0] * w;
+                        ay += v[1] * w;
+                        az += v[2] * w;
+                        wsum += 1.0f * w;
+                    }
+                }
+            }
+        }
+    }
+    if (MODE == 1 && wsum > 0.0f) {
+        const float inv = 1.0f / wsum;
+        ax *= inv; ay *= inv; az *= inv;
+    }
+    out[3 * o] = ax;
+    out[3 * o + 1] = ay;
+    out[3 * o + 2] = az;
+}
+
+TreeView make_view(const vg_tree *t) {
+    TreeView v;
+    memset(&v, 0, sizeof v);
+    v.depth = t->ng.depth;
+    for (uint32_t l = 0; l < kMaxLayers; ++l) {
+        v.layer[l] = t->ng.layer[l];
+        for (int a = 0; a < 3; ++a) v.dec[l][a] = t->ng.dec[l][a];
+        v.P[l] = t->P[l];
+        v.W[l] = t->W[l];
+    }
+    v.cell_start = t->cell_start;
+    v.cell_count = t->cell_count;
+    v.sorted_p = t->sorted_p;
+    v.sorted_w = t->sorted_w;
+    v.sorted_idx = t->sorted_idx;
+    v.n = t->n;
+    return v;
+}
+
+}  // namespace
+
+// Used by the simulation driver.
+TreeView tree_view(const vg_tree *t) { return make_view(t); }
+
+}  // namespace vg
+
+using namespace vg;
+
+VG_API vg_status vg_tree_create(vg_tree **t) {
+    VG_REQUIRE(t, "tree out is null");
+    *t = new vg_tree();
+    return VG_OK;
+}
+
+VG_API vg_status vg_tree_destroy(vg_tree *t) {
+    if (!t) return VG_OK;
+    for (uint32_t l = 0; l < kMaxLayers; ++l) {
+        hipFree(t->P[l]);
+        hipFree(t->W[l]);
+    }
+    hipFree(t->cell_start);
+    hipFree(t->cell_count);
+    hipFree(t->keys);
+    hipFree(t->slots);
+    hipFree(t->sorted_idx);
+    hipFree(t->sorted_p);
+    hipFree(t->sorted_w);
+    hipFree(t->scan_tmp);
+    delete t;
+    return VG_OK;
+}
+
+VG_API vg_status vg_tree_build(vg_tree *t, const vg_grid_desc *tmpl, const vg_vortons *v,
+                               vg_stream stream) {
+    VG_REQUIRE(t && tmpl && v, "null argument");
+    VG_REQUIRE(v->n == 0 || (v->pos && v->angvel && v->stride >= 3), "bad vorton view");
+    VG_REQUIRE(v->n < (1ull << 31), "too many vortons");
+    VG_REQUIRE(tmpl->np[0] >= 2 && tmpl->np[1] >= 2 && tmpl->np[2] >= 2,
+               "grid template is empty (define its shape first)");
+    VG_REQUIRE(capacity(*tmpl) < (1ull << 32), "grid template too large");
+    hipStream_t s = (hipStream_t)stream;
+    t->built = false;
+    nested_init(&t->ng, *tmpl);
+    const size_t n = v->n;
+    const size_t cap0 = capacity(t->ng.layer[0]);
+    for (uint32_t l = 0; l < t->ng.depth; ++l) {
+        const size_t cap = capacity(t->ng.layer[l]);
+        size_t c1 = t->layer_cap[l], c2 = t->layer_cap[l];
+        VG_TRY(grow(&t->P[l], &c1, cap));
+        VG_TRY(grow(&t->W[l], &c2, cap));
+        t->layer_cap[l] = c1 < c2 ? c1 : c2;
+    }
+    size_t cc1 = t->cell_cap, cc2 = t->cell_cap;
+    VG_TRY(grow(&t->cell_start, &cc1, cap0));
+    VG_TRY(grow(&t->cell_count, &cc2, cap0));
+    t->cell_cap = cc1 < cc2 ? cc1 : cc2;
+    const size_t nn = n ? n : 1;
+    size_t n1 = t->n_cap, n2 = t->n_cap, n3 = t->n_cap, n4 = t->n_cap, n5 = t->n_cap;
+    VG_TRY(grow(&t->keys, &n1, nn));
+    VG_TRY(grow(&t->slots, &n2, nn));
+    VG_TRY(grow(&t->sorted_idx, &n3, nn));
+    VG_TRY(grow(&t->sorted_p, &n4, nn));
+    VG_TRY(grow(&t->sorted_w, &n5, nn));
+    t->n_cap = n1;
+    if (n2 < t->n_cap) t->n_cap = n2;
+    if (n3 < t->n_cap) t->n_cap = n3;
+    if (n4 < t->n_cap) t->n_cap = n4;
+    if (n5 < t->n_cap) t->n_cap = n5;
+    VG_TRY(grow(&t->scan_tmp, &t->scan_cap, scan_scratch_elems(cap0)));
+    t->n = (uint32_t)n;
+
+    const VortonIn vin{v->pos, v->angvel, v->size, v->stride, v->uniform_size};
+    VG_TRY(launch("vg_tree_fill", k_fill_u32, dim3(ceil_div(cap0, 256)), dim3(256), 0, s,
+                  t->cell_count, cap0, 0u));
+    VG_TRY(launch("vg_tree_bin", k_bin, dim3(ceil_div(n, 256)), dim3(256), 0, s, vin, n,
+                  t->ng.layer[0], t->cell_count, t->keys, t->slots));
+    VG_TRY(exclusive_scan(t->cell_count, t->cell_start, cap0, t->scan_tmp, s));
+    VG_TRY(launch("vg_tree_scatter", k_scatter, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                  (const uint32_t *)t->keys, (const uint32_t *)t->slots,
+                  (const uint32_t *)t->cell_start, n, t->sorted_idx));
+    VG_TRY(launch("vg_tree_sort_cells", k_sort_cells, dim3(ceil_div(cap0, 256)), dim3(256), 0, s,
+                  (const uint32_t *)t->cell_start, (const uint32_t *)t->cell_count, cap0,
+                  t->sorted_idx));
+    VG_TRY(launch("vg_tree_gather", k_gather_sorted, dim3(ceil_div(n, 256)), dim3(256), 0, s, vin,
+                  (const uint32_t *)t->sorted_idx, n, t->sorted_p, t->sorted_w));
+    VG_TRY(launch("vg_tree_leaf", k_leaf, dim3(ceil_div(cap0, 256)), dim3(256), 0, s,
+                  (const uint32_t *)t->cell_start, (const uint32_t *)t->cell_count,
+                  (const float4 *)t->sorted_p, (const float4 *)t->sorted_w, cap0, t->P[0],
+                  t->W[0]));
+    for (uint32_t l = 1; l < t->ng.depth; ++l) {
+        const size_t cap = capacity(t->ng.layer[l]);
+        VG_TRY(launch("vg_tree_aggregate", k_aggregate, dim3(ceil_div(cap, 256)), dim3(256), 0, s,
+                      t->ng.layer[l], t->ng.layer[l - 1], t->ng.dec[l][0], t->ng.dec[l][1],
+                      t->ng.dec[l][2], (const float4 *)t->P[l - 1], (const float4 *)t->W[l - 1],
+                      t->P[l], t->W[l]));
+    }
+    t->built = true;
+    return VG_OK;
+}
+
+VG_API vg_status vg_tree_depth(const vg_tree *t, uint32_t *depth) {
+    VG_REQUIRE(t && depth, "null argument");
+    VG_REQUIRE(t->built, "tree not built");
+    *depth = t->ng.depth;
+    return VG_OK;
+}
+
+VG_API vg_status vg_tree_layer(const vg_tree *t, uint32_t l, vg_grid_desc *g, float *pos,
+                               float *angvel, float *size) {
+    VG_REQUIRE(t && g, "null argument");
+    VG_REQUIRE(t->built, "tree not built");
+    VG_REQUIRE(l < t->ng.depth, "layer out of range");
+    *g = t->ng.layer[l];
+    const size_t cap = capacity(*g);
+    if (!pos && !angvel && !size) return VG_OK;
+    VG_HIP(hipDeviceSynchronize());
+    float4 *hp = new float4[cap], *hw = new float4[cap];
+    hipError_t e1 = hipMemcpy(hp, t->P[l], cap * sizeof(float4), hipMemcpyDeviceToHost);
+    hipError_t e2 = hipMemcpy(hw, t->W[l], cap * sizeof(float4), hipMemcpyDeviceToHost);
+    if (e1 == hipSuccess && e2 == hipSuccess) {
+        for (size_t i = 0; i < cap; ++i) {
+            if (pos) { pos[3 * i] = hp[i].x; pos[3 * i + 1] = hp[i].y; pos[3 * i + 2] = hp[i].z; }
+            if (angvel) { angvel[3 * i] = hw[i].x; angvel[3 * i + 1] = hw[i].y; angvel[3 * i + 2] = hw[i].z; }
+            if (size) size[i] = hp[i].w;
+        }
+    }
+    delete[] hp;
+    delete[] hw;
+    VG_HIP(e1);
+    VG_HIP(e2);
+    return VG_OK;
+}
+
+static vg_status tree_ready(const vg_tree *t, int policy) {
+    VG_REQUIRE(t, "tree is null");
+    if (!t->built) {
+        set_error("tree not built (call vg_tree_build first)");
+        return VG_ESTATE;
+    }
+    VG_REQUIRE(policy == VG_RSQRT_REFERENCE || policy == VG_RSQRT_ACCURATE, "bad rsqrt policy");
+    return VG_OK;
+}
+
+template <int VECPOT>
+static vg_status tree_points(const vg_tree *t, const float *q, size_t q_stride, size_t nq,
+                             float spread, float circ, int policy, float *out, vg_stream s,
+                             const char *name) {
+    VG_TRY(tree_ready(t, policy));
+    VG_REQUIRE(nq == 0 || (q && q_stride >= 3 && out), "bad query array");
+    if (nq == 0) return VG_OK;
+    if (t->n == 0) return zero_fill(out, 3 * nq * sizeof(float), (hipStream_t)s);
+    QuerySrc qs{q, q_stride, nq, {}};
+    auto k = policy == VG_RSQRT_REFERENCE ? k_tree_points<VG_RSQRT_REFERENCE, VECPOT>
+                                          : k_tree_points<VG_RSQRT_ACCURATE, VECPOT>;
+    return launch(name, k, dim3(ceil_div(nq, 256)), dim3(256), 0, (hipStream_t)s, make_view(t),
+                  qs, spread, circ, out);
+}
+
+VG_API vg_status vg_tree_velocity(const vg_tree *t, const float *q, size_t q_stride, size_t nq,
+                                  float spread, float circ, int rsqrt_policy, float *out,
+                                  vg_stream s) {
+    return tree_points<0>(t, q, q_stride, nq, spread, circ, rsqrt_policy, out, s,
+                          "vg_tree_velocity");
+}
+
+VG_API vg_status vg_tree_vecpot(const vg_tree *t, const float *q, size_t q_stride, size_t nq,
+                                float spread, float circ, int rsqrt_policy, float *out,
+                                vg_stream s) {
+    return tree_points<1>(t, q, q_stride, nq, spread, circ, rsqrt_policy, out, s,
+                          "vg_tree_vecpot");
+}
+
+VG_API vg_status vg_tree_velocity_grid(const vg_tree *t, const vg_grid_desc *g, float spread,
+                                       float circ, int rsqrt_policy, float *out, vg_stream s) {
+    VG_TRY(tree_ready(t, rsqrt_policy));
+    VG_REQUIRE(g && out, "null argument");
+    const size_t cap = capacity(*g);
+    if (t->n == 0) return zero_fill(out, 3 * cap * sizeof(float), (hipStream_t)s);
+    auto k = rsqrt_policy == VG_RSQRT_REFERENCE ? k_tree_grid<VG_RSQRT_REFERENCE>
+                                                : k_tree_grid<VG_RSQRT_ACCURATE>;
+    return launch("vg_tree_velocity_grid", k, dim3(ceil_div(cap, 256)), dim3(256), 0,
+                  (hipStream_t)s, make_view(t), *g, spread, circ, out);
+}
+
+VG_API vg_status vg_tree_interaction_count(const vg_tree *t, const vg_grid_desc *g,
+                                           uint64_t *count, vg_stream stream) {
+    VG_TRY(tree_ready(t, VG_RSQRT_REFERENCE));
+    VG_REQUIRE(g && count, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    unsigned long long *d = nullptr;
+    VG_TRY(salloc((void **)&d, sizeof(unsigned long long), s));
+    vg_status st = zero_fill(d, sizeof(unsigned long long), s);
+    if (st == VG_OK)
+        st = launch("vg_tree_count", k_tree_count, dim3(ceil_div(capacity(*g), 256)), dim3(256), 0,
+                    s, make_view(t), *g, d);
+    unsigned long long h = 0;
+    if (st == VG_OK) {
+        hipError_t e = hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = device_error(e, "count copy", __FILE__, __LINE__);
+    }
+    sfree(d, s);
+    *count = h;
+    return st;
+}
+
+VG_API vg_status vg_tree_vecpot_boundary(const vg_tree *t, const vg_grid_desc *g, float spread,
+                                         float circ, int rsqrt_policy, int copy_odd, float *out,
+                                         vg_stream stream) {
+    VG_TRY(tree_ready(t, rsqrt_policy));
+    VG_REQUIRE(g && out, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t nf = face_count(*g);
+    // With no vortons every cluster is empty and the faces evaluate to zero.
+    auto k =rsqrt_policy == VG_RSQRT_REFERENCE ? k_tree_faces<VG_RSQRT_REFERENCE>
+                                                : k_tree_faces<VG_RSQRT_ACCURATE>;
+    VG_TRY(launch("vg_tree_vecpot_faces", k, dim3(ceil_div(nf, 256)), dim3(256), 0, s,
+                  make_view(t), *g, nf, copy_odd, spread, circ, out));
+    if (copy_odd)
+        VG_TRY(launch("vg_tree_faces_copy_odd", k_faces_copy_odd, dim3(ceil_div(nf, 256)),
+                      dim3(256), 0, s, *g, nf, out));
+    return VG_OK;
+}
+
+VG_API vg_status vg_tree_splat_vorticity(const vg_tree *t, float vorton_radius, float scale,
+                                         float *out, vg_stream s) {
+#pragma clang fp contract(off)
+    VG_TRY(tree_ready(t, VG_RSQRT_REFERENCE));
+    VG_REQUIRE(out, "out is null");
+    const vg_grid_desc &g = t->ng.layer[0];
+    const size_t cap = capacity(g);
+    const float kPi = 3.1415926535897932384626433832795f;
+    const float cell_volume = g.cell[0] * g.cell[1] * g.cell[2];
+    const float four_pi_over3 = (4.0f * kPi) / 3.0f;
+    const float vorton_volume = four_pi_over3 * (vorton_radius * vorton_radius * vorton_radius);
+    const float vc = scale * vorton_volume / cell_volume;
+    return launch("vg_splat_vorticity", k_splat_gather<0>, dim3(ceil_div(cap, 256)), dim3(256),
+                  0, (hipStream_t)s, make_view(t), vc, (const float *)nullptr, (size_t)0, out);
+}
+
+VG_API vg_status vg_tree_populate_velocity_grid(const vg_tree *t, const float *vel,
+                                                size_t vel_stride, float *out, vg_stream s) {
+    VG_TRY(tree_ready(t, VG_RSQRT_REFERENCE));
+    VG_REQUIRE(out && (vel || t->n == 0) && vel_stride >= 3, "bad argument");
+    const size_t cap = capacity(t->ng.layer[0]);
+    return launch("vg_populate_velocity_grid", k_splat_gather<1>, dim3(ceil_div(cap, 256)),
+                  dim3(256), 0, (hipStream_t)s, make_view(t), 0.0f, vel, vel_stride, out);
+}

@@ -1,0 +1,549 @@
+"""ctypes binding of libvortegrid (include/vortegrid.h).
+
+This is the Python host side of the drop-in boundary: it loads the in-tree
+HIP library and exposes its C-ABI with numpy-friendly helpers.  There is no
+CPU fallback: if the library or a GPU is missing, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvortegrid.so")
+
+VG_OK, VG_EINVAL, VG_EMPTY, VG_EDEVICE, VG_ENOMEM, VG_ESTATE = range(6)
+RSQRT_REFERENCE, RSQRT_ACCURATE = 0, 1
+TECH_DIRECT, TECH_TREE, TECH_POISSON = 0, 1, 2
+
+# Exported entry points declared in include/vortegrid.h.
+EXPORTS = (
+    "vg_last_error vg_version vg_device_count vg_malloc vg_free vg_memcpy_htod vg_memcpy_dtoh "
+    "vg_memset vg_stream_create vg_stream_destroy vg_stream_synchronize vg_profile_enable "
+    "vg_profile_reset vg_profile_get vg_profile_dump vg_define_shape vg_decimate "
+    "vg_nested_layers vg_finalize_bounding_box vg_auto_mollify vg_find_bounding_box "
+    "vg_direct_velocity vg_direct_vecpot vg_direct_velocity_grid vg_direct_velocity_f64 "
+    "vg_tree_create vg_tree_destroy vg_tree_build vg_tree_depth vg_tree_layer "
+    "vg_tree_velocity vg_tree_velocity_grid vg_tree_interaction_count vg_tree_vecpot "
+    "vg_tree_vecpot_boundary "
+    "vg_tree_splat_vorticity vg_tree_populate_velocity_grid vg_downsample_inject "
+    "vg_upsample_interior vg_sor_solve vg_curl vg_residual_norm vg_mg_create vg_mg_destroy "
+    "vg_mg_cascade vg_mg_vcycle_solve vg_assign_velocity_from_field vg_evolve "
+    "vg_advect_tracers vg_sim_config_default vg_sim_create vg_sim_destroy vg_sim_set_vortons "
+    "vg_sim_find_bounding_box vg_sim_update_bounding_box vg_sim_update vg_sim_velocity_grid "
+    "vg_sim_vector_potential vg_sim_negative_vorticity vg_sim_grid_template vg_sim_bounds "
+    "vg_sim_tree vg_sim_spreading vg_sim_poisson_residual"
+).split()
+
+
+class VGError(RuntimeError):
+    pass
+
+
+class GridDesc(C.Structure):
+    """vg_grid_desc == UniformGridGeometry (uniformGrid.h:144-860)."""
+
+    _fields_ = [
+        ("min", C.c_float * 3),
+        ("extent", C.c_float * 3),
+        ("cell", C.c_float * 3),
+        ("cpe", C.c_float * 3),
+        ("np", C.c_uint32 * 3),
+    ]
+
+    @property
+    def capacity(self) -> int:
+        return int(self.np[0]) * int(self.np[1]) * int(self.np[2])
+
+    @property
+    def shape(self):
+        return tuple(int(v) for v in self.np)
+
+    def as_dict(self):
+        d = {k: [float(v) for v in getattr(self, k)] for k in ("min", "extent", "cell", "cpe")}
+        d["np"] = [int(v) for v in self.np]
+        return d
+
+    @classmethod
+    def from_fields(cls, other) -> "GridDesc":
+        g = cls()
+        for k in ("min", "extent", "cell", "cpe", "np"):
+            for a in range(3):
+                getattr(g, k)[a] = getattr(other, k)[a]
+        return g
+
+
+class Vortons(C.Structure):
+    _fields_ = [
+        ("pos", C.c_void_p),
+        ("angvel", C.c_void_p),
+        ("size", C.c_void_p),
+        ("stride", C.c_size_t),
+        ("n", C.c_size_t),
+        ("uniform_size", C.c_float),
+    ]
+
+
+class SimConfig(C.Structure):
+    _fields_ = [
+        ("technique", C.c_int),
+        ("rsqrt_policy", C.c_int),
+        ("auto_mollify", C.c_int),
+        ("mg_steps", C.c_uint32),
+        ("boundary_copy_odd", C.c_int),
+        ("poisson_solver", C.c_int),
+        ("mg_tol", C.c_double),
+        ("mg_max_cycles", C.c_uint32),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libvortegrid.so (built by vortegrid_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise VGError(f"{LIB_PATH} is missing: build it with `python -m vortegrid_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.vg_last_error.restype = C.c_char_p
+        L.vg_version.restype = C.c_char_p
+        for name in EXPORTS:
+            fn = getattr(L, name)
+            if name not in ("vg_last_error", "vg_version"):
+                fn.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def check(status: int, what: str = ""):
+    if status != VG_OK:
+        msg = lib().vg_last_error().decode(errors="replace")
+        raise VGError(f"{what or 'vg call'} failed with status {status}: {msg}")
+
+
+def call(name: str, *args):
+    check(getattr(lib(), name)(*args), name)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    st = lib().vg_device_count(C.byref(n))
+    return n.value if st == VG_OK else 0
+
+
+def require_gpu():
+    if device_count() < 1:
+        raise VGError("no HIP device visible: the vortegrid path needs an MI355X (gfx950)")
+
+
+# ---------------------------------------------------------------- device memory
+class DeviceArray:
+    """Owning device buffer with numpy upload/download."""
+
+    def __init__(self, nbytes: int, dtype=np.float32, shape=None):
+        self.nbytes = int(nbytes)
+        self.dtype = np.dtype(dtype)
+        self.shape = shape if shape is not None else (self.nbytes // self.dtype.itemsize,)
+        self.ptr = C.c_void_p()
+        if self.nbytes:
+            call("vg_malloc", C.byref(self.ptr), C.c_size_t(self.nbytes))
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes, a.dtype, a.shape)
+        if a.nbytes:
+            call("vg_memcpy_htod", d.ptr, a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes))
+        return d
+
+    @classmethod
+    def zeros(cls, shape, dtype=np.float32) -> "DeviceArray":
+        shape = tuple(np.atleast_1d(shape).tolist())
+        nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        d = cls(nbytes, dtype, shape)
+        if nbytes:
+            call("vg_memset", d.ptr, 0, C.c_size_t(nbytes))
+        return d
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.nbytes, self.nbytes)
+        call("vg_memcpy_htod", self.ptr, a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes))
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            call("vg_memcpy_dtoh", out.ctypes.data_as(C.c_void_p), self.ptr,
+                 C.c_size_t(self.nbytes))
+        return out
+
+    def offset(self, nfloats: int) -> C.c_void_p:
+        return C.c_void_p(self.ptr.value + nfloats * 4)
+
+    def free(self):
+        if self.ptr and self.ptr.value:
+            lib().vg_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def vortons_view(pos: DeviceArray, angvel: DeviceArray, n: int, uniform_size: float = 0.0,
+                 size: DeviceArray | None = None, stride: int = 3) -> Vortons:
+    v = Vortons()
+    v.pos = pos.ptr.value if n else None
+    v.angvel = angvel.ptr.value if n else None
+    v.size = size.ptr.value if size is not None else None
+    v.stride = stride
+    v.n = n
+    v.uniform_size = uniform_size
+    return v
+
+
+class Stream:
+    def __init__(self):
+        self.handle = C.c_void_p()
+        call("vg_stream_create", C.byref(self.handle))
+
+    def sync(self):
+        call("vg_stream_synchronize", self.handle)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().vg_stream_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def synchronize(stream=None):
+    call("vg_stream_synchronize", stream.handle if stream is not None else None)
+
+
+# ---------------------------------------------------------------- geometry
+def define_shape(n: int, vmin, vmax, pow2: bool = True) -> GridDesc:
+    g = GridDesc()
+    a = (C.c_float * 3)(*[float(x) for x in vmin])
+    b = (C.c_float * 3)(*[float(x) for x in vmax])
+    call("vg_define_shape", C.byref(g), C.c_size_t(n), a, b, int(pow2))
+    return g
+
+
+def nested_layers(g: GridDesc):
+    layers = (GridDesc * 16)()
+    dec = (C.c_uint32 * 48)()
+    depth = C.c_uint32()
+    call("vg_nested_layers", C.byref(g), layers, dec, 16, C.byref(depth))
+    return [layers[i] for i in range(depth.value)], [tuple(dec[3 * i:3 * i + 3]) for i in range(depth.value)]
+
+
+def finalize_bounding_box(mn, mx, vorton_radius: float, n: int) -> GridDesc:
+    a = (C.c_float * 3)(*[float(x) for x in mn])
+    b = (C.c_float * 3)(*[float(x) for x in mx])
+    g = GridDesc()
+    call("vg_finalize_bounding_box", a, b, C.c_float(vorton_radius), C.c_size_t(n), C.byref(g))
+    return g
+
+
+def auto_mollify(g: GridDesc, vorton_radius: float):
+    s, c = C.c_float(), C.c_float()
+    call("vg_auto_mollify", C.byref(g), C.c_float(vorton_radius), C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def find_bounding_box(pos: DeviceArray, n: int, stride: int = 3, mn=None, mx=None, stream=None):
+    big = np.finfo(np.float32).max
+    a = (C.c_float * 3)(*(mn if mn is not None else [big] * 3))
+    b = (C.c_float * 3)(*(mx if mx is not None else [-big] * 3))
+    call("vg_find_bounding_box", pos.ptr, C.c_size_t(stride), C.c_size_t(n), a, b,
+         stream.handle if stream else None)
+    return np.array(a[:], np.float32), np.array(b[:], np.float32)
+
+
+# ---------------------------------------------------------------- operators
+def _s(stream):
+    return stream.handle if stream is not None else None
+
+
+def direct_velocity(v: Vortons, q: DeviceArray, nq: int, out: DeviceArray, spread=1.0, circ=1.0,
+                    policy=RSQRT_REFERENCE, q_stride=3, stream=None):
+    call("vg_direct_velocity", C.byref(v), q.ptr, C.c_size_t(q_stride), C.c_size_t(nq),
+         C.c_float(spread), C.c_float(circ), policy, out.ptr, _s(stream))
+
+
+def direct_vecpot(v: Vortons, q: DeviceArray, nq: int, out: DeviceArray, spread=1.0, circ=1.0,
+                  policy=RSQRT_REFERENCE, q_stride=3, stream=None):
+    call("vg_direct_vecpot", C.byref(v), q.ptr, C.c_size_t(q_stride), C.c_size_t(nq),
+         C.c_float(spread), C.c_float(circ), policy, out.ptr, _s(stream))
+
+
+def direct_velocity_grid(v: Vortons, g: GridDesc, out: DeviceArray, spread=1.0, circ=1.0,
+                         policy=RSQRT_REFERENCE, stream=None):
+    call("vg_direct_velocity_grid", C.byref(v), C.byref(g), C.c_float(spread), C.c_float(circ),
+         policy, out.ptr, _s(stream))
+
+
+def direct_velocity_f64(v: Vortons, q: DeviceArray, nq: int, out: DeviceArray, spread=1.0,
+                        circ=1.0, q_stride=3, stream=None):
+    call("vg_direct_velocity_f64", C.byref(v), q.ptr, C.c_size_t(q_stride), C.c_size_t(nq),
+         C.c_float(spread), C.c_float(circ), out.ptr, _s(stream))
+
+
+class Tree:
+    """vg_tree: the NestedGrid influence tree on the device."""
+
+    def __init__(self):
+        self.h = C.c_void_p()
+        call("vg_tree_create", C.byref(self.h))
+
+    def build(self, g: GridDesc, v: Vortons, stream=None):
+        call("vg_tree_build", self.h, C.byref(g), C.byref(v), _s(stream))
+
+    @property
+    def depth(self) -> int:
+        d = C.c_uint32()
+        call("vg_tree_depth", self.h, C.byref(d))
+        return d.value
+
+    def layer(self, l: int):
+        g = GridDesc()
+        call("vg_tree_layer", self.h, l, C.byref(g), None, None, None)
+        cap = g.capacity
+        pos = np.zeros(cap * 3, np.float32)
+        om = np.zeros(cap * 3, np.float32)
+        size = np.zeros(cap, np.float32)
+        call("vg_tree_layer", self.h, l, C.byref(g), pos.ctypes.data_as(C.c_void_p),
+             om.ctypes.data_as(C.c_void_p), size.ctypes.data_as(C.c_void_p))
+        return g, pos.reshape(-1, 3), om.reshape(-1, 3), size
+
+    def velocity(self, q: DeviceArray, nq: int, out: DeviceArray, spread=1.0, circ=1.0,
+                 policy=RSQRT_REFERENCE, q_stride=3, stream=None):
+        call("vg_tree_velocity", self.h, q.ptr, C.c_size_t(q_stride), C.c_size_t(nq),
+             C.c_float(spread), C.c_float(circ), policy, out.ptr, _s(stream))
+
+    def interaction_count(self, g: GridDesc, stream=None) -> int:
+        c = C.c_uint64()
+        call("vg_tree_interaction_count", self.h, C.byref(g), C.byref(c), _s(stream))
+        return c.value
+
+    def vecpot(self, q: DeviceArray, nq: int, out: DeviceArray, spread=1.0, circ=1.0,
+               policy=RSQRT_REFERENCE, q_stride=3, stream=None):
+        call("vg_tree_vecpot", self.h, q.ptr, C.c_size_t(q_stride), C.c_size_t(nq),
+             C.c_float(spread), C.c_float(circ), policy, out.ptr, _s(stream))
+
+    def velocity_grid(self, g: GridDesc, out: DeviceArray, spread=1.0, circ=1.0,
+                      policy=RSQRT_REFERENCE, stream=None):
+        call("vg_tree_velocity_grid", self.h, C.byref(g), C.c_float(spread), C.c_float(circ),
+             policy, out.ptr, _s(stream))
+
+    def vecpot_boundary(self, g: GridDesc, out: DeviceArray, spread=1.0, circ=1.0,
+                        policy=RSQRT_REFERENCE, copy_odd=True, stream=None):
+        call("vg_tree_vecpot_boundary", self.h, C.byref(g), C.c_float(spread), C.c_float(circ),
+             policy, int(copy_odd), out.ptr, _s(stream))
+
+    def splat_vorticity(self, vorton_radius: float, scale: float, out: DeviceArray, stream=None):
+        call("vg_tree_splat_vorticity", self.h, C.c_float(vorton_radius), C.c_float(scale),
+             out.ptr, _s(stream))
+
+    def populate_velocity_grid(self, vel: DeviceArray, out: DeviceArray, vel_stride=3,
+                               stream=None):
+        call("vg_tree_populate_velocity_grid", self.h, vel.ptr, C.c_size_t(vel_stride), out.ptr,
+             _s(stream))
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().vg_tree_destroy(self.h)
+        except Exception:
+            pass
+
+
+def downsample_inject(lo: GridDesc, lo_field: DeviceArray, hi: GridDesc, hi_field: DeviceArray,
+                      stream=None):
+    call("vg_downsample_inject", C.byref(lo), lo_field.ptr, C.byref(hi), hi_field.ptr, _s(stream))
+
+
+def upsample_interior(hi: GridDesc, hi_field: DeviceArray, lo: GridDesc, lo_field: DeviceArray,
+                      stream=None):
+    call("vg_upsample_interior", C.byref(hi), hi_field.ptr, C.byref(lo), lo_field.ptr, _s(stream))
+
+
+def sor_solve(g: GridDesc, soln: DeviceArray, lap: DeviceArray, steps: int, stream=None):
+    call("vg_sor_solve", C.byref(g), soln.ptr, lap.ptr, C.c_uint32(steps), _s(stream))
+
+
+def curl(g: GridDesc, a: DeviceArray, out: DeviceArray, stream=None):
+    call("vg_curl", C.byref(g), a.ptr, out.ptr, _s(stream))
+
+
+def residual_norm(g: GridDesc, u: DeviceArray, f: DeviceArray, stream=None) -> float:
+    r = C.c_double()
+    call("vg_residual_norm", C.byref(g), u.ptr, f.ptr, C.byref(r), _s(stream))
+    return r.value
+
+
+class Multigrid:
+    def __init__(self):
+        self.h = C.c_void_p()
+        call("vg_mg_create", C.byref(self.h))
+
+    def cascade(self, g: GridDesc, a0: DeviceArray, negw: DeviceArray, steps=16, stream=None):
+        call("vg_mg_cascade", self.h, C.byref(g), a0.ptr, negw.ptr, C.c_uint32(steps), _s(stream))
+
+    def vcycle_solve(self, g: GridDesc, a0: DeviceArray, negw: DeviceArray, tol=1e-6,
+                     max_cycles=30, stream=None):
+        rel = C.c_double()
+        cyc = C.c_uint32()
+        call("vg_mg_vcycle_solve", self.h, C.byref(g), a0.ptr, negw.ptr, C.c_double(tol),
+             C.c_uint32(max_cycles), C.byref(rel), C.byref(cyc), _s(stream))
+        return rel.value, cyc.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().vg_mg_destroy(self.h)
+        except Exception:
+            pass
+
+
+def assign_velocity_from_field(g: GridDesc, field: DeviceArray, pos, vel, n: int, gain=1.0,
+                               stride=3, stream=None):
+    call("vg_assign_velocity_from_field", C.byref(g), field.ptr, pos.ptr if hasattr(pos, "ptr") else pos,
+         vel.ptr if hasattr(vel, "ptr") else vel, C.c_size_t(stride), C.c_size_t(n),
+         C.c_float(gain), _s(stream))
+
+
+def evolve(pos, vel, n: int, dt: float, stride=3, stream=None):
+    call("vg_evolve", pos.ptr if hasattr(pos, "ptr") else pos, vel.ptr if hasattr(vel, "ptr") else vel,
+         C.c_size_t(stride), C.c_size_t(n), C.c_float(dt), _s(stream))
+
+
+def advect_tracers(g: GridDesc, field, pos, vel, n: int, dt: float, gain=1.0, stride=3,
+                   stream=None):
+    fp = field.ptr if hasattr(field, "ptr") else field
+    vp = None if vel is None else (vel.ptr if hasattr(vel, "ptr") else vel)
+    call("vg_advect_tracers", C.byref(g), fp, pos.ptr if hasattr(pos, "ptr") else pos, vp,
+         C.c_size_t(stride), C.c_size_t(n), C.c_float(dt), C.c_float(gain), _s(stream))
+
+
+# ---------------------------------------------------------------- simulation
+def sim_config(**kw) -> SimConfig:
+    c = SimConfig()
+    call("vg_sim_config_default", C.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+class Sim:
+    """vg_sim == VortonSim (vortonSim.h:137-623) with device-resident state."""
+
+    def __init__(self, cfg: SimConfig | None = None, stream=None):
+        self.h = C.c_void_p()
+        self.stream = stream
+        call("vg_sim_create", C.byref(self.h), C.byref(cfg) if cfg is not None else None,
+             _s(stream))
+        self._v = None
+
+    def set_vortons(self, v: Vortons):
+        self._v = v
+        call("vg_sim_set_vortons", self.h, C.byref(v))
+
+    def find_bounding_box(self):
+        call("vg_sim_find_bounding_box", self.h)
+
+    def update_bounding_box(self, mn, mx, final=True):
+        a = (C.c_float * 3)(*[float(x) for x in mn])
+        b = (C.c_float * 3)(*[float(x) for x in mx])
+        call("vg_sim_update_bounding_box", self.h, a, b, int(final))
+
+    def update(self, dt=1.0 / 30.0, frame=0, vel_out=None, vel_stride=3):
+        call("vg_sim_update", self.h, C.c_float(dt), C.c_uint32(frame),
+             vel_out.ptr if vel_out is not None else None, C.c_size_t(vel_stride))
+
+    def velocity_grid(self):
+        g = GridDesc()
+        p = C.c_void_p()
+        call("vg_sim_velocity_grid", self.h, C.byref(g), C.byref(p))
+        return g, p
+
+    def vector_potential(self):
+        g = GridDesc()
+        p = C.c_void_p()
+        call("vg_sim_vector_potential", self.h, C.byref(g), C.byref(p))
+        return g, p
+
+    def negative_vorticity(self):
+        g = GridDesc()
+        p = C.c_void_p()
+        call("vg_sim_negative_vorticity", self.h, C.byref(g), C.byref(p))
+        return g, p
+
+    def grid_template(self) -> GridDesc:
+        g = GridDesc()
+        call("vg_sim_grid_template", self.h, C.byref(g))
+        return g
+
+    def bounds(self):
+        a = (C.c_float * 3)()
+        b = (C.c_float * 3)()
+        call("vg_sim_bounds", self.h, a, b)
+        return np.array(a[:], np.float32), np.array(b[:], np.float32)
+
+    def spreading(self):
+        s, c = C.c_float(), C.c_float()
+        call("vg_sim_spreading", self.h, C.byref(s), C.byref(c))
+        return s.value, c.value
+
+    def poisson_residual(self) -> float:
+        r = C.c_double()
+        call("vg_sim_poisson_residual", self.h, C.byref(r))
+        return r.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().vg_sim_destroy(self.h)
+        except Exception:
+            pass
+
+
+def fetch(ptr: C.c_void_p, count: int, dtype=np.float32) -> np.ndarray:
+    out = np.empty(count, dtype)
+    if count:
+        call("vg_memcpy_dtoh", out.ctypes.data_as(C.c_void_p), ptr, C.c_size_t(out.nbytes))
+    return out
+
+
+# ---------------------------------------------------------------- profiling
+def profile_enable(on=True):
+    call("vg_profile_enable", int(on))
+
+
+def profile_reset():
+    call("vg_profile_reset")
+
+
+def profile_get(kernel: str):
+    ms = C.c_double()
+    cnt = C.c_uint64()
+    call("vg_profile_get", kernel.encode(), C.byref(ms), C.byref(cnt))
+    return ms.value, cnt.value
+
+
+def profile_dump() -> dict:
+    buf = C.create_string_buffer(1 << 16)
+    call("vg_profile_dump", buf, C.c_size_t(len(buf)))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, cnt, ms = line.split()
+        out[name] = (int(cnt), float(ms))
+    return out
