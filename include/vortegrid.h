@@ -162,6 +162,9 @@ vg_status vg_tree_depth(const vg_tree *t, uint32_t *depth);
  * cluster position[3*cap], angular velocity[3*cap], size[cap].  Blocks. */
 vg_status vg_tree_layer(const vg_tree *t, uint32_t l, vg_grid_desc *g,
                         float *pos, float *angvel, float *size);
+/* First moments of layer l's clusters about their centroids (no reference
+ * counterpart): dip[9*c + 3*b + e] = sum Omega_b (y - c)_e.  Blocks. */
+vg_status vg_tree_layer_moments(const vg_tree *t, uint32_t l, float *dip);
 /* ComputeVelocity_Tree from the root (vortonSim.cpp:1787-1905) at nq
  * strided query points. */
 vg_status vg_tree_velocity(const vg_tree *t, const float *q, size_t q_stride,
@@ -177,6 +180,21 @@ vg_status vg_tree_velocity_grid(const vg_tree *t, const vg_grid_desc *g,
  * FLOP-rate accounting.  Blocks until the count is on the host. */
 vg_status vg_tree_interaction_count(const vg_tree *t, const vg_grid_desc *g,
                                     uint64_t *count, vg_stream s);
+/* Accurate near-field treecode at every point of g (g must be the tree's
+ * grid template).  No reference counterpart: replaces the TREE technique's
+ * grid evaluation (vortonSim.cpp:2223-2293) when accuracy matters.  Grid
+ * points are processed in 4x4x4 bricks; on every layer the cells within
+ * near_k cells of the brick are opened, the other children of the parent's
+ * near box are applied as clusters (order 0: monopole at the |w|-weighted
+ * centroid, order 1: plus the first-moment correction), and the leaf near
+ * box is summed over its vortons. */
+vg_status vg_tree_velocity_grid_mac(const vg_tree *t, const vg_grid_desc *g, int near_k,
+                                    int order, float spread, float circ, int rsqrt_policy,
+                                    float *out, vg_stream s);
+/* Work of vg_tree_velocity_grid_mac summed over all grid points: cluster
+ * interactions and direct vorton pairs.  Blocks. */
+vg_status vg_tree_mac_work(const vg_tree *t, const vg_grid_desc *g, int near_k,
+                           uint64_t *clusters, uint64_t *pairs, vg_stream s);
 /* ComputeVectorPotential_Tree (vortonSim.cpp:1650-1768) at query points. */
 vg_status vg_tree_vecpot(const vg_tree *t, const float *q, size_t q_stride,
                          size_t nq, float spread, float circ, int rsqrt_policy,

@@ -62,6 +62,11 @@ class Tree(C.Structure):
         ("pos", C.POINTER(C.c_float) * MAX_LAYERS),
         ("om", C.POINTER(C.c_float) * MAX_LAYERS),
         ("size", C.POINTER(C.c_float) * MAX_LAYERS),
+        ("dip", C.POINTER(C.c_float) * MAX_LAYERS),
+        ("cell_start", C.POINTER(C.c_uint32)),
+        ("cell_count", C.POINTER(C.c_uint32)),
+        ("sorted_idx", C.POINTER(C.c_uint32)),
+        ("n", C.c_size_t),
     ]
 
 
@@ -119,6 +124,9 @@ def lib():
         L.vgo_velocity_grid.argtypes = [C.POINTER(Tree), C.POINTER(Grid), f32p, f32p, f32p,
                                         C.c_size_t, C.c_int, C.c_float, C.c_float, C.c_int,
                                         f32p, C.c_int]
+        L.vgo_mac_velocity_grid.argtypes = [C.POINTER(Tree), C.POINTER(Grid), f32p, f32p, f32p,
+                                            C.c_int, C.c_int, C.c_float, C.c_float, C.c_int,
+                                            f32p, C.c_int]
         L.vgo_vecpot_boundary.argtypes = [C.POINTER(Tree), C.POINTER(Grid), f32p, f32p, f32p,
                                           C.c_size_t, C.c_float, C.c_float, C.c_int, C.c_int,
                                           f32p, C.c_int]
@@ -298,6 +306,10 @@ class InfluenceTree:
     def decimations(self, l):
         return tuple(int(v) for v in self._t.ng.dec[l])
 
+    def dipoles(self, l):
+        cap = self._t.ng.layer[l].capacity
+        return np.ctypeslib.as_array(self._t.dip[l], shape=(cap * 9,)).reshape(-1, 3, 3).copy()
+
     def velocity(self, q, spread=1.0, circ=1.0, policy=0):
         out = np.zeros(3, np.float32)
         lib().vgo_tree_velocity(C.byref(self._t), _f32(q), spread, circ, policy, out)
@@ -320,6 +332,18 @@ def velocity_grid(tree, g: Grid, pos, om, size, technique=1, spread=1.0, circ=1.
     lib().vgo_velocity_grid(tree.ptr() if tree is not None else None, C.byref(g), pos,
                             _f32(om).reshape(-1), _size_arr(size, n), n, technique, spread,
                             circ, policy, out, threads)
+    return out.reshape(-1, 3)
+
+
+def mac_velocity_grid(tree, g: Grid, pos, om, size, near_k=2, order=1, spread=1.0, circ=1.0,
+                      policy=1, threads=1):
+    """Near-field treecode at every grid point (bricks of 4^3 points)."""
+    pos = _f32(pos).reshape(-1)
+    n = pos.size // 3
+    out = np.zeros(g.capacity * 3, np.float32)
+    lib().vgo_mac_velocity_grid(tree.ptr(), C.byref(g), pos, _f32(om).reshape(-1),
+                                _size_arr(size, n), near_k, order, spread, circ, policy, out,
+                                threads)
     return out.reshape(-1, 3)
 
 

@@ -334,7 +334,8 @@ int vgo_tree_build(vgo_tree *t, const vgo_grid *grid_template,
         t->pos[l] = (float *)calloc(cap * 3, sizeof(float));
         t->om[l] = (float *)calloc(cap * 3, sizeof(float));
         t->size[l] = (float *)calloc(cap, sizeof(float));
-        if (!t->pos[l] || !t->om[l] || !t->size[l]) return -1;
+        t->dip[l] = (float *)calloc(cap * 9, sizeof(float));
+        if (!t->pos[l] || !t->om[l] || !t->size[l] || !t->dip[l]) return -1;
     }
     const vgo_grid *g0 = &t->ng.layer[0];
     const size_t cap0 = vgo_grid_capacity(g0);
@@ -358,6 +359,41 @@ int vgo_tree_build(vgo_tree *t, const vgo_grid *grid_template,
         }
     }
     free(aux);
+    /* Leaf cell lists (stable counting sort by cell) and leaf first moments
+     * about the cell centroid, summed in vorton index order. */
+    t->n = n;
+    t->cell_start = (uint32_t *)calloc(cap0, sizeof(uint32_t));
+    t->cell_count = (uint32_t *)calloc(cap0, sizeof(uint32_t));
+    t->sorted_idx = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+    uint32_t *keys = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+    if (!t->cell_start || !t->cell_count || !t->sorted_idx || !keys) return -1;
+    for (size_t v = 0; v < n; ++v) {
+        uint32_t idx[3];
+        indices_of_position(g0, pos + 3 * v, idx);
+        keys[v] = (uint32_t)(idx[0] + (size_t)g0->np[0] * (idx[1] + (size_t)g0->np[1] * idx[2]));
+        t->cell_count[keys[v]]++;
+    }
+    {
+        uint32_t run = 0;
+        for (size_t c = 0; c < cap0; ++c) { t->cell_start[c] = run; run += t->cell_count[c]; }
+        uint32_t *fill = (uint32_t *)calloc(cap0, sizeof(uint32_t));
+        for (size_t v = 0; v < n; ++v) {
+            const uint32_t k = keys[v];
+            t->sorted_idx[t->cell_start[k] + fill[k]++] = (uint32_t)v;
+        }
+        free(fill);
+    }
+    free(keys);
+    for (size_t c = 0; c < cap0; ++c) {
+        const float *cp = t->pos[0] + 3 * c;
+        float *d = t->dip[0] + 9 * c;
+        for (uint32_t k = 0; k < t->cell_count[c]; ++k) {
+            const size_t v = t->sorted_idx[t->cell_start[c] + k];
+            const float e[3] = {pos[3 * v] - cp[0], pos[3 * v + 1] - cp[1], pos[3 * v + 2] - cp[2]};
+            for (int b = 0; b < 3; ++b)
+                for (int a = 0; a < 3; ++a) d[3 * b + a] += om[3 * v + b] * e[a];
+        }
+    }
     for (uint32_t pl = 1; pl < t->ng.depth; ++pl) {
         const vgo_grid *gp = &t->ng.layer[pl], *gc = &t->ng.layer[pl - 1];
         const uint32_t *dims = t->ng.dec[pl];
@@ -386,6 +422,21 @@ int vgo_tree_build(vgo_tree *t, const vgo_grid *grid_template,
                             }
                     const float inv = 1.0f / asum;
                     for (int a = 0; a < 3; ++a) pp[a] *= inv;
+                    /* first moment about the new centroid (parallel-axis shift) */
+                    float *pd = t->dip[pl] + 9 * po;
+                    for (uint32_t cz = 0; cz < dims[2]; ++cz)
+                        for (uint32_t cy = 0; cy < dims[1]; ++cy)
+                            for (uint32_t cx = 0; cx < dims[0]; ++cx) {
+                                const size_t co = (ix * dims[0] + cx) +
+                                    (size_t)gc->np[0] * ((iy * dims[1] + cy) +
+                                    (size_t)gc->np[1] * (iz * dims[2] + cz));
+                                const float *cw = t->om[pl - 1] + 3 * co;
+                                const float *cpos = t->pos[pl - 1] + 3 * co;
+                                const float *cd = t->dip[pl - 1] + 9 * co;
+                                for (int b = 0; b < 3; ++b)
+                                    for (int a = 0; a < 3; ++a)
+                                        pd[3 * b + a] += cd[3 * b + a] + cw[b] * (cpos[a] - pp[a]);
+                            }
                 }
     }
     return 0;
@@ -393,9 +444,172 @@ int vgo_tree_build(vgo_tree *t, const vgo_grid *grid_template,
 
 void vgo_tree_free(vgo_tree *t) {
     for (uint32_t l = 0; l < VGO_MAX_LAYERS; ++l) {
-        free(t->pos[l]); free(t->om[l]); free(t->size[l]);
-        t->pos[l] = t->om[l] = t->size[l] = NULL;
+        free(t->pos[l]); free(t->om[l]); free(t->size[l]); free(t->dip[l]);
+        t->pos[l] = t->om[l] = t->size[l] = t->dip[l] = NULL;
     }
+    free(t->cell_start); free(t->cell_count); free(t->sorted_idx);
+    t->cell_start = t->cell_count = t->sorted_idx = NULL;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Near-field treecode over 4x4x4 grid-point bricks (accurate mode). */
+static void kahan3(float sum[3], float comp[3], const float part[3]) {
+    for (int a = 0; a < 3; ++a) {
+        const float y = part[a] - comp[a];
+        const float t = sum[a] + y;
+        comp[a] = (t - sum[a]) - y;
+        sum[a] = t;
+    }
+}
+
+static void cluster_contrib(float acc[3], const float q[3], const float *cp, const float *w,
+                            float size, const float *D, int order, float spread, int policy) {
+    const float radius = size * 0.5f * spread;
+    const float r2 = radius * radius;
+    const float s = (2.0f / 3.0f) * r2 * radius;
+    const float d[3] = {q[0] - cp[0], q[1] - cp[1], q[2] - cp[2]};
+    const float d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    float law;
+    if (d2 < r2) {
+        law = 1.0f / (r2 * radius);
+    } else if (policy) {
+        const float rs = 1.0f / sqrtf(d2);
+        law = rs * rs * rs;
+    } else {
+        law = vgo_finvsqrtf(d2) / d2;
+    }
+    const float W[3] = {w[0] * s, w[1] * s, w[2] * s};
+    acc[0] += (W[1] * d[2] - W[2] * d[1]) * law;
+    acc[1] += (W[2] * d[0] - W[0] * d[2]) * law;
+    acc[2] += (W[0] * d[1] - W[1] * d[0]) * law;
+    if (order >= 1 && d2 >= r2) {
+        /* u -= s/|d|^3 [ a - 3/|d|^2 (D d) x d ],  a = antisymmetric part of D */
+        const float a[3] = {D[5] - D[7], D[6] - D[2], D[1] - D[3]};
+        const float tt[3] = {D[0] * d[0] + D[1] * d[1] + D[2] * d[2],
+                             D[3] * d[0] + D[4] * d[1] + D[5] * d[2],
+                             D[6] * d[0] + D[7] * d[1] + D[8] * d[2]};
+        const float c3 = 3.0f / d2;
+        const float x[3] = {tt[1] * d[2] - tt[2] * d[1], tt[2] * d[0] - tt[0] * d[2],
+                            tt[0] * d[1] - tt[1] * d[0]};
+        const float sl = s * law;
+        for (int k = 0; k < 3; ++k) acc[k] -= (a[k] - c3 * x[k]) * sl;
+    }
+}
+
+typedef struct {
+    const vgo_tree *t;
+    const vgo_grid *g;
+    const float *pos, *om, *size;
+    int k, order, policy;
+    float spread, circ;
+    float *out;
+    uint32_t nb[3];
+} mac_ctx;
+
+static void mac_brick(mac_ctx *c, uint32_t bx, uint32_t by, uint32_t bz) {
+    const vgo_tree *t = c->t;
+    const vgo_grid *g = c->g;
+    const uint32_t depth = t->ng.depth;
+    const uint32_t b0[3] = {4 * bx, 4 * by, 4 * bz};
+    uint32_t b1[3];
+    const uint32_t bb[3] = {bx, by, bz};
+    for (int a = 0; a < 3; ++a) {
+        b1[a] = 4 * bb[a] + 3;
+        if (b1[a] > g->np[a] - 1) b1[a] = g->np[a] - 1;
+    }
+    /* leaf cells of the brick's first and last points; cumulative scales */
+    int cl0[3], cl1[3];
+    for (int a = 0; a < 3; ++a) {
+        const int nc = (int)g->np[a] - 1;
+        cl0[a] = (int)b0[a] < nc - 1 ? (int)b0[a] : nc - 1;
+        cl1[a] = (int)b1[a] < nc - 1 ? (int)b1[a] : nc - 1;
+    }
+    uint32_t scale[VGO_MAX_LAYERS][3];
+    for (int a = 0; a < 3; ++a) scale[0][a] = 1;
+    for (uint32_t l = 1; l < depth; ++l)
+        for (int a = 0; a < 3; ++a) scale[l][a] = scale[l - 1][a] * t->ng.dec[l][a];
+    for (uint32_t pz = b0[2]; pz <= b1[2]; ++pz)
+        for (uint32_t py = b0[1]; py <= b1[1]; ++py)
+            for (uint32_t px = b0[0]; px <= b1[0]; ++px) {
+                float q[3];
+                vgo_grid_query_point(g, px, py, pz, q);
+                float sum[3] = {0, 0, 0}, comp[3] = {0, 0, 0};
+                int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+                if (depth == 1) {
+                    float part[3] = {0, 0, 0};
+                    for (size_t v = 0; v < t->n; ++v)
+                        vgo_accumulate_velocity(part, q, c->pos + 3 * v, c->om + 3 * v, c->size[v],
+                                                c->spread, 1.0f, c->policy);
+                    kahan3(sum, comp, part);
+                }
+                for (int l = (int)depth - 2; l >= 0; --l) {
+                    const vgo_grid *gl = &t->ng.layer[l];
+                    const uint32_t *dec = t->ng.dec[l + 1];
+                    int c0[3], c1[3], n0[3], n1[3];
+                    for (int a = 0; a < 3; ++a) {
+                        c0[a] = lo[a] * (int)dec[a];
+                        c1[a] = hi[a] * (int)dec[a] + (int)dec[a] - 1;
+                        const int ncl = (int)gl->np[a] - 1;
+                        int blo = cl0[a] / (int)scale[l][a] - c->k;
+                        int bhi = cl1[a] / (int)scale[l][a] + c->k;
+                        if (blo < 0) blo = 0;
+                        if (bhi > ncl - 1) bhi = ncl - 1;
+                        n0[a] = blo > c0[a] ? blo : c0[a];
+                        n1[a] = bhi < c1[a] ? bhi : c1[a];
+                    }
+                    float part[3] = {0, 0, 0};
+                    for (int cz = c0[2]; cz <= c1[2]; ++cz)
+                        for (int cy = c0[1]; cy <= c1[1]; ++cy)
+                            for (int cx = c0[0]; cx <= c1[0]; ++cx) {
+                                if (cx >= n0[0] && cx <= n1[0] && cy >= n0[1] && cy <= n1[1] &&
+                                    cz >= n0[2] && cz <= n1[2])
+                                    continue;
+                                const size_t off = cx + (size_t)gl->np[0] * (cy + (size_t)gl->np[1] * cz);
+                                const float sz = t->size[l][off];
+                                if (sz == 0.0f) continue;
+                                cluster_contrib(part, q, t->pos[l] + 3 * off, t->om[l] + 3 * off, sz,
+                                                t->dip[l] + 9 * off, c->order, c->spread, c->policy);
+                            }
+                    kahan3(sum, comp, part);
+                    for (int a = 0; a < 3; ++a) { lo[a] = n0[a]; hi[a] = n1[a]; }
+                    if (l == 0) {
+                        float pp[3] = {0, 0, 0};
+                        for (int cz = n0[2]; cz <= n1[2]; ++cz)
+                            for (int cy = n0[1]; cy <= n1[1]; ++cy)
+                                for (int cx = n0[0]; cx <= n1[0]; ++cx) {
+                                    const size_t off = cx + (size_t)gl->np[0] * (cy + (size_t)gl->np[1] * cz);
+                                    for (uint32_t j = 0; j < t->cell_count[off]; ++j) {
+                                        const size_t v = t->sorted_idx[t->cell_start[off] + j];
+                                        vgo_accumulate_velocity(pp, q, c->pos + 3 * v, c->om + 3 * v,
+                                                                c->size[v], c->spread, 1.0f,
+                                                                c->policy);
+                                    }
+                                }
+                        kahan3(sum, comp, pp);
+                    }
+                }
+                const size_t o = px + (size_t)g->np[0] * (py + (size_t)g->np[1] * pz);
+                for (int a = 0; a < 3; ++a) c->out[3 * o + a] = sum[a] * c->circ;
+            }
+}
+
+static void mac_range(void *vc, size_t b, size_t e) {
+    mac_ctx *c = (mac_ctx *)vc;
+    for (size_t bz = b; bz < e; ++bz)
+        for (uint32_t by = 0; by < c->nb[1]; ++by)
+            for (uint32_t bx = 0; bx < c->nb[0]; ++bx) mac_brick(c, bx, by, (uint32_t)bz);
+}
+
+void vgo_mac_velocity_grid(const vgo_tree *t, const vgo_grid *g, const float *pos,
+                           const float *om, const float *size, int near_k, int order,
+                           float spread, float circ, int rsqrt_policy, float *out,
+                           int nthreads) {
+    mac_ctx c;
+    c.t = t; c.g = g; c.pos = pos; c.om = om; c.size = size;
+    c.k = near_k; c.order = order; c.policy = rsqrt_policy;
+    c.spread = spread; c.circ = circ; c.out = out;
+    for (int a = 0; a < 3; ++a) c.nb[a] = (g->np[a] + 3) / 4;
+    parallel_range(c.nb[2], nthreads, mac_range, &c);
 }
 
 /* vortonSim.cpp:1650-1768 / 1787-1905: recursive traversal.  A cluster of

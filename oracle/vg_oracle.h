@@ -92,6 +92,14 @@ typedef struct vgo_tree {
     float *pos[VGO_MAX_LAYERS];
     float *om[VGO_MAX_LAYERS];
     float *size[VGO_MAX_LAYERS];
+    /* First moment of each cluster about its centroid (not in the
+     * reference): dip[9*c + 3*b + e] = sum_i Omega_i[b] * (y_i - c)[e]. */
+    float *dip[VGO_MAX_LAYERS];
+    /* Leaf cell lists: vortons of leaf cell c (point offset) are
+     * sorted_idx[cell_start[c] .. cell_start[c] + cell_count[c]) in index
+     * order. */
+    uint32_t *cell_start, *cell_count, *sorted_idx;
+    size_t n;
 } vgo_tree;
 int  vgo_tree_build(vgo_tree *t, const vgo_grid *grid_template,
                     const float *pos, const float *om, const float *size,
@@ -104,6 +112,19 @@ void vgo_tree_velocity(const vgo_tree *t, const float q[3], float spread,
                        float circ, int rsqrt_policy, float out[3]);
 void vgo_tree_vecpot(const vgo_tree *t, const float q[3], float spread,
                      float circ, int rsqrt_policy, float out[3]);
+
+/* Near-field treecode at every grid point (this build's accurate mode; no
+ * reference counterpart).  Grid points are taken in 4x4x4 bricks; for each
+ * brick and layer the near box is the brick's cells +-near_k, clipped to
+ * the children of the parent layer's near box.  Children outside the near
+ * box are applied as clusters (monopole, plus the first-moment correction
+ * when order == 1); leaf cells inside the leaf near box are summed over
+ * their vortons.  Per-lane sums follow the GPU order: layers from the top,
+ * cells z,y,x, one Kahan-compensated partial per layer. */
+void vgo_mac_velocity_grid(const vgo_tree *t, const vgo_grid *g, const float *pos,
+                           const float *om, const float *size, int near_k,
+                           int order, float spread, float circ, int rsqrt_policy,
+                           float *out, int nthreads);
 
 /* Auto-mollification (vortonSim.cpp:2458-2477). */
 void vgo_auto_mollify(const vgo_grid *vel_grid, float vorton_radius,

@@ -401,3 +401,58 @@ def test_deterministic_reruns(vg, rings2):
         vg.synchronize()
         res.append((out.numpy(), d.numpy()))
     assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+
+
+# ------------------------------------------------------------ near-field treecode (accurate mode)
+def test_tree_moments_bit_exact(vg, oracle, rings2):
+    pos, om, radius, og = rings2
+    case("tree first moments vs oracle")
+    v, keep = vortons(vg, pos, om, radius)
+    t = vg.Tree()
+    t.build(vg.GridDesc.from_fields(og), v)
+    vg.synchronize()
+    ot = oracle.InfluenceTree(og, pos, om, 2 * radius)
+    for l in range(ot.depth):
+        assert np.array_equal(t.moments(l), ot.dipoles(l)), l
+
+
+@pytest.mark.parametrize("k,order,policy", [(1, 0, 1), (1, 1, 1), (2, 1, 1), (2, 1, 0), (0, 1, 1)])
+def test_mac_velocity_grid(vg, oracle, rings2, k, order, policy):
+    pos, om, radius, og = rings2
+    case(f"mac velocity grid k={k} order={order} policy={policy}")
+    v, keep = vortons(vg, pos, om, radius)
+    g = vg.GridDesc.from_fields(og)
+    t = vg.Tree()
+    t.build(g, v)
+    spread, circ = vg.auto_mollify(g, radius)
+    out = vg.DeviceArray.zeros((g.capacity, 3))
+    t.velocity_grid_mac(g, out, k, order, spread, circ, policy)
+    vg.synchronize()
+    ot = oracle.InfluenceTree(og, pos, om, 2 * radius)
+    want = oracle.mac_velocity_grid(ot, og, pos, om, 2 * radius, k, order, spread, circ, policy,
+                                    threads=8)
+    assert rel_l2(out.numpy(), want) < 1e-5
+
+
+def test_mac_work_and_accuracy(vg, oracle):
+    pos, om, radius = scenarios.random_rings(32768, seed=33, rings=16)
+    case("mac work counts + accuracy vs fp64 N=32768")
+    og = oracle.grid_for(pos, radius)
+    v, keep = vortons(vg, pos, om, radius)
+    g = vg.GridDesc.from_fields(og)
+    t = vg.Tree()
+    t.build(g, v)
+    spread, circ = vg.auto_mollify(g, radius)
+    c1, p1 = t.mac_work(g, 1)
+    c2, p2 = t.mac_work(g, 2)
+    assert c2 > c1 > 0 and p2 > p1 > 0
+    out = vg.DeviceArray.zeros((g.capacity, 3))
+    t.velocity_grid_mac(g, out, 2, 1, spread, circ, 1)
+    q = oracle.query_points(og)
+    sel = np.random.default_rng(3).choice(len(q), 2000, replace=False)
+    dq = upload(vg, q[sel])
+    ref = vg.DeviceArray.zeros((len(sel), 3), np.float64)
+    vg.direct_velocity_f64(v, dq, len(sel), ref, spread, circ)
+    vg.synchronize()
+    err = rel_l2(out.numpy()[sel], ref.numpy())
+    assert err < 5e-3, err

@@ -265,8 +265,12 @@ float ord_dec(uint32_t u) {
     return f;
 }
 
+// Pass 1: each block reduces a grid-stride share of the particles and writes
+// its 6 extrema (ordered-integer encoded) to part[6 * block].  Pass 2 folds
+// the block results.  Min/max are exact, so the result is order-independent.
 __global__ __launch_bounds__(256) void k_bbox(const float *__restrict__ pos, size_t stride,
-                                              size_t n, uint32_t *__restrict__ res) {
+                                              size_t n, uint32_t *__restrict__ part) {
+    __shared__ uint32_t s[4][6];
     float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
@@ -284,19 +288,40 @@ __global__ __launch_bounds__(256) void k_bbox(const float *__restrict__ pos, siz
             mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], off, 64));
         }
     }
+    const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            atomicMin(&res[a], ord_enc(mn[a]));
-            atomicMax(&res[3 + a], ord_enc(mx[a]));
+            s[wave][a] = ord_enc(mn[a]);
+            s[wave][3 + a] = ord_enc(mx[a]);
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        uint32_t v = s[0][threadIdx.x];
+        for (int w = 1; w < 4; ++w)
+            v = threadIdx.x < 3 ? min(v, s[w][threadIdx.x]) : max(v, s[w][threadIdx.x]);
+        part[6 * blockIdx.x + threadIdx.x] = v;
     }
 }
 
-__global__ void k_bbox_init(uint32_t *res) {
-    const int a = threadIdx.x;
-    if (a < 3) res[a] = 0xffffffffu;
-    else if (a < 6) res[a] = 0u;
+__global__ __launch_bounds__(256) void k_bbox_final(const uint32_t *__restrict__ part,
+                                                    uint32_t nblocks, uint32_t *__restrict__ res) {
+    __shared__ uint32_t s[256][6];
+    uint32_t v[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+    for (uint32_t b = threadIdx.x; b < nblocks; b += blockDim.x)
+        for (int k = 0; k < 6; ++k)
+            v[k] = k < 3 ? min(v[k], part[6 * b + k]) : max(v[k], part[6 * b + k]);
+    for (int k = 0; k < 6; ++k) s[threadIdx.x][k] = v[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 6; ++k)
+                s[threadIdx.x][k] = k < 3 ? min(s[threadIdx.x][k], s[threadIdx.x + w][k])
+                                          : max(s[threadIdx.x][k], s[threadIdx.x + w][k]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) res[threadIdx.x] = s[0][threadIdx.x];
 }
 
 }  // namespace
@@ -487,12 +512,15 @@ VG_API vg_status vg_find_bounding_box(const float *pos, size_t stride, size_t n,
     if (n == 0) return VG_OK;
     VG_REQUIRE(pos && stride >= 3, "bad particle array");
     hipStream_t s = (hipStream_t)stream;
+    uint32_t nb = ceil_div(n, 256 * 8);
+    if (nb > 1024) nb = 1024;
+    if (nb < 1) nb = 1;
     uint32_t *res = nullptr;
-    VG_TRY(salloc((void **)&res, 6 * sizeof(uint32_t), s));
-    vg_status st = launch("vg_bbox_init", k_bbox_init, dim3(1), dim3(64), 0, s, res);
-    uint32_t nb = ceil_div(n, 256);
-    if (nb > 2048) nb = 2048;
-    if (st == VG_OK) st = launch("vg_bbox", k_bbox, dim3(nb), dim3(256), 0, s, pos, stride, n, res);
+    VG_TRY(salloc((void **)&res, (6 + 6 * (size_t)nb) * sizeof(uint32_t), s));
+    vg_status st = launch("vg_bbox", k_bbox, dim3(nb), dim3(256), 0, s, pos, stride, n, res + 6);
+    if (st == VG_OK)
+        st = launch("vg_bbox_final", k_bbox_final, dim3(1), dim3(256), 0, s,
+                    (const uint32_t *)(res + 6), nb, res);
     uint32_t h[6] = {0, 0, 0, 0, 0, 0};
     if (st == VG_OK) {
         hipError_t e = hipMemcpyAsync(h, res, sizeof h, hipMemcpyDeviceToHost, s);

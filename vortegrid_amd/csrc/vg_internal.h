@@ -95,6 +95,8 @@ struct TreeView {
     vg_grid_desc layer[kMaxLayers];
     const float4 *P[kMaxLayers];   // cluster (x, y, z, size)
     const float4 *W[kMaxLayers];   // cluster (Omega_x, Omega_y, Omega_z, sum|w|)
+    const float4 *M[kMaxLayers];   // 3 per cluster: rows of the first moment
+                                   // D[b][e] = sum Omega_b (y - c)_e, .w = antisymmetric part
     // Leaf cell lists over the layer-0 points: vortons sorted by cell.
     const uint32_t *cell_start;    // per layer-0 point
     const uint32_t *cell_count;

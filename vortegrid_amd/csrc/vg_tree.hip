@@ -24,6 +24,8 @@ struct vg_tree {
     vg::Nested ng;
     float4 *P[vg::kMaxLayers] = {};
     float4 *W[vg::kMaxLayers] = {};
+    float4 *M[vg::kMaxLayers] = {};  // 3 float4 per cluster: first moment rows + antisymmetric part
+    size_t m_cap[vg::kMaxLayers] = {};
     size_t layer_cap[vg::kMaxLayers] = {};
     uint32_t *cell_start = nullptr, *cell_count = nullptr;
     size_t cell_cap = 0;
@@ -180,7 +182,7 @@ __global__ void k_gather_sorted(VortonIn vin, const uint32_t *__restrict__ sorte
 // MakeBaseVortonGrid accumulation + normalisation (vortonSim.cpp:1237-1310).
 __global__ void k_leaf(const uint32_t *__restrict__ start, const uint32_t *__restrict__ count,
                        const float4 *__restrict__ sp, const float4 *__restrict__ sw, size_t cap,
-                       float4 *__restrict__ P, float4 *__restrict__ W) {
+                       float4 *__restrict__ P, float4 *__restrict__ W, float4 *__restrict__ M) {
 #pragma clang fp contract(off)
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= cap) return;
@@ -203,13 +205,29 @@ __global__ void k_leaf(const uint32_t *__restrict__ start, const uint32_t *__res
     }
     P[c] = make_float4(px, py, pz, sz);
     W[c] = make_float4(wx, wy, wz, aux);
+    // First moment about the centroid, D[b][e] = sum Omega_b (y - c)_e, in
+    // index order (no reference counterpart; used by the near-field mode).
+    float D[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (uint32_t k = 0; k < cnt; ++k) {
+        const float4 p = sp[b + k], w = sw[b + k];
+        const float e[3] = {p.x - px, p.y - py, p.z - pz};
+        const float om[3] = {w.x, w.y, w.z};
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) D[3 * bb + a] += om[bb] * e[a];
+    }
+    M[3 * c] = make_float4(D[0], D[1], D[2], D[5] - D[7]);
+    M[3 * c + 1] = make_float4(D[3], D[4], D[5], D[6] - D[2]);
+    M[3 * c + 2] = make_float4(D[6], D[7], D[8], D[1] - D[3]);
 }
 
 // AggregateClusters (vortonSim.cpp:1337-1569) for one parent layer.
 __global__ void k_aggregate(vg_grid_desc gp, vg_grid_desc gc, uint32_t dx, uint32_t dy,
                             uint32_t dz, const float4 *__restrict__ Pc,
-                            const float4 *__restrict__ Wc, float4 *__restrict__ Pp,
-                            float4 *__restrict__ Wp) {
+                            const float4 *__restrict__ Wc, const float4 *__restrict__ Mc,
+                            float4 *__restrict__ Pp, float4 *__restrict__ Wp,
+                            float4 *__restrict__ Mp) {
 #pragma clang fp contract(off)
     const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t cap = capacity(gp);
@@ -219,6 +237,7 @@ __global__ void k_aggregate(vg_grid_desc gp, vg_grid_desc gc, uint32_t dx, uint3
     const uint32_t iz = (uint32_t)(o / ((size_t)gp.np[0] * gp.np[1]));
     float px = 0.f, py = 0.f, pz = 0.f, wx = 0.f, wy = 0.f, wz = 0.f, sz = 0.f;
     float aux = FLT_MIN;
+    float D[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (ix < gp.np[0] - 1 && iy < gp.np[1] - 1 && iz < gp.np[2] - 1) {
         for (uint32_t cz = 0; cz < dz; ++cz)
             for (uint32_t cy = 0; cy < dy; ++cy)
@@ -234,11 +253,30 @@ __global__ void k_aggregate(vg_grid_desc gp, vg_grid_desc gc, uint32_t dx, uint3
                 }
         const float inv = 1.0f / aux;
         px *= inv; py *= inv; pz *= inv;
+        // first moment about the new centroid: D_p += D_c + Omega_c (c_c - c_p)
+        const float cp[3] = {px, py, pz};
+        for (uint32_t cz = 0; cz < dz; ++cz)
+            for (uint32_t cy = 0; cy < dy; ++cy)
+                for (uint32_t cx = 0; cx < dx; ++cx) {
+                    const size_t co = point_offset(gc, ix * dx + cx, iy * dy + cy, iz * dz + cz);
+                    const float4 p = Pc[co], w = Wc[co];
+                    const float4 m0 = Mc[3 * co], m1 = Mc[3 * co + 1], m2 = Mc[3 * co + 2];
+                    const float cd[9] = {m0.x, m0.y, m0.z, m1.x, m1.y, m1.z, m2.x, m2.y, m2.z};
+                    const float cw[3] = {w.x, w.y, w.z}, cc[3] = {p.x, p.y, p.z};
+#pragma unroll
+                    for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+                        for (int a = 0; a < 3; ++a)
+                            D[3 * bb + a] += cd[3 * bb + a] + cw[bb] * (cc[a] - cp[a]);
+                }
     } else {
         aux = 0.f;
     }
     Pp[o] = make_float4(px, py, pz, sz);
     Wp[o] = make_float4(wx, wy, wz, aux);
+    Mp[3 * o] = make_float4(D[0], D[1], D[2], D[5] - D[7]);
+    Mp[3 * o + 1] = make_float4(D[3], D[4], D[5], D[6] - D[2]);
+    Mp[3 * o + 2] = make_float4(D[6], D[7], D[8], D[1] - D[3]);
 }
 
 template <typename T>
@@ -564,6 +602,7 @@ TreeView make_view(const vg_tree *t) {
         for (int a = 0; a < 3; ++a) v.dec[l][a] = t->ng.dec[l][a];
         v.P[l] = t->P[l];
         v.W[l] = t->W[l];
+        v.M[l] = t->M[l];
     }
     v.cell_start = t->cell_start;
     v.cell_count = t->cell_count;
@@ -578,6 +617,16 @@ TreeView make_view(const vg_tree *t) {
 
 // Used by the simulation driver.
 TreeView tree_view(const vg_tree *t) { return make_view(t); }
+
+vg_status tree_checked_view(const vg_tree *t, TreeView *v) {
+    VG_REQUIRE(t && v, "tree is null");
+    if (!t->built) {
+        set_error("tree not built (call vg_tree_build first)");
+        return VG_ESTATE;
+    }
+    *v = make_view(t);
+    return VG_OK;
+}
 
 }  // namespace vg
 
@@ -594,6 +643,7 @@ VG_API vg_status vg_tree_destroy(vg_tree *t) {
     for (uint32_t l = 0; l < kMaxLayers; ++l) {
         hipFree(t->P[l]);
         hipFree(t->W[l]);
+        hipFree(t->M[l]);
     }
     hipFree(t->cell_start);
     hipFree(t->cell_count);
@@ -626,6 +676,7 @@ VG_API vg_status vg_tree_build(vg_tree *t, const vg_grid_desc *tmpl, const vg_vo
         VG_TRY(grow(&t->P[l], &c1, cap));
         VG_TRY(grow(&t->W[l], &c2, cap));
         t->layer_cap[l] = c1 < c2 ? c1 : c2;
+        VG_TRY(grow(&t->M[l], &t->m_cap[l], 3 * cap));
     }
     size_t cc1 = t->cell_cap, cc2 = t->cell_cap;
     VG_TRY(grow(&t->cell_start, &cc1, cap0));
@@ -663,13 +714,13 @@ VG_API vg_status vg_tree_build(vg_tree *t, const vg_grid_desc *tmpl, const vg_vo
     VG_TRY(launch("vg_tree_leaf", k_leaf, dim3(ceil_div(cap0, 256)), dim3(256), 0, s,
                   (const uint32_t *)t->cell_start, (const uint32_t *)t->cell_count,
                   (const float4 *)t->sorted_p, (const float4 *)t->sorted_w, cap0, t->P[0],
-                  t->W[0]));
+                  t->W[0], t->M[0]));
     for (uint32_t l = 1; l < t->ng.depth; ++l) {
         const size_t cap = capacity(t->ng.layer[l]);
         VG_TRY(launch("vg_tree_aggregate", k_aggregate, dim3(ceil_div(cap, 256)), dim3(256), 0, s,
                       t->ng.layer[l], t->ng.layer[l - 1], t->ng.dec[l][0], t->ng.dec[l][1],
                       t->ng.dec[l][2], (const float4 *)t->P[l - 1], (const float4 *)t->W[l - 1],
-                      t->P[l], t->W[l]));
+                      (const float4 *)t->M[l - 1], t->P[l], t->W[l], t->M[l]));
     }
     t->built = true;
     return VG_OK;
@@ -705,6 +756,26 @@ VG_API vg_status vg_tree_layer(const vg_tree *t, uint32_t l, vg_grid_desc *g, fl
     delete[] hw;
     VG_HIP(e1);
     VG_HIP(e2);
+    return VG_OK;
+}
+
+VG_API vg_status vg_tree_layer_moments(const vg_tree *t, uint32_t l, float *dip) {
+    VG_REQUIRE(t && dip, "null argument");
+    VG_REQUIRE(t->built, "tree not built");
+    VG_REQUIRE(l < t->ng.depth, "layer out of range");
+    const size_t cap = capacity(t->ng.layer[l]);
+    VG_HIP(hipDeviceSynchronize());
+    float4 *h = new float4[3 * cap];
+    hipError_t e = hipMemcpy(h, t->M[l], 3 * cap * sizeof(float4), hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+        for (size_t i = 0; i < cap; ++i)
+            for (int b = 0; b < 3; ++b) {
+                dip[9 * i + 3 * b] = h[3 * i + b].x;
+                dip[9 * i + 3 * b + 1] = h[3 * i + b].y;
+                dip[9 * i + 3 * b + 2] = h[3 * i + b].z;
+            }
+    delete[] h;
+    VG_HIP(e);
     return VG_OK;
 }
 

@@ -26,6 +26,7 @@ EXPORTS = (
     "vg_nested_layers vg_finalize_bounding_box vg_auto_mollify vg_find_bounding_box "
     "vg_direct_velocity vg_direct_vecpot vg_direct_velocity_grid vg_direct_velocity_f64 "
     "vg_tree_create vg_tree_destroy vg_tree_build vg_tree_depth vg_tree_layer "
+    "vg_tree_layer_moments vg_tree_velocity_grid_mac vg_tree_mac_work "
     "vg_tree_velocity vg_tree_velocity_grid vg_tree_interaction_count vg_tree_vecpot "
     "vg_tree_vecpot_boundary "
     "vg_tree_splat_vorticity vg_tree_populate_velocity_grid vg_downsample_inject "
@@ -328,6 +329,24 @@ class Tree:
                  policy=RSQRT_REFERENCE, q_stride=3, stream=None):
         call("vg_tree_velocity", self.h, q.ptr, C.c_size_t(q_stride), C.c_size_t(nq),
              C.c_float(spread), C.c_float(circ), policy, out.ptr, _s(stream))
+
+    def moments(self, l: int) -> np.ndarray:
+        g = GridDesc()
+        call("vg_tree_layer", self.h, l, C.byref(g), None, None, None)
+        out = np.zeros(g.capacity * 9, np.float32)
+        call("vg_tree_layer_moments", self.h, l, out.ctypes.data_as(C.c_void_p))
+        return out.reshape(-1, 3, 3)
+
+    def velocity_grid_mac(self, g: GridDesc, out: DeviceArray, near_k=2, order=1, spread=1.0,
+                          circ=1.0, policy=RSQRT_ACCURATE, stream=None):
+        call("vg_tree_velocity_grid_mac", self.h, C.byref(g), int(near_k), int(order),
+             C.c_float(spread), C.c_float(circ), policy, out.ptr, _s(stream))
+
+    def mac_work(self, g: GridDesc, near_k=2, stream=None):
+        c, p = C.c_uint64(), C.c_uint64()
+        call("vg_tree_mac_work", self.h, C.byref(g), int(near_k), C.byref(c), C.byref(p),
+             _s(stream))
+        return c.value, p.value
 
     def interaction_count(self, g: GridDesc, stream=None) -> int:
         c = C.c_uint64()
