@@ -29,7 +29,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-FLOP_PER_PAIR = 26.0
+FLOP_PER_PAIR = 26.0   # monopole / direct pair (FMA = 2, rsqrt = 1)
+FLOP_DIPOLE = 42.0     # first-moment correction of one cluster interaction
 
 
 def log(msg):
@@ -43,7 +44,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--vortons", type=int, default=1 << 20)
     ap.add_argument("--tracers", type=int, default=1 << 22)
-    ap.add_argument("--policy", type=int, default=0, help="0 reference rsqrt, 1 accurate")
+    ap.add_argument("--policy", type=int, default=1, help="0 reference rsqrt, 1 accurate")
+    ap.add_argument("--near-k", type=int, default=2,
+                    help="near-field radius of the accurate treecode; -1 = reference opening rule")
+    ap.add_argument("--order", type=int, default=1, help="cluster order: 0 monopole, 1 dipole")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true")
     args = ap.parse_args()
@@ -65,7 +69,8 @@ def main():
     d_vel = vg.DeviceArray.zeros(pos.shape)
     d_tr = vg.DeviceArray.from_numpy(tr)
     v = vg.vortons_view(d_pos, d_om, len(pos), uniform_size=2 * radius)
-    sim = vg.Sim(vg.sim_config(technique=vg.TECH_TREE, rsqrt_policy=args.policy), stream)
+    sim = vg.Sim(vg.sim_config(technique=vg.TECH_TREE, rsqrt_policy=args.policy,
+                               tree_near_k=args.near_k, tree_order=args.order), stream)
     sim.set_vortons(v)
     grid_holder = {"g": None, "field": None}
     evals = [0]
@@ -120,17 +125,26 @@ def main():
     # ---- roofline of the dominant kernel (treecode at grid points): kernel
     # time averaged over the timed launches, interactions counted on the final
     # frame's tree (the count drifts by well under 1% frame to frame)
-    kname = "vg_tree_velocity_grid"
-    cnt, ms = prof.get(kname, (0, 0.0))
     t = vg.Tree()
     t.build(g, v, stream)
-    interactions = t.interaction_count(g, stream)
+    if args.near_k < 0:
+        kname = "vg_tree_velocity_grid"
+        clusters, pairs = t.interaction_count(g, stream), 0
+        flop_cluster = FLOP_PER_PAIR
+    else:
+        kname = "vg_tree_velocity_grid_mac"
+        clusters, pairs = t.mac_work(g, args.near_k, stream)
+        flop_cluster = FLOP_PER_PAIR + (FLOP_DIPOLE if args.order else 0.0)
+    flops = flop_cluster * clusters + FLOP_PER_PAIR * pairs
+    cnt, ms = prof.get(kname, (0, 0.0))
     avg_s = (ms / max(cnt, 1)) * 1e-3
-    achieved = FLOP_PER_PAIR * interactions / avg_s / 1e12 if avg_s > 0 else 0.0
+    achieved = flops / avg_s / 1e12 if avg_s > 0 else 0.0
     roofline = {"kernel": kname, "bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                "avg_launch_ms": ms / max(cnt, 1), "interactions_per_launch": interactions,
-                "flop_per_interaction": FLOP_PER_PAIR}
+                "avg_launch_ms": ms / max(cnt, 1), "cluster_interactions_per_launch": clusters,
+                "direct_pairs_per_launch": pairs, "flop_per_cluster": flop_cluster,
+                "flop_per_pair": FLOP_PER_PAIR,
+                "note": "FP32 VALU kernel (no MFMA); peak is the published FP32 vector peak"}
     # tracer advection (HBM bound): 24 B per tracer (pos read + write)
     acnt, ams = prof.get("vg_advect_tracers", (0, 0.0))
     adv = None
@@ -183,16 +197,23 @@ def main():
             ot = O.InfluenceTree(og, cur, om, 2 * radius)
             t_build = time.perf_counter() - tb
             nz = int(g.np[2])
-            slices = max(1, min(nz, 24))
-            sub = O.Grid()
-            for k in ("min", "extent", "cell", "cpe", "np"):
-                for a in range(3):
-                    getattr(sub, k)[a] = getattr(og, k)[a]
-            sub.np[2] = slices
             spread, circ = sim.spreading()
             tv = time.perf_counter()
-            O.velocity_grid(ot, sub, cur, om, 2 * radius, 1, spread, circ, args.policy,
-                            threads=cores)
+            if args.near_k < 0:
+                slices = max(1, min(nz, 24))
+                sub = O.Grid()
+                for k in ("min", "extent", "cell", "cpe", "np"):
+                    for a in range(3):
+                        getattr(sub, k)[a] = getattr(og, k)[a]
+                sub.np[2] = slices
+                O.velocity_grid(ot, sub, cur, om, 2 * radius, 1, spread, circ, args.policy,
+                                threads=cores)
+            else:
+                rows = 1 if args.near_k >= 2 else 2
+                bz0 = (nz + 3) // 8  # a brick row a quarter of the way up the grid
+                O.mac_velocity_grid_bricks(ot, og, cur, om, 2 * radius, args.near_k, args.order,
+                                           spread, circ, args.policy, cores, bz0, bz0 + rows)
+                slices = 4 * rows
             t_vel = time.perf_counter() - tv
             pts = int(g.np[0]) * int(g.np[1]) * slices
             ntr = min(len(tr), 1 << 20)
@@ -223,10 +244,14 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "treecode frame (TREE technique, reference opening rule, "
-                               "auto-mollified) + 4M-tracer advection",
+        "config": {"workload": ("treecode frame (TREE technique, " + (
+                       "reference opening rule" if args.near_k < 0 else
+                       f"near-field treecode near_k={args.near_k} "
+                       f"{'dipole' if args.order else 'monopole'} clusters") +
+                       ", auto-mollified) + 4M-tracer advection"),
                    "vortons": len(pos), "tracers": len(tr), "grid": list(g.shape),
                    "scenario": "64 random rings seed 5", "rsqrt_policy": args.policy,
+                   "near_k": args.near_k, "order": args.order,
                    "parallelism": "single GPU"},
         "rel_l2_vs_fp64_direct": rel,
         "roofline": roofline,

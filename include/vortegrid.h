@@ -292,6 +292,11 @@ typedef struct vg_sim_config {
     int poisson_solver;  /* 0: reference cascade, 1: V-cycles to mg_tol */
     double mg_tol;
     uint32_t mg_max_cycles;
+    /* TREE technique grid evaluation: tree_near_k < 0 keeps the reference
+     * opening rule; tree_near_k >= 0 selects the near-field treecode with
+     * that near radius and cluster order tree_order (0 monopole, 1 dipole). */
+    int tree_near_k;
+    int tree_order;
 } vg_sim_config;
 vg_status vg_sim_config_default(vg_sim_config *cfg);
 vg_status vg_sim_create(vg_sim **sim, const vg_sim_config *cfg, vg_stream s);

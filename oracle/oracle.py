@@ -127,6 +127,10 @@ def lib():
         L.vgo_mac_velocity_grid.argtypes = [C.POINTER(Tree), C.POINTER(Grid), f32p, f32p, f32p,
                                             C.c_int, C.c_int, C.c_float, C.c_float, C.c_int,
                                             f32p, C.c_int]
+        L.vgo_mac_velocity_grid_bricks.argtypes = [C.POINTER(Tree), C.POINTER(Grid), f32p, f32p,
+                                                   f32p, C.c_int, C.c_int, C.c_float, C.c_float,
+                                                   C.c_int, f32p, C.c_int, C.c_uint32,
+                                                   C.c_uint32]
         L.vgo_vecpot_boundary.argtypes = [C.POINTER(Tree), C.POINTER(Grid), f32p, f32p, f32p,
                                           C.c_size_t, C.c_float, C.c_float, C.c_int, C.c_int,
                                           f32p, C.c_int]
@@ -344,6 +348,18 @@ def mac_velocity_grid(tree, g: Grid, pos, om, size, near_k=2, order=1, spread=1.
     lib().vgo_mac_velocity_grid(tree.ptr(), C.byref(g), pos, _f32(om).reshape(-1),
                                 _size_arr(size, n), near_k, order, spread, circ, policy, out,
                                 threads)
+    return out.reshape(-1, 3)
+
+
+def mac_velocity_grid_bricks(tree, g: Grid, pos, om, size, near_k, order, spread, circ, policy,
+                             threads, bz_begin, bz_end):
+    """Near-field treecode on z-brick rows [bz_begin, bz_end) only."""
+    pos = _f32(pos).reshape(-1)
+    n = pos.size // 3
+    out = np.zeros(g.capacity * 3, np.float32)
+    lib().vgo_mac_velocity_grid_bricks(tree.ptr(), C.byref(g), pos, _f32(om).reshape(-1),
+                                       _size_arr(size, n), near_k, order, spread, circ, policy,
+                                       out, threads, bz_begin, bz_end)
     return out.reshape(-1, 3)
 
 

@@ -600,16 +600,34 @@ static void mac_range(void *vc, size_t b, size_t e) {
             for (uint32_t bx = 0; bx < c->nb[0]; ++bx) mac_brick(c, bx, by, (uint32_t)bz);
 }
 
-void vgo_mac_velocity_grid(const vgo_tree *t, const vgo_grid *g, const float *pos,
-                           const float *om, const float *size, int near_k, int order,
-                           float spread, float circ, int rsqrt_policy, float *out,
-                           int nthreads) {
+typedef struct { mac_ctx *c; uint32_t bz0; } mac_slab;
+
+static void mac_slab_range(void *vs, size_t b, size_t e) {
+    mac_slab *s = (mac_slab *)vs;
+    mac_range(s->c, s->bz0 + b, s->bz0 + e);
+}
+
+void vgo_mac_velocity_grid_bricks(const vgo_tree *t, const vgo_grid *g, const float *pos,
+                                  const float *om, const float *size, int near_k, int order,
+                                  float spread, float circ, int rsqrt_policy, float *out,
+                                  int nthreads, uint32_t bz_begin, uint32_t bz_end) {
     mac_ctx c;
     c.t = t; c.g = g; c.pos = pos; c.om = om; c.size = size;
     c.k = near_k; c.order = order; c.policy = rsqrt_policy;
     c.spread = spread; c.circ = circ; c.out = out;
     for (int a = 0; a < 3; ++a) c.nb[a] = (g->np[a] + 3) / 4;
-    parallel_range(c.nb[2], nthreads, mac_range, &c);
+    if (bz_end > c.nb[2]) bz_end = c.nb[2];
+    if (bz_begin >= bz_end) return;
+    mac_slab s = {&c, bz_begin};
+    parallel_range(bz_end - bz_begin, nthreads, mac_slab_range, &s);
+}
+
+void vgo_mac_velocity_grid(const vgo_tree *t, const vgo_grid *g, const float *pos,
+                           const float *om, const float *size, int near_k, int order,
+                           float spread, float circ, int rsqrt_policy, float *out,
+                           int nthreads) {
+    vgo_mac_velocity_grid_bricks(t, g, pos, om, size, near_k, order, spread, circ, rsqrt_policy,
+                                 out, nthreads, 0, (g->np[2] + 3) / 4);
 }
 
 /* vortonSim.cpp:1650-1768 / 1787-1905: recursive traversal.  A cluster of

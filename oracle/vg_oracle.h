@@ -125,6 +125,11 @@ void vgo_mac_velocity_grid(const vgo_tree *t, const vgo_grid *g, const float *po
                            const float *om, const float *size, int near_k,
                            int order, float spread, float circ, int rsqrt_policy,
                            float *out, int nthreads);
+/* Same, restricted to z-brick rows [bz_begin, bz_end) (timing samples). */
+void vgo_mac_velocity_grid_bricks(const vgo_tree *t, const vgo_grid *g, const float *pos,
+                                  const float *om, const float *size, int near_k, int order,
+                                  float spread, float circ, int rsqrt_policy, float *out,
+                                  int nthreads, uint32_t bz_begin, uint32_t bz_end);
 
 /* Auto-mollification (vortonSim.cpp:2458-2477). */
 void vgo_auto_mollify(const vgo_grid *vel_grid, float vorton_radius,

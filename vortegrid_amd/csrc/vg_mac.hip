@@ -83,6 +83,8 @@ __global__ __launch_bounds__(256) void k_mac_grid(const TreeView tv, vg_grid_des
     unsigned long long n_clusters = 0, n_pairs = 0;
     const float spread = mp.spread;
     const int depth = (int)tv.depth;
+    // per-size kernel factors, recomputed only when the cluster size changes
+    float f_size = -1.0f, f_r2 = 0.f, f_s = 0.f, f_ir3 = 0.f;
     int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     for (int l = depth - 2; l >= 0; --l) {
         const vg_grid_desc &gl = tv.layer[l];
@@ -118,12 +120,17 @@ __global__ __launch_bounds__(256) void k_mac_grid(const TreeView tv, vg_grid_des
                         continue;
                     }
                     const float4 w = W[off];
-                    const float radius = p.w * 0.5f * spread;
-                    const float r2 = radius * radius;
-                    const float s = (2.0f / 3.0f) * r2 * radius;
+                    if (p.w != f_size) {  // uniform branch: sizes rarely change
+                        f_size = p.w;
+                        const float radius = p.w * 0.5f * spread;
+                        f_r2 = radius * radius;
+                        f_s = (2.0f / 3.0f) * f_r2 * radius;
+                        f_ir3 = __builtin_amdgcn_rcpf(f_r2 * radius);
+                    }
+                    const float r2 = f_r2, s = f_s;
                     const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
                     const float d2 = dx * dx + dy * dy + dz * dz;
-                    const float law = kernel_law<POLICY>(d2, r2, __builtin_amdgcn_rcpf(r2 * radius));
+                    const float law = kernel_law<POLICY>(d2, r2, f_ir3);
                     const float wx = w.x * s, wy = w.y * s, wz = w.z * s;
                     part.x += (wy * dz - wz * dy) * law;
                     part.y += (wz * dx - wx * dz) * law;
@@ -167,19 +174,24 @@ __global__ __launch_bounds__(256) void k_mac_grid(const TreeView tv, vg_grid_des
                 for (uint32_t j = 0; j < cnt; ++j) {
                     const float4 p = tv.sorted_p[st + j];
                     const float4 w = tv.sorted_w[st + j];
-                    const float radius = p.w * 0.5f * spread;
-                    const float r2 = radius * radius;
-                    const float s = (2.0f / 3.0f) * r2 * radius;
-                    const float4 A = make_float4(p.x, p.y, p.z, r2);
-                    const float4 B = make_float4(w.x * s, w.y * s, w.z * s,
-                                                 __builtin_amdgcn_rcpf(r2 * radius));
+                    if (p.w != f_size) {
+                        f_size = p.w;
+                        const float radius = p.w * 0.5f * spread;
+                        f_r2 = radius * radius;
+                        f_s = (2.0f / 3.0f) * f_r2 * radius;
+                        f_ir3 = __builtin_amdgcn_rcpf(f_r2 * radius);
+                    }
+                    const float4 A = make_float4(p.x, p.y, p.z, f_r2);
+                    const float4 B = make_float4(w.x * f_s, w.y * f_s, w.z * f_s, f_ir3);
                     pair_velocity<POLICY>(part, q, A, B);
                 }
             }
     kahan_add(sum, comp, part);
     if (COUNT) {
+        // the ballot must run on every lane: inside the lane-0 branch only
+        // lane 0 would vote
+        const unsigned long long nv = (unsigned long long)__popcll(__ballot(valid));
         if (lane == 0) {
-            const unsigned long long nv = (unsigned long long)__popcll(__ballot(valid));
             atomicAdd(&counts[0], n_clusters * nv);
             atomicAdd(&counts[1], n_pairs * nv);
         }

@@ -61,6 +61,8 @@ VG_API vg_status vg_sim_config_default(vg_sim_config *c) {
     c->poisson_solver = 0;
     c->mg_tol = 1e-6;
     c->mg_max_cycles = 30;
+    c->tree_near_k = -1;
+    c->tree_order = 1;
     return VG_OK;
 }
 
@@ -192,7 +194,13 @@ VG_API vg_status vg_sim_update(vg_sim *sim, float dt, uint32_t frame, float *vel
         sim->spread = 1.f;
         sim->circ = 1.f;
         if (sim->cfg.auto_mollify) VG_TRY(vg_auto_mollify(&g, radius, &sim->spread, &sim->circ));
-        VG_TRY(vg_tree_velocity_grid(sim->tree, &g, sim->spread, sim->circ, pol, sim->vel_grid, s));
+        if (sim->cfg.tree_near_k < 0)
+            VG_TRY(vg_tree_velocity_grid(sim->tree, &g, sim->spread, sim->circ, pol, sim->vel_grid,
+                                         s));
+        else
+            VG_TRY(vg_tree_velocity_grid_mac(sim->tree, &g, sim->cfg.tree_near_k,
+                                             sim->cfg.tree_order, sim->spread, sim->circ, pol,
+                                             sim->vel_grid, s));
     } else {
         sim->spread = 1.f;
         sim->circ = 1.f;

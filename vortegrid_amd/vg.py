@@ -97,6 +97,8 @@ class SimConfig(C.Structure):
         ("poisson_solver", C.c_int),
         ("mg_tol", C.c_double),
         ("mg_max_cycles", C.c_uint32),
+        ("tree_near_k", C.c_int),
+        ("tree_order", C.c_int),
     ]
 
 
