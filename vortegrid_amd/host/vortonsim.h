@@ -1,0 +1,438 @@
+// vortonsim.h -- C++ facade with VorteGrid's VortonSim / particle-operation API
+// (Samples/MjgIntelFluidDemo_Part20: VortonFluid/vortonSim.h:137-623,
+// VortonFluid/pclOpVortonSim.h:46-71, Particles/Operation/particleOperation.h:
+// 27-44, pclOpAdvect.h, pclOpEvolve.h, pclOpFindBoundingBox.h) over the C-ABI
+// in include/vortegrid.h.  Header-only; link with libvortegrid.so.
+//
+// Host-side particle arrays keep the reference's 72-byte Particle record, so
+// an application that owns VECTOR<Particle> groups can swap its VortonSim /
+// operators for these classes.  Each Operate/Update call stages the records on
+// the device (one copy in, one copy out), runs the HIP path, and writes
+// velocities back into the records.  Grids are materialised on the host on
+// demand (GetVelocityGrid and friends).
+#pragma once
+
+#include <cfloat>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/vortegrid.h"
+
+namespace vgfacade {
+
+inline void check(vg_status s, const char *what) {
+    if (s != VG_OK) throw std::runtime_error(std::string(what) + ": " + vg_last_error());
+}
+
+// ------------------------------------------------------------------ math / particles
+struct Vec3 {
+    float x, y, z;
+    Vec3() : x(0), y(0), z(0) {}
+    Vec3(float a, float b, float c) : x(a), y(b), z(c) {}
+    Vec3 operator+(const Vec3 &o) const { return Vec3(x + o.x, y + o.y, z + o.z); }
+    Vec3 operator-(const Vec3 &o) const { return Vec3(x - o.x, y - o.y, z - o.z); }
+    Vec3 operator*(float f) const { return Vec3(x * f, y * f, z * f); }
+    Vec3 operator-() const { return Vec3(-x, -y, -z); }
+};
+
+// particle.h:425-446 (72 bytes, ENABLE_FIRE layout).
+struct Particle {
+    Vec3 mPosition;
+    Vec3 mVelocity;
+    Vec3 mOrientation;
+    Vec3 mAngularVelocity;  // = vorticity / 2 for vortons
+    float mDensity = 1.0f;
+    float mSize = 0.0f;     // diameter
+    int mBirthTime = 0;
+    float mFuelFraction = 0.0f, mFlameFraction = 0.0f, mSmokeFraction = 0.0f;
+    float GetRadius() const { return mSize * 0.5f; }
+    void SetRadius(float r) { mSize = 2.0f * r; }
+};
+static_assert(sizeof(Particle) == 72, "Particle must keep the reference's 72-byte record");
+
+// vorton.h:198-387: a Particle whose angular velocity is half its vorticity.
+struct Vorton : Particle {
+    Vorton() = default;
+    Vorton(const Vec3 &pos, const Vec3 &vort, float radius) {
+        mPosition = pos;
+        mAngularVelocity = vort * 0.5f;
+        SetRadius(radius);
+    }
+    Vec3 GetVorticity() const { return mAngularVelocity * 2.0f; }
+    void SetVorticity(const Vec3 &w) { mAngularVelocity = w * 0.5f; }
+};
+static_assert(sizeof(Vorton) == 72, "Vorton adds no data to Particle");
+
+template <typename T>
+using VECTOR = std::vector<T>;
+
+constexpr size_t kStride = sizeof(Particle) / sizeof(float);  // 18 floats
+constexpr size_t kOffVel = offsetof(Particle, mVelocity) / sizeof(float);
+constexpr size_t kOffAngVel = offsetof(Particle, mAngularVelocity) / sizeof(float);
+constexpr size_t kOffSize = offsetof(Particle, mSize) / sizeof(float);
+
+// ------------------------------------------------------------------ grids
+// UniformGridGeometry (uniformGrid.h:144-860).
+class UniformGridGeometry {
+  public:
+    UniformGridGeometry() { std::memset(&g_, 0, sizeof g_); }
+    explicit UniformGridGeometry(const vg_grid_desc &g) : g_(g) {}
+    void DefineShape(size_t n, const Vec3 &mn, const Vec3 &mx, bool pow2) {
+        const float a[3] = {mn.x, mn.y, mn.z}, b[3] = {mx.x, mx.y, mx.z};
+        check(vg_define_shape(&g_, n, a, b, pow2 ? 1 : 0), "DefineShape");
+    }
+    unsigned GetNumPoints(unsigned i) const { return g_.np[i]; }
+    unsigned GetNumCells(unsigned i) const { return g_.np[i] - 1; }
+    unsigned GetGridCapacity() const { return g_.np[0] * g_.np[1] * g_.np[2]; }
+    Vec3 GetMinCorner() const { return Vec3(g_.min[0], g_.min[1], g_.min[2]); }
+    Vec3 GetExtent() const { return Vec3(g_.extent[0], g_.extent[1], g_.extent[2]); }
+    Vec3 GetCellSpacing() const { return Vec3(g_.cell[0], g_.cell[1], g_.cell[2]); }
+    Vec3 GetCellsPerExtent() const { return Vec3(g_.cpe[0], g_.cpe[1], g_.cpe[2]); }
+    bool HasZeroExtent() const {
+        return g_.extent[0] == 0.f && g_.extent[1] == 0.f && g_.extent[2] == 0.f;
+    }
+    size_t OffsetFromIndices(size_t ix, size_t iy, size_t iz) const {
+        return ix + g_.np[0] * (iy + g_.np[1] * iz);
+    }
+    const vg_grid_desc &desc() const { return g_; }
+
+  protected:
+    vg_grid_desc g_;
+};
+
+// UniformGrid<T> (uniformGrid.h:862-1785), host copy of a device grid.
+template <class ItemT>
+class UniformGrid : public UniformGridGeometry {
+  public:
+    UniformGrid() = default;
+    explicit UniformGrid(const UniformGridGeometry &geom) : UniformGridGeometry(geom) {}
+    void Init() { contents_.assign(GetGridCapacity(), ItemT()); }
+    size_t Size() const { return contents_.size(); }
+    bool Empty() const { return contents_.empty(); }
+    ItemT &operator[](size_t o) { return contents_[o]; }
+    const ItemT &operator[](size_t o) const { return contents_[o]; }
+    ItemT *data() { return contents_.data(); }
+    // Copy a packed device field of this geometry into the host grid.
+    void Download(const vg_grid_desc &g, const void *device) {
+        g_ = g;
+        Init();
+        if (!contents_.empty())
+            check(vg_memcpy_dtoh(contents_.data(), device, contents_.size() * sizeof(ItemT)),
+                  "UniformGrid download");
+    }
+
+  private:
+    std::vector<ItemT> contents_;
+};
+
+// ------------------------------------------------------------------ device staging
+class DeviceBuffer {
+  public:
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    ~DeviceBuffer() { vg_free(p_); }
+    void *reserve(size_t bytes) {
+        if (bytes > cap_) {
+            vg_free(p_);
+            p_ = nullptr;
+            cap_ = 0;
+            check(vg_malloc(&p_, bytes), "device allocation");
+            cap_ = bytes;
+        }
+        return p_;
+    }
+    void *get() const { return p_; }
+
+  private:
+    void *p_ = nullptr;
+    size_t cap_ = 0;
+};
+
+// ------------------------------------------------------------------ operators
+// particleOperation.h:27-44
+class IParticleOperation {
+  public:
+    virtual ~IParticleOperation() {}
+    virtual IParticleOperation *Create() = 0;
+    virtual IParticleOperation *Clone() = 0;
+    virtual void Operate(VECTOR<Particle> &particles, float timeStep, unsigned uFrame) = 0;
+};
+
+#define VGFACADE_VIRTUAL_CONSTRUCTORS(ClassT)                         \
+    IParticleOperation *Create() override { return new ClassT(); }    \
+    IParticleOperation *Clone() override { return new ClassT(*this); }
+
+// ------------------------------------------------------------------ VortonSim
+// vortonSim.h:137-623, velocity-from-vorticity subset.
+class VortonSim {
+  public:
+    explicit VortonSim(float viscosity = 0.0f, float ambientFluidDensity = 1.0f,
+                       int technique = VG_TECH_POISSON)
+        : viscosity_(viscosity), density_(ambientFluidDensity) {
+        vg_sim_config_default(&cfg_);
+        cfg_.technique = technique;
+    }
+    VortonSim(const VortonSim &that) : cfg_(that.cfg_) {}  // copies drop the vortons (vortonSim.h:286-295)
+    VortonSim &operator=(const VortonSim &that) {
+        if (this != &that) {
+            cfg_ = that.cfg_;
+            mVortons = nullptr;
+        }
+        return *this;
+    }
+    ~VortonSim() { vg_sim_destroy(sim_); }
+
+    vg_sim_config &Config() { return cfg_; }
+
+    void SetVortons(VECTOR<Vorton> *vortons) { mVortons = vortons; }
+    VECTOR<Vorton> *GetVortons() { return mVortons; }
+    const VECTOR<Vorton> *GetVortons() const { return mVortons; }
+
+    // Initialize (vortonSim.cpp:933-992), velocity-path part: bounding box.
+    void Initialize(bool /*systemTracksFluidSurfaceUsingSdf*/ = false) {
+        FindBoundingBox();
+        if (mVortons && !mVortons->empty()) {
+            const Vec3 p = (*mVortons)[0].mPosition;
+            UpdateBoundingBox(p, p, true);
+        }
+    }
+
+    // vortonSim.cpp:1001-1008
+    void FindBoundingBox() {
+        ensure();
+        stage_vortons();
+        check(vg_sim_find_bounding_box(sim_), "FindBoundingBox");
+        sync_bounds();
+    }
+
+    // vortonSim.cpp:1027-1096
+    void UpdateBoundingBox(const Vec3 &minCorner, const Vec3 &maxCorner, bool bFinal) {
+        ensure();
+        const float a[3] = {minCorner.x, minCorner.y, minCorner.z};
+        const float b[3] = {maxCorner.x, maxCorner.y, maxCorner.z};
+        check(vg_sim_update_bounding_box(sim_, a, b, bFinal ? 1 : 0), "UpdateBoundingBox");
+        sync_bounds();
+        if (bFinal) {
+            vg_grid_desc g;
+            check(vg_sim_grid_template(sim_, &g), "GetGrid");
+            grid_ = UniformGridGeometry(g);
+        }
+    }
+
+    // vortonSim.cpp:4279-4344 (velocity-from-vorticity part of
+    // UpdateVortexParticleMethod, :4117-4256): vorton velocities are written
+    // into the records (DIRECT: direct velocity; otherwise grid velocity).
+    void Update(float timeStep, unsigned uFrame) {
+        ensure();
+        if (!mVortons || mVortons->empty()) return;
+        stage_vortons();
+        float *base = static_cast<float *>(dev_.get());
+        check(vg_sim_update(sim_, timeStep, uFrame, base + kOffVel, kStride), "Update");
+        check(vg_memcpy_dtoh(mVortons->data(), dev_.get(), mVortons->size() * sizeof(Vorton)),
+              "download vortons");
+        vel_stale_ = pot_stale_ = negw_stale_ = true;
+    }
+
+    void Clear() {
+        vg_sim_destroy(sim_);
+        sim_ = nullptr;
+        mVortons = nullptr;
+    }
+
+    // vortonSim.h:334-343 (host copies, materialised on demand)
+    const UniformGrid<Vec3> &GetVelocityGrid() {
+        if (vel_stale_ && sim_) {
+            vg_grid_desc g;
+            const float *f = nullptr;
+            check(vg_sim_velocity_grid(sim_, &g, &f), "GetVelocityGrid");
+            if (f) velGrid_.Download(g, f);
+            vel_stale_ = false;
+        }
+        return velGrid_;
+    }
+    const UniformGrid<Vec3> &GetVectorPotentialGrid() {
+        if (pot_stale_ && sim_) {
+            vg_grid_desc g;
+            const float *f = nullptr;
+            check(vg_sim_vector_potential(sim_, &g, &f), "GetVectorPotentialMultiGrid");
+            if (f) potGrid_.Download(g, f);
+            pot_stale_ = false;
+        }
+        return potGrid_;
+    }
+    const UniformGrid<Vec3> &GetNegativeVorticityGrid() {
+        if (negw_stale_ && sim_) {
+            vg_grid_desc g;
+            const float *f = nullptr;
+            check(vg_sim_negative_vorticity(sim_, &g, &f), "GetNegativeVorticityMultiGrid");
+            if (f) negwGrid_.Download(g, f);
+            negw_stale_ = false;
+        }
+        return negwGrid_;
+    }
+    // Device view of the velocity grid (no copy), for device-side operators.
+    const float *GetVelocityGridDevice(vg_grid_desc *g) const {
+        const float *f = nullptr;
+        check(vg_sim_velocity_grid(sim_, g, &f), "GetVelocityGrid");
+        return f;
+    }
+    double GetPoissonResidual() const {
+        double r = -1.0;
+        if (sim_) check(vg_sim_poisson_residual(sim_, &r), "GetPoissonResidualStats");
+        return r;
+    }
+    const UniformGridGeometry &GetGrid() const { return grid_; }
+    const Vec3 &GetMinCorner() const { return minCorner_; }
+    const Vec3 &GetMaxCorner() const { return maxCorner_; }
+
+  private:
+    void ensure() {
+        if (!sim_) check(vg_sim_create(&sim_, &cfg_, nullptr), "VortonSim");
+    }
+    void stage_vortons() {
+        const size_t n = mVortons ? mVortons->size() : 0;
+        vg_vortons v;
+        std::memset(&v, 0, sizeof v);
+        v.stride = kStride;
+        v.n = n;
+        if (n) {
+            dev_.reserve(n * sizeof(Vorton));
+            check(vg_memcpy_htod(dev_.get(), mVortons->data(), n * sizeof(Vorton)),
+                  "upload vortons");
+            float *base = static_cast<float *>(dev_.get());
+            v.pos = base;
+            v.angvel = base + kOffAngVel;
+            v.size = base + kOffSize;
+        }
+        check(vg_sim_set_vortons(sim_, &v), "SetVortons");
+    }
+    void sync_bounds() {
+        float a[3], b[3];
+        check(vg_sim_bounds(sim_, a, b), "bounds");
+        minCorner_ = Vec3(a[0], a[1], a[2]);
+        maxCorner_ = Vec3(b[0], b[1], b[2]);
+    }
+
+    vg_sim_config cfg_;
+    vg_sim *sim_ = nullptr;
+    VECTOR<Vorton> *mVortons = nullptr;
+    DeviceBuffer dev_;
+    UniformGridGeometry grid_;
+    UniformGrid<Vec3> velGrid_, potGrid_, negwGrid_;
+    bool vel_stale_ = true, pot_stale_ = true, negw_stale_ = true;
+    Vec3 minCorner_, maxCorner_;
+    float viscosity_ = 0.0f, density_ = 1.0f;
+};
+
+// pclOpVortonSim.h:46-71 / pclOpVortonSim.cpp:51-71
+class PclOpVortonSim : public IParticleOperation {
+  public:
+    PclOpVortonSim() = default;
+    PclOpVortonSim(const PclOpVortonSim &that) : mVortonSim(that.mVortonSim) {}
+    VGFACADE_VIRTUAL_CONSTRUCTORS(PclOpVortonSim)
+    void Operate(VECTOR<Particle> &particles, float timeStep, unsigned uFrame) override {
+        mVortonSim.SetVortons(reinterpret_cast<VECTOR<Vorton> *>(&particles));
+        mVortonSim.FindBoundingBox();
+        if (mMinCorner && mMaxCorner) {
+            mVortonSim.UpdateBoundingBox(*mMinCorner, *mMaxCorner, true);
+        } else if (!particles.empty()) {
+            const Vec3 p = particles[0].mPosition;
+            mVortonSim.UpdateBoundingBox(p, p, true);
+        }
+        mVortonSim.Update(timeStep, uFrame);
+    }
+    VortonSim mVortonSim;
+    const Vec3 *mMinCorner = nullptr;
+    const Vec3 *mMaxCorner = nullptr;
+};
+
+// Particle group staged on the device for the tracer operators below.
+class DeviceParticles {
+  public:
+    float *upload(const VECTOR<Particle> &p) {
+        buf_.reserve(p.size() * sizeof(Particle));
+        if (!p.empty())
+            check(vg_memcpy_htod(buf_.get(), p.data(), p.size() * sizeof(Particle)), "upload");
+        return static_cast<float *>(buf_.get());
+    }
+    void download(VECTOR<Particle> &p) {
+        if (!p.empty())
+            check(vg_memcpy_dtoh(p.data(), buf_.get(), p.size() * sizeof(Particle)), "download");
+    }
+
+  private:
+    DeviceBuffer buf_;
+};
+
+// pclOpAdvect.cpp:132-163 (AssignVelocityFromField)
+class PclOpAssignVelocityFromField : public IParticleOperation {
+  public:
+    PclOpAssignVelocityFromField() = default;
+    PclOpAssignVelocityFromField(const PclOpAssignVelocityFromField &o)
+        : mVortonSim(o.mVortonSim), mGain(o.mGain) {}
+    VGFACADE_VIRTUAL_CONSTRUCTORS(PclOpAssignVelocityFromField)
+    void Operate(VECTOR<Particle> &particles, float, unsigned) override {
+        if (!mVortonSim || particles.empty()) return;
+        vg_grid_desc g;
+        const float *f = mVortonSim->GetVelocityGridDevice(&g);
+        if (!f) return;
+        float *base = dev_.upload(particles);
+        check(vg_assign_velocity_from_field(&g, f, base, base + kOffVel, kStride,
+                                            particles.size(), mGain, nullptr),
+              "AssignVelocityFromField");
+        dev_.download(particles);
+    }
+    const VortonSim *mVortonSim = nullptr;  // source of the velocity grid
+    float mGain = 1.0f;
+
+  private:
+    DeviceParticles dev_;
+};
+
+// pclOpEvolve.cpp:118-123
+class PclOpEvolve : public IParticleOperation {
+  public:
+    VGFACADE_VIRTUAL_CONSTRUCTORS(PclOpEvolve)
+    PclOpEvolve() = default;
+    PclOpEvolve(const PclOpEvolve &) {}
+    void Operate(VECTOR<Particle> &particles, float timeStep, unsigned) override {
+        if (particles.empty()) return;
+        float *base = dev_.upload(particles);
+        check(vg_evolve(base, base + kOffVel, kStride, particles.size(), timeStep, nullptr),
+              "Evolve");
+        dev_.download(particles);
+    }
+
+  private:
+    DeviceParticles dev_;
+};
+
+// pclOpFindBoundingBox.cpp:141-148
+class PclOpFindBoundingBox : public IParticleOperation {
+  public:
+    VGFACADE_VIRTUAL_CONSTRUCTORS(PclOpFindBoundingBox)
+    PclOpFindBoundingBox() = default;
+    PclOpFindBoundingBox(const PclOpFindBoundingBox &o)
+        : mMinCorner(o.mMinCorner), mMaxCorner(o.mMaxCorner) {}
+    void Operate(VECTOR<Particle> &particles, float, unsigned) override {
+        float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        if (!particles.empty()) {
+            float *base = dev_.upload(particles);
+            check(vg_find_bounding_box(base, kStride, particles.size(), mn, mx, nullptr),
+                  "FindBoundingBox");
+        }
+        mMinCorner = Vec3(mn[0], mn[1], mn[2]);
+        mMaxCorner = Vec3(mx[0], mx[1], mx[2]);
+    }
+    Vec3 mMinCorner, mMaxCorner;
+
+  private:
+    DeviceParticles dev_;
+};
+
+}  // namespace vgfacade
