@@ -63,9 +63,26 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    if os.path.exists(FACADE_SRC):
+        build_facade_driver()
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+FACADE_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "facade_frame.cpp")
+FACADE_EXE = os.path.join(os.path.dirname(HERE), "tests", "cpp", "facade_frame")
+
+
+def build_facade_driver() -> str:
+    """Compile the C++ facade frame driver (tests/cpp) against libvortegrid."""
+    root = os.path.dirname(HERE)
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", f"-I{root}", FACADE_SRC,
+           f"-L{HERE}", "-lvortegrid", "-Wl,-rpath,$ORIGIN/../../vortegrid_amd", "-o", FACADE_EXE]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"facade driver build failed:\n{r.stdout}\n{r.stderr}")
+    return FACADE_EXE
 
 
 def clean():

@@ -40,25 +40,73 @@ __device__ __forceinline__ uint32_t layer_scale(const TreeView &tv, int l, int a
     return s;
 }
 
-template <int POLICY>
-__device__ __forceinline__ float kernel_law(float d2, float r2, float inv_r3) {
-    float law;
+// Per-wave LDS staging area: 64 fetched clusters (or vortons) at a time.
+struct WaveStage {
+    float4 P[64], W[64], M0[64], M1[64], M2[64];
+    uint32_t row_off[64], row_start[64];
+};
+
+// Kernel factors for one cluster size; contributions are accumulated
+// unscaled and multiplied by s = (2/3) r^3 when the size changes or the
+// layer ends (every cluster of a tree normally has the vorton size).
+struct SizeFactors {
+    float size = -1.0f, r2 = 0.f, s = 0.f, ir3 = 0.f;
+};
+
+__device__ __forceinline__ void set_size(SizeFactors &f, float size, float spread, float3 &part,
+                                         float3 &scaled) {
+    scaled.x += part.x * f.s;
+    scaled.y += part.y * f.s;
+    scaled.z += part.z * f.s;
+    part = make_float3(0.f, 0.f, 0.f);
+    f.size = size;
+    const float radius = size * 0.5f * spread;
+    f.r2 = radius * radius;
+    f.s = (2.0f / 3.0f) * f.r2 * radius;
+    f.ir3 = __builtin_amdgcn_rcpf(f.r2 * radius);
+}
+
+// Monopole (+ first-moment) contribution of one cluster, unscaled by s.
+template <int POLICY, int ORDER>
+__device__ __forceinline__ void cluster_term(float3 &part, float3 q, float4 p, float4 w, float4 m0,
+                                             float4 m1, float4 m2, const SizeFactors &f) {
+    const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    float law, rs2;
     if (POLICY == VG_RSQRT_REFERENCE) {
-        law = finvsqrtf(d2) * __builtin_amdgcn_rcpf(d2);
+        rs2 = __builtin_amdgcn_rcpf(d2);
+        law = finvsqrtf(d2) * rs2;
     } else {
         const float rs = __builtin_amdgcn_rsqf(d2);
-        law = rs * rs * rs;
+        rs2 = rs * rs;
+        law = rs2 * rs;
     }
-    return d2 < r2 ? inv_r3 : law;
+    const bool inside = d2 < f.r2;
+    const float lm = inside ? f.ir3 : law;
+    part.x += (w.y * dz - w.z * dy) * lm;
+    part.y += (w.z * dx - w.x * dz) * lm;
+    part.z += (w.x * dy - w.y * dx) * lm;
+    if (ORDER >= 1) {
+        const float tx = m0.x * dx + m0.y * dy + m0.z * dz;
+        const float ty = m1.x * dx + m1.y * dy + m1.z * dz;
+        const float tz = m2.x * dx + m2.y * dy + m2.z * dz;
+        const float c3 = 3.0f * rs2;
+        const float ld = inside ? 0.0f : law;
+        part.x -= (m0.w - c3 * (ty * dz - tz * dy)) * ld;
+        part.y -= (m1.w - c3 * (tz * dx - tx * dz)) * ld;
+        part.z -= (m2.w - c3 * (tx * dy - ty * dx)) * ld;
+    }
 }
 
 template <int POLICY, int ORDER, int COUNT>
 __global__ __launch_bounds__(256) void k_mac_grid(const TreeView tv, vg_grid_desc g, MacParams mp,
                                                   float *__restrict__ out,
                                                   unsigned long long *__restrict__ counts) {
+    __shared__ WaveStage stage[4];
     const uint32_t wave = uni(threadIdx.x >> 6);
     const uint32_t brick = uni(blockIdx.x * 4 + wave);
     if (brick >= mp.nbricks) return;
+    WaveStage &S = stage[wave];
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t bb[3] = {brick % mp.nb[0], (brick / mp.nb[0]) % mp.nb[1],
                             brick / (mp.nb[0] * mp.nb[1])};
@@ -83,8 +131,7 @@ __global__ __launch_bounds__(256) void k_mac_grid(const TreeView tv, vg_grid_des
     unsigned long long n_clusters = 0, n_pairs = 0;
     const float spread = mp.spread;
     const int depth = (int)tv.depth;
-    // per-size kernel factors, recomputed only when the cluster size changes
-    float f_size = -1.0f, f_r2 = 0.f, f_s = 0.f, f_ir3 = 0.f;
+    SizeFactors f;
     int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     for (int l = depth - 2; l >= 0; --l) {
         const vg_grid_desc &gl = tv.layer[l];
@@ -105,54 +152,71 @@ __global__ __launch_bounds__(256) void k_mac_grid(const TreeView tv, vg_grid_des
         const float4 *__restrict__ P = tv.P[l];
         const float4 *__restrict__ W = tv.W[l];
         const float4 *__restrict__ M = tv.M[l];
-        float3 part = make_float3(0.f, 0.f, 0.f);
-        for (int cz = c0[2]; cz <= c1[2]; ++cz)
-            for (int cy = c0[1]; cy <= c1[1]; ++cy)
-                for (int cx = c0[0]; cx <= c1[0]; ++cx) {
-                    if (cx >= n0[0] && cx <= n1[0] && cy >= n0[1] && cy <= n1[1] &&
-                        cz >= n0[2] && cz <= n1[2])
-                        continue;
+        const int bx = c1[0] - c0[0] + 1, by = c1[1] - c0[1] + 1, bz = c1[2] - c0[2] + 1;
+        const int total = bx * by * bz;
+        float3 part = make_float3(0.f, 0.f, 0.f), scaled = part;
+        for (int base = 0; base < total; base += 64) {
+            // lanes fetch 64 cells of the children box
+            const int t = base + (int)lane;
+            bool use = false;
+            float4 p = make_float4(0.f, 0.f, 0.f, 0.f), w = p, m0 = p, m1 = p, m2 = p;
+            if (t < total) {
+                const int cx = c0[0] + t % bx, cy = c0[1] + (t / bx) % by, cz = c0[2] + t / (bx * by);
+                const bool near = cx >= n0[0] && cx <= n1[0] && cy >= n0[1] && cy <= n1[1] &&
+                                  cz >= n0[2] && cz <= n1[2];
+                if (!near) {
                     const size_t off = point_offset(gl, cx, cy, cz);
-                    const float4 p = P[off];
-                    if (p.w == 0.0f) continue;
-                    if (COUNT) {
-                        ++n_clusters;
-                        continue;
-                    }
-                    const float4 w = W[off];
-                    if (p.w != f_size) {  // uniform branch: sizes rarely change
-                        f_size = p.w;
-                        const float radius = p.w * 0.5f * spread;
-                        f_r2 = radius * radius;
-                        f_s = (2.0f / 3.0f) * f_r2 * radius;
-                        f_ir3 = __builtin_amdgcn_rcpf(f_r2 * radius);
-                    }
-                    const float r2 = f_r2, s = f_s;
-                    const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
-                    const float d2 = dx * dx + dy * dy + dz * dz;
-                    const float law = kernel_law<POLICY>(d2, r2, f_ir3);
-                    const float wx = w.x * s, wy = w.y * s, wz = w.z * s;
-                    part.x += (wy * dz - wz * dy) * law;
-                    part.y += (wz * dx - wx * dz) * law;
-                    part.z += (wx * dy - wy * dx) * law;
-                    if (ORDER >= 1 && d2 >= r2) {
-                        const float4 m0 = M[3 * off], m1 = M[3 * off + 1], m2 = M[3 * off + 2];
-                        const float tx = m0.x * dx + m0.y * dy + m0.z * dz;
-                        const float ty = m1.x * dx + m1.y * dy + m1.z * dz;
-                        const float tz = m2.x * dx + m2.y * dy + m2.z * dz;
-                        const float c3 = 3.0f * __builtin_amdgcn_rcpf(d2);
-                        const float sl = s * law;
-                        part.x -= (m0.w - c3 * (ty * dz - tz * dy)) * sl;
-                        part.y -= (m1.w - c3 * (tz * dx - tx * dz)) * sl;
-                        part.z -= (m2.w - c3 * (tx * dy - ty * dx)) * sl;
+                    p = P[off];
+                    use = p.w != 0.0f;
+                    if (use && !COUNT) {
+                        w = W[off];
+                        if (ORDER) {
+                            m0 = M[3 * off];
+                            m1 = M[3 * off + 1];
+                            m2 = M[3 * off + 2];
+                        }
                     }
                 }
-        kahan_add(sum, comp, part);
+            }
+            unsigned long long mask = __ballot(use);
+            if (COUNT) {
+                n_clusters += __popcll(mask);
+                continue;
+            }
+            S.P[lane] = p;
+            S.W[lane] = w;
+            if (ORDER) {
+                S.M0[lane] = m0;
+                S.M1[lane] = m1;
+                S.M2[lane] = m2;
+            }
+            __builtin_amdgcn_wave_barrier();
+            while (mask) {
+                const int j = __builtin_ctzll(mask);
+                mask &= mask - 1;
+                const float4 cp = S.P[j];
+                if (cp.w != f.size) set_size(f, cp.w, spread, part, scaled);
+                const float4 cw = S.W[j];
+                float4 a0 = cp, a1 = cp, a2 = cp;
+                if (ORDER) {
+                    a0 = S.M0[j];
+                    a1 = S.M1[j];
+                    a2 = S.M2[j];
+                }
+                cluster_term<POLICY, ORDER>(part, q, cp, cw, a0, a1, a2, f);
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        scaled.x += part.x * f.s;
+        scaled.y += part.y * f.s;
+        scaled.z += part.z * f.s;
+        kahan_add(sum, comp, scaled);
 #pragma unroll
         for (int a = 0; a < 3; ++a) { lo[a] = n0[a]; hi[a] = n1[a]; }
     }
-    // Leaf near box (or every vorton when the tree has one layer).
-    float3 part = make_float3(0.f, 0.f, 0.f);
+    // Leaf near box (or every vorton when the tree has one layer), fetched by
+    // rows: the vortons of cells x0..x1 of one (y, z) row are contiguous in
+    // the cell-sorted arrays.
     const vg_grid_desc &g0 = tv.layer[0];
     if (depth == 1) {
         lo[0] = lo[1] = lo[2] = 0;
@@ -160,33 +224,68 @@ __global__ __launch_bounds__(256) void k_mac_grid(const TreeView tv, vg_grid_des
         hi[1] = (int)g0.np[1] - 2;
         hi[2] = (int)g0.np[2] - 2;
     }
-    for (int cz = lo[2]; cz <= hi[2]; ++cz)
-        for (int cy = lo[1]; cy <= hi[1]; ++cy)
-            for (int cx = lo[0]; cx <= hi[0]; ++cx) {
-                const size_t key = point_offset(g0, cx, cy, cz);
-                const uint32_t cnt = tv.cell_count[key];
-                if (cnt == 0) continue;
-                if (COUNT) {
-                    n_pairs += cnt;
-                    continue;
-                }
-                const uint32_t st = tv.cell_start[key];
-                for (uint32_t j = 0; j < cnt; ++j) {
-                    const float4 p = tv.sorted_p[st + j];
-                    const float4 w = tv.sorted_w[st + j];
-                    if (p.w != f_size) {
-                        f_size = p.w;
-                        const float radius = p.w * 0.5f * spread;
-                        f_r2 = radius * radius;
-                        f_s = (2.0f / 3.0f) * f_r2 * radius;
-                        f_ir3 = __builtin_amdgcn_rcpf(f_r2 * radius);
-                    }
-                    const float4 A = make_float4(p.x, p.y, p.z, f_r2);
-                    const float4 B = make_float4(w.x * f_s, w.y * f_s, w.z * f_s, f_ir3);
-                    pair_velocity<POLICY>(part, q, A, B);
-                }
+    {
+        float3 part = make_float3(0.f, 0.f, 0.f), scaled = part;
+        const int ry = hi[1] - lo[1] + 1, rz = hi[2] - lo[2] + 1;
+        const int nrows = ry * rz;
+        for (int rbase = 0; rbase < nrows; rbase += 64) {
+            const int r = rbase + (int)lane;
+            uint32_t rs = 0, rc = 0;
+            if (r < nrows) {
+                const int cy = lo[1] + r % ry, cz = lo[2] + r / ry;
+                const size_t k0 = point_offset(g0, lo[0], cy, cz);
+                const size_t k1 = point_offset(g0, hi[0], cy, cz);
+                rs = tv.cell_start[k0];
+                rc = tv.cell_start[k1] + tv.cell_count[k1] - rs;
             }
-    kahan_add(sum, comp, part);
+            // inclusive scan of the row counts across the wave
+            uint32_t incl = rc;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t tt = __shfl_up(incl, off, 64);
+                if ((int)lane >= off) incl += tt;
+            }
+            const uint32_t total = uni(__shfl(incl, 63, 64));
+            if (COUNT) {
+                n_pairs += total;
+                continue;
+            }
+            S.row_off[lane] = incl - rc;
+            S.row_start[lane] = rs;
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t vbase = 0; vbase < total; vbase += 64) {
+                const uint32_t t = vbase + lane;
+                float4 p = make_float4(0.f, 0.f, 0.f, 0.f), w = p;
+                if (t < total) {
+                    // last row whose offset is <= t
+                    int lo_i = 0, hi_i = 63;
+                    while (lo_i < hi_i) {
+                        const int mid = (lo_i + hi_i + 1) >> 1;
+                        if (S.row_off[mid] <= t) lo_i = mid;
+                        else hi_i = mid - 1;
+                    }
+                    const uint32_t idx = S.row_start[lo_i] + (t - S.row_off[lo_i]);
+                    p = tv.sorted_p[idx];
+                    w = tv.sorted_w[idx];
+                }
+                __builtin_amdgcn_wave_barrier();
+                S.P[lane] = p;
+                S.W[lane] = w;
+                __builtin_amdgcn_wave_barrier();
+                const uint32_t cnt = total - vbase < 64u ? total - vbase : 64u;
+                for (uint32_t j = 0; j < cnt; ++j) {
+                    const float4 vp = S.P[j];
+                    if (vp.w != f.size) set_size(f, vp.w, spread, part, scaled);
+                    cluster_term<POLICY, 0>(part, q, vp, S.W[j], vp, vp, vp, f);
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        scaled.x += part.x * f.s;
+        scaled.y += part.y * f.s;
+        scaled.z += part.z * f.s;
+        kahan_add(sum, comp, scaled);
+    }
     if (COUNT) {
         // the ballot must run on every lane: inside the lane-0 branch only
         // lane 0 would vote
