@@ -45,15 +45,26 @@ def main():
     ap.add_argument("--vortons", type=int, default=1 << 20)
     ap.add_argument("--tracers", type=int, default=1 << 22)
     ap.add_argument("--policy", type=int, default=1, help="0 reference rsqrt, 1 accurate")
-    ap.add_argument("--near-k", type=int, default=2,
+    ap.add_argument("--near-k", type=int, default=3,
                     help="near-field radius of the accurate treecode; -1 = reference opening rule")
     ap.add_argument("--order", type=int, default=1, help="cluster order: 0 monopole, 1 dipole")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--suite", action="store_true",
+                    help="per-kernel records of the direct, treecode+tracer and multigrid "
+                         "workloads instead of the frame line")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="one 256 MiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.suite:
+        suite(args)
+        return
+    if args.copy_probe:
+        copy_probe()
+        return
 
     from vortegrid_amd import scenarios, vg
 
@@ -260,6 +271,180 @@ def main():
         "kernels_ms": {k: {"count": c, "total_ms": m} for k, (c, m) in prof.items()},
     }
     print(json.dumps(out))
+
+
+def copy_probe():
+    """A plain copy of known size: 256 MiB read + 256 MiB written.  Profiled
+    with FETCH_SIZE / WRITE_SIZE it gives the factor between those counters
+    and bytes moved to and from HBM."""
+    import torch
+
+    n = 64 << 20
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_each_way": 4 * n}))
+
+
+def _timed(vg, name, fn, reps):
+    """Run fn reps times under the HIP-event profiler; average ms of kernel name."""
+    fn()
+    vg.synchronize()
+    vg.profile_reset()
+    vg.profile_enable(True)
+    for _ in range(reps):
+        fn()
+    vg.synchronize()
+    vg.profile_enable(False)
+    prof = vg.profile_dump()
+    cnt, ms = prof.get(name, (0, 0.0))
+    return ms / max(cnt, 1), prof
+
+
+def suite(args):
+    """One JSON line per workload: the direct sum (4096-vorton ring and a
+    65,536-vorton pair of rings), the monopole treecode + 1M-tracer frame on
+    65^3, and the vector-Poisson solve on 129^3 with per-kernel achieved
+    FLOP/s or GB/s (algorithmic bytes: each input read once, each output
+    written once)."""
+    from vortegrid_amd import scenarios, vg
+
+    vg.require_gpu()
+    recs = []
+
+    def gbs(nbytes, ms):
+        return nbytes / (ms * 1e-3) / 1e9
+
+    # ---- direct Biot-Savart (K1): 26 flop per pair
+    for label, (pos, om, radius), grid_pts in (
+            ("direct ring N=4096 (vortons + 33^3 grid)", scenarios.vortex_ring(4096, seed=1), True),
+            ("direct rings N=65536 (at vortons)", scenarios.colliding_rings(65536, seed=2), False)):
+        n = len(pos)
+        dp, dw = vg.DeviceArray.from_numpy(pos), vg.DeviceArray.from_numpy(om)
+        v = vg.vortons_view(dp, dw, n, uniform_size=2 * radius)
+        out = vg.DeviceArray.zeros(pos.shape)
+        ms, _ = _timed(vg, "vg_direct_velocity",
+                       lambda: vg.direct_velocity(v, dp, n, out, policy=args.policy), 10)
+        pairs = float(n) * n
+        rec = {"workload": label, "kernel": "vg_direct_velocity", "avg_launch_ms": ms,
+               "pairs": pairs, "pairs_per_s": pairs / (ms * 1e-3),
+               "achieved_TFLOPs": FLOP_PER_PAIR * pairs / (ms * 1e-3) / 1e12}
+        rec["frac_fp32_peak"] = rec["achieved_TFLOPs"] / FP32_PEAK_TFLOPS
+        if grid_pts:
+            mn, mx = vg.find_bounding_box(dp, n)
+            g = vg.finalize_bounding_box(mn, mx, radius, n)
+            g = vg.define_shape(32 ** 3, g.min, [g.min[a] + g.extent[a] for a in range(3)])
+            og = vg.DeviceArray.zeros((g.capacity, 3))
+            gms, _ = _timed(vg, "vg_direct_velocity_grid",
+                            lambda: vg.direct_velocity_grid(v, g, og, policy=args.policy), 10)
+            gp = float(n) * g.capacity
+            rec.update({"grid": list(g.shape), "grid_launch_ms": gms,
+                        "grid_TFLOPs": FLOP_PER_PAIR * gp / (gms * 1e-3) / 1e12})
+        recs.append(rec)
+
+    # ---- monopole treecode frame (reference opening rule) + 1M tracers, 65^3
+    pos, om, radius = scenarios.colliding_rings(65536, seed=2)
+    mn, mx = pos.min(0), pos.max(0)
+    tr = scenarios.tracers_in_box(1 << 20, mn, mx, seed=3)
+    dp, dw = vg.DeviceArray.from_numpy(pos), vg.DeviceArray.from_numpy(om)
+    dvel, dtr = vg.DeviceArray.zeros(pos.shape), vg.DeviceArray.from_numpy(tr)
+    v = vg.vortons_view(dp, dw, len(pos), uniform_size=2 * radius)
+    sim = vg.Sim(vg.sim_config(technique=vg.TECH_TREE, rsqrt_policy=args.policy))
+    sim.set_vortons(v)
+    state = {"g": None, "f": None}
+    dt = 1.0 / 30.0
+
+    def frame():
+        if state["g"] is not None:
+            vg.advect_tracers(state["g"], state["f"], dtr, None, len(tr), dt, 1.0)
+        tmn, tmx = vg.find_bounding_box(dtr, len(tr))
+        sim.find_bounding_box()
+        sim.update_bounding_box(tmn, tmx, True)
+        sim.update(dt, 0, dvel)
+        vg.evolve(dp, dvel, len(pos), dt)
+        state["g"], state["f"] = sim.velocity_grid()
+
+    for _ in range(3):
+        frame()
+    vg.synchronize()
+    vg.profile_reset()
+    vg.profile_enable(True)
+    t0 = time.perf_counter()
+    steps = 100
+    for _ in range(steps):
+        frame()
+    vg.synchronize()
+    el = time.perf_counter() - t0
+    vg.profile_enable(False)
+    prof = vg.profile_dump()
+    g = state["g"]
+    c, ms = prof.get("vg_advect_tracers", (1, 0.0))
+    recs.append({"workload": "treecode frame, 2 colliding rings N=65536 + 1M tracers, 100 steps",
+                 "grid": list(g.shape), "ms_per_step": el * 1e3 / steps,
+                 "evals_per_s": g.capacity * steps / el,
+                 "tree_grid_avg_ms": prof.get("vg_tree_velocity_grid", (1, 0.0))[1] / steps,
+                 "advect_avg_ms": ms / max(c, 1),
+                 "advect_GBs": gbs(24.0 * len(tr), ms / max(c, 1)),
+                 "kernels_ms": {k: {"count": c, "total_ms": m} for k, (c, m) in prof.items()}})
+
+    # ---- vector Poisson on 129^3: plume vorticity, treecode faces, V-cycles
+    pos, om, radius = scenarios.jet_plume(262144, seed=3)
+    n = len(pos)
+    dp, dw = vg.DeviceArray.from_numpy(pos), vg.DeviceArray.from_numpy(om)
+    v = vg.vortons_view(dp, dw, n, uniform_size=2 * radius)
+    mn, mx = vg.find_bounding_box(dp, n)
+    g0 = vg.finalize_bounding_box(mn, mx, radius, n)
+    g = vg.define_shape(128 ** 3, g0.min, [g0.min[a] + g0.extent[a] for a in range(3)])
+    t = vg.Tree()
+    t.build(g, v)
+    cap = g.capacity
+    negw, a0 = vg.DeviceArray.zeros((cap, 3)), vg.DeviceArray.zeros((cap, 3))
+    ms_splat, _ = _timed(vg, "vg_splat_vorticity", lambda: t.splat_vorticity(radius, -1.0, negw), 5)
+    ms_faces, _ = _timed(vg, "vg_tree_vecpot_faces", lambda: t.vecpot_boundary(g, a0), 5)
+    a = vg.DeviceArray.zeros((cap, 3))
+    a.upload(a0.numpy())
+    mg = vg.Multigrid()
+    vg.synchronize()
+    vg.profile_reset()
+    vg.profile_enable(True)
+    t0 = time.perf_counter()
+    rel, cycles = mg.vcycle_solve(g, a, negw, tol=1e-6, max_cycles=40)
+    vg.synchronize()
+    solve_ms = (time.perf_counter() - t0) * 1e3
+    vg.profile_enable(False)
+    mgprof = vg.profile_dump()
+    vel = vg.DeviceArray.zeros((cap, 3))
+    ms_curl, _ = _timed(vg, "vg_curl", lambda: vg.curl(g, a, vel), 10)
+    ms_res, _ = _timed(vg, "vg_residual_norm", lambda: vg.residual_norm(g, a, negw), 10)
+    soln = vg.DeviceArray.zeros((cap, 3))
+    ms_sor, _ = _timed(vg, "vg_sor_rows", lambda: vg.sor_solve(g, soln, negw, 1), 10)
+    lo = vg.GridDesc()
+    vg.check(vg.lib().vg_decimate(vg.C.byref(lo), vg.C.byref(g), 2), "decimate")
+    flo = vg.DeviceArray.zeros((lo.capacity, 3))
+    ms_inj, _ = _timed(vg, "vg_restrict_inject", lambda: vg.downsample_inject(lo, flo, g, a), 10)
+    ms_pro, _ = _timed(vg, "vg_prolong_interior", lambda: vg.upsample_interior(g, soln, lo, flo), 10)
+    interior = (g.np[0] - 2) * (g.np[1] - 2) * (g.np[2] - 2)
+    # the sor_solve launch count per step: one launch per colour
+    recs.append({
+        "workload": "vector Poisson 129^3, jet plume N=262144, treecode Dirichlet faces, "
+                    "V-cycles to 1e-6 (FP64 coarsest level and norms)",
+        "grid": list(g.shape), "vcycle_solve_ms": solve_ms, "cycles": cycles,
+        "final_rel_residual": rel,
+        "kernels": {
+            "vg_splat_vorticity": {"ms": ms_splat, "GBs": gbs(24.0 * n + 12.0 * cap, ms_splat)},
+            "vg_tree_vecpot_faces": {"ms": ms_faces},
+            "vg_curl": {"ms": ms_curl, "GBs": gbs(24.0 * cap, ms_curl)},
+            "vg_residual_norm": {"ms": ms_res, "GBs": gbs(24.0 * cap, ms_res)},
+            "vg_sor_rows (one colour)": {"ms": ms_sor, "GBs": gbs(18.0 * interior, ms_sor)},
+            "vg_restrict_inject": {"ms": ms_inj, "GBs": gbs(24.0 * lo.capacity, ms_inj)},
+            "vg_prolong_interior": {"ms": ms_pro, "GBs": gbs(12.0 * interior, ms_pro)},
+        },
+        "vcycle_kernels_ms": {k: {"count": c, "total_ms": m} for k, (c, m) in mgprof.items()},
+    })
+    for r in recs:
+        print(json.dumps(r), flush=True)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,246 @@
+"""Learned far-field correction on the GPU: MFMA forward / backward against
+the CPU oracle, training pairs against direct sums over the cluster members,
+the fused grid kernel against the oracle walk, and the gain of a trained
+network over the monopole treecode it corrects."""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as M
+from vortegrid_amd import scenarios
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+
+
+def case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_forward_matches_oracle(vg):
+    case("mlp forward (MFMA GEMM path) n=3001")
+    rng = np.random.default_rng(1)
+    p = (M.init_params(11) + 0.05 * rng.standard_normal(M.N_PARAMS)).astype(F32)
+    X = rng.standard_normal((3001, M.N_IN)).astype(F32)
+    m = vg.Mlp(seed=3)
+    m.set_params(p)
+    assert np.array_equal(m.params(), p)
+    dx = vg.DeviceArray.from_numpy(X)
+    dy = vg.DeviceArray.zeros((len(X), 3))
+    m.forward(dx, len(X), dy)
+    vg.synchronize()
+    got = dy.numpy()
+    want_bf, _ = M.forward(p, X, emulate_bf16=True)
+    want, _ = M.forward(p, X)
+    print(f"rel vs bf16-emulated {rel(got, want_bf):.2e}, vs float64 {rel(got, want):.2e}")
+    assert rel(got, want_bf) < 3e-3
+    assert rel(got, want) < 3e-2
+
+
+def test_gradients_match_oracle(vg):
+    case("mlp gradients n=2500")
+    rng = np.random.default_rng(2)
+    p = (M.init_params(12) + 0.05 * rng.standard_normal(M.N_PARAMS)).astype(F32)
+    X = rng.standard_normal((2500, M.N_IN)).astype(F32)
+    T = rng.standard_normal((2500, 3)).astype(F32)
+    m = vg.Mlp()
+    m.set_params(p)
+    loss, g = m.gradients(vg.DeviceArray.from_numpy(X), vg.DeviceArray.from_numpy(T), len(X))
+    lw, gw = M.backward(p, X, T, emulate_bf16=True)
+    l64, g64 = M.backward(p, X, T)
+    assert abs(loss - lw) < 1e-3 * lw
+    o = 0
+    for name, shp in M.SHAPES:
+        n = int(np.prod(shp))
+        e_bf, e64 = rel(g[o:o + n], gw[o:o + n]), rel(g[o:o + n], g64[o:o + n])
+        print(f"{name}: rel vs bf16-emulated {e_bf:.2e}, vs float64 {e64:.2e}")
+        assert e_bf < 1e-2, name
+        assert e64 < 5e-2, name
+        o += n
+
+
+def test_training_fits_a_teacher(vg):
+    case("mlp training, teacher network, 400 Adam steps")
+    rng = np.random.default_rng(3)
+    teacher = M.init_params(21)
+    X = rng.standard_normal((16384, M.N_IN)).astype(F32)
+    T, _ = M.forward(teacher, X)
+    T = T.astype(F32)
+    m = vg.Mlp(seed=5)
+    dx, dt = vg.DeviceArray.from_numpy(X), vg.DeviceArray.from_numpy(T)
+    l0 = M.loss(m.params(), X, T)
+    last = m.train(dx, dt, len(X), batch=2048, steps=400, lr=2e-3)
+    l1 = M.loss(m.params(), X, T)
+    print(f"loss {l0:.3e} -> {l1:.3e} (last minibatch {last:.3e})")
+    assert l1 < 0.1 * l0
+
+
+@pytest.fixture(scope="module")
+def ring_tree(vg, oracle):
+    pos, om, radius = scenarios.colliding_rings(8192, seed=41)
+    og = oracle.grid_for(pos, radius)
+    ot = oracle.InfluenceTree(og, pos, om, 2 * radius)
+    dp, dw = vg.DeviceArray.from_numpy(pos), vg.DeviceArray.from_numpy(om)
+    v = vg.vortons_view(dp, dw, len(pos), uniform_size=2 * radius)
+    g = vg.GridDesc.from_fields(og)
+    t = vg.Tree()
+    t.build(g, v)
+    spread, circ = oracle.auto_mollify(og, radius)
+    return dict(pos=pos, om=om, radius=radius, og=og, ot=ot, g=g, t=t, keep=(dp, dw, v),
+                spread=spread, circ=circ)
+
+
+def _cluster_arrays(ot, pos, om):
+    S = M.weight_sums(ot, pos, om)
+    out = []
+    for l in range(ot.depth):
+        _, P, W, size = ot.layer(l)
+        D = ot.dipoles(l).reshape(-1, 9)
+        Pa = np.concatenate([P, size[:, None]], 1)
+        Wa = np.concatenate([W, S[l][:, None]], 1)
+        Ma = np.zeros((len(P), 3, 4))
+        Ma[:, :, :3] = D.reshape(-1, 3, 3)
+        out.append((Pa, Wa, Ma.reshape(-1, 12)))
+    return out
+
+
+def test_samples_match_member_direct_sums(vg, oracle, ring_tree):
+    case("mlp training pairs vs direct sums over cluster members")
+    r = ring_tree
+    x, t, meta, failed = r["t"].mlp_samples(r["g"], 256, near_k=1, spread=r["spread"],
+                                            circ=r["circ"], seed=9)
+    vg.synchronize()
+    assert failed == 0
+    X, T, meta = x.numpy(), t.numpy(), meta.numpy()
+    ot, og = r["ot"], r["og"]
+    cl = _cluster_arrays(ot, r["pos"], r["om"])
+    h = M.layer_h(ot)
+    q_all = oracle.query_points(og).reshape(-1, 3)
+    layers_seen = set()
+    for i in range(len(X)):
+        l, off = int(meta[i, 1] >> 28), int(meta[i, 1] & ((1 << 28) - 1))
+        layers_seen.add(l)
+        q = q_all[meta[i, 0]]
+        P, W, Mm = (a[off:off + 1] for a in cl[l])
+        f = M.features(q[None], P, W, Mm, h[l])[0]
+        assert np.allclose(X[i], f, rtol=2e-5, atol=2e-5 * np.abs(f).max()), i
+        mem = M.cluster_members(ot, l, off)
+        exact = oracle.direct_velocity_f64(r["pos"][mem], r["om"][mem], 2 * r["radius"], q[None],
+                                           r["spread"], r["circ"])[0]
+        mono = oracle.direct_velocity_f64(P[:, :3], W[:, :3], 2 * r["radius"], q[None], r["spread"],
+                                          r["circ"])[0]
+        sc = M.correction_scale(P, W, h[l])[0]
+        want = (exact - mono) / sc
+        tol = 1e-4 * np.linalg.norm(exact) / sc + 1e-6
+        assert np.all(np.abs(T[i] - want) <= tol), (i, l, T[i], want)
+    assert len(layers_seen) == ot.depth - 1
+
+
+def test_mlp_grid_matches_oracle(vg, oracle, ring_tree):
+    case("fused MLP grid kernel vs oracle walk + oracle network, k=1")
+    r = ring_tree
+    rng = np.random.default_rng(4)
+    p = (M.init_params(31) + 0.02 * rng.standard_normal(M.N_PARAMS)).astype(F32)
+    m = vg.Mlp()
+    m.set_params(p)
+    g = r["g"]
+    base = vg.DeviceArray.zeros((g.capacity, 3))
+    corr = vg.DeviceArray.zeros((g.capacity, 3))
+    r["t"].velocity_grid_mac(g, base, 1, 0, r["spread"], r["circ"], policy=1)
+    r["t"].velocity_grid_mlp(g, m, corr, near_k=1, spread=r["spread"], circ=r["circ"])
+    vg.synchronize()
+    got = corr.numpy() - base.numpy()
+    ot, og = r["ot"], r["og"]
+    cl = _cluster_arrays(ot, r["pos"], r["om"])
+    h = M.layer_h(ot)
+    q_all = oracle.query_points(og).reshape(-1, 3)
+    pts = [[int(rng.integers(0, og.np[a])) for a in range(3)] for _ in range(40)]
+    pts += [[og.np[0] - 1, og.np[1] - 1, og.np[2] - 1], [0, 0, 0]]
+    want, have = [], []
+    for pt in pts:
+        o = pt[0] + og.np[0] * (pt[1] + og.np[1] * pt[2])
+        acc, _ = M.accepted_clusters(ot, og, pt, 1)
+        u = np.zeros(3)
+        by_layer = {}
+        for l, off in acc:
+            by_layer.setdefault(l, []).append(off)
+        for l, offs in by_layer.items():
+            P, W, Mm = (a[offs] for a in cl[l])
+            keep = W[:, 3] > 1e-30
+            P, W, Mm = P[keep], W[keep], Mm[keep]
+            F = M.features(np.repeat(q_all[o][None], len(P), 0), P, W, Mm, h[l])
+            y, _ = M.forward(p, F, emulate_bf16=True)
+            u += (y * M.correction_scale(P, W, h[l])[:, None]).sum(0)
+        want.append(u)
+        have.append(got[o])
+    e = rel(have, want)
+    print(f"rel-L2 of the correction at {len(pts)} points: {e:.2e}")
+    assert e < 1e-2
+
+
+def test_trained_correction_beats_monopole(vg, oracle):
+    """Train on direct-solver pairs from ring, colliding-ring and plume trees;
+    judge on a held-out random-rings field against the FP64 direct sum."""
+    case("mlp gain: train on 3 scenarios, evaluate on held-out random rings N=65536")
+    xs, ts = [], []
+    keep = []
+    for k, (pos, om, radius) in enumerate((scenarios.vortex_ring(16384, seed=51),
+                                          scenarios.colliding_rings(32768, seed=52),
+                                          scenarios.jet_plume(32768, seed=53))):
+        dp, dw = vg.DeviceArray.from_numpy(pos), vg.DeviceArray.from_numpy(om)
+        v = vg.vortons_view(dp, dw, len(pos), uniform_size=2 * radius)
+        mn, mx = vg.find_bounding_box(dp, len(pos))
+        g = vg.finalize_bounding_box(mn, mx, radius, len(pos))
+        t = vg.Tree()
+        t.build(g, v)
+        spread, circ = vg.auto_mollify(g, radius)
+        x, tt, _, failed = t.mlp_samples(g, 65536, near_k=1, spread=spread, circ=circ, seed=60 + k)
+        assert failed == 0
+        xs.append(x.numpy())
+        ts.append(tt.numpy())
+        keep.append((dp, dw, t))
+    X = np.concatenate(xs)
+    T = np.concatenate(ts)
+    perm = np.random.default_rng(0).permutation(len(X))
+    X, T = X[perm], T[perm]
+    m = vg.Mlp(seed=7)
+    dx, dt = vg.DeviceArray.from_numpy(X), vg.DeviceArray.from_numpy(T)
+    m.train(dx, dt, len(X), batch=4096, steps=1500, lr=2e-3)
+    m.train(dx, dt, len(X), batch=4096, steps=1500, lr=5e-4)
+    # held-out field
+    pos, om, radius = scenarios.random_rings(65536, seed=54)
+    dp, dw = vg.DeviceArray.from_numpy(pos), vg.DeviceArray.from_numpy(om)
+    v = vg.vortons_view(dp, dw, len(pos), uniform_size=2 * radius)
+    mn, mx = vg.find_bounding_box(dp, len(pos))
+    g = vg.finalize_bounding_box(mn, mx, radius, len(pos))
+    t = vg.Tree()
+    t.build(g, v)
+    spread, circ = vg.auto_mollify(g, radius)
+    cap = g.capacity
+    rng = np.random.default_rng(8)
+    sel = np.sort(rng.choice(cap, 4000, replace=False))
+    nudge = F32(1) - F32(2) * F32(np.finfo(F32).eps)
+    idx = (sel % g.np[0], (sel // g.np[0]) % g.np[1], sel // (g.np[0] * g.np[1]))
+    q = np.stack([F32(g.min[a]) + np.asarray(i, F32) * (F32(g.cell[a]) * nudge)
+                  for a, i in enumerate(idx)], 1).astype(F32)
+    ref = vg.DeviceArray.zeros((len(q), 3), np.float64)
+    vg.direct_velocity_f64(v, vg.DeviceArray.from_numpy(q), len(q), ref, spread, circ)
+    mono = vg.DeviceArray.zeros((cap, 3))
+    dip = vg.DeviceArray.zeros((cap, 3))
+    learned = vg.DeviceArray.zeros((cap, 3))
+    t.velocity_grid_mac(g, mono, 1, 0, spread, circ, policy=1)
+    t.velocity_grid_mac(g, dip, 1, 1, spread, circ, policy=1)
+    t.velocity_grid_mlp(g, m, learned, near_k=1, spread=spread, circ=circ)
+    vg.synchronize()
+    R = ref.numpy()
+    e_mono = rel(mono.numpy()[sel], R)
+    e_dip = rel(dip.numpy()[sel], R)
+    e_mlp = rel(learned.numpy()[sel], R)
+    print(f"k=1 rel-L2 vs FP64 direct: monopole {e_mono:.3e}, monopole+MLP {e_mlp:.3e}, "
+          f"analytic dipole {e_dip:.3e}")
+    assert e_mlp < 0.7 * e_mono

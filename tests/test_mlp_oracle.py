@@ -1,0 +1,139 @@
+"""The learned far-field correction's CPU oracle, pinned against what the
+mathematics fixes: finite differences, a scalar restatement, the erf GELU,
+torch's bfloat16 rounding, the monopole law and the C oracle's tree walk."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as M
+from vortegrid_amd import scenarios
+
+
+def test_param_layout():
+    assert M.N_PARAMS == 9603
+    p = M.init_params(3)
+    assert np.array_equal(M.pack(M.unpack(p)), p)
+
+
+def test_backward_matches_finite_differences():
+    rng = np.random.default_rng(0)
+    p = M.init_params(1)
+    p += 0.05 * rng.standard_normal(p.size)  # non-zero biases too
+    X = rng.standard_normal((40, M.N_IN))
+    T = rng.standard_normal((40, M.N_OUT))
+    _, g = M.backward(p, X, T)
+    idx = rng.choice(p.size, 120, replace=False)
+    num = M.numeric_gradient(p, X, T, idx)
+    assert np.allclose(g[idx], num, rtol=1e-5, atol=1e-9)
+
+
+def test_forward_matches_scalar_loop():
+    rng = np.random.default_rng(2)
+    p = M.init_params(4) + 0.02 * rng.standard_normal(M.N_PARAMS)
+    X = rng.standard_normal((3, M.N_IN))
+    Y, _ = M.forward(p, X)
+    P = M.unpack(p)
+
+    def g(z):  # erf-free tanh GELU, scalar
+        return 0.5 * z * (1 + math.tanh(math.sqrt(2 / math.pi) * (z + 0.044715 * z ** 3)))
+
+    for r in range(3):
+        h = list(X[r])
+        for W, b in ((P["W1"], P["b1"]), (P["W2"], P["b2"]), (P["W3"], P["b3"])):
+            h = [g(sum(h[i] * W[i][j] for i in range(len(h))) + b[j]) for j in range(W.shape[1])]
+        y = [sum(h[i] * P["W4"][i][j] for i in range(64)) + P["b4"][j] for j in range(3)]
+        assert np.allclose(Y[r], y, rtol=1e-12, atol=1e-12)
+
+
+def test_gelu_against_erf_form():
+    z = np.linspace(-6, 6, 241)
+    exact = np.array([0.5 * v * (1 + math.erf(v / math.sqrt(2))) for v in z])
+    assert np.max(np.abs(M.gelu(z) - exact)) < 1e-3
+    assert M.gelu(np.array([0.0]))[0] == 0.0
+    # derivative by central differences
+    h = 1e-6
+    assert np.allclose(M.gelu_grad(z), (M.gelu(z + h) - M.gelu(z - h)) / (2 * h), atol=1e-7)
+
+
+def test_bf16_rounding_matches_torch():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(5)
+    a = np.concatenate([rng.standard_normal(10000).astype(np.float32) * 10.0 ** rng.integers(-8, 8, 10000),
+                        np.float32([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -9, -1.0 - 2 ** -8, 0.0])])
+    want = torch.tensor(a).to(torch.bfloat16).to(torch.float64).numpy()
+    assert np.array_equal(M.bf16(a), want)
+
+
+def test_features_normalise_the_monopole_law():
+    """A single vorton far from its cluster's query: the monopole velocity
+    equals scale * (Omega/S x x) |x|^-3 in feature units."""
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(7)
+    for _ in range(20):
+        c = rng.uniform(-1, 1, 3).astype(np.float32)
+        om = rng.standard_normal(3).astype(np.float32)
+        q = (c + rng.uniform(2, 4) * M.bf16(rng.standard_normal(3))).astype(np.float32)
+        size, h = 0.1, 0.3
+        S = 2 * np.linalg.norm(om.astype(np.float64))
+        P = np.array([[*c, size]])
+        W = np.array([[*om, S]])
+        f = M.features(q[None], P, W, np.zeros((1, 12)), h)[0]
+        assert np.isclose(f[3], h / np.linalg.norm(q.astype(np.float64) - c))
+        u = M.correction_scale(P, W, h)[0] * np.cross(f[4:7], f[:3]) * f[3] ** 3
+        want = O.direct_velocity(c[None], om[None], size, q[None], policy=1)[0]
+        assert np.allclose(u, want, rtol=1e-5)
+
+
+@pytest.fixture(scope="module")
+def small_tree(oracle):
+    pos, om, radius = scenarios.colliding_rings(4096, seed=31)
+    g = oracle.grid_for(pos, radius)
+    ot = oracle.InfluenceTree(g, pos, om, 2 * radius)
+    return pos, om, radius, g, ot
+
+
+def test_cluster_members_partition_the_vortons(small_tree):
+    pos, om, radius, g, ot = small_tree
+    for l in range(ot.depth):
+        gl = ot.layer(l)[0]
+        seen = []
+        for off in range(gl.capacity):
+            seen.extend(M.cluster_members(ot, l, off).tolist())
+        assert sorted(seen) == list(range(len(pos)))
+
+
+@pytest.mark.parametrize("near_k", [0, 1, 2])
+def test_accepted_clusters_restate_the_walk(oracle, small_tree, near_k):
+    """Monopoles of the enumerated clusters plus the direct sum over the
+    leaf near box reproduce the C oracle's near-field treecode."""
+    pos, om, radius, g, ot = small_tree
+    spread, circ = oracle.auto_mollify(g, radius)
+    want = oracle.mac_velocity_grid(ot, g, pos, om, 2 * radius, near_k, 0, spread, circ, 1,
+                                    threads=8).reshape(-1, 3)
+    start, count, order = M.cell_lists(ot)
+    rng = np.random.default_rng(near_k)
+    g0 = ot.layer(0)[0]
+    for _ in range(12):
+        pt = [int(rng.integers(0, g.np[a])) for a in range(3)]
+        q = oracle.query_points(g).reshape(-1, 3)[pt[0] + g.np[0] * (pt[1] + g.np[1] * pt[2])]
+        acc, (lo, hi) = M.accepted_clusters(ot, g, pt, near_k)
+        srcs_p, srcs_w = [], []
+        for l, off in acc:
+            _, P, W, _ = ot.layer(l)
+            srcs_p.append(P[off])
+            srcs_w.append(W[off])
+        near = []
+        for z in range(lo[2], hi[2] + 1):
+            for y in range(lo[1], hi[1] + 1):
+                for x in range(lo[0], hi[0] + 1):
+                    k = x + g0.np[0] * (y + g0.np[1] * z)
+                    near.extend(order[start[k]:start[k] + count[k]].tolist())
+        u = oracle.direct_velocity(np.array(srcs_p), np.array(srcs_w), 2 * radius, q[None],
+                                   spread, circ, policy=1)[0] if srcs_p else 0.0
+        if near:
+            u = u + oracle.direct_velocity(pos[near], om[near], 2 * radius, q[None], spread, circ,
+                                           policy=1)[0]
+        o = pt[0] + g.np[0] * (pt[1] + g.np[1] * pt[2])
+        assert np.allclose(u, want[o], rtol=1e-4, atol=1e-6 * np.abs(want).max())

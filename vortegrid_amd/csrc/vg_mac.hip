@@ -334,6 +334,11 @@ vg_status run_mac(const vg_tree *t, const vg_grid_desc *g, int near_k, int order
 }
 
 }  // namespace
+
+vg_status mac_velocity_grid(const vg_tree *t, const vg_grid_desc *g, int near_k, int order,
+                            float spread, float circ, int policy, float *out, hipStream_t s) {
+    return run_mac<0>(t, g, near_k, order, spread, circ, policy, out, nullptr, s);
+}
 }  // namespace vg
 
 using namespace vg;

@@ -35,8 +35,12 @@ EXPORTS = (
     "vg_advect_tracers vg_sim_config_default vg_sim_create vg_sim_destroy vg_sim_set_vortons "
     "vg_sim_find_bounding_box vg_sim_update_bounding_box vg_sim_update vg_sim_velocity_grid "
     "vg_sim_vector_potential vg_sim_negative_vorticity vg_sim_grid_template vg_sim_bounds "
-    "vg_sim_tree vg_sim_spreading vg_sim_poisson_residual"
+    "vg_sim_tree vg_sim_spreading vg_sim_poisson_residual "
+    "vg_mlp_create vg_mlp_destroy vg_mlp_set_params vg_mlp_get_params vg_mlp_forward "
+    "vg_mlp_gradients vg_mlp_train vg_tree_mlp_samples vg_tree_velocity_grid_mlp"
 ).split()
+
+MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS, MLP_PARAMS = 16, 64, 3, 9603
 
 
 class VGError(RuntimeError):
@@ -350,6 +354,26 @@ class Tree:
              _s(stream))
         return c.value, p.value
 
+    def mlp_samples(self, g: GridDesc, n: int, near_k=1, layer_mask=0xFFFF, spread=1.0,
+                    circ=1.0, seed=1, stream=None):
+        """(x [n,16], target [n,3], meta [n,2] uint32) device arrays of
+        training pairs and the number of pairs left empty."""
+        x = DeviceArray.zeros((max(n, 1), MLP_INPUTS))
+        t = DeviceArray.zeros((max(n, 1), MLP_OUTPUTS))
+        meta = DeviceArray.zeros((max(n, 1), 2), np.uint32)
+        failed = C.c_uint32()
+        call("vg_tree_mlp_samples", self.h, C.byref(g), int(near_k), C.c_uint32(layer_mask),
+             C.c_float(spread), C.c_float(circ), C.c_uint32(n), C.c_uint64(seed), x.ptr, t.ptr,
+             meta.ptr, C.byref(failed), _s(stream))
+        return x, t, meta, failed.value
+
+    def velocity_grid_mlp(self, g: GridDesc, mlp: "Mlp", out: DeviceArray, near_k=1,
+                          layer_mask=0xFFFF, spread=1.0, circ=1.0, policy=RSQRT_ACCURATE,
+                          stream=None):
+        call("vg_tree_velocity_grid_mlp", self.h, C.byref(g), mlp.h, int(near_k),
+             C.c_uint32(layer_mask), C.c_float(spread), C.c_float(circ), policy, out.ptr,
+             _s(stream))
+
     def interaction_count(self, g: GridDesc, stream=None) -> int:
         c = C.c_uint64()
         call("vg_tree_interaction_count", self.h, C.byref(g), C.byref(c), _s(stream))
@@ -431,6 +455,48 @@ class Multigrid:
         try:
             if self.h:
                 lib().vg_mg_destroy(self.h)
+        except Exception:
+            pass
+
+
+class Mlp:
+    """Learned far-field correction network (vg_mlp)."""
+
+    def __init__(self, seed: int = 1):
+        self.h = C.c_void_p()
+        call("vg_mlp_create", C.byref(self.h), C.c_uint64(seed))
+
+    def set_params(self, params: np.ndarray):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == MLP_PARAMS
+        call("vg_mlp_set_params", self.h, p.ctypes.data_as(C.c_void_p))
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(MLP_PARAMS, np.float32)
+        call("vg_mlp_get_params", self.h, p.ctypes.data_as(C.c_void_p))
+        return p
+
+    def forward(self, x: DeviceArray, n: int, y: DeviceArray, stream=None):
+        call("vg_mlp_forward", self.h, x.ptr, C.c_uint32(n), y.ptr, _s(stream))
+
+    def gradients(self, x: DeviceArray, t: DeviceArray, n: int, stream=None):
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        call("vg_mlp_gradients", self.h, x.ptr, t.ptr, C.c_uint32(n),
+             g.ctypes.data_as(C.c_void_p), C.byref(loss), _s(stream))
+        return loss.value, g
+
+    def train(self, x: DeviceArray, t: DeviceArray, n: int, batch=4096, steps=1000, lr=1e-3,
+              stream=None) -> float:
+        loss = C.c_float()
+        call("vg_mlp_train", self.h, x.ptr, t.ptr, C.c_uint32(n), C.c_uint32(batch),
+             C.c_uint32(steps), C.c_float(lr), C.byref(loss), _s(stream))
+        return loss.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().vg_mlp_destroy(self.h)
         except Exception:
             pass
 
