@@ -31,6 +31,8 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FLOP_PER_PAIR = 26.0   # monopole / direct pair (FMA = 2, rsqrt = 1)
 FLOP_DIPOLE = 42.0     # first-moment correction of one cluster interaction
+BF16_PEAK_TFLOPS = 2500.0
+MLP_FLOP = 2.0 * (16 * 64 + 64 * 64 + 64 * 64 + 64 * 3)  # per (point, cluster) pair
 
 
 def log(msg):
@@ -53,6 +55,9 @@ def main():
     ap.add_argument("--suite", action="store_true",
                     help="per-kernel records of the direct, treecode+tracer and multigrid "
                          "workloads instead of the frame line")
+    ap.add_argument("--mlp", action="store_true",
+                    help="learned far-field correction on the monopole near-field treecode "
+                         "(trained on the GPU before the timed frames)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="one 256 MiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -83,6 +88,16 @@ def main():
     sim = vg.Sim(vg.sim_config(technique=vg.TECH_TREE, rsqrt_policy=args.policy,
                                tree_near_k=args.near_k, tree_order=args.order), stream)
     sim.set_vortons(v)
+    mlp_info = None
+    if args.mlp:
+        if args.near_k < 0:
+            raise SystemExit("--mlp needs --near-k >= 0")
+        from vortegrid_amd import learned
+
+        log("training the far-field correction (untimed)")
+        mlp, mlp_info = learned.train_far_field(near_k=args.near_k)
+        log(f"trained: {mlp_info}")
+        sim.set_mlp(mlp)
     grid_holder = {"g": None, "field": None}
     evals = [0]
 
@@ -138,7 +153,14 @@ def main():
     # frame's tree (the count drifts by well under 1% frame to frame)
     t = vg.Tree()
     t.build(g, v, stream)
-    if args.near_k < 0:
+    peak, note = FP32_PEAK_TFLOPS, "FP32 VALU kernel (no MFMA); peak is the published FP32 vector peak"
+    if args.mlp:
+        kname = "vg_tree_velocity_grid_mlp"
+        clusters, _ = t.mac_work(g, args.near_k, stream)
+        pairs = 0
+        flop_cluster = MLP_FLOP
+        peak, note = BF16_PEAK_TFLOPS, "BF16 MFMA with FP32 accumulation; dense BF16 peak"
+    elif args.near_k < 0:
         kname = "vg_tree_velocity_grid"
         clusters, pairs = t.interaction_count(g, stream), 0
         flop_cluster = FLOP_PER_PAIR
@@ -150,12 +172,12 @@ def main():
     cnt, ms = prof.get(kname, (0, 0.0))
     avg_s = (ms / max(cnt, 1)) * 1e-3
     achieved = flops / avg_s / 1e12 if avg_s > 0 else 0.0
-    roofline = {"kernel": kname, "bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+    roofline = {"kernel": kname, "bound": "mfma", "achieved": achieved, "peak": peak,
+                "unit": "TFLOP/s", "frac": achieved / peak,
+                "traffic": measured_traffic(args),
                 "avg_launch_ms": ms / max(cnt, 1), "cluster_interactions_per_launch": clusters,
                 "direct_pairs_per_launch": pairs, "flop_per_cluster": flop_cluster,
-                "flop_per_pair": FLOP_PER_PAIR,
-                "note": "FP32 VALU kernel (no MFMA); peak is the published FP32 vector peak"}
+                "flop_per_pair": FLOP_PER_PAIR, "note": note}
     # tracer advection (HBM bound): 24 B per tracer (pos read + write)
     acnt, ams = prof.get("vg_advect_tracers", (0, 0.0))
     adv = None
@@ -257,6 +279,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": ("treecode frame (TREE technique, " + (
                        "reference opening rule" if args.near_k < 0 else
+                       f"near-field treecode near_k={args.near_k} monopole clusters + learned "
+                       f"correction" if args.mlp else
                        f"near-field treecode near_k={args.near_k} "
                        f"{'dipole' if args.order else 'monopole'} clusters") +
                        ", auto-mollified) + 4M-tracer advection"),
@@ -265,12 +289,30 @@ def main():
                    "near_k": args.near_k, "order": args.order,
                    "parallelism": "single GPU"},
         "rel_l2_vs_fp64_direct": rel,
+        "learned_correction": mlp_info,
         "roofline": roofline,
         "tracer_advection": adv,
         "cpu_baseline": cpu,
         "kernels_ms": {k: {"count": c, "total_ms": m} for k, (c, m) in prof.items()},
     }
     print(json.dumps(out))
+
+
+# HBM-side bytes per launch of the dominant kernel, from the two counter
+# runs (FETCH_SIZE, WRITE_SIZE) summarised under profiles/ for the default
+# workload; None for other settings.
+TRAFFIC_FILES = {(3, 1, 1 << 20): "tree_mac_k3_1M_traffic.txt"}
+
+
+def measured_traffic(args):
+    name = TRAFFIC_FILES.get((args.near_k, args.order, args.vortons))
+    path = os.path.join(ROOT, "profiles", name) if name else None
+    if not path or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def copy_probe():

@@ -357,6 +357,12 @@ typedef struct vg_sim_config {
 vg_status vg_sim_config_default(vg_sim_config *cfg);
 vg_status vg_sim_create(vg_sim **sim, const vg_sim_config *cfg, vg_stream s);
 vg_status vg_sim_destroy(vg_sim *sim);
+/* Learned far-field correction for the TREE technique (no reference
+ * counterpart): with tree_near_k >= 0 and m != NULL the velocity grid is
+ * vg_tree_velocity_grid_mlp(near_k, layer_mask) instead of the near-field
+ * treecode of order tree_order; m == NULL restores the plain treecode.  The
+ * network is not owned and must outlive its use. */
+vg_status vg_sim_set_mlp(vg_sim *sim, vg_mlp *m, uint32_t layer_mask);
 /* VortonSim::SetVortons (vortonSim.h:329): keeps the device view. */
 vg_status vg_sim_set_vortons(vg_sim *sim, const vg_vortons *v);
 /* VortonSim::FindBoundingBox (vortonSim.cpp:1001-1008). */

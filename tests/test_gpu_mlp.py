@@ -187,31 +187,10 @@ def test_trained_correction_beats_monopole(vg, oracle):
     """Train on direct-solver pairs from ring, colliding-ring and plume trees;
     judge on a held-out random-rings field against the FP64 direct sum."""
     case("mlp gain: train on 3 scenarios, evaluate on held-out random rings N=65536")
-    xs, ts = [], []
-    keep = []
-    for k, (pos, om, radius) in enumerate((scenarios.vortex_ring(16384, seed=51),
-                                          scenarios.colliding_rings(32768, seed=52),
-                                          scenarios.jet_plume(32768, seed=53))):
-        dp, dw = vg.DeviceArray.from_numpy(pos), vg.DeviceArray.from_numpy(om)
-        v = vg.vortons_view(dp, dw, len(pos), uniform_size=2 * radius)
-        mn, mx = vg.find_bounding_box(dp, len(pos))
-        g = vg.finalize_bounding_box(mn, mx, radius, len(pos))
-        t = vg.Tree()
-        t.build(g, v)
-        spread, circ = vg.auto_mollify(g, radius)
-        x, tt, _, failed = t.mlp_samples(g, 65536, near_k=1, spread=spread, circ=circ, seed=60 + k)
-        assert failed == 0
-        xs.append(x.numpy())
-        ts.append(tt.numpy())
-        keep.append((dp, dw, t))
-    X = np.concatenate(xs)
-    T = np.concatenate(ts)
-    perm = np.random.default_rng(0).permutation(len(X))
-    X, T = X[perm], T[perm]
-    m = vg.Mlp(seed=7)
-    dx, dt = vg.DeviceArray.from_numpy(X), vg.DeviceArray.from_numpy(T)
-    m.train(dx, dt, len(X), batch=4096, steps=1500, lr=2e-3)
-    m.train(dx, dt, len(X), batch=4096, steps=1500, lr=5e-4)
+    from vortegrid_amd import learned
+
+    m, info = learned.train_far_field(near_k=1)
+    print(f"training: {info}")
     # held-out field
     pos, om, radius = scenarios.random_rings(65536, seed=54)
     dp, dw = vg.DeviceArray.from_numpy(pos), vg.DeviceArray.from_numpy(om)

@@ -36,6 +36,8 @@ struct vg_sim {
     vg_grid_desc vel_desc{};
     float spread = 1.f, circ = 1.f;
     double residual = -1.0;
+    vg_mlp *mlp = nullptr;     // learned correction (not owned)
+    uint32_t mlp_layers = 0;
 };
 
 using namespace vg;
@@ -98,6 +100,13 @@ VG_API vg_status vg_sim_destroy(vg_sim *sim) {
     hipFree(sim->negw);
     hipFree(sim->vort_vel);
     delete sim;
+    return VG_OK;
+}
+
+VG_API vg_status vg_sim_set_mlp(vg_sim *sim, vg_mlp *m, uint32_t layer_mask) {
+    VG_REQUIRE(sim, "null argument");
+    sim->mlp = m;
+    sim->mlp_layers = layer_mask;
     return VG_OK;
 }
 
@@ -197,6 +206,10 @@ VG_API vg_status vg_sim_update(vg_sim *sim, float dt, uint32_t frame, float *vel
         if (sim->cfg.tree_near_k < 0)
             VG_TRY(vg_tree_velocity_grid(sim->tree, &g, sim->spread, sim->circ, pol, sim->vel_grid,
                                          s));
+        else if (sim->mlp)
+            VG_TRY(vg_tree_velocity_grid_mlp(sim->tree, &g, sim->mlp, sim->cfg.tree_near_k,
+                                             sim->mlp_layers, sim->spread, sim->circ, pol,
+                                             sim->vel_grid, s));
         else
             VG_TRY(vg_tree_velocity_grid_mac(sim->tree, &g, sim->cfg.tree_near_k,
                                              sim->cfg.tree_order, sim->spread, sim->circ, pol,

@@ -37,7 +37,7 @@ EXPORTS = (
     "vg_sim_vector_potential vg_sim_negative_vorticity vg_sim_grid_template vg_sim_bounds "
     "vg_sim_tree vg_sim_spreading vg_sim_poisson_residual "
     "vg_mlp_create vg_mlp_destroy vg_mlp_set_params vg_mlp_get_params vg_mlp_forward "
-    "vg_mlp_gradients vg_mlp_train vg_tree_mlp_samples vg_tree_velocity_grid_mlp"
+    "vg_mlp_gradients vg_mlp_train vg_tree_mlp_samples vg_tree_velocity_grid_mlp vg_sim_set_mlp"
 ).split()
 
 MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS, MLP_PARAMS = 16, 64, 3, 9603
@@ -539,6 +539,11 @@ class Sim:
         call("vg_sim_create", C.byref(self.h), C.byref(cfg) if cfg is not None else None,
              _s(stream))
         self._v = None
+
+    def set_mlp(self, mlp, layer_mask=0xFFFF):
+        """Learned correction for TREE with tree_near_k >= 0 (None = off)."""
+        self._mlp = mlp
+        call("vg_sim_set_mlp", self.h, mlp.h if mlp is not None else None, C.c_uint32(layer_mask))
 
     def set_vortons(self, v: Vortons):
         self._v = v
