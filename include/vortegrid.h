@@ -335,6 +335,60 @@ vg_status vg_tree_velocity_grid_mlp(const vg_tree *t, const vg_grid_desc *g, vg_
                                     int near_k, uint32_t layer_mask, float spread, float circ,
                                     int rsqrt_policy, float *out, vg_stream s);
 
+/* ------------------------------------------------------------ batched effects */
+/* Many independent small simulations ("effects") advanced together; no
+ * reference counterpart (the reference runs one VortonSim per particle
+ * group, vortonSim.cpp:4117-4256).  Each effect has its own vortons, tracers
+ * and vorton radius, and a grid of cells^3 cells over its own bounding box
+ * (vortons u tracers, padded by 3 radii as UpdateBoundingBox does).  A frame,
+ * in the reference's order (particleSystemConfiguration.cpp:282-296):
+ * tracers AssignVelocityFromField(previous grid) + Evolve; bounds; influence
+ * tree; near-field treecode velocity at the grid points (near_k, order,
+ * auto-mollified); vortons AssignVelocityFromField + Evolve.  Every stage is
+ * one launch over all effects and nothing returns to the host, so a frame
+ * can be recorded into a hipGraph (vg_batch_capture) and replayed with one
+ * launch (vg_batch_launch). */
+typedef struct vg_batch vg_batch;
+typedef struct vg_batch_config {
+    uint32_t cells;    /* cells per axis of every effect's grid (default 32) */
+    int near_k;        /* near-field treecode radius (default 2) */
+    int order;         /* cluster order: 0 monopole, 1 dipole (default 1) */
+    int rsqrt_policy;  /* VG_RSQRT_* (default ACCURATE) */
+} vg_batch_config;
+vg_status vg_batch_config_default(vg_batch_config *cfg);
+/* Host arrays of `effects` entries; tracer_counts may be NULL (none). */
+vg_status vg_batch_create(vg_batch **b, uint32_t effects, const uint32_t *vorton_counts,
+                          const uint32_t *tracer_counts, const float *vorton_radii,
+                          const vg_batch_config *cfg);
+vg_status vg_batch_destroy(vg_batch *b);
+/* Device buffers owned by the batch, packed xyz, effects back to back in
+ * order: vorton positions, angular velocities (vorticity / 2), velocities,
+ * tracer positions.  Any pointer may be NULL. */
+vg_status vg_batch_buffers(vg_batch *b, float **vorton_pos, float **vorton_angvel,
+                           float **vorton_vel, float **tracer_pos, size_t *n_vortons,
+                           size_t *n_tracers);
+/* One frame, launched eagerly. */
+vg_status vg_batch_step(vg_batch *b, float dt, vg_stream s);
+/* Records one frame with time step dt into a hipGraph (s must be a created
+ * stream); vg_batch_launch replays it.  Changing the launch shapes drops the
+ * recording. */
+vg_status vg_batch_capture(vg_batch *b, float dt, vg_stream s);
+vg_status vg_batch_launch(vg_batch *b, vg_stream s);
+/* Grid descriptor (host copy) and device field of one effect's last frame,
+ * and its mollification factors.  Synchronise the device. */
+vg_status vg_batch_grid(const vg_batch *b, uint32_t effect, vg_grid_desc *g, const float **field);
+vg_status vg_batch_mollify(const vg_batch *b, uint32_t effect, float *spread, float *circ);
+/* Launch shapes: treecode waves per workgroup {1, 2, 4}; tracer workgroup
+ * size (multiple of 64, <= 1024) and tracers per thread {1, 2, 4}. */
+vg_status vg_batch_set_launch(vg_batch *b, int walk_waves, int tracer_block, int tracer_items);
+vg_status vg_batch_get_launch(const vg_batch *b, int *walk_waves, int *tracer_block,
+                              int *tracer_items);
+/* Times every candidate launch shape on the live batch (after at least one
+ * vg_batch_step) and keeps the fastest; report (may be NULL) receives one
+ * line per trial.  Blocks.  Results of later frames do not depend on the
+ * shapes chosen. */
+vg_status vg_batch_autotune(vg_batch *b, vg_stream s, char *report, size_t cap);
+
 /* ------------------------------------------------------------ simulation */
 /* vg_sim = VortonSim (vortonSim.h:137-623) with device-resident state:
  * grid template, influence tree, multigrid pyramids and velocity grid. */

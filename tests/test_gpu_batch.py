@@ -1,0 +1,141 @@
+"""Batched effects: every effect of a batch frame equals the single-tree
+path on the same geometry (bit for bit), the recorded hipGraph frame equals
+the eager frame over several replays, and autotuning changes no result."""
+import numpy as np
+import pytest
+
+from vortegrid_amd import scenarios
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+DT = F32(1.0 / 30.0)
+
+
+def case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def make_effects(counts, seed):
+    """Rings and plumes of the given vorton counts, 4 tracers per vorton."""
+    vp, vw, tp, radii = [], [], [], []
+    for i, n in enumerate(counts):
+        if n == 0:
+            pos = np.zeros((0, 3), F32)
+            om = np.zeros((0, 3), F32)
+            radius = 0.05
+            tr = scenarios.tracers_in_box(64, np.float32([-1, -1, -1]), np.float32([1, 1, 1]), seed=seed + i)
+        else:
+            gen = scenarios.vortex_ring if i % 2 == 0 else scenarios.jet_plume
+            pos, om, radius = gen(n, seed=seed + i)
+            pos = pos + F32(3.0 * i)  # effects live in different places
+            tr = scenarios.tracers_in_box(4 * n, pos.min(0), pos.max(0), seed=seed + 100 + i)
+        vp.append(pos)
+        vw.append(om)
+        tp.append(tr)
+        radii.append(radius)
+    return vp, vw, tp, radii
+
+
+def new_batch(vg, counts, seed=1, **kw):
+    vp, vw, tp, radii = make_effects(counts, seed)
+    b = vg.Batch([len(p) for p in vp], [len(t) for t in tp], radii, **kw)
+    b.upload(np.concatenate(vp) if vp else np.zeros((0, 3), F32), np.concatenate(vw),
+             np.concatenate(tp))
+    return b, vp, vw, tp, radii
+
+
+COUNTS = [700, 2500, 0, 1800, 300, 4096]
+
+
+def test_batch_frame_matches_single_trees(vg):
+    case(f"batched frame vs single trees, effects {COUNTS}, cells 16, k=1 dipole")
+    b, vp, vw, tp, radii = new_batch(vg, COUNTS, cells=16, near_k=1, order=1)
+    b.step(DT)
+    vg.synchronize()
+    vpos1, vvel1, tpos1 = b.download("vpos"), b.download("vvel"), b.download("tpos")
+    vo = np.cumsum([0] + [len(p) for p in vp])
+    to = np.cumsum([0] + [len(t) for t in tp])
+    grids = []
+    for e in range(len(COUNTS)):
+        g, field = b.grid(e)
+        spread, circ = b.mollify(e)
+        grids.append((g, field))
+        # geometry: bounds of vortons u tracers padded by 3 radii, 16^3 cells
+        pts = np.concatenate([vp[e], tp[e]])
+        pad = F32(3.0) * F32(radii[e])
+        mn, mx = pts.min(0) - pad, pts.max(0) + pad
+        assert list(g.np) == [17, 17, 17]
+        assert np.array_equal(np.float32(g.min), mn)
+        assert np.array_equal(np.float32(g.extent), (mx - mn).astype(F32))
+        # the same tree and walk on this effect alone
+        n = len(vp[e])
+        if n:
+            dp, dw = vg.DeviceArray.from_numpy(vp[e]), vg.DeviceArray.from_numpy(vw[e])
+            v = vg.vortons_view(dp, dw, n, uniform_size=2 * radii[e])
+            t = vg.Tree()
+            t.build(g, v)
+            out = vg.DeviceArray.zeros((g.capacity, 3))
+            t.velocity_grid_mac(g, out, 1, 1, spread, circ, policy=1)
+            vg.synchronize()
+            assert np.array_equal(out.numpy(), field), e
+            # vortons: AssignVelocityFromField + Evolve with that grid
+            fld = vg.DeviceArray.from_numpy(field)
+            pv = vg.DeviceArray.from_numpy(vp[e])
+            vv = vg.DeviceArray.zeros((n, 3))
+            vg.assign_velocity_from_field(g, fld, pv, vv, n)
+            vg.evolve(pv, vv, n, float(DT))
+            vg.synchronize()
+            assert np.array_equal(vv.numpy(), vvel1[vo[e]:vo[e + 1]]), e
+            assert np.array_equal(pv.numpy(), vpos1[vo[e]:vo[e + 1]]), e
+        else:
+            assert not np.any(field)
+        # frame 1 tracers moved with the zero initial grids
+        assert np.array_equal(tpos1[to[e]:to[e + 1]], tp[e])
+    # frame 2: tracers advect with each effect's frame-1 grid
+    b.step(DT)
+    vg.synchronize()
+    tpos2 = b.download("tpos")
+    for e in range(len(COUNTS)):
+        g, field = grids[e]
+        tr = vg.DeviceArray.from_numpy(tp[e])
+        vg.advect_tracers(g, vg.DeviceArray.from_numpy(field), tr, None, len(tp[e]), float(DT), 1.0)
+        vg.synchronize()
+        assert np.array_equal(tr.numpy(), tpos2[to[e]:to[e + 1]]), e
+
+
+def test_graph_replay_matches_eager(vg):
+    case("batched frame: hipGraph replays vs eager frames, 4 frames")
+    counts = [1200, 3000, 500, 2200]
+    a, *_ = new_batch(vg, counts, seed=7, cells=16, near_k=2, order=1)
+    b, *_ = new_batch(vg, counts, seed=7, cells=16, near_k=2, order=1)
+    s = vg.Stream()
+    b.capture(DT, s)
+    for f in range(4):
+        a.step(DT)
+        b.launch(s)
+        s.sync()
+        vg.synchronize()
+        for key in ("vpos", "vvel", "tpos"):
+            assert np.array_equal(a.download(key), b.download(key)), (f, key)
+        for e in range(len(counts)):
+            assert np.array_equal(a.grid(e)[1], b.grid(e)[1]), (f, e)
+
+
+def test_autotune_keeps_results(vg):
+    case("batched frame: autotune changes launch shapes only")
+    counts = [900, 2600, 1500]
+    a, *_ = new_batch(vg, counts, seed=11, cells=16, near_k=1, order=1)
+    b, *_ = new_batch(vg, counts, seed=11, cells=16, near_k=1, order=1)
+    a.step(DT)
+    b.step(DT)
+    report = b.autotune()
+    print(report)
+    assert "chosen" in report
+    b.set_launch(1, 512, 4)
+    assert b.get_launch() == (1, 512, 4)
+    for _ in range(2):
+        a.step(DT)
+        b.step(DT)
+    vg.synchronize()
+    for key in ("vpos", "vvel", "tpos"):
+        assert np.array_equal(a.download(key), b.download(key)), key

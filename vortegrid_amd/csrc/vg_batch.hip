@@ -1,0 +1,630 @@
+// vg_batch.hip -- many independent small vorton simulations ("effects")
+// advanced together (no reference counterpart: the reference runs one
+// VortonSim per particle group, vortonSim.cpp:4117-4256, each on its own).
+//
+// Every effect owns a grid of a fixed shape (cells^3 cells over its own
+// padded bounding box) and so a NestedGrid of one fixed layout: all effects'
+// trees are stored back to back and every stage of the frame is ONE launch
+// over all effects:
+//
+//   tracers   AssignVelocityFromField(previous grid) + Evolve   (per tracer)
+//   bounds    vortons u tracers AABB per effect                 (per item)
+//   geometry  pad 3 radii, grid + layer descriptors, mollifier  (per effect)
+//   tree      bin / scan / scatter / cell sort / gather / leaf / aggregate
+//   velocity  near-field treecode at grid points                (per brick)
+//   vortons   AssignVelocityFromField + Evolve                  (per vorton)
+//
+// Nothing in the frame goes back to the host (geometry is computed on the
+// device), so a frame can be recorded once into a hipGraph and replayed.
+// Launch shapes of the walk and the tracer kernel are chosen by
+// vg_batch_autotune from timed trials on the live batch.
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "vg_internal.h"
+
+namespace vg {
+vg_status mac_velocity_grid_batch(const TreeView *views, const float2 *mollify, uint32_t trees,
+                                  const uint32_t np[3], int near_k, int order, int policy,
+                                  int waves_per_block, float *out, size_t out_stride,
+                                  hipStream_t s);
+}
+
+struct vg_batch {
+    uint32_t effects = 0;
+    vg_batch_config cfg{};
+    vg::Nested shape{};
+    size_t cap[vg::kMaxLayers] = {};
+    size_t cap0 = 0, nv = 0, nt = 0;
+    // particles (packed xyz, effects back to back) and their owners
+    float *vpos = nullptr, *vom = nullptr, *vvel = nullptr, *tpos = nullptr;
+    uint32_t *vtree = nullptr, *ttree = nullptr;
+    float *radius = nullptr, *size = nullptr;  // per effect
+    // trees
+    float4 *P[vg::kMaxLayers] = {}, *W[vg::kMaxLayers] = {}, *M[vg::kMaxLayers] = {};
+    uint32_t *cell_count = nullptr, *cell_start = nullptr, *keys = nullptr, *slots = nullptr,
+             *sorted_idx = nullptr, *scan_tmp = nullptr;
+    float4 *sorted_p = nullptr, *sorted_w = nullptr;
+    vg::TreeView *views = nullptr;  // per effect, device
+    vg_grid_desc *leaf_desc = nullptr;
+    float2 *mollify = nullptr;
+    uint32_t *bbox = nullptr;  // 6 ordered-encoded extrema per effect
+    float *grid = nullptr;     // effects * cap0 * 3
+    // launch shapes
+    int walk_waves = 4, adv_block = 256, adv_items = 1;
+    // recorded frame
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool stepped = false;  // trees and grids hold a computed frame
+    std::vector<void *> allocs;
+};
+
+namespace {
+using namespace vg;
+
+__device__ __forceinline__ uint32_t ord_enc(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord_dec(uint32_t u) {
+    u = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    return __uint_as_float(u);
+}
+
+__global__ void k_bbox_init(uint32_t *__restrict__ bbox, uint32_t effects) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 6 * effects) return;
+    bbox[i] = (i % 6) < 3 ? 0xffffffffu : 0u;
+}
+
+// Extrema of items (positions packed xyz) per owner; a wave whose items share
+// one owner reduces before its atomics.  Min / max are exact, so the result
+// does not depend on the order.
+__global__ __launch_bounds__(256) void k_bbox_batch(const float *__restrict__ pos,
+                                                    const uint32_t *__restrict__ owner, size_t n,
+                                                    uint32_t *__restrict__ bbox) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < n;
+    const uint32_t k = live ? owner[i] : 0xffffffffu;
+    float p[3] = {0.f, 0.f, 0.f};
+    if (live) {
+        p[0] = pos[3 * i];
+        p[1] = pos[3 * i + 1];
+        p[2] = pos[3 * i + 2];
+    }
+    const uint32_t k0 = __builtin_amdgcn_readfirstlane(k);
+    const bool uniform = __ballot(live && k != k0) == 0ull;
+    if (uniform) {
+        if (k0 == 0xffffffffu) return;
+        uint32_t e[6];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            e[a] = live ? ord_enc(p[a]) : 0xffffffffu;
+            e[3 + a] = live ? ord_enc(p[a]) : 0u;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                e[a] = min(e[a], (uint32_t)__shfl_xor((int)e[a], off, 64));
+                e[3 + a] = max(e[3 + a], (uint32_t)__shfl_xor((int)e[3 + a], off, 64));
+            }
+        if ((threadIdx.x & 63) == 0)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                atomicMin(&bbox[6 * k0 + a], e[a]);
+                atomicMax(&bbox[6 * k0 + 3 + a], e[3 + a]);
+            }
+    } else if (live) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&bbox[6 * k + a], ord_enc(p[a]));
+            atomicMax(&bbox[6 * k + 3 + a], ord_enc(p[a]));
+        }
+    }
+}
+
+struct Shape {
+    uint32_t depth;
+    uint32_t np[kMaxLayers][3];
+};
+
+// UniformGridGeometry spacing (uniformGrid.h:794-809), as the host code.
+__device__ __forceinline__ void spacing(vg_grid_desc &g) {
+#pragma clang fp contract(off)
+    for (int a = 0; a < 3; ++a) g.cell[a] = g.extent[a] / (float)(g.np[a] - 1);
+    g.cpe[0] = (float)(g.np[0] - 1) / g.extent[0];
+    g.cpe[1] = (float)(g.np[1] - 1) / g.extent[1];
+    g.cpe[2] = (0.0f == g.extent[2]) ? 1.0f / FLT_MIN : (float)(g.np[2] - 1) / g.extent[2];
+}
+
+// Per effect: UpdateBoundingBox(final) padding (vortonSim.cpp:1027-1096) with
+// the fixed grid shape, every layer's descriptor, auto-mollification
+// (vortonSim.cpp:2458-2477).  An effect with no items keeps a unit box.
+__global__ void k_geometry(const uint32_t *__restrict__ bbox, const float *__restrict__ radius,
+                           uint32_t effects, Shape shape, TreeView *__restrict__ views,
+                           vg_grid_desc *__restrict__ leaf_desc, float2 *__restrict__ mollify) {
+#pragma clang fp contract(off)
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= effects) return;
+    float mn[3], mx[3];
+    bool empty = false;
+    for (int a = 0; a < 3; ++a) {
+        mn[a] = ord_dec(bbox[6 * e + a]);
+        mx[a] = ord_dec(bbox[6 * e + 3 + a]);
+        empty = empty || !(mn[a] <= mx[a]);
+    }
+    const float r = radius[e];
+    const float pad = (1.0f + 2.0f) * r;
+    for (int a = 0; a < 3; ++a) {
+        if (empty) {
+            mn[a] = 0.f;
+            mx[a] = 1.f;
+        } else {
+            mn[a] -= pad;
+            mx[a] += pad;
+        }
+    }
+    TreeView &tv = views[e];
+    for (uint32_t l = 0; l < shape.depth; ++l) {
+        vg_grid_desc g;
+        for (int a = 0; a < 3; ++a) {
+            g.min[a] = mn[a];
+            g.extent[a] = mx[a] - mn[a];
+            g.np[a] = shape.np[l][a];
+        }
+        spacing(g);
+        tv.layer[l] = g;
+        if (l == 0) leaf_desc[e] = g;
+    }
+    const vg_grid_desc &g0 = tv.layer[0];
+    const float cell_volume = g0.cell[0] * g0.cell[1] * g0.cell[2];
+    float s = powf(cell_volume, 1.0f / 3.0f) / r;
+    if (s < 1.0f) s = 1.0f;
+    mollify[e] = make_float2(s, 1.0f / (s * s * s));
+}
+
+// AssignVelocityFromField (gain 1) + Evolve for items owned by effects, each
+// interpolating its owner's grid; vel (may be null) receives the velocity.
+template <int ITEMS>
+__global__ void k_advect_batch(const TreeView *__restrict__ views, const float *__restrict__ grid,
+                               size_t cap0, float *__restrict__ pos, float *__restrict__ vel,
+                               const uint32_t *__restrict__ owner, size_t n, float dt) {
+#pragma clang fp contract(off)
+    const size_t base = ((size_t)blockIdx.x * ITEMS) * blockDim.x + threadIdx.x;
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+        const size_t i = base + (size_t)it * blockDim.x;
+        if (i >= n) return;
+        const uint32_t k = owner[i];
+        const vg_grid_desc &g = views[k].layer[0];
+        float *p = pos + 3 * i;
+        const float3 v = interpolate(g, grid + 3 * cap0 * k, make_float3(p[0], p[1], p[2]));
+        if (vel) {
+            vel[3 * i] = v.x;
+            vel[3 * i + 1] = v.y;
+            vel[3 * i + 2] = v.z;
+        }
+        p[0] = p[0] + v.x * dt;
+        p[1] = p[1] + v.y * dt;
+        p[2] = p[2] + v.z * dt;
+    }
+}
+
+vg_status advect(vg_batch *b, float *pos, float *vel, const uint32_t *owner, size_t n, float dt,
+                 int block, int items, hipStream_t s) {
+    if (n == 0) return VG_OK;
+    const dim3 grid(ceil_div(n, (size_t)block * items));
+    const char *name = vel ? "vg_batch_vortons" : "vg_batch_tracers";
+    switch (items) {
+        case 1:
+            return launch(name, k_advect_batch<1>, grid, dim3(block), 0, s,
+                          (const TreeView *)b->views, (const float *)b->grid, b->cap0, pos, vel,
+                          owner, n, dt);
+        case 2:
+            return launch(name, k_advect_batch<2>, grid, dim3(block), 0, s,
+                          (const TreeView *)b->views, (const float *)b->grid, b->cap0, pos, vel,
+                          owner, n, dt);
+        default:
+            return launch(name, k_advect_batch<4>, grid, dim3(block), 0, s,
+                          (const TreeView *)b->views, (const float *)b->grid, b->cap0, pos, vel,
+                          owner, n, dt);
+    }
+}
+
+template <typename T>
+vg_status dalloc(vg_batch *b, T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    VG_HIP(hipMalloc((void **)p, count * sizeof(T)));
+    b->allocs.push_back(*p);
+    return VG_OK;
+}
+
+__global__ void k_zero_f(float *p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+vg_status frame(vg_batch *b, float dt, hipStream_t s) {
+    // tracers first, with the grids of the previous frame
+    VG_TRY(advect(b, b->tpos, nullptr, b->ttree, b->nt, dt, b->adv_block, b->adv_items, s));
+    VG_TRY(launch("vg_batch_bbox", k_bbox_init, dim3(ceil_div(6 * b->effects, 256)), dim3(256), 0, s,
+                  b->bbox, b->effects));
+    if (b->nv)
+        VG_TRY(launch("vg_batch_bbox", k_bbox_batch, dim3(ceil_div(b->nv, 256)), dim3(256), 0, s,
+                      (const float *)b->vpos, (const uint32_t *)b->vtree, b->nv, b->bbox));
+    if (b->nt)
+        VG_TRY(launch("vg_batch_bbox", k_bbox_batch, dim3(ceil_div(b->nt, 256)), dim3(256), 0, s,
+                      (const float *)b->tpos, (const uint32_t *)b->ttree, b->nt, b->bbox));
+    Shape sh;
+    sh.depth = b->shape.depth;
+    for (uint32_t l = 0; l < kMaxLayers; ++l)
+        for (int a = 0; a < 3; ++a) sh.np[l][a] = l < sh.depth ? b->shape.layer[l].np[a] : 0;
+    VG_TRY(launch("vg_batch_geometry", k_geometry, dim3(ceil_div(b->effects, 64)), dim3(64), 0, s,
+                  (const uint32_t *)b->bbox, (const float *)b->radius, b->effects, sh, b->views,
+                  b->leaf_desc, b->mollify));
+    BatchTreeArgs ta;
+    ta.trees = b->effects;
+    ta.shape = &b->shape;
+    ta.n = b->nv;
+    ta.pos = b->vpos;
+    ta.angvel = b->vom;
+    ta.tree_of = b->vtree;
+    ta.leaf_desc = b->leaf_desc;
+    ta.size = b->size;
+    ta.cell_count = b->cell_count;
+    ta.cell_start = b->cell_start;
+    ta.keys = b->keys;
+    ta.slots = b->slots;
+    ta.sorted_idx = b->sorted_idx;
+    ta.sorted_p = b->sorted_p;
+    ta.sorted_w = b->sorted_w;
+    ta.scan_tmp = b->scan_tmp;
+    for (uint32_t l = 0; l < b->shape.depth; ++l) {
+        ta.P[l] = b->P[l];
+        ta.W[l] = b->W[l];
+        ta.M[l] = b->M[l];
+    }
+    VG_TRY(batch_tree_build(ta, s));
+    VG_TRY(mac_velocity_grid_batch(b->views, b->mollify, b->effects, b->shape.layer[0].np,
+                                   b->cfg.near_k, b->cfg.order, b->cfg.rsqrt_policy, b->walk_waves,
+                                   b->grid, 3 * b->cap0, s));
+    return advect(b, b->vpos, b->vvel, b->vtree, b->nv, dt, b->adv_block, b->adv_items, s);
+}
+
+void drop_graph(vg_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+}  // namespace
+
+using namespace vg;
+
+VG_API vg_status vg_batch_config_default(vg_batch_config *cfg) {
+    VG_REQUIRE(cfg, "null argument");
+    cfg->cells = 32;
+    cfg->near_k = 2;
+    cfg->order = 1;
+    cfg->rsqrt_policy = VG_RSQRT_ACCURATE;
+    return VG_OK;
+}
+
+VG_API vg_status vg_batch_create(vg_batch **out, uint32_t effects, const uint32_t *vorton_counts,
+                                 const uint32_t *tracer_counts, const float *vorton_radii,
+                                 const vg_batch_config *cfg) {
+    VG_REQUIRE(out && vorton_counts && vorton_radii, "null argument");
+    VG_REQUIRE(effects > 0, "no effects");
+    vg_batch_config c;
+    if (cfg) c = *cfg;
+    else vg_batch_config_default(&c);
+    VG_REQUIRE(c.cells >= 4 && c.cells <= 256, "cells must be in [4, 256]");
+    VG_REQUIRE(c.near_k >= 0 && c.near_k <= 16, "near_k must be in [0, 16]");
+    VG_REQUIRE(c.order == 0 || c.order == 1, "order must be 0 or 1");
+    VG_REQUIRE(c.rsqrt_policy == VG_RSQRT_REFERENCE || c.rsqrt_policy == VG_RSQRT_ACCURATE,
+               "bad rsqrt policy");
+    *out = nullptr;
+    vg_batch *b = new vg_batch();
+    b->effects = effects;
+    b->cfg = c;
+    auto fail = [&](vg_status st) {
+        vg_batch_destroy(b);
+        return st;
+    };
+    // the fixed shape: (cells + 1)^3 points over a unit box
+    vg_grid_desc tmpl;
+    for (int a = 0; a < 3; ++a) {
+        tmpl.min[a] = 0.f;
+        tmpl.extent[a] = 1.f;
+        tmpl.np[a] = c.cells + 1;
+        tmpl.cell[a] = 1.f / (float)c.cells;
+        tmpl.cpe[a] = (float)c.cells;
+    }
+    nested_init(&b->shape, tmpl);
+    b->cap0 = capacity(b->shape.layer[0]);
+    std::vector<uint32_t> vt, tt;
+    std::vector<float> rad(effects), sz(effects);
+    for (uint32_t e = 0; e < effects; ++e) {
+        if (!(vorton_radii[e] > 0.f)) return fail((set_error("vorton radii must be positive"), VG_EINVAL));
+        rad[e] = vorton_radii[e];
+        sz[e] = 2.0f * vorton_radii[e];
+        vt.insert(vt.end(), vorton_counts[e], e);
+        if (tracer_counts) tt.insert(tt.end(), tracer_counts[e], e);
+    }
+    b->nv = vt.size();
+    b->nt = tt.size();
+    if (b->nv >= (1ull << 31) || b->nt >= (1ull << 34))
+        return fail((set_error("too many particles"), VG_EINVAL));
+    if ((size_t)effects * b->cap0 >= (1ull << 32))
+        return fail((set_error("effects x grid points too large"), VG_EINVAL));
+    vg_status st = VG_OK;
+#define VG_B(expr)                                   \
+    do {                                             \
+        st = (expr);                                 \
+        if (st != VG_OK) return fail(st);            \
+    } while (0)
+    VG_B(dalloc(b, &b->vpos, 3 * b->nv));
+    VG_B(dalloc(b, &b->vom, 3 * b->nv));
+    VG_B(dalloc(b, &b->vvel, 3 * b->nv));
+    VG_B(dalloc(b, &b->tpos, 3 * b->nt));
+    VG_B(dalloc(b, &b->vtree, b->nv));
+    VG_B(dalloc(b, &b->ttree, b->nt));
+    VG_B(dalloc(b, &b->radius, effects));
+    VG_B(dalloc(b, &b->size, effects));
+    for (uint32_t l = 0; l < b->shape.depth; ++l) {
+        b->cap[l] = capacity(b->shape.layer[l]);
+        VG_B(dalloc(b, &b->P[l], effects * b->cap[l]));
+        VG_B(dalloc(b, &b->W[l], effects * b->cap[l]));
+        VG_B(dalloc(b, &b->M[l], 3 * effects * b->cap[l]));
+    }
+    const size_t cells = (size_t)effects * b->cap0;
+    VG_B(dalloc(b, &b->cell_count, cells));
+    VG_B(dalloc(b, &b->cell_start, cells));
+    VG_B(dalloc(b, &b->keys, b->nv));
+    VG_B(dalloc(b, &b->slots, b->nv));
+    VG_B(dalloc(b, &b->sorted_idx, b->nv));
+    VG_B(dalloc(b, &b->sorted_p, b->nv));
+    VG_B(dalloc(b, &b->sorted_w, b->nv));
+    VG_B(dalloc(b, &b->scan_tmp, scan_scratch_elems(cells)));
+    VG_B(dalloc(b, &b->views, effects));
+    VG_B(dalloc(b, &b->leaf_desc, effects));
+    VG_B(dalloc(b, &b->mollify, effects));
+    VG_B(dalloc(b, &b->bbox, 6 * (size_t)effects));
+    VG_B(dalloc(b, &b->grid, 3 * cells));
+#undef VG_B
+    auto up = [&](void *dst, const void *src, size_t bytes) -> vg_status {
+        if (bytes) VG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+        return VG_OK;
+    };
+    if ((st = up(b->vtree, vt.data(), vt.size() * 4)) != VG_OK) return fail(st);
+    if ((st = up(b->ttree, tt.data(), tt.size() * 4)) != VG_OK) return fail(st);
+    if ((st = up(b->radius, rad.data(), effects * 4)) != VG_OK) return fail(st);
+    if ((st = up(b->size, sz.data(), effects * 4)) != VG_OK) return fail(st);
+    // per-effect views: fixed pointers and decimations, unit-box geometry
+    // until the first frame computes the real one
+    std::vector<TreeView> views(effects);
+    for (uint32_t e = 0; e < effects; ++e) {
+        TreeView &v = views[e];
+        memset(&v, 0, sizeof v);
+        v.depth = b->shape.depth;
+        for (uint32_t l = 0; l < kMaxLayers; ++l)
+            for (int a = 0; a < 3; ++a) v.dec[l][a] = b->shape.dec[l][a];
+        for (uint32_t l = 0; l < b->shape.depth; ++l) {
+            v.layer[l] = b->shape.layer[l];
+            v.P[l] = b->P[l] + e * b->cap[l];
+            v.W[l] = b->W[l] + e * b->cap[l];
+            v.M[l] = b->M[l] + 3 * e * b->cap[l];
+        }
+        v.cell_start = b->cell_start + e * b->cap0;
+        v.cell_count = b->cell_count + e * b->cap0;
+        v.sorted_p = b->sorted_p;
+        v.sorted_w = b->sorted_w;
+        v.sorted_idx = b->sorted_idx;
+        v.n = (uint32_t)b->nv;
+    }
+    if ((st = up(b->views, views.data(), effects * sizeof(TreeView))) != VG_OK) return fail(st);
+    hipError_t e = hipMemset(b->grid, 0, 3 * cells * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(b->vvel, 0, 3 * (b->nv ? b->nv : 1) * sizeof(float));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(device_error(e, "batch init", __FILE__, __LINE__));
+    *out = b;
+    return VG_OK;
+}
+
+VG_API vg_status vg_batch_destroy(vg_batch *b) {
+    if (!b) return VG_OK;
+    drop_graph(b);
+    for (void *p : b->allocs) (void)hipFree(p);
+    delete b;
+    return VG_OK;
+}
+
+VG_API vg_status vg_batch_buffers(vg_batch *b, float **vorton_pos, float **vorton_angvel,
+                                  float **vorton_vel, float **tracer_pos, size_t *n_vortons,
+                                  size_t *n_tracers) {
+    VG_REQUIRE(b, "null batch");
+    if (vorton_pos) *vorton_pos = b->vpos;
+    if (vorton_angvel) *vorton_angvel = b->vom;
+    if (vorton_vel) *vorton_vel = b->vvel;
+    if (tracer_pos) *tracer_pos = b->tpos;
+    if (n_vortons) *n_vortons = b->nv;
+    if (n_tracers) *n_tracers = b->nt;
+    return VG_OK;
+}
+
+VG_API vg_status vg_batch_step(vg_batch *b, float dt, vg_stream s) {
+    VG_REQUIRE(b, "null batch");
+    VG_TRY(frame(b, dt, (hipStream_t)s));
+    b->stepped = true;
+    return VG_OK;
+}
+
+VG_API vg_status vg_batch_capture(vg_batch *b, float dt, vg_stream stream) {
+    VG_REQUIRE(b, "null batch");
+    VG_REQUIRE(stream, "capture needs a created stream (not the legacy stream)");
+    hipStream_t s = (hipStream_t)stream;
+    drop_graph(b);
+    const bool prof = profiling_enabled();
+    if (prof) vg_profile_enable(0);
+    VG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    vg_status st = frame(b, dt, s);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (prof) vg_profile_enable(1);
+    if (st != VG_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return device_error(e, "hipStreamEndCapture", __FILE__, __LINE__);
+    b->graph = g;
+    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        drop_graph(b);
+        return device_error(e, "hipGraphInstantiate", __FILE__, __LINE__);
+    }
+    return VG_OK;
+}
+
+VG_API vg_status vg_batch_launch(vg_batch *b, vg_stream s) {
+    VG_REQUIRE(b, "null batch");
+    if (!b->exec) {
+        set_error("no recorded frame (call vg_batch_capture first)");
+        return VG_ESTATE;
+    }
+    VG_HIP(hipGraphLaunch(b->exec, (hipStream_t)s));
+    b->stepped = true;
+    return VG_OK;
+}
+
+VG_API vg_status vg_batch_grid(const vg_batch *b, uint32_t effect, vg_grid_desc *g,
+                               const float **field) {
+    VG_REQUIRE(b && g, "null argument");
+    VG_REQUIRE(effect < b->effects, "effect out of range");
+    VG_HIP(hipDeviceSynchronize());
+    VG_HIP(hipMemcpy(g, b->leaf_desc + effect, sizeof *g, hipMemcpyDeviceToHost));
+    if (field) *field = b->grid + 3 * b->cap0 * effect;
+    return VG_OK;
+}
+
+VG_API vg_status vg_batch_mollify(const vg_batch *b, uint32_t effect, float *spread, float *circ) {
+    VG_REQUIRE(b && spread && circ, "null argument");
+    VG_REQUIRE(effect < b->effects, "effect out of range");
+    float2 m;
+    VG_HIP(hipDeviceSynchronize());
+    VG_HIP(hipMemcpy(&m, b->mollify + effect, sizeof m, hipMemcpyDeviceToHost));
+    *spread = m.x;
+    *circ = m.y;
+    return VG_OK;
+}
+
+VG_API vg_status vg_batch_set_launch(vg_batch *b, int walk_waves, int tracer_block,
+                                     int tracer_items) {
+    VG_REQUIRE(b, "null batch");
+    VG_REQUIRE(walk_waves == 1 || walk_waves == 2 || walk_waves == 4, "walk_waves in {1, 2, 4}");
+    VG_REQUIRE(tracer_block >= 64 && tracer_block <= 1024 && tracer_block % 64 == 0,
+               "tracer_block must be a multiple of 64 in [64, 1024]");
+    VG_REQUIRE(tracer_items == 1 || tracer_items == 2 || tracer_items == 4,
+               "tracer_items in {1, 2, 4}");
+    b->walk_waves = walk_waves;
+    b->adv_block = tracer_block;
+    b->adv_items = tracer_items;
+    drop_graph(b);
+    return VG_OK;
+}
+
+VG_API vg_status vg_batch_get_launch(const vg_batch *b, int *walk_waves, int *tracer_block,
+                                     int *tracer_items) {
+    VG_REQUIRE(b && walk_waves && tracer_block && tracer_items, "null argument");
+    *walk_waves = b->walk_waves;
+    *tracer_block = b->adv_block;
+    *tracer_items = b->adv_items;
+    return VG_OK;
+}
+
+// Times each candidate launch shape of the two dominant kernels on the live
+// batch.  The walk trials rewrite the grids with the values they already
+// hold (same trees, same arithmetic); the tracer trials move a scratch copy
+// of the positions.
+VG_API vg_status vg_batch_autotune(vg_batch *b, vg_stream stream, char *report, size_t cap) {
+    VG_REQUIRE(b, "null batch");
+    if (!b->stepped) {
+        set_error("autotune needs a computed frame (call vg_batch_step first)");
+        return VG_ESTATE;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    drop_graph(b);
+    hipEvent_t e0, e1;
+    VG_HIP(hipEventCreate(&e0));
+    VG_HIP(hipEventCreate(&e1));
+    std::string rep;
+    auto timed = [&](auto &&fn, float *ms) -> vg_status {
+        VG_TRY(fn());  // warm
+        VG_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < 3; ++r) VG_TRY(fn());
+        VG_HIP(hipEventRecord(e1, s));
+        VG_HIP(hipEventSynchronize(e1));
+        VG_HIP(hipEventElapsedTime(ms, e0, e1));
+        *ms /= 3.0f;
+        return VG_OK;
+    };
+    vg_status st = VG_OK;
+    // walk: waves per block
+    float best = FLT_MAX;
+    int best_w = b->walk_waves;
+    for (int w : {1, 2, 4}) {
+        float ms = 0.f;
+        st = timed([&]() {
+            return mac_velocity_grid_batch(b->views, b->mollify, b->effects, b->shape.layer[0].np,
+                                           b->cfg.near_k, b->cfg.order, b->cfg.rsqrt_policy, w,
+                                           b->grid, 3 * b->cap0, s);
+        }, &ms);
+        if (st != VG_OK) break;
+        char line[96];
+        snprintf(line, sizeof line, "walk waves_per_block=%d: %.3f ms\n", w, ms);
+        rep += line;
+        if (ms < best) { best = ms; best_w = w; }
+    }
+    // tracers: block size x items per thread, on a scratch copy of positions
+    float *scratch = nullptr;
+    int best_b = b->adv_block, best_i = b->adv_items;
+    if (st == VG_OK && b->nt) {
+        st = salloc((void **)&scratch, 3 * b->nt * sizeof(float), s);
+        best = FLT_MAX;
+        for (int blk : {128, 256, 512, 1024}) {
+            for (int it : {1, 2, 4}) {
+                if (st != VG_OK) break;
+                float ms = 0.f;
+                hipError_t he = hipMemcpyAsync(scratch, b->tpos, 3 * b->nt * sizeof(float),
+                                               hipMemcpyDeviceToDevice, s);
+                if (he != hipSuccess) { st = device_error(he, "autotune copy", __FILE__, __LINE__); break; }
+                st = timed([&]() { return advect(b, scratch, nullptr, b->ttree, b->nt, 0.0f, blk, it, s); }, &ms);
+                char line[96];
+                snprintf(line, sizeof line, "tracers block=%d items=%d: %.3f ms\n", blk, it, ms);
+                rep += line;
+                if (ms < best) { best = ms; best_b = blk; best_i = it; }
+            }
+        }
+        if (scratch) sfree(scratch, s);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != VG_OK) return st;
+    b->walk_waves = best_w;
+    b->adv_block = best_b;
+    b->adv_items = best_i;
+    char line[128];
+    snprintf(line, sizeof line, "chosen: walk waves_per_block=%d, tracers block=%d items=%d\n", best_w,
+             best_b, best_i);
+    rep += line;
+    if (report && cap) {
+        snprintf(report, cap, "%s", rep.c_str());
+    }
+    return VG_OK;
+}

@@ -106,6 +106,28 @@ struct TreeView {
     uint32_t n;
 };
 
+// ------------------------------------------------------------------ batched trees
+// `trees` influence trees of one shape (one Nested layout), stored back to
+// back: layer l of tree k occupies points [k cap_l, (k+1) cap_l).  Vortons
+// of all trees are concatenated; tree_of[v] names the owner, leaf_desc[k]
+// (device) the tree's leaf geometry, size[k] (device) its vorton diameter.
+struct BatchTreeArgs {
+    uint32_t trees = 0;
+    const Nested *shape = nullptr;
+    size_t n = 0;
+    const float *pos = nullptr, *angvel = nullptr;  // packed xyz
+    const uint32_t *tree_of = nullptr;
+    const vg_grid_desc *leaf_desc = nullptr;
+    const float *size = nullptr;
+    uint32_t *cell_count = nullptr, *cell_start = nullptr;  // trees * cap0
+    uint32_t *keys = nullptr, *slots = nullptr, *sorted_idx = nullptr;  // n
+    float4 *sorted_p = nullptr, *sorted_w = nullptr;                    // n
+    uint32_t *scan_tmp = nullptr;  // scan_scratch_elems(trees * cap0)
+    float4 *P[kMaxLayers] = {}, *W[kMaxLayers] = {}, *M[kMaxLayers] = {};
+};
+size_t scan_scratch_elems(size_t n);
+vg_status batch_tree_build(const BatchTreeArgs &a, hipStream_t s);
+
 // ------------------------------------------------------------------ device helpers
 #ifdef __HIPCC__
 

@@ -98,15 +98,12 @@ __device__ __forceinline__ void cluster_term(float3 &part, float3 q, float4 p, f
     }
 }
 
+// One wave evaluates one brick of grid points; S is the wave's LDS slot.
 template <int POLICY, int ORDER, int COUNT>
-__global__ __launch_bounds__(256) void k_mac_grid(const TreeView tv, vg_grid_desc g, MacParams mp,
-                                                  float *__restrict__ out,
-                                                  unsigned long long *__restrict__ counts) {
-    __shared__ WaveStage stage[4];
-    const uint32_t wave = uni(threadIdx.x >> 6);
-    const uint32_t brick = uni(blockIdx.x * 4 + wave);
-    if (brick >= mp.nbricks) return;
-    WaveStage &S = stage[wave];
+__device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc &g,
+                                          const MacParams &mp, uint32_t brick, WaveStage &S,
+                                          float *__restrict__ out,
+                                          unsigned long long *__restrict__ counts) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t bb[3] = {brick % mp.nb[0], (brick / mp.nb[0]) % mp.nb[1],
                             brick / (mp.nb[0] * mp.nb[1])};
@@ -304,6 +301,38 @@ __global__ __launch_bounds__(256) void k_mac_grid(const TreeView tv, vg_grid_des
     }
 }
 
+template <int POLICY, int ORDER, int COUNT>
+__global__ __launch_bounds__(256) void k_mac_grid(const TreeView tv, vg_grid_desc g, MacParams mp,
+                                                  float *__restrict__ out,
+                                                  unsigned long long *__restrict__ counts) {
+    __shared__ WaveStage stage[4];
+    const uint32_t wave = uni(threadIdx.x >> 6);
+    const uint32_t brick = uni(blockIdx.x * 4 + wave);
+    if (brick >= mp.nbricks) return;
+    mac_brick<POLICY, ORDER, COUNT>(tv, g, mp, brick, stage[wave], out, counts);
+}
+
+// Many trees of one shape at once: blockIdx.y selects the tree (its view,
+// mollification and output slab), WPB waves per block cover WPB bricks.
+template <int POLICY, int ORDER, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_mac_grid_batch(const TreeView *__restrict__ views,
+                                                             const float2 *__restrict__ mollify,
+                                                             MacParams mp, float *__restrict__ out,
+                                                             size_t out_stride) {
+    __shared__ WaveStage stage[WPB];
+    const uint32_t wave = uni(threadIdx.x >> 6);
+    const uint32_t brick = uni(blockIdx.x * WPB + wave);
+    if (brick >= mp.nbricks) return;
+    const uint32_t tree = blockIdx.y;
+    const TreeView &tv = views[tree];
+    const float2 sc = mollify[tree];
+    MacParams p = mp;
+    p.spread = sc.x;
+    p.circ = sc.y;
+    mac_brick<POLICY, ORDER, 0>(tv, tv.layer[0], p, brick, stage[wave], out + tree * out_stride,
+                                nullptr);
+}
+
 template <int COUNT>
 vg_status run_mac(const vg_tree *t, const vg_grid_desc *g, int near_k, int order, float spread,
                   float circ, int policy, float *out, unsigned long long *counts, hipStream_t s) {
@@ -338,6 +367,36 @@ vg_status run_mac(const vg_tree *t, const vg_grid_desc *g, int near_k, int order
 vg_status mac_velocity_grid(const vg_tree *t, const vg_grid_desc *g, int near_k, int order,
                             float spread, float circ, int policy, float *out, hipStream_t s) {
     return run_mac<0>(t, g, near_k, order, spread, circ, policy, out, nullptr, s);
+}
+
+// Batched walk over `trees` trees of one shape (views and per-tree
+// (spread, circ) in device memory); out holds trees slabs of out_stride
+// floats.  waves_per_block in {1, 2, 4}.
+vg_status mac_velocity_grid_batch(const TreeView *views, const float2 *mollify, uint32_t trees,
+                                  const uint32_t np[3], int near_k, int order, int policy,
+                                  int waves_per_block, float *out, size_t out_stride,
+                                  hipStream_t s) {
+    VG_REQUIRE(near_k >= 0 && near_k <= 16, "near_k must be in [0, 16]");
+    VG_REQUIRE(order == 0 || order == 1, "order must be 0 or 1");
+    VG_REQUIRE(policy == VG_RSQRT_REFERENCE || policy == VG_RSQRT_ACCURATE, "bad rsqrt policy");
+    VG_REQUIRE(waves_per_block == 1 || waves_per_block == 2 || waves_per_block == 4,
+               "waves_per_block must be 1, 2 or 4");
+    MacParams mp;
+    mp.k = near_k;
+    mp.spread = 1.0f;
+    mp.circ = 1.0f;
+    for (int a = 0; a < 3; ++a) mp.nb[a] = (np[a] + 3) / 4;
+    mp.nbricks = mp.nb[0] * mp.nb[1] * mp.nb[2];
+    using K = void (*)(const TreeView *, const float2 *, MacParams, float *, size_t);
+    K tab[2][2][3] = {
+        {{k_mac_grid_batch<0, 0, 1>, k_mac_grid_batch<0, 0, 2>, k_mac_grid_batch<0, 0, 4>},
+         {k_mac_grid_batch<0, 1, 1>, k_mac_grid_batch<0, 1, 2>, k_mac_grid_batch<0, 1, 4>}},
+        {{k_mac_grid_batch<1, 0, 1>, k_mac_grid_batch<1, 0, 2>, k_mac_grid_batch<1, 0, 4>},
+         {k_mac_grid_batch<1, 1, 1>, k_mac_grid_batch<1, 1, 2>, k_mac_grid_batch<1, 1, 4>}}};
+    const int wi = waves_per_block == 1 ? 0 : waves_per_block == 2 ? 1 : 2;
+    return launch("vg_batch_velocity_grid", tab[policy][order][wi],
+                  dim3(ceil_div(mp.nbricks, waves_per_block), trees), dim3(64 * waves_per_block), 0,
+                  s, views, mollify, mp, out, out_stride);
 }
 }  // namespace vg
 

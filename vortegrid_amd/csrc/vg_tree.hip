@@ -90,7 +90,7 @@ __global__ void k_scan_add(uint32_t *__restrict__ out, size_t n,
     if (i < n) out[i] += offs[i / kScanTile];
 }
 
-size_t scan_scratch_elems(size_t n) {
+size_t scan_elems(size_t n) {
     size_t total = 0;
     while (n > (size_t)kScanTile) {
         n = (n + kScanTile - 1) / kScanTile;
@@ -232,6 +232,12 @@ __global__ void k_aggregate(vg_grid_desc gp, vg_grid_desc gc, uint32_t dx, uint3
     const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t cap = capacity(gp);
     if (o >= cap) return;
+    // blockIdx.y selects one of several trees of this shape stored back to back
+    {
+        const size_t tp = (size_t)blockIdx.y * cap, tc = (size_t)blockIdx.y * capacity(gc);
+        Pc += tc; Wc += tc; Mc += 3 * tc;
+        Pp += tp; Wp += tp; Mp += 3 * tp;
+    }
     const uint32_t ix = (uint32_t)(o % gp.np[0]);
     const uint32_t iy = (uint32_t)((o / gp.np[0]) % gp.np[1]);
     const uint32_t iz = (uint32_t)(o / ((size_t)gp.np[0] * gp.np[1]));
@@ -613,7 +619,75 @@ TreeView make_view(const vg_tree *t) {
     return v;
 }
 
+// Batched binning: key = tree * cap0 + leaf cell of the owner's grid.
+__global__ void k_bin_batch(const float *__restrict__ pos, size_t n,
+                            const uint32_t *__restrict__ tree_of,
+                            const vg_grid_desc *__restrict__ leaf_desc, size_t cap0,
+                            uint32_t *__restrict__ count, uint32_t *__restrict__ keys,
+                            uint32_t *__restrict__ slots) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = tree_of[i];
+    const vg_grid_desc g = leaf_desc[k];
+    const float *p = pos + 3 * i;
+    const int ix = cell_of(p[0], g.min[0], g.cpe[0], (int)g.np[0] - 1);
+    const int iy = cell_of(p[1], g.min[1], g.cpe[1], (int)g.np[1] - 1);
+    const int iz = cell_of(p[2], g.min[2], g.cpe[2], (int)g.np[2] - 1);
+    const uint32_t key = (uint32_t)((size_t)k * cap0 + point_offset(g, ix, iy, iz));
+    keys[i] = key;
+    slots[i] = atomicAdd(&count[key], 1u);
+}
+
+__global__ void k_gather_batch(const float *__restrict__ pos, const float *__restrict__ angvel,
+                               const uint32_t *__restrict__ tree_of,
+                               const float *__restrict__ size,
+                               const uint32_t *__restrict__ sorted_idx, size_t n,
+                               float4 *__restrict__ sp, float4 *__restrict__ sw) {
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const size_t i = sorted_idx[j];
+    const float *p = pos + 3 * i, *w = angvel + 3 * i;
+    sp[j] = make_float4(p[0], p[1], p[2], size[tree_of[i]]);
+    sw[j] = make_float4(w[0], w[1], w[2], 0.f);
+}
+
 }  // namespace
+
+size_t scan_scratch_elems(size_t n) { return scan_elems(n); }
+
+vg_status batch_tree_build(const BatchTreeArgs &a, hipStream_t s) {
+    const Nested &ng = *a.shape;
+    const size_t cap0 = capacity(ng.layer[0]);
+    const size_t cells = (size_t)a.trees * cap0;
+    VG_REQUIRE(cells < (1ull << 32), "batched trees too large");
+    const size_t n = a.n;
+    VG_TRY(launch("vg_batch_fill", k_fill_u32, dim3(ceil_div(cells, 256)), dim3(256), 0, s,
+                  a.cell_count, cells, 0u));
+    VG_TRY(launch("vg_batch_bin", k_bin_batch, dim3(ceil_div(n, 256)), dim3(256), 0, s, a.pos, n,
+                  a.tree_of, a.leaf_desc, cap0, a.cell_count, a.keys, a.slots));
+    VG_TRY(exclusive_scan(a.cell_count, a.cell_start, cells, a.scan_tmp, s));
+    VG_TRY(launch("vg_batch_scatter", k_scatter, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                  (const uint32_t *)a.keys, (const uint32_t *)a.slots,
+                  (const uint32_t *)a.cell_start, n, a.sorted_idx));
+    VG_TRY(launch("vg_batch_sort_cells", k_sort_cells, dim3(ceil_div(cells, 256)), dim3(256), 0, s,
+                  (const uint32_t *)a.cell_start, (const uint32_t *)a.cell_count, cells,
+                  a.sorted_idx));
+    VG_TRY(launch("vg_batch_gather", k_gather_batch, dim3(ceil_div(n, 256)), dim3(256), 0, s, a.pos,
+                  a.angvel, a.tree_of, a.size, (const uint32_t *)a.sorted_idx, n, a.sorted_p,
+                  a.sorted_w));
+    VG_TRY(launch("vg_batch_leaf", k_leaf, dim3(ceil_div(cells, 256)), dim3(256), 0, s,
+                  (const uint32_t *)a.cell_start, (const uint32_t *)a.cell_count,
+                  (const float4 *)a.sorted_p, (const float4 *)a.sorted_w, cells, a.P[0], a.W[0],
+                  a.M[0]));
+    for (uint32_t l = 1; l < ng.depth; ++l) {
+        const size_t cap = capacity(ng.layer[l]);
+        VG_TRY(launch("vg_batch_aggregate", k_aggregate, dim3(ceil_div(cap, 256), a.trees), dim3(256),
+                      0, s, ng.layer[l], ng.layer[l - 1], ng.dec[l][0], ng.dec[l][1], ng.dec[l][2],
+                      (const float4 *)a.P[l - 1], (const float4 *)a.W[l - 1],
+                      (const float4 *)a.M[l - 1], a.P[l], a.W[l], a.M[l]));
+    }
+    return VG_OK;
+}
 
 // Used by the simulation driver.
 TreeView tree_view(const vg_tree *t) { return make_view(t); }
@@ -694,7 +768,7 @@ VG_API vg_status vg_tree_build(vg_tree *t, const vg_grid_desc *tmpl, const vg_vo
     if (n3 < t->n_cap) t->n_cap = n3;
     if (n4 < t->n_cap) t->n_cap = n4;
     if (n5 < t->n_cap) t->n_cap = n5;
-    VG_TRY(grow(&t->scan_tmp, &t->scan_cap, scan_scratch_elems(cap0)));
+    VG_TRY(grow(&t->scan_tmp, &t->scan_cap, scan_elems(cap0)));
     t->n = (uint32_t)n;
 
     const VortonIn vin{v->pos, v->angvel, v->size, v->stride, v->uniform_size};

@@ -37,7 +37,10 @@ EXPORTS = (
     "vg_sim_vector_potential vg_sim_negative_vorticity vg_sim_grid_template vg_sim_bounds "
     "vg_sim_tree vg_sim_spreading vg_sim_poisson_residual "
     "vg_mlp_create vg_mlp_destroy vg_mlp_set_params vg_mlp_get_params vg_mlp_forward "
-    "vg_mlp_gradients vg_mlp_train vg_tree_mlp_samples vg_tree_velocity_grid_mlp vg_sim_set_mlp"
+    "vg_mlp_gradients vg_mlp_train vg_tree_mlp_samples vg_tree_velocity_grid_mlp vg_sim_set_mlp "
+    "vg_batch_config_default vg_batch_create vg_batch_destroy vg_batch_buffers vg_batch_step "
+    "vg_batch_capture vg_batch_launch vg_batch_grid vg_batch_mollify vg_batch_set_launch "
+    "vg_batch_get_launch vg_batch_autotune"
 ).split()
 
 MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS, MLP_PARAMS = 16, 64, 3, 9603
@@ -455,6 +458,91 @@ class Multigrid:
         try:
             if self.h:
                 lib().vg_mg_destroy(self.h)
+        except Exception:
+            pass
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("cells", C.c_uint32), ("near_k", C.c_int), ("order", C.c_int),
+                ("rsqrt_policy", C.c_int)]
+
+
+class Batch:
+    """Many small effects advanced together (vg_batch)."""
+
+    def __init__(self, vorton_counts, tracer_counts, radii, cells=32, near_k=2, order=1,
+                 policy=1):
+        cfg = BatchConfig()
+        call("vg_batch_config_default", C.byref(cfg))
+        cfg.cells, cfg.near_k, cfg.order, cfg.rsqrt_policy = cells, near_k, order, policy
+        vc = np.ascontiguousarray(vorton_counts, np.uint32)
+        tc = np.ascontiguousarray(tracer_counts, np.uint32)
+        rr = np.ascontiguousarray(radii, np.float32)
+        self.effects = len(vc)
+        self.vorton_counts, self.tracer_counts = vc.copy(), tc.copy()
+        self.h = C.c_void_p()
+        call("vg_batch_create", C.byref(self.h), C.c_uint32(len(vc)), vc.ctypes.data_as(C.c_void_p),
+             tc.ctypes.data_as(C.c_void_p), rr.ctypes.data_as(C.c_void_p), C.byref(cfg))
+        ptrs = [C.c_void_p() for _ in range(4)]
+        nv, nt = C.c_size_t(), C.c_size_t()
+        call("vg_batch_buffers", self.h, *[C.byref(p) for p in ptrs], C.byref(nv), C.byref(nt))
+        self.nv, self.nt = nv.value, nt.value
+        self._ptr = {k: p.value for k, p in zip(("vpos", "vom", "vvel", "tpos"), ptrs)}
+
+    def upload(self, vpos, vom, tpos):
+        for key, a, n in (("vpos", vpos, self.nv), ("vom", vom, self.nv), ("tpos", tpos, self.nt)):
+            a = np.ascontiguousarray(a, np.float32).reshape(-1, 3)
+            assert len(a) == n
+            if n:
+                call("vg_memcpy_htod", C.c_void_p(self._ptr[key]), a.ctypes.data_as(C.c_void_p),
+                     C.c_size_t(a.nbytes))
+
+    def download(self, key):
+        n = self.nt if key == "tpos" else self.nv
+        out = np.zeros((n, 3), np.float32)
+        if n:
+            synchronize()
+            call("vg_memcpy_dtoh", out.ctypes.data_as(C.c_void_p), C.c_void_p(self._ptr[key]),
+                 C.c_size_t(out.nbytes))
+        return out
+
+    def step(self, dt, stream=None):
+        call("vg_batch_step", self.h, C.c_float(dt), _s(stream))
+
+    def capture(self, dt, stream):
+        call("vg_batch_capture", self.h, C.c_float(dt), _s(stream))
+
+    def launch(self, stream=None):
+        call("vg_batch_launch", self.h, _s(stream))
+
+    def grid(self, effect):
+        g = GridDesc()
+        f = C.c_void_p()
+        call("vg_batch_grid", self.h, C.c_uint32(effect), C.byref(g), C.byref(f))
+        return g, fetch(f, 3 * g.capacity).reshape(-1, 3)
+
+    def mollify(self, effect):
+        s, c = C.c_float(), C.c_float()
+        call("vg_batch_mollify", self.h, C.c_uint32(effect), C.byref(s), C.byref(c))
+        return s.value, c.value
+
+    def set_launch(self, walk_waves, tracer_block, tracer_items):
+        call("vg_batch_set_launch", self.h, int(walk_waves), int(tracer_block), int(tracer_items))
+
+    def get_launch(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        call("vg_batch_get_launch", self.h, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
+    def autotune(self, stream=None) -> str:
+        buf = C.create_string_buffer(8192)
+        call("vg_batch_autotune", self.h, _s(stream), buf, C.c_size_t(len(buf)))
+        return buf.value.decode()
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().vg_batch_destroy(self.h)
         except Exception:
             pass
 
