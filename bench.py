@@ -31,6 +31,14 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FLOP_PER_PAIR = 26.0   # monopole / direct pair (FMA = 2, rsqrt = 1)
 FLOP_DIPOLE = 42.0     # first-moment correction of one cluster interaction
+# brick-centred local expansion of one cluster (FMA = 2): monopole to second
+# order (offset and distance powers 20, Omega x d 9, value 6, gradient 42,
+# Hessian 150) and first moment to first order (moments and t x d 51,
+# gradient 99); per grid point, the 30-coefficient wave sum (180) and the
+# polynomial (60)
+FLOP_EXPAND_MONO = 227.0
+FLOP_EXPAND_DIPOLE = 150.0
+FLOP_EXPAND_EVAL = 240.0
 BF16_PEAK_TFLOPS = 2500.0
 MLP_FLOP = 2.0 * (16 * 64 + 64 * 64 + 64 * 64 + 64 * 3)  # per (point, cluster) pair
 
@@ -50,6 +58,9 @@ def main():
     ap.add_argument("--near-k", type=int, default=3,
                     help="near-field radius of the accurate treecode; -1 = reference opening rule")
     ap.add_argument("--order", type=int, default=1, help="cluster order: 0 monopole, 1 dipole")
+    ap.add_argument("--expand-from", type=int, default=-1,
+                    help="near-field treecode: first layer applied through brick-centred local "
+                         "expansions (-1 = none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--suite", action="store_true",
@@ -92,7 +103,8 @@ def main():
     d_tr = vg.DeviceArray.from_numpy(tr)
     v = vg.vortons_view(d_pos, d_om, len(pos), uniform_size=2 * radius)
     sim = vg.Sim(vg.sim_config(technique=vg.TECH_TREE, rsqrt_policy=args.policy,
-                               tree_near_k=args.near_k, tree_order=args.order), stream)
+                               tree_near_k=args.near_k, tree_order=args.order,
+                               tree_expand_from=args.expand_from), stream)
     sim.set_vortons(v)
     mlp_info = None
     if args.mlp:
@@ -162,19 +174,22 @@ def main():
     peak, note = FP32_PEAK_TFLOPS, "FP32 VALU kernel (no MFMA); peak is the published FP32 vector peak"
     if args.mlp:
         kname = "vg_tree_velocity_grid_mlp"
-        clusters, _ = t.mac_work(g, args.near_k, stream)
-        pairs = 0
+        clusters, _, _ = t.mac_work(g, args.near_k, -1, stream)
+        pairs = expansions = 0
         flop_cluster = MLP_FLOP
         peak, note = BF16_PEAK_TFLOPS, "BF16 MFMA with FP32 accumulation; dense BF16 peak"
     elif args.near_k < 0:
         kname = "vg_tree_velocity_grid"
-        clusters, pairs = t.interaction_count(g, stream), 0
+        clusters, pairs, expansions = t.interaction_count(g, stream), 0, 0
         flop_cluster = FLOP_PER_PAIR
     else:
         kname = "vg_tree_velocity_grid_mac"
-        clusters, pairs = t.mac_work(g, args.near_k, stream)
+        clusters, pairs, expansions = t.mac_work(g, args.near_k, args.expand_from, stream)
         flop_cluster = FLOP_PER_PAIR + (FLOP_DIPOLE if args.order else 0.0)
-    flops = flop_cluster * clusters + FLOP_PER_PAIR * pairs
+    flop_expand = FLOP_EXPAND_MONO + (FLOP_EXPAND_DIPOLE if args.order else 0.0)
+    flops = flop_cluster * clusters + FLOP_PER_PAIR * pairs + flop_expand * expansions
+    if expansions:
+        flops += FLOP_EXPAND_EVAL * g.capacity
     cnt, ms = prof.get(kname, (0, 0.0))
     avg_s = (ms / max(cnt, 1)) * 1e-3
     achieved = flops / avg_s / 1e12 if avg_s > 0 else 0.0
@@ -183,6 +198,7 @@ def main():
                 "traffic": measured_traffic(args),
                 "avg_launch_ms": ms / max(cnt, 1), "cluster_interactions_per_launch": clusters,
                 "direct_pairs_per_launch": pairs, "flop_per_cluster": flop_cluster,
+                "clusters_expanded_per_launch": expansions, "flop_per_expansion": flop_expand,
                 "flop_per_pair": FLOP_PER_PAIR, "note": note}
     # tracer advection (HBM bound): 24 B per tracer (pos read + write)
     acnt, ams = prof.get("vg_advect_tracers", (0, 0.0))
@@ -288,11 +304,13 @@ def main():
                        f"near-field treecode near_k={args.near_k} monopole clusters + learned "
                        f"correction" if args.mlp else
                        f"near-field treecode near_k={args.near_k} "
-                       f"{'dipole' if args.order else 'monopole'} clusters") +
+                       f"{'dipole' if args.order else 'monopole'} clusters" +
+                       (f", local expansions from layer {args.expand_from}"
+                        if args.expand_from >= 1 else "")) +
                        ", auto-mollified) + 4M-tracer advection"),
                    "vortons": len(pos), "tracers": len(tr), "grid": list(g.shape),
                    "scenario": "64 random rings seed 5", "rsqrt_policy": args.policy,
-                   "near_k": args.near_k, "order": args.order,
+                   "near_k": args.near_k, "order": args.order, "expand_from": args.expand_from,
                    "parallelism": "single GPU"},
         "rel_l2_vs_fp64_direct": rel,
         "learned_correction": mlp_info,

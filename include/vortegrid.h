@@ -189,12 +189,19 @@ vg_status vg_tree_interaction_count(const vg_tree *t, const vg_grid_desc *g,
  * centroid, order 1: plus the first-moment correction), and the leaf near
  * box is summed over its vortons. */
 vg_status vg_tree_velocity_grid_mac(const vg_tree *t, const vg_grid_desc *g, int near_k,
-                                    int order, float spread, float circ, int rsqrt_policy,
-                                    float *out, vg_stream s);
+                                    int order, int expand_from, float spread, float circ,
+                                    int rsqrt_policy, float *out, vg_stream s);
+/* expand_from (>= 1; < 1 or > depth-2 disables it): clusters of layers >=
+ * expand_from are applied through a local expansion about the brick centre
+ * (monopole to second order, first moment to first order), one per lane and
+ * summed over the wave, instead of at every point; 2 or 3 keeps the error of
+ * the near-field treecode within ~10%. */
 /* Work of vg_tree_velocity_grid_mac summed over all grid points: cluster
- * interactions and direct vorton pairs.  Blocks. */
-vg_status vg_tree_mac_work(const vg_tree *t, const vg_grid_desc *g, int near_k,
-                           uint64_t *clusters, uint64_t *pairs, vg_stream s);
+ * interactions and direct vorton pairs at points, and clusters expanded
+ * (once per brick; may be NULL).  Blocks. */
+vg_status vg_tree_mac_work(const vg_tree *t, const vg_grid_desc *g, int near_k, int expand_from,
+                           uint64_t *clusters, uint64_t *pairs, uint64_t *expansions,
+                           vg_stream s);
 /* ComputeVectorPotential_Tree (vortonSim.cpp:1650-1768) at query points. */
 vg_status vg_tree_vecpot(const vg_tree *t, const float *q, size_t q_stride,
                          size_t nq, float spread, float circ, int rsqrt_policy,
@@ -354,6 +361,7 @@ typedef struct vg_batch_config {
     int near_k;        /* near-field treecode radius (default 2) */
     int order;         /* cluster order: 0 monopole, 1 dipole (default 1) */
     int rsqrt_policy;  /* VG_RSQRT_* (default ACCURATE) */
+    int expand_from;   /* local expansions from this layer (default -1: off) */
 } vg_batch_config;
 vg_status vg_batch_config_default(vg_batch_config *cfg);
 /* Host arrays of `effects` entries; tracer_counts may be NULL (none). */
@@ -407,6 +415,9 @@ typedef struct vg_sim_config {
      * that near radius and cluster order tree_order (0 monopole, 1 dipole). */
     int tree_near_k;
     int tree_order;
+    /* near-field treecode: first layer applied through brick-centred local
+     * expansions (see vg_tree_velocity_grid_mac); < 1 disables. */
+    int tree_expand_from;
 } vg_sim_config;
 vg_status vg_sim_config_default(vg_sim_config *cfg);
 vg_status vg_sim_create(vg_sim **sim, const vg_sim_config *cfg, vg_stream s);

@@ -131,6 +131,13 @@ def lib():
                                                    f32p, C.c_int, C.c_int, C.c_float, C.c_float,
                                                    C.c_int, f32p, C.c_int, C.c_uint32,
                                                    C.c_uint32]
+        L.vgo_mac_velocity_grid_ex.argtypes = [C.POINTER(Tree), C.POINTER(Grid), f32p, f32p,
+                                               f32p, C.c_int, C.c_int, C.c_int, C.c_float,
+                                               C.c_float, C.c_int, f32p, C.c_int, C.c_uint32,
+                                               C.c_uint32]
+        L.vgo_local_expansion.argtypes = [f32p, f32p, f32p, C.c_float, f32p, C.c_int, C.c_float,
+                                          C.POINTER(C.c_double)]
+        L.vgo_local_expansion.restype = None
         L.vgo_vecpot_boundary.argtypes = [C.POINTER(Tree), C.POINTER(Grid), f32p, f32p, f32p,
                                           C.c_size_t, C.c_float, C.c_float, C.c_int, C.c_int,
                                           f32p, C.c_int]
@@ -361,6 +368,30 @@ def mac_velocity_grid_bricks(tree, g: Grid, pos, om, size, near_k, order, spread
                                        _size_arr(size, n), near_k, order, spread, circ, policy,
                                        out, threads, bz_begin, bz_end)
     return out.reshape(-1, 3)
+
+
+def mac_velocity_grid_ex(tree, g: Grid, pos, om, size, near_k, order, expand_from, spread=1.0,
+                         circ=1.0, policy=1, threads=1, bz_begin=0, bz_end=None):
+    """Near-field treecode with brick-centred local expansions from layer
+    expand_from (z-brick rows [bz_begin, bz_end))."""
+    pos = _f32(pos).reshape(-1)
+    n = pos.size // 3
+    out = np.zeros(g.capacity * 3, np.float32)
+    if bz_end is None:
+        bz_end = (g.np[2] + 3) // 4
+    lib().vgo_mac_velocity_grid_ex(tree.ptr(), C.byref(g), pos, _f32(om).reshape(-1),
+                                   _size_arr(size, n), near_k, order, expand_from, spread, circ,
+                                   policy, out, threads, bz_begin, bz_end)
+    return out.reshape(-1, 3)
+
+
+def local_expansion(cb, cp, w, size, D, order=1, spread=1.0):
+    """(c0 [3], c1 [3,3], c2 [3,3,3]) of one cluster's field about cb."""
+    out = np.zeros(39, np.float64)
+    lib().vgo_local_expansion(_f32(cb), _f32(cp), _f32(w), C.c_float(size),
+                              _f32(np.asarray(D).reshape(-1)), int(order), C.c_float(spread),
+                              out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out[:3], out[3:12].reshape(3, 3), out[12:].reshape(3, 3, 3)
 
 
 def vecpot_boundary(tree, g: Grid, pos, om, size, spread=1.0, circ=1.0, policy=0,

@@ -500,11 +500,102 @@ typedef struct {
     const vgo_tree *t;
     const vgo_grid *g;
     const float *pos, *om, *size;
-    int k, order, policy;
+    int k, order, policy, expand_from;
     float spread, circ;
     float *out;
     uint32_t nb[3];
 } mac_ctx;
+
+/* Local expansion about a brick centre (no reference counterpart), in
+ * float64 from the tensors of K(d) = d/|d|^3:
+ *   J_ij  = d_j K_i  = delta_ij/r^3 - 3 d_i d_j/r^5
+ *   T_ijk = d_k J_ij = -3 (delta_ij d_k + delta_ik d_j + delta_jk d_i)/r^5
+ *                      + 15 d_i d_j d_k/r^7
+ * u(x) = s [Om x K(d) - sum_be D_be e_b x d_e K(d)], d = x - c: the monopole
+ * term is kept to second order in x - c_b, the first-moment term to first
+ * order.  c0[i], c1[i][j] = du_i/dx_j, c2[i][j][k] = d2u_i/dx_j dx_k. */
+typedef struct { double c0[3], c1[3][3], c2[3][3][3]; } expansion;
+
+static void cross_d(const double a[3], const double b[3], double o[3]) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+static void expansion_add(expansion *E, const float cb[3], const float *cp, const float *w,
+                          float size, const float *D, int order, float spread) {
+    double d[3], J[3][3], T[3][3][3];
+    for (int a = 0; a < 3; ++a) d[a] = (double)cb[a] - (double)cp[a];
+    const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const double r = sqrt(r2), ir3 = 1.0 / (r2 * r), ir5 = ir3 / r2, ir7 = ir5 / r2;
+    const double radius = (double)size * 0.5 * (double)spread;
+    const double s = (2.0 / 3.0) * radius * radius * radius;
+    double K[3];
+    for (int i = 0; i < 3; ++i) {
+        K[i] = d[i] * ir3;
+        for (int j = 0; j < 3; ++j) {
+            J[i][j] = (i == j ? ir3 : 0.0) - 3.0 * d[i] * d[j] * ir5;
+            for (int k = 0; k < 3; ++k)
+                T[i][j][k] = -3.0 * ((i == j ? d[k] : 0.0) + (i == k ? d[j] : 0.0) +
+                                     (j == k ? d[i] : 0.0)) * ir5 +
+                             15.0 * d[i] * d[j] * d[k] * ir7;
+        }
+    }
+    const double om[3] = {s * w[0], s * w[1], s * w[2]};
+    double v[3], o[3];
+    cross_d(om, K, o);
+    for (int i = 0; i < 3; ++i) E->c0[i] += o[i];
+    for (int j = 0; j < 3; ++j) {
+        for (int i = 0; i < 3; ++i) v[i] = J[i][j];
+        cross_d(om, v, o);
+        for (int i = 0; i < 3; ++i) E->c1[i][j] += o[i];
+        for (int k = 0; k < 3; ++k) {
+            for (int i = 0; i < 3; ++i) v[i] = T[i][j][k];
+            cross_d(om, v, o);
+            for (int i = 0; i < 3; ++i) E->c2[i][j][k] += o[i];
+        }
+    }
+    if (order >= 1) {
+        /* -s sum_b e_b x (sum_e D_be dK/dx_e) and its gradient */
+        for (int b = 0; b < 3; ++b) {
+            double eb[3] = {0.0, 0.0, 0.0};
+            eb[b] = 1.0;
+            double g0[3] = {0.0, 0.0, 0.0};
+            for (int e = 0; e < 3; ++e)
+                for (int i = 0; i < 3; ++i) g0[i] += s * (double)D[3 * b + e] * J[i][e];
+            cross_d(eb, g0, o);
+            for (int i = 0; i < 3; ++i) E->c0[i] -= o[i];
+            for (int j = 0; j < 3; ++j) {
+                double g1[3] = {0.0, 0.0, 0.0};
+                for (int e = 0; e < 3; ++e)
+                    for (int i = 0; i < 3; ++i) g1[i] += s * (double)D[3 * b + e] * T[i][e][j];
+                cross_d(eb, g1, o);
+                for (int i = 0; i < 3; ++i) E->c1[i][j] -= o[i];
+            }
+        }
+    }
+}
+
+void vgo_local_expansion(const float cb[3], const float cp[3], const float w[3], float size,
+                         const float D[9], int order, float spread, double out[39]) {
+    expansion E;
+    memset(&E, 0, sizeof E);
+    expansion_add(&E, cb, cp, w, size, D, order, spread);
+    memcpy(out, &E, sizeof E);
+}
+
+static void expansion_eval(const expansion *E, const float cb[3], const float q[3], float out[3]) {
+    double x[3];
+    for (int a = 0; a < 3; ++a) x[a] = (double)q[a] - (double)cb[a];
+    for (int i = 0; i < 3; ++i) {
+        double u = E->c0[i];
+        for (int j = 0; j < 3; ++j) {
+            u += E->c1[i][j] * x[j];
+            for (int k = 0; k < 3; ++k) u += 0.5 * E->c2[i][j][k] * x[j] * x[k];
+        }
+        out[i] = (float)u;
+    }
+}
 
 static void mac_brick(mac_ctx *c, uint32_t bx, uint32_t by, uint32_t bz) {
     const vgo_tree *t = c->t;
@@ -528,6 +619,48 @@ static void mac_brick(mac_ctx *c, uint32_t bx, uint32_t by, uint32_t bz) {
     for (int a = 0; a < 3; ++a) scale[0][a] = 1;
     for (uint32_t l = 1; l < depth; ++l)
         for (int a = 0; a < 3; ++a) scale[l][a] = scale[l - 1][a] * t->ng.dec[l][a];
+    /* brick-level local expansion of the layers >= expand_from */
+    const int expanding = c->expand_from >= 1 && c->expand_from <= (int)depth - 2;
+    expansion E;
+    memset(&E, 0, sizeof E);
+    float cb[3];
+    {
+        const float nudge = 1.0f - 2.0f * FLT_EPSILON;
+        for (int a = 0; a < 3; ++a)
+            cb[a] = g->min[a] + ((float)(4 * bb[a]) + 1.5f) * (g->cell[a] * nudge);
+    }
+    if (expanding) {
+        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        for (int l = (int)depth - 2; l >= c->expand_from; --l) {
+            const vgo_grid *gl = &t->ng.layer[l];
+            const uint32_t *dec = t->ng.dec[l + 1];
+            int c0[3], c1[3], n0[3], n1[3];
+            for (int a = 0; a < 3; ++a) {
+                c0[a] = lo[a] * (int)dec[a];
+                c1[a] = hi[a] * (int)dec[a] + (int)dec[a] - 1;
+                const int ncl = (int)gl->np[a] - 1;
+                int blo = cl0[a] / (int)scale[l][a] - c->k;
+                int bhi = cl1[a] / (int)scale[l][a] + c->k;
+                if (blo < 0) blo = 0;
+                if (bhi > ncl - 1) bhi = ncl - 1;
+                n0[a] = blo > c0[a] ? blo : c0[a];
+                n1[a] = bhi < c1[a] ? bhi : c1[a];
+            }
+            for (int cz = c0[2]; cz <= c1[2]; ++cz)
+                for (int cy = c0[1]; cy <= c1[1]; ++cy)
+                    for (int cx = c0[0]; cx <= c1[0]; ++cx) {
+                        if (cx >= n0[0] && cx <= n1[0] && cy >= n0[1] && cy <= n1[1] &&
+                            cz >= n0[2] && cz <= n1[2])
+                            continue;
+                        const size_t off = cx + (size_t)gl->np[0] * (cy + (size_t)gl->np[1] * cz);
+                        const float sz = t->size[l][off];
+                        if (sz == 0.0f) continue;
+                        expansion_add(&E, cb, t->pos[l] + 3 * off, t->om[l] + 3 * off, sz,
+                                      t->dip[l] + 9 * off, c->order, c->spread);
+                    }
+            for (int a = 0; a < 3; ++a) { lo[a] = n0[a]; hi[a] = n1[a]; }
+        }
+    }
     for (uint32_t pz = b0[2]; pz <= b1[2]; ++pz)
         for (uint32_t py = b0[1]; py <= b1[1]; ++py)
             for (uint32_t px = b0[0]; px <= b1[0]; ++px) {
@@ -556,6 +689,15 @@ static void mac_brick(mac_ctx *c, uint32_t bx, uint32_t by, uint32_t bz) {
                         if (bhi > ncl - 1) bhi = ncl - 1;
                         n0[a] = blo > c0[a] ? blo : c0[a];
                         n1[a] = bhi < c1[a] ? bhi : c1[a];
+                    }
+                    if (expanding && l >= c->expand_from) {
+                        for (int a = 0; a < 3; ++a) { lo[a] = n0[a]; hi[a] = n1[a]; }
+                        if (l == c->expand_from) {
+                            float ue[3];
+                            expansion_eval(&E, cb, q, ue);
+                            kahan3(sum, comp, ue);
+                        }
+                        continue;
                     }
                     float part[3] = {0, 0, 0};
                     for (int cz = c0[2]; cz <= c1[2]; ++cz)
@@ -607,19 +749,27 @@ static void mac_slab_range(void *vs, size_t b, size_t e) {
     mac_range(s->c, s->bz0 + b, s->bz0 + e);
 }
 
-void vgo_mac_velocity_grid_bricks(const vgo_tree *t, const vgo_grid *g, const float *pos,
-                                  const float *om, const float *size, int near_k, int order,
-                                  float spread, float circ, int rsqrt_policy, float *out,
-                                  int nthreads, uint32_t bz_begin, uint32_t bz_end) {
+void vgo_mac_velocity_grid_ex(const vgo_tree *t, const vgo_grid *g, const float *pos,
+                              const float *om, const float *size, int near_k, int order,
+                              int expand_from, float spread, float circ, int rsqrt_policy,
+                              float *out, int nthreads, uint32_t bz_begin, uint32_t bz_end) {
     mac_ctx c;
     c.t = t; c.g = g; c.pos = pos; c.om = om; c.size = size;
-    c.k = near_k; c.order = order; c.policy = rsqrt_policy;
+    c.k = near_k; c.order = order; c.policy = rsqrt_policy; c.expand_from = expand_from;
     c.spread = spread; c.circ = circ; c.out = out;
     for (int a = 0; a < 3; ++a) c.nb[a] = (g->np[a] + 3) / 4;
     if (bz_end > c.nb[2]) bz_end = c.nb[2];
     if (bz_begin >= bz_end) return;
     mac_slab s = {&c, bz_begin};
     parallel_range(bz_end - bz_begin, nthreads, mac_slab_range, &s);
+}
+
+void vgo_mac_velocity_grid_bricks(const vgo_tree *t, const vgo_grid *g, const float *pos,
+                                  const float *om, const float *size, int near_k, int order,
+                                  float spread, float circ, int rsqrt_policy, float *out,
+                                  int nthreads, uint32_t bz_begin, uint32_t bz_end) {
+    vgo_mac_velocity_grid_ex(t, g, pos, om, size, near_k, order, -1, spread, circ, rsqrt_policy,
+                             out, nthreads, bz_begin, bz_end);
 }
 
 void vgo_mac_velocity_grid(const vgo_tree *t, const vgo_grid *g, const float *pos,

@@ -131,6 +131,20 @@ void vgo_mac_velocity_grid_bricks(const vgo_tree *t, const vgo_grid *g, const fl
                                   float spread, float circ, int rsqrt_policy, float *out,
                                   int nthreads, uint32_t bz_begin, uint32_t bz_end);
 
+/* Same with clusters of layers >= expand_from applied through a local
+ * expansion about each brick's centre (no reference counterpart; monopole
+ * to second order, first moment to first order, float64 tensors);
+ * expand_from < 1 or > depth-2 disables it. */
+void vgo_mac_velocity_grid_ex(const vgo_tree *t, const vgo_grid *g, const float *pos,
+                              const float *om, const float *size, int near_k, int order,
+                              int expand_from, float spread, float circ, int rsqrt_policy,
+                              float *out, int nthreads, uint32_t bz_begin, uint32_t bz_end);
+
+/* The local expansion of one cluster about cb: out = c0[3], c1[3][3],
+ * c2[3][3][3] (see vgo_mac_velocity_grid_ex). */
+void vgo_local_expansion(const float cb[3], const float cp[3], const float w[3], float size,
+                         const float D[9], int order, float spread, double out[39]);
+
 /* Auto-mollification (vortonSim.cpp:2458-2477). */
 void vgo_auto_mollify(const vgo_grid *vel_grid, float vorton_radius,
                       float *spread, float *circ);

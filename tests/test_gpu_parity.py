@@ -434,6 +434,29 @@ def test_mac_velocity_grid(vg, oracle, rings2, k, order, policy):
     assert rel_l2(out.numpy(), want) < 1e-5
 
 
+@pytest.mark.parametrize("k,order,expand_from", [(1, 1, 2), (2, 1, 3), (1, 0, 2), (2, 1, 99)])
+def test_mac_local_expansions(vg, oracle, rings2, k, order, expand_from):
+    pos, om, radius, og = rings2
+    case(f"mac velocity grid with local expansions k={k} order={order} from={expand_from}")
+    v, keep = vortons(vg, pos, om, radius)
+    g = vg.GridDesc.from_fields(og)
+    t = vg.Tree()
+    t.build(g, v)
+    spread, circ = vg.auto_mollify(g, radius)
+    out = vg.DeviceArray.zeros((g.capacity, 3))
+    t.velocity_grid_mac(g, out, k, order, spread, circ, 1, expand_from=expand_from)
+    vg.synchronize()
+    ot = oracle.InfluenceTree(og, pos, om, 2 * radius)
+    want = oracle.mac_velocity_grid_ex(ot, og, pos, om, 2 * radius, k, order, expand_from, spread,
+                                       circ, 1, threads=8)
+    assert rel_l2(out.numpy(), want) < 1e-5
+    c, p, e = t.mac_work(g, k, expand_from)
+    c0, p0, e0 = t.mac_work(g, k, -1)
+    assert p == p0 and e0 == 0
+    if expand_from <= t.depth() - 2:
+        assert e > 0 and c < c0
+
+
 def test_mac_work_and_accuracy(vg, oracle):
     pos, om, radius = scenarios.random_rings(32768, seed=33, rings=16)
     case("mac work counts + accuracy vs fp64 N=32768")
@@ -443,8 +466,8 @@ def test_mac_work_and_accuracy(vg, oracle):
     t = vg.Tree()
     t.build(g, v)
     spread, circ = vg.auto_mollify(g, radius)
-    c1, p1 = t.mac_work(g, 1)
-    c2, p2 = t.mac_work(g, 2)
+    c1, p1, _ = t.mac_work(g, 1)
+    c2, p2, _ = t.mac_work(g, 2)
     assert c2 > c1 > 0 and p2 > p1 > 0
     out = vg.DeviceArray.zeros((g.capacity, 3))
     t.velocity_grid_mac(g, out, 2, 1, spread, circ, 1)

@@ -137,3 +137,17 @@ def test_accepted_clusters_restate_the_walk(oracle, small_tree, near_k):
                                            policy=1)[0]
         o = pt[0] + g.np[0] * (pt[1] + g.np[1] * pt[2])
         assert np.allclose(u, want[o], rtol=1e-4, atol=1e-6 * np.abs(want).max())
+
+
+def test_folded_standardisation_is_the_same_network():
+    """Training on standardised inputs / targets and folding the scalings
+    into the first and last layers gives the network on raw values."""
+    from vortegrid_amd import learned
+
+    rng = np.random.default_rng(9)
+    p = M.init_params(2) + 0.05 * rng.standard_normal(M.N_PARAMS)
+    X = rng.standard_normal((50, M.N_IN)) * 10.0 ** rng.uniform(-3, 1, M.N_IN) + rng.standard_normal(M.N_IN)
+    mu, sigma, ts = X.mean(0), X.std(0), 3.7e-3
+    want = M.forward(p, (X - mu) / sigma)[0] * ts
+    got = M.forward(learned.fold_scalings(p, mu, sigma, ts).astype(np.float64), X)[0]
+    assert np.allclose(got, want, rtol=2e-4, atol=1e-6 * np.abs(want).max())

@@ -450,3 +450,77 @@ def test_mac_accuracy_improves_with_order_and_near_radius(oracle):
     assert err[1, 1] < 0.5 * err[1, 0] and err[2, 1] < 0.5 * err[2, 0]
     assert err[2, 1] < err[1, 1] and err[2, 0] < err[1, 0]
     assert err[2, 1] < 5e-3
+
+
+# ------------------------------------------------------------ local expansions
+@pytest.fixture(scope="module")
+def rings2(oracle):
+    from vortegrid_amd import scenarios
+
+    pos, om, radius = scenarios.colliding_rings(16384, seed=2)
+    return pos, om, radius, oracle.grid_for(pos, radius)
+
+
+def _cluster_field(q, cp, w, size, D, order):
+    """Monopole + first-moment far field of one cluster, float64."""
+    d = np.asarray(q, np.float64) - np.asarray(cp, np.float64)
+    r2 = d @ d
+    r = np.sqrt(r2)
+    s = (2.0 / 3.0) * (0.5 * size) ** 3
+    u = s * np.cross(np.asarray(w, np.float64), d) / r ** 3
+    if order:
+        D = np.asarray(D, np.float64).reshape(3, 3)
+        a = np.array([D[1, 2] - D[2, 1], D[2, 0] - D[0, 2], D[0, 1] - D[1, 0]])
+        u -= s * (a - 3.0 / r2 * np.cross(D @ d, d)) / r ** 3
+    return u
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_local_expansion_matches_derivatives(oracle, order):
+    """Value, gradient and (monopole) Hessian of the expansion equal the
+    cluster field and its central differences at the expansion centre."""
+    rng = np.random.default_rng(order)
+    for _ in range(10):
+        cp = rng.uniform(-1, 1, 3).astype(np.float32)
+        cb = (cp + rng.uniform(3, 6) * rng.standard_normal(3) / np.sqrt(3)).astype(np.float32)
+        w = rng.standard_normal(3).astype(np.float32)
+        D = (0.3 * rng.standard_normal(9)).astype(np.float32)
+        size = 0.2
+        c0, c1, c2 = oracle.local_expansion(cb, cp, w, size, D, order)
+        f = lambda x, o=order: _cluster_field(x, cp, w, size, D, o)
+        x0 = cb.astype(np.float64)
+        assert np.allclose(c0, f(x0), rtol=1e-6, atol=1e-12)
+        h = 1e-4
+        E = np.eye(3)
+        for j in range(3):
+            g = (f(x0 + h * E[j]) - f(x0 - h * E[j])) / (2 * h)
+            assert np.allclose(c1[:, j], g, rtol=1e-5, atol=1e-9 * np.abs(c1).max())
+            for k in range(3):
+                fm = lambda x: _cluster_field(x, cp, w, size, D, 0)
+                hess = (fm(x0 + h * (E[j] + E[k])) - fm(x0 + h * (E[j] - E[k])) -
+                        fm(x0 - h * (E[j] - E[k])) + fm(x0 - h * (E[j] + E[k]))) / (4 * h * h)
+                assert np.allclose(c2[:, j, k], hess, rtol=2e-4, atol=1e-6 * np.abs(c2).max())
+
+
+def test_mac_expansion_small_change(oracle, rings2):
+    """Expanding the upper layers changes the near-field treecode little,
+    and leaves its error against the FP64 direct sum nearly unchanged;
+    expand_from out of range is the plain walk bit for bit."""
+    pos, om, radius, g = rings2
+    t = oracle.InfluenceTree(g, pos, om, 2 * radius)
+    spread, circ = oracle.auto_mollify(g, radius)
+    base = oracle.mac_velocity_grid(t, g, pos, om, 2 * radius, 1, 1, spread, circ, 1, threads=8)
+    off = oracle.mac_velocity_grid_ex(t, g, pos, om, 2 * radius, 1, 1, 99, spread, circ, 1,
+                                      threads=8)
+    assert np.array_equal(base, off)
+    ex = oracle.mac_velocity_grid_ex(t, g, pos, om, 2 * radius, 1, 1, 2, spread, circ, 1,
+                                     threads=8)
+    rel = np.linalg.norm(ex - base) / np.linalg.norm(base)
+    assert 0 < rel < 5e-3
+    rng = np.random.default_rng(0)
+    sel = rng.choice(g.capacity, 300, replace=False)
+    q = oracle.query_points(g).reshape(-1, 3)[sel]
+    ref = oracle.direct_velocity_f64(pos, om, 2 * radius, q, spread, circ, threads=8)
+    e_base = np.linalg.norm(base[sel] - ref) / np.linalg.norm(ref)
+    e_ex = np.linalg.norm(ex[sel] - ref) / np.linalg.norm(ref)
+    assert e_ex < 1.25 * e_base + 1e-4

@@ -29,8 +29,8 @@
 
 namespace vg {
 vg_status mac_velocity_grid_batch(const TreeView *views, const float2 *mollify, uint32_t trees,
-                                  const uint32_t np[3], int near_k, int order, int policy,
-                                  int waves_per_block, float *out, size_t out_stride,
+                                  const uint32_t np[3], int near_k, int order, int expand_from,
+                                  int policy, int waves_per_block, float *out, size_t out_stride,
                                   hipStream_t s);
 }
 
@@ -292,7 +292,8 @@ vg_status frame(vg_batch *b, float dt, hipStream_t s) {
     }
     VG_TRY(batch_tree_build(ta, s));
     VG_TRY(mac_velocity_grid_batch(b->views, b->mollify, b->effects, b->shape.layer[0].np,
-                                   b->cfg.near_k, b->cfg.order, b->cfg.rsqrt_policy, b->walk_waves,
+                                   b->cfg.near_k, b->cfg.order, b->cfg.expand_from,
+                                   b->cfg.rsqrt_policy, b->walk_waves,
                                    b->grid, 3 * b->cap0, s));
     return advect(b, b->vpos, b->vvel, b->vtree, b->nv, dt, b->adv_block, b->adv_items, s);
 }
@@ -314,6 +315,7 @@ VG_API vg_status vg_batch_config_default(vg_batch_config *cfg) {
     cfg->near_k = 2;
     cfg->order = 1;
     cfg->rsqrt_policy = VG_RSQRT_ACCURATE;
+    cfg->expand_from = -1;
     return VG_OK;
 }
 
@@ -582,7 +584,8 @@ VG_API vg_status vg_batch_autotune(vg_batch *b, vg_stream stream, char *report, 
         float ms = 0.f;
         st = timed([&]() {
             return mac_velocity_grid_batch(b->views, b->mollify, b->effects, b->shape.layer[0].np,
-                                           b->cfg.near_k, b->cfg.order, b->cfg.rsqrt_policy, w,
+                                           b->cfg.near_k, b->cfg.order, b->cfg.expand_from,
+                                           b->cfg.rsqrt_policy, w,
                                            b->grid, 3 * b->cap0, s);
         }, &ms);
         if (st != VG_OK) break;

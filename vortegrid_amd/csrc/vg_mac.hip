@@ -29,6 +29,7 @@ struct MacParams {
     float circ;
     uint32_t nb[3];
     uint32_t nbricks;
+    int expand_from;  // layers >= expand_from go through brick-centred local expansions
 };
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -98,6 +99,121 @@ __device__ __forceinline__ void cluster_term(float3 &part, float3 q, float4 p, f
     }
 }
 
+// Local (Taylor) expansion about the brick centre of one cluster's field:
+// monopole to second order, first moment to first order.  With d = c_b - c,
+// K = d/|d|^3, u = s [Om x K(d) - sum_b e_b x (D_b . grad) K(d)]:
+//   c0     = u(c_b)
+//   c1[i][j] = du_i/dx_j                                 (9)
+//   c2[i][p] = d2u_i/dx_j dx_k, p = 00 11 22 01 02 12   (18)
+// Each lane expands its own cluster; the brick sums the lanes.
+struct Expansion {
+    float c0[3], c1[3][3], c2[3][6];
+};
+
+__device__ __forceinline__ void expansion_zero(Expansion &E) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        E.c0[i] = 0.f;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) E.c1[i][j] = 0.f;
+#pragma unroll
+        for (int p = 0; p < 6; ++p) E.c2[i][p] = 0.f;
+    }
+}
+
+__device__ __forceinline__ float3 cross3(float3 a, float3 b) {
+    return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+template <int ORDER>
+__device__ __forceinline__ void expansion_add(Expansion &E, float3 cb, float4 p, float4 w, float4 m0,
+                                              float4 m1, float4 m2, float spread) {
+    const float3 d = make_float3(cb.x - p.x, cb.y - p.y, cb.z - p.z);
+    const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float ir2 = ir * ir, ir3 = ir2 * ir, ir5 = ir3 * ir2, ir7 = ir5 * ir2;
+    const float radius = p.w * 0.5f * spread;
+    const float s = (2.0f / 3.0f) * radius * radius * radius;
+    const float3 om = make_float3(s * w.x, s * w.y, s * w.z);
+    const float dv[3] = {d.x, d.y, d.z};
+    // Om x e_j and Om x d
+    const float3 oe[3] = {make_float3(0.f, om.z, -om.y), make_float3(-om.z, 0.f, om.x),
+                          make_float3(om.y, -om.x, 0.f)};
+    const float3 od = cross3(om, d);
+    E.c0[0] += od.x * ir3;
+    E.c0[1] += od.y * ir3;
+    E.c0[2] += od.z * ir3;
+    // J[:, j] = e_j ir3 - 3 d d_j ir5
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const float f = -3.0f * ir5 * dv[j];
+        E.c1[0][j] += oe[j].x * ir3 + od.x * f;
+        E.c1[1][j] += oe[j].y * ir3 + od.y * f;
+        E.c1[2][j] += oe[j].z * ir3 + od.z * f;
+    }
+    // T[:, j, k] = -3 ir5 (e_j d_k + e_k d_j + delta_jk d) + 15 ir7 d_j d_k d
+    const int pj[6] = {0, 1, 2, 0, 0, 1}, pk[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int j = pj[q], k = pk[q];
+        const float a = -3.0f * ir5;
+        const float g = 15.0f * ir7 * dv[j] * dv[k] + (j == k ? a : 0.f);
+        E.c2[0][q] += a * (dv[k] * oe[j].x + dv[j] * oe[k].x) + g * od.x;
+        E.c2[1][q] += a * (dv[k] * oe[j].y + dv[j] * oe[k].y) + g * od.y;
+        E.c2[2][q] += a * (dv[k] * oe[j].z + dv[j] * oe[k].z) + g * od.z;
+    }
+    if (ORDER >= 1) {
+        // D rows (scaled), antisymmetric part a, t_b = D_b . d
+        const float3 D0 = make_float3(s * m0.x, s * m0.y, s * m0.z);
+        const float3 D1 = make_float3(s * m1.x, s * m1.y, s * m1.z);
+        const float3 D2 = make_float3(s * m2.x, s * m2.y, s * m2.z);
+        const float3 av = make_float3(s * m0.w, s * m1.w, s * m2.w);
+        const float3 t = make_float3(D0.x * d.x + D0.y * d.y + D0.z * d.z,
+                                     D1.x * d.x + D1.y * d.y + D1.z * d.z,
+                                     D2.x * d.x + D2.y * d.y + D2.z * d.z);
+        const float3 td = cross3(t, d);
+        // c0 += -a ir3 + 3 ir5 (t x d)
+        E.c0[0] += -av.x * ir3 + 3.0f * ir5 * td.x;
+        E.c0[1] += -av.y * ir3 + 3.0f * ir5 * td.y;
+        E.c0[2] += -av.z * ir3 + 3.0f * ir5 * td.z;
+        // c1[:, j] += 3 ir5 [t x e_j + d_j a + col_j x d] - 15 ir7 d_j (t x d)
+        const float3 col[3] = {make_float3(D0.x, D1.x, D2.x), make_float3(D0.y, D1.y, D2.y),
+                               make_float3(D0.z, D1.z, D2.z)};
+        const float3 te[3] = {make_float3(0.f, t.z, -t.y), make_float3(-t.z, 0.f, t.x),
+                              make_float3(t.y, -t.x, 0.f)};
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const float3 cd = cross3(col[j], d);
+            const float a3 = 3.0f * ir5, b15 = 15.0f * ir7 * dv[j];
+            E.c1[0][j] += a3 * (te[j].x + dv[j] * av.x + cd.x) - b15 * td.x;
+            E.c1[1][j] += a3 * (te[j].y + dv[j] * av.y + cd.y) - b15 * td.y;
+            E.c1[2][j] += a3 * (te[j].z + dv[j] * av.z + cd.z) - b15 * td.z;
+        }
+    }
+}
+
+// Sum of the lanes' expansions (xor butterfly: every lane ends with the same
+// bits) evaluated at offset x from the brick centre.
+__device__ __forceinline__ float3 expansion_eval(Expansion &E, float3 x) {
+    float *v = &E.c0[0];
+#pragma unroll
+    for (int i = 0; i < 30; ++i) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_xor(v[i], off, 64);
+    }
+    const float xv[3] = {x.x, x.y, x.z};
+    float u[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        u[i] = E.c0[i] + E.c1[i][0] * xv[0] + E.c1[i][1] * xv[1] + E.c1[i][2] * xv[2] +
+               0.5f * (E.c2[i][0] * xv[0] * xv[0] + E.c2[i][1] * xv[1] * xv[1] +
+                       E.c2[i][2] * xv[2] * xv[2]) +
+               E.c2[i][3] * xv[0] * xv[1] + E.c2[i][4] * xv[0] * xv[2] +
+               E.c2[i][5] * xv[1] * xv[2];
+    }
+    return make_float3(u[0], u[1], u[2]);
+}
+
 // One wave evaluates one brick of grid points; S is the wave's LDS slot.
 template <int POLICY, int ORDER, int COUNT>
 __device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc &g,
@@ -125,7 +241,18 @@ __device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc
     }
     const float3 q = grid_query_point(g, pt[0], pt[1], pt[2]);
     float3 sum = make_float3(0.f, 0.f, 0.f), comp = sum;
-    unsigned long long n_clusters = 0, n_pairs = 0;
+    unsigned long long n_clusters = 0, n_pairs = 0, n_expand = 0;
+    // brick centre (the centre of its nominal 4x4x4 points)
+    float3 cb;
+    {
+        const float nudge = 1.0f - 2.0f * FLT_EPSILON;
+        cb.x = g.min[0] + ((float)(4 * bb[0]) + 1.5f) * (g.cell[0] * nudge);
+        cb.y = g.min[1] + ((float)(4 * bb[1]) + 1.5f) * (g.cell[1] * nudge);
+        cb.z = g.min[2] + ((float)(4 * bb[2]) + 1.5f) * (g.cell[2] * nudge);
+    }
+    Expansion E;
+    const bool expanding = mp.expand_from >= 1 && mp.expand_from <= (int)tv.depth - 2;
+    if (expanding && !COUNT) expansion_zero(E);
     const float spread = mp.spread;
     const int depth = (int)tv.depth;
     SizeFactors f;
@@ -151,6 +278,44 @@ __device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc
         const float4 *__restrict__ M = tv.M[l];
         const int bx = c1[0] - c0[0] + 1, by = c1[1] - c0[1] + 1, bz = c1[2] - c0[2] + 1;
         const int total = bx * by * bz;
+        if (expanding && l >= mp.expand_from) {
+            // every lane expands its own applied cluster about the brick centre
+            for (int base = 0; base < total; base += 64) {
+                const int t = base + (int)lane;
+                bool use = false;
+                float4 p = make_float4(0.f, 0.f, 0.f, 0.f), w = p, m0 = p, m1 = p, m2 = p;
+                if (t < total) {
+                    const int cx = c0[0] + t % bx, cy = c0[1] + (t / bx) % by, cz = c0[2] + t / (bx * by);
+                    const bool near = cx >= n0[0] && cx <= n1[0] && cy >= n0[1] && cy <= n1[1] &&
+                                      cz >= n0[2] && cz <= n1[2];
+                    if (!near) {
+                        const size_t off = point_offset(gl, cx, cy, cz);
+                        p = P[off];
+                        use = p.w != 0.0f;
+                        if (use && !COUNT) {
+                            w = W[off];
+                            if (ORDER) {
+                                m0 = M[3 * off];
+                                m1 = M[3 * off + 1];
+                                m2 = M[3 * off + 2];
+                            }
+                        }
+                    }
+                }
+                if (COUNT) {
+                    n_expand += __popcll(__ballot(use));
+                } else if (use) {
+                    expansion_add<ORDER>(E, cb, p, w, m0, m1, m2, spread);
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) { lo[a] = n0[a]; hi[a] = n1[a]; }
+            if (l == mp.expand_from && !COUNT) {
+                const float3 u = expansion_eval(E, make_float3(q.x - cb.x, q.y - cb.y, q.z - cb.z));
+                kahan_add(sum, comp, u);
+            }
+            continue;
+        }
         float3 part = make_float3(0.f, 0.f, 0.f), scaled = part;
         for (int base = 0; base < total; base += 64) {
             // lanes fetch 64 cells of the children box
@@ -290,6 +455,7 @@ __device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc
         if (lane == 0) {
             atomicAdd(&counts[0], n_clusters * nv);
             atomicAdd(&counts[1], n_pairs * nv);
+            atomicAdd(&counts[2], n_expand);
         }
         return;
     }
@@ -334,8 +500,9 @@ __global__ __launch_bounds__(64 * WPB) void k_mac_grid_batch(const TreeView *__r
 }
 
 template <int COUNT>
-vg_status run_mac(const vg_tree *t, const vg_grid_desc *g, int near_k, int order, float spread,
-                  float circ, int policy, float *out, unsigned long long *counts, hipStream_t s) {
+vg_status run_mac(const vg_tree *t, const vg_grid_desc *g, int near_k, int order, int expand_from,
+                  float spread, float circ, int policy, float *out, unsigned long long *counts,
+                  hipStream_t s) {
     TreeView tv;
     VG_TRY(tree_checked_view(t, &tv));
     VG_REQUIRE(g, "grid is null");
@@ -351,6 +518,7 @@ vg_status run_mac(const vg_tree *t, const vg_grid_desc *g, int near_k, int order
     mp.circ = circ;
     for (int a = 0; a < 3; ++a) mp.nb[a] = (g->np[a] + 3) / 4;
     mp.nbricks = mp.nb[0] * mp.nb[1] * mp.nb[2];
+    mp.expand_from = expand_from;
     const dim3 grid(ceil_div(mp.nbricks, 4));
     const char *name = COUNT ? "vg_tree_mac_count" : "vg_tree_velocity_grid_mac";
     using K = void (*)(const TreeView, vg_grid_desc, MacParams, float *, unsigned long long *);
@@ -366,15 +534,15 @@ vg_status run_mac(const vg_tree *t, const vg_grid_desc *g, int near_k, int order
 
 vg_status mac_velocity_grid(const vg_tree *t, const vg_grid_desc *g, int near_k, int order,
                             float spread, float circ, int policy, float *out, hipStream_t s) {
-    return run_mac<0>(t, g, near_k, order, spread, circ, policy, out, nullptr, s);
+    return run_mac<0>(t, g, near_k, order, -1, spread, circ, policy, out, nullptr, s);
 }
 
 // Batched walk over `trees` trees of one shape (views and per-tree
 // (spread, circ) in device memory); out holds trees slabs of out_stride
 // floats.  waves_per_block in {1, 2, 4}.
 vg_status mac_velocity_grid_batch(const TreeView *views, const float2 *mollify, uint32_t trees,
-                                  const uint32_t np[3], int near_k, int order, int policy,
-                                  int waves_per_block, float *out, size_t out_stride,
+                                  const uint32_t np[3], int near_k, int order, int expand_from,
+                                  int policy, int waves_per_block, float *out, size_t out_stride,
                                   hipStream_t s) {
     VG_REQUIRE(near_k >= 0 && near_k <= 16, "near_k must be in [0, 16]");
     VG_REQUIRE(order == 0 || order == 1, "order must be 0 or 1");
@@ -387,6 +555,7 @@ vg_status mac_velocity_grid_batch(const TreeView *views, const float2 *mollify, 
     mp.circ = 1.0f;
     for (int a = 0; a < 3; ++a) mp.nb[a] = (np[a] + 3) / 4;
     mp.nbricks = mp.nb[0] * mp.nb[1] * mp.nb[2];
+    mp.expand_from = expand_from;
     using K = void (*)(const TreeView *, const float2 *, MacParams, float *, size_t);
     K tab[2][2][3] = {
         {{k_mac_grid_batch<0, 0, 1>, k_mac_grid_batch<0, 0, 2>, k_mac_grid_batch<0, 0, 4>},
@@ -403,22 +572,23 @@ vg_status mac_velocity_grid_batch(const TreeView *views, const float2 *mollify, 
 using namespace vg;
 
 VG_API vg_status vg_tree_velocity_grid_mac(const vg_tree *t, const vg_grid_desc *g, int near_k,
-                                           int order, float spread, float circ, int rsqrt_policy,
-                                           float *out, vg_stream s) {
+                                           int order, int expand_from, float spread, float circ,
+                                           int rsqrt_policy, float *out, vg_stream s) {
     VG_REQUIRE(out, "out is null");
-    return run_mac<0>(t, g, near_k, order, spread, circ, rsqrt_policy, out, nullptr,
+    return run_mac<0>(t, g, near_k, order, expand_from, spread, circ, rsqrt_policy, out, nullptr,
                       (hipStream_t)s);
 }
 
 VG_API vg_status vg_tree_mac_work(const vg_tree *t, const vg_grid_desc *g, int near_k,
-                                  uint64_t *clusters, uint64_t *pairs, vg_stream stream) {
+                                  int expand_from, uint64_t *clusters, uint64_t *pairs,
+                                  uint64_t *expansions, vg_stream stream) {
     VG_REQUIRE(clusters && pairs, "null output");
     hipStream_t s = (hipStream_t)stream;
     unsigned long long *d = nullptr;
-    VG_TRY(salloc((void **)&d, 2 * sizeof(unsigned long long), s));
-    vg_status st = zero_fill(d, 2 * sizeof(unsigned long long), s);
-    if (st == VG_OK) st = run_mac<1>(t, g, near_k, 0, 1.0f, 1.0f, 0, nullptr, d, s);
-    unsigned long long h[2] = {0, 0};
+    VG_TRY(salloc((void **)&d, 3 * sizeof(unsigned long long), s));
+    vg_status st = zero_fill(d, 3 * sizeof(unsigned long long), s);
+    if (st == VG_OK) st = run_mac<1>(t, g, near_k, 0, expand_from, 1.0f, 1.0f, 0, nullptr, d, s);
+    unsigned long long h[3] = {0, 0, 0};
     if (st == VG_OK) {
         hipError_t e = hipMemcpyAsync(h, d, sizeof h, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -427,5 +597,6 @@ VG_API vg_status vg_tree_mac_work(const vg_tree *t, const vg_grid_desc *g, int n
     sfree(d, s);
     *clusters = h[0];
     *pairs = h[1];
+    if (expansions) *expansions = h[2];
     return st;
 }

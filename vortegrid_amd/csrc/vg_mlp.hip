@@ -82,33 +82,37 @@ __device__ __forceinline__ float gelu_grad(float z) {
 // A[m][k] and opB(k, n) = TB ? B[n][k] : B[k][n].  FP32 in memory, operands
 // rounded to bf16 when staged to LDS, FP32 accumulation.  Block = 4 waves,
 // 64x64 tile, each wave 32x32 = 2x2 MFMA 16x16x32 tiles, K step 32.
+// blockIdx.z splits K into chunks of kc: chunk z writes its partial product
+// to C + z * M * ldc (reduced afterwards by k_reduce_split).
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float *__restrict__ A,
                                               int lda, const float *__restrict__ B, int ldb,
-                                              float *__restrict__ C, int ldc) {
+                                              float *__restrict__ C, int ldc, int kc) {
     __shared__ __attribute__((aligned(16))) __bf16 As[64][40];
     __shared__ __attribute__((aligned(16))) __bf16 Bs[64][40];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int kb = blockIdx.z * kc, ke = min(K, kb + kc);
+    C += (size_t)blockIdx.z * M * ldc;
     f32x4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += 32) {
+    for (int k0 = kb; k0 < ke; k0 += 32) {
         for (int e = tid; e < 64 * 32; e += 256) {
             const int mm = TA ? (e & 63) : (e >> 5), kk = TA ? (e >> 6) : (e & 31);
             const int gm = m0 + mm, gk = k0 + kk;
             float v = 0.f;
-            if (gm < M && gk < K) v = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
+            if (gm < M && gk < ke) v = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
             As[mm][kk] = (__bf16)v;
         }
         for (int e = tid; e < 64 * 32; e += 256) {
             const int nn = TB ? (e >> 5) : (e & 63), kk = TB ? (e & 31) : (e >> 6);
             const int gn = n0 + nn, gk = k0 + kk;
             float v = 0.f;
-            if (gn < N && gk < K) v = TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
+            if (gn < N && gk < ke) v = TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
             Bs[nn][kk] = (__bf16)v;
         }
         __syncthreads();
@@ -137,11 +141,38 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float *
             }
 }
 
+// C[m][n] = sum of the nsplit partial slabs, in slab order.
+__global__ void k_reduce_split(const float *__restrict__ part, int nsplit, int M, int N, int ldp,
+                               float *__restrict__ C, int ldc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M * N) return;
+    const int m = i / N, n = i % N;
+    float s = 0.f;
+    for (int z = 0; z < nsplit; ++z) s += part[(size_t)z * M * ldp + (size_t)m * ldp + n];
+    C[(size_t)m * ldc + n] = s;
+}
+
+constexpr int kSplitChunk = 256;  // K per block when the product is split
+constexpr int kMaxSplit = 64;
+
+// Products with a small output and a long K (the weight gradients, K = the
+// batch) are split over K into kSplitChunk chunks reduced in fixed order;
+// `split` (kMaxSplit * 64 * 64 floats) holds the partials.
 template <bool TA, bool TB>
 vg_status gemm(int M, int N, int K, const float *A, int lda, const float *B, int ldb, float *C,
-               int ldc, hipStream_t s) {
-    return vg::launch("vg_mlp_gemm", k_gemm<TA, TB>, dim3(vg::ceil_div(M, 64), vg::ceil_div(N, 64)),
-                      dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc);
+               int ldc, hipStream_t s, float *split = nullptr) {
+    const dim3 tiles(vg::ceil_div(M, 64), vg::ceil_div(N, 64));
+    int nsplit = split && M <= 64 && N <= 64 ? (int)vg::ceil_div(K, kSplitChunk) : 1;
+    if (nsplit > kMaxSplit) nsplit = kMaxSplit;
+    if (nsplit <= 1)
+        return vg::launch("vg_mlp_gemm", k_gemm<TA, TB>, tiles, dim3(256), 0, s, M, N, K, A, lda, B,
+                          ldb, C, ldc, K);
+    const int kc = (int)(vg::ceil_div(vg::ceil_div(K, nsplit), 32) * 32);
+    nsplit = (int)vg::ceil_div(K, kc);
+    VG_TRY(vg::launch("vg_mlp_gemm", k_gemm<TA, TB>, dim3(tiles.x, tiles.y, nsplit), dim3(256), 0, s,
+                      M, N, K, A, lda, B, ldb, split, N, kc));
+    return vg::launch("vg_mlp_gemm_reduce", k_reduce_split, dim3(vg::ceil_div(M * N, 256)), dim3(256),
+                      0, s, (const float *)split, nsplit, M, N, N, C, ldc);
 }
 // C = A B, C = A^T B, C = A B^T
 vg_status gemm_nn(int M, int N, int K, const float *A, int lda, const float *B, int ldb, float *C,
@@ -149,8 +180,8 @@ vg_status gemm_nn(int M, int N, int K, const float *A, int lda, const float *B, 
     return gemm<false, false>(M, N, K, A, lda, B, ldb, C, ldc, s);
 }
 vg_status gemm_tn(int M, int N, int K, const float *A, int lda, const float *B, int ldb, float *C,
-                  int ldc, hipStream_t s) {
-    return gemm<true, false>(M, N, K, A, lda, B, ldb, C, ldc, s);
+                  int ldc, hipStream_t s, float *split) {
+    return gemm<true, false>(M, N, K, A, lda, B, ldb, C, ldc, s, split);
 }
 vg_status gemm_nt(int M, int N, int K, const float *A, int lda, const float *B, int ldb, float *C,
                   int ldc, hipStream_t s) {
@@ -727,20 +758,22 @@ vg_status ensure_packed(vg_mlp *m, hipStream_t s) {
 
 // Activations of one batch.
 struct Acts {
-    float *Z1, *A1, *Z2, *A2, *Z3, *A3, *Y, *dY, *dA, *partial, *loss;
+    float *Z1, *A1, *Z2, *A2, *Z3, *A3, *Y, *dY, *dA, *partial, *loss, *split;
     void *base = nullptr;
 };
 
 vg_status acts_alloc(Acts *a, uint32_t rows, hipStream_t s) {
     const size_t h = (size_t)rows * kHid, y = (size_t)rows * kOutPad;
     const size_t nparts = ceil_div(rows, 256);
-    const size_t total = 7 * h + 2 * y + nparts + 1;
+    const size_t nsplit = (size_t)kMaxSplit * 64 * 64;
+    const size_t total = 7 * h + 2 * y + nparts + 1 + nsplit;
     VG_TRY(salloc(&a->base, total * sizeof(float), s));
     float *p = (float *)a->base;
     a->Z1 = p; p += h; a->A1 = p; p += h; a->Z2 = p; p += h; a->A2 = p; p += h;
     a->Z3 = p; p += h; a->A3 = p; p += h; a->dA = p; p += h;
     a->Y = p; p += y; a->dY = p; p += y;
-    a->partial = p; p += nparts; a->loss = p;
+    a->partial = p; p += nparts; a->loss = p; p += 1;
+    a->split = p;
     return VG_OK;
 }
 
@@ -774,7 +807,7 @@ vg_status backward(vg_mlp *m, const float *X, const float *T, uint32_t rows, Act
     VG_TRY(launch("vg_mlp_loss", k_sum_partials, dim3(1), dim3(64), 0, s, (const float *)a.partial,
                   nparts, a.loss));
     // output layer
-    VG_TRY(gemm_tn(kHid, kOutPad, rows, a.A3, kHid, a.dY, kOutPad, G + kOffW4, kOutPad, s));
+    VG_TRY(gemm_tn(kHid, kOutPad, rows, a.A3, kHid, a.dY, kOutPad, G + kOffW4, kOutPad, s, a.split));
     VG_TRY(launch("vg_mlp_colsum", k_colsum, dim3(1), dim3(256), 0, s, (const float *)a.dY, rows,
                   kOutPad, G + kOffB4));
     VG_TRY(gemm_nt(rows, kHid, kOutPad, a.dY, kOutPad, P + kOffW4, kOutPad, a.dA, kHid, s));
@@ -788,7 +821,7 @@ vg_status backward(vg_mlp *m, const float *X, const float *T, uint32_t rows, Act
         VG_TRY(launch("vg_mlp_gelu_back", k_gelu_back, eg, dim3(256), 0, s, a.dA,
                       (const float *)L_.Z, h));
         VG_TRY(gemm_tn(L_.kprev, kHid, rows, L_.Aprev, L_.lda_prev, a.dA, kHid,
-                                 G + L_.offW, kHid, s));
+                       G + L_.offW, kHid, s, a.split));
         VG_TRY(launch("vg_mlp_colsum", k_colsum, dim3(1), dim3(256), 0, s, (const float *)a.dA,
                       rows, kHid, G + L_.offB));
         if (li < 2) {

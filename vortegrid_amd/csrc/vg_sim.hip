@@ -65,6 +65,7 @@ VG_API vg_status vg_sim_config_default(vg_sim_config *c) {
     c->mg_max_cycles = 30;
     c->tree_near_k = -1;
     c->tree_order = 1;
+    c->tree_expand_from = -1;
     return VG_OK;
 }
 
@@ -212,7 +213,8 @@ VG_API vg_status vg_sim_update(vg_sim *sim, float dt, uint32_t frame, float *vel
                                              sim->vel_grid, s));
         else
             VG_TRY(vg_tree_velocity_grid_mac(sim->tree, &g, sim->cfg.tree_near_k,
-                                             sim->cfg.tree_order, sim->spread, sim->circ, pol,
+                                             sim->cfg.tree_order, sim->cfg.tree_expand_from,
+                                             sim->spread, sim->circ, pol,
                                              sim->vel_grid, s));
     } else {
         sim->spread = 1.f;

@@ -106,6 +106,7 @@ class SimConfig(C.Structure):
         ("mg_max_cycles", C.c_uint32),
         ("tree_near_k", C.c_int),
         ("tree_order", C.c_int),
+        ("tree_expand_from", C.c_int),
     ]
 
 
@@ -347,15 +348,16 @@ class Tree:
         return out.reshape(-1, 3, 3)
 
     def velocity_grid_mac(self, g: GridDesc, out: DeviceArray, near_k=2, order=1, spread=1.0,
-                          circ=1.0, policy=RSQRT_ACCURATE, stream=None):
+                          circ=1.0, policy=RSQRT_ACCURATE, expand_from=-1, stream=None):
         call("vg_tree_velocity_grid_mac", self.h, C.byref(g), int(near_k), int(order),
-             C.c_float(spread), C.c_float(circ), policy, out.ptr, _s(stream))
+             int(expand_from), C.c_float(spread), C.c_float(circ), policy, out.ptr, _s(stream))
 
-    def mac_work(self, g: GridDesc, near_k=2, stream=None):
-        c, p = C.c_uint64(), C.c_uint64()
-        call("vg_tree_mac_work", self.h, C.byref(g), int(near_k), C.byref(c), C.byref(p),
-             _s(stream))
-        return c.value, p.value
+    def mac_work(self, g: GridDesc, near_k=2, expand_from=-1, stream=None):
+        """(cluster interactions at points, direct pairs, clusters expanded)."""
+        c, p, e = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        call("vg_tree_mac_work", self.h, C.byref(g), int(near_k), int(expand_from), C.byref(c),
+             C.byref(p), C.byref(e), _s(stream))
+        return c.value, p.value, e.value
 
     def mlp_samples(self, g: GridDesc, n: int, near_k=1, layer_mask=0xFFFF, spread=1.0,
                     circ=1.0, seed=1, stream=None):
@@ -464,17 +466,18 @@ class Multigrid:
 
 class BatchConfig(C.Structure):
     _fields_ = [("cells", C.c_uint32), ("near_k", C.c_int), ("order", C.c_int),
-                ("rsqrt_policy", C.c_int)]
+                ("rsqrt_policy", C.c_int), ("expand_from", C.c_int)]
 
 
 class Batch:
     """Many small effects advanced together (vg_batch)."""
 
     def __init__(self, vorton_counts, tracer_counts, radii, cells=32, near_k=2, order=1,
-                 policy=1):
+                 policy=1, expand_from=-1):
         cfg = BatchConfig()
         call("vg_batch_config_default", C.byref(cfg))
         cfg.cells, cfg.near_k, cfg.order, cfg.rsqrt_policy = cells, near_k, order, policy
+        cfg.expand_from = expand_from
         vc = np.ascontiguousarray(vorton_counts, np.uint32)
         tc = np.ascontiguousarray(tracer_counts, np.uint32)
         rr = np.ascontiguousarray(radii, np.float32)
