@@ -290,8 +290,9 @@ vg_status vg_advect_tracers(const vg_grid_desc *g, const float *field, float *po
 /* No reference counterpart: the reference applies every cluster as a
  * monopole supervorton (vorton.h:161-179 on the clusters of
  * vortonSim.cpp:1337-1569).  A 16 -> 64 -> 64 -> 64 -> 3 GELU (tanh form)
- * network maps a (query, cluster) pair to the cluster's exact-minus-monopole
- * velocity.  Inputs, h = cube root of the layer's cell volume, S = the
+ * network maps a (query, cluster) pair to the part of the cluster's velocity
+ * that its monopole and analytic first moment miss (exact - monopole -
+ * first-moment term).  Inputs, h = cube root of the layer's cell volume, S = the
  * cluster's weight sum:  (q - c)/h, h/|q - c|, Omega/S, D/(h S) with D the
  * first moment (row-major [b][e]).  Output y -> velocity correction
  * y (2/3)(size/2)^3 S / h^2.  Products run on BF16 MFMA with FP32
@@ -326,7 +327,8 @@ vg_status vg_mlp_train(vg_mlp *m, const float *x, const float *t, uint32_t n, ui
  * pairs of the near-field treecode with radius near_k on the layers in
  * layer_mask (bit l = cluster layer l); x[n][16] the network inputs,
  * target[n][3] = (direct sum over the cluster's member vortons - its
- * monopole) scaled by the output factor, both with (spread, circ).
+ * monopole - its first-moment term) scaled by the output factor, with
+ * (spread, circ).
  * meta (device, may be NULL): per pair the grid point offset and
  * layer * 2^28 + cluster point offset (0xffffffff for a pair left empty).
  * n_failed (host, may be NULL): pairs left zero because no cluster was
@@ -335,9 +337,9 @@ vg_status vg_tree_mlp_samples(const vg_tree *t, const vg_grid_desc *g, int near_
                               uint32_t layer_mask, float spread, float circ, uint32_t n,
                               uint64_t seed, float *x, float *target, uint32_t *meta,
                               uint32_t *n_failed, vg_stream s);
-/* Monopole near-field treecode (vg_tree_velocity_grid_mac with order 0)
- * plus the network's correction for every applied cluster on the layers in
- * layer_mask.  layer_mask 0 leaves the plain monopole result. */
+/* Near-field treecode with first moments (vg_tree_velocity_grid_mac with
+ * order 1) plus the network's correction for every applied cluster on the
+ * layers in layer_mask.  layer_mask 0 leaves the plain treecode result. */
 vg_status vg_tree_velocity_grid_mlp(const vg_tree *t, const vg_grid_desc *g, vg_mlp *m,
                                     int near_k, uint32_t layer_mask, float spread, float circ,
                                     int rsqrt_policy, float *out, vg_stream s);

@@ -453,7 +453,7 @@ def test_mac_local_expansions(vg, oracle, rings2, k, order, expand_from):
     c, p, e = t.mac_work(g, k, expand_from)
     c0, p0, e0 = t.mac_work(g, k, -1)
     assert p == p0 and e0 == 0
-    if expand_from <= t.depth() - 2:
+    if expand_from <= t.depth - 2:
         assert e > 0 and c < c0
 
 

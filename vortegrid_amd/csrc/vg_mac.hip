@@ -242,17 +242,7 @@ __device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc
     const float3 q = grid_query_point(g, pt[0], pt[1], pt[2]);
     float3 sum = make_float3(0.f, 0.f, 0.f), comp = sum;
     unsigned long long n_clusters = 0, n_pairs = 0, n_expand = 0;
-    // brick centre (the centre of its nominal 4x4x4 points)
-    float3 cb;
-    {
-        const float nudge = 1.0f - 2.0f * FLT_EPSILON;
-        cb.x = g.min[0] + ((float)(4 * bb[0]) + 1.5f) * (g.cell[0] * nudge);
-        cb.y = g.min[1] + ((float)(4 * bb[1]) + 1.5f) * (g.cell[1] * nudge);
-        cb.z = g.min[2] + ((float)(4 * bb[2]) + 1.5f) * (g.cell[2] * nudge);
-    }
-    Expansion E;
     const bool expanding = mp.expand_from >= 1 && mp.expand_from <= (int)tv.depth - 2;
-    if (expanding && !COUNT) expansion_zero(E);
     const float spread = mp.spread;
     const int depth = (int)tv.depth;
     SizeFactors f;
@@ -279,41 +269,27 @@ __device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc
         const int bx = c1[0] - c0[0] + 1, by = c1[1] - c0[1] + 1, bz = c1[2] - c0[2] + 1;
         const int total = bx * by * bz;
         if (expanding && l >= mp.expand_from) {
-            // every lane expands its own applied cluster about the brick centre
-            for (int base = 0; base < total; base += 64) {
-                const int t = base + (int)lane;
-                bool use = false;
-                float4 p = make_float4(0.f, 0.f, 0.f, 0.f), w = p, m0 = p, m1 = p, m2 = p;
-                if (t < total) {
-                    const int cx = c0[0] + t % bx, cy = c0[1] + (t / bx) % by, cz = c0[2] + t / (bx * by);
-                    const bool near = cx >= n0[0] && cx <= n1[0] && cy >= n0[1] && cy <= n1[1] &&
-                                      cz >= n0[2] && cz <= n1[2];
-                    if (!near) {
-                        const size_t off = point_offset(gl, cx, cy, cz);
-                        p = P[off];
-                        use = p.w != 0.0f;
-                        if (use && !COUNT) {
-                            w = W[off];
-                            if (ORDER) {
-                                m0 = M[3 * off];
-                                m1 = M[3 * off + 1];
-                                m2 = M[3 * off + 2];
-                            }
-                        }
+            // these layers were applied by k_mac_expand, which left the raw
+            // (unscaled by circ) expansion value in out
+            if (COUNT) {
+                for (int base = 0; base < total; base += 64) {
+                    const int t = base + (int)lane;
+                    bool use = false;
+                    if (t < total) {
+                        const int cx = c0[0] + t % bx, cy = c0[1] + (t / bx) % by,
+                                  cz = c0[2] + t / (bx * by);
+                        const bool near = cx >= n0[0] && cx <= n1[0] && cy >= n0[1] &&
+                                          cy <= n1[1] && cz >= n0[2] && cz <= n1[2];
+                        if (!near) use = P[point_offset(gl, cx, cy, cz)].w != 0.0f;
                     }
-                }
-                if (COUNT) {
                     n_expand += __popcll(__ballot(use));
-                } else if (use) {
-                    expansion_add<ORDER>(E, cb, p, w, m0, m1, m2, spread);
                 }
+            } else if (l == mp.expand_from && valid) {
+                const size_t o = point_offset(g, pt[0], pt[1], pt[2]);
+                kahan_add(sum, comp, make_float3(out[3 * o], out[3 * o + 1], out[3 * o + 2]));
             }
 #pragma unroll
             for (int a = 0; a < 3; ++a) { lo[a] = n0[a]; hi[a] = n1[a]; }
-            if (l == mp.expand_from && !COUNT) {
-                const float3 u = expansion_eval(E, make_float3(q.x - cb.x, q.y - cb.y, q.z - cb.z));
-                kahan_add(sum, comp, u);
-            }
             continue;
         }
         float3 part = make_float3(0.f, 0.f, 0.f), scaled = part;
@@ -467,6 +443,117 @@ __device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc
     }
 }
 
+// Layers >= expand_from of the near-field walk for one brick: every lane
+// expands its fetched clusters about the brick centre, the wave sums the
+// expansions and each point writes their raw value (not yet scaled by circ)
+// to out, where the per-point walk picks it up.  Kept apart from mac_brick
+// so the per-point walk keeps its small register footprint.
+template <int ORDER>
+__device__ __forceinline__ void expand_brick(const TreeView &tv, const vg_grid_desc &g,
+                                             const MacParams &mp, uint32_t brick,
+                                             float *__restrict__ out) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t bb[3] = {brick % mp.nb[0], (brick / mp.nb[0]) % mp.nb[1],
+                            brick / (mp.nb[0] * mp.nb[1])};
+    const uint32_t lp[3] = {lane & 3u, (lane >> 2) & 3u, lane >> 4};
+    uint32_t pt[3];
+    bool valid = true;
+    int cl0[3], cl1[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t p = 4 * bb[a] + lp[a];
+        valid = valid && p < g.np[a];
+        pt[a] = p < g.np[a] ? p : g.np[a] - 1;
+        const int nc = (int)g.np[a] - 1;
+        const int first = (int)(4 * bb[a]);
+        int last = first + 3;
+        if (last > (int)g.np[a] - 1) last = (int)g.np[a] - 1;
+        cl0[a] = first < nc - 1 ? first : nc - 1;
+        cl1[a] = last < nc - 1 ? last : nc - 1;
+    }
+    const float3 q = grid_query_point(g, pt[0], pt[1], pt[2]);
+    float3 cb;
+    {
+        const float nudge = 1.0f - 2.0f * FLT_EPSILON;
+        cb.x = g.min[0] + ((float)(4 * bb[0]) + 1.5f) * (g.cell[0] * nudge);
+        cb.y = g.min[1] + ((float)(4 * bb[1]) + 1.5f) * (g.cell[1] * nudge);
+        cb.z = g.min[2] + ((float)(4 * bb[2]) + 1.5f) * (g.cell[2] * nudge);
+    }
+    Expansion E;
+    expansion_zero(E);
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int l = (int)tv.depth - 2; l >= mp.expand_from; --l) {
+        const vg_grid_desc &gl = tv.layer[l];
+        int c0[3], c1[3], n0[3], n1[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int dec = (int)tv.dec[l + 1][a];
+            const int sc = (int)layer_scale(tv, l, a);
+            c0[a] = lo[a] * dec;
+            c1[a] = hi[a] * dec + dec - 1;
+            const int ncl = (int)gl.np[a] - 1;
+            int blo = cl0[a] / sc - mp.k, bhi = cl1[a] / sc + mp.k;
+            blo = blo < 0 ? 0 : blo;
+            bhi = bhi > ncl - 1 ? ncl - 1 : bhi;
+            n0[a] = blo > c0[a] ? blo : c0[a];
+            n1[a] = bhi < c1[a] ? bhi : c1[a];
+        }
+        const int bx = c1[0] - c0[0] + 1, by = c1[1] - c0[1] + 1, bz = c1[2] - c0[2] + 1;
+        const int total = bx * by * bz;
+        for (int base = 0; base < total; base += 64) {
+            const int t = base + (int)lane;
+            if (t >= total) continue;
+            const int cx = c0[0] + t % bx, cy = c0[1] + (t / bx) % by, cz = c0[2] + t / (bx * by);
+            if (cx >= n0[0] && cx <= n1[0] && cy >= n0[1] && cy <= n1[1] && cz >= n0[2] &&
+                cz <= n1[2])
+                continue;
+            const size_t off = point_offset(gl, cx, cy, cz);
+            const float4 p = tv.P[l][off];
+            if (p.w == 0.0f) continue;
+            const float4 w = tv.W[l][off];
+            float4 m0 = p, m1 = p, m2 = p;
+            if (ORDER) {
+                m0 = tv.M[l][3 * off];
+                m1 = tv.M[l][3 * off + 1];
+                m2 = tv.M[l][3 * off + 2];
+            }
+            expansion_add<ORDER>(E, cb, p, w, m0, m1, m2, mp.spread);
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { lo[a] = n0[a]; hi[a] = n1[a]; }
+    }
+    const float3 u = expansion_eval(E, make_float3(q.x - cb.x, q.y - cb.y, q.z - cb.z));
+    if (valid) {
+        const size_t o = point_offset(g, pt[0], pt[1], pt[2]);
+        out[3 * o] = u.x;
+        out[3 * o + 1] = u.y;
+        out[3 * o + 2] = u.z;
+    }
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_mac_expand(const TreeView tv, vg_grid_desc g, MacParams mp,
+                                                    float *__restrict__ out) {
+    const uint32_t brick = uni(blockIdx.x * 4 + uni(threadIdx.x >> 6));
+    if (brick >= mp.nbricks) return;
+    expand_brick<ORDER>(tv, g, mp, brick, out);
+}
+
+template <int ORDER, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_mac_expand_batch(const TreeView *__restrict__ views,
+                                                               const float2 *__restrict__ mollify,
+                                                               MacParams mp,
+                                                               float *__restrict__ out,
+                                                               size_t out_stride) {
+    const uint32_t brick = uni(blockIdx.x * WPB + uni(threadIdx.x >> 6));
+    if (brick >= mp.nbricks) return;
+    const uint32_t tree = blockIdx.y;
+    const TreeView &tv = views[tree];
+    MacParams p = mp;
+    p.spread = mollify[tree].x;
+    expand_brick<ORDER>(tv, tv.layer[0], p, brick, out + tree * out_stride);
+}
+
 template <int POLICY, int ORDER, int COUNT>
 __global__ __launch_bounds__(256) void k_mac_grid(const TreeView tv, vg_grid_desc g, MacParams mp,
                                                   float *__restrict__ out,
@@ -527,6 +614,10 @@ vg_status run_mac(const vg_tree *t, const vg_grid_desc *g, int near_k, int order
         k = order ? k_mac_grid<VG_RSQRT_REFERENCE, 1, COUNT> : k_mac_grid<VG_RSQRT_REFERENCE, 0, COUNT>;
     else
         k = order ? k_mac_grid<VG_RSQRT_ACCURATE, 1, COUNT> : k_mac_grid<VG_RSQRT_ACCURATE, 0, COUNT>;
+    if (!COUNT && expand_from >= 1 && expand_from <= (int)tv.depth - 2) {
+        VG_TRY(launch("vg_tree_mac_expand", order ? k_mac_expand<1> : k_mac_expand<0>, grid,
+                      dim3(256), 0, s, tv, *g, mp, out));
+    }
     return launch(name, k, grid, dim3(256), 0, s, tv, *g, mp, out, counts);
 }
 
@@ -563,6 +654,15 @@ vg_status mac_velocity_grid_batch(const TreeView *views, const float2 *mollify, 
         {{k_mac_grid_batch<1, 0, 1>, k_mac_grid_batch<1, 0, 2>, k_mac_grid_batch<1, 0, 4>},
          {k_mac_grid_batch<1, 1, 1>, k_mac_grid_batch<1, 1, 2>, k_mac_grid_batch<1, 1, 4>}}};
     const int wi = waves_per_block == 1 ? 0 : waves_per_block == 2 ? 1 : 2;
+    if (expand_from >= 1) {
+        // trees of the batch share one shape; the walk checks the depth
+        using KE = void (*)(const TreeView *, const float2 *, MacParams, float *, size_t);
+        KE te[2][3] = {{k_mac_expand_batch<0, 1>, k_mac_expand_batch<0, 2>, k_mac_expand_batch<0, 4>},
+                       {k_mac_expand_batch<1, 1>, k_mac_expand_batch<1, 2>, k_mac_expand_batch<1, 4>}};
+        VG_TRY(launch("vg_batch_expand", te[order][wi],
+                      dim3(ceil_div(mp.nbricks, waves_per_block), trees), dim3(64 * waves_per_block),
+                      0, s, views, mollify, mp, out, out_stride));
+    }
     return launch("vg_batch_velocity_grid", tab[policy][order][wi],
                   dim3(ceil_div(mp.nbricks, waves_per_block), trees), dim3(64 * waves_per_block), 0,
                   s, views, mollify, mp, out, out_stride);

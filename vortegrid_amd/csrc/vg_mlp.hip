@@ -4,8 +4,9 @@
 //
 // A 16 -> 64 -> 64 -> 64 -> 3 network with GELU (tanh form) maps a
 // (query, cluster) pair to the difference between the cluster's exact
-// velocity (the direct sum over its member vortons) and its monopole
-// velocity.  Inputs, with h the cube root of the layer's cell volume and S
+// velocity (the direct sum over its member vortons) and its monopole plus
+// analytic first-moment velocity: the part of the cluster's field its
+// moments do not capture.  Inputs, with h the cube root of the layer's cell volume and S
 // the cluster's weight sum (W.w):
 //     x = (q - c) / h,  1/|x|,  Omega / S,  D / (h S)
 // Output y becomes the velocity correction y * (2/3) (size/2)^3 S / h^2.
@@ -676,6 +677,22 @@ __global__ __launch_bounds__(256) void k_mlp_samples(const vg::TreeView tv, vg_g
             float4 A, B;
             vg::prepare_source(p, make_float4(w.x, w.y, w.z, 0.f), sp.spread, A, B);
             vg::pair_velocity<VG_RSQRT_ACCURATE>(mono, q, A, B);
+            // analytic first-moment term: -s/|d|^3 [a - 3/|d|^2 (D d) x d],
+            // zero inside the core
+            const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 >= A.w) {
+                const float rs = __builtin_amdgcn_rsqf(d2);
+                const float law = rs * rs * rs, c3 = 3.0f * rs * rs;
+                const float rad = p.w * 0.5f * sp.spread;
+                const float s = (2.0f / 3.0f) * rad * rad * rad;
+                const float tx = m0.x * dx + m0.y * dy + m0.z * dz;
+                const float ty = m1.x * dx + m1.y * dy + m1.z * dz;
+                const float tz = m2.x * dx + m2.y * dy + m2.z * dz;
+                mono.x -= (m0.w - c3 * (ty * dz - tz * dy)) * law * s;
+                mono.y -= (m1.w - c3 * (tz * dx - tx * dz)) * law * s;
+                mono.z -= (m2.w - c3 * (tx * dy - ty * dx)) * law * s;
+            }
         }
         const float isc = 1.0f / correction_scale(p, w, inv_h);
         for (int c = 0; c < kIn; ++c) X[(size_t)i * kIn + c] = f[c];
@@ -1046,9 +1063,9 @@ VG_API vg_status vg_tree_velocity_grid_mlp(const vg_tree *t, const vg_grid_desc 
                                            vg_stream stream) {
     VG_REQUIRE(m && out, "null argument");
     hipStream_t s = (hipStream_t)stream;
-    // plain monopole near-field treecode (the fallback), then the learned
-    // correction of the clusters on the masked layers
-    VG_TRY(mac_velocity_grid(t, g, near_k, 0, spread, circ, rsqrt_policy, out, s));
+    // near-field treecode with the analytic first moments (the fallback),
+    // then the learned residual of the clusters on the masked layers
+    VG_TRY(mac_velocity_grid(t, g, near_k, 1, spread, circ, rsqrt_policy, out, s));
     TreeView tv;
     VG_TRY(tree_checked_view(t, &tv));
     if (tv.depth < 2) return VG_OK;
