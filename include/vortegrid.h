@@ -290,14 +290,19 @@ vg_status vg_advect_tracers(const vg_grid_desc *g, const float *field, float *po
 /* No reference counterpart: the reference applies every cluster as a
  * monopole supervorton (vorton.h:161-179 on the clusters of
  * vortonSim.cpp:1337-1569).  A 16 -> 64 -> 64 -> 64 -> 3 GELU (tanh form)
- * network maps a (query, cluster) pair to the part of the cluster's velocity
- * that its monopole and analytic first moment miss (exact - monopole -
- * first-moment term).  Inputs, h = cube root of the layer's cell volume, S = the
- * cluster's weight sum:  (q - c)/h, h/|q - c|, Omega/S, D/(h S) with D the
- * first moment (row-major [b][e]).  Output y -> velocity correction
- * y (2/3)(size/2)^3 S / h^2.  Products run on BF16 MFMA with FP32
- * accumulation.  Parameters, compact order (VG_MLP_PARAMS floats): W1[16][64]
- * b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64][3] b4[3], W row-major
+ * network learns, per (query, cluster) pair, the part of the cluster's
+ * velocity that its monopole and analytic first moment miss.  With h the
+ * cube root of the layer's cell volume, S the cluster's weight sum,
+ * x = (q - c)/h, r = |x|, e = x/r, O = Omega/S, Dt = D/(h S) (D the first
+ * moment, row-major [b][e]), at its antisymmetric vector, u = Dt e,
+ * v = Dt^T e, the inputs are the rotation invariants
+ *   1/r, 1/r^2, |O|^2, (O.e)^2, |at|^2, (at.e)^2, |u|^2, |v|^2, (e.u)^2,
+ *   |Dt|_F^2, (tr Dt)^2, O.at, (O.e)(e.u), O.u, O.v, at.(u x e)
+ * and the outputs c weight B = r^-3 [O x e, at, u x e] (columns): the
+ * correction (2/3)(size/2)^3 S/h^2 B c rotates with the field and reverses
+ * with the vorticity.  Products run on BF16 MFMA with FP32 accumulation.
+ * Parameters, compact order (VG_MLP_PARAMS floats): W1[16][64] b1[64]
+ * W2[64][64] b2[64] W3[64][64] b3[64] W4[64][3] b4[3], W row-major
  * [in][out]. */
 #define VG_MLP_INPUTS 16
 #define VG_MLP_HIDDEN 64
@@ -311,32 +316,34 @@ vg_status vg_mlp_destroy(vg_mlp *m);
 /* Host arrays of VG_MLP_PARAMS floats; set_params also clears Adam state. */
 vg_status vg_mlp_set_params(vg_mlp *m, const float *params);
 vg_status vg_mlp_get_params(vg_mlp *m, float *params);
-/* y[n][3] = net(x[n][16]) (device pointers). */
+/* c[n][3] = net(x[n][16]) (device pointers). */
 vg_status vg_mlp_forward(vg_mlp *m, const float *x, uint32_t n, float *y, vg_stream s);
-/* Gradient (host, VG_MLP_PARAMS floats) of mean((net(x) - t)^2) over the
- * n*3 outputs, and that loss (host); t[n][3] on the device.  Blocks. */
-vg_status vg_mlp_gradients(vg_mlp *m, const float *x, const float *t, uint32_t n,
-                           float *grad, float *loss, vg_stream s);
+/* Gradient (host, VG_MLP_PARAMS floats) of mean((B net(x) - t)^2) over the
+ * n*3 components, and that loss (host); basis[n][3][3] (row i, column k;
+ * NULL: identity) and t[n][3] on the device.  Blocks. */
+vg_status vg_mlp_gradients(vg_mlp *m, const float *x, const float *basis, const float *t,
+                           uint32_t n, float *grad, float *loss, vg_stream s);
 /* steps Adam updates (lr, beta 0.9 / 0.999, eps 1e-8) on minibatches of
  * `batch` consecutive rows of (x, t), offsets (step * batch) mod
  * (n - batch + 1).  final_loss (host, may be NULL): loss of the last
  * minibatch before its update.  Blocks when final_loss is given. */
-vg_status vg_mlp_train(vg_mlp *m, const float *x, const float *t, uint32_t n, uint32_t batch,
-                       uint32_t steps, float lr, float *final_loss, vg_stream s);
+vg_status vg_mlp_train(vg_mlp *m, const float *x, const float *basis, const float *t,
+                       uint32_t n, uint32_t batch, uint32_t steps, float lr, float *final_loss,
+                       vg_stream s);
 /* Training pairs from a tree: n random (grid point, accepted cluster)
  * pairs of the near-field treecode with radius near_k on the layers in
  * layer_mask (bit l = cluster layer l); x[n][16] the network inputs,
- * target[n][3] = (direct sum over the cluster's member vortons - its
- * monopole - its first-moment term) scaled by the output factor, with
- * (spread, circ).
+ * basis[n][3][3] the output vectors (may be NULL), target[n][3] = (direct
+ * sum over the cluster's member vortons - its monopole - its first-moment
+ * term) divided by (2/3)(size/2)^3 S/h^2, with (spread, circ).
  * meta (device, may be NULL): per pair the grid point offset and
  * layer * 2^28 + cluster point offset (0xffffffff for a pair left empty).
  * n_failed (host, may be NULL): pairs left zero because no cluster was
  * found in 256 draws.  Blocks. */
 vg_status vg_tree_mlp_samples(const vg_tree *t, const vg_grid_desc *g, int near_k,
                               uint32_t layer_mask, float spread, float circ, uint32_t n,
-                              uint64_t seed, float *x, float *target, uint32_t *meta,
-                              uint32_t *n_failed, vg_stream s);
+                              uint64_t seed, float *x, float *basis, float *target,
+                              uint32_t *meta, uint32_t *n_failed, vg_stream s);
 /* Near-field treecode with first moments (vg_tree_velocity_grid_mac with
  * order 1) plus the network's correction for every applied cluster on the
  * layers in layer_mask.  layer_mask 0 leaves the plain treecode result. */

@@ -6,16 +6,18 @@ the reference's only cluster representation is the monopole supervorton
 This module restates, in float64 numpy, what the HIP kernels compute so that
 the tests can check them:
 
-  features(q, cluster)  16 inputs per (query, cluster) pair
+  features(q, cluster)  16 invariant inputs and the 3 equivariant output
+                        vectors per (query, cluster) pair
   forward / backward    the 16 -> 64 -> 64 -> 64 -> 3 GELU(tanh) network and
                         the gradient of its mean-squared loss
   bf16 emulation        optional round-to-nearest-even of every GEMM operand
                         (the MFMA kernels read bf16 operands, accumulate FP32)
 
 Pinned by the tests: the backward pass against central finite differences in
-float64, the forward pass against a per-sample scalar loop, GELU against its
-closed form at known points, and the feature normaliser against the
-monopole law (a cluster seen from distance |x| h has 1/|x| = h/|d|).
+float64 (with and without an output basis), the forward pass against a
+per-sample scalar loop, GELU against its closed form at known points, and the
+features against their symmetries (rotating the field rotates B c and leaves
+the inputs unchanged; reversing every vorticity reverses B c).
 """
 from __future__ import annotations
 
@@ -88,21 +90,30 @@ def forward(flat, X, emulate_bf16=False):
     return Y, (X, Z1, A1, Z2, A2, Z3, A3)
 
 
-def loss(flat, X, T, emulate_bf16=False):
+def apply_basis(Y, B):
+    """Correction vectors B c for network outputs c (B None: c itself)."""
+    return Y if B is None else np.einsum("nik,nk->ni", np.asarray(B, np.float64), Y)
+
+
+def loss(flat, X, T, emulate_bf16=False, B=None):
     Y, _ = forward(flat, X, emulate_bf16)
-    return float(np.mean((Y - np.asarray(T, np.float64)) ** 2))
+    return float(np.mean((apply_basis(Y, B) - np.asarray(T, np.float64)) ** 2))
 
 
-def backward(flat, X, T, emulate_bf16=False):
-    """Gradient of mean((Y - T)^2) over all n*3 outputs; returns (loss,
-    compact gradient vector).  With emulate_bf16 the backward GEMM operands
-    (dY, dZ, activations, weights) are rounded to bf16 too."""
+def backward(flat, X, T, emulate_bf16=False, B=None):
+    """Gradient of mean((B c - T)^2) over all n*3 components (B None: the
+    identity); returns (loss, compact gradient vector).  With emulate_bf16
+    the backward GEMM operands (dY, dZ, activations, weights) are rounded to
+    bf16 too."""
     p = unpack(flat)
     e = emulate_bf16
     Y, (X, Z1, A1, Z2, A2, Z3, A3) = forward(flat, X, e)
     T = np.asarray(T, np.float64)
     n = X.shape[0]
-    dY = 2.0 * (Y - T) / (3.0 * n)
+    R = apply_basis(Y, B) - T
+    dY = 2.0 * R / (3.0 * n)
+    if B is not None:
+        dY = np.einsum("nik,ni->nk", np.asarray(B, np.float64), dY)
     g = {}
     g["W4"] = _op(A3, e).T @ _op(dY, e)
     g["b4"] = dY.sum(0)
@@ -115,19 +126,19 @@ def backward(flat, X, T, emulate_bf16=False):
     dZ1 = (_op(dZ2, e) @ _op(p["W2"], e).T) * gelu_grad(Z1)
     g["W1"] = _op(X, e).T @ _op(dZ1, e)
     g["b1"] = dZ1.sum(0)
-    return float(np.mean((Y - T) ** 2)), pack(g)
+    return float(np.mean(R ** 2)), pack(g)
 
 
-def numeric_gradient(flat, X, T, idx, h=1e-6):
+def numeric_gradient(flat, X, T, idx, h=1e-6, B=None):
     """Central differences of the float64 loss for parameters idx."""
     flat = np.asarray(flat, np.float64).copy()
     out = np.empty(len(idx))
     for j, i in enumerate(idx):
         old = flat[i]
         flat[i] = old + h
-        lp = loss(flat, X, T)
+        lp = loss(flat, X, T, B=B)
         flat[i] = old - h
-        lm = loss(flat, X, T)
+        lm = loss(flat, X, T, B=B)
         flat[i] = old
         out[j] = (lp - lm) / (2 * h)
     return out
@@ -149,25 +160,42 @@ def init_params(seed):
 
 # --------------------------------------------------------------- features
 def features(q, P, W, M, h):
-    """Inputs of the correction network for query points q [n,3] against
-    clusters given by P (x, y, z, size), W (Omega, S), M (3 rows of the
-    first moment D[b][e] = sum Omega_b (y - c)_e; the .w column is ignored)
-    on a layer of cell size h (the cube root of the cell volume):
+    """Inputs and output basis of the correction network for query points q
+    [n,3] against clusters P (x, y, z, size), W (Omega, S), M (3 rows of the
+    first moment D[b][e] = sum Omega_b (y - c)_e, .w = antisymmetric part a)
+    on a layer of cell size h (the cube root of the cell volume).
 
-      x = (q - c) / h,  1/|x|,  Omega / S,  D / (h S)      (16 values)
-
-    S is the cluster's weight sum (the tree's W.w).  The network output y
-    becomes a velocity correction y * (2/3) (size/2)^3 S / h^2."""
+    With x = (q - c)/h, r = |x|, e = x/r, O = Omega/S, Dt = D/(h S),
+    at = a/(h S), u = Dt e, v = Dt^T e, the 16 inputs are rotation invariants
+    that do not change when every vorticity changes sign:
+      1/r, 1/r^2, |O|^2, (O.e)^2, |at|^2, (at.e)^2, |u|^2, |v|^2, (e.u)^2,
+      |Dt|_F^2, (tr Dt)^2, O.at, (O.e)(e.u), O.u, O.v, at.(u x e)
+    and the outputs c_k weight three vectors that rotate with the field and
+    change sign with the vorticity, times r^-3:
+      B = r^-3 [O x e, at, u x e]          (columns)
+    The correction is (2/3)(size/2)^3 S / h^2 * B c.  Returns (X [n,16],
+    B [n,3,3])."""
     q = np.asarray(q, np.float64)
     P = np.asarray(P, np.float64)
     W = np.asarray(W, np.float64)
     M = np.asarray(M, np.float64).reshape(-1, 3, 4)
     x = (q - P[:, :3]) / h
     ir = 1.0 / np.sqrt((x * x).sum(1))
+    e = x * ir[:, None]
     S = W[:, 3]
-    om = W[:, :3] / S[:, None]
-    D = M[:, :, :3].reshape(-1, 9) / (h * S[:, None])
-    return np.concatenate([x, ir[:, None], om, D], 1)
+    O = W[:, :3] / S[:, None]
+    Dt = M[:, :, :3] / (h * S)[:, None, None]
+    at = M[:, :, 3] / (h * S)[:, None]
+    u = np.einsum("nbe,ne->nb", Dt, e)
+    v = np.einsum("nbe,nb->ne", Dt, e)
+    dot = lambda a, b: (a * b).sum(1)
+    eu = dot(e, u)
+    X = np.stack([ir, ir * ir, dot(O, O), dot(O, e) ** 2, dot(at, at), dot(at, e) ** 2,
+                  dot(u, u), dot(v, v), eu ** 2, (Dt * Dt).sum((1, 2)),
+                  np.trace(Dt, axis1=1, axis2=2) ** 2, dot(O, at), dot(O, e) * eu, dot(O, u),
+                  dot(O, v), dot(at, np.cross(u, e))], 1)
+    B = np.stack([np.cross(O, e), at, np.cross(u, e)], 2) * (ir ** 3)[:, None, None]
+    return X, B
 
 
 def correction_scale(P, W, h):

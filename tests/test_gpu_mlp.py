@@ -42,17 +42,20 @@ def test_forward_matches_oracle(vg):
     assert rel(got, want) < 3e-2
 
 
-def test_gradients_match_oracle(vg):
-    case("mlp gradients n=2500")
+@pytest.mark.parametrize("with_basis", [False, True])
+def test_gradients_match_oracle(vg, with_basis):
+    case(f"mlp gradients n=2500 basis={with_basis}")
     rng = np.random.default_rng(2)
     p = (M.init_params(12) + 0.05 * rng.standard_normal(M.N_PARAMS)).astype(F32)
     X = rng.standard_normal((2500, M.N_IN)).astype(F32)
     T = rng.standard_normal((2500, 3)).astype(F32)
+    Bm = rng.standard_normal((2500, 3, 3)).astype(F32) if with_basis else None
     m = vg.Mlp()
     m.set_params(p)
-    loss, g = m.gradients(vg.DeviceArray.from_numpy(X), vg.DeviceArray.from_numpy(T), len(X))
-    lw, gw = M.backward(p, X, T, emulate_bf16=True)
-    l64, g64 = M.backward(p, X, T)
+    loss, g = m.gradients(vg.DeviceArray.from_numpy(X), vg.DeviceArray.from_numpy(T), len(X),
+                          basis=vg.DeviceArray.from_numpy(Bm) if with_basis else None)
+    lw, gw = M.backward(p, X, T, emulate_bf16=True, B=Bm)
+    l64, g64 = M.backward(p, X, T, B=Bm)
     assert abs(loss - lw) < 1e-3 * lw
     o = 0
     for name, shp in M.SHAPES:
@@ -112,11 +115,11 @@ def _cluster_arrays(ot, pos, om):
 def test_samples_match_member_direct_sums(vg, oracle, ring_tree):
     case("mlp training pairs vs direct sums over cluster members")
     r = ring_tree
-    x, t, meta, failed = r["t"].mlp_samples(r["g"], 256, near_k=1, spread=r["spread"],
-                                            circ=r["circ"], seed=9)
+    x, b, t, meta, failed = r["t"].mlp_samples(r["g"], 256, near_k=1, spread=r["spread"],
+                                               circ=r["circ"], seed=9)
     vg.synchronize()
     assert failed == 0
-    X, T, meta = x.numpy(), t.numpy(), meta.numpy()
+    X, Bs, T, meta = x.numpy(), b.numpy(), t.numpy(), meta.numpy()
     ot, og = r["ot"], r["og"]
     cl = _cluster_arrays(ot, r["pos"], r["om"])
     h = M.layer_h(ot)
@@ -127,8 +130,10 @@ def test_samples_match_member_direct_sums(vg, oracle, ring_tree):
         layers_seen.add(l)
         q = q_all[meta[i, 0]]
         P, W, Mm = (a[off:off + 1] for a in cl[l])
-        f = M.features(q[None], P, W, Mm, h[l])[0]
-        assert np.allclose(X[i], f, rtol=2e-5, atol=2e-5 * np.abs(f).max()), i
+        f, fb = M.features(q[None], P, W, Mm, h[l])
+        f, fb = f[0], fb[0]
+        assert np.allclose(X[i], f, rtol=1e-4, atol=1e-4 * np.abs(f).max()), i
+        assert np.allclose(Bs[i], fb, rtol=1e-4, atol=1e-4 * np.abs(fb).max()), i
         mem = M.cluster_members(ot, l, off)
         exact = oracle.direct_velocity_f64(r["pos"][mem], r["om"][mem], 2 * r["radius"], q[None],
                                            r["spread"], r["circ"])[0]
@@ -183,9 +188,9 @@ def test_mlp_grid_matches_oracle(vg, oracle, ring_tree):
             P, W, Mm = (a[offs] for a in cl[l])
             keep = W[:, 3] > 1e-30
             P, W, Mm = P[keep], W[keep], Mm[keep]
-            F = M.features(np.repeat(q_all[o][None], len(P), 0), P, W, Mm, h[l])
+            F, Bq = M.features(np.repeat(q_all[o][None], len(P), 0), P, W, Mm, h[l])
             y, _ = M.forward(p, F, emulate_bf16=True)
-            u += (y * M.correction_scale(P, W, h[l])[:, None]).sum(0)
+            u += (M.apply_basis(y, Bq) * M.correction_scale(P, W, h[l])[:, None]).sum(0)
         want.append(u)
         have.append(got[o])
     e = rel(have, want)

@@ -65,25 +65,73 @@ def test_bf16_rounding_matches_torch():
     assert np.array_equal(M.bf16(a), want)
 
 
-def test_features_normalise_the_monopole_law():
-    """A single vorton far from its cluster's query: the monopole velocity
-    equals scale * (Omega/S x x) |x|^-3 in feature units."""
-    from oracle import oracle as O
+def _random_cluster(rng):
+    c = rng.uniform(-1, 1, 3)
+    om = rng.standard_normal(3)
+    S = 2 * np.linalg.norm(om) * rng.uniform(1.0, 3.0)
+    D = 0.05 * rng.standard_normal((3, 3))
+    a = np.array([D[1, 2] - D[2, 1], D[2, 0] - D[0, 2], D[0, 1] - D[1, 0]])
+    P = np.array([[*c, 0.1]])
+    W = np.array([[*om, S]])
+    Mm = np.concatenate([D, a[:, None]], 1).reshape(1, 12)
+    return c, om, S, D, P, W, Mm
 
+
+def test_features_are_invariant_and_basis_equivariant():
+    """Rotating the field leaves the inputs unchanged and rotates B c;
+    reversing every vorticity leaves the inputs unchanged and reverses B c."""
     rng = np.random.default_rng(7)
+    from vortegrid_amd import learned
+
     for _ in range(20):
-        c = rng.uniform(-1, 1, 3).astype(np.float32)
-        om = rng.standard_normal(3).astype(np.float32)
-        q = (c + rng.uniform(2, 4) * M.bf16(rng.standard_normal(3))).astype(np.float32)
-        size, h = 0.1, 0.3
-        S = 2 * np.linalg.norm(om.astype(np.float64))
-        P = np.array([[*c, size]])
-        W = np.array([[*om, S]])
-        f = M.features(q[None], P, W, np.zeros((1, 12)), h)[0]
-        assert np.isclose(f[3], h / np.linalg.norm(q.astype(np.float64) - c))
-        u = M.correction_scale(P, W, h)[0] * np.cross(f[4:7], f[:3]) * f[3] ** 3
-        want = O.direct_velocity(c[None], om[None], size, q[None], policy=1)[0]
-        assert np.allclose(u, want, rtol=1e-5)
+        c, om, S, D, P, W, Mm = _random_cluster(rng)
+        q = c + rng.uniform(2, 4) * rng.standard_normal(3)
+        h = 0.3
+        X, B = M.features(q[None], P, W, Mm, h)
+        coef = rng.standard_normal(3)
+        R = learned.random_rotation(rng)
+        Dr = R @ D @ R.T
+        ar = np.array([Dr[1, 2] - Dr[2, 1], Dr[2, 0] - Dr[0, 2], Dr[0, 1] - Dr[1, 0]])
+        Pr = np.array([[*(R @ c), 0.1]])
+        Wr = np.array([[*(R @ om), S]])
+        Mr = np.concatenate([Dr, ar[:, None]], 1).reshape(1, 12)
+        Xr, Br = M.features((R @ q)[None], Pr, Wr, Mr, h)
+        assert np.allclose(Xr, X, rtol=1e-10, atol=1e-14)
+        assert np.allclose(Br[0] @ coef, R @ (B[0] @ coef), rtol=1e-10, atol=1e-14)
+        Xn, Bn = M.features(q[None], P, np.array([[*(-om), S]]), -Mm, h)
+        assert np.allclose(Xn, X, rtol=1e-12, atol=1e-15)
+        assert np.allclose(Bn, -B, rtol=1e-12, atol=1e-15)
+
+
+def test_basis_spans_monopole_and_first_moment():
+    """c = (r, 0, 0) gives the monopole law and c = (0, -1, 3) the analytic
+    first-moment term, both times the output scale."""
+    rng = np.random.default_rng(8)
+    for _ in range(20):
+        c, om, S, D, P, W, Mm = _random_cluster(rng)
+        q = c + rng.uniform(2, 4) * rng.standard_normal(3)
+        h = 0.3
+        X, B = M.features(q[None], P, W, Mm, h)
+        sc = M.correction_scale(P, W, h)[0]
+        d = q - c
+        s = (2.0 / 3.0) * 0.05 ** 3
+        mono = s * np.cross(om, d) / np.linalg.norm(d) ** 3
+        a = Mm.reshape(3, 4)[:, 3]
+        dip = -s * (a - 3.0 / (d @ d) * np.cross(D @ d, d)) / np.linalg.norm(d) ** 3
+        r = 1.0 / X[0, 0]
+        assert np.allclose(sc * B[0] @ np.array([r, 0, 0]), mono, rtol=1e-10)
+        assert np.allclose(sc * B[0] @ np.array([0, -1, 3]), dip, rtol=1e-10)
+
+
+def test_basis_loss_gradient_matches_finite_differences():
+    rng = np.random.default_rng(3)
+    p = M.init_params(5) + 0.05 * rng.standard_normal(M.N_PARAMS)
+    X = rng.standard_normal((30, M.N_IN))
+    Bm = rng.standard_normal((30, 3, 3))
+    T = rng.standard_normal((30, 3))
+    _, g = M.backward(p, X, T, B=Bm)
+    idx = rng.choice(p.size, 80, replace=False)
+    assert np.allclose(g[idx], M.numeric_gradient(p, X, T, idx, B=Bm), rtol=1e-5, atol=1e-9)
 
 
 @pytest.fixture(scope="module")

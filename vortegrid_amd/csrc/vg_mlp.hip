@@ -6,10 +6,15 @@
 // (query, cluster) pair to the difference between the cluster's exact
 // velocity (the direct sum over its member vortons) and its monopole plus
 // analytic first-moment velocity: the part of the cluster's field its
-// moments do not capture.  Inputs, with h the cube root of the layer's cell volume and S
-// the cluster's weight sum (W.w):
-//     x = (q - c) / h,  1/|x|,  Omega / S,  D / (h S)
-// Output y becomes the velocity correction y * (2/3) (size/2)^3 S / h^2.
+// moments do not capture.  With h the cube root of the layer's cell volume,
+// S the cluster's weight sum (W.w), x = (q - c)/h, r = |x|, e = x/r,
+// O = Omega/S, Dt = D/(h S), at = its antisymmetric vector, u = Dt e,
+// v = Dt^T e, the inputs are 16 rotation invariants that are even in the
+// vorticity and the 3 outputs c weight three vectors that rotate with the
+// field and are odd in the vorticity:
+//     correction = (2/3)(size/2)^3 S/h^2 * r^-3 [O x e, at, u x e] c
+// so the correction is exactly rotation-equivariant and reverses with the
+// vorticity (c = (r, 0, 0) is the monopole, (0, -1, 3) the first moment).
 //
 // Everything that multiplies matrices runs on BF16 MFMA with FP32
 // accumulation:
@@ -211,9 +216,11 @@ __global__ void k_gelu_back(float *__restrict__ dA, const float *__restrict__ Z,
     dA[i] *= gelu_grad(Z[i]);
 }
 
-// dY = 2 (Y - T) / (3 rows) on the 3 live columns, 0 on the padding; one
-// block partial of sum (Y - T)^2 per block.
+// Loss of the correction vectors B c (B = identity when null) against T:
+// dY = 2 B^T (B c - T) / (3 rows) on the 3 live columns, 0 on the padding;
+// one block partial of sum |B c - T|^2 per block.
 __global__ __launch_bounds__(256) void k_loss_grad(const float *__restrict__ Y,
+                                                   const float *__restrict__ Bm,
                                                    const float *__restrict__ T,
                                                    float *__restrict__ dY, uint32_t rows,
                                                    float scale, float *__restrict__ partial) {
@@ -221,16 +228,28 @@ __global__ __launch_bounds__(256) void k_loss_grad(const float *__restrict__ Y,
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     float s = 0.f;
     if (r < rows) {
+        const float *y = Y + (size_t)r * kOutPad;
+        float e[3], g[3];
 #pragma unroll
-        for (int c = 0; c < kOutPad; ++c) {
-            float d = 0.f;
-            if (c < kOut) {
-                const float e = Y[(size_t)r * kOutPad + c] - T[(size_t)r * kOut + c];
-                s += e * e;
-                d = scale * e;
+        for (int i = 0; i < 3; ++i) {
+            float v = y[i];
+            if (Bm) {
+                const float *b = Bm + (size_t)r * 9 + 3 * i;
+                v = b[0] * y[0] + b[1] * y[1] + b[2] * y[2];
             }
-            dY[(size_t)r * kOutPad + c] = d;
+            e[i] = v - T[(size_t)r * kOut + i];
+            s += e[i] * e[i];
         }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            g[k] = e[k];
+            if (Bm) {
+                const float *b = Bm + (size_t)r * 9;
+                g[k] = b[k] * e[0] + b[3 + k] * e[1] + b[6 + k] * e[2];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < kOutPad; ++c) dY[(size_t)r * kOutPad + c] = c < kOut ? scale * g[c] : 0.f;
     }
     red[threadIdx.x] = s;
     __syncthreads();
@@ -320,16 +339,59 @@ struct Feat {
     float v[kIn];
 };
 
-__device__ __forceinline__ void cluster_features(float *f, float3 q, float4 p, float4 w, float4 m0,
-                                                 float4 m1, float4 m2, float inv_h) {
+// 16 invariants of one (point, cluster) pair and the three output vectors
+// (times r^-3), B[i][k] = component i of vector k.
+__device__ __forceinline__ void cluster_features(float *f, float *B, float3 q, float4 p, float4 w,
+                                                 float4 m0, float4 m1, float4 m2, float inv_h) {
     const float x = (q.x - p.x) * inv_h, y = (q.y - p.y) * inv_h, z = (q.z - p.z) * inv_h;
+    const float ir = __builtin_amdgcn_rsqf(x * x + y * y + z * z);
+    const float ex = x * ir, ey = y * ir, ez = z * ir;
     const float iS = 1.0f / w.w, iSh = iS * inv_h;
-    f[0] = x; f[1] = y; f[2] = z;
-    f[3] = __builtin_amdgcn_rsqf(x * x + y * y + z * z);
-    f[4] = w.x * iS; f[5] = w.y * iS; f[6] = w.z * iS;
-    f[7] = m0.x * iSh; f[8] = m0.y * iSh; f[9] = m0.z * iSh;
-    f[10] = m1.x * iSh; f[11] = m1.y * iSh; f[12] = m1.z * iSh;
-    f[13] = m2.x * iSh; f[14] = m2.y * iSh; f[15] = m2.z * iSh;
+    const float ox = w.x * iS, oy = w.y * iS, oz = w.z * iS;
+    const float D[3][3] = {{m0.x * iSh, m0.y * iSh, m0.z * iSh},
+                           {m1.x * iSh, m1.y * iSh, m1.z * iSh},
+                           {m2.x * iSh, m2.y * iSh, m2.z * iSh}};
+    const float ax = m0.w * iSh, ay = m1.w * iSh, az = m2.w * iSh;
+    const float ux = D[0][0] * ex + D[0][1] * ey + D[0][2] * ez;
+    const float uy = D[1][0] * ex + D[1][1] * ey + D[1][2] * ez;
+    const float uz = D[2][0] * ex + D[2][1] * ey + D[2][2] * ez;
+    const float vx = D[0][0] * ex + D[1][0] * ey + D[2][0] * ez;
+    const float vy = D[0][1] * ex + D[1][1] * ey + D[2][1] * ez;
+    const float vz = D[0][2] * ex + D[1][2] * ey + D[2][2] * ez;
+    const float oe = ox * ex + oy * ey + oz * ez;
+    const float ae = ax * ex + ay * ey + az * ez;
+    const float eu = ex * ux + ey * uy + ez * uz;
+    // u x e
+    const float wx = uy * ez - uz * ey, wy = uz * ex - ux * ez, wz = ux * ey - uy * ex;
+    float fro = 0.f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) fro += D[i][j] * D[i][j];
+    const float tr = D[0][0] + D[1][1] + D[2][2];
+    f[0] = ir;
+    f[1] = ir * ir;
+    f[2] = ox * ox + oy * oy + oz * oz;
+    f[3] = oe * oe;
+    f[4] = ax * ax + ay * ay + az * az;
+    f[5] = ae * ae;
+    f[6] = ux * ux + uy * uy + uz * uz;
+    f[7] = vx * vx + vy * vy + vz * vz;
+    f[8] = eu * eu;
+    f[9] = fro;
+    f[10] = tr * tr;
+    f[11] = ox * ax + oy * ay + oz * az;
+    f[12] = oe * eu;
+    f[13] = ox * ux + oy * uy + oz * uz;
+    f[14] = ox * vx + oy * vy + oz * vz;
+    f[15] = ax * wx + ay * wy + az * wz;
+    if (B) {
+        const float ir3 = ir * ir * ir;
+        // columns O x e, at, u x e
+        B[0] = (oy * ez - oz * ey) * ir3; B[1] = ax * ir3; B[2] = wx * ir3;
+        B[3] = (oz * ex - ox * ez) * ir3; B[4] = ay * ir3; B[5] = wy * ir3;
+        B[6] = (ox * ey - oy * ex) * ir3; B[7] = az * ir3; B[8] = wz * ir3;
+    }
 }
 
 __device__ __forceinline__ float correction_scale(float4 p, float4 w, float inv_h) {
@@ -450,7 +512,7 @@ __global__ __launch_bounds__(256) void k_mlp_grid(const vg::TreeView tv, vg_grid
     const uint32_t bb[3] = {brick % mp.nb[0], (brick / mp.nb[0]) % mp.nb[1],
                             brick / (mp.nb[0] * mp.nb[1])};
     bool valid;
-    (void)brick_point(g, bb, lane, &valid);
+    const float3 qo = brick_point(g, bb, lane, &valid);  // this lane's own point
     const float3 qa = brick_point(g, bb, col, nullptr);       // pair column tile 0
     const float3 qb = brick_point(g, bb, col + 32, nullptr);  // pair column tile 1
     int cl0[3], cl1[3];
@@ -500,8 +562,8 @@ __global__ __launch_bounds__(256) void k_mlp_grid(const vg::TreeView tv, vg_grid
                     const float4 a0 = S.M0[j], a1 = S.M1[j], a2 = S.M2[j];
                     // layer-1 inputs: lane group 0 holds features 0..7, group 1 8..15
                     float fa[kIn], fb[kIn];
-                    cluster_features(fa, qa, cp, cw, a0, a1, a2, inv_h);
-                    cluster_features(fb, qb, cp, cw, a0, a1, a2, inv_h);
+                    cluster_features(fa, nullptr, qa, cp, cw, a0, a1, a2, inv_h);
+                    cluster_features(fb, nullptr, qb, cp, cw, a0, a1, a2, inv_h);
                     const bf16x8 xa = pack8(fa + 8 * grp), xb = pack8(fb + 8 * grp);
                     f32x16 acc[2][2];
 #pragma unroll
@@ -558,10 +620,13 @@ __global__ __launch_bounds__(256) void k_mlp_grid(const vg::TreeView tv, vg_grid
                         const float other = __shfl(y1[v], (int)col, 64);
                         o[v] = (lane < 32 ? y0[v] : other) + bias[3 * kHid + v];
                     }
+                    // correction = scale * B c at this lane's own point
+                    float fo[kIn], Bo[9];
+                    cluster_features(fo, Bo, qo, cp, cw, a0, a1, a2, inv_h);
                     const float sc = correction_scale(cp, cw, inv_h);
-                    corr.x += o[0] * sc;
-                    corr.y += o[1] * sc;
-                    corr.z += o[2] * sc;
+                    corr.x += (Bo[0] * o[0] + Bo[1] * o[1] + Bo[2] * o[2]) * sc;
+                    corr.y += (Bo[3] * o[0] + Bo[4] * o[1] + Bo[5] * o[2]) * sc;
+                    corr.z += (Bo[6] * o[0] + Bo[7] * o[1] + Bo[8] * o[2]) * sc;
                 }
                 __builtin_amdgcn_wave_barrier();
             }
@@ -607,6 +672,7 @@ struct SampleParams {
 // (direct sum over the cluster's member vortons - its monopole) / scale.
 __global__ __launch_bounds__(256) void k_mlp_samples(const vg::TreeView tv, vg_grid_desc g,
                                                      SampleParams sp, float *__restrict__ X,
+                                                     float *__restrict__ Bout,
                                                      float *__restrict__ T,
                                                      uint32_t *__restrict__ meta,
                                                      uint32_t *__restrict__ failed) {
@@ -646,9 +712,9 @@ __global__ __launch_bounds__(256) void k_mlp_samples(const vg::TreeView tv, vg_g
         if (!mlp_cluster(p, w)) continue;
         const float4 m0 = tv.M[layer][3 * off], m1 = tv.M[layer][3 * off + 1],
                      m2 = tv.M[layer][3 * off + 2];
-        float f[kIn];
+        float f[kIn], Bm[9];
         const float inv_h = sp.inv_h[layer];
-        cluster_features(f, q, p, w, m0, m1, m2, inv_h);
+        cluster_features(f, Bm, q, p, w, m0, m1, m2, inv_h);
         // member vortons: leaf cells of the cluster, row by row
         int x0[3], x1[3];
         for (int a = 0; a < 3; ++a) {
@@ -696,6 +762,8 @@ __global__ __launch_bounds__(256) void k_mlp_samples(const vg::TreeView tv, vg_g
         }
         const float isc = 1.0f / correction_scale(p, w, inv_h);
         for (int c = 0; c < kIn; ++c) X[(size_t)i * kIn + c] = f[c];
+        if (Bout)
+            for (int c = 0; c < 9; ++c) Bout[(size_t)i * 9 + c] = Bm[c];
         T[(size_t)i * kOut + 0] = (sum.x - mono.x) * sp.circ * isc;
         T[(size_t)i * kOut + 1] = (sum.y - mono.y) * sp.circ * isc;
         T[(size_t)i * kOut + 2] = (sum.z - mono.z) * sp.circ * isc;
@@ -713,6 +781,8 @@ __global__ __launch_bounds__(256) void k_mlp_samples(const vg::TreeView tv, vg_g
     }
     for (int c = 0; c < kIn; ++c) X[(size_t)i * kIn + c] = 0.f;
     for (int c = 0; c < kOut; ++c) T[(size_t)i * kOut + c] = 0.f;
+    if (Bout)
+        for (int c = 0; c < 9; ++c) Bout[(size_t)i * 9 + c] = 0.f;
     if (meta) meta[2 * (size_t)i] = meta[2 * (size_t)i + 1] = 0xffffffffu;
     atomicAdd(failed, 1u);
 }
@@ -812,15 +882,15 @@ vg_status forward(vg_mlp *m, const float *X, uint32_t rows, Acts &a, hipStream_t
 }
 
 // Loss gradient and parameter gradients into m->grad (padded layout).
-vg_status backward(vg_mlp *m, const float *X, const float *T, uint32_t rows, Acts &a,
-                   hipStream_t s) {
+vg_status backward(vg_mlp *m, const float *X, const float *Bm, const float *T, uint32_t rows,
+                   Acts &a, hipStream_t s) {
     const float *P = m->params;
     float *G = m->grad;
     const size_t h = (size_t)rows * kHid;
     const dim3 eg(ceil_div(h, 256));
     const uint32_t nparts = ceil_div(rows, 256);
-    VG_TRY(launch("vg_mlp_loss", k_loss_grad, dim3(nparts), dim3(256), 0, s, (const float *)a.Y, T,
-                  a.dY, rows, 2.0f / (3.0f * (float)rows), a.partial));
+    VG_TRY(launch("vg_mlp_loss", k_loss_grad, dim3(nparts), dim3(256), 0, s, (const float *)a.Y, Bm,
+                  T, a.dY, rows, 2.0f / (3.0f * (float)rows), a.partial));
     VG_TRY(launch("vg_mlp_loss", k_sum_partials, dim3(1), dim3(64), 0, s, (const float *)a.partial,
                   nparts, a.loss));
     // output layer
@@ -958,15 +1028,15 @@ VG_API vg_status vg_mlp_forward(vg_mlp *m, const float *x, uint32_t n, float *y,
     return st;
 }
 
-VG_API vg_status vg_mlp_gradients(vg_mlp *m, const float *x, const float *t, uint32_t n,
-                                  float *grad, float *loss, vg_stream stream) {
+VG_API vg_status vg_mlp_gradients(vg_mlp *m, const float *x, const float *basis, const float *t,
+                                  uint32_t n, float *grad, float *loss, vg_stream stream) {
     VG_REQUIRE(m && x && t && grad, "null argument");
     VG_REQUIRE(n > 0, "empty batch");
     hipStream_t s = (hipStream_t)stream;
     Acts a;
     VG_TRY(acts_alloc(&a, n, s));
     vg_status st = forward(m, x, n, a, s);
-    if (st == VG_OK) st = backward(m, x, t, n, a, s);
+    if (st == VG_OK) st = backward(m, x, basis, t, n, a, s);
     float l = 0.f;
     std::vector<float> g(kPadParams);
     if (st == VG_OK) {
@@ -983,9 +1053,9 @@ VG_API vg_status vg_mlp_gradients(vg_mlp *m, const float *x, const float *t, uin
     return VG_OK;
 }
 
-VG_API vg_status vg_mlp_train(vg_mlp *m, const float *x, const float *t, uint32_t n,
-                              uint32_t batch, uint32_t steps, float lr, float *final_loss,
-                              vg_stream stream) {
+VG_API vg_status vg_mlp_train(vg_mlp *m, const float *x, const float *basis, const float *t,
+                              uint32_t n, uint32_t batch, uint32_t steps, float lr,
+                              float *final_loss, vg_stream stream) {
     VG_REQUIRE(m && x && t, "null argument");
     VG_REQUIRE(batch > 0 && batch <= n, "batch must be in [1, n]");
     hipStream_t s = (hipStream_t)stream;
@@ -996,8 +1066,9 @@ VG_API vg_status vg_mlp_train(vg_mlp *m, const float *x, const float *t, uint32_
     for (uint32_t it = 0; it < steps && st == VG_OK; ++it) {
         const uint64_t off = ((uint64_t)it * batch) % span;
         const float *xb = x + off * kIn, *tb = t + off * kOut;
+        const float *bb = basis ? basis + off * 9 : nullptr;
         st = forward(m, xb, batch, a, s);
-        if (st == VG_OK) st = backward(m, xb, tb, batch, a, s);
+        if (st == VG_OK) st = backward(m, xb, bb, tb, batch, a, s);
         if (st == VG_OK) st = adam_step(m, lr, s);
     }
     float l = 0.f;
@@ -1013,8 +1084,8 @@ VG_API vg_status vg_mlp_train(vg_mlp *m, const float *x, const float *t, uint32_
 
 VG_API vg_status vg_tree_mlp_samples(const vg_tree *t, const vg_grid_desc *g, int near_k,
                                      uint32_t layer_mask, float spread, float circ, uint32_t n,
-                                     uint64_t seed, float *x, float *target, uint32_t *meta,
-                                     uint32_t *n_failed, vg_stream stream) {
+                                     uint64_t seed, float *x, float *basis, float *target,
+                                     uint32_t *meta, uint32_t *n_failed, vg_stream stream) {
     TreeView tv;
     VG_TRY(tree_checked_view(t, &tv));
     VG_REQUIRE(g && x && target, "null argument");
@@ -1045,7 +1116,7 @@ VG_API vg_status vg_tree_mlp_samples(const vg_tree *t, const vg_grid_desc *g, in
     vg_status st = zero_fill(d, sizeof(uint32_t), s);
     if (st == VG_OK)
         st = launch("vg_mlp_samples", k_mlp_samples, dim3(ceil_div(n, 256)), dim3(256), 0, s, tv, *g,
-                    sp, x, target, meta, d);
+                    sp, x, basis, target, meta, d);
     uint32_t h = 0;
     if (st == VG_OK) {
         hipError_t e = hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, s);

@@ -3,8 +3,10 @@
 The network (vg_mlp) is trained on pairs the direct solver generates: for
 random (grid point, applied cluster) pairs of the near-field treecode, the
 target is the direct sum over the cluster's member vortons minus the
-cluster's monopole (vg_tree_mlp_samples).  The training fields are drawn
-from the vortex-ring, colliding-ring and buoyant-plume distributions, each
+cluster's monopole and analytic first-moment term (vg_tree_mlp_samples);
+the loss is on the correction vector B c that the network's outputs c
+weight.  The training fields are drawn from the vortex-ring, colliding-ring
+and buoyant-plume distributions, each
 under several random rotations (the Biot-Savart law is rotation-equivariant
 and these generators are axis-aligned); nothing here reads a file.
 
@@ -39,10 +41,10 @@ def random_rotation(rng) -> np.ndarray:
 
 def training_pairs(near_k: int, pairs_per_field: int = 32768, rotations: int = 4,
                    layer_mask: int = 0xFFFF, fields=TRAINING_FIELDS, seed: int = 60):
-    """Host arrays (x [n,16], target [n,3]) from rotated copies of the
-    training fields."""
+    """Host arrays (x [n,16], basis [n,3,3], target [n,3]) from rotated
+    copies of the training fields."""
     rng = np.random.default_rng(seed)
-    xs, ts = [], []
+    xs, bs, ts = [], [], []
     for k, (gen, n, s) in enumerate(fields):
         pos0, om0, radius = gen(n, seed=s)
         centre = pos0.mean(0)
@@ -57,17 +59,19 @@ def training_pairs(near_k: int, pairs_per_field: int = 32768, rotations: int = 4
             t = vg.Tree()
             t.build(g, v)
             spread, circ = vg.auto_mollify(g, radius)
-            x, tt, _, failed = t.mlp_samples(g, pairs_per_field, near_k=near_k,
-                                             layer_mask=layer_mask, spread=spread, circ=circ,
-                                             seed=seed + 16 * k + r)
+            x, b, tt, _, failed = t.mlp_samples(g, pairs_per_field, near_k=near_k,
+                                                layer_mask=layer_mask, spread=spread, circ=circ,
+                                                seed=seed + 16 * k + r)
             if failed:
                 raise RuntimeError(f"{failed} training pairs could not be drawn")
             xs.append(x.numpy())
+            bs.append(b.numpy())
             ts.append(tt.numpy())
     X = np.concatenate(xs)
+    B = np.concatenate(bs)
     T = np.concatenate(ts)
     perm = rng.permutation(len(X))
-    return X[perm], T[perm]
+    return X[perm], B[perm], T[perm]
 
 
 def fold_scalings(params: np.ndarray, mu: np.ndarray, sigma: np.ndarray, t_scale: float):
@@ -90,7 +94,7 @@ def train_far_field(near_k: int = 1, pairs_per_field: int = 32768, rotations: in
                     seed: int = 7, layer_mask: int = 0xFFFF):
     """Returns (trained vg.Mlp, info dict with the final loss and timings)."""
     t0 = time.perf_counter()
-    X, T = training_pairs(near_k, pairs_per_field, rotations, layer_mask)
+    X, B, T = training_pairs(near_k, pairs_per_field, rotations, layer_mask)
     t1 = time.perf_counter()
     mu = X.mean(0).astype(np.float64)
     sigma = X.std(0).astype(np.float64)
@@ -100,9 +104,10 @@ def train_far_field(near_k: int = 1, pairs_per_field: int = 32768, rotations: in
     Tn = (T / t_scale).astype(np.float32)
     m = vg.Mlp(seed=seed)
     dx, dt = vg.DeviceArray.from_numpy(Xn), vg.DeviceArray.from_numpy(Tn)
+    db = vg.DeviceArray.from_numpy(B.astype(np.float32))
     loss = None
     for n_steps, lr in zip(steps, lrs):
-        loss = m.train(dx, dt, len(X), batch=batch, steps=n_steps, lr=lr)
+        loss = m.train(dx, dt, len(X), batch=batch, steps=n_steps, lr=lr, basis=db)
     vg.synchronize()
     t2 = time.perf_counter()
     m.set_params(fold_scalings(m.params(), mu, sigma, t_scale))

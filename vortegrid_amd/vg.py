@@ -361,16 +361,17 @@ class Tree:
 
     def mlp_samples(self, g: GridDesc, n: int, near_k=1, layer_mask=0xFFFF, spread=1.0,
                     circ=1.0, seed=1, stream=None):
-        """(x [n,16], target [n,3], meta [n,2] uint32) device arrays of
-        training pairs and the number of pairs left empty."""
+        """(x [n,16], basis [n,3,3], target [n,3], meta [n,2] uint32) device
+        arrays of training pairs and the number of pairs left empty."""
         x = DeviceArray.zeros((max(n, 1), MLP_INPUTS))
+        b = DeviceArray.zeros((max(n, 1), 3, 3))
         t = DeviceArray.zeros((max(n, 1), MLP_OUTPUTS))
         meta = DeviceArray.zeros((max(n, 1), 2), np.uint32)
         failed = C.c_uint32()
         call("vg_tree_mlp_samples", self.h, C.byref(g), int(near_k), C.c_uint32(layer_mask),
-             C.c_float(spread), C.c_float(circ), C.c_uint32(n), C.c_uint64(seed), x.ptr, t.ptr,
-             meta.ptr, C.byref(failed), _s(stream))
-        return x, t, meta, failed.value
+             C.c_float(spread), C.c_float(circ), C.c_uint32(n), C.c_uint64(seed), x.ptr, b.ptr,
+             t.ptr, meta.ptr, C.byref(failed), _s(stream))
+        return x, b, t, meta, failed.value
 
     def velocity_grid_mlp(self, g: GridDesc, mlp: "Mlp", out: DeviceArray, near_k=1,
                           layer_mask=0xFFFF, spread=1.0, circ=1.0, policy=RSQRT_ACCURATE,
@@ -570,18 +571,19 @@ class Mlp:
     def forward(self, x: DeviceArray, n: int, y: DeviceArray, stream=None):
         call("vg_mlp_forward", self.h, x.ptr, C.c_uint32(n), y.ptr, _s(stream))
 
-    def gradients(self, x: DeviceArray, t: DeviceArray, n: int, stream=None):
+    def gradients(self, x: DeviceArray, t: DeviceArray, n: int, basis=None, stream=None):
         g = np.zeros(MLP_PARAMS, np.float32)
         loss = C.c_float()
-        call("vg_mlp_gradients", self.h, x.ptr, t.ptr, C.c_uint32(n),
-             g.ctypes.data_as(C.c_void_p), C.byref(loss), _s(stream))
+        call("vg_mlp_gradients", self.h, x.ptr, basis.ptr if basis is not None else None, t.ptr,
+             C.c_uint32(n), g.ctypes.data_as(C.c_void_p), C.byref(loss), _s(stream))
         return loss.value, g
 
     def train(self, x: DeviceArray, t: DeviceArray, n: int, batch=4096, steps=1000, lr=1e-3,
-              stream=None) -> float:
+              basis=None, stream=None) -> float:
         loss = C.c_float()
-        call("vg_mlp_train", self.h, x.ptr, t.ptr, C.c_uint32(n), C.c_uint32(batch),
-             C.c_uint32(steps), C.c_float(lr), C.byref(loss), _s(stream))
+        call("vg_mlp_train", self.h, x.ptr, basis.ptr if basis is not None else None, t.ptr,
+             C.c_uint32(n), C.c_uint32(batch), C.c_uint32(steps), C.c_float(lr), C.byref(loss),
+             _s(stream))
         return loss.value
 
     def __del__(self):
