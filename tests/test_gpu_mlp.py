@@ -107,7 +107,12 @@ def _cluster_arrays(ot, pos, om):
         Pa = np.concatenate([P, size[:, None]], 1)
         Wa = np.concatenate([W, S[l][:, None]], 1)
         Ma = np.zeros((len(P), 3, 4))
-        Ma[:, :, :3] = D.reshape(-1, 3, 3)
+        D3 = D.reshape(-1, 3, 3)
+        Ma[:, :, :3] = D3
+        # antisymmetric part, as the tree stores it in .w
+        Ma[:, 0, 3] = D3[:, 1, 2] - D3[:, 2, 1]
+        Ma[:, 1, 3] = D3[:, 2, 0] - D3[:, 0, 2]
+        Ma[:, 2, 3] = D3[:, 0, 1] - D3[:, 1, 0]
         out.append((Pa, Wa, Ma.reshape(-1, 12)))
     return out
 

@@ -22,11 +22,16 @@ import numpy as np
 
 from . import scenarios, vg
 
-# (generator, vorton count, seed) of the training fields
+# (generator, vorton count, seed, keyword arguments) of the training fields:
+# the ring, colliding-ring and plume families over a range of shapes
 TRAINING_FIELDS = (
-    (scenarios.vortex_ring, 16384, 51),
-    (scenarios.colliding_rings, 32768, 52),
-    (scenarios.jet_plume, 32768, 53),
+    (scenarios.vortex_ring, 16384, 51, {}),
+    (scenarios.vortex_ring, 16384, 54, {"R": 0.5, "r": 0.15}),
+    (scenarios.vortex_ring, 16384, 55, {"R": 1.2, "r": 0.35}),
+    (scenarios.colliding_rings, 32768, 52, {}),
+    (scenarios.colliding_rings, 32768, 56, {"separation": 0.6, "r": 0.3}),
+    (scenarios.jet_plume, 32768, 53, {}),
+    (scenarios.jet_plume, 32768, 57, {"sigma": 0.4, "length": 2.0}),
 )
 N_IN = vg.MLP_INPUTS
 
@@ -39,14 +44,14 @@ def random_rotation(rng) -> np.ndarray:
     return q
 
 
-def training_pairs(near_k: int, pairs_per_field: int = 32768, rotations: int = 4,
+def training_pairs(near_k: int, pairs_per_field: int = 24576, rotations: int = 4,
                    layer_mask: int = 0xFFFF, fields=TRAINING_FIELDS, seed: int = 60):
     """Host arrays (x [n,16], basis [n,3,3], target [n,3]) from rotated
     copies of the training fields."""
     rng = np.random.default_rng(seed)
     xs, bs, ts = [], [], []
-    for k, (gen, n, s) in enumerate(fields):
-        pos0, om0, radius = gen(n, seed=s)
+    for k, (gen, n, s, kw) in enumerate(fields):
+        pos0, om0, radius = gen(n, seed=s, **kw)
         centre = pos0.mean(0)
         for r in range(rotations):
             R = random_rotation(rng).astype(np.float32)
@@ -89,7 +94,7 @@ def fold_scalings(params: np.ndarray, mu: np.ndarray, sigma: np.ndarray, t_scale
     return p.astype(np.float32)
 
 
-def train_far_field(near_k: int = 1, pairs_per_field: int = 32768, rotations: int = 4,
+def train_far_field(near_k: int = 1, pairs_per_field: int = 24576, rotations: int = 4,
                     steps=(3000, 2000, 1000), lrs=(2e-3, 5e-4, 1e-4), batch: int = 4096,
                     seed: int = 7, layer_mask: int = 0xFFFF):
     """Returns (trained vg.Mlp, info dict with the final loss and timings)."""
