@@ -58,7 +58,7 @@ def main():
     ap.add_argument("--near-k", type=int, default=3,
                     help="near-field radius of the accurate treecode; -1 = reference opening rule")
     ap.add_argument("--order", type=int, default=1, help="cluster order: 0 monopole, 1 dipole")
-    ap.add_argument("--expand-from", type=int, default=-1,
+    ap.add_argument("--expand-from", type=int, default=2,
                     help="near-field treecode: first layer applied through brick-centred local "
                          "expansions (-1 = none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -325,11 +325,13 @@ def main():
 # HBM-side bytes per launch of the dominant kernel, from the two counter
 # runs (FETCH_SIZE, WRITE_SIZE) summarised under profiles/ for the default
 # workload; None for other settings.
-TRAFFIC_FILES = {(3, 1, 1 << 20): "tree_mac_k3_1M_traffic.txt"}
+TRAFFIC_FILES = {(3, 1, 2, 1 << 20): "tree_mac_k3_e2_1M_traffic.txt"}
 
 
 def measured_traffic(args):
-    name = TRAFFIC_FILES.get((args.near_k, args.order, args.vortons))
+    if args.mlp:
+        return None
+    name = TRAFFIC_FILES.get((args.near_k, args.order, args.expand_from, args.vortons))
     path = os.path.join(ROOT, "profiles", name) if name else None
     if not path or not os.path.exists(path):
         return None
