@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--batched", action="store_true",
                     help="batched effects: 256 independent effects, one recorded frame launch")
     ap.add_argument("--effects", type=int, default=256)
+    ap.add_argument("--batch-near-k", type=int, default=2)
+    ap.add_argument("--batch-expand-from", type=int, default=2)
     ap.add_argument("--copy-probe", action="store_true",
                     help="one 256 MiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -191,6 +193,10 @@ def main():
     if expansions:
         flops += FLOP_EXPAND_EVAL * g.capacity
     cnt, ms = prof.get(kname, (0, 0.0))
+    if kname == "vg_tree_velocity_grid_mac" and expansions:
+        # the expansion launch belongs to the same solve
+        ms += prof.get("vg_tree_mac_expand", (0, 0.0))[1]
+        kname = "vg_tree_velocity_grid_mac + vg_tree_mac_expand"
     avg_s = (ms / max(cnt, 1)) * 1e-3
     achieved = flops / avg_s / 1e12 if avg_s > 0 else 0.0
     roofline = {"kernel": kname, "bound": "mfma", "achieved": achieved, "peak": peak,
@@ -266,8 +272,9 @@ def main():
             else:
                 rows = 1 if args.near_k >= 2 else 2
                 bz0 = (nz + 3) // 8  # a brick row a quarter of the way up the grid
-                O.mac_velocity_grid_bricks(ot, og, cur, om, 2 * radius, args.near_k, args.order,
-                                           spread, circ, args.policy, cores, bz0, bz0 + rows)
+                O.mac_velocity_grid_ex(ot, og, cur, om, 2 * radius, args.near_k, args.order,
+                                       args.expand_from, spread, circ, args.policy, cores,
+                                       bz0, bz0 + rows)
                 slices = 4 * rows
             t_vel = time.perf_counter() - tv
             pts = int(g.np[0]) * int(g.np[1]) * slices
@@ -367,9 +374,9 @@ def batched(args):
     dt = np.float32(1.0 / 30.0)
     log(f"generating {args.effects} effects")
     vp, vw, tp, radii = effect_fields(args.effects)
-    nk = args.near_k if args.near_k >= 0 else 2
+    nk = args.batch_near_k
     b = vg.Batch([len(p) for p in vp], [len(t) for t in tp], radii, cells=32, near_k=nk,
-                 order=args.order, policy=args.policy)
+                 order=args.order, policy=args.policy, expand_from=args.batch_expand_from)
     b.upload(np.concatenate(vp), np.concatenate(vw), np.concatenate(tp))
     s = vg.Stream()
     b.step(dt, s)
@@ -432,7 +439,9 @@ def batched(args):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{args.effects} effects, N log-uniform [512, 32768], rings or "
                                f"plumes, 16 tracers per vorton, 32^3-cell grid each, near-field "
-                               f"treecode near_k={nk} {'dipole' if args.order else 'monopole'}",
+                               f"treecode near_k={nk} {'dipole' if args.order else 'monopole'}"
+                               + (f", local expansions from layer {args.batch_expand_from}"
+                                  if args.batch_expand_from >= 1 else ""),
                    "vortons": b.nv, "tracers": b.nt, "effects": args.effects},
         "rel_l2_vs_fp64_direct": rel,
         "launch_shapes": dict(zip(("walk_waves", "tracer_block", "tracer_items"), b.get_launch())),

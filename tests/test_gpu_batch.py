@@ -47,9 +47,12 @@ def new_batch(vg, counts, seed=1, **kw):
 COUNTS = [700, 2500, 0, 1800, 300, 4096]
 
 
-def test_batch_frame_matches_single_trees(vg):
-    case(f"batched frame vs single trees, effects {COUNTS}, cells 16, k=1 dipole")
-    b, vp, vw, tp, radii = new_batch(vg, COUNTS, cells=16, near_k=1, order=1)
+@pytest.mark.parametrize("expand_from", [-1, 2])
+def test_batch_frame_matches_single_trees(vg, expand_from):
+    case(f"batched frame vs single trees, effects {COUNTS}, cells 16, k=1 dipole, "
+         f"expand_from={expand_from}")
+    b, vp, vw, tp, radii = new_batch(vg, COUNTS, cells=16, near_k=1, order=1,
+                                     expand_from=expand_from)
     b.step(DT)
     vg.synchronize()
     vpos1, vvel1, tpos1 = b.download("vpos"), b.download("vvel"), b.download("tpos")
@@ -75,7 +78,7 @@ def test_batch_frame_matches_single_trees(vg):
             t = vg.Tree()
             t.build(g, v)
             out = vg.DeviceArray.zeros((g.capacity, 3))
-            t.velocity_grid_mac(g, out, 1, 1, spread, circ, policy=1)
+            t.velocity_grid_mac(g, out, 1, 1, spread, circ, policy=1, expand_from=expand_from)
             vg.synchronize()
             assert np.array_equal(out.numpy(), field), e
             # vortons: AssignVelocityFromField + Evolve with that grid

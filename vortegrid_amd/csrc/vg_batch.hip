@@ -81,31 +81,51 @@ __global__ void k_bbox_init(uint32_t *__restrict__ bbox, uint32_t effects) {
     bbox[i] = (i % 6) < 3 ? 0xffffffffu : 0u;
 }
 
-// Extrema of items (positions packed xyz) per owner; a wave whose items share
-// one owner reduces before its atomics.  Min / max are exact, so the result
+// Extrema of items (positions packed xyz) per owner.  Owners are contiguous,
+// so each wave folds a contiguous chunk of kBoxItems items: a lane keeps
+// running extrema of its items while the owner stays the same and flushes
+// them with atomics when it changes; at the end a wave whose lanes share one
+// owner reduces before its 6 atomics.  Min / max are exact, so the result
 // does not depend on the order.
+constexpr int kBoxItems = 16;  // items per lane
+
+__device__ __forceinline__ void box_flush(uint32_t *bbox, uint32_t k, const uint32_t *e) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        atomicMin(&bbox[6 * k + a], e[a]);
+        atomicMax(&bbox[6 * k + 3 + a], e[3 + a]);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_bbox_batch(const float *__restrict__ pos,
                                                     const uint32_t *__restrict__ owner, size_t n,
                                                     uint32_t *__restrict__ bbox) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < n;
-    const uint32_t k = live ? owner[i] : 0xffffffffu;
-    float p[3] = {0.f, 0.f, 0.f};
-    if (live) {
-        p[0] = pos[3 * i];
-        p[1] = pos[3 * i + 1];
-        p[2] = pos[3 * i + 2];
-    }
-    const uint32_t k0 = __builtin_amdgcn_readfirstlane(k);
-    const bool uniform = __ballot(live && k != k0) == 0ull;
-    if (uniform) {
-        if (k0 == 0xffffffffu) return;
-        uint32_t e[6];
+    const uint32_t lane = threadIdx.x & 63;
+    const size_t wave = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const size_t base = wave * 64 * kBoxItems;
+    uint32_t cur = 0xffffffffu;
+    uint32_t e[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+    for (int it = 0; it < kBoxItems; ++it) {
+        const size_t i = base + (size_t)it * 64 + lane;
+        if (i >= n) break;
+        const uint32_t k = owner[i];
+        if (k != cur) {
+            if (cur != 0xffffffffu) box_flush(bbox, cur, e);
+            cur = k;
+            e[0] = e[1] = e[2] = 0xffffffffu;
+            e[3] = e[4] = e[5] = 0u;
+        }
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            e[a] = live ? ord_enc(p[a]) : 0xffffffffu;
-            e[3 + a] = live ? ord_enc(p[a]) : 0u;
+            const uint32_t v = ord_enc(pos[3 * i + a]);
+            e[a] = min(e[a], v);
+            e[3 + a] = max(e[3 + a], v);
         }
+    }
+    const uint32_t k0 = __builtin_amdgcn_readfirstlane(cur);
+    const bool uniform = __ballot(cur != 0xffffffffu && cur != k0) == 0ull;
+    if (uniform) {
+        if (k0 == 0xffffffffu) return;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
@@ -113,18 +133,9 @@ __global__ __launch_bounds__(256) void k_bbox_batch(const float *__restrict__ po
                 e[a] = min(e[a], (uint32_t)__shfl_xor((int)e[a], off, 64));
                 e[3 + a] = max(e[3 + a], (uint32_t)__shfl_xor((int)e[3 + a], off, 64));
             }
-        if ((threadIdx.x & 63) == 0)
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                atomicMin(&bbox[6 * k0 + a], e[a]);
-                atomicMax(&bbox[6 * k0 + 3 + a], e[3 + a]);
-            }
-    } else if (live) {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            atomicMin(&bbox[6 * k + a], ord_enc(p[a]));
-            atomicMax(&bbox[6 * k + 3 + a], ord_enc(p[a]));
-        }
+        if (lane == 0) box_flush(bbox, k0, e);
+    } else if (cur != 0xffffffffu) {
+        box_flush(bbox, cur, e);
     }
 }
 
@@ -256,11 +267,13 @@ vg_status frame(vg_batch *b, float dt, hipStream_t s) {
     VG_TRY(launch("vg_batch_bbox", k_bbox_init, dim3(ceil_div(6 * b->effects, 256)), dim3(256), 0, s,
                   b->bbox, b->effects));
     if (b->nv)
-        VG_TRY(launch("vg_batch_bbox", k_bbox_batch, dim3(ceil_div(b->nv, 256)), dim3(256), 0, s,
-                      (const float *)b->vpos, (const uint32_t *)b->vtree, b->nv, b->bbox));
+        VG_TRY(launch("vg_batch_bbox", k_bbox_batch, dim3(ceil_div(b->nv, 256 * kBoxItems)),
+                      dim3(256), 0, s, (const float *)b->vpos, (const uint32_t *)b->vtree, b->nv,
+                      b->bbox));
     if (b->nt)
-        VG_TRY(launch("vg_batch_bbox", k_bbox_batch, dim3(ceil_div(b->nt, 256)), dim3(256), 0, s,
-                      (const float *)b->tpos, (const uint32_t *)b->ttree, b->nt, b->bbox));
+        VG_TRY(launch("vg_batch_bbox", k_bbox_batch, dim3(ceil_div(b->nt, 256 * kBoxItems)),
+                      dim3(256), 0, s, (const float *)b->tpos, (const uint32_t *)b->ttree, b->nt,
+                      b->bbox));
     Shape sh;
     sh.depth = b->shape.depth;
     for (uint32_t l = 0; l < kMaxLayers; ++l)
