@@ -432,6 +432,56 @@ def batched(args):
     log(f"rel-L2 vs FP64 direct (8 effects, 1% of their grid points): {rel:.3e}")
     cnt, wms = prof.get("vg_batch_velocity_grid", (1, 0.0))
     tcnt, tms = prof.get("vg_batch_tracers", (1, 0.0))
+    # CPU baseline: the oracle on 8 sampled effects (tree build, the same
+    # near-field walk with expansions, vorton and tracer advection), scaled to
+    # all effects by vortons (build, vorton advection), tracers (tracer
+    # advection) and effect count (walk on the fixed-size grids)
+    cpu = None
+    if not args.no_cpu:
+        try:
+            from oracle import oracle as O
+
+            O.build()
+            cores = min(16, os.cpu_count() or 1)
+            tpos = b.download("tpos")
+            toff = np.cumsum([0] + [len(t) for t in tp])
+            t_build = t_walk = t_adv = 0.0
+            nv_s = nt_s = 0
+            sample = rng.choice(args.effects, 8, replace=False)
+            for e in sample:
+                g, field = b.grid(int(e))
+                spread, circ = b.mollify(int(e))
+                og = O.Grid()
+                for k in ("min", "extent", "cell", "cpe", "np"):
+                    for a in range(3):
+                        getattr(og, k)[a] = getattr(g, k)[a]
+                pe = cur[off[e]:off[e + 1]]
+                t0 = time.perf_counter()
+                ot = O.InfluenceTree(og, pe, vw[e], 2 * radii[e])
+                t1 = time.perf_counter()
+                vf = O.mac_velocity_grid_ex(ot, og, pe, vw[e], 2 * radii[e], nk, args.order,
+                                            args.batch_expand_from, spread, circ, args.policy,
+                                            cores)
+                t2 = time.perf_counter()
+                O.advect(og, vf, pe, np.zeros_like(pe), float(dt), 1.0)
+                O.advect(og, vf, tpos[toff[e]:toff[e + 1]],
+                         np.zeros((toff[e + 1] - toff[e], 3), np.float32), float(dt), 1.0)
+                t3 = time.perf_counter()
+                t_build += t1 - t0
+                t_walk += t2 - t1
+                t_adv += t3 - t2
+                nv_s += len(pe)
+                nt_s += toff[e + 1] - toff[e]
+            frame_s = (t_build * b.nv / nv_s + t_walk * args.effects / len(sample) +
+                       t_adv * (b.nv + b.nt) / (nv_s + nt_s))
+            cpu = {"value": pts / frame_s, "unit": "velocity evals/s", "cores": cores,
+                   "kind": "port", "ms_per_frame": frame_s * 1e3,
+                   "sample": (f"oracle on {len(sample)} of {args.effects} effects: tree build "
+                              f"(1 thread), near-field walk ({cores} threads), vorton + tracer "
+                              f"advection (1 thread); scaled by vortons, effects and particles")}
+            log(f"cpu baseline {cpu['ms_per_frame']:.1f} ms/frame")
+        except Exception as ex:  # the CPU leg never blocks the GPU number
+            log(f"cpu baseline skipped: {ex!r}")
     out = {
         "metric": "batched effects: ms/frame and grid velocity evals/s, one recorded frame launch",
         "value": pts / (ms * 1e-3), "unit": "velocity evals/s", "n_gpus": 1, "steps": args.steps,
@@ -447,6 +497,7 @@ def batched(args):
         "launch_shapes": dict(zip(("walk_waves", "tracer_block", "tracer_items"), b.get_launch())),
         "walk_ms": wms / max(cnt, 1), "tracers_ms": tms / max(tcnt, 1),
         "tracers_GBs": 24.0 * b.nt / (tms / max(tcnt, 1) * 1e-3) / 1e9,
+        "cpu_baseline": cpu,
         "kernels_ms_one_frame": {k: {"count": c, "total_ms": m} for k, (c, m) in prof.items()},
     }
     print(json.dumps(out))
