@@ -13,6 +13,11 @@ second over the whole frame; rel-L2 of the treecode against an FP64 direct
 sum on a seeded 1% sample of grid points is reported beside it.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--vortons N] [--tracers T]
+               [--near-k K] [--order O] [--expand-from L] [--no-cpu] [--no-accuracy]
+Other workloads (one JSON line each): --suite (direct sums, a 65k-vorton
+treecode frame with 1M tracers, the vector-Poisson solve), --batched (256
+effects in one recorded frame launch), --mlp (the learned far-field
+correction), --copy-probe (counter calibration).
 """
 from __future__ import annotations
 
