@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "vg_internal.h"
+#include "vg_rbgs.h"
 
 namespace vg {
 vg_status sor_solve(const vg_grid_desc &g, float *soln, const float *lap, uint32_t steps,
@@ -58,6 +59,20 @@ vg_status grow(T **p, size_t *cap, size_t need) {
     return VG_OK;
 }
 
+// Point o of a grid as (ix, iy, iz), in 32-bit arithmetic (solves require
+// fewer than 2^32 points): 64-bit division costs several times more.
+struct Pt {
+    uint32_t x, y, z;
+};
+__device__ inline Pt split(uint32_t o, uint32_t nx, uint32_t ny) {
+    const uint32_t row = o / nx;
+    return Pt{o - row * nx, row % ny, row / ny};
+}
+__device__ inline bool on_boundary(const Pt &p, const vg_grid_desc &g) {
+    return p.x == 0 || p.y == 0 || p.z == 0 || p.x + 1 == g.np[0] || p.y + 1 == g.np[1] ||
+           p.z + 1 == g.np[2];
+}
+
 __global__ void k_zero_interior(vg_grid_desc g, float *__restrict__ a) {
     const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= capacity(g)) return;
@@ -76,6 +91,7 @@ __global__ void k_zero_all(float *__restrict__ a, size_t n) {
 
 struct Lap {
     float r2x, r2y, r2z, diag;  // diag = 2 (r2x + r2y + r2z)
+    float inv;                  // 1 / diag
 };
 
 Lap lap_coef(const vg_grid_desc &g) {
@@ -85,6 +101,7 @@ Lap lap_coef(const vg_grid_desc &g) {
     c.r2y = ry * ry;
     c.r2z = rz * rz;
     c.diag = 2.0f * (c.r2x + c.r2y + c.r2z);
+    c.inv = 1.0f / c.diag;
     return c;
 }
 
@@ -92,13 +109,14 @@ Lap lap_coef(const vg_grid_desc &g) {
 // the colour's points of one x row: ix = 1 + ((iy + iz + color + 1) & 1) + 2k.
 __global__ void k_rbgs(vg_grid_desc g, float *__restrict__ u, const float *__restrict__ f,
                        int color, Lap c) {
+#pragma clang fp contract(off)
     const uint32_t iy = 1 + blockIdx.y, iz = 1 + blockIdx.z;
     const uint32_t first = 1 + ((iy + iz + 1 + (uint32_t)color) & 1u);
     const uint32_t ix = first + 2 * (blockIdx.x * blockDim.x + threadIdx.x);
     const size_t nx = g.np[0], nxy = (size_t)g.np[0] * g.np[1];
     if (ix + 1 >= nx) return;
     const size_t o = ix + (size_t)iy * nx + (size_t)iz * nxy;
-    const float inv = 1.0f / c.diag;
+    const float inv = c.inv;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const float s = (u[3 * (o + 1) + k] + u[3 * (o - 1) + k]) * c.r2x +
@@ -113,9 +131,8 @@ __global__ void k_residual32(vg_grid_desc g, const float *__restrict__ u,
                              const float *__restrict__ f, float *__restrict__ r, Lap c) {
     const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= capacity(g)) return;
-    const size_t nx = g.np[0], ny = g.np[1], nz = g.np[2], nxy = nx * ny;
-    const size_t ix = o % nx, iy = (o / nx) % ny, iz = o / nxy;
-    if (ix == 0 || iy == 0 || iz == 0 || ix + 1 == nx || iy + 1 == ny || iz + 1 == nz) {
+    const size_t nx = g.np[0], nxy = nx * g.np[1];
+    if (on_boundary(split((uint32_t)o, g.np[0], g.np[1]), g)) {
         r[3 * o] = r[3 * o + 1] = r[3 * o + 2] = 0.f;
         return;
     }
@@ -137,9 +154,9 @@ __global__ void k_restrict_fw(vg_grid_desc lo, float *__restrict__ flo, float *_
     const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= capacity(lo)) return;
     ulo[3 * o] = ulo[3 * o + 1] = ulo[3 * o + 2] = 0.f;
-    const size_t nx = lo.np[0], ny = lo.np[1], nz = lo.np[2];
-    const size_t ix = o % nx, iy = (o / nx) % ny, iz = o / (nx * ny);
-    if (ix == 0 || iy == 0 || iz == 0 || ix + 1 == nx || iy + 1 == ny || iz + 1 == nz) {
+    const Pt p = split((uint32_t)o, lo.np[0], lo.np[1]);
+    const size_t ix = p.x, iy = p.y, iz = p.z;
+    if (on_boundary(p, lo)) {
         flo[3 * o] = flo[3 * o + 1] = flo[3 * o + 2] = 0.f;
         return;
     }
@@ -164,9 +181,9 @@ __global__ void k_prolong_add(vg_grid_desc hi, float *__restrict__ uhi, vg_grid_
                               const float *__restrict__ ulo) {
     const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= capacity(hi)) return;
-    const size_t nx = hi.np[0], ny = hi.np[1], nz = hi.np[2];
-    const size_t ix = o % nx, iy = (o / nx) % ny, iz = o / (nx * ny);
-    if (ix == 0 || iy == 0 || iz == 0 || ix + 1 == nx || iy + 1 == ny || iz + 1 == nz) return;
+    const Pt p = split((uint32_t)o, hi.np[0], hi.np[1]);
+    if (on_boundary(p, hi)) return;
+    const size_t ix = p.x, iy = p.y, iz = p.z;
     const size_t lx = lo.np[0], lxy = (size_t)lo.np[0] * lo.np[1];
     const size_t x0 = ix / 2, y0 = iy / 2, z0 = iz / 2;
     const size_t x1 = x0 + (ix & 1), y1 = y0 + (iy & 1), z1 = z0 + (iz & 1);
@@ -186,33 +203,38 @@ __global__ void k_prolong_add(vg_grid_desc hi, float *__restrict__ uhi, vg_grid_
     for (int k = 0; k < 3; ++k) uhi[3 * o + k] += acc[k];
 }
 
-// FP64 coarsest solve: one workgroup, red-black Gauss-Seidel in double.
+// FP64 coarsest solve: one workgroup, red-black Gauss-Seidel in double, on
+// a copy of u in LDS (IN_LDS, grids up to kCoarseLdsPoints) or in `work`.
+constexpr uint32_t kCoarseLdsPoints = 2048;  // 48 KB of doubles
+
+template <bool IN_LDS>
 __global__ __launch_bounds__(256) void k_coarse_f64(vg_grid_desc g, float *__restrict__ u,
                                                     const float *__restrict__ f, int sweeps,
                                                     double *__restrict__ work) {
-    const size_t nx = g.np[0], ny = g.np[1], nz = g.np[2], nxy = nx * ny, n = nxy * nz;
+    __shared__ double lds[IN_LDS ? 3 * kCoarseLdsPoints : 1];
+    double *w = IN_LDS ? lds : work;
+    const uint32_t nx = g.np[0], nxy = nx * g.np[1], n = nxy * g.np[2];
     const double rx = 1.0 / g.cell[0], ry = 1.0 / g.cell[1], rz = 1.0 / g.cell[2];
     const double r2x = rx * rx, r2y = ry * ry, r2z = rz * rz;
     const double inv = 1.0 / (2.0 * (r2x + r2y + r2z));
-    for (size_t i = threadIdx.x; i < 3 * n; i += blockDim.x) work[i] = u[i];
+    for (uint32_t i = threadIdx.x; i < 3 * n; i += blockDim.x) w[i] = u[i];
     __syncthreads();
     for (int it = 0; it < sweeps; ++it) {
-        for (int color = 0; color < 2; ++color) {
-            for (size_t o = threadIdx.x; o < n; o += blockDim.x) {
-                const size_t ix = o % nx, iy = (o / nx) % ny, iz = o / nxy;
-                if (ix == 0 || iy == 0 || iz == 0 || ix + 1 == nx || iy + 1 == ny || iz + 1 == nz) continue;
-                if (((ix + iy + iz) & 1) != (size_t)color) continue;
+        for (uint32_t color = 0; color < 2; ++color) {
+            for (uint32_t o = threadIdx.x; o < n; o += blockDim.x) {
+                const Pt p = split(o, g.np[0], g.np[1]);
+                if (on_boundary(p, g) || ((p.x + p.y + p.z) & 1u) != color) continue;
                 for (int k = 0; k < 3; ++k) {
-                    const double s = (work[3 * (o + 1) + k] + work[3 * (o - 1) + k]) * r2x +
-                                     (work[3 * (o + nx) + k] + work[3 * (o - nx) + k]) * r2y +
-                                     (work[3 * (o + nxy) + k] + work[3 * (o - nxy) + k]) * r2z;
-                    work[3 * o + k] = (s - (double)f[3 * o + k]) * inv;
+                    const double s = (w[3 * (o + 1) + k] + w[3 * (o - 1) + k]) * r2x +
+                                     (w[3 * (o + nx) + k] + w[3 * (o - nx) + k]) * r2y +
+                                     (w[3 * (o + nxy) + k] + w[3 * (o - nxy) + k]) * r2z;
+                    w[3 * o + k] = (s - (double)f[3 * o + k]) * inv;
                 }
             }
             __syncthreads();
         }
     }
-    for (size_t i = threadIdx.x; i < 3 * n; i += blockDim.x) u[i] = (float)work[i];
+    for (uint32_t i = threadIdx.x; i < 3 * n; i += blockDim.x) u[i] = (float)w[i];
 }
 
 // FP64 residual of the compensated solution (hi + lo): writes the FP32
@@ -224,13 +246,12 @@ __global__ __launch_bounds__(256) void k_residual_pair(vg_grid_desc g, const flo
                                                        double r2y, double r2z,
                                                        double *__restrict__ part) {
     __shared__ double sn[256], sd[256];
-    const size_t nx = g.np[0], ny = g.np[1], nz = g.np[2], nxy = nx * ny;
+    const size_t nx = g.np[0], nxy = nx * g.np[1];
     const size_t total = capacity(g);
     double num = 0.0, den = 0.0;
     for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < total;
          o += (size_t)gridDim.x * blockDim.x) {
-        const size_t ix = o % nx, iy = (o / nx) % ny, iz = o / nxy;
-        if (ix == 0 || iy == 0 || iz == 0 || ix + 1 == nx || iy + 1 == ny || iz + 1 == nz) {
+        if (on_boundary(split((uint32_t)o, g.np[0], g.np[1]), g)) {
             r32[3 * o] = r32[3 * o + 1] = r32[3 * o + 2] = 0.f;
             continue;
         }
@@ -265,9 +286,10 @@ __global__ __launch_bounds__(256) void k_residual_pair(vg_grid_desc g, const flo
     }
 }
 
-// (hi, lo) += e with an error-free TwoSum per component.
+// (hi, lo) += e with an error-free TwoSum per component; e is cleared (it
+// is the next cycle's zero initial correction).
 __global__ void k_twosum_add(float *__restrict__ hi, float *__restrict__ lo,
-                             const float *__restrict__ e, size_t n) {
+                             float *__restrict__ e, size_t n) {
 #pragma clang fp contract(off)
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -277,9 +299,12 @@ __global__ void k_twosum_add(float *__restrict__ hi, float *__restrict__ lo,
     const float err = (a - (s - bb)) + (b - bb);
     hi[i] = s;
     lo[i] = err;
+    e[i] = 0.f;
 }
 
-constexpr uint32_t kNormBlocks = 512;
+// Workgroups of the FP64 residual: enough to keep every CU's loads in
+// flight (512 left it latency-bound at 110 us on a 129x129x257 grid).
+constexpr uint32_t kNormBlocks = 2048;
 
 vg_status fp64_residual(vg_multigrid *m, const vg_grid_desc &g, const float *hi, const float *lo,
                         const float *f, float *r32, double *rel, hipStream_t s) {
@@ -299,9 +324,46 @@ vg_status fp64_residual(vg_multigrid *m, const vg_grid_desc &g, const float *hi,
     return VG_OK;
 }
 
-vg_status smooth(const vg_grid_desc &g, float *u, const float *f, int sweeps, hipStream_t s) {
+// Workgroups of the fused sweep for tile depth zc.
+size_t sweep_blocks(const vg_grid_desc &g, int zc) {
+    return (size_t)ceil_div(g.np[0] - 2, rbgs::kSX) * ceil_div(g.np[1] - 2, rbgs::kSY) *
+           ceil_div(g.np[2] - 2, zc);
+}
+
+// Tile depth of the fused sweep: 16 planes, halved while fewer than 512
+// workgroups would result (measured on MI355X: 129x129x257 sweeps in 58 us
+// at 16 planes and 512 workgroups, 70 us at 8 planes; 65x65x129 in 14 us at
+// 2 planes).  0 = one launch per colour is faster (grids too small to fill
+// the chip with tiles: 33x33x65 sweeps in 6.8 us that way, 12 us fused).
+int sweep_depth(const vg_grid_desc &g) {
+    if (g.np[0] < 33 || sweep_blocks(g, 2) < 256) return 0;
+    int zc = 16;
+    while (zc > 2 && sweep_blocks(g, zc) < 512) zc /= 2;
+    return zc;
+}
+
+// `sweeps` red-black sweeps of u.  Even counts on grids large enough run the
+// fused out-of-place sweep, ping-ponging through tmp (whose boundary must
+// equal u's); otherwise one launch per colour.  Both give the same values
+// bit for bit.
+vg_status smooth(const vg_grid_desc &g, float *u, float *tmp, const float *f, int sweeps,
+                 hipStream_t s) {
     if (g.np[0] < 3 || g.np[1] < 3 || g.np[2] < 3) return VG_OK;
     const Lap c = lap_coef(g);
+    const int zc = sweep_depth(g);
+    if (tmp && sweeps % 2 == 0 && zc > 0) {
+        rbgs::Tile t{(int)g.np[0], (int)g.np[1], (int)g.np[2], 0, 0, 3 * (size_t)g.np[0],
+                     3 * (size_t)g.np[0] * g.np[1], c.r2x, c.r2y, c.r2z, c.inv};
+        const dim3 grid(ceil_div(g.np[0] - 2, rbgs::kSX), ceil_div(g.np[1] - 2, rbgs::kSY),
+                        ceil_div(g.np[2] - 2, zc));
+        for (int it = 0; it < sweeps; it += 2) {
+            VG_TRY(launch("vg_mg_rbgs_sweep", rbgs::k_rbgs_sweep, grid, dim3(rbgs::kThreads), 0, s,
+                          t, (const float *)u, tmp, f, zc));
+            VG_TRY(launch("vg_mg_rbgs_sweep", rbgs::k_rbgs_sweep, grid, dim3(rbgs::kThreads), 0, s,
+                          t, (const float *)tmp, u, f, zc));
+        }
+        return VG_OK;
+    }
     const dim3 grid(ceil_div((g.np[0] - 1) / 2 + 1, 64), g.np[1] - 2, g.np[2] - 2);
     for (int it = 0; it < sweeps; ++it) {
         VG_TRY(launch("vg_mg_rbgs", k_rbgs, grid, dim3(64), 0, s, g, u, f, 0, c));
@@ -315,8 +377,9 @@ vg_status vcycle(vg_multigrid *m, uint32_t l, hipStream_t s) {
     const vg_grid_desc &g = m->vg[l];
     const size_t cap = capacity(g);
     if (l + 1 == m->vdepth) {
+        const bool in_lds = cap <= kCoarseLdsPoints;
         double *work = nullptr;
-        VG_TRY(salloc((void **)&work, 3 * cap * sizeof(double), s));
+        if (!in_lds) VG_TRY(salloc((void **)&work, 3 * cap * sizeof(double), s));
         uint32_t dmax = g.np[0];
         if (g.np[1] > dmax) dmax = g.np[1];
         if (g.np[2] > dmax) dmax = g.np[2];
@@ -326,13 +389,16 @@ vg_status vcycle(vg_multigrid *m, uint32_t l, hipStream_t s) {
         // (dmax <= 33): more than one V-cycle needs
         int sweeps = (int)(dmax * dmax + 16);
         if (sweeps > 20000) sweeps = 20000;
-        vg_status st = launch("vg_mg_coarse_f64", k_coarse_f64, dim3(1), dim3(256), 0, s, g, m->u[l],
-                              (const float *)m->f[l], sweeps, work);
-        sfree(work, s);
+        vg_status st =
+            in_lds ? launch("vg_mg_coarse_f64", k_coarse_f64<true>, dim3(1), dim3(256), 0, s, g,
+                            m->u[l], (const float *)m->f[l], sweeps, work)
+                   : launch("vg_mg_coarse_f64", k_coarse_f64<false>, dim3(1), dim3(256), 0, s, g,
+                            m->u[l], (const float *)m->f[l], sweeps, work);
+        if (work) sfree(work, s);
         return st;
     }
     const int nu1 = 2, nu2 = 2;
-    VG_TRY(smooth(g, m->u[l], m->f[l], nu1, s));
+    VG_TRY(smooth(g, m->u[l], m->r[l], m->f[l], nu1, s));
     VG_TRY(launch("vg_mg_residual", k_residual32, dim3(ceil_div(cap, 256)), dim3(256), 0, s, g,
                   (const float *)m->u[l], (const float *)m->f[l], m->r[l], lap_coef(g)));
     const vg_grid_desc &gc = m->vg[l + 1];
@@ -342,7 +408,7 @@ vg_status vcycle(vg_multigrid *m, uint32_t l, hipStream_t s) {
     VG_TRY(vcycle(m, l + 1, s));
     VG_TRY(launch("vg_mg_prolong_add", k_prolong_add, dim3(ceil_div(cap, 256)), dim3(256), 0, s, g,
                   m->u[l], gc, (const float *)m->u[l + 1]));
-    return smooth(g, m->u[l], m->f[l], nu2, s);
+    return smooth(g, m->u[l], m->r[l], m->f[l], nu2, s);
 }
 
 }  // namespace
@@ -424,6 +490,7 @@ VG_API vg_status vg_mg_vcycle_solve(vg_multigrid *m, const vg_grid_desc *g, floa
     VG_REQUIRE(m && g && a0 && negw0, "null argument");
     VG_REQUIRE(g->np[0] >= 3 && g->np[1] >= 3 && g->np[2] >= 3, "grid has no interior");
     VG_REQUIRE(g->cell[0] > 0.f && g->cell[1] > 0.f && g->cell[2] > 0.f, "degenerate spacing");
+    VG_REQUIRE(capacity(*g) < (size_t(1) << 32), "grid has 2^32 points or more");
     hipStream_t s = (hipStream_t)stream;
     // Hierarchy: halve every axis while all have >= 4 even cells.
     m->vg[0] = *g;
@@ -453,6 +520,12 @@ VG_API vg_status vg_mg_vcycle_solve(vg_multigrid *m, const vg_grid_desc *g, floa
         VG_TRY(grow(&m->r[l], &e, c));
         m->vcap[l] = a < b ? (a < e ? a : e) : (b < e ? b : e);
     }
+    // r[l] is the smoother's ping-pong buffer: its boundary must be zero, as
+    // u[l]'s is (the residual kernel keeps it so once set)
+    for (uint32_t l = 0; l + 1 < d; ++l) {
+        const size_t c = 3 * capacity(m->vg[l]);
+        VG_TRY(launch("vg_mg_zero", k_zero_all, dim3(ceil_div(c, 256)), dim3(256), 0, s, m->r[l], c));
+    }
     VG_TRY(grow(&m->lo, &m->lo_cap, 3 * cap0));
     if (!m->part) VG_HIP(hipMalloc((void **)&m->part, 2 * kNormBlocks * sizeof(double)));
     VG_TRY(launch("vg_mg_zero", k_zero_all, dim3(ceil_div(3 * cap0, 256)), dim3(256), 0, s, m->lo,
@@ -461,12 +534,13 @@ VG_API vg_status vg_mg_vcycle_solve(vg_multigrid *m, const vg_grid_desc *g, floa
     uint32_t it = 0;
     // f[0] holds the FP32 residual r = -w - L(hi + lo); u[0] the correction.
     VG_TRY(fp64_residual(m, *g, a0, m->lo, negw0, m->f[0], &rel, s));
+    VG_TRY(launch("vg_mg_zero", k_zero_all, dim3(ceil_div(3 * cap0, 256)), dim3(256), 0, s, m->u[0],
+                  3 * cap0));
     while (rel > tol && it < max_cycles) {
-        VG_TRY(launch("vg_mg_zero", k_zero_all, dim3(ceil_div(3 * cap0, 256)), dim3(256), 0, s,
-                      m->u[0], 3 * cap0));
+        // u[0] is zero here: cleared above, then by each two-sum
         VG_TRY(vcycle(m, 0, s));
         VG_TRY(launch("vg_mg_twosum", k_twosum_add, dim3(ceil_div(3 * cap0, 256)), dim3(256), 0, s,
-                      a0, m->lo, (const float *)m->u[0], 3 * cap0));
+                      a0, m->lo, m->u[0], 3 * cap0));
         ++it;
         VG_TRY(fp64_residual(m, *g, a0, m->lo, negw0, m->f[0], &rel, s));
     }
