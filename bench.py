@@ -693,6 +693,51 @@ def suite(args):
         },
         "vcycle_kernels_ms": {k: {"count": c, "total_ms": m} for k, (c, m) in mgprof.items()},
     })
+    # ---- stretching and tilting: 1M vortons (64 random rings) on their 129^3-class grid
+    pos, om, radius = scenarios.random_rings(1 << 20, seed=5)
+    n = len(pos)
+    dp, dw = vg.DeviceArray.from_numpy(pos), vg.DeviceArray.from_numpy(om)
+    mn, mx = vg.find_bounding_box(dp, n)
+    g = vg.finalize_bounding_box(mn, mx, radius, n)
+    cap = g.capacity
+    rng = np.random.default_rng(32)
+    ijk = np.stack(np.meshgrid(*[np.arange(g.np[a], dtype=np.float32) for a in (2, 1, 0)],
+                               indexing="ij")[::-1], -1).reshape(-1, 3)
+    vel = np.zeros((cap, 3), np.float32)
+    for _ in range(4):  # a smooth synthetic velocity field
+        k = rng.normal(size=3).astype(np.float32) * np.float32(0.2)
+        vel += np.sin(ijk @ k + np.float32(rng.uniform(0, 6)))[:, None] * \
+            rng.normal(size=3).astype(np.float32)
+    dvel = vg.DeviceArray.from_numpy(vel)
+    jac = vg.DeviceArray.zeros((cap, 3, 3))
+    dt = 1.0 / 30.0
+    order = vg.DeviceArray.from_numpy(vg.cell_order(g, pos))  # the influence tree's order
+    ms_fused, _ = _timed(vg, "vg_stretch_tilt_fused",
+                         lambda: vg.stretch_tilt_from_velocity(g, dvel, dp, dw, n, dt), 20)
+    ms_fused_o, _ = _timed(vg, "vg_stretch_tilt_fused",
+                           lambda: vg.stretch_tilt_from_velocity(g, dvel, dp, dw, n, dt,
+                                                                 order=order), 20)
+    ms_jac, _ = _timed(vg, "vg_jacobian", lambda: vg.jacobian(g, dvel, jac), 20)
+    ms_interp, _ = _timed(vg, "vg_stretch_tilt", lambda: vg.stretch_tilt(g, jac, dp, dw, n, dt), 20)
+    ms_interp_o, _ = _timed(vg, "vg_stretch_tilt",
+                            lambda: vg.stretch_tilt(g, jac, dp, dw, n, dt, order=order), 20)
+    recs.append({
+        "workload": "stretch/tilt: 1M vortons (64 random rings), Jacobian of the velocity grid "
+                    "interpolated at each vorton, Omega += 0.5 (Omega J) dt",
+        "grid": list(g.shape), "vortons": n,
+        "fused_ms": ms_fused,
+        "fused_GBs": gbs(36.0 * n + 12.0 * cap, ms_fused),
+        "fused_cell_order_ms": ms_fused_o,
+        "fused_cell_order_GBs": gbs(40.0 * n + 12.0 * cap, ms_fused_o),
+        "two_step_ms": ms_jac + ms_interp,
+        "jacobian_ms": ms_jac, "jacobian_GBs": gbs(48.0 * cap, ms_jac),
+        "interp_ms": ms_interp, "interp_GBs": gbs(36.0 * n + 36.0 * cap, ms_interp),
+        "interp_cell_order_ms": ms_interp_o,
+        "bytes_note": "algorithmic: per vorton pos + Omega read, Omega written (36 B, +4 B of "
+                      "order index); each grid read once (12 B/point velocity, 36 B/point "
+                      "Jacobian)",
+    })
+
     for r in recs:
         print(json.dumps(r), flush=True)
 

@@ -286,6 +286,37 @@ vg_status vg_advect_tracers(const vg_grid_desc *g, const float *field, float *po
                             float *vel, size_t stride, size_t n, float dt,
                             float gain, vg_stream s);
 
+/* ------------------------------------------------------------ stretching and tilting */
+/* ComputeJacobian (uniformGridMath.cpp:304-451): jac[9 o + 3 a + b] =
+ * d v_b / d a (the Mat33 members x, y, z hold d/dx, d/dy, d/dz): central
+ * 1/(2h) inside, one-sided 1/h on the faces, 1/h_z = 0 when
+ * h_z <= FLT_EPSILON.  jac holds 9 * capacity floats. */
+vg_status vg_jacobian(const vg_grid_desc *g, const float *vel, float *jac,
+                      vg_stream s);
+/* The per-vorton loop of VortonSim::StretchAndTiltVortons
+ * (vortonSim.cpp:2796-2812): J = trilinear interpolation of the Jacobian
+ * grid at the position (UniformGrid<Mat33>::Interpolate,
+ * uniformGrid.h:1117-1148), s = Omega * J (transpose formulation,
+ * mat33.h:131-145: s_a = J.a . Omega), Omega += 0.5 s dt.  A no-op when any
+ * grid extent is zero (2-D domain, :2754-2759).  Positions and angular
+ * velocities are strided separately (3 for packed arrays, 18 for 72-byte
+ * Particle records).  order (device, may be NULL) is a permutation of
+ * 0..n-1 giving the order in which vortons are processed: it changes no
+ * value, only cache behaviour (the influence tree's cell order makes the
+ * grid gathers of neighbouring threads hit the same cache lines). */
+vg_status vg_stretch_tilt(const vg_grid_desc *g, const float *jac,
+                          const float *pos, size_t pos_stride, float *angvel,
+                          size_t ang_stride, size_t n, float dt,
+                          const uint32_t *order, vg_stream s);
+/* The same update from the velocity grid: the Jacobians of the 8 corners of
+ * each vorton's cell are formed on the fly, so no Jacobian grid is written.
+ * Equal to vg_jacobian + vg_stretch_tilt bit for bit. */
+vg_status vg_stretch_tilt_from_velocity(const vg_grid_desc *g, const float *vel,
+                                        const float *pos, size_t pos_stride,
+                                        float *angvel, size_t ang_stride,
+                                        size_t n, float dt,
+                                        const uint32_t *order, vg_stream s);
+
 /* ------------------------------------------------------------ learned far-field correction */
 /* No reference counterpart: the reference applies every cluster as a
  * monopole supervorton (vorton.h:161-179 on the clusters of
@@ -454,6 +485,14 @@ vg_status vg_sim_update(vg_sim *sim, float dt, uint32_t frame, float *vel_out,
  * the next update. */
 vg_status vg_sim_velocity_grid(const vg_sim *sim, vg_grid_desc *g,
                                const float **field);
+/* VortonSim::StretchAndTiltVortons (vortonSim.cpp:2748-2820) on the last
+ * update's velocity grid and the vortons' positions: angvel (strided by
+ * ang_stride, normally the vortons' own angular velocities) is updated in
+ * place, processed in the cell order of the update's influence tree (the
+ * vorton count must not have changed since).  The reference runs it inside
+ * Update for the vortex particle method (:4208-4215). */
+vg_status vg_sim_stretch_tilt(vg_sim *sim, float dt, float *angvel,
+                              size_t ang_stride);
 /* GetVectorPotentialMultiGrid()[0] / GetNegativeVorticityMultiGrid()[0]. */
 vg_status vg_sim_vector_potential(const vg_sim *sim, vg_grid_desc *g,
                                   const float **field);

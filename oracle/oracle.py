@@ -152,6 +152,9 @@ def lib():
         L.vgo_solve_vector_poisson.argtypes = [C.POINTER(Grid), f32p, f32p, C.c_size_t]
         L.vgo_mg_cascade.argtypes = [C.POINTER(Grid), f32p, f32p, C.c_size_t]
         L.vgo_curl.argtypes = [C.POINTER(Grid), f32p, f32p]
+        L.vgo_jacobian.argtypes = [C.POINTER(Grid), f32p, f32p]
+        L.vgo_stretch_tilt.argtypes = [C.POINTER(Grid), f32p, f32p, C.c_size_t, f32p, C.c_size_t,
+                                       C.c_size_t, C.c_float]
         L.vgo_residual_rel.restype = C.c_double
         L.vgo_residual_rel.argtypes = [C.POINTER(Grid), f32p, f32p]
         L.vgo_assign_velocity_from_field.argtypes = [C.POINTER(Grid), f32p, f32p, f32p,
@@ -464,6 +467,22 @@ def curl(g: Grid, a):
     out = np.zeros(g.capacity * 3, np.float32)
     lib().vgo_curl(C.byref(g), _f32(a).reshape(-1), out)
     return out.reshape(-1, 3)
+
+
+def jacobian(g: Grid, v):
+    """ComputeJacobian: (capacity, 3, 3) with [o, a, b] = d v_b / d a."""
+    out = np.zeros(g.capacity * 9, np.float32)
+    lib().vgo_jacobian(C.byref(g), _f32(v).reshape(-1), out)
+    return out.reshape(-1, 3, 3)
+
+
+def stretch_tilt(g: Grid, jac, pos, angvel, dt):
+    """StretchAndTiltVortons on packed (n, 3) arrays; returns the new angvel."""
+    p = _f32(pos).reshape(-1)
+    w = _f32(angvel).reshape(-1).copy()
+    n = p.size // 3
+    lib().vgo_stretch_tilt(C.byref(g), _f32(jac).reshape(-1), p, 3, w, 3, n, dt)
+    return w.reshape(-1, 3)
 
 
 def residual_rel(g: Grid, u, f) -> float:

@@ -1215,6 +1215,75 @@ void vgo_curl(const vgo_grid *g, const float *v, float *curl) {
             }
 }
 
+/* uniformGridMath.cpp:304-451.  The reference computes the interior with
+ * central differences, then overwrites the six faces (in the order -x, -y,
+ * -z, +x, +y, +z) with the per-point rule: one-sided 1/h at an index 0 or
+ * n-1 along that axis, central 1/(2h) otherwise.  Every face pass applies the
+ * same per-point rule, so the result is that rule at every point. */
+void vgo_jacobian(const vgo_grid *g, const float *v, float *jac) {
+    const float rx = 1.0f / g->cell[0], ry = 1.0f / g->cell[1];
+    const float rz = g->cell[2] > FLT_EPSILON ? 1.0f / g->cell[2] : 0.0f;
+    const float r[3] = {rx, ry, rz};
+    const float hr[3] = {0.5f * rx, 0.5f * ry, 0.5f * rz};
+    const size_t n[3] = {g->np[0], g->np[1], g->np[2]};
+    const size_t stride[3] = {1, n[0], n[0] * n[1]};
+    for (size_t iz = 0; iz < n[2]; ++iz)
+        for (size_t iy = 0; iy < n[1]; ++iy)
+            for (size_t ix = 0; ix < n[0]; ++ix) {
+                const size_t idx[3] = {ix, iy, iz};
+                const size_t o = ix + n[0] * iy + n[0] * n[1] * iz;
+                for (int a = 0; a < 3; ++a) {
+                    const size_t s = stride[a];
+                    size_t lo = o - s, hi = o + s;
+                    float scale = hr[a];
+                    if (idx[a] == 0) { lo = o; scale = r[a]; }
+                    else if (idx[a] == n[a] - 1) { hi = o; scale = r[a]; }
+                    for (int b = 0; b < 3; ++b)
+                        jac[9 * o + 3 * a + b] = (v[3 * hi + b] - v[3 * lo + b]) * scale;
+                }
+            }
+}
+
+/* Trilinear interpolation of an ncomp-component grid, the nested-lerp
+ * order of uniformGrid.h:1137-1144 applied to every component. */
+static void interpolate_n(const vgo_grid *g, const float *f, int ncomp, const float p[3],
+                          float *out) {
+    uint32_t idx[3];
+    indices_of_position(g, p, idx);
+    float t[3], o[3];
+    for (int a = 0; a < 3; ++a) {
+        const float cmin = g->min[a] + (float)idx[a] * g->cell[a];
+        t[a] = (p[a] - cmin) * g->cpe[a];
+        o[a] = 1.0f - t[a];
+    }
+    const size_t nx = g->np[0], nxy = (size_t)g->np[0] * g->np[1];
+    const size_t o000 = idx[0] + nx * (idx[1] + (size_t)g->np[1] * idx[2]);
+    const size_t c[8] = {o000, o000 + 1, o000 + nx, o000 + nx + 1,
+                         o000 + nxy, o000 + nxy + 1, o000 + nxy + nx, o000 + nxy + nx + 1};
+    for (int k = 0; k < ncomp; ++k) {
+#define F(i) f[(size_t)ncomp * c[i] + k]
+        out[k] = ((o[0] * F(0) + t[0] * F(1)) * o[1] + (o[0] * F(2) + t[0] * F(3)) * t[1]) * o[2] +
+                 ((o[0] * F(4) + t[0] * F(5)) * o[1] + (o[0] * F(6) + t[0] * F(7)) * t[1]) * t[2];
+#undef F
+    }
+}
+
+/* vortonSim.cpp:2748-2820 (transpose formulation, :2800-2801). */
+void vgo_stretch_tilt(const vgo_grid *g, const float *jac, const float *pos,
+                      size_t pos_stride, float *angvel, size_t ang_stride,
+                      size_t n, float dt) {
+    if (g->extent[0] == 0.0f || g->extent[1] == 0.0f || g->extent[2] == 0.0f) return;
+    for (size_t i = 0; i < n; ++i) {
+        float j[9];
+        interpolate_n(g, jac, 9, pos + i * pos_stride, j);
+        float *w = angvel + i * ang_stride;
+        float st[3];
+        for (int a = 0; a < 3; ++a)  /* Vec3 dot, scalar build: x*x + y*y + z*z */
+            st[a] = j[3 * a] * w[0] + j[3 * a + 1] * w[1] + j[3 * a + 2] * w[2];
+        for (int a = 0; a < 3; ++a) w[a] += st[a] * 0.5f * dt;
+    }
+}
+
 double vgo_residual_rel(const vgo_grid *g, const float *u, const float *f) {
     const double rx = 1.0 / g->cell[0], ry = 1.0 / g->cell[1], rz = 1.0 / g->cell[2];
     const double r2[3] = {rx * rx, ry * ry, rz * rz};

@@ -196,6 +196,16 @@ void vgo_mg_cascade(const vgo_grid *g, float *a0, const float *negw0,
                     size_t steps_per_level);
 /* ComputeJacobian + ComputeCurlFromJacobian (uniformGridMath.cpp:207-451). */
 void vgo_curl(const vgo_grid *g, const float *a, float *curl);
+/* ComputeJacobian (uniformGridMath.cpp:304-451): jac[9 o + 3 a + b] =
+ * d v_b / d a (Mat33 members x, y, z hold d/dx, d/dy, d/dz). */
+void vgo_jacobian(const vgo_grid *g, const float *v, float *jac);
+/* StretchAndTiltVortons (vortonSim.cpp:2748-2820): per vorton, trilinear
+ * Jacobian at the position (UniformGrid<Mat33>::Interpolate,
+ * uniformGrid.h:1117-1148), s = Omega * J (mat33.h:131-145: s_a = J.a . Omega),
+ * Omega += 0.5 s dt.  No-op when any grid extent is zero. */
+void vgo_stretch_tilt(const vgo_grid *g, const float *jac, const float *pos,
+                      size_t pos_stride, float *angvel, size_t ang_stride,
+                      size_t n, float dt);
 /* FP64 residual ||f - L u|| / ||f|| over interior points, 7-point Laplacian. */
 double vgo_residual_rel(const vgo_grid *g, const float *u, const float *f);
 

@@ -30,9 +30,9 @@ def rel_l2(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("technique", [1, 2])
-def test_facade_frame(vg, oracle, tmp_path, technique):
-    print(f"[case] C++ facade frame technique={technique}", flush=True)
+@pytest.mark.parametrize("technique,vpm", [(1, False), (2, False), (1, True)])
+def test_facade_frame(vg, oracle, tmp_path, technique, vpm):
+    print(f"[case] C++ facade frame technique={technique} vpm={vpm}", flush=True)
     if not os.path.exists(EXE):
         from vortegrid_amd import build
 
@@ -46,8 +46,8 @@ def test_facade_frame(vg, oracle, tmp_path, technique):
         np.array([len(pos), len(tr)], np.uint64).tofile(f)
         vrec.tofile(f)
         trec.tofile(f)
-    r = subprocess.run([EXE, str(technique), str(inp), str(outp)], capture_output=True, text=True,
-                       timeout=120)
+    r = subprocess.run([EXE, str(technique), str(inp), str(outp)] + (["vpm"] if vpm else []),
+                       capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
     raw = open(outp, "rb").read()
@@ -75,7 +75,16 @@ def test_facade_frame(vg, oracle, tmp_path, technique):
     wp, wv = oracle.advect(og, want, pos, np.zeros_like(pos), dt, 1.0)
     assert rel_l2(vout[:, 3:6], wv) < 1e-5
     assert rel_l2(vout[:, 0:3], wp) < 1e-6
-    assert np.array_equal(vout[:, 9:12], om)  # vorticity untouched
+    if vpm:  # StretchAndTiltVortons on the frame's own velocity grid, before Evolve
+        fg = oracle.Grid()
+        for f in ("min", "extent", "cell", "cpe", "np"):
+            for a in range(3):
+                getattr(fg, f)[a] = gd[f][a]
+        assert np.array_equal(vout[:, 9:12],
+                              oracle.stretch_tilt(fg, oracle.jacobian(fg, grid), pos, om, dt))
+        assert not np.array_equal(vout[:, 9:12], om)
+    else:  # the default VPM-SPH hybrid leaves vorticity untouched
+        assert np.array_equal(vout[:, 9:12], om)
     tp, tv = oracle.advect(og, want, tr, np.zeros_like(tr), dt, 1.0)
     assert rel_l2(tout[:, 3:6], tv) < 1e-5
     assert rel_l2(tout[:, 0:3], tp) < 1e-6

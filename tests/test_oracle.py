@@ -524,3 +524,100 @@ def test_mac_expansion_small_change(oracle, rings2):
     e_base = np.linalg.norm(base[sel] - ref) / np.linalg.norm(ref)
     e_ex = np.linalg.norm(ex[sel] - ref) / np.linalg.norm(ref)
     assert e_ex < 1.25 * e_base + 1e-4
+
+
+def _dyadic_grid(oracle):
+    """A grid whose point coordinates and spacings are exact binary fractions."""
+    g = oracle.Grid()
+    for a, (n, h) in enumerate([(17, 0.125), (9, 0.25), (33, 0.125)]):
+        g.min[a] = -0.5 * a
+        g.cell[a] = h
+        g.cpe[a] = 1.0 / h
+        g.np[a] = n
+        g.extent[a] = (n - 1) * h
+    return g
+
+
+def test_jacobian_exact_on_linear_fields(oracle):
+    """ComputeJacobian (uniformGridMath.cpp:304-451): v = M x + c has
+    d v_b / d a = M[b, a] everywhere, faces (one-sided) included; with
+    dyadic M, c and coordinates every difference is exact."""
+    g = _dyadic_grid(oracle)
+    rng = np.random.default_rng(21)
+    M = rng.integers(-8, 9, (3, 3)) / 4.0
+    c0 = rng.integers(-8, 9, 3) / 8.0
+    p = oracle_query_points(g).astype(np.float64)
+    v = (p @ M.T + c0).astype(F32)
+    J = oracle.jacobian(g, v)
+    assert np.array_equal(J, np.broadcast_to(M.T.astype(F32), J.shape))
+
+
+def test_jacobian_one_sided_on_faces_central_inside(oracle):
+    """v_b = x_a^2: the central difference is exact inside (2 x_a) and the
+    one-sided face differences give 2 x_a + h at index 0 and 2 x_a - h at
+    n-1, so using 1/(2h) or a central stencil on a face fails this."""
+    g = _dyadic_grid(oracle)
+    p = oracle_query_points(g).astype(np.float64)
+    idx = np.stack(np.meshgrid(np.arange(g.np[2]), np.arange(g.np[1]), np.arange(g.np[0]),
+                               indexing="ij")[::-1], -1).reshape(-1, 3)
+    for a in range(3):
+        v = np.zeros_like(p)
+        v[:, (a + 1) % 3] = p[:, a] ** 2
+        J = oracle.jacobian(g, v.astype(F32)).astype(np.float64)
+        h = float(g.cell[a])
+        want = 2 * p[:, a]
+        want = np.where(idx[:, a] == 0, want + h, want)
+        want = np.where(idx[:, a] == g.np[a] - 1, want - h, want)
+        assert np.array_equal(J[:, a, (a + 1) % 3], want)
+        others = [(aa, bb) for aa in range(3) for bb in range(3) if (aa, bb) != (a, (a + 1) % 3)]
+        for aa, bb in others:
+            assert not np.any(J[:, aa, bb])
+
+
+def test_jacobian_agrees_with_curl(oracle):
+    """The curl kernel's internal Jacobian (ComputeCurlFromJacobian,
+    uniformGridMath.cpp:207-231) and ComputeJacobian are the same stencil."""
+    g = oracle.define_shape(3000, [-1, 0, 0.5], [1, 2, 1.5])
+    rng = np.random.default_rng(22)
+    v = rng.normal(size=(g.capacity, 3)).astype(F32)
+    J = oracle.jacobian(g, v)
+    cu = np.stack([J[:, 1, 2] - J[:, 2, 1], J[:, 2, 0] - J[:, 0, 2], J[:, 0, 1] - J[:, 1, 0]], 1)
+    assert np.array_equal(cu, oracle.curl(g, v))
+
+
+def test_stretch_tilt_closed_form(oracle):
+    """v = M x: J is the constant M^T, so the transpose formulation
+    (vortonSim.cpp:2800-2801, s = Omega * J, mat33.h:141-145) gives
+    Omega' = Omega + 0.5 dt M^T Omega.  Dyadic data keep every operation
+    exact, and a non-symmetric M makes the classical J Omega fail."""
+    g = _dyadic_grid(oracle)
+    rng = np.random.default_rng(23)
+    M = rng.integers(-8, 9, (3, 3)) / 4.0
+    assert not np.array_equal(M, M.T)
+    p = oracle_query_points(g).astype(np.float64)
+    v = (p @ M.T).astype(F32)
+    J = oracle.jacobian(g, v)
+    # positions at quarter-cell offsets: trilinear weights are dyadic
+    n = 300
+    cells = np.stack([rng.integers(0, g.np[a] - 1, n) for a in range(3)], 1)
+    frac = rng.integers(0, 4, (n, 3)) / 4.0
+    pos = (np.float64(g.min) + (cells + frac) * np.float64(g.cell)).astype(F32)
+    om = (rng.integers(-16, 17, (n, 3)) / 16.0).astype(F32)
+    dt = 0.125
+    w = oracle.stretch_tilt(g, J, pos, om, dt)
+    want = om.astype(np.float64) + 0.5 * dt * (om.astype(np.float64) @ M)
+    assert np.array_equal(w.astype(np.float64), want)
+    classical = om.astype(np.float64) + 0.5 * dt * (om.astype(np.float64) @ M.T)
+    assert not np.array_equal(w.astype(np.float64), classical)
+
+
+def test_stretch_tilt_skips_flat_domains(oracle):
+    """StretchAndTiltVortons returns early when any grid extent is zero
+    (vortonSim.cpp:2754-2759)."""
+    g = _dyadic_grid(oracle)
+    g.extent[2] = 0.0
+    rng = np.random.default_rng(24)
+    J = rng.normal(size=(g.capacity, 3, 3)).astype(F32)
+    pos = rng.uniform(0.1, 0.9, (50, 3)).astype(F32)
+    om = rng.normal(size=(50, 3)).astype(F32)
+    assert np.array_equal(oracle.stretch_tilt(g, J, pos, om, 0.1), om)

@@ -177,19 +177,36 @@ __device__ __forceinline__ float3 grid_query_point(const vg_grid_desc &g, uint32
     return p;
 }
 
+// The cell containing p and the trilinear tweens of Interpolate
+// (uniformGrid.h:1117-1135): t = (p - cell min corner) * cellsPerExtent,
+// o = 1 - t.
+struct CellWeights {
+    int ix, iy, iz;
+    float tx, ty, tz, ox, oy, oz;
+};
+__device__ __forceinline__ CellWeights cell_weights(const vg_grid_desc &g, float3 p) {
+#pragma clang fp contract(off)
+    CellWeights w;
+    w.ix = cell_of(p.x, g.min[0], g.cpe[0], (int)g.np[0] - 1);
+    w.iy = cell_of(p.y, g.min[1], g.cpe[1], (int)g.np[1] - 1);
+    w.iz = cell_of(p.z, g.min[2], g.cpe[2], (int)g.np[2] - 1);
+    w.tx = (p.x - (g.min[0] + (float)w.ix * g.cell[0])) * g.cpe[0];
+    w.ty = (p.y - (g.min[1] + (float)w.iy * g.cell[1])) * g.cpe[1];
+    w.tz = (p.z - (g.min[2] + (float)w.iz * g.cell[2])) * g.cpe[2];
+    w.ox = 1.0f - w.tx;
+    w.oy = 1.0f - w.ty;
+    w.oz = 1.0f - w.tz;
+    return w;
+}
+
 // Interpolate (uniformGrid.h:1117-1148, 1225-1258): nested lerp x, y, z.
 __device__ __forceinline__ float3 interpolate(const vg_grid_desc &g, const float *__restrict__ f,
                                               float3 p) {
 #pragma clang fp contract(off)
-    const int ix = cell_of(p.x, g.min[0], g.cpe[0], (int)g.np[0] - 1);
-    const int iy = cell_of(p.y, g.min[1], g.cpe[1], (int)g.np[1] - 1);
-    const int iz = cell_of(p.z, g.min[2], g.cpe[2], (int)g.np[2] - 1);
-    const float tx = (p.x - (g.min[0] + (float)ix * g.cell[0])) * g.cpe[0];
-    const float ty = (p.y - (g.min[1] + (float)iy * g.cell[1])) * g.cpe[1];
-    const float tz = (p.z - (g.min[2] + (float)iz * g.cell[2])) * g.cpe[2];
-    const float ox = 1.0f - tx, oy = 1.0f - ty, oz = 1.0f - tz;
+    const CellWeights w = cell_weights(g, p);
+    const float tx = w.tx, ty = w.ty, tz = w.tz, ox = w.ox, oy = w.oy, oz = w.oz;
     const size_t nx = g.np[0], nxy = (size_t)g.np[0] * g.np[1];
-    const size_t o000 = (size_t)ix + nx * ((size_t)iy + (size_t)g.np[1] * iz);
+    const size_t o000 = (size_t)w.ix + nx * ((size_t)w.iy + (size_t)g.np[1] * w.iz);
     const float *a000 = f + 3 * o000, *a100 = a000 + 3, *a010 = a000 + 3 * nx,
                 *a110 = a010 + 3, *a001 = a000 + 3 * nxy, *a101 = a001 + 3,
                 *a011 = a001 + 3 * nx, *a111 = a011 + 3;

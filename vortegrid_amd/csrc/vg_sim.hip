@@ -12,6 +12,8 @@
 //   AssignVelocityFromField(vortons, grid, 1)    (not for DIRECT, whose
 //                                                 vortons keep the direct
 //                                                 velocity)
+//   StretchAndTiltVortons                        -> vg_sim_stretch_tilt
+//                                                 (called after the update)
 #include <cstring>
 
 #include "vg_internal.h"
@@ -20,6 +22,10 @@ namespace vg {
 vg_status curl(const vg_grid_desc &g, const float *a, float *out, hipStream_t s);
 vg_status residual_norm(const vg_grid_desc &g, const float *u, const float *f, double *rel,
                         hipStream_t s);
+vg_status stretch_tilt_fused(const vg_grid_desc &g, const float *vel, const float *pos,
+                             size_t pos_stride, float *angvel, size_t ang_stride, size_t n,
+                             float dt, const uint32_t *order, hipStream_t s);
+vg_status tree_checked_view(const vg_tree *t, TreeView *v);
 }  // namespace vg
 
 struct vg_sim {
@@ -245,6 +251,18 @@ VG_API vg_status vg_sim_update(vg_sim *sim, float dt, uint32_t frame, float *vel
         VG_TRY(vg_assign_velocity_from_field(&g, sim->vel_grid, sim->v.pos, vel_out, vel_stride,
                                              sim->v.n, 1.0f, s));
     return VG_OK;
+}
+
+VG_API vg_status vg_sim_stretch_tilt(vg_sim *sim, float dt, float *angvel, size_t ang_stride) {
+    VG_REQUIRE(sim && sim->have_vortons, "no vortons set");
+    VG_REQUIRE(angvel && ang_stride >= 3, "bad angular velocity array");
+    if (sim->v.n == 0 || capacity(sim->vel_desc) == 0) return VG_OK;
+    VG_REQUIRE(sim->vel_grid, "no velocity grid: call vg_sim_update first");
+    TreeView tv;
+    VG_TRY(tree_checked_view(sim->tree, &tv));
+    VG_REQUIRE(tv.n == sim->v.n, "vorton count changed since the update");
+    return stretch_tilt_fused(sim->vel_desc, sim->vel_grid, sim->v.pos, sim->v.stride, angvel,
+                              ang_stride, sim->v.n, dt, tv.sorted_idx, sim->stream);
 }
 
 VG_API vg_status vg_sim_velocity_grid(const vg_sim *sim, vg_grid_desc *g, const float **field) {

@@ -171,16 +171,36 @@ class IParticleOperation {
 // vortonSim.h:137-623, velocity-from-vorticity subset.
 class VortonSim {
   public:
+    // vortonSim.h:143-149, 256-264
+    enum FluidSimulationTechniqueE {
+        FLUID_SIM_VORTEX_PARTICLE_METHOD,
+        FLUID_SIM_SMOOTHED_PARTICLE_HYDRODYNAMICS,
+        FLUID_SIM_VPM_SPH_HYBRID,
+        FLUID_SIM_NUM
+    };
+    enum InvestigationTermE {
+        INVESTIGATE_ALL,
+        INVESTIGATE_STRETCHING_TILTING,
+        INVESTIGATE_BAROCLINIC,
+        INVESTIGATE_VISCOUS_DIFFUSION,
+        INVESTIGATE_THERMAL_DIFFUSION,
+        INVESTIGATE_NONE
+    };
+
     explicit VortonSim(float viscosity = 0.0f, float ambientFluidDensity = 1.0f,
                        int technique = VG_TECH_POISSON)
         : viscosity_(viscosity), density_(ambientFluidDensity) {
         vg_sim_config_default(&cfg_);
         cfg_.technique = technique;
     }
-    VortonSim(const VortonSim &that) : cfg_(that.cfg_) {}  // copies drop the vortons (vortonSim.h:286-295)
+    VortonSim(const VortonSim &that)  // copies drop the vortons (vortonSim.h:286-295)
+        : cfg_(that.cfg_), fluidSimTechnique_(that.fluidSimTechnique_),
+          investigationTerm_(that.investigationTerm_) {}
     VortonSim &operator=(const VortonSim &that) {
         if (this != &that) {
             cfg_ = that.cfg_;
+            fluidSimTechnique_ = that.fluidSimTechnique_;
+            investigationTerm_ = that.investigationTerm_;
             mVortons = nullptr;
         }
         return *this;
@@ -188,6 +208,14 @@ class VortonSim {
     ~VortonSim() { vg_sim_destroy(sim_); }
 
     vg_sim_config &Config() { return cfg_; }
+
+    // vortonSim.h:325-326, 422; defaults of vortonSim.cpp:392, 398.  Of the
+    // vorticity terms only stretching and tilting is on this path.
+    void SetFluidSimulationTechnique(FluidSimulationTechniqueE t) { fluidSimTechnique_ = t; }
+    const FluidSimulationTechniqueE &GetFluidSimulationTechnique() const {
+        return fluidSimTechnique_;
+    }
+    InvestigationTermE &GetInvestigationTerm() { return investigationTerm_; }
 
     void SetVortons(VECTOR<Vorton> *vortons) { mVortons = vortons; }
     VECTOR<Vorton> *GetVortons() { return mVortons; }
@@ -226,13 +254,21 @@ class VortonSim {
 
     // vortonSim.cpp:4279-4344 (velocity-from-vorticity part of
     // UpdateVortexParticleMethod, :4117-4256): vorton velocities are written
-    // into the records (DIRECT: direct velocity; otherwise grid velocity).
+    // into the records (DIRECT: direct velocity; otherwise grid velocity);
+    // for the vortex particle method with stretching investigated
+    // (:4208-4215), StretchAndTiltVortons then updates their angular
+    // velocities from the same velocity grid.
     void Update(float timeStep, unsigned uFrame) {
         ensure();
         if (!mVortons || mVortons->empty()) return;
         stage_vortons();
         float *base = static_cast<float *>(dev_.get());
         check(vg_sim_update(sim_, timeStep, uFrame, base + kOffVel, kStride), "Update");
+        if (fluidSimTechnique_ == FLUID_SIM_VORTEX_PARTICLE_METHOD &&
+            (investigationTerm_ == INVESTIGATE_ALL ||
+             investigationTerm_ == INVESTIGATE_STRETCHING_TILTING))
+            check(vg_sim_stretch_tilt(sim_, timeStep, base + kOffAngVel, kStride),
+                  "StretchAndTiltVortons");
         check(vg_memcpy_dtoh(mVortons->data(), dev_.get(), mVortons->size() * sizeof(Vorton)),
               "download vortons");
         vel_stale_ = pot_stale_ = negw_stale_ = true;
@@ -327,6 +363,8 @@ class VortonSim {
     bool vel_stale_ = true, pot_stale_ = true, negw_stale_ = true;
     Vec3 minCorner_, maxCorner_;
     float viscosity_ = 0.0f, density_ = 1.0f;
+    FluidSimulationTechniqueE fluidSimTechnique_ = FLUID_SIM_VPM_SPH_HYBRID;
+    InvestigationTermE investigationTerm_ = INVESTIGATE_ALL;
 };
 
 // pclOpVortonSim.h:46-71 / pclOpVortonSim.cpp:51-71

@@ -40,7 +40,8 @@ EXPORTS = (
     "vg_mlp_gradients vg_mlp_train vg_tree_mlp_samples vg_tree_velocity_grid_mlp vg_sim_set_mlp "
     "vg_batch_config_default vg_batch_create vg_batch_destroy vg_batch_buffers vg_batch_step "
     "vg_batch_capture vg_batch_launch vg_batch_grid vg_batch_mollify vg_batch_set_launch "
-    "vg_batch_get_launch vg_batch_autotune"
+    "vg_batch_get_launch vg_batch_autotune vg_jacobian vg_stretch_tilt "
+    "vg_stretch_tilt_from_velocity vg_sim_stretch_tilt"
 ).split()
 
 MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS, MLP_PARAMS = 16, 64, 3, 9603
@@ -614,6 +615,42 @@ def advect_tracers(g: GridDesc, field, pos, vel, n: int, dt: float, gain=1.0, st
          C.c_size_t(stride), C.c_size_t(n), C.c_float(dt), C.c_float(gain), _s(stream))
 
 
+def _p(x):
+    return x.ptr if hasattr(x, "ptr") else x
+
+
+def jacobian(g: GridDesc, vel, jac, stream=None):
+    """ComputeJacobian: jac (capacity, 3, 3) with [o, a, b] = d v_b / d a."""
+    call("vg_jacobian", C.byref(g), _p(vel), _p(jac), _s(stream))
+
+
+def stretch_tilt(g: GridDesc, jac, pos, angvel, n: int, dt: float, pos_stride=3, ang_stride=3,
+                 order=None, stream=None):
+    """StretchAndTiltVortons from a Jacobian grid (angvel updated in place);
+    order: optional device uint32 permutation giving the processing order."""
+    call("vg_stretch_tilt", C.byref(g), _p(jac), _p(pos), C.c_size_t(pos_stride), _p(angvel),
+         C.c_size_t(ang_stride), C.c_size_t(n), C.c_float(dt),
+         _p(order) if order is not None else None, _s(stream))
+
+
+def stretch_tilt_from_velocity(g: GridDesc, vel, pos, angvel, n: int, dt: float, pos_stride=3,
+                               ang_stride=3, order=None, stream=None):
+    """StretchAndTiltVortons straight from the velocity grid (no Jacobian grid)."""
+    call("vg_stretch_tilt_from_velocity", C.byref(g), _p(vel), _p(pos), C.c_size_t(pos_stride),
+         _p(angvel), C.c_size_t(ang_stride), C.c_size_t(n), C.c_float(dt),
+         _p(order) if order is not None else None, _s(stream))
+
+
+def cell_order(g: GridDesc, pos: np.ndarray) -> np.ndarray:
+    """The influence tree's vorton order (by leaf cell, index order within a
+    cell) computed on the host, as a uint32 permutation for `order`."""
+    idx = [np.clip(np.floor((pos[:, a].astype(np.float32) - np.float32(g.min[a]))
+                            * np.float32(g.cpe[a])), 0, g.np[a] - 2).astype(np.int64)
+           for a in range(3)]
+    key = idx[0] + g.np[0] * (idx[1] + g.np[1] * idx[2])
+    return np.argsort(key, kind="stable").astype(np.uint32)
+
+
 # ---------------------------------------------------------------- simulation
 def sim_config(**kw) -> SimConfig:
     c = SimConfig()
@@ -659,6 +696,10 @@ class Sim:
         p = C.c_void_p()
         call("vg_sim_velocity_grid", self.h, C.byref(g), C.byref(p))
         return g, p
+
+    def stretch_tilt(self, dt, angvel, ang_stride=3):
+        """StretchAndTiltVortons on the last update's velocity grid."""
+        call("vg_sim_stretch_tilt", self.h, C.c_float(dt), _p(angvel), C.c_size_t(ang_stride))
 
     def vector_potential(self):
         g = GridDesc()
