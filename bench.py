@@ -738,6 +738,30 @@ def suite(args):
                       "Jacobian)",
     })
 
+    # ---- PSE viscous diffusion on the same 1M vortons (their tree's partition)
+    t = vg.Tree()
+    v = vg.vortons_view(dp, dw, n, uniform_size=2 * radius)
+    t.build(g, v)
+    pse_labels = ("vg_pse_gather", "vg_pse_phase", "vg_pse_scatter")
+    t.pse_diffuse(dw, 0.01, dt)
+    vg.synchronize()
+    vg.profile_reset()
+    vg.profile_enable(True)
+    reps = 5
+    for _ in range(reps):
+        t.pse_diffuse(dw, 0.01, dt)
+    vg.synchronize()
+    vg.profile_enable(False)
+    prof = vg.profile_dump()
+    pse = {k: prof.get(k, (0, 0.0))[1] / reps for k in pse_labels}
+    recs.append({
+        "workload": "PSE viscous diffusion: 1M vortons (64 random rings), nu = 0.01, dt = 1/30, "
+                    "reference threaded-build exchange order reproduced bit for bit",
+        "grid": list(g.shape), "vortons": n,
+        "ms": sum(pse.values()), "phases_ms": pse["vg_pse_phase"],
+        "gather_scatter_ms": pse["vg_pse_gather"] + pse["vg_pse_scatter"],
+    })
+
     for r in recs:
         print(json.dumps(r), flush=True)
 

@@ -317,6 +317,30 @@ vg_status vg_stretch_tilt_from_velocity(const vg_grid_desc *g, const float *vel,
                                         size_t n, float dt,
                                         const uint32_t *order, vg_stream s);
 
+/* ------------------------------------------------------------ viscous diffusion (PSE) */
+/* PartitionVortons (vortonSim.cpp:3578-3607) / Particles::PartitionParticles
+ * (particle.cpp:398-435): the tree's leaf cell lists are the reference's
+ * ugVortonIndices on the grid template.  Vortons of cell c (indexed by its
+ * minimum-corner point) are vorton_index[cell_start[c] .. cell_start[c] +
+ * cell_count[c]), in increasing index order.  Device pointers owned by the
+ * tree, valid until it is rebuilt; *cells = leaf grid capacity. */
+vg_status vg_tree_partition(const vg_tree *t, const uint32_t **cell_start,
+                            const uint32_t **cell_count,
+                            const uint32_t **vorton_index, size_t *cells);
+/* VortonSim::DiffuseAndDissipateVorticityPSE (vortonSim.cpp:3616-3922) on
+ * the vortons the tree was built from (they must not have moved since):
+ * particle strength exchange over each cell and its 7 forward neighbours,
+ * f = clamp(2 nu dt exp(-d^2/(4 size)^2), +-0.5), then dissipation
+ * Omega -= nu dt Omega, in the order of the reference's threaded build (even
+ * z slices, then odd; the result depends on that order and is reproduced
+ * bit for bit).  angvel (strided) is read and updated in place; birth
+ * (int32, strided, may be NULL) marks dead vortons (birth < 0), which
+ * receive exchanges but neither initiate them nor dissipate.  Requires
+ * nu dt <= 1 (the reference's assertion). */
+vg_status vg_tree_pse_diffuse(const vg_tree *t, float *angvel, size_t ang_stride,
+                              const int32_t *birth, size_t birth_stride,
+                              float viscosity, float dt, vg_stream s);
+
 /* ------------------------------------------------------------ learned far-field correction */
 /* No reference counterpart: the reference applies every cluster as a
  * monopole supervorton (vorton.h:161-179 on the clusters of
@@ -493,6 +517,14 @@ vg_status vg_sim_velocity_grid(const vg_sim *sim, vg_grid_desc *g,
  * Update for the vortex particle method (:4208-4215). */
 vg_status vg_sim_stretch_tilt(vg_sim *sim, float dt, float *angvel,
                               size_t ang_stride);
+/* VortonSim::DiffuseAndDissipateVorticityPSE (vortonSim.cpp:3616-3922) on
+ * the update's partition: vg_tree_pse_diffuse on the sim's tree.  Call
+ * after vg_sim_update (and vg_sim_stretch_tilt), before the vortons move;
+ * the reference runs it inside Update when its investigation term is ALL or
+ * VISCOUS_DIFFUSION (:4238-4241). */
+vg_status vg_sim_pse_diffuse(vg_sim *sim, float viscosity, float dt, float *angvel,
+                             size_t ang_stride, const int32_t *birth,
+                             size_t birth_stride);
 /* GetVectorPotentialMultiGrid()[0] / GetNegativeVorticityMultiGrid()[0]. */
 vg_status vg_sim_vector_potential(const vg_sim *sim, vg_grid_desc *g,
                                   const float **field);

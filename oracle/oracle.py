@@ -153,6 +153,10 @@ def lib():
         L.vgo_mg_cascade.argtypes = [C.POINTER(Grid), f32p, f32p, C.c_size_t]
         L.vgo_curl.argtypes = [C.POINTER(Grid), f32p, f32p]
         L.vgo_jacobian.argtypes = [C.POINTER(Grid), f32p, f32p]
+        L.vgo_pse_exp.restype = C.c_float
+        L.vgo_pse_exp.argtypes = [C.c_float]
+        L.vgo_pse_diffuse.argtypes = [C.POINTER(Grid), f32p, f32p, C.c_void_p, C.c_float,
+                                      C.c_void_p, C.c_size_t, C.c_float, C.c_float]
         L.vgo_stretch_tilt.argtypes = [C.POINTER(Grid), f32p, f32p, C.c_size_t, f32p, C.c_size_t,
                                        C.c_size_t, C.c_float]
         L.vgo_residual_rel.restype = C.c_double
@@ -482,6 +486,29 @@ def stretch_tilt(g: Grid, jac, pos, angvel, dt):
     w = _f32(angvel).reshape(-1).copy()
     n = p.size // 3
     lib().vgo_stretch_tilt(C.byref(g), _f32(jac).reshape(-1), p, 3, w, 3, n, dt)
+    return w.reshape(-1, 3)
+
+
+def pse_exp(x: float) -> float:
+    return float(lib().vgo_pse_exp(float(x)))
+
+
+def pse_diffuse(g: Grid, pos, angvel, size, viscosity, dt, alive=None):
+    """DiffuseAndDissipateVorticityPSE (threaded-build order); size: scalar
+    diameter or per-vorton array; alive: optional bool array.  Returns the
+    new angular velocities."""
+    p = _f32(pos).reshape(-1)
+    w = _f32(angvel).reshape(-1).copy()
+    n = p.size // 3
+    if np.ndim(size) == 0:
+        sz_arr, uni = None, float(size)
+    else:
+        sz_arr, uni = _f32(size).reshape(-1), 0.0
+    al = None if alive is None else np.ascontiguousarray(alive, np.uint8)
+    lib().vgo_pse_diffuse(C.byref(g), p, w,
+                          sz_arr.ctypes.data_as(C.c_void_p) if sz_arr is not None else None, uni,
+                          al.ctypes.data_as(C.c_void_p) if al is not None else None, n,
+                          float(viscosity), float(dt))
     return w.reshape(-1, 3)
 
 

@@ -1284,6 +1284,88 @@ void vgo_stretch_tilt(const vgo_grid *g, const float *jac, const float *pos,
     }
 }
 
+/* Cody & Waite (1980) range reduction with the 16-bit split of ln 2. */
+float vgo_pse_exp(float x) {
+    if (x != x) return x;
+    if (x < -87.0f) return 0.0f;
+    const float k = rintf(x * 1.44269504f);
+    const float r = (x - k * 0.693145751953125f) - k * 1.42860677e-06f;
+    const float p =
+        1.0f + r * (1.0f + r * (0.5f + r * (0.166666672f + r * (0.0416666679f +
+        r * (0.00833333377f + r * (0.00138888892f + r * 0.000198412701f))))));
+    const int32_t bits = ((int32_t)k + 127) << 23;
+    float scale;
+    memcpy(&scale, &bits, 4);
+    return p * scale;
+}
+
+/* vortonSim.cpp:3618-3668 ExchangeVorticityOrMergeVortons (merging off). */
+static void pse_exchange(const float *ph, float r2, float *wh, const float *pt, float *wt,
+                         float nu_dt2) {
+    const float sx = ph[0] - pt[0], sy = ph[1] - pt[1], sz = ph[2] - pt[2];
+    const float d2 = sx * sx + sy * sy + sz * sz;
+    const float law = vgo_pse_exp(-d2 / r2);
+    float f = nu_dt2 * law;
+    f = f < -0.5f ? -0.5f : (f > 0.5f ? 0.5f : f);
+    for (int c = 0; c < 3; ++c) {
+        const float e = (wh[c] - wt[c]) * f;
+        wh[c] -= e;
+        wt[c] += e;
+    }
+}
+
+void vgo_pse_diffuse(const vgo_grid *g, const float *pos, float *angvel,
+                     const float *size, float uniform_size,
+                     const unsigned char *alive, size_t n, float viscosity,
+                     float dt) {
+    const size_t nx = g->np[0], ny = g->np[1], nz = g->np[2], nxy = nx * ny;
+    const size_t cap = nxy * nz;
+    if (n == 0 || nx < 2 || ny < 2 || nz < 2) return;
+    /* PartitionParticles: per cell, vorton indices in increasing order */
+    uint32_t *count = (uint32_t *)calloc(cap, sizeof(uint32_t));
+    uint32_t *start = (uint32_t *)calloc(cap + 1, sizeof(uint32_t));
+    uint32_t *list = (uint32_t *)malloc(n * sizeof(uint32_t));
+    size_t *cell = (size_t *)malloc(n * sizeof(size_t));
+    for (size_t v = 0; v < n; ++v) {
+        uint32_t idx[3];
+        indices_of_position(g, pos + 3 * v, idx);
+        cell[v] = idx[0] + nx * (idx[1] + ny * (size_t)idx[2]);
+        ++count[cell[v]];
+    }
+    for (size_t c = 0; c < cap; ++c) start[c + 1] = start[c] + count[c];
+    memset(count, 0, cap * sizeof(uint32_t));
+    for (size_t v = 0; v < n; ++v) list[start[cell[v]] + count[cell[v]]++] = (uint32_t)v;
+    const float nu_dt2 = 2.0f * viscosity * dt;  /* 2.0f * mViscosity * timeStep */
+    const float dissipation = viscosity * dt;
+    const size_t fwd[7] = {1, nx, 1 + nx, nxy, 1 + nxy, nx + nxy, 1 + nx + nxy};
+    for (size_t phase = 0; phase < 2; ++phase)
+        for (size_t z = phase; z + 1 < nz; z += 2)
+            for (size_t y = 0; y + 1 < ny; ++y)
+                for (size_t x = 0; x + 1 < nx; ++x) {
+                    const size_t c = x + nx * y + nxy * z;
+                    for (uint32_t i = start[c]; i < start[c + 1]; ++i) {
+                        const uint32_t h = list[i];
+                        if (alive && !alive[h]) continue;
+                        const float sh = size ? size[h] : uniform_size;
+                        const float range = 4.0f * sh;
+                        const float r2 = range * range;
+                        float *wh = angvel + 3 * (size_t)h;
+                        for (uint32_t j = i + 1; j < start[c + 1]; ++j)
+                            pse_exchange(pos + 3 * (size_t)h, r2, wh, pos + 3 * (size_t)list[j],
+                                         angvel + 3 * (size_t)list[j], nu_dt2);
+                        for (int k = 0; k < 7; ++k) {
+                            const size_t cn = c + fwd[k];
+                            for (uint32_t j = start[cn]; j < start[cn + 1]; ++j)
+                                pse_exchange(pos + 3 * (size_t)h, r2, wh,
+                                             pos + 3 * (size_t)list[j],
+                                             angvel + 3 * (size_t)list[j], nu_dt2);
+                        }
+                        for (int a = 0; a < 3; ++a) wh[a] -= wh[a] * dissipation;
+                    }
+                }
+    free(count); free(start); free(list); free(cell);
+}
+
 double vgo_residual_rel(const vgo_grid *g, const float *u, const float *f) {
     const double rx = 1.0 / g->cell[0], ry = 1.0 / g->cell[1], rz = 1.0 / g->cell[2];
     const double r2[3] = {rx * rx, ry * ry, rz * rz};

@@ -206,6 +206,27 @@ void vgo_jacobian(const vgo_grid *g, const float *v, float *jac);
 void vgo_stretch_tilt(const vgo_grid *g, const float *jac, const float *pos,
                       size_t pos_stride, float *angvel, size_t ang_stride,
                       size_t n, float dt);
+/* exp(x) by Cody-Waite reduction x = k ln2 + r (ln2 split into a 16-bit
+ * high part and a low part), the degree-7 Taylor polynomial of exp(r) in
+ * Horner form, and 2^k built from the exponent bits; 0 below -87.  The
+ * PSE kernel's distance law, evaluated identically on the CPU and GPU. */
+float vgo_pse_exp(float x);
+/* VortonSim::DiffuseAndDissipateVorticityPSE (vortonSim.cpp:3616-3922) in
+ * the order of the reference's threaded build (USE_TBB, :3855-3860): all
+ * even z cell slices, then all odd ones; within a slice cells in (y, x)
+ * order, within a cell vortons in index order (the partition of
+ * PartitionVortons, :3578-3607, particle.cpp:398-435).  Each live vorton h
+ * exchanges with the later vortons of its cell, then with those of the 7
+ * forward cells (+x, +y, +x+y, +z, +x+z, +y+z, +x+y+z, :3711-3728):
+ * f = clamp(2 nu dt exp(-d^2 / (4 size_h)^2), +-0.5), e = f (Omega_h -
+ * Omega_t), Omega_h -= e, Omega_t += e; then Omega_h -= nu dt Omega_h.
+ * pos, angvel: packed n x 3 (angvel in/out); size: n diameters or NULL
+ * for uniform_size; alive: n flags or NULL (all alive).  Merging is off in
+ * the reference build (ENABLE_MERGING_VORTONS undefined). */
+void vgo_pse_diffuse(const vgo_grid *g, const float *pos, float *angvel,
+                     const float *size, float uniform_size,
+                     const unsigned char *alive, size_t n, float viscosity,
+                     float dt);
 /* FP64 residual ||f - L u|| / ||f|| over interior points, 7-point Laplacian. */
 double vgo_residual_rel(const vgo_grid *g, const float *u, const float *f);
 

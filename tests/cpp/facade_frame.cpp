@@ -3,9 +3,10 @@
 //   tracers: FindBoundingBox
 //   vortons: PclOpVortonSim (bounding box includes the tracer box) -> Evolve
 //   tracers: AssignVelocityFromField(grid) -> Evolve
-// usage: facade_frame <technique> <in.bin> <out.bin> [vpm]
+// usage: facade_frame <technique> <in.bin> <out.bin> [vpm [viscosity]]
 //   vpm: run the vortex particle method (stretching and tilting included)
 //        instead of the reference's default VPM-SPH hybrid
+//   viscosity: VortonSim viscosity (default 0: no PSE diffusion)
 //   in.bin : u64 nv, u64 nt, nv Particle records, nt Particle records
 //   out.bin: vg_grid_desc, grid (3*cap floats), nv records, nt records
 #include <cstdio>
@@ -18,11 +19,12 @@
 using namespace vgfacade;
 
 int main(int argc, char **argv) {
-    if (argc != 4 && argc != 5) {
-        std::fprintf(stderr, "usage: %s technique in.bin out.bin [vpm]\n", argv[0]);
+    if (argc < 4 || argc > 6) {
+        std::fprintf(stderr, "usage: %s technique in.bin out.bin [vpm [viscosity]]\n", argv[0]);
         return 2;
     }
-    const bool vpm = argc == 5 && std::string(argv[4]) == "vpm";
+    const bool vpm = argc >= 5 && std::string(argv[4]) == "vpm";
+    const float viscosity = argc == 6 ? (float)std::atof(argv[5]) : 0.0f;
     const int technique = std::atoi(argv[1]);
     FILE *f = std::fopen(argv[2], "rb");
     if (!f) return 3;
@@ -37,7 +39,7 @@ int main(int argc, char **argv) {
         PclOpFindBoundingBox tracerBox;
         tracerBox.Operate(tracers, dt, 0);
         PclOpVortonSim sim;
-        sim.mVortonSim = VortonSim(0.0f, 1.0f, technique);
+        sim.mVortonSim = VortonSim(viscosity, 1.0f, technique);
         if (vpm) sim.mVortonSim.SetFluidSimulationTechnique(VortonSim::FLUID_SIM_VORTEX_PARTICLE_METHOD);
         sim.mMinCorner = &tracerBox.mMinCorner;
         sim.mMaxCorner = &tracerBox.mMaxCorner;

@@ -30,9 +30,12 @@ def rel_l2(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("technique,vpm", [(1, False), (2, False), (1, True)])
-def test_facade_frame(vg, oracle, tmp_path, technique, vpm):
-    print(f"[case] C++ facade frame technique={technique} vpm={vpm}", flush=True)
+@pytest.mark.parametrize("technique,vpm,viscosity", [(1, False, 0.0), (2, False, 0.0),
+                                                     (1, True, 0.0), (2, True, 0.05),
+                                                     (1, False, 0.05)])
+def test_facade_frame(vg, oracle, tmp_path, technique, vpm, viscosity):
+    print(f"[case] C++ facade frame technique={technique} vpm={vpm} viscosity={viscosity}",
+          flush=True)
     if not os.path.exists(EXE):
         from vortegrid_amd import build
 
@@ -46,7 +49,8 @@ def test_facade_frame(vg, oracle, tmp_path, technique, vpm):
         np.array([len(pos), len(tr)], np.uint64).tofile(f)
         vrec.tofile(f)
         trec.tofile(f)
-    r = subprocess.run([EXE, str(technique), str(inp), str(outp)] + (["vpm"] if vpm else []),
+    extra = (["vpm" if vpm else "hybrid", str(viscosity)] if (vpm or viscosity) else [])
+    r = subprocess.run([EXE, str(technique), str(inp), str(outp)] + extra,
                        capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
@@ -75,16 +79,20 @@ def test_facade_frame(vg, oracle, tmp_path, technique, vpm):
     wp, wv = oracle.advect(og, want, pos, np.zeros_like(pos), dt, 1.0)
     assert rel_l2(vout[:, 3:6], wv) < 1e-5
     assert rel_l2(vout[:, 0:3], wp) < 1e-6
-    if vpm:  # StretchAndTiltVortons on the frame's own velocity grid, before Evolve
-        fg = oracle.Grid()
-        for f in ("min", "extent", "cell", "cpe", "np"):
-            for a in range(3):
-                getattr(fg, f)[a] = gd[f][a]
-        assert np.array_equal(vout[:, 9:12],
-                              oracle.stretch_tilt(fg, oracle.jacobian(fg, grid), pos, om, dt))
-        assert not np.array_equal(vout[:, 9:12], om)
-    else:  # the default VPM-SPH hybrid leaves vorticity untouched
-        assert np.array_equal(vout[:, 9:12], om)
+    # vorticity terms, before Evolve: StretchAndTiltVortons on the frame's own
+    # velocity grid (vortex particle method only), then PSE diffusion on the
+    # frame's partition (any technique, when the viscosity is not zero)
+    fg = oracle.Grid()
+    for f in ("min", "extent", "cell", "cpe", "np"):
+        for a in range(3):
+            getattr(fg, f)[a] = gd[f][a]
+    w = om
+    if vpm:
+        w = oracle.stretch_tilt(fg, oracle.jacobian(fg, grid), pos, w, dt)
+    if viscosity:
+        w = oracle.pse_diffuse(fg, pos, w, 2 * radius, viscosity, dt)
+    assert np.array_equal(vout[:, 9:12], w)
+    assert np.array_equal(vout[:, 9:12], om) == (not vpm and not viscosity)
     tp, tv = oracle.advect(og, want, tr, np.zeros_like(tr), dt, 1.0)
     assert rel_l2(tout[:, 3:6], tv) < 1e-5
     assert rel_l2(tout[:, 0:3], tp) < 1e-6

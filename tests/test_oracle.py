@@ -621,3 +621,127 @@ def test_stretch_tilt_skips_flat_domains(oracle):
     pos = rng.uniform(0.1, 0.9, (50, 3)).astype(F32)
     om = rng.normal(size=(50, 3)).astype(F32)
     assert np.array_equal(oracle.stretch_tilt(g, J, pos, om, 0.1), om)
+
+
+# ------------------------------------------------------------ PSE diffusion
+def test_pse_exp_against_libm(oracle):
+    """The PSE distance law's exp (Cody-Waite + degree-7 Taylor) stays within
+    2 ulp of the C library's expf over its whole argument range [-87, 0]."""
+    import ctypes
+
+    libm = ctypes.CDLL("libm.so.6")
+    libm.expf.restype = ctypes.c_float
+    libm.expf.argtypes = [ctypes.c_float]
+    xs = np.concatenate([np.linspace(-87.0, 0.0, 20001, dtype=F32),
+                         -np.geomspace(1e-8, 1.0, 2001).astype(F32), [F32(0.0)]])
+    worst = 0
+    for x in xs:
+        a = np.float32(oracle.pse_exp(float(x)))
+        b = np.float32(libm.expf(float(x)))
+        ulp = abs(int(a.view(np.int32)) - int(b.view(np.int32)))
+        worst = max(worst, ulp)
+    assert worst <= 2
+    assert oracle.pse_exp(0.0) == 1.0
+    assert oracle.pse_exp(-100.0) == 0.0
+
+
+def _pse_grid(oracle):
+    g = oracle.Grid()
+    for a in range(3):
+        g.min[a], g.cell[a], g.cpe[a], g.np[a] = 0.0, 0.25, 4.0, 9
+        g.extent[a] = 2.0
+    return g
+
+
+def _cell_center(ix, iy, iz, frac=(0.5, 0.5, 0.5)):
+    return [0.25 * (ix + frac[0]), 0.25 * (iy + frac[1]), 0.25 * (iz + frac[2])]
+
+
+def _law(oracle, p, q, size, nu, dt):
+    d2 = float(np.sum((np.float64(p) - np.float64(q)) ** 2))
+    return min(0.5, 2 * nu * dt * np.exp(-d2 / (4 * size) ** 2))
+
+
+def test_pse_equal_vorticity_only_dissipates(oracle):
+    """Equal Omega within one z slice: every vorton meets only partners not
+    yet dissipated, every exchange is exactly zero, so each ends at
+    Omega - Omega nu dt (vortonSim.cpp:3800)."""
+    g = _pse_grid(oracle)
+    rng = np.random.default_rng(40)
+    pos = rng.uniform(0.01, 1.99, (200, 3)).astype(F32)
+    pos[:, 2] = rng.uniform(0.51, 0.74, 200).astype(F32)   # cell slice z = 2
+    om = np.tile(np.float32([0.3, -0.7, 1.1]), (200, 1))
+    nu, dt = 0.2, 0.1
+    out = oracle.pse_diffuse(g, pos, om, 0.2, nu, dt)
+    f = np.float32(nu) * np.float32(dt)
+    assert np.array_equal(out, om - om * f)
+
+
+def test_pse_neighbour_set_is_the_forward_half(oracle):
+    """Vortons exchange with their own cell and the 7 forward cells only
+    (vortonSim.cpp:3711-3728): a pair across the (-1,+1,0) diagonal or two
+    cells apart never exchanges; a pair across (+1,0,0) exchanges once."""
+    g = _pse_grid(oracle)
+    nu, dt, size = 0.5, 0.2, 0.2
+    diss = np.float32(nu) * np.float32(dt)
+    a, b = np.float32([1, 0, 0]), np.float32([0, 1, 0])
+    for (c1, c2, exchanges) in [((3, 3, 3), (2, 4, 3), False), ((3, 3, 3), (5, 3, 3), False),
+                                ((3, 3, 3), (4, 3, 3), True), ((3, 3, 2), (4, 4, 3), True)]:
+        pos = np.float32([_cell_center(*c1), _cell_center(*c2)])
+        out = oracle.pse_diffuse(g, pos, np.stack([a, b]), size, nu, dt).astype(np.float64)
+        if not exchanges:
+            want = np.stack([a - a * diss, b - b * diss]).astype(np.float64)
+            assert np.array_equal(out, want), (c1, c2)
+        else:
+            # the vorton in the earlier cell (an even or the same slice)
+            # exchanges once, then both dissipate
+            first, second = (0, 1)
+            f = _law(oracle, pos[first], pos[second], size, nu, dt)
+            wa = a - f * (a - b)
+            wb = b + f * (a - b)
+            want = np.stack([wa * (1 - nu * dt), wb * (1 - nu * dt)])
+            assert np.allclose(out, want, rtol=2e-6, atol=1e-7), (c1, c2)
+            once_more = np.stack([wa - f * (wa - wb), wb + f * (wa - wb)]) * (1 - nu * dt)
+            assert not np.allclose(out, once_more, rtol=2e-6, atol=1e-7)
+
+
+def test_pse_even_slices_before_odd(oracle):
+    """The threaded reference runs all even z slices, then all odd ones
+    (vortonSim.cpp:3855-3860).  B in slice 2 is processed (and dissipates)
+    before A in slice 1 exchanges with it; the serial order would exchange
+    first and dissipate after."""
+    g = _pse_grid(oracle)
+    nu, dt, size = 0.5, 0.2, 0.2
+    d = nu * dt
+    A, B = np.float64([1, 0, 0]), np.float64([0, 0, 1])
+    pos = np.float32([_cell_center(3, 3, 1), _cell_center(3, 3, 2)])
+    out = oracle.pse_diffuse(g, pos, np.float32([A, B]), size, nu, dt).astype(np.float64)
+    f = _law(oracle, pos[0], pos[1], size, nu, dt)
+    b1 = B * (1 - d)                      # even phase: B alone, dissipates
+    a2 = A - f * (A - b1)                 # odd phase: A exchanges with B
+    b2 = b1 + f * (A - b1)
+    want = np.stack([a2 * (1 - d), b2])
+    assert np.allclose(out, want, rtol=2e-6, atol=1e-7)
+    serial_b = (B + f * (A - B)) * (1 - d)
+    assert not np.allclose(out[1], serial_b, rtol=2e-6, atol=1e-7)
+
+
+def test_pse_clamp_dead_vortons_and_cell_order(oracle):
+    """The exchange fraction clamps at 0.5 (vortonSim.cpp:3651); a dead
+    vorton initiates nothing and does not dissipate but still receives
+    exchanges; within a cell the lower index goes first."""
+    g = _pse_grid(oracle)
+    nu, dt, size = 5.0, 0.2, 0.2     # 2 nu dt = 2: clamped to 0.5 at short range
+    A, B, C = np.float64([1, 0, 0]), np.float64([0, 1, 0]), np.float64([0, 0, 1])
+    pos = np.float32([_cell_center(2, 2, 2), _cell_center(2, 2, 2, (0.6, 0.5, 0.5)),
+                      _cell_center(6, 6, 6)])
+    alive = np.array([1, 0, 1], bool)
+    out = oracle.pse_diffuse(g, pos, np.float32([A, B, C]), size, 0.5 / (2 * dt) * 2, dt,
+                             alive=alive).astype(np.float64)
+    nu = 0.5 / (2 * dt) * 2
+    d = np.float64(np.float32(nu) * np.float32(dt))
+    f = _law(oracle, pos[0], pos[1], size, nu, dt)
+    assert f == 0.5
+    a1, b1 = A - f * (A - B), B + f * (A - B)     # A (alive, lower index) exchanges with B
+    want = np.stack([a1 * (1 - d), b1, C * (1 - d)])   # dead B neither initiates nor dissipates
+    assert np.allclose(out, want, rtol=2e-6, atol=1e-7)

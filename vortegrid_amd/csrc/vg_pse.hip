@@ -1,0 +1,204 @@
+// vg_pse.hip -- viscous diffusion of vorticity by particle strength exchange
+// (VortonSim::DiffuseAndDissipateVorticityPSE, vortonSim.cpp:3616-3922), on
+// the partition of PartitionVortons (:3578-3607, particle.cpp:398-435),
+// which is the influence tree's leaf cell lists.
+//
+// The reference's result depends on the order of its in-place exchanges.
+// This follows its threaded build (USE_TBB, :3855-3860): every even z slice
+// of cells, then every odd one; inside a slice, cells in (y, x) order,
+// inside a cell vortons in index order.  Each live vorton h exchanges with
+// the later vortons of its cell and then with those of 7 forward cells
+// (+x, +y, +x+y, +z, +x+z, +y+z, +x+y+z; :3711-3728):
+//   f = clamp(2 nu dt exp(-d^2 / (4 size_h)^2), -0.5, 0.5)
+//   e = f (Omega_h - Omega_t),  Omega_h -= e,  Omega_t += e
+// and then dissipates, Omega_h -= nu dt Omega_h.
+//
+// Schedule.  Cell (x, y) of a slice writes the vortons of cells
+// {x, x+1} x {y, y+1} of that slice and the next, so two cells conflict only
+// when |dx| <= 1 and |dy| <= 1.  Stepping T = x + 2y visits conflicting
+// cells in the reference's (y, x) order (T grows by 1 for (+1, 0) and by
+// 1..3 for (-1..+1, +1)) and never puts two of them in one step, so the
+// cells of a step run in parallel and the result equals the serial
+// traversal bit for bit.  Slices of one parity touch disjoint vortons
+// (z, z+1), so each gets a workgroup; a barrier separates steps and a
+// launch separates the phases.  The vorton state lives in the tree's
+// cell-sorted order (positions and sizes from the tree, angular velocities
+// gathered from the caller and scattered back), so a cell's vortons are
+// contiguous.  exp is vg_pse_exp, the same function as the oracle's.
+#include "vg_internal.h"
+
+namespace vg {
+vg_status tree_checked_view(const vg_tree *t, TreeView *v);
+
+namespace {
+
+constexpr int kPseThreads = 256;
+
+// Cody-Waite reduction with the 16-bit split of ln 2, degree-7 Taylor
+// polynomial, 2^k from the exponent bits; 0 below -87.
+__device__ __forceinline__ float pse_exp(float x) {
+#pragma clang fp contract(off)
+    if (x != x) return x;
+    if (x < -87.0f) return 0.0f;
+    const float k = __builtin_rintf(x * 1.44269504f);
+    const float r = (x - k * 0.693145751953125f) - k * 1.42860677e-06f;
+    const float p =
+        1.0f + r * (1.0f + r * (0.5f + r * (0.166666672f + r * (0.0416666679f +
+        r * (0.00833333377f + r * (0.00138888892f + r * 0.000198412701f))))));
+    return p * __int_as_float(((int)k + 127) << 23);
+}
+
+__device__ __forceinline__ void exchange(const float4 ph, float r2, float4 &wh, const float4 pt,
+                                         float4 *__restrict__ wt, float nu_dt2) {
+#pragma clang fp contract(off)
+    const float sx = ph.x - pt.x, sy = ph.y - pt.y, sz = ph.z - pt.z;
+    const float d2 = sx * sx + sy * sy + sz * sz;
+    float f = nu_dt2 * pse_exp(-d2 / r2);
+    f = f < -0.5f ? -0.5f : (f > 0.5f ? 0.5f : f);
+    float4 w = *wt;
+    const float ex = (wh.x - w.x) * f, ey = (wh.y - w.y) * f, ez = (wh.z - w.z) * f;
+    wh.x -= ex;
+    wh.y -= ey;
+    wh.z -= ez;
+    w.x += ex;
+    w.y += ey;
+    w.z += ez;
+    *wt = w;
+}
+
+struct PseArgs {
+    uint32_t nx, ny, nz;
+    const uint32_t *start, *count;  // per leaf point (cell = its min corner)
+    const float4 *p;                // sorted (x, y, z, size)
+    float4 *w;                      // sorted (Omega, alive)
+    float nu_dt2, dissipation;
+};
+
+__device__ void pse_cell(const PseArgs &a, uint32_t x, uint32_t y, uint32_t z) {
+#pragma clang fp contract(off)
+    const uint32_t nx = a.nx, nxy = a.nx * a.ny;
+    const uint32_t c = x + nx * y + nxy * z;
+    const uint32_t fwd[7] = {1, nx, 1 + nx, nxy, 1 + nxy, nx + nxy, 1 + nx + nxy};
+    const uint32_t b = a.start[c], e = b + a.count[c];
+    for (uint32_t i = b; i < e; ++i) {
+        float4 wh = a.w[i];
+        if (wh.w == 0.f) continue;  // dead: neither initiates nor dissipates
+        const float4 ph = a.p[i];
+        const float range = 4.0f * ph.w;
+        const float r2 = range * range;
+        for (uint32_t j = i + 1; j < e; ++j) exchange(ph, r2, wh, a.p[j], a.w + j, a.nu_dt2);
+        for (int k = 0; k < 7; ++k) {
+            const uint32_t cn = c + fwd[k];
+            const uint32_t bn = a.start[cn], en = bn + a.count[cn];
+            for (uint32_t j = bn; j < en; ++j) exchange(ph, r2, wh, a.p[j], a.w + j, a.nu_dt2);
+        }
+        wh.x -= wh.x * a.dissipation;
+        wh.y -= wh.y * a.dissipation;
+        wh.z -= wh.z * a.dissipation;
+        a.w[i] = wh;
+    }
+}
+
+// One workgroup per slice z = phase + 2 blockIdx.x (z < nz - 1).
+__global__ __launch_bounds__(kPseThreads) void k_pse_phase(PseArgs a, uint32_t phase) {
+    const uint32_t z = phase + 2 * blockIdx.x;
+    if (z + 1 >= a.nz) return;  // uniform per workgroup
+    const int cx = (int)a.nx - 1, cy = (int)a.ny - 1;  // cells per axis
+    const int steps = (cx - 1) + 2 * (cy - 1) + 1;
+    for (int t = 0; t < steps; ++t) {
+        // cells with x + 2y = t: y from max(0, ceil((t - cx + 1) / 2)) to min(cy - 1, t / 2)
+        const int ylo = t - (cx - 1) > 0 ? (t - (cx - 1) + 1) / 2 : 0;
+        const int yhi = t / 2 < cy - 1 ? t / 2 : cy - 1;
+        for (int y = ylo + (int)threadIdx.x; y <= yhi; y += kPseThreads)
+            pse_cell(a, (uint32_t)(t - 2 * y), (uint32_t)y, z);
+        __syncthreads();
+    }
+}
+
+__global__ void k_pse_gather(const uint32_t *__restrict__ idx, const float *__restrict__ angvel,
+                             size_t ang_stride, const int32_t *__restrict__ birth,
+                             size_t birth_stride, size_t n, float4 *__restrict__ w) {
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const size_t i = idx[j];
+    const float *o = angvel + i * ang_stride;
+    const bool alive = birth ? birth[i * birth_stride] >= 0 : true;
+    w[j] = make_float4(o[0], o[1], o[2], alive ? 1.f : 0.f);
+}
+
+__global__ void k_pse_scatter(const uint32_t *__restrict__ idx, const float4 *__restrict__ w,
+                              size_t n, float *__restrict__ angvel, size_t ang_stride) {
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    float *o = angvel + (size_t)idx[j] * ang_stride;
+    const float4 v = w[j];
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+}
+
+}  // namespace
+
+vg_status pse_diffuse(const vg_tree *t, float *angvel, size_t ang_stride, const int32_t *birth,
+                      size_t birth_stride, float viscosity, float dt, hipStream_t s) {
+    TreeView tv;
+    VG_TRY(tree_checked_view(t, &tv));
+    const size_t n = tv.n;
+    if (n == 0) return VG_OK;
+    const vg_grid_desc &g = tv.layer[0];
+    VG_REQUIRE(g.np[0] >= 2 && g.np[1] >= 2 && g.np[2] >= 2, "leaf grid is degenerate");
+    VG_REQUIRE(capacity(g) < (size_t(1) << 32), "leaf grid has 2^32 points or more");
+    float4 *w = nullptr;
+    VG_TRY(salloc((void **)&w, n * sizeof(float4), s));
+    PseArgs a;
+    a.nx = g.np[0];
+    a.ny = g.np[1];
+    a.nz = g.np[2];
+    a.start = tv.cell_start;
+    a.count = tv.cell_count;
+    a.p = tv.sorted_p;
+    a.w = w;
+    a.nu_dt2 = 2.0f * viscosity * dt;   // 2.0f * mViscosity * timeStep
+    a.dissipation = viscosity * dt;     // mViscosity * timeStep
+    vg_status st = launch("vg_pse_gather", k_pse_gather, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                          tv.sorted_idx, (const float *)angvel, ang_stride, birth, birth_stride,
+                          n, w);
+    const uint32_t slices = a.nz - 1;  // cell slices 0 .. nz-2
+    for (uint32_t phase = 0; phase < 2 && st == VG_OK; ++phase) {
+        const uint32_t groups = (slices - phase + 1) / 2;
+        if (groups)
+            st = launch("vg_pse_phase", k_pse_phase, dim3(groups), dim3(kPseThreads), 0, s, a,
+                        phase);
+    }
+    if (st == VG_OK)
+        st = launch("vg_pse_scatter", k_pse_scatter, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                    tv.sorted_idx, (const float4 *)w, n, angvel, ang_stride);
+    sfree(w, s);
+    return st;
+}
+
+}  // namespace vg
+
+using namespace vg;
+
+VG_API vg_status vg_tree_partition(const vg_tree *t, const uint32_t **cell_start,
+                                   const uint32_t **cell_count, const uint32_t **vorton_index,
+                                   size_t *cells) {
+    VG_REQUIRE(cell_start && cell_count && vorton_index && cells, "null argument");
+    TreeView tv;
+    VG_TRY(tree_checked_view(t, &tv));
+    *cell_start = tv.cell_start;
+    *cell_count = tv.cell_count;
+    *vorton_index = tv.sorted_idx;
+    *cells = capacity(tv.layer[0]);
+    return VG_OK;
+}
+
+VG_API vg_status vg_tree_pse_diffuse(const vg_tree *t, float *angvel, size_t ang_stride,
+                                     const int32_t *birth, size_t birth_stride, float viscosity,
+                                     float dt, vg_stream s) {
+    VG_REQUIRE(angvel && ang_stride >= 3, "bad angular velocity array");
+    VG_REQUIRE(!birth || birth_stride >= 1, "bad birth stride");
+    VG_REQUIRE(viscosity * dt <= 1.0f, "viscosity * dt exceeds 1 (vorticity would flip)");
+    return pse_diffuse(t, angvel, ang_stride, birth, birth_stride, viscosity, dt, (hipStream_t)s);
+}

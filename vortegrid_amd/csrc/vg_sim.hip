@@ -13,7 +13,8 @@
 //                                                 vortons keep the direct
 //                                                 velocity)
 //   StretchAndTiltVortons                        -> vg_sim_stretch_tilt
-//                                                 (called after the update)
+//   DiffuseAndDissipateVorticityPSE              -> vg_sim_pse_diffuse
+//                                                 (both called after the update)
 #include <cstring>
 
 #include "vg_internal.h"
@@ -26,6 +27,8 @@ vg_status stretch_tilt_fused(const vg_grid_desc &g, const float *vel, const floa
                              size_t pos_stride, float *angvel, size_t ang_stride, size_t n,
                              float dt, const uint32_t *order, hipStream_t s);
 vg_status tree_checked_view(const vg_tree *t, TreeView *v);
+vg_status pse_diffuse(const vg_tree *t, float *angvel, size_t ang_stride, const int32_t *birth,
+                      size_t birth_stride, float viscosity, float dt, hipStream_t s);
 }  // namespace vg
 
 struct vg_sim {
@@ -263,6 +266,20 @@ VG_API vg_status vg_sim_stretch_tilt(vg_sim *sim, float dt, float *angvel, size_
     VG_REQUIRE(tv.n == sim->v.n, "vorton count changed since the update");
     return stretch_tilt_fused(sim->vel_desc, sim->vel_grid, sim->v.pos, sim->v.stride, angvel,
                               ang_stride, sim->v.n, dt, tv.sorted_idx, sim->stream);
+}
+
+VG_API vg_status vg_sim_pse_diffuse(vg_sim *sim, float viscosity, float dt, float *angvel,
+                                    size_t ang_stride, const int32_t *birth,
+                                    size_t birth_stride) {
+    VG_REQUIRE(sim && sim->have_vortons, "no vortons set");
+    VG_REQUIRE(angvel && ang_stride >= 3, "bad angular velocity array");
+    VG_REQUIRE(viscosity * dt <= 1.0f, "viscosity * dt exceeds 1 (vorticity would flip)");
+    if (sim->v.n == 0) return VG_OK;
+    TreeView tv;
+    VG_TRY(tree_checked_view(sim->tree, &tv));
+    VG_REQUIRE(tv.n == sim->v.n, "vorton count changed since the update");
+    return pse_diffuse(sim->tree, angvel, ang_stride, birth, birth_stride, viscosity, dt,
+                       sim->stream);
 }
 
 VG_API vg_status vg_sim_velocity_grid(const vg_sim *sim, vg_grid_desc *g, const float **field) {

@@ -74,6 +74,7 @@ constexpr size_t kStride = sizeof(Particle) / sizeof(float);  // 18 floats
 constexpr size_t kOffVel = offsetof(Particle, mVelocity) / sizeof(float);
 constexpr size_t kOffAngVel = offsetof(Particle, mAngularVelocity) / sizeof(float);
 constexpr size_t kOffSize = offsetof(Particle, mSize) / sizeof(float);
+constexpr size_t kOffBirth = offsetof(Particle, mBirthTime) / sizeof(float);
 
 // ------------------------------------------------------------------ grids
 // UniformGridGeometry (uniformGrid.h:144-860).
@@ -194,11 +195,14 @@ class VortonSim {
         cfg_.technique = technique;
     }
     VortonSim(const VortonSim &that)  // copies drop the vortons (vortonSim.h:286-295)
-        : cfg_(that.cfg_), fluidSimTechnique_(that.fluidSimTechnique_),
+        : cfg_(that.cfg_), viscosity_(that.viscosity_), density_(that.density_),
+          fluidSimTechnique_(that.fluidSimTechnique_),
           investigationTerm_(that.investigationTerm_) {}
     VortonSim &operator=(const VortonSim &that) {
         if (this != &that) {
             cfg_ = that.cfg_;
+            viscosity_ = that.viscosity_;
+            density_ = that.density_;
             fluidSimTechnique_ = that.fluidSimTechnique_;
             investigationTerm_ = that.investigationTerm_;
             mVortons = nullptr;
@@ -210,7 +214,8 @@ class VortonSim {
     vg_sim_config &Config() { return cfg_; }
 
     // vortonSim.h:325-326, 422; defaults of vortonSim.cpp:392, 398.  Of the
-    // vorticity terms only stretching and tilting is on this path.
+    // vorticity terms, stretching/tilting and viscous diffusion are on this
+    // path (baroclinic generation and heat diffusion are not).
     void SetFluidSimulationTechnique(FluidSimulationTechniqueE t) { fluidSimTechnique_ = t; }
     const FluidSimulationTechniqueE &GetFluidSimulationTechnique() const {
         return fluidSimTechnique_;
@@ -257,7 +262,8 @@ class VortonSim {
     // into the records (DIRECT: direct velocity; otherwise grid velocity);
     // for the vortex particle method with stretching investigated
     // (:4208-4215), StretchAndTiltVortons then updates their angular
-    // velocities from the same velocity grid.
+    // velocities from the same velocity grid, and DiffuseAndDissipateVorticityPSE
+    // (:4238-4241) exchanges vorticity between neighbours with the viscosity.
     void Update(float timeStep, unsigned uFrame) {
         ensure();
         if (!mVortons || mVortons->empty()) return;
@@ -269,6 +275,14 @@ class VortonSim {
              investigationTerm_ == INVESTIGATE_STRETCHING_TILTING))
             check(vg_sim_stretch_tilt(sim_, timeStep, base + kOffAngVel, kStride),
                   "StretchAndTiltVortons");
+        // :4238-4241 (every technique); with zero viscosity every exchange
+        // and the dissipation are zero, so the pass is skipped.
+        if (viscosity_ != 0.0f && (investigationTerm_ == INVESTIGATE_ALL ||
+                                   investigationTerm_ == INVESTIGATE_VISCOUS_DIFFUSION))
+            check(vg_sim_pse_diffuse(sim_, viscosity_, timeStep, base + kOffAngVel, kStride,
+                                     reinterpret_cast<const int32_t *>(base + kOffBirth),
+                                     kStride),
+                  "DiffuseAndDissipateVorticityPSE");
         check(vg_memcpy_dtoh(mVortons->data(), dev_.get(), mVortons->size() * sizeof(Vorton)),
               "download vortons");
         vel_stale_ = pot_stale_ = negw_stale_ = true;

@@ -41,7 +41,8 @@ EXPORTS = (
     "vg_batch_config_default vg_batch_create vg_batch_destroy vg_batch_buffers vg_batch_step "
     "vg_batch_capture vg_batch_launch vg_batch_grid vg_batch_mollify vg_batch_set_launch "
     "vg_batch_get_launch vg_batch_autotune vg_jacobian vg_stretch_tilt "
-    "vg_stretch_tilt_from_velocity vg_sim_stretch_tilt"
+    "vg_stretch_tilt_from_velocity vg_sim_stretch_tilt vg_tree_partition vg_tree_pse_diffuse "
+    "vg_sim_pse_diffuse"
 ).split()
 
 MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS, MLP_PARAMS = 16, 64, 3, 9603
@@ -318,6 +319,7 @@ class Tree:
 
     def build(self, g: GridDesc, v: Vortons, stream=None):
         call("vg_tree_build", self.h, C.byref(g), C.byref(v), _s(stream))
+        self._n = int(v.n)
 
     @property
     def depth(self) -> int:
@@ -352,6 +354,26 @@ class Tree:
                           circ=1.0, policy=RSQRT_ACCURATE, expand_from=-1, stream=None):
         call("vg_tree_velocity_grid_mac", self.h, C.byref(g), int(near_k), int(order),
              int(expand_from), C.c_float(spread), C.c_float(circ), policy, out.ptr, _s(stream))
+
+    def partition(self):
+        """PartitionVortons: (cell_start, cell_count, vorton_index) as numpy."""
+        st, ct, ix = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        cells = C.c_size_t()
+        call("vg_tree_partition", self.h, C.byref(st), C.byref(ct), C.byref(ix), C.byref(cells))
+        out = []
+        for ptr, count in ((st, cells.value), (ct, cells.value), (ix, self._n)):
+            a = np.empty(count, np.uint32)
+            if count:
+                call("vg_memcpy_dtoh", a.ctypes.data_as(C.c_void_p), ptr, C.c_size_t(a.nbytes))
+            out.append(a)
+        return tuple(out)
+
+    def pse_diffuse(self, angvel, viscosity, dt, ang_stride=3, birth=None, birth_stride=18,
+                    stream=None):
+        """DiffuseAndDissipateVorticityPSE on the vortons this tree was built from."""
+        call("vg_tree_pse_diffuse", self.h, _p(angvel), C.c_size_t(ang_stride),
+             _p(birth) if birth is not None else None, C.c_size_t(birth_stride),
+             C.c_float(viscosity), C.c_float(dt), _s(stream))
 
     def mac_work(self, g: GridDesc, near_k=2, expand_from=-1, stream=None):
         """(cluster interactions at points, direct pairs, clusters expanded)."""
@@ -700,6 +722,12 @@ class Sim:
     def stretch_tilt(self, dt, angvel, ang_stride=3):
         """StretchAndTiltVortons on the last update's velocity grid."""
         call("vg_sim_stretch_tilt", self.h, C.c_float(dt), _p(angvel), C.c_size_t(ang_stride))
+
+    def pse_diffuse(self, viscosity, dt, angvel, ang_stride=3, birth=None, birth_stride=18):
+        """DiffuseAndDissipateVorticityPSE on the last update's partition."""
+        call("vg_sim_pse_diffuse", self.h, C.c_float(viscosity), C.c_float(dt), _p(angvel),
+             C.c_size_t(ang_stride), _p(birth) if birth is not None else None,
+             C.c_size_t(birth_stride))
 
     def vector_potential(self):
         g = GridDesc()
