@@ -58,6 +58,27 @@ def test_pse_bit_exact(vg, oracle, viscosity, dt):
     assert np.array_equal(dw.numpy(), want)
 
 
+@pytest.mark.parametrize("dense", [120, 900])
+def test_pse_dense_cells(vg, oracle, dense):
+    """Cells holding more than a wave's worth of vortons, and neighbourhoods
+    beyond the register-resident partner list (900 in one cell), next to a
+    sparse ring: the same bits as the oracle."""
+    pos, om, radius = scenarios.vortex_ring(2048, seed=56)
+    rng = np.random.default_rng(56)
+    centre = pos[0]
+    blob = (centre + rng.uniform(-0.01, 0.01, (dense, 3))).astype(F32)
+    pos = np.concatenate([pos, blob]).astype(F32)
+    om = np.concatenate([om, rng.normal(size=(dense, 3)).astype(F32)])
+    case(f"PSE with a dense blob of {dense} vortons")
+    og, g, t, (dp, dw) = build_tree(vg, oracle, pos, om, radius)
+    start, count, index = t.partition()
+    assert count.max() >= dense
+    t.pse_diffuse(dw, 0.5, 0.2)
+    vg.synchronize()
+    want = oracle.pse_diffuse(og, pos, om, 2 * radius, 0.5, 0.2)
+    assert np.array_equal(dw.numpy(), want)
+
+
 def test_pse_records_and_dead_vortons(vg, oracle):
     """72-byte records (angular velocity at float 9, birth time as int32 at
     14); vortons with a negative birth time are dead."""

@@ -32,7 +32,7 @@ vg_status tree_checked_view(const vg_tree *t, TreeView *v);
 
 namespace {
 
-constexpr int kPseThreads = 256;
+constexpr int kPseThreads = 512;
 
 // Cody-Waite reduction with the 16-bit split of ln 2, degree-7 Taylor
 // polynomial, 2^k from the exponent bits; 0 below -87.
@@ -74,7 +74,7 @@ struct PseArgs {
     float nu_dt2, dissipation;
 };
 
-__device__ void pse_cell(const PseArgs &a, uint32_t x, uint32_t y, uint32_t z) {
+__device__ __forceinline__ void pse_cell(const PseArgs &a, uint32_t x, uint32_t y, uint32_t z) {
 #pragma clang fp contract(off)
     const uint32_t nx = a.nx, nxy = a.nx * a.ny;
     const uint32_t c = x + nx * y + nxy * z;
@@ -87,6 +87,7 @@ __device__ void pse_cell(const PseArgs &a, uint32_t x, uint32_t y, uint32_t z) {
         const float range = 4.0f * ph.w;
         const float r2 = range * range;
         for (uint32_t j = i + 1; j < e; ++j) exchange(ph, r2, wh, a.p[j], a.w + j, a.nu_dt2);
+#pragma unroll
         for (int k = 0; k < 7; ++k) {
             const uint32_t cn = c + fwd[k];
             const uint32_t bn = a.start[cn], en = bn + a.count[cn];
@@ -99,18 +100,133 @@ __device__ void pse_cell(const PseArgs &a, uint32_t x, uint32_t y, uint32_t z) {
     }
 }
 
-// One workgroup per slice z = phase + 2 blockIdx.x (z < nz - 1).
+// Wave-wide form of pse_cell: the cell's partner list (its own vortons, then
+// those of the 7 forward cells, in order) is held in registers, partner j in
+// lane j % 64, slot j / 64.  For each live here-vorton the lanes compute the
+// exchange fractions of all its partners at once (positions only), then the
+// wave walks the partners in order: the running Omega_h is wave-uniform and
+// each step reads one partner through readlane and selects its new value
+// into the owning lane, with the same operations, in the same order, as
+// pse_cell.  Cells with more than kSlots * 64 partners fall back to
+// pse_cell on lane 0.
+constexpr int kSlots = 9;
+
+__device__ __forceinline__ float lane_get(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+// The value v (wave-uniform) into lane `lane` of old.
+__device__ __forceinline__ float lane_set(float v, float old, int lane) {
+    return (int)(threadIdx.x & 63) == lane ? v : old;
+}
+
+__device__ __forceinline__ void pse_cell_wave(const PseArgs &a, uint32_t x, uint32_t y, uint32_t z) {
+#pragma clang fp contract(off)
+    const int lane = (int)(threadIdx.x & 63);
+    const uint32_t nx = a.nx, nxy = a.nx * a.ny;
+    const uint32_t c = x + nx * y + nxy * z;
+    const uint32_t offs[8] = {0, 1, nx, 1 + nx, nxy, 1 + nxy, nx + nxy, 1 + nx + nxy};
+    uint32_t base[8], pre[9];
+    pre[0] = 0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        base[r] = a.start[c + offs[r]];
+        pre[r + 1] = pre[r] + a.count[c + offs[r]];
+    }
+    const uint32_t k = pre[1], n = pre[8];
+    if (k == 0) return;
+    if (n > 64u * kSlots) {
+        if (lane == 0) pse_cell(a, x, y, z);
+        return;
+    }
+    float4 P[kSlots], W[kSlots];
+    int G[kSlots];
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+        const uint32_t j = (uint32_t)lane + 64u * s;
+        G[s] = -1;
+        P[s] = W[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (j < n) {
+            uint32_t g = base[0] + j;
+#pragma unroll
+            for (int q = 1; q < 8; ++q) g = j >= pre[q] ? base[q] + (j - pre[q]) : g;
+            G[s] = (int)g;
+            P[s] = a.p[g];
+            W[s] = a.w[g];
+        }
+    }
+    for (uint32_t i = 0; i < k; ++i) {
+        const int si = (int)(i >> 6), li = (int)(i & 63);
+        float4 wh = make_float4(0.f, 0.f, 0.f, 0.f), ph = wh;
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s)
+            if (s == si) {
+                wh = make_float4(lane_get(W[s].x, li), lane_get(W[s].y, li), lane_get(W[s].z, li),
+                                 lane_get(W[s].w, li));
+                ph = make_float4(lane_get(P[s].x, li), lane_get(P[s].y, li), lane_get(P[s].z, li),
+                                 lane_get(P[s].w, li));
+            }
+        if (wh.w == 0.f) continue;  // dead: neither initiates nor dissipates
+        const float range = 4.0f * ph.w;
+        const float r2 = range * range;
+        float f[kSlots];
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+            const float sx = ph.x - P[s].x, sy = ph.y - P[s].y, sz = ph.z - P[s].z;
+            const float d2 = sx * sx + sy * sy + sz * sz;
+            float fs = a.nu_dt2 * pse_exp(-d2 / r2);
+            f[s] = fs < -0.5f ? -0.5f : (fs > 0.5f ? 0.5f : fs);
+        }
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+            const uint32_t j0 = 64u * s;
+            const uint32_t lo = i + 1 > j0 ? i + 1 - j0 : 0;
+            const uint32_t hi = n > j0 ? (n - j0 < 64u ? n - j0 : 64u) : 0;
+            for (uint32_t L = lo; L < hi; ++L) {
+                const int l = (int)L;
+                const float fj = lane_get(f[s], l);
+                const float tx = lane_get(W[s].x, l), ty = lane_get(W[s].y, l),
+                            tz = lane_get(W[s].z, l);
+                const float ex = (wh.x - tx) * fj, ey = (wh.y - ty) * fj, ez = (wh.z - tz) * fj;
+                wh.x -= ex;
+                wh.y -= ey;
+                wh.z -= ez;
+                W[s].x = lane_set(tx + ex, W[s].x, l);
+                W[s].y = lane_set(ty + ey, W[s].y, l);
+                W[s].z = lane_set(tz + ez, W[s].z, l);
+            }
+        }
+        wh.x -= wh.x * a.dissipation;
+        wh.y -= wh.y * a.dissipation;
+        wh.z -= wh.z * a.dissipation;
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s)
+            if (s == si) {
+                W[s].x = lane_set(wh.x, W[s].x, li);
+                W[s].y = lane_set(wh.y, W[s].y, li);
+                W[s].z = lane_set(wh.z, W[s].z, li);
+            }
+    }
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s)
+        if (G[s] >= 0) a.w[G[s]] = W[s];
+}
+
+// One workgroup per slice z = phase + 2 blockIdx.x (z < nz - 1); each wave
+// takes whole cells of the current wavefront step.
+constexpr int kPseWaves = kPseThreads / 64;
+
 __global__ __launch_bounds__(kPseThreads) void k_pse_phase(PseArgs a, uint32_t phase) {
     const uint32_t z = phase + 2 * blockIdx.x;
     if (z + 1 >= a.nz) return;  // uniform per workgroup
+    const int wave = (int)(threadIdx.x >> 6);
     const int cx = (int)a.nx - 1, cy = (int)a.ny - 1;  // cells per axis
     const int steps = (cx - 1) + 2 * (cy - 1) + 1;
     for (int t = 0; t < steps; ++t) {
         // cells with x + 2y = t: y from max(0, ceil((t - cx + 1) / 2)) to min(cy - 1, t / 2)
         const int ylo = t - (cx - 1) > 0 ? (t - (cx - 1) + 1) / 2 : 0;
         const int yhi = t / 2 < cy - 1 ? t / 2 : cy - 1;
-        for (int y = ylo + (int)threadIdx.x; y <= yhi; y += kPseThreads)
-            pse_cell(a, (uint32_t)(t - 2 * y), (uint32_t)y, z);
+        for (int y = ylo + wave; y <= yhi; y += kPseWaves)
+            pse_cell_wave(a, (uint32_t)(t - 2 * y), (uint32_t)y, z);
         __syncthreads();
     }
 }
