@@ -754,12 +754,23 @@ def suite(args):
     vg.profile_enable(False)
     prof = vg.profile_dump()
     pse = {k: prof.get(k, (0, 0.0))[1] / reps for k in pse_labels}
+    vg.profile_reset()
+    vg.profile_enable(True)
+    for _ in range(reps):
+        t.pse_diffuse(dw, 0.01, dt, order=1)
+    vg.synchronize()
+    vg.profile_enable(False)
+    prof = vg.profile_dump()
+    pse_sim = {k: prof.get(k, (0, 0.0))[1] / reps
+               for k in ("vg_pse_gather", "vg_pse_simultaneous", "vg_pse_scatter")}
     recs.append({
         "workload": "PSE viscous diffusion: 1M vortons (64 random rings), nu = 0.01, dt = 1/30, "
                     "reference threaded-build exchange order reproduced bit for bit",
         "grid": list(g.shape), "vortons": n,
         "ms": sum(pse.values()), "phases_ms": pse["vg_pse_phase"],
         "gather_scatter_ms": pse["vg_pse_gather"] + pse["vg_pse_scatter"],
+        "simultaneous_ms": sum(pse_sim.values()),
+        "simultaneous_kernel_ms": pse_sim["vg_pse_simultaneous"],
     })
 
     for r in recs:

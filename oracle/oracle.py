@@ -157,6 +157,7 @@ def lib():
         L.vgo_pse_exp.argtypes = [C.c_float]
         L.vgo_pse_diffuse.argtypes = [C.POINTER(Grid), f32p, f32p, C.c_void_p, C.c_float,
                                       C.c_void_p, C.c_size_t, C.c_float, C.c_float]
+        L.vgo_pse_diffuse_simultaneous.argtypes = L.vgo_pse_diffuse.argtypes
         L.vgo_stretch_tilt.argtypes = [C.POINTER(Grid), f32p, f32p, C.c_size_t, f32p, C.c_size_t,
                                        C.c_size_t, C.c_float]
         L.vgo_residual_rel.restype = C.c_double
@@ -493,8 +494,9 @@ def pse_exp(x: float) -> float:
     return float(lib().vgo_pse_exp(float(x)))
 
 
-def pse_diffuse(g: Grid, pos, angvel, size, viscosity, dt, alive=None):
-    """DiffuseAndDissipateVorticityPSE (threaded-build order); size: scalar
+def pse_diffuse(g: Grid, pos, angvel, size, viscosity, dt, alive=None, simultaneous=False):
+    """DiffuseAndDissipateVorticityPSE (threaded-build order, or all
+    exchanges from the starting values with simultaneous=True); size: scalar
     diameter or per-vorton array; alive: optional bool array.  Returns the
     new angular velocities."""
     p = _f32(pos).reshape(-1)
@@ -505,7 +507,8 @@ def pse_diffuse(g: Grid, pos, angvel, size, viscosity, dt, alive=None):
     else:
         sz_arr, uni = _f32(size).reshape(-1), 0.0
     al = None if alive is None else np.ascontiguousarray(alive, np.uint8)
-    lib().vgo_pse_diffuse(C.byref(g), p, w,
+    fn = lib().vgo_pse_diffuse_simultaneous if simultaneous else lib().vgo_pse_diffuse
+    fn(C.byref(g), p, w,
                           sz_arr.ctypes.data_as(C.c_void_p) if sz_arr is not None else None, uni,
                           al.ctypes.data_as(C.c_void_p) if al is not None else None, n,
                           float(viscosity), float(dt))

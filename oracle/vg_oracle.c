@@ -1366,6 +1366,96 @@ void vgo_pse_diffuse(const vgo_grid *g, const float *pos, float *angvel,
     free(count); free(start); free(list); free(cell);
 }
 
+/* e = f (Omega_h - Omega_t) with f from h's size, as in pse_exchange. */
+static void pse_pair(const float *ph, float r2h, const float *wh, const float *pt,
+                     const float *wt, float nu_dt2, float e[3]) {
+    const float sx = ph[0] - pt[0], sy = ph[1] - pt[1], sz = ph[2] - pt[2];
+    const float d2 = sx * sx + sy * sy + sz * sz;
+    float f = nu_dt2 * vgo_pse_exp(-d2 / r2h);
+    f = f < -0.5f ? -0.5f : (f > 0.5f ? 0.5f : f);
+    for (int c = 0; c < 3; ++c) e[c] = (wh[c] - wt[c]) * f;
+}
+
+void vgo_pse_diffuse_simultaneous(const vgo_grid *g, const float *pos, float *angvel,
+                                  const float *size, float uniform_size,
+                                  const unsigned char *alive, size_t n,
+                                  float viscosity, float dt) {
+    const size_t nx = g->np[0], ny = g->np[1], nz = g->np[2], nxy = nx * ny;
+    const size_t cap = nxy * nz;
+    if (n == 0 || nx < 2 || ny < 2 || nz < 2) return;
+    uint32_t *count = (uint32_t *)calloc(cap, sizeof(uint32_t));
+    uint32_t *start = (uint32_t *)calloc(cap + 1, sizeof(uint32_t));
+    uint32_t *list = (uint32_t *)malloc(n * sizeof(uint32_t));
+    uint32_t *slot = (uint32_t *)malloc(n * sizeof(uint32_t));
+    size_t *cell = (size_t *)malloc(n * sizeof(size_t));
+    uint32_t (*cidx)[3] = malloc(n * sizeof *cidx);
+    float *w0 = (float *)malloc(3 * n * sizeof(float));
+    memcpy(w0, angvel, 3 * n * sizeof(float));
+    for (size_t v = 0; v < n; ++v) {
+        indices_of_position(g, pos + 3 * v, cidx[v]);
+        cell[v] = cidx[v][0] + nx * (cidx[v][1] + ny * (size_t)cidx[v][2]);
+        ++count[cell[v]];
+    }
+    for (size_t c = 0; c < cap; ++c) start[c + 1] = start[c] + count[c];
+    memset(count, 0, cap * sizeof(uint32_t));
+    for (size_t v = 0; v < n; ++v) {
+        slot[v] = start[cell[v]] + count[cell[v]]++;
+        list[slot[v]] = (uint32_t)v;
+    }
+    const float nu_dt2 = 2.0f * viscosity * dt;
+    const float dissipation = viscosity * dt;
+    const int d[7][3] = {{1, 0, 0}, {0, 1, 0}, {1, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+    for (size_t i = 0; i < n; ++i) {
+        const size_t c = cell[i];
+        const float *pi = pos + 3 * i, *wi = w0 + 3 * i;
+        float acc[3] = {0.f, 0.f, 0.f}, e[3];
+        for (int k = 0; k < 7; ++k) {  /* received from backward cells */
+            if ((int)cidx[i][0] < d[k][0] || (int)cidx[i][1] < d[k][1] ||
+                (int)cidx[i][2] < d[k][2])
+                continue;
+            const size_t cb = c - (d[k][0] + nx * d[k][1] + nxy * d[k][2]);
+            for (uint32_t j = start[cb]; j < start[cb + 1]; ++j) {
+                const uint32_t h = list[j];
+                if (alive && !alive[h]) continue;
+                const float sh = 4.0f * (size ? size[h] : uniform_size);
+                pse_pair(pos + 3 * (size_t)h, sh * sh, w0 + 3 * (size_t)h, pi, wi, nu_dt2, e);
+                for (int a = 0; a < 3; ++a) acc[a] += e[a];
+            }
+        }
+        for (uint32_t j = start[c]; j < slot[i]; ++j) {  /* received from earlier in the cell */
+            const uint32_t h = list[j];
+            if (alive && !alive[h]) continue;
+            const float sh = 4.0f * (size ? size[h] : uniform_size);
+            pse_pair(pos + 3 * (size_t)h, sh * sh, w0 + 3 * (size_t)h, pi, wi, nu_dt2, e);
+            for (int a = 0; a < 3; ++a) acc[a] += e[a];
+        }
+        const int live = !alive || alive[i];
+        if (live) {
+            const float si = 4.0f * (size ? size[i] : uniform_size);
+            const float r2 = si * si;
+            for (uint32_t j = slot[i] + 1; j < start[c + 1]; ++j) {
+                const uint32_t t = list[j];
+                pse_pair(pi, r2, wi, pos + 3 * (size_t)t, w0 + 3 * (size_t)t, nu_dt2, e);
+                for (int a = 0; a < 3; ++a) acc[a] -= e[a];
+            }
+            for (int k = 0; k < 7; ++k) {
+                const size_t cf = c + (d[k][0] + nx * d[k][1] + nxy * d[k][2]);
+                for (uint32_t j = start[cf]; j < start[cf + 1]; ++j) {
+                    const uint32_t t = list[j];
+                    pse_pair(pi, r2, wi, pos + 3 * (size_t)t, w0 + 3 * (size_t)t, nu_dt2, e);
+                    for (int a = 0; a < 3; ++a) acc[a] -= e[a];
+                }
+            }
+        }
+        float *wo = angvel + 3 * i;
+        for (int a = 0; a < 3; ++a) {
+            wo[a] = wi[a] + acc[a];
+            if (live) wo[a] -= wo[a] * dissipation;
+        }
+    }
+    free(count); free(start); free(list); free(slot); free(cell); free(cidx); free(w0);
+}
+
 double vgo_residual_rel(const vgo_grid *g, const float *u, const float *f) {
     const double rx = 1.0 / g->cell[0], ry = 1.0 / g->cell[1], rz = 1.0 / g->cell[2];
     const double r2[3] = {rx * rx, ry * ry, rz * rz};

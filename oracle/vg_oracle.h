@@ -227,6 +227,19 @@ void vgo_pse_diffuse(const vgo_grid *g, const float *pos, float *angvel,
                      const float *size, float uniform_size,
                      const unsigned char *alive, size_t n, float viscosity,
                      float dt);
+/* The same pairs and formulas applied simultaneously (no reference
+ * counterpart): every exchange e = f (Omega_h - Omega_t) uses the starting
+ * values, vorton i sums, in this order, the exchanges it receives from live
+ * vortons of the 7 backward cells (offsets -fwd[0..6], each cell in index
+ * order), from live earlier vortons of its own cell, then (if alive) minus
+ * those it makes with the later vortons of its cell and the 7 forward
+ * cells; Omega_i += sum, then Omega_i -= nu dt Omega_i if alive.  Differs
+ * from the in-place order at second order in the exchange fractions a
+ * vorton accumulates over its partners. */
+void vgo_pse_diffuse_simultaneous(const vgo_grid *g, const float *pos, float *angvel,
+                                  const float *size, float uniform_size,
+                                  const unsigned char *alive, size_t n,
+                                  float viscosity, float dt);
 /* FP64 residual ||f - L u|| / ||f|| over interior points, 7-point Laplacian. */
 double vgo_residual_rel(const vgo_grid *g, const float *u, const float *f);
 

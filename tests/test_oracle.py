@@ -745,3 +745,30 @@ def test_pse_clamp_dead_vortons_and_cell_order(oracle):
     a1, b1 = A - f * (A - B), B + f * (A - B)     # A (alive, lower index) exchanges with B
     want = np.stack([a1 * (1 - d), b1, C * (1 - d)])   # dead B neither initiates nor dissipates
     assert np.allclose(out, want, rtol=2e-6, atol=1e-7)
+
+
+def test_pse_simultaneous_closed_forms(oracle):
+    """Simultaneous PSE: every exchange from the starting values.  Equal
+    Omega anywhere is pure dissipation; the cross-slice pair that the
+    threaded order treats in two phases gets both exchange and dissipation
+    from the starting values; with every vorton alive the exchanges cancel in
+    the sum, so sum(Omega' / (1 - nu dt)) = sum(Omega)."""
+    g = _pse_grid(oracle)
+    nu, dt, size = 0.5, 0.2, 0.2
+    d = np.float32(nu) * np.float32(dt)
+    rng = np.random.default_rng(41)
+    pos = rng.uniform(0.01, 1.99, (300, 3)).astype(F32)
+    om = np.tile(np.float32([0.3, -0.7, 1.1]), (300, 1))
+    out = oracle.pse_diffuse(g, pos, om, 0.2, nu, dt, simultaneous=True)
+    assert np.array_equal(out, om - om * d)
+    A, B = np.float64([1, 0, 0]), np.float64([0, 0, 1])
+    p2 = np.float32([_cell_center(3, 3, 1), _cell_center(3, 3, 2)])
+    out2 = oracle.pse_diffuse(g, p2, np.float32([A, B]), size, nu, dt,
+                              simultaneous=True).astype(np.float64)
+    f = _law(oracle, p2[0], p2[1], size, nu, dt)
+    want = np.stack([A - f * (A - B), B + f * (A - B)]) * (1 - nu * dt)
+    assert np.allclose(out2, want, rtol=2e-6, atol=1e-7)
+    om3 = rng.normal(size=(300, 3)).astype(F32)
+    out3 = oracle.pse_diffuse(g, pos, om3, 0.2, 0.05, 0.1, simultaneous=True).astype(np.float64)
+    dd = np.float64(np.float32(0.05) * np.float32(0.1))
+    assert np.allclose((out3 / (1 - dd)).sum(0), om3.astype(np.float64).sum(0), atol=2e-5)

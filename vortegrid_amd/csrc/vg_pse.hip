@@ -231,6 +231,96 @@ __global__ __launch_bounds__(kPseThreads) void k_pse_phase(PseArgs a, uint32_t p
     }
 }
 
+// Simultaneous exchange (no reference counterpart): one thread per vorton
+// in cell-sorted order, every exchange from the starting values w0, summed
+// in the order of vgo_pse_diffuse_simultaneous: received from the live
+// vortons of the 7 backward cells, from the live earlier vortons of the own
+// cell, then (if alive) given to the later vortons of the own cell and the
+// vortons of the 7 forward cells.
+__device__ __forceinline__ void pse_pair(const float4 ph, float r2h, const float4 wh,
+                                         const float4 pt, const float4 wt, float nu_dt2,
+                                         float &ex, float &ey, float &ez) {
+#pragma clang fp contract(off)
+    const float sx = ph.x - pt.x, sy = ph.y - pt.y, sz = ph.z - pt.z;
+    const float d2 = sx * sx + sy * sy + sz * sz;
+    float f = nu_dt2 * pse_exp(-d2 / r2h);
+    f = f < -0.5f ? -0.5f : (f > 0.5f ? 0.5f : f);
+    ex = (wh.x - wt.x) * f;
+    ey = (wh.y - wt.y) * f;
+    ez = (wh.z - wt.z) * f;
+}
+
+__global__ void k_pse_simultaneous(PseArgs a, const vg_grid_desc g, const float4 *__restrict__ w0,
+                                   float4 *__restrict__ w1, uint32_t n) {
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 pi = a.p[i], wi = w0[i];
+    const int ix = cell_of(pi.x, g.min[0], g.cpe[0], (int)g.np[0] - 1);
+    const int iy = cell_of(pi.y, g.min[1], g.cpe[1], (int)g.np[1] - 1);
+    const int iz = cell_of(pi.z, g.min[2], g.cpe[2], (int)g.np[2] - 1);
+    const uint32_t nx = a.nx, nxy = a.nx * a.ny;
+    const uint32_t c = (uint32_t)ix + nx * (uint32_t)iy + nxy * (uint32_t)iz;
+    const int d[7][3] = {{1, 0, 0}, {0, 1, 0}, {1, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+    float ax = 0.f, ay = 0.f, az = 0.f, ex, ey, ez;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        if (ix < d[k][0] || iy < d[k][1] || iz < d[k][2]) continue;
+        const uint32_t cb = c - ((uint32_t)d[k][0] + nx * d[k][1] + nxy * d[k][2]);
+        const uint32_t b = a.start[cb], e = b + a.count[cb];
+        for (uint32_t j = b; j < e; ++j) {
+            const float4 wh = w0[j];
+            if (wh.w == 0.f) continue;
+            const float4 ph = a.p[j];
+            const float sh = 4.0f * ph.w;
+            pse_pair(ph, sh * sh, wh, pi, wi, a.nu_dt2, ex, ey, ez);
+            ax += ex;
+            ay += ey;
+            az += ez;
+        }
+    }
+    const uint32_t b = a.start[c], e = b + a.count[c];
+    for (uint32_t j = b; j < i; ++j) {
+        const float4 wh = w0[j];
+        if (wh.w == 0.f) continue;
+        const float4 ph = a.p[j];
+        const float sh = 4.0f * ph.w;
+        pse_pair(ph, sh * sh, wh, pi, wi, a.nu_dt2, ex, ey, ez);
+        ax += ex;
+        ay += ey;
+        az += ez;
+    }
+    const bool live = wi.w != 0.f;
+    if (live) {
+        const float si = 4.0f * pi.w;
+        const float r2 = si * si;
+        for (uint32_t j = i + 1; j < e; ++j) {
+            pse_pair(pi, r2, wi, a.p[j], w0[j], a.nu_dt2, ex, ey, ez);
+            ax -= ex;
+            ay -= ey;
+            az -= ez;
+        }
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            const uint32_t cf = c + ((uint32_t)d[k][0] + nx * d[k][1] + nxy * d[k][2]);
+            const uint32_t bf = a.start[cf], ef = bf + a.count[cf];
+            for (uint32_t j = bf; j < ef; ++j) {
+                pse_pair(pi, r2, wi, a.p[j], w0[j], a.nu_dt2, ex, ey, ez);
+                ax -= ex;
+                ay -= ey;
+                az -= ez;
+            }
+        }
+    }
+    float4 o = make_float4(wi.x + ax, wi.y + ay, wi.z + az, wi.w);
+    if (live) {
+        o.x -= o.x * a.dissipation;
+        o.y -= o.y * a.dissipation;
+        o.z -= o.z * a.dissipation;
+    }
+    w1[i] = o;
+}
+
 __global__ void k_pse_gather(const uint32_t *__restrict__ idx, const float *__restrict__ angvel,
                              size_t ang_stride, const int32_t *__restrict__ birth,
                              size_t birth_stride, size_t n, float4 *__restrict__ w) {
@@ -256,7 +346,7 @@ __global__ void k_pse_scatter(const uint32_t *__restrict__ idx, const float4 *__
 }  // namespace
 
 vg_status pse_diffuse(const vg_tree *t, float *angvel, size_t ang_stride, const int32_t *birth,
-                      size_t birth_stride, float viscosity, float dt, hipStream_t s) {
+                      size_t birth_stride, float viscosity, float dt, int order, hipStream_t s) {
     TreeView tv;
     VG_TRY(tree_checked_view(t, &tv));
     const size_t n = tv.n;
@@ -264,8 +354,10 @@ vg_status pse_diffuse(const vg_tree *t, float *angvel, size_t ang_stride, const 
     const vg_grid_desc &g = tv.layer[0];
     VG_REQUIRE(g.np[0] >= 2 && g.np[1] >= 2 && g.np[2] >= 2, "leaf grid is degenerate");
     VG_REQUIRE(capacity(g) < (size_t(1) << 32), "leaf grid has 2^32 points or more");
+    VG_REQUIRE(order == VG_PSE_REFERENCE_ORDER || order == VG_PSE_SIMULTANEOUS, "bad PSE order");
+    const bool sim = order == VG_PSE_SIMULTANEOUS;
     float4 *w = nullptr;
-    VG_TRY(salloc((void **)&w, n * sizeof(float4), s));
+    VG_TRY(salloc((void **)&w, (sim ? 2 : 1) * n * sizeof(float4), s));
     PseArgs a;
     a.nx = g.np[0];
     a.ny = g.np[1];
@@ -279,16 +371,24 @@ vg_status pse_diffuse(const vg_tree *t, float *angvel, size_t ang_stride, const 
     vg_status st = launch("vg_pse_gather", k_pse_gather, dim3(ceil_div(n, 256)), dim3(256), 0, s,
                           tv.sorted_idx, (const float *)angvel, ang_stride, birth, birth_stride,
                           n, w);
-    const uint32_t slices = a.nz - 1;  // cell slices 0 .. nz-2
-    for (uint32_t phase = 0; phase < 2 && st == VG_OK; ++phase) {
-        const uint32_t groups = (slices - phase + 1) / 2;
-        if (groups)
-            st = launch("vg_pse_phase", k_pse_phase, dim3(groups), dim3(kPseThreads), 0, s, a,
-                        phase);
+    const float4 *result = w;
+    if (sim) {
+        if (st == VG_OK)
+            st = launch("vg_pse_simultaneous", k_pse_simultaneous, dim3(ceil_div(n, 256)),
+                        dim3(256), 0, s, a, g, (const float4 *)w, w + n, (uint32_t)n);
+        result = w + n;
+    } else {
+        const uint32_t slices = a.nz - 1;  // cell slices 0 .. nz-2
+        for (uint32_t phase = 0; phase < 2 && st == VG_OK; ++phase) {
+            const uint32_t groups = (slices - phase + 1) / 2;
+            if (groups)
+                st = launch("vg_pse_phase", k_pse_phase, dim3(groups), dim3(kPseThreads), 0, s,
+                            a, phase);
+        }
     }
     if (st == VG_OK)
         st = launch("vg_pse_scatter", k_pse_scatter, dim3(ceil_div(n, 256)), dim3(256), 0, s,
-                    tv.sorted_idx, (const float4 *)w, n, angvel, ang_stride);
+                    tv.sorted_idx, result, n, angvel, ang_stride);
     sfree(w, s);
     return st;
 }
@@ -312,9 +412,10 @@ VG_API vg_status vg_tree_partition(const vg_tree *t, const uint32_t **cell_start
 
 VG_API vg_status vg_tree_pse_diffuse(const vg_tree *t, float *angvel, size_t ang_stride,
                                      const int32_t *birth, size_t birth_stride, float viscosity,
-                                     float dt, vg_stream s) {
+                                     float dt, int order, vg_stream s) {
     VG_REQUIRE(angvel && ang_stride >= 3, "bad angular velocity array");
     VG_REQUIRE(!birth || birth_stride >= 1, "bad birth stride");
     VG_REQUIRE(viscosity * dt <= 1.0f, "viscosity * dt exceeds 1 (vorticity would flip)");
-    return pse_diffuse(t, angvel, ang_stride, birth, birth_stride, viscosity, dt, (hipStream_t)s);
+    return pse_diffuse(t, angvel, ang_stride, birth, birth_stride, viscosity, dt, order,
+                       (hipStream_t)s);
 }

@@ -28,7 +28,7 @@ vg_status stretch_tilt_fused(const vg_grid_desc &g, const float *vel, const floa
                              float dt, const uint32_t *order, hipStream_t s);
 vg_status tree_checked_view(const vg_tree *t, TreeView *v);
 vg_status pse_diffuse(const vg_tree *t, float *angvel, size_t ang_stride, const int32_t *birth,
-                      size_t birth_stride, float viscosity, float dt, hipStream_t s);
+                      size_t birth_stride, float viscosity, float dt, int order, hipStream_t s);
 }  // namespace vg
 
 struct vg_sim {
@@ -270,7 +270,7 @@ VG_API vg_status vg_sim_stretch_tilt(vg_sim *sim, float dt, float *angvel, size_
 
 VG_API vg_status vg_sim_pse_diffuse(vg_sim *sim, float viscosity, float dt, float *angvel,
                                     size_t ang_stride, const int32_t *birth,
-                                    size_t birth_stride) {
+                                    size_t birth_stride, int order) {
     VG_REQUIRE(sim && sim->have_vortons, "no vortons set");
     VG_REQUIRE(angvel && ang_stride >= 3, "bad angular velocity array");
     VG_REQUIRE(viscosity * dt <= 1.0f, "viscosity * dt exceeds 1 (vorticity would flip)");
@@ -278,7 +278,7 @@ VG_API vg_status vg_sim_pse_diffuse(vg_sim *sim, float viscosity, float dt, floa
     TreeView tv;
     VG_TRY(tree_checked_view(sim->tree, &tv));
     VG_REQUIRE(tv.n == sim->v.n, "vorton count changed since the update");
-    return pse_diffuse(sim->tree, angvel, ang_stride, birth, birth_stride, viscosity, dt,
+    return pse_diffuse(sim->tree, angvel, ang_stride, birth, birth_stride, viscosity, dt, order,
                        sim->stream);
 }
 

@@ -197,7 +197,7 @@ class VortonSim {
     VortonSim(const VortonSim &that)  // copies drop the vortons (vortonSim.h:286-295)
         : cfg_(that.cfg_), viscosity_(that.viscosity_), density_(that.density_),
           fluidSimTechnique_(that.fluidSimTechnique_),
-          investigationTerm_(that.investigationTerm_) {}
+          investigationTerm_(that.investigationTerm_), pseOrder_(that.pseOrder_) {}
     VortonSim &operator=(const VortonSim &that) {
         if (this != &that) {
             cfg_ = that.cfg_;
@@ -205,6 +205,7 @@ class VortonSim {
             density_ = that.density_;
             fluidSimTechnique_ = that.fluidSimTechnique_;
             investigationTerm_ = that.investigationTerm_;
+            pseOrder_ = that.pseOrder_;
             mVortons = nullptr;
         }
         return *this;
@@ -221,6 +222,9 @@ class VortonSim {
         return fluidSimTechnique_;
     }
     InvestigationTermE &GetInvestigationTerm() { return investigationTerm_; }
+    // No reference counterpart: VG_PSE_REFERENCE_ORDER (default, the
+    // reference's result bit for bit) or VG_PSE_SIMULTANEOUS (parallel).
+    void SetPseOrder(int order) { pseOrder_ = order; }
 
     void SetVortons(VECTOR<Vorton> *vortons) { mVortons = vortons; }
     VECTOR<Vorton> *GetVortons() { return mVortons; }
@@ -281,7 +285,7 @@ class VortonSim {
                                    investigationTerm_ == INVESTIGATE_VISCOUS_DIFFUSION))
             check(vg_sim_pse_diffuse(sim_, viscosity_, timeStep, base + kOffAngVel, kStride,
                                      reinterpret_cast<const int32_t *>(base + kOffBirth),
-                                     kStride),
+                                     kStride, pseOrder_),
                   "DiffuseAndDissipateVorticityPSE");
         check(vg_memcpy_dtoh(mVortons->data(), dev_.get(), mVortons->size() * sizeof(Vorton)),
               "download vortons");
@@ -379,6 +383,7 @@ class VortonSim {
     float viscosity_ = 0.0f, density_ = 1.0f;
     FluidSimulationTechniqueE fluidSimTechnique_ = FLUID_SIM_VPM_SPH_HYBRID;
     InvestigationTermE investigationTerm_ = INVESTIGATE_ALL;
+    int pseOrder_ = VG_PSE_REFERENCE_ORDER;
 };
 
 // pclOpVortonSim.h:46-71 / pclOpVortonSim.cpp:51-71

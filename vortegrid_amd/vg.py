@@ -369,11 +369,12 @@ class Tree:
         return tuple(out)
 
     def pse_diffuse(self, angvel, viscosity, dt, ang_stride=3, birth=None, birth_stride=18,
-                    stream=None):
-        """DiffuseAndDissipateVorticityPSE on the vortons this tree was built from."""
+                    order=0, stream=None):
+        """DiffuseAndDissipateVorticityPSE on the vortons this tree was built
+        from (order 0: the reference's in-place order; 1: simultaneous)."""
         call("vg_tree_pse_diffuse", self.h, _p(angvel), C.c_size_t(ang_stride),
              _p(birth) if birth is not None else None, C.c_size_t(birth_stride),
-             C.c_float(viscosity), C.c_float(dt), _s(stream))
+             C.c_float(viscosity), C.c_float(dt), int(order), _s(stream))
 
     def mac_work(self, g: GridDesc, near_k=2, expand_from=-1, stream=None):
         """(cluster interactions at points, direct pairs, clusters expanded)."""
@@ -723,11 +724,12 @@ class Sim:
         """StretchAndTiltVortons on the last update's velocity grid."""
         call("vg_sim_stretch_tilt", self.h, C.c_float(dt), _p(angvel), C.c_size_t(ang_stride))
 
-    def pse_diffuse(self, viscosity, dt, angvel, ang_stride=3, birth=None, birth_stride=18):
+    def pse_diffuse(self, viscosity, dt, angvel, ang_stride=3, birth=None, birth_stride=18,
+                    order=0):
         """DiffuseAndDissipateVorticityPSE on the last update's partition."""
         call("vg_sim_pse_diffuse", self.h, C.c_float(viscosity), C.c_float(dt), _p(angvel),
              C.c_size_t(ang_stride), _p(birth) if birth is not None else None,
-             C.c_size_t(birth_stride))
+             C.c_size_t(birth_stride), int(order))
 
     def vector_potential(self):
         g = GridDesc()
