@@ -153,6 +153,7 @@ def lib():
         L.vgo_mg_cascade.argtypes = [C.POINTER(Grid), f32p, f32p, C.c_size_t]
         L.vgo_curl.argtypes = [C.POINTER(Grid), f32p, f32p]
         L.vgo_jacobian.argtypes = [C.POINTER(Grid), f32p, f32p]
+        L.vgo_brick_of_bytes.argtypes = [C.POINTER(Grid), f32p, C.c_void_p, f32p]
         L.vgo_pse_exp.restype = C.c_float
         L.vgo_pse_exp.argtypes = [C.c_float]
         L.vgo_pse_diffuse.argtypes = [C.POINTER(Grid), f32p, f32p, C.c_void_p, C.c_float,
@@ -513,6 +514,15 @@ def pse_diffuse(g: Grid, pos, angvel, size, viscosity, dt, alive=None, simultane
                           al.ctypes.data_as(C.c_void_p) if al is not None else None, n,
                           float(viscosity), float(dt))
     return w.reshape(-1, 3)
+
+
+def brick_of_bytes(g: Grid, field):
+    """GenerateBrickOfBytes data: (planes (4, capacity) uint8, stats[7])."""
+    planes = np.zeros((4, g.capacity), np.uint8)
+    stats = np.zeros(7, np.float32)
+    lib().vgo_brick_of_bytes(C.byref(g), _f32(field).reshape(-1),
+                             planes.ctypes.data_as(C.c_void_p), stats)
+    return planes, stats
 
 
 def residual_rel(g: Grid, u, f) -> float:

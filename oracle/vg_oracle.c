@@ -1456,6 +1456,43 @@ void vgo_pse_diffuse_simultaneous(const vgo_grid *g, const float *pos, float *an
     free(count); free(start); free(list); free(slot); free(cell); free(cidx); free(w0);
 }
 
+void vgo_brick_of_bytes(const vgo_grid *g, const float *field, unsigned char *planes,
+                        float stats[7]) {
+    const size_t cap = (size_t)g->np[0] * g->np[1] * g->np[2];
+    float mn[3] = {field[0], field[1], field[2]}, mx[3] = {field[0], field[1], field[2]};
+    for (size_t o = 0; o < cap; ++o) {
+        const float *r = field + 3 * o;
+        /* Min2(a, b) = a < b ? a : b with Vec3 '<' true only when all three
+         * components are strictly less; likewise Max2 with '>' */
+        if (!(mn[0] < r[0] && mn[1] < r[1] && mn[2] < r[2]))
+            for (int a = 0; a < 3; ++a) mn[a] = r[a];
+        if (!(mx[0] > r[0] && mx[1] > r[1] && mx[2] > r[2]))
+            for (int a = 0; a < 3; ++a) mx[a] = r[a];
+    }
+    float ext[3], range[3];
+    for (int a = 0; a < 3; ++a) {
+        ext[a] = -mn[a] > mx[a] ? -mn[a] : mx[a];
+        const float d = mx[a] - mn[a];
+        range[a] = FLT_MIN > d ? FLT_MIN : d;
+    }
+    const float magmax = sqrtf(ext[0] * ext[0] + ext[1] * ext[1] + ext[2] * ext[2]);
+    const float almost256 = 256.0f * (1.0f - FLT_EPSILON);
+    for (size_t o = 0; o < cap; ++o) {
+        const float *v = field + 3 * o;
+        const float mag = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        for (int a = 0; a < 3; ++a) {
+            const float t = (v[a] - mn[a]) / range[a];
+            planes[a * cap + o] = (unsigned char)(int)(t * almost256);
+        }
+        planes[3 * cap + o] = (unsigned char)(int)(almost256 * mag / (magmax + FLT_MIN));
+    }
+    for (int a = 0; a < 3; ++a) {
+        stats[a] = mn[a];
+        stats[3 + a] = mx[a];
+    }
+    stats[6] = magmax;
+}
+
 double vgo_residual_rel(const vgo_grid *g, const float *u, const float *f) {
     const double rx = 1.0 / g->cell[0], ry = 1.0 / g->cell[1], rz = 1.0 / g->cell[2];
     const double r2[3] = {rx * rx, ry * ry, rz * rz};

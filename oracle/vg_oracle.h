@@ -240,6 +240,12 @@ void vgo_pse_diffuse_simultaneous(const vgo_grid *g, const float *pos, float *an
                                   const float *size, float uniform_size,
                                   const unsigned char *alive, size_t n,
                                   float viscosity, float dt);
+/* UniformGrid<Vec3>::GenerateBrickOfBytes (uniformGridDiagnostics.cpp:
+ * 209-314), data part: planes[k * cap + o] for k = x, y, z, magnitude;
+ * stats = {min x, y, z, max x, y, z, magnitude max} with ComputeStatistics'
+ * all-components Min2/Max2 scan (uniformGrid.h:1095-1107, macros.h:61-64). */
+void vgo_brick_of_bytes(const vgo_grid *g, const float *field, unsigned char *planes,
+                        float stats[7]);
 /* FP64 residual ||f - L u|| / ||f|| over interior points, 7-point Laplacian. */
 double vgo_residual_rel(const vgo_grid *g, const float *u, const float *f);
 

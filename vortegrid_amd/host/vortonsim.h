@@ -12,8 +12,12 @@
 // demand (GetVelocityGrid and friends).
 #pragma once
 
+#include <sys/stat.h>
+
 #include <cfloat>
+#include <cmath>
 #include <cstddef>
+#include <cstdio>
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
@@ -125,10 +129,80 @@ class UniformGrid : public UniformGridGeometry {
             check(vg_memcpy_dtoh(contents_.data(), device, contents_.size() * sizeof(ItemT)),
                   "UniformGrid download");
     }
+    // Wrap host values of the given geometry.
+    void Assign(const vg_grid_desc &g, const ItemT *values) {
+        g_ = g;
+        contents_.assign(values, values + GetGridCapacity());
+    }
+    // uniformGridDiagnostics.cpp:209-314 (Vec3 grids only): OGLE brick of bytes.
+    void GenerateBrickOfBytes(const char *strFilenameBase, unsigned uFrame) const;
 
   private:
     std::vector<ItemT> contents_;
 };
+
+// uniformGridDiagnostics.cpp:209-314 with ENFORCE_SYMMETRIC_RANGE off, as the
+// reference builds it.  Writes Ogle/Vols/<base>{X,Y,Z,M}<frame>-<nx>x<ny>x<nz>.dat
+// (one byte per point, x fastest, then "MIN %g MAX %g\n") and appends two
+// lines to Ogle/<base>.ogle.  The range comes from ComputeStatistics
+// (uniformGrid.h:1095-1107): Min2/Max2 (macros.h:61-64) on Vec3 keep the
+// running extreme only when it is strictly below (above) the next value in
+// all three components (vec3.h:389-392), a sequential, order-dependent scan
+// that the bytes depend on.
+template <>
+inline void UniformGrid<Vec3>::GenerateBrickOfBytes(const char *strFilenameBase,
+                                                    unsigned uFrame) const {
+    if (contents_.empty()) return;
+    Vec3 vMin = contents_[0], vMax = vMin;
+    for (const Vec3 &r : contents_) {
+        const bool lt = vMin.x < r.x && vMin.y < r.y && vMin.z < r.z;
+        const bool gt = vMax.x > r.x && vMax.y > r.y && vMax.z > r.z;
+        if (!lt) vMin = r;
+        if (!gt) vMax = r;
+    }
+    auto max2 = [](float a, float b) { return a > b ? a : b; };
+    const float ex = max2(-vMin.x, vMax.x), ey = max2(-vMin.y, vMax.y), ez = max2(-vMin.z, vMax.z);
+    const float fMagMax = std::sqrt(ex * ex + ey * ey + ez * ez);
+    const float range[3] = {max2(FLT_MIN, vMax.x - vMin.x), max2(FLT_MIN, vMax.y - vMin.y),
+                            max2(FLT_MIN, vMax.z - vMin.z)};
+    const unsigned nx = GetNumPoints(0), ny = GetNumPoints(1), nz = GetNumPoints(2);
+    char names[4][512];
+    const char axis[4] = {'X', 'Y', 'Z', 'M'};
+    for (int k = 0; k < 4; ++k)
+        std::snprintf(names[k], sizeof names[k], "Ogle/Vols/%s%c%05u-%ux%ux%u.dat",
+                      strFilenameBase, axis[k], uFrame, nx, ny, nz);
+    ::mkdir("Ogle", 0777);
+    ::mkdir("Ogle/Vols", 0777);
+    {
+        char script[512];
+        std::snprintf(script, sizeof script, "Ogle/%s.ogle", strFilenameBase);
+        FILE *f = std::fopen(script, "a");
+        if (!f) throw std::runtime_error("GenerateBrickOfBytes: cannot open the script file");
+        std::fprintf(f, "# %s ranges: {%9.7g,%9.7g,%9.7g} to {%9.7g,%9.7g,%9.7g}\n",
+                     strFilenameBase, vMin.x, vMin.y, vMin.z, vMax.x, vMax.y, vMax.z);
+        std::fprintf(f, "%ux%ux%u %s %s %s\n", nx, ny, nz, names[0], names[1], names[2]);
+        std::fclose(f);
+    }
+    const float fAlmost256 = 256.0f * (1.0f - FLT_EPSILON);
+    std::vector<unsigned char> planes[4];
+    for (auto &p : planes) p.resize(contents_.size());
+    for (size_t o = 0; o < contents_.size(); ++o) {
+        const Vec3 &v = contents_[o];
+        const float fMag = std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+        const float t[3] = {(v.x - vMin.x) / range[0], (v.y - vMin.y) / range[1],
+                            (v.z - vMin.z) / range[2]};
+        for (int k = 0; k < 3; ++k) planes[k][o] = (unsigned char)(int)(t[k] * fAlmost256);
+        planes[3][o] = (unsigned char)(int)(fAlmost256 * fMag / (fMagMax + FLT_MIN));
+    }
+    const float lo[4] = {vMin.x, vMin.y, vMin.z, 0.0f}, hi[4] = {vMax.x, vMax.y, vMax.z, fMagMax};
+    for (int k = 0; k < 4; ++k) {
+        FILE *f = std::fopen(names[k], "wb");
+        if (!f) throw std::runtime_error("GenerateBrickOfBytes: cannot create a data file");
+        std::fwrite(planes[k].data(), 1, planes[k].size(), f);
+        std::fprintf(f, "MIN %g MAX %g\n", lo[k], hi[k]);
+        std::fclose(f);
+    }
+}
 
 // ------------------------------------------------------------------ device staging
 class DeviceBuffer {
