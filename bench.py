@@ -79,6 +79,10 @@ def main():
     ap.add_argument("--effects", type=int, default=256)
     ap.add_argument("--batch-near-k", type=int, default=2)
     ap.add_argument("--batch-expand-from", type=int, default=2)
+    ap.add_argument("--graph-frame", action="store_true",
+                    help="the default frame workload (1M vortons, 4M tracers) as one effect of the "
+                         "batched path: bounding box, grid definition, tree, walk and Euler steps "
+                         "on the device, replayed as one hipGraph launch per frame")
     ap.add_argument("--copy-probe", action="store_true",
                     help="one 256 MiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -93,6 +97,9 @@ def main():
         return
     if args.batched:
         batched(args)
+        return
+    if args.graph_frame:
+        graph_frame(args)
         return
 
     from vortegrid_amd import scenarios, vg
@@ -536,6 +543,43 @@ def _timed(vg, name, fn, reps):
     prof = vg.profile_dump()
     cnt, ms = prof.get(name, (0, 0.0))
     return ms / max(cnt, 1), prof
+
+
+def graph_frame(args):
+    """The default frame's workload with no host round trip: one effect of the
+    batched path (device-side bounding box and grid definition at a fixed
+    128^3-cell shape, which is what DefineShape picks for this scene), eager
+    and as a captured hipGraph."""
+    from vortegrid_amd import scenarios, vg
+
+    vg.require_gpu()
+    pos, om, radius = scenarios.random_rings(args.vortons, seed=5)
+    tr = scenarios.tracers_in_box(args.tracers, pos.min(0), pos.max(0), seed=6)
+    b = vg.Batch([len(pos)], [len(tr)], [radius], cells=128, near_k=args.near_k, order=args.order,
+                 policy=args.policy, expand_from=args.expand_from)
+    b.upload(pos, om, tr)
+    dt = 1.0 / 30.0
+    s = vg.Stream()
+    res = {}
+    for mode in ("eager", "graph"):
+        if mode == "graph":
+            b.capture(dt, s)
+        run = (lambda: b.step(dt, s)) if mode == "eager" else (lambda: b.launch(s))
+        for _ in range(args.warmup):
+            run()
+        s.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            run()
+        s.sync()
+        res[mode] = (time.perf_counter() - t0) * 1e3 / args.steps
+    g, _ = b.grid(0)
+    print(json.dumps({
+        "workload": "default frame workload as one batched effect (device geometry, no host "
+                    "round trip): 1M vortons (64 random rings), 4M tracers, near_k=%d, order=%d, "
+                    "expand_from=%d" % (args.near_k, args.order, args.expand_from),
+        "grid": list(g.np), "eager_ms_per_frame": res["eager"],
+        "graph_ms_per_frame": res["graph"], "steps": args.steps}), flush=True)
 
 
 def suite(args):
