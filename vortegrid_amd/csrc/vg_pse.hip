@@ -119,6 +119,92 @@ __device__ __forceinline__ float lane_set(float v, float old, int lane) {
     return (int)(threadIdx.x & 63) == lane ? v : old;
 }
 
+// Systolic form for cells with several vortons.  Pair (i, j) needs Omega_h
+// after pairs (i, j' < j) and Omega_t after pairs (i' < i, j); lane l holds
+// here-vorton i = B + l while the partners stream down the lanes, one lane
+// per step: partner j enters lane 0 at step j - B and meets lane l at step
+// j - B + l, after lanes 0 .. l-1 have exchanged with it.  At j == i the
+// incoming value is the here-vorton itself, already updated by the earlier
+// here-vortons, and the lane adopts it.  Every variable sees the same
+// operations in the same order as in pse_cell, with the serial chain
+// shortened from k * n exchanges to k + n steps.  Here-vortons go in chunks
+// of 64 lanes. Partners are read from memory in windows of 64 and leave the
+// last lane back into memory, with a fence before the next chunk reads them.
+constexpr uint32_t kSystolicMin = 4;
+
+__device__ __forceinline__ uint32_t partner_index(const uint32_t base[8], const uint32_t pre[9],
+                                                  uint32_t j) {
+    uint32_t g = base[0] + j;
+#pragma unroll
+    for (int q = 1; q < 8; ++q) g = j >= pre[q] ? base[q] + (j - pre[q]) : g;
+    return g;
+}
+
+__device__ __forceinline__ float4 shfl_up4(float4 v) {
+    return make_float4(__shfl_up(v.x, 1, 64), __shfl_up(v.y, 1, 64), __shfl_up(v.z, 1, 64),
+                       __shfl_up(v.w, 1, 64));
+}
+
+__device__ __forceinline__ float4 lane4(float4 v, int l) {
+    return make_float4(lane_get(v.x, l), lane_get(v.y, l), lane_get(v.z, l), lane_get(v.w, l));
+}
+
+__device__ __forceinline__ void pse_cell_systolic(const PseArgs &a, const uint32_t base[8],
+                                                  const uint32_t pre[9], uint32_t k, uint32_t n) {
+#pragma clang fp contract(off)
+    const int lane = (int)(threadIdx.x & 63);
+    for (uint32_t B = 0; B < k; B += 64) {
+        const uint32_t L = k - B < 64u ? k - B : 64u;
+        const uint32_t i = B + (uint32_t)lane;
+        float4 wh = make_float4(0.f, 0.f, 0.f, 0.f), ph = wh, sp = wh, sw = wh, bp = wh, bw = wh;
+        float r2 = 0.f;
+        bool live = false;
+        const uint32_t steps = (n - B) + L - 1;
+        for (uint32_t t = 0; t < steps; ++t) {
+            if ((t & 63u) == 0) {  // the next 64 partners to enter, one per lane
+                const uint32_t jj = B + t + (uint32_t)lane;
+                if (jj < n) {
+                    const uint32_t g = partner_index(base, pre, jj);
+                    bp = a.p[g];
+                    bw = a.w[g];
+                }
+            }
+            sp = shfl_up4(sp);
+            sw = shfl_up4(sw);
+            const float4 ip = lane4(bp, (int)(t & 63u)), iw = lane4(bw, (int)(t & 63u));
+            if (lane == 0) {
+                sp = ip;
+                sw = iw;
+            }
+            const int jl = (int)t - lane;  // partner j - B at this lane
+            const uint32_t j = B + (uint32_t)jl;
+            if ((uint32_t)lane < L && jl >= 0 && j < n) {
+                if (j == i) {
+                    wh = sw;
+                    ph = sp;
+                    live = sw.w != 0.f;
+                    const float range = 4.0f * ph.w;
+                    r2 = range * range;
+                } else if (j > i && live) {
+                    exchange(ph, r2, wh, sp, &sw, a.nu_dt2);
+                }
+                if (j == n - 1) {  // the lane's last partner: done
+                    if (live) {
+                        wh.x -= wh.x * a.dissipation;
+                        wh.y -= wh.y * a.dissipation;
+                        wh.z -= wh.z * a.dissipation;
+                    }
+                    a.w[partner_index(base, pre, i)] = wh;
+                }
+                // a partner that is not a here-vorton of this chunk leaves
+                // the last lane with its value for the next chunk
+                if ((uint32_t)lane == L - 1 && j >= B + L) a.w[partner_index(base, pre, j)] = sw;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    }
+}
+
 __device__ __forceinline__ void pse_cell_wave(const PseArgs &a, uint32_t x, uint32_t y, uint32_t z) {
 #pragma clang fp contract(off)
     const int lane = (int)(threadIdx.x & 63);
@@ -134,6 +220,10 @@ __device__ __forceinline__ void pse_cell_wave(const PseArgs &a, uint32_t x, uint
     }
     const uint32_t k = pre[1], n = pre[8];
     if (k == 0) return;
+    if (k >= kSystolicMin) {
+        pse_cell_systolic(a, base, pre, k, n);
+        return;
+    }
     if (n > 64u * kSlots) {
         if (lane == 0) pse_cell(a, x, y, z);
         return;
