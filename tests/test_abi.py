@@ -71,3 +71,42 @@ def test_status_codes_without_gpu_work():
     assert L.vg_decimate(C.byref(g), C.byref(g), 2) == vg.VG_EINVAL  # empty source
     assert b"empty" in L.vg_last_error()
     assert L.vg_version().startswith(b"vortegrid-amd")
+
+
+def test_status_codes_of_the_vorticity_terms():
+    """Argument checks of the stretching, partition and PSE entry points
+    return before any device work."""
+    vg, L = _lib()
+    g = vg.GridDesc()
+    buf = (C.c_float * 64)()
+    assert L.vg_jacobian(None, buf, buf, None) == vg.VG_EINVAL
+    assert L.vg_jacobian(C.byref(g), buf, buf, None) == vg.VG_EINVAL      # np < 2
+    for a in range(3):
+        g.np[a], g.extent[a], g.cell[a], g.cpe[a] = 3, 1.0, 0.5, 2.0
+    assert L.vg_jacobian(C.byref(g), None, buf, None) == vg.VG_EINVAL
+    # no vortons: nothing to do, no device work
+    assert L.vg_stretch_tilt(C.byref(g), buf, buf, C.c_size_t(3), buf, C.c_size_t(3),
+                             C.c_size_t(0), C.c_float(0.1), None, None) == vg.VG_OK
+    assert L.vg_stretch_tilt_from_velocity(C.byref(g), buf, buf, C.c_size_t(2), buf,
+                                           C.c_size_t(3), C.c_size_t(5), C.c_float(0.1), None,
+                                           None) == vg.VG_EINVAL  # stride < 3
+    assert L.vg_sim_stretch_tilt(None, C.c_float(0.1), buf, C.c_size_t(3)) == vg.VG_EINVAL
+    assert L.vg_sim_pse_diffuse(None, C.c_float(0.1), C.c_float(0.1), buf, C.c_size_t(3), None,
+                                C.c_size_t(18), 0) == vg.VG_EINVAL
+    t = C.c_void_p()
+    assert L.vg_tree_create(C.byref(t)) == vg.VG_OK
+    try:
+        p = C.c_void_p()
+        cells = C.c_size_t()
+        assert L.vg_tree_partition(t, None, C.byref(p), C.byref(p), C.byref(cells)) == vg.VG_EINVAL
+        assert L.vg_tree_partition(t, C.byref(p), C.byref(p), C.byref(p),
+                                   C.byref(cells)) == vg.VG_ESTATE   # not built
+        assert L.vg_tree_pse_diffuse(t, None, C.c_size_t(3), None, C.c_size_t(18), C.c_float(0.1),
+                                     C.c_float(0.1), 0, None) == vg.VG_EINVAL
+        assert L.vg_tree_pse_diffuse(t, buf, C.c_size_t(3), None, C.c_size_t(18), C.c_float(20.0),
+                                     C.c_float(0.1), 0, None) == vg.VG_EINVAL  # nu dt > 1
+        assert b"flip" in L.vg_last_error()
+        assert L.vg_tree_pse_diffuse(t, buf, C.c_size_t(3), None, C.c_size_t(18), C.c_float(0.1),
+                                     C.c_float(0.1), 0, None) == vg.VG_ESTATE  # not built
+    finally:
+        L.vg_tree_destroy(t)
