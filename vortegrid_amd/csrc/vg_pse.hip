@@ -140,9 +140,17 @@ __device__ __forceinline__ uint32_t partner_index(const uint32_t base[8], const 
     return g;
 }
 
-__device__ __forceinline__ float4 shfl_up4(float4 v) {
-    return make_float4(__shfl_up(v.x, 1, 64), __shfl_up(v.y, 1, 64), __shfl_up(v.z, 1, 64),
-                       __shfl_up(v.w, 1, 64));
+// Lane l takes v from lane l - 1 and lane 0 takes `in`: the whole-wave DPP
+// shift (wave_shr:1; lane 0 has no source lane and keeps the old value,
+// here `in`), a register move instead of an LDS permute on the serial chain.
+__device__ __forceinline__ float wave_shr(float v, float in) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(in), __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float4 wave_shr4(float4 v, float4 in) {
+    return make_float4(wave_shr(v.x, in.x), wave_shr(v.y, in.y), wave_shr(v.z, in.z),
+                       wave_shr(v.w, in.w));
 }
 
 __device__ __forceinline__ float4 lane4(float4 v, int l) {
@@ -169,13 +177,8 @@ __device__ __forceinline__ void pse_cell_systolic(const PseArgs &a, const uint32
                     bw = a.w[g];
                 }
             }
-            sp = shfl_up4(sp);
-            sw = shfl_up4(sw);
-            const float4 ip = lane4(bp, (int)(t & 63u)), iw = lane4(bw, (int)(t & 63u));
-            if (lane == 0) {
-                sp = ip;
-                sw = iw;
-            }
+            sp = wave_shr4(sp, lane4(bp, (int)(t & 63u)));
+            sw = wave_shr4(sw, lane4(bw, (int)(t & 63u)));
             const int jl = (int)t - lane;  // partner j - B at this lane
             const uint32_t j = B + (uint32_t)jl;
             if ((uint32_t)lane < L && jl >= 0 && j < n) {
