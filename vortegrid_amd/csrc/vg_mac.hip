@@ -47,6 +47,45 @@ struct WaveStage {
     uint32_t row_off[64], row_start[64];
 };
 
+// Lane-strided walk over the cells of a bx x by x bz box: lane `lane` visits
+// cells t = lane, lane + 64, ... as (x, y, z) = (t % bx, t / bx % by,
+// t / (bx by)), stepped without divisions (64 = qz bx by + ry bx + rx, one
+// carry per axis).  t < bx by bz exactly when z < bz.
+struct BoxIter {
+    int x, y, z, bx, by, rx, ry, qz;
+};
+
+__device__ __forceinline__ BoxIter box_begin(int bx, int by, uint32_t lane) {
+    BoxIter it;
+    const int bxy = bx * by;
+    it.bx = bx;
+    it.by = by;
+    it.z = (int)lane / bxy;
+    const int r = (int)lane - it.z * bxy;
+    it.y = r / bx;
+    it.x = r - it.y * bx;
+    it.qz = 64 / bxy;
+    const int r64 = 64 - it.qz * bxy;
+    it.ry = r64 / bx;
+    it.rx = r64 - it.ry * bx;
+    return it;
+}
+
+__device__ __forceinline__ void box_next(BoxIter &it) {
+    it.x += it.rx;
+    int c = it.x >= it.bx ? 1 : 0;
+    it.x -= c ? it.bx : 0;
+    it.y += it.ry + c;
+    c = it.y >= it.by ? 1 : 0;
+    it.y -= c ? it.by : 0;
+    it.z += it.qz + c;
+}
+
+// Offset of a cell of a layer (the tree's grids hold fewer than 2^32 points).
+__device__ __forceinline__ uint32_t cell_offset(const vg_grid_desc &g, int x, int y, int z) {
+    return (uint32_t)x + g.np[0] * ((uint32_t)y + g.np[1] * (uint32_t)z);
+}
+
 // Kernel factors for one cluster size; contributions are accumulated
 // unscaled and multiplied by s = (2/3) r^3 when the size changes or the
 // layer ends (every cluster of a tree normally has the vorton size).
@@ -272,15 +311,14 @@ __device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc
             // these layers were applied by k_mac_expand, which left the raw
             // (unscaled by circ) expansion value in out
             if (COUNT) {
-                for (int base = 0; base < total; base += 64) {
-                    const int t = base + (int)lane;
+                BoxIter it = box_begin(bx, by, lane);
+                for (int base = 0; base < total; base += 64, box_next(it)) {
                     bool use = false;
-                    if (t < total) {
-                        const int cx = c0[0] + t % bx, cy = c0[1] + (t / bx) % by,
-                                  cz = c0[2] + t / (bx * by);
+                    if (it.z < bz) {
+                        const int cx = c0[0] + it.x, cy = c0[1] + it.y, cz = c0[2] + it.z;
                         const bool near = cx >= n0[0] && cx <= n1[0] && cy >= n0[1] &&
                                           cy <= n1[1] && cz >= n0[2] && cz <= n1[2];
-                        if (!near) use = P[point_offset(gl, cx, cy, cz)].w != 0.0f;
+                        if (!near) use = P[cell_offset(gl, cx, cy, cz)].w != 0.0f;
                     }
                     n_expand += __popcll(__ballot(use));
                 }
@@ -293,25 +331,25 @@ __device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc
             continue;
         }
         float3 part = make_float3(0.f, 0.f, 0.f), scaled = part;
-        for (int base = 0; base < total; base += 64) {
+        BoxIter it = box_begin(bx, by, lane);
+        for (int base = 0; base < total; base += 64, box_next(it)) {
             // lanes fetch 64 cells of the children box
-            const int t = base + (int)lane;
             bool use = false;
             float4 p = make_float4(0.f, 0.f, 0.f, 0.f), w = p, m0 = p, m1 = p, m2 = p;
-            if (t < total) {
-                const int cx = c0[0] + t % bx, cy = c0[1] + (t / bx) % by, cz = c0[2] + t / (bx * by);
+            if (it.z < bz) {
+                const int cx = c0[0] + it.x, cy = c0[1] + it.y, cz = c0[2] + it.z;
                 const bool near = cx >= n0[0] && cx <= n1[0] && cy >= n0[1] && cy <= n1[1] &&
                                   cz >= n0[2] && cz <= n1[2];
                 if (!near) {
-                    const size_t off = point_offset(gl, cx, cy, cz);
+                    const uint32_t off = cell_offset(gl, cx, cy, cz);
                     p = P[off];
                     use = p.w != 0.0f;
                     if (use && !COUNT) {
                         w = W[off];
                         if (ORDER) {
-                            m0 = M[3 * off];
-                            m1 = M[3 * off + 1];
-                            m2 = M[3 * off + 2];
+                            m0 = M[3 * (size_t)off];
+                            m1 = M[3 * (size_t)off + 1];
+                            m2 = M[3 * (size_t)off + 2];
                         }
                     }
                 }
@@ -451,7 +489,7 @@ __device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc
 template <int ORDER>
 __device__ __forceinline__ void expand_brick(const TreeView &tv, const vg_grid_desc &g,
                                              const MacParams &mp, uint32_t brick,
-                                             float *__restrict__ out) {
+                                             uint32_t *__restrict__ Q, float *__restrict__ out) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t bb[3] = {brick % mp.nb[0], (brick / mp.nb[0]) % mp.nb[1],
                             brick / (mp.nb[0] * mp.nb[1])};
@@ -481,6 +519,23 @@ __device__ __forceinline__ void expand_brick(const TreeView &tv, const vg_grid_d
     }
     Expansion E;
     expansion_zero(E);
+    // Accepted clusters (layer << 28 | offset) queue up in Q and are expanded
+    // 64 at a time, one per lane, so that no lane idles on an empty or near
+    // candidate cell while the others expand.
+    uint32_t qn = 0;  // wave-uniform queue length, < 128
+    auto expand_one = [&](uint32_t e) {
+        const int l = (int)(e >> 28);
+        const uint32_t off = e & 0x0fffffffu;
+        const float4 p = tv.P[l][off];
+        const float4 w = tv.W[l][off];
+        float4 m0 = p, m1 = p, m2 = p;
+        if (ORDER) {
+            m0 = tv.M[l][3 * (size_t)off];
+            m1 = tv.M[l][3 * (size_t)off + 1];
+            m2 = tv.M[l][3 * (size_t)off + 2];
+        }
+        expansion_add<ORDER>(E, cb, p, w, m0, m1, m2, mp.spread);
+    };
     int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     for (int l = (int)tv.depth - 2; l >= mp.expand_from; --l) {
         const vg_grid_desc &gl = tv.layer[l];
@@ -500,28 +555,41 @@ __device__ __forceinline__ void expand_brick(const TreeView &tv, const vg_grid_d
         }
         const int bx = c1[0] - c0[0] + 1, by = c1[1] - c0[1] + 1, bz = c1[2] - c0[2] + 1;
         const int total = bx * by * bz;
-        for (int base = 0; base < total; base += 64) {
-            const int t = base + (int)lane;
-            if (t >= total) continue;
-            const int cx = c0[0] + t % bx, cy = c0[1] + (t / bx) % by, cz = c0[2] + t / (bx * by);
-            if (cx >= n0[0] && cx <= n1[0] && cy >= n0[1] && cy <= n1[1] && cz >= n0[2] &&
-                cz <= n1[2])
-                continue;
-            const size_t off = point_offset(gl, cx, cy, cz);
-            const float4 p = tv.P[l][off];
-            if (p.w == 0.0f) continue;
-            const float4 w = tv.W[l][off];
-            float4 m0 = p, m1 = p, m2 = p;
-            if (ORDER) {
-                m0 = tv.M[l][3 * off];
-                m1 = tv.M[l][3 * off + 1];
-                m2 = tv.M[l][3 * off + 2];
+        BoxIter it = box_begin(bx, by, lane);
+        for (int base = 0; base < total; base += 64, box_next(it)) {
+            bool take = false;
+            uint32_t off = 0;
+            if (it.z < bz) {
+                const int cx = c0[0] + it.x, cy = c0[1] + it.y, cz = c0[2] + it.z;
+                const bool near = cx >= n0[0] && cx <= n1[0] && cy >= n0[1] && cy <= n1[1] &&
+                                  cz >= n0[2] && cz <= n1[2];
+                if (!near) {
+                    off = cell_offset(gl, cx, cy, cz);
+                    take = tv.P[l][off].w != 0.0f;
+                }
             }
-            expansion_add<ORDER>(E, cb, p, w, m0, m1, m2, mp.spread);
+            const unsigned long long m = __ballot(take);
+            if (take) {
+                const uint32_t below = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                Q[qn + below] = ((uint32_t)l << 28) | off;
+            }
+            qn += (uint32_t)__popcll(m);
+            if (qn >= 64) {
+                __builtin_amdgcn_wave_barrier();
+                expand_one(Q[lane]);
+                const uint32_t rest = qn - 64;
+                const uint32_t moved = lane < rest ? Q[64 + lane] : 0u;
+                __builtin_amdgcn_wave_barrier();
+                if (lane < rest) Q[lane] = moved;
+                __builtin_amdgcn_wave_barrier();
+                qn = rest;
+            }
         }
 #pragma unroll
         for (int a = 0; a < 3; ++a) { lo[a] = n0[a]; hi[a] = n1[a]; }
     }
+    __builtin_amdgcn_wave_barrier();
+    if (lane < qn) expand_one(Q[lane]);
     const float3 u = expansion_eval(E, make_float3(q.x - cb.x, q.y - cb.y, q.z - cb.z));
     if (valid) {
         const size_t o = point_offset(g, pt[0], pt[1], pt[2]);
@@ -534,9 +602,11 @@ __device__ __forceinline__ void expand_brick(const TreeView &tv, const vg_grid_d
 template <int ORDER>
 __global__ __launch_bounds__(256) void k_mac_expand(const TreeView tv, vg_grid_desc g, MacParams mp,
                                                     float *__restrict__ out) {
-    const uint32_t brick = uni(blockIdx.x * 4 + uni(threadIdx.x >> 6));
+    __shared__ uint32_t queue[4][128];
+    const uint32_t wave = uni(threadIdx.x >> 6);
+    const uint32_t brick = uni(blockIdx.x * 4 + wave);
     if (brick >= mp.nbricks) return;
-    expand_brick<ORDER>(tv, g, mp, brick, out);
+    expand_brick<ORDER>(tv, g, mp, brick, queue[wave], out);
 }
 
 template <int ORDER, int WPB>
@@ -545,13 +615,15 @@ __global__ __launch_bounds__(64 * WPB) void k_mac_expand_batch(const TreeView *_
                                                                MacParams mp,
                                                                float *__restrict__ out,
                                                                size_t out_stride) {
-    const uint32_t brick = uni(blockIdx.x * WPB + uni(threadIdx.x >> 6));
+    __shared__ uint32_t queue[WPB][128];
+    const uint32_t wave = uni(threadIdx.x >> 6);
+    const uint32_t brick = uni(blockIdx.x * WPB + wave);
     if (brick >= mp.nbricks) return;
     const uint32_t tree = blockIdx.y;
     const TreeView &tv = views[tree];
     MacParams p = mp;
     p.spread = mollify[tree].x;
-    expand_brick<ORDER>(tv, tv.layer[0], p, brick, out + tree * out_stride);
+    expand_brick<ORDER>(tv, tv.layer[0], p, brick, queue[wave], out + tree * out_stride);
 }
 
 template <int POLICY, int ORDER, int COUNT>
@@ -615,6 +687,9 @@ vg_status run_mac(const vg_tree *t, const vg_grid_desc *g, int near_k, int order
     else
         k = order ? k_mac_grid<VG_RSQRT_ACCURATE, 1, COUNT> : k_mac_grid<VG_RSQRT_ACCURATE, 0, COUNT>;
     if (!COUNT && expand_from >= 1 && expand_from <= (int)tv.depth - 2) {
+        // the expansion queue packs (layer << 28 | offset)
+        VG_REQUIRE(tv.depth <= 17 && capacity(tv.layer[expand_from]) < (1u << 28),
+                   "grid too large for local expansions");
         VG_TRY(launch("vg_tree_mac_expand", order ? k_mac_expand<1> : k_mac_expand<0>, grid,
                       dim3(256), 0, s, tv, *g, mp, out));
     }
@@ -655,7 +730,10 @@ vg_status mac_velocity_grid_batch(const TreeView *views, const float2 *mollify, 
          {k_mac_grid_batch<1, 1, 1>, k_mac_grid_batch<1, 1, 2>, k_mac_grid_batch<1, 1, 4>}}};
     const int wi = waves_per_block == 1 ? 0 : waves_per_block == 2 ? 1 : 2;
     if (expand_from >= 1) {
-        // trees of the batch share one shape; the walk checks the depth
+        // trees of the batch share one shape; the walk checks the depth.
+        // The expansion queue packs (layer << 28 | offset).
+        VG_REQUIRE((size_t)np[0] * np[1] * np[2] < (1u << 28),
+                   "grid too large for local expansions");
         using KE = void (*)(const TreeView *, const float2 *, MacParams, float *, size_t);
         KE te[2][3] = {{k_mac_expand_batch<0, 1>, k_mac_expand_batch<0, 2>, k_mac_expand_batch<0, 4>},
                        {k_mac_expand_batch<1, 1>, k_mac_expand_batch<1, 2>, k_mac_expand_batch<1, 4>}};
