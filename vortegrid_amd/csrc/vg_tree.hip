@@ -147,24 +147,22 @@ __global__ void k_scatter(const uint32_t *__restrict__ keys, const uint32_t *__r
     if (i < n) sorted_idx[start[keys[i]] + slots[i]] = (uint32_t)i;
 }
 
-// Restore index order inside each cell (the atomic slots are arbitrary).
-__global__ void k_sort_cells(const uint32_t *__restrict__ start,
-                             const uint32_t *__restrict__ count, size_t ncells,
+// Restore index order inside each cell (the atomic slots are arbitrary):
+// vorton v, found at position p of its cell's segment of `unsorted`, goes to
+// the position given by its rank, the number of smaller indices in the cell.
+// One thread per vorton; the indices are distinct, so the ranks are too.
+__global__ void k_rank_cells(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ start,
+                             const uint32_t *__restrict__ count,
+                             const uint32_t *__restrict__ unsorted, size_t n,
                              uint32_t *__restrict__ sorted_idx) {
-    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells) return;
-    const uint32_t cnt = count[c];
-    if (cnt < 2) return;
-    uint32_t *seg = sorted_idx + start[c];
-    for (uint32_t a = 1; a < cnt; ++a) {
-        const uint32_t v = seg[a];
-        uint32_t b = a;
-        while (b > 0 && seg[b - 1] > v) {
-            seg[b] = seg[b - 1];
-            --b;
-        }
-        seg[b] = v;
-    }
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t v = unsorted[p];
+    const uint32_t c = keys[v];
+    const uint32_t b = start[c], e = b + count[c];
+    uint32_t r = 0;
+    for (uint32_t q = b; q < e; ++q) r += unsorted[q] < v ? 1u : 0u;
+    sorted_idx[b + r] = v;
 }
 
 __global__ void k_gather_sorted(VortonIn vin, const uint32_t *__restrict__ sorted_idx, size_t n,
@@ -666,12 +664,13 @@ vg_status batch_tree_build(const BatchTreeArgs &a, hipStream_t s) {
     VG_TRY(launch("vg_batch_bin", k_bin_batch, dim3(ceil_div(n, 256)), dim3(256), 0, s, a.pos, n,
                   a.tree_of, a.leaf_desc, cap0, a.cell_count, a.keys, a.slots));
     VG_TRY(exclusive_scan(a.cell_count, a.cell_start, cells, a.scan_tmp, s));
+    uint32_t *unsorted = (uint32_t *)a.sorted_p;  // the gather fills sorted_p afterwards
     VG_TRY(launch("vg_batch_scatter", k_scatter, dim3(ceil_div(n, 256)), dim3(256), 0, s,
                   (const uint32_t *)a.keys, (const uint32_t *)a.slots,
-                  (const uint32_t *)a.cell_start, n, a.sorted_idx));
-    VG_TRY(launch("vg_batch_sort_cells", k_sort_cells, dim3(ceil_div(cells, 256)), dim3(256), 0, s,
-                  (const uint32_t *)a.cell_start, (const uint32_t *)a.cell_count, cells,
-                  a.sorted_idx));
+                  (const uint32_t *)a.cell_start, n, unsorted));
+    VG_TRY(launch("vg_batch_sort_cells", k_rank_cells, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                  (const uint32_t *)a.keys, (const uint32_t *)a.cell_start,
+                  (const uint32_t *)a.cell_count, (const uint32_t *)unsorted, n, a.sorted_idx));
     VG_TRY(launch("vg_batch_gather", k_gather_batch, dim3(ceil_div(n, 256)), dim3(256), 0, s, a.pos,
                   a.angvel, a.tree_of, a.size, (const uint32_t *)a.sorted_idx, n, a.sorted_p,
                   a.sorted_w));
@@ -777,11 +776,15 @@ VG_API vg_status vg_tree_build(vg_tree *t, const vg_grid_desc *tmpl, const vg_vo
     VG_TRY(launch("vg_tree_bin", k_bin, dim3(ceil_div(n, 256)), dim3(256), 0, s, vin, n,
                   t->ng.layer[0], t->cell_count, t->keys, t->slots));
     VG_TRY(exclusive_scan(t->cell_count, t->cell_start, cap0, t->scan_tmp, s));
+    // the cell-grouped (not yet index-ordered) list goes through sorted_p,
+    // which the gather fills afterwards
+    uint32_t *unsorted = (uint32_t *)t->sorted_p;
     VG_TRY(launch("vg_tree_scatter", k_scatter, dim3(ceil_div(n, 256)), dim3(256), 0, s,
                   (const uint32_t *)t->keys, (const uint32_t *)t->slots,
-                  (const uint32_t *)t->cell_start, n, t->sorted_idx));
-    VG_TRY(launch("vg_tree_sort_cells", k_sort_cells, dim3(ceil_div(cap0, 256)), dim3(256), 0, s,
-                  (const uint32_t *)t->cell_start, (const uint32_t *)t->cell_count, cap0,
+                  (const uint32_t *)t->cell_start, n, unsorted));
+    VG_TRY(launch("vg_tree_sort_cells", k_rank_cells, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                  (const uint32_t *)t->keys, (const uint32_t *)t->cell_start,
+                  (const uint32_t *)t->cell_count, (const uint32_t *)unsorted, n,
                   t->sorted_idx));
     VG_TRY(launch("vg_tree_gather", k_gather_sorted, dim3(ceil_div(n, 256)), dim3(256), 0, s, vin,
                   (const uint32_t *)t->sorted_idx, n, t->sorted_p, t->sorted_w));
