@@ -59,6 +59,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--vortons", type=int, default=1 << 20)
     ap.add_argument("--tracers", type=int, default=1 << 22)
+    ap.add_argument("--tracer-order", choices=("random", "emitted"), default="random",
+                    help="tracer memory order: uniform random, or the reference emitter's "
+                         "cell-by-cell jittered lattice (pclOpEmit.cpp:77-172)")
     ap.add_argument("--policy", type=int, default=1, help="0 reference rsqrt, 1 accurate")
     ap.add_argument("--near-k", type=int, default=3,
                     help="near-field radius of the accurate treecode; -1 = reference opening rule")
@@ -109,7 +112,8 @@ def main():
     log(f"generating {args.vortons} vortons (random rings) + {args.tracers} tracers")
     pos, om, radius = scenarios.random_rings(args.vortons, seed=5)
     mn, mx = pos.min(0), pos.max(0)
-    tr = scenarios.tracers_in_box(args.tracers, mn, mx, seed=6)
+    tr = (scenarios.tracers_emitted(args.tracers, mn, mx, seed=6)
+          if args.tracer_order == "emitted" else scenarios.tracers_in_box(args.tracers, mn, mx, seed=6))
     stream = vg.Stream()
     d_pos = vg.DeviceArray.from_numpy(pos)
     d_om = vg.DeviceArray.from_numpy(om)
@@ -224,7 +228,26 @@ def main():
     if acnt:
         adv_s = ams / acnt * 1e-3
         gbs = 24.0 * len(tr) / adv_s / 1e9
-        adv = {"avg_launch_ms": ams / acnt, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+        adv = {"tracer_order": args.tracer_order, "avg_launch_ms": ams / acnt,
+               "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+        # the same launch on the same number of tracers in the other memory
+        # order (the gathers of spatially ordered tracers hit the caches)
+        other = "random" if args.tracer_order == "emitted" else "emitted"
+        tr_o = (scenarios.tracers_emitted(len(tr), mn, mx, seed=6) if other == "emitted"
+                else scenarios.tracers_in_box(len(tr), mn, mx, seed=6))
+        d_tro = vg.DeviceArray.from_numpy(tr_o)
+        vg.profile_reset()
+        vg.profile_enable(True)
+        for _ in range(10):
+            vg.advect_tracers(g, gptr, d_tro, None, len(tr_o), dt, 1.0, stream=stream)
+        stream.sync()
+        vg.profile_enable(False)
+        ocnt, oms = vg.profile_dump().get("vg_advect_tracers", (0, 0.0))
+        if ocnt:
+            ogbs = 24.0 * len(tr_o) / (oms / ocnt * 1e-3) / 1e9
+            adv["other_order"] = {"tracer_order": other, "avg_launch_ms": oms / ocnt,
+                                  "achieved_GBs": ogbs, "frac": ogbs / HBM_PEAK_GBS}
+        del d_tro
 
     # ---- accuracy: treecode vs FP64 direct on a 1% sample of grid points
     rel = None

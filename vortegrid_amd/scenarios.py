@@ -105,6 +105,35 @@ def tracers_in_box(n, mn, mx, seed=7):
     return (mn + (mx - mn) * rng.uniform(0.0, 1.0, (n, 3))).astype(np.float32)
 
 
+def tracers_emitted(n, mn, mx, per_cell=2, seed=7):
+    """n tracers as PclOpEmit::Emit places them (pclOpEmit.cpp:77-172): the
+    box is cut into cells (about cubic, enough for n), each cell holds
+    per_cell^3 tracers, one per sub-cell at the sub-cell centre plus a
+    uniform jitter of up to half a sub-cell, emitted cell by cell in z, y, x
+    order and inside a cell sub-cell by sub-cell in z, y, x order; the first
+    n are kept.  Memory order follows space, as in the reference's tracer
+    arrays."""
+    rng = np.random.default_rng(seed)
+    mn = np.asarray(mn, np.float64)
+    mx = np.asarray(mx, np.float64)
+    ext = np.maximum(mx - mn, 1e-30)
+    cells = n / per_cell ** 3
+    side = (cells / np.prod(ext)) ** (1.0 / 3.0)
+    nc = np.maximum(1, np.ceil(ext * side)).astype(np.int64)
+    while np.prod(nc) * per_cell ** 3 < n:
+        nc[np.argmin(nc / ext)] += 1
+    h = ext / nc
+    sub = h / per_cell
+    cz, cy, cx, sz, sy, sx = np.meshgrid(np.arange(nc[2]), np.arange(nc[1]), np.arange(nc[0]),
+                                         np.arange(per_cell), np.arange(per_cell),
+                                         np.arange(per_cell), indexing="ij")
+    cell = np.stack([cx, cy, cz], -1).reshape(-1, 3)[:n]
+    sc = np.stack([sx, sy, sz], -1).reshape(-1, 3)[:n]
+    jitter = rng.uniform(-0.5, 0.5, (n, 3)) * 0.9999
+    pos = mn + cell * h + (sc + 0.5 + jitter) * sub
+    return pos.astype(np.float32)
+
+
 SCENARIOS = {
     "vortex_ring": vortex_ring,
     "colliding_rings": colliding_rings,
