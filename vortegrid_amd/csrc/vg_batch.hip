@@ -105,21 +105,40 @@ __global__ __launch_bounds__(256) void k_bbox_batch(const float *__restrict__ po
     const size_t base = wave * 64 * kBoxItems;
     uint32_t cur = 0xffffffffu;
     uint32_t e[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
-    for (int it = 0; it < kBoxItems; ++it) {
-        const size_t i = base + (size_t)it * 64 + lane;
-        if (i >= n) break;
-        const uint32_t k = owner[i];
+    auto fold = [&](uint32_t k, float x, float y, float z) {
         if (k != cur) {
             if (cur != 0xffffffffu) box_flush(bbox, cur, e);
             cur = k;
             e[0] = e[1] = e[2] = 0xffffffffu;
             e[3] = e[4] = e[5] = 0u;
         }
+        const uint32_t v[3] = {ord_enc(x), ord_enc(y), ord_enc(z)};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const uint32_t v = ord_enc(pos[3 * i + a]);
-            e[a] = min(e[a], v);
-            e[3 + a] = max(e[3 + a], v);
+            e[a] = min(e[a], v[a]);
+            e[3 + a] = max(e[3 + a], v[a]);
+        }
+    };
+    if (base + 64 * kBoxItems <= n) {
+        // whole chunk in range: lane takes 4 consecutive items per group of
+        // 256, read as one uint4 of owners and three float4 of positions
+        // (the chunk starts at a multiple of 1024 items, so 16-byte aligned)
+#pragma unroll
+        for (int grp = 0; grp < kBoxItems / 4; ++grp) {
+            const size_t j = base + 256 * (size_t)grp + 4 * lane;
+            const uint4 k = *reinterpret_cast<const uint4 *>(owner + j);
+            const float4 *p4 = reinterpret_cast<const float4 *>(pos + 3 * j);
+            const float4 a = p4[0], b = p4[1], c = p4[2];
+            fold(k.x, a.x, a.y, a.z);
+            fold(k.y, a.w, b.x, b.y);
+            fold(k.z, b.z, b.w, c.x);
+            fold(k.w, c.y, c.z, c.w);
+        }
+    } else {
+        for (int it = 0; it < kBoxItems; ++it) {
+            const size_t i = base + (size_t)it * 64 + lane;
+            if (i >= n) break;
+            fold(owner[i], pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
         }
     }
     const uint32_t k0 = __builtin_amdgcn_readfirstlane(cur);
