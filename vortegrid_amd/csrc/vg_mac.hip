@@ -366,20 +366,40 @@ __device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc
                 S.M1[lane] = m1;
                 S.M2[lane] = m2;
             }
+            // every accepted cluster of the chunk has the current size (the
+            // usual case: one vorton size per tree)?
+            const bool one_size = __ballot(use && p.w != f.size) == 0ull;
             __builtin_amdgcn_wave_barrier();
-            while (mask) {
-                const int j = __builtin_ctzll(mask);
-                mask &= mask - 1;
-                const float4 cp = S.P[j];
-                if (cp.w != f.size) set_size(f, cp.w, spread, part, scaled);
-                const float4 cw = S.W[j];
-                float4 a0 = cp, a1 = cp, a2 = cp;
-                if (ORDER) {
-                    a0 = S.M0[j];
-                    a1 = S.M1[j];
-                    a2 = S.M2[j];
+            if (one_size) {
+                // no size test: the five LDS reads of a cluster go out
+                // together, one wait per cluster
+                while (mask) {
+                    const int j = __builtin_ctzll(mask);
+                    mask &= mask - 1;
+                    const float4 cp = S.P[j], cw = S.W[j];
+                    float4 a0 = cp, a1 = cp, a2 = cp;
+                    if (ORDER) {
+                        a0 = S.M0[j];
+                        a1 = S.M1[j];
+                        a2 = S.M2[j];
+                    }
+                    cluster_term<POLICY, ORDER>(part, q, cp, cw, a0, a1, a2, f);
                 }
-                cluster_term<POLICY, ORDER>(part, q, cp, cw, a0, a1, a2, f);
+            } else {
+                while (mask) {
+                    const int j = __builtin_ctzll(mask);
+                    mask &= mask - 1;
+                    const float4 cp = S.P[j];
+                    if (cp.w != f.size) set_size(f, cp.w, spread, part, scaled);
+                    const float4 cw = S.W[j];
+                    float4 a0 = cp, a1 = cp, a2 = cp;
+                    if (ORDER) {
+                        a0 = S.M0[j];
+                        a1 = S.M1[j];
+                        a2 = S.M2[j];
+                    }
+                    cluster_term<POLICY, ORDER>(part, q, cp, cw, a0, a1, a2, f);
+                }
             }
             __builtin_amdgcn_wave_barrier();
         }
@@ -444,15 +464,23 @@ __device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc
                     p = tv.sorted_p[idx];
                     w = tv.sorted_w[idx];
                 }
+                const bool one_size = __ballot(t < total && p.w != f.size) == 0ull;
                 __builtin_amdgcn_wave_barrier();
                 S.P[lane] = p;
                 S.W[lane] = w;
                 __builtin_amdgcn_wave_barrier();
                 const uint32_t cnt = total - vbase < 64u ? total - vbase : 64u;
-                for (uint32_t j = 0; j < cnt; ++j) {
-                    const float4 vp = S.P[j];
-                    if (vp.w != f.size) set_size(f, vp.w, spread, part, scaled);
-                    cluster_term<POLICY, 0>(part, q, vp, S.W[j], vp, vp, vp, f);
+                if (one_size) {
+                    for (uint32_t j = 0; j < cnt; ++j) {
+                        const float4 vp = S.P[j], vw = S.W[j];
+                        cluster_term<POLICY, 0>(part, q, vp, vw, vp, vp, vp, f);
+                    }
+                } else {
+                    for (uint32_t j = 0; j < cnt; ++j) {
+                        const float4 vp = S.P[j];
+                        if (vp.w != f.size) set_size(f, vp.w, spread, part, scaled);
+                        cluster_term<POLICY, 0>(part, q, vp, S.W[j], vp, vp, vp, f);
+                    }
                 }
                 __builtin_amdgcn_wave_barrier();
             }
