@@ -123,18 +123,23 @@ __device__ __forceinline__ void cluster_term(float3 &part, float3 q, float4 p, f
     }
     const bool inside = d2 < f.r2;
     const float lm = inside ? f.ir3 : law;
-    part.x += (w.y * dz - w.z * dy) * lm;
-    part.y += (w.z * dx - w.x * dz) * lm;
-    part.z += (w.x * dy - w.y * dx) * lm;
     if (ORDER >= 1) {
+        // (w x d) lm - (a - 3 rs2 (t x d)) ld with t = D d folds into
+        // ((w + c3 t) x d) lm - a ld, c3 = 3 rs2 outside the core, 0 inside
+        // (where ld = 0): one cross product instead of two
         const float tx = m0.x * dx + m0.y * dy + m0.z * dz;
         const float ty = m1.x * dx + m1.y * dy + m1.z * dz;
         const float tz = m2.x * dx + m2.y * dy + m2.z * dz;
-        const float c3 = 3.0f * rs2;
+        const float c3 = inside ? 0.0f : 3.0f * rs2;
         const float ld = inside ? 0.0f : law;
-        part.x -= (m0.w - c3 * (ty * dz - tz * dy)) * ld;
-        part.y -= (m1.w - c3 * (tz * dx - tx * dz)) * ld;
-        part.z -= (m2.w - c3 * (tx * dy - ty * dx)) * ld;
+        const float vx = w.x + c3 * tx, vy = w.y + c3 * ty, vz = w.z + c3 * tz;
+        part.x += (vy * dz - vz * dy) * lm - m0.w * ld;
+        part.y += (vz * dx - vx * dz) * lm - m1.w * ld;
+        part.z += (vx * dy - vy * dx) * lm - m2.w * ld;
+    } else {
+        part.x += (w.y * dz - w.z * dy) * lm;
+        part.y += (w.z * dx - w.x * dz) * lm;
+        part.z += (w.x * dy - w.y * dx) * lm;
     }
 }
 
