@@ -107,11 +107,16 @@ __device__ __forceinline__ void set_size(SizeFactors &f, float size, float sprea
 }
 
 // Monopole (+ first-moment) contribution of one cluster, unscaled by s.
-template <int POLICY, int ORDER>
+// FAR: the caller knows the cluster lies outside the core radius (the law
+// inside the core never applies); d2 is kept away from 0 so that a cluster
+// of zero vorticity (whose centroid defaults to the origin) adds 0 even at
+// a query point on the origin.
+template <int POLICY, int ORDER, bool FAR = false>
 __device__ __forceinline__ void cluster_term(float3 &part, float3 q, float4 p, float4 w, float4 m0,
                                              float4 m1, float4 m2, const SizeFactors &f) {
     const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
-    const float d2 = dx * dx + dy * dy + dz * dz;
+    float d2 = dx * dx + dy * dy + dz * dz;
+    if (FAR) d2 = fmaxf(d2, 1e-24f);
     float law, rs2;
     if (POLICY == VG_RSQRT_REFERENCE) {
         rs2 = __builtin_amdgcn_rcpf(d2);
@@ -121,7 +126,7 @@ __device__ __forceinline__ void cluster_term(float3 &part, float3 q, float4 p, f
         rs2 = rs * rs;
         law = rs2 * rs;
     }
-    const bool inside = d2 < f.r2;
+    const bool inside = !FAR && d2 < f.r2;
     const float lm = inside ? f.ir3 : law;
     if (ORDER >= 1) {
         // (w x d) lm - (a - 3 rs2 (t x d)) ld with t = D d folds into
@@ -310,6 +315,16 @@ __device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc
         const float4 *__restrict__ P = tv.P[l];
         const float4 *__restrict__ W = tv.W[l];
         const float4 *__restrict__ M = tv.M[l];
+        // A cluster of this layer outside the near box lies at least k of
+        // its cells from every point of the brick (its centroid is inside
+        // its cell; the 0.99 absorbs the grid points' inward nudge): core
+        // radii below that never apply.
+        float far2;
+        {
+            const float h = fminf(gl.cell[0], fminf(gl.cell[1], gl.cell[2]));
+            const float dmin = 0.99f * (float)mp.k * h;
+            far2 = dmin * dmin;
+        }
         const int bx = c1[0] - c0[0] + 1, by = c1[1] - c0[1] + 1, bz = c1[2] - c0[2] + 1;
         const int total = bx * by * bz;
         if (expanding && l >= mp.expand_from) {
@@ -375,9 +390,22 @@ __device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc
             // usual case: one vorton size per tree)?
             const bool one_size = __ballot(use && p.w != f.size) == 0ull;
             __builtin_amdgcn_wave_barrier();
-            if (one_size) {
-                // no size test: the five LDS reads of a cluster go out
-                // together, one wait per cluster
+            if (one_size && f.r2 < far2) {
+                // no size test (the five LDS reads of a cluster go out
+                // together, one wait per cluster), and no core test
+                while (mask) {
+                    const int j = __builtin_ctzll(mask);
+                    mask &= mask - 1;
+                    const float4 cp = S.P[j], cw = S.W[j];
+                    float4 a0 = cp, a1 = cp, a2 = cp;
+                    if (ORDER) {
+                        a0 = S.M0[j];
+                        a1 = S.M1[j];
+                        a2 = S.M2[j];
+                    }
+                    cluster_term<POLICY, ORDER, true>(part, q, cp, cw, a0, a1, a2, f);
+                }
+            } else if (one_size) {
                 while (mask) {
                     const int j = __builtin_ctzll(mask);
                     mask &= mask - 1;
