@@ -136,11 +136,18 @@ __device__ __forceinline__ void cluster_term(float3 &part, float3 q, float4 p, f
         const float ty = m1.x * dx + m1.y * dy + m1.z * dz;
         const float tz = m2.x * dx + m2.y * dy + m2.z * dz;
         const float c3 = inside ? 0.0f : 3.0f * rs2;
-        const float ld = inside ? 0.0f : law;
         const float vx = w.x + c3 * tx, vy = w.y + c3 * ty, vz = w.z + c3 * tz;
-        part.x += (vy * dz - vz * dy) * lm - m0.w * ld;
-        part.y += (vz * dx - vx * dz) * lm - m1.w * ld;
-        part.z += (vx * dy - vy * dx) * lm - m2.w * ld;
+        if (FAR) {
+            // lm = ld = law: ((v x d) - a) law, three fused operations a row
+            part.x = fmaf(fmaf(vy, dz, -fmaf(vz, dy, m0.w)), law, part.x);
+            part.y = fmaf(fmaf(vz, dx, -fmaf(vx, dz, m1.w)), law, part.y);
+            part.z = fmaf(fmaf(vx, dy, -fmaf(vy, dx, m2.w)), law, part.z);
+        } else {
+            const float ld = inside ? 0.0f : law;
+            part.x += (vy * dz - vz * dy) * lm - m0.w * ld;
+            part.y += (vz * dx - vx * dz) * lm - m1.w * ld;
+            part.z += (vx * dy - vy * dx) * lm - m2.w * ld;
+        }
     } else {
         part.x += (w.y * dz - w.z * dy) * lm;
         part.y += (w.z * dx - w.x * dz) * lm;
