@@ -457,6 +457,34 @@ def test_mac_local_expansions(vg, oracle, rings2, k, order, expand_from):
         assert e > 0 and c < c0
 
 
+@pytest.mark.parametrize("k,expand_from", [(2, -1), (2, 2), (3, 2)])
+def test_mac_velocity_grid_mixed_sizes(vg, oracle, rings2, k, expand_from):
+    """Vortons of two sizes: clusters take the size of their last non-empty
+    child, so the walk meets size changes inside chunks (the general loop)
+    as well as chunks of one size (the loop without the size test)."""
+    pos, om, radius, og = rings2
+    case(f"mac velocity grid, two vorton sizes, k={k} from={expand_from}")
+    rng = np.random.default_rng(61)
+    size = np.where(rng.random(len(pos)) < 0.5, 2 * radius, 3 * radius).astype(F32)
+    dp, dw = upload(vg, pos), upload(vg, om)
+    ds = upload(vg, np.repeat(size[:, None], 3, axis=1))
+    v = vg.Vortons()
+    v.pos, v.angvel, v.size = dp.ptr.value, dw.ptr.value, ds.ptr.value
+    v.stride, v.n, v.uniform_size = 3, len(pos), 0.0
+    g = vg.GridDesc.from_fields(og)
+    t = vg.Tree()
+    t.build(g, v)
+    spread, circ = vg.auto_mollify(g, radius)
+    out = vg.DeviceArray.zeros((g.capacity, 3))
+    t.velocity_grid_mac(g, out, k, 1, spread, circ, 1, expand_from=expand_from)
+    vg.synchronize()
+    ot = oracle.InfluenceTree(og, pos, om, size)
+    want = oracle.mac_velocity_grid_ex(ot, og, pos, om, size, k, 1, expand_from, spread, circ, 1,
+                                       threads=8)
+    assert np.isfinite(out.numpy()).all()
+    assert rel_l2(out.numpy(), want) < 1e-5
+
+
 def test_mac_work_and_accuracy(vg, oracle):
     pos, om, radius = scenarios.random_rings(32768, seed=33, rings=16)
     case("mac work counts + accuracy vs fp64 N=32768")
