@@ -809,7 +809,7 @@ def suite(args):
     t = vg.Tree()
     v = vg.vortons_view(dp, dw, n, uniform_size=2 * radius)
     t.build(g, v)
-    pse_labels = ("vg_pse_gather", "vg_pse_phase", "vg_pse_scatter")
+    pse_labels = ("vg_pse_gather", "vg_pse_schedule", "vg_pse_phase", "vg_pse_scatter")
     t.pse_diffuse(dw, 0.01, dt)
     vg.synchronize()
     vg.profile_reset()
@@ -835,6 +835,7 @@ def suite(args):
                     "reference threaded-build exchange order reproduced bit for bit",
         "grid": list(g.shape), "vortons": n,
         "ms": sum(pse.values()), "phases_ms": pse["vg_pse_phase"],
+        "schedule_ms": pse["vg_pse_schedule"],
         "gather_scatter_ms": pse["vg_pse_gather"] + pse["vg_pse_scatter"],
         "simultaneous_ms": sum(pse_sim.values()),
         "simultaneous_kernel_ms": pse_sim["vg_pse_simultaneous"],

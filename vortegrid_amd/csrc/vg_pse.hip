@@ -21,7 +21,9 @@
 // cells of a step run in parallel and the result equals the serial
 // traversal bit for bit.  Slices of one parity touch disjoint vortons
 // (z, z+1), so each gets a workgroup; a barrier separates steps and a
-// launch separates the phases.  The vorton state lives in the tree's
+// launch separates the phases.  A step lasts as long as its busiest wave, so
+// k_pse_schedule first deals the non-empty cells of every step to the waves,
+// largest first, each to the least loaded wave.  The vorton state lives in the tree's
 // cell-sorted order (positions and sizes from the tree, angular velocities
 // gathered from the caller and scattered back), so a cell's vortons are
 // contiguous.  exp is vg_pse_exp, the same function as the oracle's.
@@ -308,18 +310,155 @@ __device__ __forceinline__ void pse_cell_wave(const PseArgs &a, uint32_t x, uint
 // takes whole cells of the current wavefront step.
 constexpr int kPseWaves = kPseThreads / 64;
 
-__global__ __launch_bounds__(kPseThreads) void k_pse_phase(PseArgs a, uint32_t phase) {
+// Cells of wavefront step t of a slice: x + 2y = t, y from
+// max(0, ceil((t - cx + 1) / 2)) to min(cy - 1, t / 2).
+__device__ __forceinline__ void step_rows(int t, int cx, int cy, int &ylo, int &yhi) {
+    ylo = t - (cx - 1) > 0 ? (t - (cx - 1) + 1) / 2 : 0;
+    yhi = t / 2 < cy - 1 ? t / 2 : cy - 1;
+}
+
+// Work of a cell in steps of pse_cell_wave: the partner stream of each chunk
+// of 64 here-vortons in the systolic form, k n in the small-cell form; 0 for
+// an empty cell (nothing to do), at least 1 otherwise.
+__device__ __forceinline__ uint32_t cell_cost(const PseArgs &a, uint32_t c) {
+    const uint32_t nx = a.nx, nxy = a.nx * a.ny;
+    const uint32_t k = a.count[c];
+    if (k == 0) return 0u;
+    const uint32_t offs[8] = {0, 1, nx, 1 + nx, nxy, 1 + nxy, nx + nxy, 1 + nx + nxy};
+    uint32_t n = 0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) n += a.count[c + offs[r]];
+    uint64_t cost = 0;
+    if (k < kSystolicMin) {
+        cost = (uint64_t)k * n;
+    } else {
+        for (uint32_t b = 0; b < k; b += 64) cost += (n - b) + (k - b < 64u ? k - b : 64u) - 1;
+    }
+    // a non-empty cell must never look empty: at least 1, saturated
+    return cost == 0 ? 1u : (cost > 0xffffffffull ? 0xffffffffu : (uint32_t)cost);
+}
+
+// Cells of one wavefront step share no vortons, so any assignment of them to
+// the waves gives the same result; the step lasts as long as the busiest
+// wave.  One wave per (step t, slice z) assigns the step's non-empty cells,
+// largest first, each to the wave with the least work so far (greedy LPT),
+// and writes them grouped by wave: perm[sched * M + i] for i in
+// [wstart[9 sched + w], wstart[9 sched + w + 1]), sched = z steps + t.
+// Up to kSchedSlots * 64 cells per step.
+constexpr int kSchedSlots = 4;
+
+__global__ __launch_bounds__(64) void k_pse_schedule(PseArgs a, uint32_t steps, uint32_t M,
+                                                     uint32_t *__restrict__ perm,
+                                                     uint32_t *__restrict__ wstart) {
+    const int t = (int)blockIdx.x;
+    const uint32_t z = blockIdx.y;
+    const int lane = (int)threadIdx.x;
+    const int cx = (int)a.nx - 1, cy = (int)a.ny - 1;
+    int ylo, yhi;
+    step_rows(t, cx, cy, ylo, yhi);
+    const uint32_t nx = a.nx, nxy = a.nx * a.ny;
+    uint32_t rem[kSchedSlots];  // cost still to assign (0 once assigned or empty)
+    int slot_w[kSchedSlots], slot_pos[kSchedSlots];
+#pragma unroll
+    for (int q = 0; q < kSchedSlots; ++q) {
+        const int y = ylo + lane + 64 * q;
+        rem[q] = y <= yhi ? cell_cost(a, (uint32_t)(t - 2 * y) + nx * (uint32_t)y + nxy * z) : 0u;
+        slot_w[q] = -1;
+        slot_pos[q] = 0;
+    }
+    uint32_t load[kPseWaves], cnt[kPseWaves];
+#pragma unroll
+    for (int w = 0; w < kPseWaves; ++w) load[w] = cnt[w] = 0u;
+    for (;;) {
+        uint32_t best = rem[0];
+#pragma unroll
+        for (int q = 1; q < kSchedSlots; ++q) best = rem[q] > best ? rem[q] : best;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint32_t o = (uint32_t)__shfl_xor((int)best, off, 64);
+            best = o > best ? o : best;
+        }
+        best = __builtin_amdgcn_readfirstlane(best);
+        if (best == 0u) break;
+        // the first (slot, lane) holding it, and the least loaded wave
+        int sel_q = 0, sel_lane = 0;
+#pragma unroll
+        for (int q = kSchedSlots - 1; q >= 0; --q) {
+            const unsigned long long m = __ballot(rem[q] == best);
+            if (m) {
+                sel_q = q;
+                sel_lane = (int)__builtin_ctzll(m);
+            }
+        }
+        int w = 0;
+#pragma unroll
+        for (int v = 1; v < kPseWaves; ++v) w = load[v] < load[w] ? v : w;
+        uint32_t pos = 0;
+#pragma unroll
+        for (int v = 0; v < kPseWaves; ++v) {
+            if (v == w) {
+                pos = cnt[v];
+                cnt[v] += 1u;
+                load[v] += best;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kSchedSlots; ++q) {
+            if (q == sel_q && lane == sel_lane) {
+                slot_w[q] = w;
+                slot_pos[q] = (int)pos;
+                rem[q] = 0u;
+            }
+        }
+    }
+    uint32_t ws[kPseWaves + 1];
+    ws[0] = 0;
+#pragma unroll
+    for (int v = 0; v < kPseWaves; ++v) ws[v + 1] = ws[v] + cnt[v];
+    const size_t sched = (size_t)z * steps + (size_t)t;
+    if (lane <= kPseWaves) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int u = 0; u <= kPseWaves; ++u) v = lane == u ? ws[u] : v;
+        wstart[(kPseWaves + 1) * sched + (size_t)lane] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < kSchedSlots; ++q) {
+        if (slot_w[q] >= 0) {
+            uint32_t base = 0;
+#pragma unroll
+            for (int u = 0; u < kPseWaves; ++u) base = slot_w[q] == u ? ws[u] : base;
+            perm[sched * M + base + (uint32_t)slot_pos[q]] = (uint32_t)(ylo + lane + 64 * q);
+        }
+    }
+}
+
+// perm / wstart: the schedule of k_pse_schedule, or null for round robin by
+// row (steps of more than kSchedSlots * 64 cells).
+__global__ __launch_bounds__(kPseThreads) void k_pse_phase(PseArgs a, uint32_t phase,
+                                                           const uint32_t *__restrict__ perm,
+                                                           const uint32_t *__restrict__ wstart,
+                                                           uint32_t M) {
     const uint32_t z = phase + 2 * blockIdx.x;
     if (z + 1 >= a.nz) return;  // uniform per workgroup
     const int wave = (int)(threadIdx.x >> 6);
     const int cx = (int)a.nx - 1, cy = (int)a.ny - 1;  // cells per axis
     const int steps = (cx - 1) + 2 * (cy - 1) + 1;
     for (int t = 0; t < steps; ++t) {
-        // cells with x + 2y = t: y from max(0, ceil((t - cx + 1) / 2)) to min(cy - 1, t / 2)
-        const int ylo = t - (cx - 1) > 0 ? (t - (cx - 1) + 1) / 2 : 0;
-        const int yhi = t / 2 < cy - 1 ? t / 2 : cy - 1;
-        for (int y = ylo + wave; y <= yhi; y += kPseWaves)
-            pse_cell_wave(a, (uint32_t)(t - 2 * y), (uint32_t)y, z);
+        int ylo, yhi;
+        step_rows(t, cx, cy, ylo, yhi);
+        if (perm) {
+            const size_t sched = (size_t)z * (size_t)steps + (size_t)t;
+            const uint32_t b = wstart[(kPseWaves + 1) * sched + (size_t)wave];
+            const uint32_t e = wstart[(kPseWaves + 1) * sched + (size_t)wave + 1];
+            for (uint32_t i = b; i < e; ++i) {
+                const int y = (int)perm[sched * M + i];
+                pse_cell_wave(a, (uint32_t)(t - 2 * y), (uint32_t)y, z);
+            }
+        } else {
+            for (int y = ylo + wave; y <= yhi; y += kPseWaves)
+                pse_cell_wave(a, (uint32_t)(t - 2 * y), (uint32_t)y, z);
+        }
         __syncthreads();
     }
 }
@@ -472,12 +611,28 @@ vg_status pse_diffuse(const vg_tree *t, float *angvel, size_t ang_stride, const 
         result = w + n;
     } else {
         const uint32_t slices = a.nz - 1;  // cell slices 0 .. nz-2
+        const uint32_t cx = a.nx - 1, cy = a.ny - 1;
+        const uint32_t steps = (cx - 1) + 2 * (cy - 1) + 1;
+        // cells per wavefront step are at most min(cy, (cx + 1) / 2)
+        const uint32_t M = cy < (cx + 1) / 2 ? cy : (cx + 1) / 2;
+        uint32_t *perm = nullptr, *wstart = nullptr;
+        const size_t nsched = (size_t)slices * steps;
+        if (st == VG_OK && M <= 64u * kSchedSlots && nsched * M < (size_t(1) << 28)) {
+            st = salloc((void **)&perm, nsched * M * sizeof(uint32_t), s);
+            if (st == VG_OK)
+                st = salloc((void **)&wstart, nsched * (kPseWaves + 1) * sizeof(uint32_t), s);
+            if (st == VG_OK)
+                st = launch("vg_pse_schedule", k_pse_schedule, dim3(steps, slices), dim3(64), 0, s,
+                            a, steps, M, perm, wstart);
+        }
         for (uint32_t phase = 0; phase < 2 && st == VG_OK; ++phase) {
             const uint32_t groups = (slices - phase + 1) / 2;
             if (groups)
                 st = launch("vg_pse_phase", k_pse_phase, dim3(groups), dim3(kPseThreads), 0, s,
-                            a, phase);
+                            a, phase, (const uint32_t *)perm, (const uint32_t *)wstart, M);
         }
+        if (perm) sfree(perm, s);
+        if (wstart) sfree(wstart, s);
     }
     if (st == VG_OK)
         st = launch("vg_pse_scatter", k_pse_scatter, dim3(ceil_div(n, 256)), dim3(256), 0, s,
