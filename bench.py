@@ -809,7 +809,8 @@ def suite(args):
     t = vg.Tree()
     v = vg.vortons_view(dp, dw, n, uniform_size=2 * radius)
     t.build(g, v)
-    pse_labels = ("vg_pse_gather", "vg_pse_schedule", "vg_pse_phase", "vg_pse_scatter")
+    pse_labels = ("vg_pse_gather", "vg_pse_frac_size", "vg_scan_tile", "vg_scan_add",
+                  "vg_pse_fractions", "vg_pse_schedule", "vg_pse_phase", "vg_pse_scatter")
     t.pse_diffuse(dw, 0.01, dt)
     vg.synchronize()
     vg.profile_reset()
@@ -836,6 +837,8 @@ def suite(args):
         "grid": list(g.shape), "vortons": n,
         "ms": sum(pse.values()), "phases_ms": pse["vg_pse_phase"],
         "schedule_ms": pse["vg_pse_schedule"],
+        "fractions_ms": pse["vg_pse_frac_size"] + pse["vg_scan_tile"] + pse["vg_scan_add"]
+        + pse["vg_pse_fractions"],
         "gather_scatter_ms": pse["vg_pse_gather"] + pse["vg_pse_scatter"],
         "simultaneous_ms": sum(pse_sim.values()),
         "simultaneous_kernel_ms": pse_sim["vg_pse_simultaneous"],

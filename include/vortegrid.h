@@ -339,7 +339,9 @@ vg_status vg_tree_partition(const vg_tree *t, const uint32_t **cell_start,
  * nu dt <= 1 (the reference's assertion).
  * order VG_PSE_REFERENCE_ORDER reproduces the reference.  The reference's
  * in-place exchanges are a long serial chain per z slice, run here as
- * wavefronts of cells: correct but latency-bound.  VG_PSE_SIMULTANEOUS (no
+ * wavefronts of cells: correct but latency-bound.  This order reads back the
+ * size of its transient fraction buffer, so the call synchronises stream s
+ * once (it cannot be recorded into a graph).  VG_PSE_SIMULTANEOUS (no
  * reference counterpart) takes every exchange from the starting values, one
  * thread per vorton.  Its gap from the in-place order is second order in
  * the total exchange fraction a vorton sees, sum_j f_j, which grows with

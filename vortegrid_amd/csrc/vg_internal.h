@@ -126,6 +126,9 @@ struct BatchTreeArgs {
     float4 *P[kMaxLayers] = {}, *W[kMaxLayers] = {}, *M[kMaxLayers] = {};
 };
 size_t scan_scratch_elems(size_t n);
+// Exclusive scan in -> out using tmp (scan_scratch_elems(n) elements).
+vg_status exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp,
+                             hipStream_t s);
 vg_status batch_tree_build(const BatchTreeArgs &a, hipStream_t s);
 
 // ------------------------------------------------------------------ device helpers

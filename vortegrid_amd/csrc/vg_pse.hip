@@ -50,14 +50,19 @@ __device__ __forceinline__ float pse_exp(float x) {
     return p * __int_as_float(((int)k + 127) << 23);
 }
 
-__device__ __forceinline__ void exchange(const float4 ph, float r2, float4 &wh, const float4 pt,
-                                         float4 *__restrict__ wt, float nu_dt2) {
+// Exchange fraction of the pair (here h, partner t): depends on positions
+// and the here-vorton's size only.
+__device__ __forceinline__ float pse_fraction(const float4 ph, float r2, const float4 pt,
+                                              float nu_dt2) {
 #pragma clang fp contract(off)
     const float sx = ph.x - pt.x, sy = ph.y - pt.y, sz = ph.z - pt.z;
     const float d2 = sx * sx + sy * sy + sz * sz;
-    float f = nu_dt2 * pse_exp(-d2 / r2);
-    f = f < -0.5f ? -0.5f : (f > 0.5f ? 0.5f : f);
-    float4 w = *wt;
+    const float f = nu_dt2 * pse_exp(-d2 / r2);
+    return f < -0.5f ? -0.5f : (f > 0.5f ? 0.5f : f);
+}
+
+__device__ __forceinline__ void exchange_f(float4 &wh, float4 &w, float f) {
+#pragma clang fp contract(off)
     const float ex = (wh.x - w.x) * f, ey = (wh.y - w.y) * f, ez = (wh.z - w.z) * f;
     wh.x -= ex;
     wh.y -= ey;
@@ -65,6 +70,12 @@ __device__ __forceinline__ void exchange(const float4 ph, float r2, float4 &wh, 
     w.x += ex;
     w.y += ey;
     w.z += ez;
+}
+
+__device__ __forceinline__ void exchange(const float4 ph, float r2, float4 &wh, const float4 pt,
+                                         float4 *__restrict__ wt, float nu_dt2) {
+    float4 w = *wt;
+    exchange_f(wh, w, pse_fraction(ph, r2, pt, nu_dt2));
     *wt = w;
 }
 
@@ -74,6 +85,11 @@ struct PseArgs {
     const float4 *p;                // sorted (x, y, z, size)
     float4 *w;                      // sorted (Omega, alive)
     float nu_dt2, dissipation;
+    // Exchange fractions of the systolic cells, precomputed (or null): the
+    // cell with leaf point c holds frac[frac_off[c] + i n + j] for here-vorton
+    // i < k and partner j < n (0 where j <= i).
+    const float *frac;
+    const uint32_t *frac_off;
 };
 
 __device__ __forceinline__ void pse_cell(const PseArgs &a, uint32_t x, uint32_t y, uint32_t z) {
@@ -210,6 +226,72 @@ __device__ __forceinline__ void pse_cell_systolic(const PseArgs &a, const uint32
     }
 }
 
+// The systolic cell with its fractions read from a.frac: only Omega streams
+// down the lanes, and each lane's fractions are loaded kFracAhead steps
+// before use (a lane walks its row of the fraction block left to right).
+constexpr int kFracAhead = 8;
+
+__device__ __forceinline__ void pse_cell_systolic_frac(const PseArgs &a, uint32_t c,
+                                                       const uint32_t base[8],
+                                                       const uint32_t pre[9], uint32_t k,
+                                                       uint32_t n) {
+#pragma clang fp contract(off)
+    const int lane = (int)(threadIdx.x & 63);
+    const float *__restrict__ fc = a.frac + a.frac_off[c];
+    for (uint32_t B = 0; B < k; B += 64) {
+        const uint32_t L = k - B < 64u ? k - B : 64u;
+        const uint32_t i = B + (uint32_t)lane;
+        float4 wh = make_float4(0.f, 0.f, 0.f, 0.f), sw = wh, bw = wh;
+        bool live = false;
+        const uint32_t steps = (n - B) + L - 1;
+        // fraction of this lane at step t (0 unless it exchanges then)
+        auto fetch = [&](uint32_t t) -> float {
+            const int jl = (int)t - lane;
+            const uint32_t j = B + (uint32_t)jl;
+            return ((uint32_t)lane < L && jl > lane && j < n) ? fc[(size_t)i * n + j] : 0.f;
+        };
+        float cur[kFracAhead], nxt[kFracAhead];
+#pragma unroll
+        for (int d = 0; d < kFracAhead; ++d) cur[d] = fetch((uint32_t)d);
+        for (uint32_t t0 = 0; t0 < steps; t0 += kFracAhead) {
+#pragma unroll
+            for (int d = 0; d < kFracAhead; ++d) nxt[d] = fetch(t0 + kFracAhead + (uint32_t)d);
+#pragma unroll
+            for (int d = 0; d < kFracAhead; ++d) {
+                const uint32_t t = t0 + (uint32_t)d;
+                if (t >= steps) break;
+                if ((t & 63u) == 0) {  // the next 64 partners to enter, one per lane
+                    const uint32_t jj = B + t + (uint32_t)lane;
+                    if (jj < n) bw = a.w[partner_index(base, pre, jj)];
+                }
+                sw = wave_shr4(sw, lane4(bw, (int)(t & 63u)));
+                const int jl = (int)t - lane;  // partner j - B at this lane
+                const uint32_t j = B + (uint32_t)jl;
+                if ((uint32_t)lane < L && jl >= 0 && j < n) {
+                    if (j == i) {
+                        wh = sw;
+                        live = sw.w != 0.f;
+                    } else if (j > i && live) {
+                        exchange_f(wh, sw, cur[d]);
+                    }
+                    if (j == n - 1) {  // the lane's last partner: done
+                        if (live) {
+                            wh.x -= wh.x * a.dissipation;
+                            wh.y -= wh.y * a.dissipation;
+                            wh.z -= wh.z * a.dissipation;
+                        }
+                        a.w[partner_index(base, pre, i)] = wh;
+                    }
+                    if ((uint32_t)lane == L - 1 && j >= B + L) a.w[partner_index(base, pre, j)] = sw;
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < kFracAhead; ++d) cur[d] = nxt[d];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    }
+}
+
 __device__ __forceinline__ void pse_cell_wave(const PseArgs &a, uint32_t x, uint32_t y, uint32_t z) {
 #pragma clang fp contract(off)
     const int lane = (int)(threadIdx.x & 63);
@@ -226,7 +308,10 @@ __device__ __forceinline__ void pse_cell_wave(const PseArgs &a, uint32_t x, uint
     const uint32_t k = pre[1], n = pre[8];
     if (k == 0) return;
     if (k >= kSystolicMin) {
-        pse_cell_systolic(a, base, pre, k, n);
+        if (a.frac)
+            pse_cell_systolic_frac(a, c, base, pre, k, n);
+        else
+            pse_cell_systolic(a, base, pre, k, n);
         return;
     }
     if (n > 64u * kSlots) {
@@ -309,6 +394,71 @@ __device__ __forceinline__ void pse_cell_wave(const PseArgs &a, uint32_t x, uint
 // One workgroup per slice z = phase + 2 blockIdx.x (z < nz - 1); each wave
 // takes whole cells of the current wavefront step.
 constexpr int kPseWaves = kPseThreads / 64;
+
+__global__ void k_pse_zero64(unsigned long long *v) { *v = 0ull; }
+
+// Fraction block size of each leaf point's cell (k n for systolic cells,
+// else 0), and the 64-bit total.
+__global__ __launch_bounds__(256) void k_pse_frac_size(PseArgs a, uint32_t cells,
+                                                       uint32_t *__restrict__ size,
+                                                       unsigned long long *__restrict__ total) {
+    __shared__ unsigned long long part[256];
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long kn = 0;
+    if (c < cells) {
+        const uint32_t k = a.count[c];
+        if (k >= kSystolicMin) {
+            const uint32_t nx = a.nx, nxy = a.nx * a.ny;
+            const uint32_t offs[8] = {0, 1, nx, 1 + nx, nxy, 1 + nxy, nx + nxy, 1 + nx + nxy};
+            uint32_t n = 0;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) n += a.count[c + offs[r]];
+            kn = (unsigned long long)k * n;
+        }
+        size[c] = kn < 0x80000000ull ? (uint32_t)kn : 0x80000000u;
+    }
+    part[threadIdx.x] = kn;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && part[0]) atomicAdd(total, part[0]);
+}
+
+// Exchange fractions of the systolic cells, one wave per leaf point's cell:
+// the same operations as the in-chain exchange, so the same bits.
+__global__ __launch_bounds__(256) void k_pse_fractions(PseArgs a, uint32_t cells,
+                                                       float *__restrict__ frac) {
+#pragma clang fp contract(off)
+    const uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (c >= cells) return;
+    const uint32_t k = a.count[c];
+    if (k < kSystolicMin) return;
+    const uint32_t nx = a.nx, nxy = a.nx * a.ny;
+    const uint32_t offs[8] = {0, 1, nx, 1 + nx, nxy, 1 + nxy, nx + nxy, 1 + nx + nxy};
+    uint32_t base[8], pre[9];
+    pre[0] = 0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        base[r] = a.start[c + offs[r]];
+        pre[r + 1] = pre[r] + a.count[c + offs[r]];
+    }
+    const uint32_t n = pre[8];
+    float *__restrict__ fc = frac + a.frac_off[c];
+    const uint32_t total = k * n;
+    for (uint32_t q = lane; q < total; q += 64) {
+        const uint32_t i = q / n, j = q - i * n;
+        float f = 0.f;
+        if (j > i) {
+            const float4 ph = a.p[base[0] + i];
+            const float range = 4.0f * ph.w;
+            f = pse_fraction(ph, range * range, a.p[partner_index(base, pre, j)], a.nu_dt2);
+        }
+        fc[q] = f;
+    }
+}
 
 // Cells of wavefront step t of a slice: x + 2y = t, y from
 // max(0, ceil((t - cx + 1) / 2)) to min(cy - 1, t / 2).
@@ -600,6 +750,8 @@ vg_status pse_diffuse(const vg_tree *t, float *angvel, size_t ang_stride, const 
     a.w = w;
     a.nu_dt2 = 2.0f * viscosity * dt;   // 2.0f * mViscosity * timeStep
     a.dissipation = viscosity * dt;     // mViscosity * timeStep
+    a.frac = nullptr;
+    a.frac_off = nullptr;
     vg_status st = launch("vg_pse_gather", k_pse_gather, dim3(ceil_div(n, 256)), dim3(256), 0, s,
                           tv.sorted_idx, (const float *)angvel, ang_stride, birth, birth_stride,
                           n, w);
@@ -617,6 +769,41 @@ vg_status pse_diffuse(const vg_tree *t, float *angvel, size_t ang_stride, const 
         const uint32_t M = cy < (cx + 1) / 2 ? cy : (cx + 1) / 2;
         uint32_t *perm = nullptr, *wstart = nullptr;
         const size_t nsched = (size_t)slices * steps;
+        // exchange fractions of the systolic cells, computed over the whole
+        // chip before the serial phases (the block sizes are read back to
+        // size the buffer: one stream synchronisation)
+        const size_t cells = capacity(g);
+        uint32_t *fsize = nullptr, *foff = nullptr, *stmp = nullptr;
+        unsigned long long *dtot = nullptr;
+        float *frac = nullptr;
+        if (st == VG_OK) st = salloc((void **)&fsize, cells * sizeof(uint32_t), s);
+        if (st == VG_OK) st = salloc((void **)&foff, cells * sizeof(uint32_t), s);
+        if (st == VG_OK) st = salloc((void **)&stmp, scan_scratch_elems(cells) * sizeof(uint32_t), s);
+        if (st == VG_OK) st = salloc((void **)&dtot, sizeof(unsigned long long), s);
+        if (st == VG_OK) st = launch("vg_pse_frac_size", k_pse_zero64, dim3(1), dim3(1), 0, s, dtot);
+        if (st == VG_OK)
+            st = launch("vg_pse_frac_size", k_pse_frac_size, dim3(ceil_div(cells, 256)), dim3(256),
+                        0, s, a, (uint32_t)cells, fsize, dtot);
+        if (st == VG_OK) st = exclusive_scan_u32(fsize, foff, cells, stmp, s);
+        unsigned long long total = 0;
+        if (st == VG_OK) {
+            hipError_t e = hipMemcpyAsync(&total, dtot, sizeof total, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) {
+                set_error(std::string("PSE fraction sizes: ") + hipGetErrorString(e));
+                st = VG_EDEVICE;
+            }
+        }
+        // blocks of 2^31 floats or more keep the in-chain fractions
+        if (st == VG_OK && total > 0 && total < (1ull << 31)) {
+            st = salloc((void **)&frac, total * sizeof(float), s);
+            if (st == VG_OK) {
+                a.frac = frac;
+                a.frac_off = foff;
+                st = launch("vg_pse_fractions", k_pse_fractions, dim3(ceil_div(cells, 4)),
+                            dim3(256), 0, s, a, (uint32_t)cells, frac);
+            }
+        }
         if (st == VG_OK && M <= 64u * kSchedSlots && nsched * M < (size_t(1) << 28)) {
             st = salloc((void **)&perm, nsched * M * sizeof(uint32_t), s);
             if (st == VG_OK)
@@ -633,6 +820,11 @@ vg_status pse_diffuse(const vg_tree *t, float *angvel, size_t ang_stride, const 
         }
         if (perm) sfree(perm, s);
         if (wstart) sfree(wstart, s);
+        if (frac) sfree(frac, s);
+        if (fsize) sfree(fsize, s);
+        if (foff) sfree(foff, s);
+        if (stmp) sfree(stmp, s);
+        if (dtot) sfree(dtot, s);
     }
     if (st == VG_OK)
         st = launch("vg_pse_scatter", k_pse_scatter, dim3(ceil_div(n, 256)), dim3(256), 0, s,

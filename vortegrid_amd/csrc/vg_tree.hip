@@ -653,6 +653,11 @@ __global__ void k_gather_batch(const float *__restrict__ pos, const float *__res
 
 size_t scan_scratch_elems(size_t n) { return scan_elems(n); }
 
+vg_status exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp,
+                             hipStream_t s) {
+    return exclusive_scan(in, out, n, tmp, s);
+}
+
 vg_status batch_tree_build(const BatchTreeArgs &a, hipStream_t s) {
     const Nested &ng = *a.shape;
     const size_t cap0 = capacity(ng.layer[0]);
