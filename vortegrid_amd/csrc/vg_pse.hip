@@ -226,9 +226,28 @@ __device__ __forceinline__ void pse_cell_systolic(const PseArgs &a, const uint32
     }
 }
 
-// The systolic cell with its fractions read from a.frac: only Omega streams
-// down the lanes, and each lane's fractions are loaded kFracAhead steps
-// before use (a lane walks its row of the fraction block left to right).
+// Lane l takes lane l + 1 (wrapping): the whole-wave DPP rotation
+// (wave_rol:1), so the window's next partner is always in lane 0.
+__device__ __forceinline__ float wave_rol(float v) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x134, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float4 wave_rol4(float4 v) {
+    return make_float4(wave_rol(v.x), wave_rol(v.y), wave_rol(v.z), wave_rol(v.w));
+}
+__device__ __forceinline__ uint32_t wave_rol_u(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x134, 0xf, 0xf, false);
+}
+__device__ __forceinline__ uint32_t wave_shr_u(uint32_t v, uint32_t in) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)in, (int)v, 0x138, 0xf, 0xf, false);
+}
+
+// The systolic cell with its fractions read from a.frac: only Omega (and
+// its vorton's index, for the write-back of partners leaving the last lane)
+// streams down the lanes.  The window of 64 loaded partners rotates by one
+// lane per step, so lane 0 always holds the next partner to enter; each
+// lane's fractions are loaded kFracAhead steps before use (a lane walks its
+// row of the fraction block left to right).
 constexpr int kFracAhead = 8;
 
 __device__ __forceinline__ void pse_cell_systolic_frac(const PseArgs &a, uint32_t c,
@@ -241,7 +260,9 @@ __device__ __forceinline__ void pse_cell_systolic_frac(const PseArgs &a, uint32_
     for (uint32_t B = 0; B < k; B += 64) {
         const uint32_t L = k - B < 64u ? k - B : 64u;
         const uint32_t i = B + (uint32_t)lane;
+        const uint32_t gi = base[0] + i;  // here-vortons are the cell's own
         float4 wh = make_float4(0.f, 0.f, 0.f, 0.f), sw = wh, bw = wh;
+        uint32_t sg = 0, bg = 0;
         bool live = false;
         const uint32_t steps = (n - B) + L - 1;
         // fraction of this lane at step t (0 unless it exchanges then)
@@ -262,16 +283,22 @@ __device__ __forceinline__ void pse_cell_systolic_frac(const PseArgs &a, uint32_
                 if (t >= steps) break;
                 if ((t & 63u) == 0) {  // the next 64 partners to enter, one per lane
                     const uint32_t jj = B + t + (uint32_t)lane;
-                    if (jj < n) bw = a.w[partner_index(base, pre, jj)];
+                    if (jj < n) {
+                        bg = partner_index(base, pre, jj);
+                        bw = a.w[bg];
+                    }
                 }
-                sw = wave_shr4(sw, lane4(bw, (int)(t & 63u)));
+                sw = wave_shr4(sw, bw);  // lane 0 takes the window's lane 0
+                sg = wave_shr_u(sg, bg);
+                bw = wave_rol4(bw);
+                bg = wave_rol_u(bg);
                 const int jl = (int)t - lane;  // partner j - B at this lane
                 const uint32_t j = B + (uint32_t)jl;
                 if ((uint32_t)lane < L && jl >= 0 && j < n) {
-                    if (j == i) {
+                    if (jl == lane) {  // j == i: the here-vorton itself
                         wh = sw;
                         live = sw.w != 0.f;
-                    } else if (j > i && live) {
+                    } else if (jl > lane && live) {
                         exchange_f(wh, sw, cur[d]);
                     }
                     if (j == n - 1) {  // the lane's last partner: done
@@ -280,9 +307,11 @@ __device__ __forceinline__ void pse_cell_systolic_frac(const PseArgs &a, uint32_
                             wh.y -= wh.y * a.dissipation;
                             wh.z -= wh.z * a.dissipation;
                         }
-                        a.w[partner_index(base, pre, i)] = wh;
+                        a.w[gi] = wh;
                     }
-                    if ((uint32_t)lane == L - 1 && j >= B + L) a.w[partner_index(base, pre, j)] = sw;
+                    // a partner that is not a here-vorton of this chunk leaves
+                    // the last lane with its value for the next chunk
+                    if ((uint32_t)lane == L - 1 && j >= B + L) a.w[sg] = sw;
                 }
             }
 #pragma unroll
