@@ -79,6 +79,23 @@ def test_pse_dense_cells(vg, oracle, dense):
     assert np.array_equal(dw.numpy(), want)
 
 
+def test_pse_in_chain_fractions(vg, oracle, monkeypatch):
+    """VG_PSE_FRACTIONS=0 keeps the exchange fractions in the serial chain
+    (the path for fraction blocks of 2^31 floats or more): the same bits."""
+    pos, om, radius = scenarios.vortex_ring(2048, seed=58)
+    rng = np.random.default_rng(58)
+    blob = (pos[0] + rng.uniform(-0.01, 0.01, (120, 3))).astype(F32)
+    pos = np.concatenate([pos, blob]).astype(F32)
+    om = np.concatenate([om, rng.normal(size=(120, 3)).astype(F32)])
+    case("PSE with in-chain fractions")
+    og, g, t, (dp, dw) = build_tree(vg, oracle, pos, om, radius)
+    monkeypatch.setenv("VG_PSE_FRACTIONS", "0")
+    t.pse_diffuse(dw, 0.5, 0.2)
+    vg.synchronize()
+    want = oracle.pse_diffuse(og, pos, om, 2 * radius, 0.5, 0.2)
+    assert np.array_equal(dw.numpy(), want)
+
+
 @pytest.mark.parametrize("viscosity,dt", [(0.01, 1.0 / 30.0), (0.5, 0.2)])
 def test_pse_simultaneous(vg, oracle, viscosity, dt):
     """VG_PSE_SIMULTANEOUS: bit-exact against the oracle's simultaneous form;

@@ -27,6 +27,8 @@
 // cell-sorted order (positions and sizes from the tree, angular velocities
 // gathered from the caller and scattered back), so a cell's vortons are
 // contiguous.  exp is vg_pse_exp, the same function as the oracle's.
+#include <cstdlib>
+
 #include "vg_internal.h"
 
 namespace vg {
@@ -823,8 +825,12 @@ vg_status pse_diffuse(const vg_tree *t, float *angvel, size_t ang_stride, const 
                 st = VG_EDEVICE;
             }
         }
-        // blocks of 2^31 floats or more keep the in-chain fractions
-        if (st == VG_OK && total > 0 && total < (1ull << 31)) {
+        // blocks of 2^31 floats or more keep the in-chain fractions, and so
+        // does VG_PSE_FRACTIONS=0 in the environment (the same bits either
+        // way; the switch keeps the in-chain path testable)
+        const char *env = getenv("VG_PSE_FRACTIONS");
+        const bool precompute = !(env && env[0] == '0');
+        if (st == VG_OK && precompute && total > 0 && total < (1ull << 31)) {
             st = salloc((void **)&frac, total * sizeof(float), s);
             if (st == VG_OK) {
                 a.frac = frac;
