@@ -273,29 +273,6 @@ def test_vcycle_manufactured_solution(vg, oracle):
     assert np.abs(da.numpy() - exact).max() < 1e-4
 
 
-@pytest.mark.parametrize("n", [262144, 2 * 1024 * 1024])
-def test_vcycle_bottom_solve_same_bits(vg, oracle, monkeypatch, n):
-    """The one-workgroup bottom of the V-cycle (levels that fit in LDS) gives
-    the per-level launches' bits: same solution, residual and cycle count."""
-    case(f"v-cycle bottom solve vs per-level launches, {n} vortons' grid")
-    og = oracle.define_shape(n, [0, 0, 0], [1, 1, 2])
-    rng = np.random.default_rng(71)
-    rhs = rng.normal(size=(og.capacity, 3)).astype(F32)
-    start = rng.normal(size=(og.capacity, 3)).astype(F32)
-    g = vg.GridDesc.from_fields(og)
-    out = []
-    for flag in ("0", "1"):
-        monkeypatch.setenv("VG_MG_BOTTOM", flag)
-        da = upload(vg, start)
-        mg = vg.Multigrid()
-        rel, cycles = mg.vcycle_solve(g, da, upload(vg, rhs), tol=1e-6, max_cycles=40)
-        vg.synchronize()
-        out.append((da.numpy(), rel, cycles))
-    assert out[0][2] == out[1][2] and out[0][1] == out[1][1]
-    assert np.array_equal(out[0][0], out[1][0])
-    assert out[1][1] <= 1e-6
-
-
 def test_advection_and_bbox_bit_exact(vg, oracle, rings2):
     pos, om, radius, og = rings2
     case("tracer advection + bbox")

@@ -13,7 +13,6 @@
 //    field alone cannot reach a 1e-6 relative residual on large grids, since
 //    its rounding noise amplified by the Laplacian sits near 1e-4.
 #include <cmath>
-#include <cstdlib>
 #include <vector>
 
 #include "vg_internal.h"
@@ -44,7 +43,6 @@ struct vg_multigrid {
     float *lo = nullptr;       // compensation term of the finest solution
     size_t lo_cap = 0;
     double *part = nullptr;    // FP64 norm partials
-    uint32_t bottom = 0xffffffffu;  // first level of the one-workgroup bottom solve
 };
 
 namespace vg {
@@ -107,29 +105,37 @@ Lap lap_coef(const vg_grid_desc &g) {
     return c;
 }
 
-// Per-point operations of the V-cycle, shared by the per-level kernels and
-// the one-workgroup bottom solve (k_vcycle_bottom) so that both give the
-// same bits: contraction off, fixed operation order.  Arrays are 3 floats
-// per point, o the point's offset.
-
-// Red-black Gauss-Seidel update of interior point o.
-__device__ __forceinline__ void rbgs_point(float *u, const float *f, uint32_t o, uint32_t nx,
-                                           uint32_t nxy, const Lap &c) {
+// Red-black point Gauss-Seidel: colour = (ix + iy + iz) & 1.  Threads map to
+// the colour's points of one x row: ix = 1 + ((iy + iz + color + 1) & 1) + 2k.
+__global__ void k_rbgs(vg_grid_desc g, float *__restrict__ u, const float *__restrict__ f,
+                       int color, Lap c) {
 #pragma clang fp contract(off)
+    const uint32_t iy = 1 + blockIdx.y, iz = 1 + blockIdx.z;
+    const uint32_t first = 1 + ((iy + iz + 1 + (uint32_t)color) & 1u);
+    const uint32_t ix = first + 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+    const size_t nx = g.np[0], nxy = (size_t)g.np[0] * g.np[1];
+    if (ix + 1 >= nx) return;
+    const size_t o = ix + (size_t)iy * nx + (size_t)iz * nxy;
+    const float inv = c.inv;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const float s = (u[3 * (o + 1) + k] + u[3 * (o - 1) + k]) * c.r2x +
                         (u[3 * (o + nx) + k] + u[3 * (o - nx) + k]) * c.r2y +
                         (u[3 * (o + nxy) + k] + u[3 * (o - nxy) + k]) * c.r2z;
-        u[3 * o + k] = (s - f[3 * o + k]) * c.inv;
+        u[3 * o + k] = (s - f[3 * o + k]) * inv;
     }
 }
 
-// r = f - L u at interior point o.
-__device__ __forceinline__ void residual_point(const float *u, const float *f, float *r,
-                                               uint32_t o, uint32_t nx, uint32_t nxy,
-                                               const Lap &c) {
-#pragma clang fp contract(off)
+// r = f - L u on interior points, 0 on the boundary.
+__global__ void k_residual32(vg_grid_desc g, const float *__restrict__ u,
+                             const float *__restrict__ f, float *__restrict__ r, Lap c) {
+    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= capacity(g)) return;
+    const size_t nx = g.np[0], nxy = nx * g.np[1];
+    if (on_boundary(split((uint32_t)o, g.np[0], g.np[1]), g)) {
+        r[3 * o] = r[3 * o + 1] = r[3 * o + 2] = 0.f;
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const float uc = u[3 * o + k];
@@ -140,153 +146,34 @@ __device__ __forceinline__ void residual_point(const float *u, const float *f, f
     }
 }
 
-// Full weighting of rhi onto coarse interior point p (weights 8/4/2/1 by
-// Manhattan distance, /64).
-__device__ __forceinline__ void restrict_point(const Pt &p, const vg_grid_desc &hi,
-                                               const float *rhi, float out[3]) {
-#pragma clang fp contract(off)
-    const uint32_t hx = hi.np[0], hxy = hi.np[0] * hi.np[1];
-    const uint32_t c = 2 * p.x + 2 * p.y * hx + 2 * p.z * hxy;
-    float acc[3] = {0.f, 0.f, 0.f};
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int md = abs(dx) + abs(dy) + abs(dz);
-                const float w = (float)(8 >> md);
-                const uint32_t h = c + dx + dy * (int)hx + dz * (int)hxy;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) acc[k] += w * rhi[3 * h + k];
-            }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) out[k] = acc[k] * (1.0f / 64.0f);
-}
-
-// u_hi at interior point p (offset o) += trilinear(u_lo).
-__device__ __forceinline__ void prolong_point(const Pt &p, uint32_t o, float *uhi,
-                                              const vg_grid_desc &lo, const float *ulo) {
-#pragma clang fp contract(off)
-    const uint32_t lx = lo.np[0], lxy = lo.np[0] * lo.np[1];
-    const uint32_t x0 = p.x / 2, y0 = p.y / 2, z0 = p.z / 2;
-    const uint32_t xs[2] = {x0, x0 + (p.x & 1)}, ys[2] = {y0, y0 + (p.y & 1)},
-                   zs[2] = {z0, z0 + (p.z & 1)};
-    const float wx = (p.x & 1) ? 0.5f : 1.0f, wy = (p.y & 1) ? 0.5f : 1.0f,
-                wz = (p.z & 1) ? 0.5f : 1.0f;
-    const int nxs = (p.x & 1) ? 2 : 1, nys = (p.y & 1) ? 2 : 1, nzs = (p.z & 1) ? 2 : 1;
-    float acc[3] = {0.f, 0.f, 0.f};
-    for (int a = 0; a < nzs; ++a)
-        for (int b = 0; b < nys; ++b)
-            for (int d = 0; d < nxs; ++d) {
-                const uint32_t l = xs[d] + ys[b] * lx + zs[a] * lxy;
-                const float w = wx * wy * wz;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) acc[k] += w * ulo[3 * l + k];
-            }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) uhi[3 * o + k] += acc[k];
-}
-
-// FP64 red-black update of interior point o of the coarsest level.
-struct Lap64 {
-    double r2x, r2y, r2z, inv;
-};
-__device__ __forceinline__ Lap64 lap64(const vg_grid_desc &g) {
-    Lap64 c;
-    const double rx = 1.0 / g.cell[0], ry = 1.0 / g.cell[1], rz = 1.0 / g.cell[2];
-    c.r2x = rx * rx;
-    c.r2y = ry * ry;
-    c.r2z = rz * rz;
-    c.inv = 1.0 / (2.0 * (c.r2x + c.r2y + c.r2z));
-    return c;
-}
-__device__ __forceinline__ void coarse_point(double *w, const float *f, uint32_t o, uint32_t nx,
-                                             uint32_t nxy, const Lap64 &c) {
-#pragma clang fp contract(off)
-    for (int k = 0; k < 3; ++k) {
-        const double s = (w[3 * (o + 1) + k] + w[3 * (o - 1) + k]) * c.r2x +
-                         (w[3 * (o + nx) + k] + w[3 * (o - nx) + k]) * c.r2y +
-                         (w[3 * (o + nxy) + k] + w[3 * (o - nxy) + k]) * c.r2z;
-        w[3 * o + k] = (s - (double)f[3 * o + k]) * c.inv;
-    }
-}
-
-// `sweeps` FP64 red-black sweeps of the coarsest level on w (in LDS or
-// memory).  Up to 64 points the first wave does them alone, one point per
-// lane, ordered by wave barriers instead of workgroup barriers (a wave's LDS
-// operations complete in order); the caller synchronises the workgroup
-// afterwards.
-__device__ __forceinline__ void coarse_sweeps_f64(const vg_grid_desc &g, double *w,
-                                                  const float *f, int sweeps) {
-    const uint32_t nx = g.np[0], nxy = nx * g.np[1], n = nxy * g.np[2];
-    const Lap64 c = lap64(g);
-    if (n <= 64) {
-        if (threadIdx.x >= 64) return;
-        const uint32_t o = threadIdx.x;
-        bool inner = false;
-        uint32_t parity = 0;
-        if (o < n) {
-            const Pt p = split(o, g.np[0], g.np[1]);
-            inner = !on_boundary(p, g);
-            parity = (p.x + p.y + p.z) & 1u;
-        }
-        for (int it = 0; it < sweeps; ++it) {
-            for (uint32_t color = 0; color < 2; ++color) {
-                if (inner && parity == color) coarse_point(w, f, o, nx, nxy, c);
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            }
-        }
-        return;
-    }
-    for (int it = 0; it < sweeps; ++it) {
-        for (uint32_t color = 0; color < 2; ++color) {
-            for (uint32_t o = threadIdx.x; o < n; o += blockDim.x) {
-                const Pt p = split(o, g.np[0], g.np[1]);
-                if (on_boundary(p, g) || ((p.x + p.y + p.z) & 1u) != color) continue;
-                coarse_point(w, f, o, nx, nxy, c);
-            }
-            __syncthreads();
-        }
-    }
-}
-
-// Red-black point Gauss-Seidel: colour = (ix + iy + iz) & 1.  Threads map to
-// the colour's points of one x row: ix = 1 + ((iy + iz + color + 1) & 1) + 2k.
-__global__ void k_rbgs(vg_grid_desc g, float *__restrict__ u, const float *__restrict__ f,
-                       int color, Lap c) {
-    const uint32_t iy = 1 + blockIdx.y, iz = 1 + blockIdx.z;
-    const uint32_t first = 1 + ((iy + iz + 1 + (uint32_t)color) & 1u);
-    const uint32_t ix = first + 2 * (blockIdx.x * blockDim.x + threadIdx.x);
-    const uint32_t nx = g.np[0], nxy = g.np[0] * g.np[1];
-    if (ix + 1 >= nx) return;
-    rbgs_point(u, f, ix + iy * nx + iz * nxy, nx, nxy, c);
-}
-
-// r = f - L u on interior points, 0 on the boundary.
-__global__ void k_residual32(vg_grid_desc g, const float *__restrict__ u,
-                             const float *__restrict__ f, float *__restrict__ r, Lap c) {
-    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (o >= capacity(g)) return;
-    if (on_boundary(split((uint32_t)o, g.np[0], g.np[1]), g)) {
-        r[3 * o] = r[3 * o + 1] = r[3 * o + 2] = 0.f;
-        return;
-    }
-    residual_point(u, f, r, (uint32_t)o, g.np[0], g.np[0] * g.np[1], c);
-}
-
-// Full weighting onto coarse interior points; coarse boundary = 0
-// (homogeneous Dirichlet correction).  Also clears the coarse correction ulo
-// (the coarse level's initial guess).
+// Full weighting (weights 8/4/2/1 by Manhattan distance, /64) onto coarse
+// interior points; coarse boundary = 0 (homogeneous Dirichlet correction).
+// Also clears the coarse correction ulo (the coarse level's initial guess).
 __global__ void k_restrict_fw(vg_grid_desc lo, float *__restrict__ flo, float *__restrict__ ulo,
                               vg_grid_desc hi, const float *__restrict__ rhi) {
     const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= capacity(lo)) return;
     ulo[3 * o] = ulo[3 * o + 1] = ulo[3 * o + 2] = 0.f;
     const Pt p = split((uint32_t)o, lo.np[0], lo.np[1]);
-    float v[3] = {0.f, 0.f, 0.f};
-    if (!on_boundary(p, lo)) restrict_point(p, hi, rhi, v);
+    const size_t ix = p.x, iy = p.y, iz = p.z;
+    if (on_boundary(p, lo)) {
+        flo[3 * o] = flo[3 * o + 1] = flo[3 * o + 2] = 0.f;
+        return;
+    }
+    const size_t hx = hi.np[0], hxy = (size_t)hi.np[0] * hi.np[1];
+    const size_t c = 2 * ix + 2 * iy * hx + 2 * iz * hxy;
+    float acc[3] = {0.f, 0.f, 0.f};
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int md = abs(dx) + abs(dy) + abs(dz);
+                const float w = (float)(8 >> md);
+                const size_t h = c + dx + (long long)dy * (long long)hx + (long long)dz * (long long)hxy;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) flo[3 * o + k] = v[k];
+                for (int k = 0; k < 3; ++k) acc[k] += w * rhi[3 * h + k];
+            }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) flo[3 * o + k] = acc[k] * (1.0f / 64.0f);
 }
 
 // u_hi(interior) += trilinear(u_lo).
@@ -296,7 +183,24 @@ __global__ void k_prolong_add(vg_grid_desc hi, float *__restrict__ uhi, vg_grid_
     if (o >= capacity(hi)) return;
     const Pt p = split((uint32_t)o, hi.np[0], hi.np[1]);
     if (on_boundary(p, hi)) return;
-    prolong_point(p, (uint32_t)o, uhi, lo, ulo);
+    const size_t ix = p.x, iy = p.y, iz = p.z;
+    const size_t lx = lo.np[0], lxy = (size_t)lo.np[0] * lo.np[1];
+    const size_t x0 = ix / 2, y0 = iy / 2, z0 = iz / 2;
+    const size_t x1 = x0 + (ix & 1), y1 = y0 + (iy & 1), z1 = z0 + (iz & 1);
+    const float wx = (ix & 1) ? 0.5f : 1.0f, wy = (iy & 1) ? 0.5f : 1.0f, wz = (iz & 1) ? 0.5f : 1.0f;
+    float acc[3] = {0.f, 0.f, 0.f};
+    const size_t xs[2] = {x0, x1}, ys[2] = {y0, y1}, zs[2] = {z0, z1};
+    const int nxs = (ix & 1) ? 2 : 1, nys = (iy & 1) ? 2 : 1, nzs = (iz & 1) ? 2 : 1;
+    for (int a = 0; a < nzs; ++a)
+        for (int b = 0; b < nys; ++b)
+            for (int d = 0; d < nxs; ++d) {
+                const size_t l = xs[d] + ys[b] * lx + zs[a] * lxy;
+                const float w = wx * wy * wz;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) acc[k] += w * ulo[3 * l + k];
+            }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) uhi[3 * o + k] += acc[k];
 }
 
 // FP64 coarsest solve: one workgroup, red-black Gauss-Seidel in double, on
@@ -309,119 +213,56 @@ __global__ __launch_bounds__(256) void k_coarse_f64(vg_grid_desc g, float *__res
                                                     double *__restrict__ work) {
     __shared__ double lds[IN_LDS ? 3 * kCoarseLdsPoints : 1];
     double *w = IN_LDS ? lds : work;
-    const uint32_t n = g.np[0] * g.np[1] * g.np[2];
+    const uint32_t nx = g.np[0], nxy = nx * g.np[1], n = nxy * g.np[2];
+    const double rx = 1.0 / g.cell[0], ry = 1.0 / g.cell[1], rz = 1.0 / g.cell[2];
+    const double r2x = rx * rx, r2y = ry * ry, r2z = rz * rz;
+    const double inv = 1.0 / (2.0 * (r2x + r2y + r2z));
     for (uint32_t i = threadIdx.x; i < 3 * n; i += blockDim.x) w[i] = u[i];
     __syncthreads();
-    coarse_sweeps_f64(g, w, f, sweeps);
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < 3 * n; i += blockDim.x) u[i] = (float)w[i];
-}
-
-// The bottom of a V-cycle in one workgroup with every array in LDS: from
-// level 0 of `b` (u and f read from memory, u written back) down to the
-// coarsest, the same steps as the per-level launches (2 red-black sweeps,
-// residual, full weighting, FP64 coarsest solve, trilinear correction, 2
-// sweeps) with the same per-point arithmetic, so the same bits, without a
-// launch per step.
-constexpr int kBottomLevels = 4;
-constexpr uint32_t kBottomLdsFloats = 16384;  // 64 KB
-constexpr int kBottomThreads = 1024;
-
-struct Bottom {
-    uint32_t levels;  // 2 .. kBottomLevels; the last is the coarsest
-    vg_grid_desc g[kBottomLevels];
-    Lap c[kBottomLevels];
-    float *u0;
-    const float *f0;
-    int coarse_sweeps;
-};
-
-// LDS floats of the bottom solve: u, f, r per level, plus the coarsest
-// level's double copy.
-inline size_t bottom_floats(const vg_grid_desc *g, uint32_t levels) {
-    size_t n = 0;
-    for (uint32_t i = 0; i < levels; ++i) n += 9 * capacity(g[i]);
-    return n + 6 * capacity(g[levels - 1]);
-}
-
-__device__ __forceinline__ void bottom_smooth(const vg_grid_desc &g, const Lap &c, float *u,
-                                              const float *f, int sweeps) {
-    const uint32_t nx = g.np[0], nxy = g.np[0] * g.np[1], n = nxy * g.np[2];
-    for (int it = 0; it < sweeps; ++it) {
-        for (uint32_t color = 0; color < 2; ++color) {
-            for (uint32_t o = threadIdx.x; o < n; o += blockDim.x) {
+    auto update = [&](uint32_t o) {
+        for (int k = 0; k < 3; ++k) {
+            const double s = (w[3 * (o + 1) + k] + w[3 * (o - 1) + k]) * r2x +
+                             (w[3 * (o + nx) + k] + w[3 * (o - nx) + k]) * r2y +
+                             (w[3 * (o + nxy) + k] + w[3 * (o - nxy) + k]) * r2z;
+            w[3 * o + k] = (s - (double)f[3 * o + k]) * inv;
+        }
+    };
+    if (n <= 64) {
+        // one point per lane of the first wave, colours ordered by wave
+        // barriers (a wave's LDS operations complete in order) instead of
+        // workgroup barriers
+        if (threadIdx.x < 64) {
+            const uint32_t o = threadIdx.x;
+            bool inner = false;
+            uint32_t parity = 0;
+            if (o < n) {
                 const Pt p = split(o, g.np[0], g.np[1]);
-                if (on_boundary(p, g) || ((p.x + p.y + p.z) & 1u) != color) continue;
-                rbgs_point(u, f, o, nx, nxy, c);
+                inner = !on_boundary(p, g);
+                parity = (p.x + p.y + p.z) & 1u;
             }
-            __syncthreads();
-        }
-    }
-}
-
-__global__ __launch_bounds__(kBottomThreads) void k_vcycle_bottom(Bottom b) {
-    __shared__ float lds[kBottomLdsFloats];
-    float *u[kBottomLevels], *f[kBottomLevels], *r[kBottomLevels];
-    float *p = lds;
-    for (uint32_t i = 0; i < b.levels; ++i) {
-        const uint32_t n3 = 3 * b.g[i].np[0] * b.g[i].np[1] * b.g[i].np[2];
-        u[i] = p;
-        f[i] = p + n3;
-        r[i] = p + 2 * n3;
-        p += 3 * n3;
-    }
-    double *w = (double *)(((uintptr_t)p + 7) & ~(uintptr_t)7);
-    const uint32_t n0 = 3 * b.g[0].np[0] * b.g[0].np[1] * b.g[0].np[2];
-    for (uint32_t i = threadIdx.x; i < n0; i += blockDim.x) {
-        u[0][i] = b.u0[i];
-        f[0][i] = b.f0[i];
-    }
-    __syncthreads();
-    const uint32_t top = b.levels - 1;  // the coarsest
-    for (uint32_t l = 0; l < top; ++l) {
-        const vg_grid_desc &g = b.g[l], &gc = b.g[l + 1];
-        const uint32_t nx = g.np[0], nxy = g.np[0] * g.np[1], n = nxy * g.np[2];
-        bottom_smooth(g, b.c[l], u[l], f[l], 2);
-        for (uint32_t o = threadIdx.x; o < n; o += blockDim.x) {
-            if (on_boundary(split(o, g.np[0], g.np[1]), g)) {
-                r[l][3 * o] = r[l][3 * o + 1] = r[l][3 * o + 2] = 0.f;
-            } else {
-                residual_point(u[l], f[l], r[l], o, nx, nxy, b.c[l]);
+            for (int it = 0; it < sweeps; ++it) {
+                for (uint32_t color = 0; color < 2; ++color) {
+                    if (inner && parity == color) update(o);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
             }
         }
         __syncthreads();
-        const uint32_t nc = gc.np[0] * gc.np[1] * gc.np[2];
-        for (uint32_t o = threadIdx.x; o < nc; o += blockDim.x) {
-            u[l + 1][3 * o] = u[l + 1][3 * o + 1] = u[l + 1][3 * o + 2] = 0.f;
-            const Pt q = split(o, gc.np[0], gc.np[1]);
-            float v[3] = {0.f, 0.f, 0.f};
-            if (!on_boundary(q, gc)) restrict_point(q, g, r[l], v);
-#pragma unroll
-            for (int k = 0; k < 3; ++k) f[l + 1][3 * o + k] = v[k];
+    } else {
+        for (int it = 0; it < sweeps; ++it) {
+            for (uint32_t color = 0; color < 2; ++color) {
+                for (uint32_t o = threadIdx.x; o < n; o += blockDim.x) {
+                    const Pt p = split(o, g.np[0], g.np[1]);
+                    if (on_boundary(p, g) || ((p.x + p.y + p.z) & 1u) != color) continue;
+                    update(o);
+                }
+                __syncthreads();
+            }
         }
-        __syncthreads();
     }
-    {  // coarsest: FP64 red-black on a double copy of u
-        const vg_grid_desc &g = b.g[top];
-        const uint32_t n = g.np[0] * g.np[1] * g.np[2];
-        for (uint32_t i = threadIdx.x; i < 3 * n; i += blockDim.x) w[i] = u[top][i];
-        __syncthreads();
-        coarse_sweeps_f64(g, w, f[top], b.coarse_sweeps);
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < 3 * n; i += blockDim.x) u[top][i] = (float)w[i];
-        __syncthreads();
-    }
-    for (int l = (int)top - 1; l >= 0; --l) {
-        const vg_grid_desc &g = b.g[l];
-        const uint32_t n = g.np[0] * g.np[1] * g.np[2];
-        for (uint32_t o = threadIdx.x; o < n; o += blockDim.x) {
-            const Pt q = split(o, g.np[0], g.np[1]);
-            if (!on_boundary(q, g)) prolong_point(q, o, u[l], b.g[l + 1], u[l + 1]);
-        }
-        __syncthreads();
-        bottom_smooth(g, b.c[l], u[l], f[l], 2);
-    }
-    for (uint32_t i = threadIdx.x; i < n0; i += blockDim.x) b.u0[i] = u[0][i];
+    for (uint32_t i = threadIdx.x; i < 3 * n; i += blockDim.x) u[i] = (float)w[i];
 }
 
 // FP64 residual of the compensated solution (hi + lo): writes the FP32
@@ -559,39 +400,23 @@ vg_status smooth(const vg_grid_desc &g, float *u, float *tmp, const float *f, in
     return VG_OK;
 }
 
-// Red-black sweeps of the coarsest level: red-black Gauss-Seidel shrinks
-// the slowest error mode by about 1 - (pi / (dmax - 1))^2 per sweep, so
-// dmax^2 + 16 sweeps leave below 1e-4 of it for every coarsest size the
-// hierarchy allows (dmax <= 33): more than one V-cycle needs.
-int coarse_sweeps(const vg_grid_desc &g) {
-    uint32_t dmax = g.np[0];
-    if (g.np[1] > dmax) dmax = g.np[1];
-    if (g.np[2] > dmax) dmax = g.np[2];
-    const int sweeps = (int)(dmax * dmax + 16);
-    return sweeps > 20000 ? 20000 : sweeps;
-}
-
 // One V-cycle on level l (u[l] holds the initial guess).
 vg_status vcycle(vg_multigrid *m, uint32_t l, hipStream_t s) {
     const vg_grid_desc &g = m->vg[l];
     const size_t cap = capacity(g);
-    if (l == m->bottom) {
-        Bottom b;
-        b.levels = m->vdepth - l;
-        for (uint32_t i = 0; i < b.levels; ++i) {
-            b.g[i] = m->vg[l + i];
-            b.c[i] = lap_coef(b.g[i]);
-        }
-        b.u0 = m->u[l];
-        b.f0 = m->f[l];
-        b.coarse_sweeps = coarse_sweeps(m->vg[m->vdepth - 1]);
-        return launch("vg_mg_bottom", k_vcycle_bottom, dim3(1), dim3(kBottomThreads), 0, s, b);
-    }
     if (l + 1 == m->vdepth) {
         const bool in_lds = cap <= kCoarseLdsPoints;
         double *work = nullptr;
         if (!in_lds) VG_TRY(salloc((void **)&work, 3 * cap * sizeof(double), s));
-        const int sweeps = coarse_sweeps(g);
+        uint32_t dmax = g.np[0];
+        if (g.np[1] > dmax) dmax = g.np[1];
+        if (g.np[2] > dmax) dmax = g.np[2];
+        // red-black Gauss-Seidel shrinks the slowest error mode by about
+        // 1 - (pi / (dmax - 1))^2 per sweep, so dmax^2 + 16 sweeps leave
+        // below 1e-4 of it for every coarsest size the hierarchy allows
+        // (dmax <= 33): more than one V-cycle needs
+        int sweeps = (int)(dmax * dmax + 16);
+        if (sweeps > 20000) sweeps = 20000;
         vg_status st =
             in_lds ? launch("vg_mg_coarse_f64", k_coarse_f64<true>, dim3(1), dim3(256), 0, s, g,
                             m->u[l], (const float *)m->f[l], sweeps, work)
@@ -710,23 +535,6 @@ VG_API vg_status vg_mg_vcycle_solve(vg_multigrid *m, const vg_grid_desc *g, floa
         ++d;
     }
     m->vdepth = d;
-    // the deepest levels that fit in LDS go to k_vcycle_bottom (one launch
-    // per cycle instead of one per step); VG_MG_BOTTOM=0 keeps the
-    // per-level launches (the same bits)
-    m->bottom = 0xffffffffu;
-    {
-        const char *env = getenv("VG_MG_BOTTOM");
-        if (!(env && env[0] == '0')) {
-            for (uint32_t l = 0; l + 1 < d; ++l) {
-                const uint32_t levels = d - l;
-                if (levels <= (uint32_t)kBottomLevels &&
-                    bottom_floats(&m->vg[l], levels) + 1 <= kBottomLdsFloats) {
-                    m->bottom = l;
-                    break;
-                }
-            }
-        }
-    }
     // The coarsest layer is solved by one workgroup; refuse hierarchies that
     // cannot coarsen far enough for that to stay cheap.
     VG_REQUIRE(capacity(m->vg[d - 1]) <= 32768,
