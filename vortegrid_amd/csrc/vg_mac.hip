@@ -12,10 +12,12 @@
 //   * sums the leaf near box directly over its vortons (cell-sorted lists).
 //
 // GPU mapping: one wave = one 4x4x4 brick of grid points.  Near boxes are
-// per brick, so every loop bound, branch and cluster address is uniform
-// across the wave: the tree walk runs in scalar registers and each cluster
-// is fetched once per wave (one cache line, broadcast), while the 64 lanes
-// evaluate it at their own points.
+// per brick, so every loop bound and branch is uniform across the wave: the
+// lanes fetch 64 candidate cells at a time (one each), stage the accepted
+// clusters in LDS, and every lane then reads each cluster back (a broadcast)
+// to evaluate it at its own point.  Chunks whose clusters all have the
+// current size skip the per-cluster size test, and clusters that lie beyond
+// the core radius by construction skip the core test.
 #include "vg_internal.h"
 
 namespace vg {

@@ -9,8 +9,9 @@
 //   bin     vorton -> leaf cell key (truncation, as OffsetOfPosition),
 //           slot = atomicAdd(count[key])
 //   scan    count -> cell start (two-level exclusive scan)
-//   scatter sorted_idx[start + slot] = i, then each cell's short index list
-//           is insertion-sorted so cells hold vortons in index order
+//   scatter start + slot = i into a cell-grouped list, then each vorton
+//           moves to its rank among its cell's indices (k_rank_cells), so
+//           cells hold vortons in index order
 //   leaf    per leaf point: sequential |w|-weighted sums in index order
 //           (bit-identical to MakeBaseVortonGrid's serial loop)
 //   levels  per parent point: AggregateClusters' 2x2x2 sums in z,y,x order
