@@ -22,11 +22,15 @@
 // traversal bit for bit.  Slices of one parity touch disjoint vortons
 // (z, z+1), so each gets a workgroup; a barrier separates steps and a
 // launch separates the phases.  A step lasts as long as its busiest wave, so
-// k_pse_schedule first deals the non-empty cells of every step to the waves,
-// largest first, each to the least loaded wave.  The vorton state lives in the tree's
-// cell-sorted order (positions and sizes from the tree, angular velocities
-// gathered from the caller and scattered back), so a cell's vortons are
-// contiguous.  exp is vg_pse_exp, the same function as the oracle's.
+// k_pse_schedule first deals the non-empty cells of every step to the
+// waves, largest first, each to the least loaded wave.  The exchange
+// fraction f of a pair depends on positions and sizes only, so
+// k_pse_fractions computes those of the cells of 4+ vortons over the whole
+// chip beforehand, and their serial chains only move Omega.  The vorton
+// state lives in the tree's cell-sorted order (positions and sizes from the
+// tree, angular velocities gathered from the caller and scattered back), so
+// a cell's vortons are contiguous.  exp is vg_pse_exp, the same function as
+// the oracle's.
 #include <cstdlib>
 
 #include "vg_internal.h"
