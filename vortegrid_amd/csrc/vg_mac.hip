@@ -361,28 +361,37 @@ __device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc
         }
         float3 part = make_float3(0.f, 0.f, 0.f), scaled = part;
         BoxIter it = box_begin(bx, by, lane);
-        for (int base = 0; base < total; base += 64, box_next(it)) {
-            // lanes fetch 64 cells of the children box
-            bool use = false;
-            float4 p = make_float4(0.f, 0.f, 0.f, 0.f), w = p, m0 = p, m1 = p, m2 = p;
-            if (it.z < bz) {
-                const int cx = c0[0] + it.x, cy = c0[1] + it.y, cz = c0[2] + it.z;
-                const bool near = cx >= n0[0] && cx <= n1[0] && cy >= n0[1] && cy <= n1[1] &&
-                                  cz >= n0[2] && cz <= n1[2];
-                if (!near) {
-                    const uint32_t off = cell_offset(gl, cx, cy, cz);
-                    p = P[off];
-                    use = p.w != 0.0f;
-                    if (use && !COUNT) {
-                        w = W[off];
-                        if (ORDER) {
-                            m0 = M[3 * (size_t)off];
-                            m1 = M[3 * (size_t)off + 1];
-                            m2 = M[3 * (size_t)off + 2];
-                        }
-                    }
+        // Lanes fetch 64 cells of the children box per chunk.  A cell's
+        // position (whose size says whether it is empty) is loaded one chunk
+        // ahead, after the current chunk's moment loads, so that its memory
+        // round trip overlaps the current chunk's work.
+        auto candidate = [&](const BoxIter &c, uint32_t &off) -> bool {
+            if (c.z >= bz) return false;
+            const int cx = c0[0] + c.x, cy = c0[1] + c.y, cz = c0[2] + c.z;
+            const bool near = cx >= n0[0] && cx <= n1[0] && cy >= n0[1] && cy <= n1[1] &&
+                              cz >= n0[2] && cz <= n1[2];
+            off = near ? 0u : cell_offset(gl, cx, cy, cz);
+            return !near;
+        };
+        uint32_t noff = 0;
+        bool ncand = candidate(it, noff);
+        float4 np = ncand ? P[noff] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int base = 0; base < total; base += 64) {
+            const uint32_t off = noff;
+            const float4 p = np;
+            const bool use = ncand && p.w != 0.0f;
+            float4 w = make_float4(0.f, 0.f, 0.f, 0.f), m0 = w, m1 = w, m2 = w;
+            if (use && !COUNT) {
+                w = W[off];
+                if (ORDER) {
+                    m0 = M[3 * (size_t)off];
+                    m1 = M[3 * (size_t)off + 1];
+                    m2 = M[3 * (size_t)off + 2];
                 }
             }
+            box_next(it);
+            ncand = base + 64 < total && candidate(it, noff);
+            np = ncand ? P[noff] : make_float4(0.f, 0.f, 0.f, 0.f);
             unsigned long long mask = __ballot(use);
             if (COUNT) {
                 n_clusters += __popcll(mask);
