@@ -46,6 +46,7 @@ struct vg_batch {
     float *radius = nullptr, *size = nullptr;  // per effect
     // trees
     float4 *P[vg::kMaxLayers] = {}, *W[vg::kMaxLayers] = {}, *M[vg::kMaxLayers] = {};
+    uint32_t *C[vg::kMaxLayers] = {};  // non-empty child masks (layers >= 1)
     uint32_t *cell_count = nullptr, *cell_start = nullptr, *keys = nullptr, *slots = nullptr,
              *sorted_idx = nullptr, *scan_tmp = nullptr;
     float4 *sorted_p = nullptr, *sorted_w = nullptr;
@@ -321,6 +322,7 @@ vg_status frame(vg_batch *b, float dt, hipStream_t s) {
         ta.P[l] = b->P[l];
         ta.W[l] = b->W[l];
         ta.M[l] = b->M[l];
+        ta.C[l] = b->C[l];
     }
     VG_TRY(batch_tree_build(ta, s));
     VG_TRY(mac_velocity_grid_batch(b->views, b->mollify, b->effects, b->shape.layer[0].np,
@@ -417,6 +419,7 @@ VG_API vg_status vg_batch_create(vg_batch **out, uint32_t effects, const uint32_
         VG_B(dalloc(b, &b->P[l], effects * b->cap[l]));
         VG_B(dalloc(b, &b->W[l], effects * b->cap[l]));
         VG_B(dalloc(b, &b->M[l], 3 * effects * b->cap[l]));
+        if (l >= 1) VG_B(dalloc(b, &b->C[l], effects * b->cap[l]));
     }
     const size_t cells = (size_t)effects * b->cap0;
     VG_B(dalloc(b, &b->cell_count, cells));
@@ -455,6 +458,7 @@ VG_API vg_status vg_batch_create(vg_batch **out, uint32_t effects, const uint32_
             v.P[l] = b->P[l] + e * b->cap[l];
             v.W[l] = b->W[l] + e * b->cap[l];
             v.M[l] = b->M[l] + 3 * e * b->cap[l];
+            v.C[l] = l >= 1 ? b->C[l] + e * b->cap[l] : nullptr;
         }
         v.cell_start = b->cell_start + e * b->cap0;
         v.cell_count = b->cell_count + e * b->cap0;

@@ -97,6 +97,8 @@ struct TreeView {
     const float4 *W[kMaxLayers];   // cluster (Omega_x, Omega_y, Omega_z, sum|w|)
     const float4 *M[kMaxLayers];   // 3 per cluster: rows of the first moment
                                    // D[b][e] = sum Omega_b (y - c)_e, .w = antisymmetric part
+    const uint32_t *C[kMaxLayers]; // layers >= 1, per cluster: bit cx + dx (cy + dy cz) set
+                                   // when that child (dx dy dz per parent) is not empty
     // Leaf cell lists over the layer-0 points: vortons sorted by cell.
     const uint32_t *cell_start;    // per layer-0 point
     const uint32_t *cell_count;
@@ -124,6 +126,7 @@ struct BatchTreeArgs {
     float4 *sorted_p = nullptr, *sorted_w = nullptr;                    // n
     uint32_t *scan_tmp = nullptr;  // scan_scratch_elems(trees * cap0)
     float4 *P[kMaxLayers] = {}, *W[kMaxLayers] = {}, *M[kMaxLayers] = {};
+    uint32_t *C[kMaxLayers] = {};  // child masks, layers >= 1
 };
 size_t scan_scratch_elems(size_t n);
 // Exclusive scan in -> out using tmp (scan_scratch_elems(n) elements).

@@ -46,7 +46,9 @@ __device__ __forceinline__ uint32_t layer_scale(const TreeView &tv, int l, int a
 // Per-wave LDS staging area: 64 fetched clusters (or vortons) at a time.
 struct WaveStage {
     float4 P[64], W[64], M0[64], M1[64], M2[64];
-    uint32_t row_off[64], row_start[64];
+    // leaf row tables (offsets in [0, 64), starts in [64, 128)), or the
+    // cluster queue of the layers above
+    uint32_t rows[128];
 };
 
 // Lane-strided walk over the cells of a bx x by x bz box: lane `lane` visits
@@ -360,99 +362,158 @@ __device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc
             continue;
         }
         float3 part = make_float3(0.f, 0.f, 0.f), scaled = part;
-        BoxIter it = box_begin(bx, by, lane);
-        // Lanes fetch 64 cells of the children box per chunk.  A cell's
-        // position (whose size says whether it is empty) is loaded one chunk
-        // ahead, after the current chunk's moment loads, so that its memory
-        // round trip overlaps the current chunk's work.
-        auto candidate = [&](const BoxIter &c, uint32_t &off) -> bool {
-            if (c.z >= bz) return false;
-            const int cx = c0[0] + c.x, cy = c0[1] + c.y, cz = c0[2] + c.z;
-            const bool near = cx >= n0[0] && cx <= n1[0] && cy >= n0[1] && cy <= n1[1] &&
-                              cz >= n0[2] && cz <= n1[2];
-            off = near ? 0u : cell_offset(gl, cx, cy, cz);
-            return !near;
-        };
-        uint32_t noff = 0;
-        bool ncand = candidate(it, noff);
-        float4 np = ncand ? P[noff] : make_float4(0.f, 0.f, 0.f, 0.f);
-        for (int base = 0; base < total; base += 64) {
-            const uint32_t off = noff;
-            const float4 p = np;
-            const bool use = ncand && p.w != 0.0f;
-            float4 w = make_float4(0.f, 0.f, 0.f, 0.f), m0 = w, m1 = w, m2 = w;
-            if (use && !COUNT) {
-                w = W[off];
+        if (COUNT) {
+            BoxIter it = box_begin(bx, by, lane);
+            for (int base = 0; base < total; base += 64, box_next(it)) {
+                bool use = false;
+                if (it.z < bz) {
+                    const int cx = c0[0] + it.x, cy = c0[1] + it.y, cz = c0[2] + it.z;
+                    const bool near = cx >= n0[0] && cx <= n1[0] && cy >= n0[1] &&
+                                      cy <= n1[1] && cz >= n0[2] && cz <= n1[2];
+                    if (!near) use = P[cell_offset(gl, cx, cy, cz)].w != 0.0f;
+                }
+                n_clusters += __popcll(__ballot(use));
+            }
+        } else {
+            // The children box is enumerated through the parents' masks of
+            // non-empty children: each lane takes one parent of layer l + 1,
+            // clears the children inside the near box, and the wave appends
+            // the rest, one child per lane per round, to an LDS queue that is
+            // consumed 64 clusters at a time.  Empty and near cells are never
+            // loaded.
+            uint32_t *Q = S.rows;  // 128 entries; fewer than 64 wait between rounds
+            uint32_t qn = 0;
+            const vg_grid_desc &gp = tv.layer[l + 1];
+            const uint32_t *__restrict__ Cm = tv.C[l + 1];
+            const int dx = (int)tv.dec[l + 1][0], dy = (int)tv.dec[l + 1][1];
+            const int dz = (int)tv.dec[l + 1][2];
+            const int px = hi[0] - lo[0] + 1, py = hi[1] - lo[1] + 1, pz = hi[2] - lo[2] + 1;
+            const int ptotal = px * py * pz;
+            auto consume = [&](uint32_t cnt) {
+                const bool use = lane < cnt;
+                const uint32_t off = use ? Q[lane] : 0u;
+                float4 p = make_float4(0.f, 0.f, 0.f, 0.f), w = p, m0 = p, m1 = p, m2 = p;
+                if (use) {
+                    p = P[off];
+                    w = W[off];
+                    if (ORDER) {
+                        m0 = M[3 * (size_t)off];
+                        m1 = M[3 * (size_t)off + 1];
+                        m2 = M[3 * (size_t)off + 2];
+                    }
+                }
+                unsigned long long mask = __ballot(use);
+                S.P[lane] = p;
+                S.W[lane] = w;
                 if (ORDER) {
-                    m0 = M[3 * (size_t)off];
-                    m1 = M[3 * (size_t)off + 1];
-                    m2 = M[3 * (size_t)off + 2];
+                    S.M0[lane] = m0;
+                    S.M1[lane] = m1;
+                    S.M2[lane] = m2;
+                }
+                // every cluster of the batch has the current size (the usual
+                // case: one vorton size per tree)?
+                const bool one_size = __ballot(use && p.w != f.size) == 0ull;
+                __builtin_amdgcn_wave_barrier();
+                if (one_size && f.r2 < far2) {
+                    // no size test (the five LDS reads of a cluster go out
+                    // together, one wait per cluster), and no core test
+                    while (mask) {
+                        const int j = __builtin_ctzll(mask);
+                        mask &= mask - 1;
+                        const float4 cp = S.P[j], cw = S.W[j];
+                        float4 a0 = cp, a1 = cp, a2 = cp;
+                        if (ORDER) {
+                            a0 = S.M0[j];
+                            a1 = S.M1[j];
+                            a2 = S.M2[j];
+                        }
+                        cluster_term<POLICY, ORDER, true>(part, q, cp, cw, a0, a1, a2, f);
+                    }
+                } else if (one_size) {
+                    while (mask) {
+                        const int j = __builtin_ctzll(mask);
+                        mask &= mask - 1;
+                        const float4 cp = S.P[j], cw = S.W[j];
+                        float4 a0 = cp, a1 = cp, a2 = cp;
+                        if (ORDER) {
+                            a0 = S.M0[j];
+                            a1 = S.M1[j];
+                            a2 = S.M2[j];
+                        }
+                        cluster_term<POLICY, ORDER>(part, q, cp, cw, a0, a1, a2, f);
+                    }
+                } else {
+                    while (mask) {
+                        const int j = __builtin_ctzll(mask);
+                        mask &= mask - 1;
+                        const float4 cp = S.P[j];
+                        if (cp.w != f.size) set_size(f, cp.w, spread, part, scaled);
+                        const float4 cw = S.W[j];
+                        float4 a0 = cp, a1 = cp, a2 = cp;
+                        if (ORDER) {
+                            a0 = S.M0[j];
+                            a1 = S.M1[j];
+                            a2 = S.M2[j];
+                        }
+                        cluster_term<POLICY, ORDER>(part, q, cp, cw, a0, a1, a2, f);
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+            };
+            BoxIter it = box_begin(px, py, lane);
+            for (int base = 0; base < ptotal; base += 64, box_next(it)) {
+                uint32_t m = 0u, cbase = 0u;
+                if (it.z < pz) {
+                    const int ax = lo[0] + it.x, ay = lo[1] + it.y, az = lo[2] + it.z;
+                    m = Cm[cell_offset(gp, ax, ay, az)];
+                    const int cx = ax * dx, cy = ay * dy, cz = az * dz;
+                    cbase = cell_offset(gl, cx, cy, cz);
+                    // children of this parent inside the near box, per axis
+                    uint32_t nx = 0u, ny = 0u, nz = 0u;
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        if (i < dx && cx + i >= n0[0] && cx + i <= n1[0]) nx |= 1u << i;
+                        if (i < dy && cy + i >= n0[1] && cy + i <= n1[1]) ny |= 1u << i;
+                        if (i < dz && cz + i >= n0[2] && cz + i <= n1[2]) nz |= 1u << i;
+                    }
+                    if (nx && ny && nz) {
+                        uint32_t nm = 0u;
+#pragma unroll
+                        for (int k = 0; k < 3; ++k)
+#pragma unroll
+                            for (int j = 0; j < 3; ++j)
+                                if (((nz >> k) & (ny >> j) & 1u) != 0u)
+                                    nm |= nx << (dx * (j + dy * k));
+                        m &= ~nm;
+                    }
+                }
+                for (;;) {
+                    const bool take = m != 0u;
+                    const unsigned long long b = __ballot(take);
+                    if (b == 0ull) break;
+                    if (take) {
+                        const uint32_t bit = (uint32_t)__builtin_ctz(m);
+                        m &= m - 1u;
+                        const uint32_t i = bit % (uint32_t)dx, jk = bit / (uint32_t)dx;
+                        const uint32_t j = jk % (uint32_t)dy, k = jk / (uint32_t)dy;
+                        const uint32_t below = (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
+                        Q[qn + below] = cbase + i + gl.np[0] * (j + gl.np[1] * k);
+                    }
+                    qn += (uint32_t)__popcll(b);
+                    if (qn >= 64u) {
+                        __builtin_amdgcn_wave_barrier();
+                        consume(64u);
+                        const uint32_t rest = qn - 64u;
+                        const uint32_t moved = lane < rest ? Q[64 + lane] : 0u;
+                        __builtin_amdgcn_wave_barrier();
+                        if (lane < rest) Q[lane] = moved;
+                        __builtin_amdgcn_wave_barrier();
+                        qn = rest;
+                    }
                 }
             }
-            box_next(it);
-            ncand = base + 64 < total && candidate(it, noff);
-            np = ncand ? P[noff] : make_float4(0.f, 0.f, 0.f, 0.f);
-            unsigned long long mask = __ballot(use);
-            if (COUNT) {
-                n_clusters += __popcll(mask);
-                continue;
-            }
-            S.P[lane] = p;
-            S.W[lane] = w;
-            if (ORDER) {
-                S.M0[lane] = m0;
-                S.M1[lane] = m1;
-                S.M2[lane] = m2;
-            }
-            // every accepted cluster of the chunk has the current size (the
-            // usual case: one vorton size per tree)?
-            const bool one_size = __ballot(use && p.w != f.size) == 0ull;
             __builtin_amdgcn_wave_barrier();
-            if (one_size && f.r2 < far2) {
-                // no size test (the five LDS reads of a cluster go out
-                // together, one wait per cluster), and no core test
-                while (mask) {
-                    const int j = __builtin_ctzll(mask);
-                    mask &= mask - 1;
-                    const float4 cp = S.P[j], cw = S.W[j];
-                    float4 a0 = cp, a1 = cp, a2 = cp;
-                    if (ORDER) {
-                        a0 = S.M0[j];
-                        a1 = S.M1[j];
-                        a2 = S.M2[j];
-                    }
-                    cluster_term<POLICY, ORDER, true>(part, q, cp, cw, a0, a1, a2, f);
-                }
-            } else if (one_size) {
-                while (mask) {
-                    const int j = __builtin_ctzll(mask);
-                    mask &= mask - 1;
-                    const float4 cp = S.P[j], cw = S.W[j];
-                    float4 a0 = cp, a1 = cp, a2 = cp;
-                    if (ORDER) {
-                        a0 = S.M0[j];
-                        a1 = S.M1[j];
-                        a2 = S.M2[j];
-                    }
-                    cluster_term<POLICY, ORDER>(part, q, cp, cw, a0, a1, a2, f);
-                }
-            } else {
-                while (mask) {
-                    const int j = __builtin_ctzll(mask);
-                    mask &= mask - 1;
-                    const float4 cp = S.P[j];
-                    if (cp.w != f.size) set_size(f, cp.w, spread, part, scaled);
-                    const float4 cw = S.W[j];
-                    float4 a0 = cp, a1 = cp, a2 = cp;
-                    if (ORDER) {
-                        a0 = S.M0[j];
-                        a1 = S.M1[j];
-                        a2 = S.M2[j];
-                    }
-                    cluster_term<POLICY, ORDER>(part, q, cp, cw, a0, a1, a2, f);
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
+            if (qn) consume(qn);
         }
         scaled.x += part.x * f.s;
         scaled.y += part.y * f.s;
@@ -497,8 +558,8 @@ __device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc
                 n_pairs += total;
                 continue;
             }
-            S.row_off[lane] = incl - rc;
-            S.row_start[lane] = rs;
+            S.rows[lane] = incl - rc;
+            S.rows[64 + lane] = rs;
             __builtin_amdgcn_wave_barrier();
             for (uint32_t vbase = 0; vbase < total; vbase += 64) {
                 const uint32_t t = vbase + lane;
@@ -508,10 +569,10 @@ __device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc
                     int lo_i = 0, hi_i = 63;
                     while (lo_i < hi_i) {
                         const int mid = (lo_i + hi_i + 1) >> 1;
-                        if (S.row_off[mid] <= t) lo_i = mid;
+                        if (S.rows[mid] <= t) lo_i = mid;
                         else hi_i = mid - 1;
                     }
-                    const uint32_t idx = S.row_start[lo_i] + (t - S.row_off[lo_i]);
+                    const uint32_t idx = S.rows[64 + lo_i] + (t - S.rows[lo_i]);
                     p = tv.sorted_p[idx];
                     w = tv.sorted_w[idx];
                 }

@@ -26,6 +26,8 @@ struct vg_tree {
     float4 *P[vg::kMaxLayers] = {};
     float4 *W[vg::kMaxLayers] = {};
     float4 *M[vg::kMaxLayers] = {};  // 3 float4 per cluster: first moment rows + antisymmetric part
+    uint32_t *C[vg::kMaxLayers] = {};  // non-empty child masks (layers >= 1)
+    size_t c_cap[vg::kMaxLayers] = {};
     size_t m_cap[vg::kMaxLayers] = {};
     size_t layer_cap[vg::kMaxLayers] = {};
     uint32_t *cell_start = nullptr, *cell_count = nullptr;
@@ -226,7 +228,7 @@ __global__ void k_aggregate(vg_grid_desc gp, vg_grid_desc gc, uint32_t dx, uint3
                             uint32_t dz, const float4 *__restrict__ Pc,
                             const float4 *__restrict__ Wc, const float4 *__restrict__ Mc,
                             float4 *__restrict__ Pp, float4 *__restrict__ Wp,
-                            float4 *__restrict__ Mp) {
+                            float4 *__restrict__ Mp, uint32_t *__restrict__ Cp) {
 #pragma clang fp contract(off)
     const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t cap = capacity(gp);
@@ -235,13 +237,14 @@ __global__ void k_aggregate(vg_grid_desc gp, vg_grid_desc gc, uint32_t dx, uint3
     {
         const size_t tp = (size_t)blockIdx.y * cap, tc = (size_t)blockIdx.y * capacity(gc);
         Pc += tc; Wc += tc; Mc += 3 * tc;
-        Pp += tp; Wp += tp; Mp += 3 * tp;
+        Pp += tp; Wp += tp; Mp += 3 * tp; Cp += tp;
     }
     const uint32_t ix = (uint32_t)(o % gp.np[0]);
     const uint32_t iy = (uint32_t)((o / gp.np[0]) % gp.np[1]);
     const uint32_t iz = (uint32_t)(o / ((size_t)gp.np[0] * gp.np[1]));
     float px = 0.f, py = 0.f, pz = 0.f, wx = 0.f, wy = 0.f, wz = 0.f, sz = 0.f;
     float aux = FLT_MIN;
+    uint32_t cmask = 0;
     float D[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (ix < gp.np[0] - 1 && iy < gp.np[1] - 1 && iz < gp.np[2] - 1) {
         for (uint32_t cz = 0; cz < dz; ++cz)
@@ -254,7 +257,10 @@ __global__ void k_aggregate(vg_grid_desc gp, vg_grid_desc gc, uint32_t dx, uint3
                     px += p.x * mag; py += p.y * mag; pz += p.z * mag;
                     wx += w.x; wy += w.y; wz += w.z;
                     aux += mag;
-                    if (p.w != 0.0f) sz = p.w;
+                    if (p.w != 0.0f) {
+                        sz = p.w;
+                        cmask |= 1u << (cx + dx * (cy + dy * cz));
+                    }
                 }
         const float inv = 1.0f / aux;
         px *= inv; py *= inv; pz *= inv;
@@ -279,6 +285,7 @@ __global__ void k_aggregate(vg_grid_desc gp, vg_grid_desc gc, uint32_t dx, uint3
     }
     Pp[o] = make_float4(px, py, pz, sz);
     Wp[o] = make_float4(wx, wy, wz, aux);
+    Cp[o] = cmask;
     Mp[3 * o] = make_float4(D[0], D[1], D[2], D[5] - D[7]);
     Mp[3 * o + 1] = make_float4(D[3], D[4], D[5], D[6] - D[2]);
     Mp[3 * o + 2] = make_float4(D[6], D[7], D[8], D[1] - D[3]);
@@ -608,6 +615,7 @@ TreeView make_view(const vg_tree *t) {
         v.P[l] = t->P[l];
         v.W[l] = t->W[l];
         v.M[l] = t->M[l];
+        v.C[l] = t->C[l];
     }
     v.cell_start = t->cell_start;
     v.cell_count = t->cell_count;
@@ -689,7 +697,7 @@ vg_status batch_tree_build(const BatchTreeArgs &a, hipStream_t s) {
         VG_TRY(launch("vg_batch_aggregate", k_aggregate, dim3(ceil_div(cap, 256), a.trees), dim3(256),
                       0, s, ng.layer[l], ng.layer[l - 1], ng.dec[l][0], ng.dec[l][1], ng.dec[l][2],
                       (const float4 *)a.P[l - 1], (const float4 *)a.W[l - 1],
-                      (const float4 *)a.M[l - 1], a.P[l], a.W[l], a.M[l]));
+                      (const float4 *)a.M[l - 1], a.P[l], a.W[l], a.M[l], a.C[l]));
     }
     return VG_OK;
 }
@@ -723,6 +731,7 @@ VG_API vg_status vg_tree_destroy(vg_tree *t) {
         hipFree(t->P[l]);
         hipFree(t->W[l]);
         hipFree(t->M[l]);
+        hipFree(t->C[l]);
     }
     hipFree(t->cell_start);
     hipFree(t->cell_count);
@@ -756,6 +765,7 @@ VG_API vg_status vg_tree_build(vg_tree *t, const vg_grid_desc *tmpl, const vg_vo
         VG_TRY(grow(&t->W[l], &c2, cap));
         t->layer_cap[l] = c1 < c2 ? c1 : c2;
         VG_TRY(grow(&t->M[l], &t->m_cap[l], 3 * cap));
+        if (l >= 1) VG_TRY(grow(&t->C[l], &t->c_cap[l], cap));
     }
     size_t cc1 = t->cell_cap, cc2 = t->cell_cap;
     VG_TRY(grow(&t->cell_start, &cc1, cap0));
@@ -803,7 +813,7 @@ VG_API vg_status vg_tree_build(vg_tree *t, const vg_grid_desc *tmpl, const vg_vo
         VG_TRY(launch("vg_tree_aggregate", k_aggregate, dim3(ceil_div(cap, 256)), dim3(256), 0, s,
                       t->ng.layer[l], t->ng.layer[l - 1], t->ng.dec[l][0], t->ng.dec[l][1],
                       t->ng.dec[l][2], (const float4 *)t->P[l - 1], (const float4 *)t->W[l - 1],
-                      (const float4 *)t->M[l - 1], t->P[l], t->W[l], t->M[l]));
+                      (const float4 *)t->M[l - 1], t->P[l], t->W[l], t->M[l], t->C[l]));
     }
     t->built = true;
     return VG_OK;
