@@ -275,6 +275,79 @@ __device__ __forceinline__ float3 expansion_eval(Expansion &E, float3 x) {
 }
 
 // One wave evaluates one brick of grid points; S is the wave's LDS slot.
+// Appends to the LDS queue Q (128 entries, qn < 64 on entry) the non-empty
+// cells of layer l that are children of the cells lo..hi of layer l + 1 and
+// lie outside the near box n0..n1, as tag | offset.  Each lane takes one
+// parent and reads its mask of non-empty children (so empty and near cells
+// are never loaded); the wave then appends one child per lane per round.
+// Whenever 64 entries are queued, flush() consumes Q[0, 64) and the rest
+// moves down.
+template <class Flush>
+__device__ __forceinline__ void queue_far_children(const TreeView &tv, int l, const int lo[3],
+                                                   const int hi[3], const int n0[3],
+                                                   const int n1[3], uint32_t tag,
+                                                   uint32_t *Q, uint32_t &qn, uint32_t lane,
+                                                   Flush &&flush) {
+    const vg_grid_desc &gl = tv.layer[l], &gp = tv.layer[l + 1];
+    const uint32_t *__restrict__ Cm = tv.C[l + 1];
+    const int dx = (int)tv.dec[l + 1][0], dy = (int)tv.dec[l + 1][1];
+    const int dz = (int)tv.dec[l + 1][2];
+    const int px = hi[0] - lo[0] + 1, py = hi[1] - lo[1] + 1, pz = hi[2] - lo[2] + 1;
+    const int ptotal = px * py * pz;
+    BoxIter it = box_begin(px, py, lane);
+    for (int base = 0; base < ptotal; base += 64, box_next(it)) {
+        uint32_t m = 0u, cbase = 0u;
+        if (it.z < pz) {
+            const int ax = lo[0] + it.x, ay = lo[1] + it.y, az = lo[2] + it.z;
+            m = Cm[cell_offset(gp, ax, ay, az)];
+            const int cx = ax * dx, cy = ay * dy, cz = az * dz;
+            cbase = cell_offset(gl, cx, cy, cz);
+            // children of this parent inside the near box, per axis (a
+            // decimation is at most 3)
+            uint32_t nx = 0u, ny = 0u, nz = 0u;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                if (i < dx && cx + i >= n0[0] && cx + i <= n1[0]) nx |= 1u << i;
+                if (i < dy && cy + i >= n0[1] && cy + i <= n1[1]) ny |= 1u << i;
+                if (i < dz && cz + i >= n0[2] && cz + i <= n1[2]) nz |= 1u << i;
+            }
+            if (nx && ny && nz) {
+                uint32_t nm = 0u;
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+                        if (((nz >> k) & (ny >> j) & 1u) != 0u) nm |= nx << (dx * (j + dy * k));
+                m &= ~nm;
+            }
+        }
+        for (;;) {
+            const bool take = m != 0u;
+            const unsigned long long b = __ballot(take);
+            if (b == 0ull) break;
+            if (take) {
+                const uint32_t bit = (uint32_t)__builtin_ctz(m);
+                m &= m - 1u;
+                const uint32_t i = bit % (uint32_t)dx, jk = bit / (uint32_t)dx;
+                const uint32_t j = jk % (uint32_t)dy, k = jk / (uint32_t)dy;
+                const uint32_t below = (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
+                Q[qn + below] = tag | (cbase + i + gl.np[0] * (j + gl.np[1] * k));
+            }
+            qn += (uint32_t)__popcll(b);
+            if (qn >= 64u) {
+                __builtin_amdgcn_wave_barrier();
+                flush();
+                const uint32_t rest = qn - 64u;
+                const uint32_t moved = lane < rest ? Q[64 + lane] : 0u;
+                __builtin_amdgcn_wave_barrier();
+                if (lane < rest) Q[lane] = moved;
+                __builtin_amdgcn_wave_barrier();
+                qn = rest;
+            }
+        }
+    }
+}
+
 template <int POLICY, int ORDER, int COUNT>
 __device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc &g,
                                           const MacParams &mp, uint32_t brick, WaveStage &S,
@@ -375,20 +448,10 @@ __device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc
                 n_clusters += __popcll(__ballot(use));
             }
         } else {
-            // The children box is enumerated through the parents' masks of
-            // non-empty children: each lane takes one parent of layer l + 1,
-            // clears the children inside the near box, and the wave appends
-            // the rest, one child per lane per round, to an LDS queue that is
-            // consumed 64 clusters at a time.  Empty and near cells are never
-            // loaded.
-            uint32_t *Q = S.rows;  // 128 entries; fewer than 64 wait between rounds
+            // Far clusters of the layer come through the LDS queue, 64 at a
+            // time (fewer than 64 wait between rounds).
+            uint32_t *Q = S.rows;
             uint32_t qn = 0;
-            const vg_grid_desc &gp = tv.layer[l + 1];
-            const uint32_t *__restrict__ Cm = tv.C[l + 1];
-            const int dx = (int)tv.dec[l + 1][0], dy = (int)tv.dec[l + 1][1];
-            const int dz = (int)tv.dec[l + 1][2];
-            const int px = hi[0] - lo[0] + 1, py = hi[1] - lo[1] + 1, pz = hi[2] - lo[2] + 1;
-            const int ptotal = px * py * pz;
             auto consume = [&](uint32_t cnt) {
                 const bool use = lane < cnt;
                 const uint32_t off = use ? Q[lane] : 0u;
@@ -460,58 +523,7 @@ __device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc
                 }
                 __builtin_amdgcn_wave_barrier();
             };
-            BoxIter it = box_begin(px, py, lane);
-            for (int base = 0; base < ptotal; base += 64, box_next(it)) {
-                uint32_t m = 0u, cbase = 0u;
-                if (it.z < pz) {
-                    const int ax = lo[0] + it.x, ay = lo[1] + it.y, az = lo[2] + it.z;
-                    m = Cm[cell_offset(gp, ax, ay, az)];
-                    const int cx = ax * dx, cy = ay * dy, cz = az * dz;
-                    cbase = cell_offset(gl, cx, cy, cz);
-                    // children of this parent inside the near box, per axis
-                    uint32_t nx = 0u, ny = 0u, nz = 0u;
-#pragma unroll
-                    for (int i = 0; i < 3; ++i) {
-                        if (i < dx && cx + i >= n0[0] && cx + i <= n1[0]) nx |= 1u << i;
-                        if (i < dy && cy + i >= n0[1] && cy + i <= n1[1]) ny |= 1u << i;
-                        if (i < dz && cz + i >= n0[2] && cz + i <= n1[2]) nz |= 1u << i;
-                    }
-                    if (nx && ny && nz) {
-                        uint32_t nm = 0u;
-#pragma unroll
-                        for (int k = 0; k < 3; ++k)
-#pragma unroll
-                            for (int j = 0; j < 3; ++j)
-                                if (((nz >> k) & (ny >> j) & 1u) != 0u)
-                                    nm |= nx << (dx * (j + dy * k));
-                        m &= ~nm;
-                    }
-                }
-                for (;;) {
-                    const bool take = m != 0u;
-                    const unsigned long long b = __ballot(take);
-                    if (b == 0ull) break;
-                    if (take) {
-                        const uint32_t bit = (uint32_t)__builtin_ctz(m);
-                        m &= m - 1u;
-                        const uint32_t i = bit % (uint32_t)dx, jk = bit / (uint32_t)dx;
-                        const uint32_t j = jk % (uint32_t)dy, k = jk / (uint32_t)dy;
-                        const uint32_t below = (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
-                        Q[qn + below] = cbase + i + gl.np[0] * (j + gl.np[1] * k);
-                    }
-                    qn += (uint32_t)__popcll(b);
-                    if (qn >= 64u) {
-                        __builtin_amdgcn_wave_barrier();
-                        consume(64u);
-                        const uint32_t rest = qn - 64u;
-                        const uint32_t moved = lane < rest ? Q[64 + lane] : 0u;
-                        __builtin_amdgcn_wave_barrier();
-                        if (lane < rest) Q[lane] = moved;
-                        __builtin_amdgcn_wave_barrier();
-                        qn = rest;
-                    }
-                }
-            }
+            queue_far_children(tv, l, lo, hi, n0, n1, 0u, Q, qn, lane, [&] { consume(64u); });
             __builtin_amdgcn_wave_barrier();
             if (qn) consume(qn);
         }
