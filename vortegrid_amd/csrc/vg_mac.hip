@@ -12,12 +12,15 @@
 //   * sums the leaf near box directly over its vortons (cell-sorted lists).
 //
 // GPU mapping: one wave = one 4x4x4 brick of grid points.  Near boxes are
-// per brick, so every loop bound and branch is uniform across the wave: the
-// lanes fetch 64 candidate cells at a time (one each), stage the accepted
-// clusters in LDS, and every lane then reads each cluster back (a broadcast)
-// to evaluate it at its own point.  Chunks whose clusters all have the
-// current size skip the per-cluster size test, and clusters that lie beyond
-// the core radius by construction skip the core test.
+// per brick, so every loop bound and branch is uniform across the wave.  The
+// clusters of a layer are enumerated from the parent layer's near box: each
+// lane reads one parent's mask of non-empty children (written by the tree
+// build), and the children outside this layer's near box are queued in LDS.
+// Every 64 queued clusters are staged in LDS, and every lane then reads each
+// one back (a broadcast) to evaluate it at its own point.  Batches whose
+// clusters all have the current size skip the per-cluster size test, and
+// clusters that lie beyond the core radius by construction skip the core
+// test.
 #include "vg_internal.h"
 
 namespace vg {
