@@ -457,6 +457,39 @@ def test_mac_local_expansions(vg, oracle, rings2, k, order, expand_from):
         assert e > 0 and c < c0
 
 
+@pytest.mark.parametrize("cells,k,expand_from", [(16384, 2, -1), (16384, 2, 2), (25000, 3, 2)])
+def test_mac_velocity_grid_odd_shapes(vg, oracle, rings2, cells, k, expand_from):
+    """Grids whose cell counts are not powers of two: the NestedGrid
+    decimation is then 3 or 1 on some axes (3 cells under one parent, or a
+    2-point axis kept), and odd layers leave their last cell without a
+    parent.  The walk's per-parent child masks must give the same clusters
+    as the oracle's children box."""
+    pos, om, radius, _ = rings2
+    mn, mx = oracle.find_bbox(pos)
+    og = oracle.define_shape(cells, mn, mx, pow2=False)
+    ng = oracle.nested(og)
+    decs = {int(ng.dec[l][a]) for l in range(1, ng.depth) for a in range(3)}
+    case(f"mac velocity grid, grid {og.shape}, decimations {sorted(decs)}, k={k} from={expand_from}")
+    assert 3 in decs
+    v, keep = vortons(vg, pos, om, radius)
+    g = vg.GridDesc.from_fields(og)
+    t = vg.Tree()
+    t.build(g, v)
+    spread, circ = vg.auto_mollify(g, radius)
+    out = vg.DeviceArray.zeros((g.capacity, 3))
+    t.velocity_grid_mac(g, out, k, 1, spread, circ, 1, expand_from=expand_from)
+    vg.synchronize()
+    ot = oracle.InfluenceTree(og, pos, om, 2 * radius)
+    for l in range(ot.depth):
+        g1, p1, w1, s1 = t.layer(l)
+        g2, p2, w2, s2 = ot.layer(l)
+        assert np.array_equal(p1, p2) and np.array_equal(w1, w2), l
+    want = oracle.mac_velocity_grid_ex(ot, og, pos, om, 2 * radius, k, 1, expand_from, spread,
+                                       circ, 1, threads=8)
+    assert np.isfinite(out.numpy()).all()
+    assert rel_l2(out.numpy(), want) < 1e-5
+
+
 @pytest.mark.parametrize("k,expand_from", [(2, -1), (2, 2), (3, 2)])
 def test_mac_velocity_grid_mixed_sizes(vg, oracle, rings2, k, expand_from):
     """Vortons of two sizes: clusters take the size of their last non-empty
