@@ -234,7 +234,7 @@ def _grid_errors(vg, m, pos, om, radius, near_k=1, nsample=4000, seed=8):
 
 def test_trained_correction_gain(vg, oracle):
     """Train on direct-solver pairs from rotated ring, colliding-ring and
-    plume fields; judge against the FP64 direct sum on an unseen field of the
+    plume fields; measure against the FP64 direct sum on an unseen field of the
     training distributions and on held-out random rings."""
     case("mlp gain: train on rotated rings / plumes, evaluate unseen + held-out fields")
     from vortegrid_amd import learned
