@@ -215,7 +215,8 @@ def main():
         kname = "vg_tree_velocity_grid_mac + vg_tree_mac_expand"
     avg_s = (ms / max(cnt, 1)) * 1e-3
     achieved = flops / avg_s / 1e12 if avg_s > 0 else 0.0
-    roofline = {"kernel": kname, "bound": "mfma", "achieved": achieved, "peak": peak,
+    roofline = {"kernel": kname, "bound": "mfma" if args.mlp else "fp32", "achieved": achieved,
+                "peak": peak,
                 "unit": "TFLOP/s", "frac": achieved / peak,
                 "traffic": measured_traffic(args),
                 "avg_launch_ms": ms / max(cnt, 1), "cluster_interactions_per_launch": clusters,
@@ -274,57 +275,16 @@ def main():
         log(f"rel-L2 vs FP64 direct on {len(q)} grid points: {rel:.4e} "
             f"(fp64 direct {time.perf_counter() - tq:.2f}s)")
 
-    # ---- CPU baseline: the oracle restatement of the same frame, on a
-    # bounded sample (treecode on a band of z-slices + tree build + advection
-    # of a tracer subset), scaled to evals/s
+    # ---- CPU baseline: the oracle restatement of the same frame, timed on
+    # bounded samples and scaled to the whole frame
     cpu = None
     if not args.no_cpu:
         try:
-            from oracle import oracle as O
-
-            O.build()
-            cores = min(16, os.cpu_count() or 1)
-            og = O.Grid()
-            for k in ("min", "extent", "cell", "cpe", "np"):
-                for a in range(3):
-                    getattr(og, k)[a] = getattr(g, k)[a]
-            cur = d_pos.numpy()
-            tb = time.perf_counter()
-            ot = O.InfluenceTree(og, cur, om, 2 * radius)
-            t_build = time.perf_counter() - tb
-            nz = int(g.np[2])
-            spread, circ = sim.spreading()
-            tv = time.perf_counter()
-            if args.near_k < 0:
-                slices = max(1, min(nz, 24))
-                sub = O.Grid()
-                for k in ("min", "extent", "cell", "cpe", "np"):
-                    for a in range(3):
-                        getattr(sub, k)[a] = getattr(og, k)[a]
-                sub.np[2] = slices
-                O.velocity_grid(ot, sub, cur, om, 2 * radius, 1, spread, circ, args.policy,
-                                threads=cores)
-            else:
-                rows = 1 if args.near_k >= 2 else 2
-                bz0 = (nz + 3) // 8  # a brick row a quarter of the way up the grid
-                O.mac_velocity_grid_ex(ot, og, cur, om, 2 * radius, args.near_k, args.order,
-                                       args.expand_from, spread, circ, args.policy, cores,
-                                       bz0, bz0 + rows)
-                slices = 4 * rows
-            t_vel = time.perf_counter() - tv
-            pts = int(g.np[0]) * int(g.np[1]) * slices
-            ntr = min(len(tr), 1 << 20)
-            gf = vg.fetch(sim.velocity_grid()[1], 3 * g.capacity)
-            ta = time.perf_counter()
-            O.advect(og, gf, tr[:ntr], np.zeros((ntr, 3), np.float32), dt, 1.0)
-            t_adv = time.perf_counter() - ta
-            frame_s = t_build + t_vel * (g.capacity / pts) + t_adv * (len(tr) / ntr)
-            cpu = {"value": g.capacity / frame_s, "unit": "velocity evals/s", "cores": cores,
-                   "kind": "port", "ms_per_frame": frame_s * 1e3,
-                   "sample": (f"oracle tree build on all {len(pos)} vortons (1 thread), treecode "
-                              f"on {slices} of {nz} z-slices ({pts} grid points, {cores} threads)"
-                              f", advection of {ntr} tracers (1 thread); scaled to the full frame")}
-            log(f"cpu baseline {cpu['ms_per_frame']:.1f} ms/frame")
+            cpu = cpu_baseline(args, g, sim.spreading(), d_pos.numpy(), om, radius, tr, dt,
+                               vg.fetch(sim.velocity_grid()[1], 3 * g.capacity))
+            log(f"cpu baseline {cpu['ms_per_frame']:.1f} ms/frame on {cpu['cores']} threads, "
+                f"{cpu['one_thread']['ms_per_frame']:.1f} ms/frame on 1; reference rule "
+                f"{cpu['reference_rule']['ms_per_frame']:.1f} ms/frame")
         except Exception as e:  # the CPU leg never blocks the GPU number
             log(f"cpu baseline skipped: {e!r}")
 
@@ -362,6 +322,101 @@ def main():
         "kernels_ms": {k: {"count": c, "total_ms": m} for k, (c, m) in prof.items()},
     }
     print(json.dumps(out))
+
+
+def cpu_threads():
+    """Threads for the CPU legs: the cores this process may run on (its
+    affinity set), at most 16, the host share of one GPU on the machines
+    this runs on."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(args, g, mollify, cur, om, radius, tr, dt, grid_field):
+    """The oracle restatement of one frame on the host cores: tree build
+    (serial in the oracle, as MakeBaseVortonGrid is), the velocity grid by
+    the same method as the GPU line on a band of bricks (all threads, and a
+    smaller band on one thread), the reference opening rule on a band of z
+    slices (all threads and one thread), and advection of a tracer subset;
+    each timed part is scaled to the whole frame.  About 10-30 s in all."""
+    from oracle import oracle as O
+
+    O.build()
+    cores = cpu_threads()
+    spread, circ = mollify
+    og = O.Grid()
+    for k in ("min", "extent", "cell", "cpe", "np"):
+        for a in range(3):
+            getattr(og, k)[a] = getattr(g, k)[a]
+    cap = g.capacity
+    nx, ny, nz = (int(v) for v in g.np)
+    size = 2 * radius
+    tb = time.perf_counter()
+    ot = O.InfluenceTree(og, cur, om, size)
+    t_build = time.perf_counter() - tb
+    ntr = min(len(tr), 1 << 20)
+    ta = time.perf_counter()
+    O.advect(og, grid_field, tr[:ntr], np.zeros((ntr, 3), np.float32), dt, 1.0)
+    t_adv = (time.perf_counter() - ta) * (len(tr) / ntr)
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        fn()
+        return time.perf_counter() - t0
+
+    # reference opening rule on the lowest z slices
+    slices = max(1, min(nz, 16))
+    sub = O.Grid()
+    for k in ("min", "extent", "cell", "cpe", "np"):
+        for a in range(3):
+            getattr(sub, k)[a] = getattr(og, k)[a]
+    sub.np[2] = slices
+    ref_pts = nx * ny * slices
+    ref_all = timed(lambda: O.velocity_grid(ot, sub, cur, om, size, 1, spread, circ, args.policy,
+                                            threads=cores)) * cap / ref_pts
+    sub.np[2] = 2
+    ref_one = timed(lambda: O.velocity_grid(ot, sub, cur, om, size, 1, spread, circ, args.policy,
+                                            threads=1)) * cap / (nx * ny * 2)
+    rec = {"unit": "velocity evals/s", "kind": "port"}
+    if args.near_k >= 0:
+        # the GPU line's method on one z row of bricks a quarter of the way up
+        # the grid, on all threads and on one thread
+        nbx = (nx + 3) // 4
+        bz0 = (nz + 3) // 8
+        row_pts = nx * ny * 4
+        walk = lambda th: O.mac_velocity_grid_box(ot, og, cur, om, size, args.near_k, args.order,
+                                                  args.expand_from, spread, circ, args.policy, th,
+                                                  (0, nbx), (bz0, bz0 + 1))
+        w_all = timed(lambda: walk(cores)) * cap / row_pts
+        w_one = timed(lambda: walk(1)) * cap / row_pts
+        method = (f"near-field treecode near_k={args.near_k} order={args.order} expand_from="
+                  f"{args.expand_from} on z brick row {bz0} ({row_pts} points)")
+    else:
+        w_all, w_one = ref_all, ref_one
+        method = "reference opening rule"
+    frame_all = t_build + w_all + t_adv
+    frame_one = t_build + w_one + t_adv
+    rec.update({
+        "value": cap / frame_all, "cores": cores, "host_cpus": os.cpu_count(),
+        "ms_per_frame": frame_all * 1e3,
+        "one_thread": {"value": cap / frame_one, "ms_per_frame": frame_one * 1e3},
+        "reference_rule": {"value": cap / (t_build + ref_all + t_adv),
+                           "ms_per_frame": (t_build + ref_all + t_adv) * 1e3,
+                           "one_thread_ms_per_frame": (t_build + ref_one + t_adv) * 1e3,
+                           "velocity_grid_ms": ref_all * 1e3,
+                           "velocity_grid_one_thread_ms": ref_one * 1e3},
+        "parts_ms": {"tree_build_1_thread": t_build * 1e3, "velocity_grid": w_all * 1e3,
+                     "velocity_grid_1_thread": w_one * 1e3, "advection_1_thread": t_adv * 1e3},
+        "sample": (f"oracle on the frame's final state: tree build on all {len(cur)} vortons "
+                   f"(serial), {method}, reference rule on {slices} of {nz} z-slices "
+                   f"({cores} threads) and 2 slices (1 thread), advection of {ntr} of {len(tr)} "
+                   f"tracers (serial); each scaled to the full {nx}x{ny}x{nz} frame; {cores} "
+                   f"threads = this process's CPU share (affinity, at most 16)"),
+    })
+    return rec
 
 
 # HBM-side bytes per launch of the dominant kernel, from the two counter
@@ -477,7 +532,7 @@ def batched(args):
             from oracle import oracle as O
 
             O.build()
-            cores = min(16, os.cpu_count() or 1)
+            cores = cpu_threads()
             tpos = b.download("tpos")
             toff = np.cumsum([0] + [len(t) for t in tp])
             t_build = t_walk = t_adv = 0.0

@@ -187,7 +187,10 @@ vg_status vg_tree_interaction_count(const vg_tree *t, const vg_grid_desc *g,
  * near_k cells of the brick are opened, the other children of the parent's
  * near box are applied as clusters (order 0: monopole at the |w|-weighted
  * centroid, order 1: plus the first-moment correction), and the leaf near
- * box is summed over its vortons. */
+ * box is summed over its vortons.  A tree whose grid leaves a cell without a
+ * parent cluster (a cell count odd and above 3 on some layer, possible only
+ * for grids that are not power-of-two shaped) is refused with VG_EINVAL:
+ * the layers above never see that cell's vortons. */
 vg_status vg_tree_velocity_grid_mac(const vg_tree *t, const vg_grid_desc *g, int near_k,
                                     int order, int expand_from, float spread, float circ,
                                     int rsqrt_policy, float *out, vg_stream s);
@@ -268,6 +271,13 @@ vg_status vg_mg_cascade(vg_multigrid *m, const vg_grid_desc *g, float *a0,
 vg_status vg_mg_vcycle_solve(vg_multigrid *m, const vg_grid_desc *g, float *a0,
                              const float *negw0, double tol, uint32_t max_cycles,
                              double *rel_residual, uint32_t *cycles, vg_stream s);
+/* The V-cycle solution is the compensated pair a0 + lo: an FP32 field alone
+ * has a rounding floor of ~1e-4 in the relative residual at 129^3, so the
+ * 1e-6 tolerance holds for the pair (each component summed in FP64).  This
+ * returns the device pointer of lo (count = 3 * grid points floats), valid
+ * until the next solve on m.  No reference counterpart (the reference's
+ * cascade, vortonSim.cpp:2523-2571, never converges to a tolerance). */
+vg_status vg_mg_vcycle_compensation(const vg_multigrid *m, const float **lo, size_t *count);
 
 /* ------------------------------------------------------------ particles */
 /* AssignVelocityFromField (pclOpAdvect.cpp:132-152, slice :42-81): vel =
@@ -436,7 +446,9 @@ vg_status vg_tree_velocity_grid_mlp(const vg_tree *t, const vg_grid_desc *g, vg_
  * launch (vg_batch_launch). */
 typedef struct vg_batch vg_batch;
 typedef struct vg_batch_config {
-    uint32_t cells;    /* cells per axis of every effect's grid (default 32) */
+    uint32_t cells;    /* cells per axis of every effect's grid (default 32): a power
+                        * of two or 3 * 2^k; counts that leave a tree cell without
+                        * a parent (e.g. 20, 36) are refused with VG_EINVAL */
     int near_k;        /* near-field treecode radius (default 2) */
     int order;         /* cluster order: 0 monopole, 1 dipole (default 1) */
     int rsqrt_policy;  /* VG_RSQRT_* (default ACCURATE) */

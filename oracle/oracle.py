@@ -135,6 +135,15 @@ def lib():
                                                f32p, C.c_int, C.c_int, C.c_int, C.c_float,
                                                C.c_float, C.c_int, f32p, C.c_int, C.c_uint32,
                                                C.c_uint32]
+        L.vgo_mac_velocity_grid_box.argtypes = [C.POINTER(Tree), C.POINTER(Grid), f32p, f32p,
+                                                f32p, C.c_int, C.c_int, C.c_int, C.c_float,
+                                                C.c_float, C.c_int, f32p, C.c_int, C.c_uint32,
+                                                C.c_uint32, C.c_uint32, C.c_uint32]
+        for name in ("vgo_mac_velocity_grid", "vgo_mac_velocity_grid_bricks",
+                     "vgo_mac_velocity_grid_ex", "vgo_mac_velocity_grid_box"):
+            getattr(L, name).restype = C.c_int
+        L.vgo_nested_has_orphans.restype = C.c_int
+        L.vgo_nested_has_orphans.argtypes = [C.POINTER(Nested)]
         L.vgo_local_expansion.argtypes = [f32p, f32p, f32p, C.c_float, f32p, C.c_int, C.c_float,
                                           C.POINTER(C.c_double)]
         L.vgo_local_expansion.restype = None
@@ -163,6 +172,8 @@ def lib():
                                        C.c_size_t, C.c_float]
         L.vgo_residual_rel.restype = C.c_double
         L.vgo_residual_rel.argtypes = [C.POINTER(Grid), f32p, f32p]
+        L.vgo_residual_rel_pair.restype = C.c_double
+        L.vgo_residual_rel_pair.argtypes = [C.POINTER(Grid), f32p, f32p, f32p]
         L.vgo_assign_velocity_from_field.argtypes = [C.POINTER(Grid), f32p, f32p, f32p,
                                                      C.c_size_t, C.c_size_t, C.c_float]
         L.vgo_evolve.argtypes = [f32p, f32p, C.c_size_t, C.c_size_t, C.c_float]
@@ -355,15 +366,27 @@ def velocity_grid(tree, g: Grid, pos, om, size, technique=1, spread=1.0, circ=1.
     return out.reshape(-1, 3)
 
 
+def _walk_status(rc: int):
+    if rc != 0:
+        raise ValueError("the tree's grid leaves a cell without a parent cluster (a cell count "
+                         "odd and above 3 on some layer); the near-field walk refuses it")
+
+
+def nested_has_orphans(g: Grid) -> bool:
+    """True when some NestedGrid layer of g leaves a cell without a parent."""
+    ng = nested(g)
+    return bool(lib().vgo_nested_has_orphans(C.byref(ng)))
+
+
 def mac_velocity_grid(tree, g: Grid, pos, om, size, near_k=2, order=1, spread=1.0, circ=1.0,
                       policy=1, threads=1):
     """Near-field treecode at every grid point (bricks of 4^3 points)."""
     pos = _f32(pos).reshape(-1)
     n = pos.size // 3
     out = np.zeros(g.capacity * 3, np.float32)
-    lib().vgo_mac_velocity_grid(tree.ptr(), C.byref(g), pos, _f32(om).reshape(-1),
-                                _size_arr(size, n), near_k, order, spread, circ, policy, out,
-                                threads)
+    _walk_status(lib().vgo_mac_velocity_grid(tree.ptr(), C.byref(g), pos, _f32(om).reshape(-1),
+                                             _size_arr(size, n), near_k, order, spread, circ,
+                                             policy, out, threads))
     return out.reshape(-1, 3)
 
 
@@ -373,9 +396,10 @@ def mac_velocity_grid_bricks(tree, g: Grid, pos, om, size, near_k, order, spread
     pos = _f32(pos).reshape(-1)
     n = pos.size // 3
     out = np.zeros(g.capacity * 3, np.float32)
-    lib().vgo_mac_velocity_grid_bricks(tree.ptr(), C.byref(g), pos, _f32(om).reshape(-1),
-                                       _size_arr(size, n), near_k, order, spread, circ, policy,
-                                       out, threads, bz_begin, bz_end)
+    _walk_status(lib().vgo_mac_velocity_grid_bricks(tree.ptr(), C.byref(g), pos,
+                                                    _f32(om).reshape(-1), _size_arr(size, n),
+                                                    near_k, order, spread, circ, policy, out,
+                                                    threads, bz_begin, bz_end))
     return out.reshape(-1, 3)
 
 
@@ -388,9 +412,24 @@ def mac_velocity_grid_ex(tree, g: Grid, pos, om, size, near_k, order, expand_fro
     out = np.zeros(g.capacity * 3, np.float32)
     if bz_end is None:
         bz_end = (g.np[2] + 3) // 4
-    lib().vgo_mac_velocity_grid_ex(tree.ptr(), C.byref(g), pos, _f32(om).reshape(-1),
-                                   _size_arr(size, n), near_k, order, expand_from, spread, circ,
-                                   policy, out, threads, bz_begin, bz_end)
+    _walk_status(lib().vgo_mac_velocity_grid_ex(tree.ptr(), C.byref(g), pos,
+                                                _f32(om).reshape(-1), _size_arr(size, n), near_k,
+                                                order, expand_from, spread, circ, policy, out,
+                                                threads, bz_begin, bz_end))
+    return out.reshape(-1, 3)
+
+
+def mac_velocity_grid_box(tree, g: Grid, pos, om, size, near_k, order, expand_from, spread,
+                          circ, policy, threads, bx, bz):
+    """Near-field treecode on the bricks with x index in range(*bx) and z
+    index in range(*bz) only (CPU timing samples)."""
+    pos = _f32(pos).reshape(-1)
+    n = pos.size // 3
+    out = np.zeros(g.capacity * 3, np.float32)
+    _walk_status(lib().vgo_mac_velocity_grid_box(tree.ptr(), C.byref(g), pos,
+                                                 _f32(om).reshape(-1), _size_arr(size, n), near_k,
+                                                 order, expand_from, spread, circ, policy, out,
+                                                 threads, bx[0], bx[1], bz[0], bz[1]))
     return out.reshape(-1, 3)
 
 
@@ -525,8 +564,12 @@ def brick_of_bytes(g: Grid, field):
     return planes, stats
 
 
-def residual_rel(g: Grid, u, f) -> float:
-    return float(lib().vgo_residual_rel(C.byref(g), _f32(u).reshape(-1), _f32(f).reshape(-1)))
+def residual_rel(g: Grid, u, f, lo=None) -> float:
+    """FP64 ||f - L u|| / ||f|| over the interior; u + lo when lo is given."""
+    if lo is None:
+        return float(lib().vgo_residual_rel(C.byref(g), _f32(u).reshape(-1), _f32(f).reshape(-1)))
+    return float(lib().vgo_residual_rel_pair(C.byref(g), _f32(u).reshape(-1),
+                                             _f32(lo).reshape(-1), _f32(f).reshape(-1)))
 
 
 def advect(g: Grid, field, pos, vel, dt, gain=1.0):

@@ -504,6 +504,7 @@ typedef struct {
     float spread, circ;
     float *out;
     uint32_t nb[3];
+    uint32_t bx0, bx1; /* brick columns evaluated (timing samples) */
 } mac_ctx;
 
 /* Local expansion about a brick centre (no reference counterpart), in
@@ -735,49 +736,74 @@ static void mac_brick(mac_ctx *c, uint32_t bx, uint32_t by, uint32_t bz) {
             }
 }
 
-static void mac_range(void *vc, size_t b, size_t e) {
-    mac_ctx *c = (mac_ctx *)vc;
-    for (size_t bz = b; bz < e; ++bz)
-        for (uint32_t by = 0; by < c->nb[1]; ++by)
-            for (uint32_t bx = 0; bx < c->nb[0]; ++bx) mac_brick(c, bx, by, (uint32_t)bz);
-}
-
 typedef struct { mac_ctx *c; uint32_t bz0; } mac_slab;
 
+/* Work units are single bricks, so one z row of bricks still spreads over
+ * every thread. */
 static void mac_slab_range(void *vs, size_t b, size_t e) {
     mac_slab *s = (mac_slab *)vs;
-    mac_range(s->c, s->bz0 + b, s->bz0 + e);
+    mac_ctx *c = s->c;
+    const size_t nbx = c->bx1 - c->bx0;
+    for (size_t u = b; u < e; ++u) {
+        const uint32_t bx = c->bx0 + (uint32_t)(u % nbx);
+        const uint32_t by = (uint32_t)((u / nbx) % c->nb[1]);
+        const uint32_t bz = s->bz0 + (uint32_t)(u / (nbx * c->nb[1]));
+        mac_brick(c, bx, by, bz);
+    }
 }
 
-void vgo_mac_velocity_grid_ex(const vgo_tree *t, const vgo_grid *g, const float *pos,
+int vgo_nested_has_orphans(const vgo_nested *ng) {
+    for (uint32_t l = 1; l < ng->depth; ++l)
+        for (int a = 0; a < 3; ++a)
+            if (ng->layer[l - 1].np[a] - 1 != ng->dec[l][a] * (ng->layer[l].np[a] - 1)) return 1;
+    return 0;
+}
+
+int vgo_mac_velocity_grid_box(const vgo_tree *t, const vgo_grid *g, const float *pos,
                               const float *om, const float *size, int near_k, int order,
                               int expand_from, float spread, float circ, int rsqrt_policy,
-                              float *out, int nthreads, uint32_t bz_begin, uint32_t bz_end) {
+                              float *out, int nthreads, uint32_t bx_begin, uint32_t bx_end,
+                              uint32_t bz_begin, uint32_t bz_end) {
+    if (vgo_nested_has_orphans(&t->ng)) return -1;
     mac_ctx c;
     c.t = t; c.g = g; c.pos = pos; c.om = om; c.size = size;
     c.k = near_k; c.order = order; c.policy = rsqrt_policy; c.expand_from = expand_from;
     c.spread = spread; c.circ = circ; c.out = out;
     for (int a = 0; a < 3; ++a) c.nb[a] = (g->np[a] + 3) / 4;
+    c.bx0 = bx_begin;
+    c.bx1 = bx_end < c.nb[0] ? bx_end : c.nb[0];
     if (bz_end > c.nb[2]) bz_end = c.nb[2];
-    if (bz_begin >= bz_end) return;
+    if (bz_begin >= bz_end) return 0;
+    if (c.bx0 >= c.bx1) return 0;
     mac_slab s = {&c, bz_begin};
-    parallel_range(bz_end - bz_begin, nthreads, mac_slab_range, &s);
+    parallel_range((size_t)(bz_end - bz_begin) * c.nb[1] * (c.bx1 - c.bx0), nthreads,
+                   mac_slab_range, &s);
+    return 0;
 }
 
-void vgo_mac_velocity_grid_bricks(const vgo_tree *t, const vgo_grid *g, const float *pos,
-                                  const float *om, const float *size, int near_k, int order,
-                                  float spread, float circ, int rsqrt_policy, float *out,
-                                  int nthreads, uint32_t bz_begin, uint32_t bz_end) {
-    vgo_mac_velocity_grid_ex(t, g, pos, om, size, near_k, order, -1, spread, circ, rsqrt_policy,
-                             out, nthreads, bz_begin, bz_end);
+int vgo_mac_velocity_grid_ex(const vgo_tree *t, const vgo_grid *g, const float *pos,
+                             const float *om, const float *size, int near_k, int order,
+                             int expand_from, float spread, float circ, int rsqrt_policy,
+                             float *out, int nthreads, uint32_t bz_begin, uint32_t bz_end) {
+    return vgo_mac_velocity_grid_box(t, g, pos, om, size, near_k, order, expand_from, spread,
+                                     circ, rsqrt_policy, out, nthreads, 0, UINT32_MAX, bz_begin,
+                                     bz_end);
 }
 
-void vgo_mac_velocity_grid(const vgo_tree *t, const vgo_grid *g, const float *pos,
-                           const float *om, const float *size, int near_k, int order,
-                           float spread, float circ, int rsqrt_policy, float *out,
-                           int nthreads) {
-    vgo_mac_velocity_grid_bricks(t, g, pos, om, size, near_k, order, spread, circ, rsqrt_policy,
-                                 out, nthreads, 0, (g->np[2] + 3) / 4);
+int vgo_mac_velocity_grid_bricks(const vgo_tree *t, const vgo_grid *g, const float *pos,
+                                 const float *om, const float *size, int near_k, int order,
+                                 float spread, float circ, int rsqrt_policy, float *out,
+                                 int nthreads, uint32_t bz_begin, uint32_t bz_end) {
+    return vgo_mac_velocity_grid_ex(t, g, pos, om, size, near_k, order, -1, spread, circ,
+                                    rsqrt_policy, out, nthreads, bz_begin, bz_end);
+}
+
+int vgo_mac_velocity_grid(const vgo_tree *t, const vgo_grid *g, const float *pos,
+                          const float *om, const float *size, int near_k, int order,
+                          float spread, float circ, int rsqrt_policy, float *out,
+                          int nthreads) {
+    return vgo_mac_velocity_grid_bricks(t, g, pos, om, size, near_k, order, spread, circ,
+                                        rsqrt_policy, out, nthreads, 0, (g->np[2] + 3) / 4);
 }
 
 /* vortonSim.cpp:1650-1768 / 1787-1905: recursive traversal.  A cluster of
@@ -1491,6 +1517,35 @@ void vgo_brick_of_bytes(const vgo_grid *g, const float *field, unsigned char *pl
         stats[3 + a] = mx[a];
     }
     stats[6] = magmax;
+}
+
+/* FP64 true relative residual of the 7-point Dirichlet Laplacian over the
+ * interior: ||f - L u|| / ||f||, u = hi (+ lo when lo is not NULL, the
+ * compensated pair of a solver that carries its solution in two floats),
+ * each value summed in double. */
+double vgo_residual_rel_pair(const vgo_grid *g, const float *hi, const float *lo,
+                             const float *f) {
+    const double rx = 1.0 / g->cell[0], ry = 1.0 / g->cell[1], rz = 1.0 / g->cell[2];
+    const double r2[3] = {rx * rx, ry * ry, rz * rz};
+    const size_t nx = g->np[0], ny = g->np[1], nz = g->np[2], nxy = nx * ny;
+#define UP(o) ((double)hi[3 * (o) + c] + (lo ? (double)lo[3 * (o) + c] : 0.0))
+    double num = 0, den = 0;
+    for (size_t iz = 1; iz + 1 < nz; ++iz)
+        for (size_t iy = 1; iy + 1 < ny; ++iy)
+            for (size_t ix = 1; ix + 1 < nx; ++ix) {
+                const size_t o = ix + iy * nx + iz * nxy;
+                for (int c = 0; c < 3; ++c) {
+                    const double uc = UP(o);
+                    const double lap = (UP(o + 1) + UP(o - 1) - 2 * uc) * r2[0] +
+                                       (UP(o + nx) + UP(o - nx) - 2 * uc) * r2[1] +
+                                       (UP(o + nxy) + UP(o - nxy) - 2 * uc) * r2[2];
+                    const double res = (double)f[3 * o + c] - lap;
+                    num += res * res;
+                    den += (double)f[3 * o + c] * f[3 * o + c];
+                }
+            }
+#undef UP
+    return den > 0 ? sqrt(num / den) : sqrt(num);
 }
 
 double vgo_residual_rel(const vgo_grid *g, const float *u, const float *f) {

@@ -121,24 +121,39 @@ void vgo_tree_vecpot(const vgo_tree *t, const float q[3], float spread,
  * when order == 1); leaf cells inside the leaf near box are summed over
  * their vortons.  Per-lane sums follow the GPU order: layers from the top,
  * cells z,y,x, one Kahan-compensated partial per layer. */
-void vgo_mac_velocity_grid(const vgo_tree *t, const vgo_grid *g, const float *pos,
-                           const float *om, const float *size, int near_k,
-                           int order, float spread, float circ, int rsqrt_policy,
-                           float *out, int nthreads);
+int vgo_mac_velocity_grid(const vgo_tree *t, const vgo_grid *g, const float *pos,
+                          const float *om, const float *size, int near_k,
+                          int order, float spread, float circ, int rsqrt_policy,
+                          float *out, int nthreads);
 /* Same, restricted to z-brick rows [bz_begin, bz_end) (timing samples). */
-void vgo_mac_velocity_grid_bricks(const vgo_tree *t, const vgo_grid *g, const float *pos,
-                                  const float *om, const float *size, int near_k, int order,
-                                  float spread, float circ, int rsqrt_policy, float *out,
-                                  int nthreads, uint32_t bz_begin, uint32_t bz_end);
+int vgo_mac_velocity_grid_bricks(const vgo_tree *t, const vgo_grid *g, const float *pos,
+                                 const float *om, const float *size, int near_k, int order,
+                                 float spread, float circ, int rsqrt_policy, float *out,
+                                 int nthreads, uint32_t bz_begin, uint32_t bz_end);
 
 /* Same with clusters of layers >= expand_from applied through a local
  * expansion about each brick's centre (no reference counterpart; monopole
  * to second order, first moment to first order, float64 tensors);
- * expand_from < 1 or > depth-2 disables it. */
-void vgo_mac_velocity_grid_ex(const vgo_tree *t, const vgo_grid *g, const float *pos,
+ * expand_from < 1 or > depth-2 disables it.
+ * All three return 0, or -1 without touching out when the tree's grid leaves
+ * a cell without a parent cluster (vgo_nested_has_orphans): the walk from the
+ * upper layers would never reach that cell's vortons. */
+int vgo_mac_velocity_grid_ex(const vgo_tree *t, const vgo_grid *g, const float *pos,
+                             const float *om, const float *size, int near_k, int order,
+                             int expand_from, float spread, float circ, int rsqrt_policy,
+                             float *out, int nthreads, uint32_t bz_begin, uint32_t bz_end);
+/* Same on the bricks with x index in [bx_begin, bx_end) and z index in
+ * [bz_begin, bz_end) only (timing samples). */
+int vgo_mac_velocity_grid_box(const vgo_tree *t, const vgo_grid *g, const float *pos,
                               const float *om, const float *size, int near_k, int order,
                               int expand_from, float spread, float circ, int rsqrt_policy,
-                              float *out, int nthreads, uint32_t bz_begin, uint32_t bz_end);
+                              float *out, int nthreads, uint32_t bx_begin, uint32_t bx_end,
+                              uint32_t bz_begin, uint32_t bz_end);
+/* 1 when some layer l has more cells on an axis than dec[l] x its parent's
+ * (an odd count above 3 halved, e.g. 5 cells under 2 parents): the last
+ * cell then has no parent in AggregateClusters (vortonSim.cpp:1337-1569).
+ * Power-of-two grids, the only kind the reference builds, never do. */
+int vgo_nested_has_orphans(const vgo_nested *ng);
 
 /* The local expansion of one cluster about cb: out = c0[3], c1[3][3],
  * c2[3][3][3] (see vgo_mac_velocity_grid_ex). */
@@ -248,6 +263,9 @@ void vgo_brick_of_bytes(const vgo_grid *g, const float *field, unsigned char *pl
                         float stats[7]);
 /* FP64 residual ||f - L u|| / ||f|| over interior points, 7-point Laplacian. */
 double vgo_residual_rel(const vgo_grid *g, const float *u, const float *f);
+/* The same for the compensated pair u = hi + lo (lo may be NULL). */
+double vgo_residual_rel_pair(const vgo_grid *g, const float *hi, const float *lo,
+                             const float *f);
 
 /* ---- particle operators ---- */
 /* AssignParticleVelocityFromField_Slice (pclOpAdvect.cpp:42-81). */

@@ -22,10 +22,17 @@ def oracle():
 
 @pytest.fixture(scope="session")
 def vg():
-    """The HIP library; built here only when no built copy exists."""
+    """The HIP library, rebuilt first when it is missing or was built from
+    sources other than the ones in the tree (compared by content hash)."""
+    import time
+
     from vortegrid_amd import build, vg as V
 
-    if not os.path.exists(V.LIB_PATH):
+    if not build.is_current():
+        print(f"[vg] library missing or stale (built {build.built_hash()}, sources "
+              f"{build.source_hash()}): rebuilding", flush=True)
         build.build()
+    print(f"[vg] {V.LIB_PATH} built {time.ctime(os.path.getmtime(V.LIB_PATH))}, "
+          f"source hash {build.built_hash()}", flush=True)
     V.require_gpu()
     return V

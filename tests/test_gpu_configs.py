@@ -1,0 +1,261 @@
+"""The three large workloads at their own sizes, through the C-ABI:
+
+* the vector-Poisson solve: a 262,144-vorton jet plume on its 128-cell
+  grid (129 x 129 x 257 points), treecode Dirichlet faces against the
+  oracle, then V-cycles judged by the oracle's FP64 true residual;
+* batched effects: 16 effects with vorton counts log-uniform in
+  [512, 32768] on 32^3-cell grids, every grid against the oracle's
+  near-field walk, plus one full 256-effect frame checked against FP64
+  direct sums;
+* the 1M-vorton, 64-ring treecode frame at the default settings (near
+  radius 3, dipole clusters, expansions from layer 2, auto-mollified):
+  rel-L2 against the FP64 direct sum must stay at or below 1e-3.
+"""
+import numpy as np
+import pytest
+
+from vortegrid_amd import scenarios
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+DT = F32(1.0 / 30.0)
+
+
+def case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def to_oracle_grid(oracle, g):
+    og = oracle.Grid()
+    for k in ("min", "extent", "cell", "cpe", "np"):
+        for a in range(3):
+            getattr(og, k)[a] = getattr(g, k)[a]
+    return og
+
+
+def face_mask(shape):
+    nz, ny, nx = shape
+    m = np.zeros((nz, ny, nx), bool)
+    m[0] = m[-1] = True
+    m[:, 0] = m[:, -1] = True
+    m[:, :, 0] = m[:, :, -1] = True
+    return m.reshape(-1)
+
+
+def grid_points(g, idx):
+    """Query positions of grid point offsets exactly as the kernels form them."""
+    nudge = F32(1.0) - F32(2.0) * F32(np.finfo(F32).eps)
+    ix = idx % g.np[0]
+    iy = (idx // g.np[0]) % g.np[1]
+    iz = idx // (g.np[0] * g.np[1])
+    return np.stack([F32(g.min[a]) + np.asarray(i, F32) * (F32(g.cell[a]) * nudge)
+                     for a, i in enumerate((ix, iy, iz))], 1).astype(F32)
+
+
+# ------------------------------------------------------------ vector Poisson
+@pytest.fixture(scope="module")
+def plume(vg, oracle):
+    pos, om, radius = scenarios.jet_plume(262144, seed=3)
+    n = len(pos)
+    dp, dw = vg.DeviceArray.from_numpy(pos), vg.DeviceArray.from_numpy(om)
+    v = vg.vortons_view(dp, dw, n, uniform_size=2 * radius)
+    mn, mx = vg.find_bounding_box(dp, n)
+    g0 = vg.finalize_bounding_box(mn, mx, radius, n)
+    g = vg.define_shape(128 ** 3, g0.min, [g0.min[a] + g0.extent[a] for a in range(3)])
+    t = vg.Tree()
+    t.build(g, v)
+    og = to_oracle_grid(oracle, g)
+    ot = oracle.InfluenceTree(og, pos, om, 2 * radius)
+    return pos, om, radius, g, og, t, ot, (dp, dw, v)
+
+
+@pytest.mark.parametrize("copy_odd", [True, False])
+def test_plume_treecode_faces(vg, oracle, plume, copy_odd):
+    pos, om, radius, g, og, t, ot, keep = plume
+    case(f"plume N=262144 grid {g.shape}: treecode vector potential on the faces, "
+         f"copy_odd={copy_odd}")
+    assert g.shape == (129, 129, 257)
+    a0 = vg.DeviceArray.zeros((g.capacity, 3))
+    t.vecpot_boundary(g, a0, copy_odd=copy_odd)
+    vg.synchronize()
+    got = a0.numpy()
+    want = oracle.vecpot_boundary(ot, og, pos, om, 2 * radius, 1.0, 1.0, 0, copy_odd, threads=8)
+    faces = face_mask(g.shape[::-1])
+    assert not np.any(got[~faces]) and not np.any(want[~faces])
+    assert rel_l2(got[faces], want[faces]) < 1e-5
+
+
+def test_plume_vcycles_to_1e6(vg, oracle, plume):
+    """Splat (against the oracle), treecode faces, then V-cycles to 1e-6;
+    the oracle recomputes the FP64 true relative residual ||f - L u|| / ||f||
+    of the delivered solution u = a0 + lo on the CPU.  The finest level
+    (129 x 129 x 257) runs the fused LDS sweep at its full 16-plane depth."""
+    pos, om, radius, g, og, t, ot, keep = plume
+    case(f"plume N=262144 grid {g.shape}: splat, faces, V-cycles to 1e-6")
+    negw = vg.DeviceArray.zeros((g.capacity, 3))
+    t.splat_vorticity(radius, -1.0, negw)
+    a = vg.DeviceArray.zeros((g.capacity, 3))
+    t.vecpot_boundary(g, a)
+    vg.synchronize()
+    f = negw.numpy()
+    wf = oracle.splat_vorticity(og, pos, om, radius, -1.0)
+    assert rel_l2(f, wf) < 1e-6
+    faces_in = a.numpy()
+    mg = vg.Multigrid()
+    rel, cycles = mg.vcycle_solve(g, a, negw, tol=1e-6, max_cycles=40)
+    vg.synchronize()
+    hi = a.numpy()
+    lo = mg.compensation()
+    r_pair = oracle.residual_rel(og, hi, f, lo=lo)
+    r_hi = oracle.residual_rel(og, hi, f)
+    print(f"[result] V-cycles {cycles}, solver residual {rel:.3e}, oracle FP64 residual of "
+          f"a0+lo {r_pair:.3e}, of a0 alone {r_hi:.3e}", flush=True)
+    assert rel <= 1e-6 and cycles <= 20
+    assert r_pair <= 1e-6
+    assert abs(r_pair - rel) <= 0.05 * rel + 1e-9
+    # Dirichlet: the faces are kept bit for bit
+    faces = face_mask(g.shape[::-1])
+    assert np.array_equal(hi[faces], faces_in[faces])
+    assert np.all(np.isfinite(hi)) and np.all(np.isfinite(lo))
+
+
+# ------------------------------------------------------------ batched effects
+def effect_fields(n_effects, seed, tracers_per_vorton):
+    """Vorton counts log-uniform in [512, 32768], each a ring or a plume
+    from its own seed, placed on a lattice."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(np.log(512), np.log(32768), n_effects)).astype(np.int64)
+    vp, vw, tp, radii = [], [], [], []
+    for i, n in enumerate(counts):
+        gen = scenarios.vortex_ring if rng.random() < 0.5 else scenarios.jet_plume
+        pos, om, radius = gen(int(n), seed=1000 + i)
+        pos = pos + F32(8.0) * F32([i % 16, i // 16, 0])
+        vp.append(pos)
+        vw.append(om)
+        tp.append(scenarios.tracers_in_box(tracers_per_vorton * int(n), pos.min(0), pos.max(0),
+                                           seed=5000 + i))
+        radii.append(radius)
+    return vp, vw, tp, radii
+
+
+def make_batch(vg, vp, vw, tp, radii, **kw):
+    b = vg.Batch([len(p) for p in vp], [len(t) for t in tp], radii, **kw)
+    b.upload(np.concatenate(vp), np.concatenate(vw), np.concatenate(tp))
+    return b
+
+
+def test_batch_16_uneven_effects_vs_oracle(vg, oracle):
+    """Every effect's grid after one frame equals the oracle's near-field
+    walk (near radius 2, dipole, expansions from layer 2) on that effect's
+    own vortons and geometry."""
+    vp, vw, tp, radii = effect_fields(16, seed=21, tracers_per_vorton=2)
+    counts = [len(p) for p in vp]
+    case(f"batch of 16 effects, 32^3 cells, counts {counts}")
+    assert min(counts) >= 512 and max(counts) <= 32768
+    b = make_batch(vg, vp, vw, tp, radii, cells=32, near_k=2, order=1, policy=1, expand_from=2)
+    b.step(DT)
+    vg.synchronize()
+    worst = 0.0
+    for e in range(len(vp)):
+        g, field = b.grid(e)
+        assert list(g.np) == [33, 33, 33]
+        spread, circ = b.mollify(e)
+        og = to_oracle_grid(oracle, g)
+        ot = oracle.InfluenceTree(og, vp[e], vw[e], 2 * radii[e])
+        want = oracle.mac_velocity_grid_ex(ot, og, vp[e], vw[e], 2 * radii[e], 2, 1, 2, spread,
+                                           circ, 1, threads=8)
+        err = rel_l2(field, want)
+        worst = max(worst, err)
+        assert err < 1e-5, (e, counts[e], err)
+    print(f"[result] worst effect rel-L2 vs oracle {worst:.2e}", flush=True)
+
+
+def test_batch_256_effects_full_frame(vg):
+    """The full batched workload (256 effects, ~2M vortons, 16 tracers per
+    vorton): a recorded frame replayed 3 times stays finite, and each
+    effect's grid agrees with the FP64 direct sum on a sample of points."""
+    vp, vw, tp, radii = effect_fields(256, seed=4, tracers_per_vorton=16)
+    nv, nt = sum(len(p) for p in vp), sum(len(t) for t in tp)
+    case(f"batch of 256 effects, {nv} vortons, {nt} tracers, 32^3 cells, k=2 + expansions")
+    b = make_batch(vg, vp, vw, tp, radii, cells=32, near_k=2, order=1, policy=1, expand_from=2)
+    s = vg.Stream()
+    b.step(DT, stream=s)
+    b.capture(DT, s)
+    for _ in range(3):
+        b.launch(s)
+    s.sync()
+    vpos = b.download("vpos")
+    tpos = b.download("tpos")
+    assert np.all(np.isfinite(vpos)) and np.all(np.isfinite(tpos))
+    vo = np.cumsum([0] + [len(p) for p in vp])
+    # the grids of the last replay belong to the vortons before its Euler
+    # step: reconstruct them from the stored velocities
+    vvel = b.download("vvel")
+    errs = []
+    rng = np.random.default_rng(3)
+    for e in range(len(vp)):
+        g, field = b.grid(e)
+        assert np.all(np.isfinite(field))
+        spread, circ = b.mollify(e)
+        src = (vpos[vo[e]:vo[e + 1]] - DT * vvel[vo[e]:vo[e + 1]]).astype(F32)
+        idx = np.sort(rng.choice(g.capacity, 256, replace=False))
+        q = grid_points(g, idx)
+        dsrc = vg.DeviceArray.from_numpy(src)
+        dom = vg.DeviceArray.from_numpy(vw[e])
+        v = vg.vortons_view(dsrc, dom, len(src), uniform_size=2 * radii[e])
+        dq = vg.DeviceArray.from_numpy(q)
+        ref = vg.DeviceArray.zeros((len(q), 3), np.float64)
+        vg.direct_velocity_f64(v, dq, len(q), ref, spread, circ)
+        vg.synchronize()
+        errs.append(rel_l2(field[idx], ref.numpy()))
+    errs = np.array(errs)
+    print(f"[result] per-effect rel-L2 vs FP64 direct: median {np.median(errs):.2e}, "
+          f"90th percentile {np.percentile(errs, 90):.2e}, max {errs.max():.2e}", flush=True)
+    assert np.median(errs) < 2e-2 and errs.max() < 8e-2
+
+
+# ------------------------------------------------------------ headline accuracy
+def test_default_frame_accuracy_1m(vg):
+    """1,048,576 vortons in 64 random rings (seed 5), the frame's grid from
+    UpdateBoundingBox + DefineShape (129^3), the default near-field
+    treecode: rel-L2 against the FP64 direct sum on a seeded 1% sample of
+    grid points is at most 1e-3, on the first frame and after 10 frames of
+    vorton motion."""
+    pos, om, radius = scenarios.random_rings(1 << 20, seed=5)
+    n = len(pos)
+    case(f"default frame N={n}: near_k=3, dipole, expansions from layer 2, vs FP64 direct")
+    s = vg.Stream()
+    dp, dw = vg.DeviceArray.from_numpy(pos), vg.DeviceArray.from_numpy(om)
+    dv = vg.DeviceArray.zeros(pos.shape)
+    v = vg.vortons_view(dp, dw, n, uniform_size=2 * radius)
+    sim = vg.Sim(vg.sim_config(technique=vg.TECH_TREE, rsqrt_policy=1, tree_near_k=3,
+                               tree_order=1, tree_expand_from=2), s)
+    sim.set_vortons(v)
+    mn, mx = pos.min(0), pos.max(0)
+    errs = []
+    for frame in range(11):
+        sim.find_bounding_box()
+        sim.update_bounding_box(mn, mx, True)
+        sim.update(float(DT), frame, dv)
+        if frame in (0, 10):
+            s.sync()
+            g, gptr = sim.velocity_grid()
+            spread, circ = sim.spreading()
+            cap = g.capacity
+            idx = np.sort(np.random.default_rng(1234).choice(cap, cap // 100, replace=False))
+            got = vg.fetch(gptr, 3 * cap).reshape(-1, 3)[idx]
+            dq = vg.DeviceArray.from_numpy(grid_points(g, idx))
+            ref = vg.DeviceArray.zeros((len(idx), 3), np.float64)
+            vg.direct_velocity_f64(v, dq, len(idx), ref, spread, circ, stream=s)
+            s.sync()
+            errs.append(rel_l2(got, ref.numpy()))
+            print(f"[result] frame {frame} grid {g.shape}: rel-L2 {errs[-1]:.4e}", flush=True)
+            assert g.shape == (129, 129, 129)
+        vg.evolve(dp, dv, n, float(DT), stream=s)
+    assert max(errs) <= 1e-3, errs

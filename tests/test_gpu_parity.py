@@ -457,20 +457,33 @@ def test_mac_local_expansions(vg, oracle, rings2, k, order, expand_from):
         assert e > 0 and c < c0
 
 
-@pytest.mark.parametrize("cells,k,expand_from", [(16384, 2, -1), (16384, 2, 2), (25000, 3, 2)])
-def test_mac_velocity_grid_odd_shapes(vg, oracle, rings2, cells, k, expand_from):
-    """Grids whose cell counts are not powers of two: the NestedGrid
-    decimation is then 3 or 1 on some axes (3 cells under one parent, or a
-    2-point axis kept), and odd layers leave their last cell without a
-    parent.  The walk's per-parent child masks must give the same clusters
-    as the oracle's children box."""
-    pos, om, radius, _ = rings2
+def shaped_grid(oracle, pos, radius, cells):
+    """A grid of exactly cells[a] cells per axis over the padded vorton box
+    (extents proportional to the cell counts, one spacing h)."""
     mn, mx = oracle.find_bbox(pos)
-    og = oracle.define_shape(cells, mn, mx, pow2=False)
+    mn = mn - 3 * radius
+    ext = (mx + 3 * radius) - mn
+    h = max(float(ext[a]) / cells[a] for a in range(3))
+    og = oracle.define_shape(int(np.prod(cells)), mn, mn + h * np.array(cells, np.float64),
+                             pow2=False)
+    assert og.shape == tuple(c + 1 for c in cells)
+    return og
+
+
+@pytest.mark.parametrize("cells,k,expand_from", [((48, 24, 12), 2, -1), ((48, 24, 12), 2, 2),
+                                                 ((24, 24, 24), 3, 2), ((12, 24, 48), 2, 2)])
+def test_mac_velocity_grid_odd_shapes(vg, oracle, rings2, cells, k, expand_from):
+    """Grids whose cell counts are not powers of two but leave no cell
+    without a parent: the NestedGrid decimation is then 3 or 1 on some axes
+    (3 cells under one parent, or a 2-point axis kept).  The walk's
+    per-parent child masks must give the same clusters as the oracle's
+    children box."""
+    pos, om, radius, _ = rings2
+    og = shaped_grid(oracle, pos, radius, cells)
     ng = oracle.nested(og)
     decs = {int(ng.dec[l][a]) for l in range(1, ng.depth) for a in range(3)}
     case(f"mac velocity grid, grid {og.shape}, decimations {sorted(decs)}, k={k} from={expand_from}")
-    assert 3 in decs
+    assert 3 in decs and not oracle.nested_has_orphans(og)
     v, keep = vortons(vg, pos, om, radius)
     g = vg.GridDesc.from_fields(og)
     t = vg.Tree()
@@ -488,6 +501,32 @@ def test_mac_velocity_grid_odd_shapes(vg, oracle, rings2, cells, k, expand_from)
                                        circ, 1, threads=8)
     assert np.isfinite(out.numpy()).all()
     assert rel_l2(out.numpy(), want) < 1e-5
+
+
+@pytest.mark.parametrize("cells", [(20, 20, 20), (36, 36, 36), (40, 20, 10)])
+def test_mac_refuses_grids_with_orphan_cells(vg, oracle, rings2, cells):
+    """20 -> 10 -> 5 -> 2 leaves the fifth cell of the 5-cell layer without a
+    parent cluster, so no walk from the top would reach its vortons: the
+    near-field walk, its work count and the batch refuse such shapes.  The
+    reference-rule treecode keeps the reference's own behaviour there."""
+    pos, om, radius, _ = rings2
+    og = shaped_grid(oracle, pos, radius, cells)
+    case(f"mac refuses orphan grid {og.shape}")
+    assert oracle.nested_has_orphans(og)
+    v, keep = vortons(vg, pos, om, radius)
+    g = vg.GridDesc.from_fields(og)
+    t = vg.Tree()
+    t.build(g, v)
+    out = vg.DeviceArray.zeros((g.capacity, 3))
+    with pytest.raises(vg.VGError, match="parent"):
+        t.velocity_grid_mac(g, out, 2, 1, 1.0, 1.0, 1)
+    with pytest.raises(vg.VGError, match="parent"):
+        t.mac_work(g, 2, -1)
+    with pytest.raises(vg.VGError, match="parent"):
+        vg.Batch([100], [0], [0.05], cells=cells[0])
+    # the reference rule still runs (it reproduces the reference's layers)
+    t.velocity_grid(g, out, 1.0, 1.0, 1)
+    vg.synchronize()
 
 
 @pytest.mark.parametrize("k,expand_from", [(2, -1), (2, 2), (3, 2)])

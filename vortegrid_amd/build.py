@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import argparse
 import concurrent.futures as cf
+import hashlib
 import os
 import shutil
 import subprocess
@@ -42,6 +43,36 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
+HASH_PATH = LIB + ".srchash"
+HEADER = os.path.join(os.path.dirname(HERE), "include", "vortegrid.h")
+
+
+def source_hash() -> str:
+    """SHA-256 over every file the library is built from (csrc/ and the
+    public header), so a stale library can be told from a current one
+    without trusting file times."""
+    h = hashlib.sha256()
+    files = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC)) + [HEADER]
+    for path in files:
+        if os.path.isfile(path):
+            h.update(os.path.basename(path).encode())
+            with open(path, "rb") as fh:
+                h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def built_hash():
+    """The source hash recorded when the library was last built, or None."""
+    if not (os.path.exists(LIB) and os.path.exists(HASH_PATH)):
+        return None
+    with open(HASH_PATH) as fh:
+        return fh.read().strip()
+
+
+def is_current() -> bool:
+    return built_hash() == source_hash()
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.basename(src)[:-4] + ".o")
     cmd = [HIPCC, *CFLAGS, "-c", src, "-o", obj]
@@ -53,6 +84,7 @@ def _compile(src: str) -> str:
 
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
+    digest = source_hash()
     srcs = sources()
     jobs = min(len(srcs), int(os.environ.get("MAX_JOBS", "8")))
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
@@ -63,6 +95,8 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    with open(HASH_PATH, "w") as fh:
+        fh.write(digest + "\n")
     if os.path.exists(FACADE_SRC):
         build_facade_driver()
     if verbose:

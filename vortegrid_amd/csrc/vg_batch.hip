@@ -384,6 +384,11 @@ VG_API vg_status vg_batch_create(vg_batch **out, uint32_t effects, const uint32_
         tmpl.cpe[a] = (float)c.cells;
     }
     nested_init(&b->shape, tmpl);
+    if (has_orphan_cells(b->shape))
+        return fail((set_error("cells leave a cell without a parent cluster on some tree layer "
+                               "(an odd count above 3 after halving): use a power of two, or "
+                               "3 * 2^k"),
+                     VG_EINVAL));
     b->cap0 = capacity(b->shape.layer[0]);
     std::vector<uint32_t> vt, tt;
     std::vector<float> rad(effects), sz(effects);

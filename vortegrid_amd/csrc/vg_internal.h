@@ -88,6 +88,22 @@ struct Nested {
 };
 void nested_init(Nested *ng, const vg_grid_desc &src);
 
+// True when a layer leaves a cell without a parent: on some axis the child
+// layer has more cells than decimation x parent cells (an odd count above 3
+// halved, e.g. 5 -> 2 parents covering 4).  AggregateClusters
+// (vortonSim.cpp:1337-1569) never folds such a cell into the layer above,
+// so a walk from the upper layers cannot reach its vortons.  The reference
+// only builds power-of-two grids (UpdateBoundingBox calls DefineShape with
+// bPowerOf2 = true, vortonSim.cpp:1089), where this cannot happen; the
+// near-field walks refuse such trees instead of dropping vorticity.
+template <class NG>
+inline bool has_orphan_cells(const NG &ng) {
+    for (uint32_t l = 1; l < ng.depth; ++l)
+        for (int a = 0; a < 3; ++a)
+            if (ng.layer[l - 1].np[a] - 1 != ng.dec[l][a] * (ng.layer[l].np[a] - 1)) return true;
+    return false;
+}
+
 // ------------------------------------------------------------------ device view of a tree
 struct TreeView {
     uint32_t depth;

@@ -826,6 +826,9 @@ vg_status run_mac(const vg_tree *t, const vg_grid_desc *g, int near_k, int order
     for (int a = 0; a < 3; ++a)
         VG_REQUIRE(g->np[a] == tv.layer[0].np[a],
                    "grid must have the tree's leaf geometry (the grid template)");
+    VG_REQUIRE(!has_orphan_cells(tv),
+               "the tree's grid leaves cells without a parent cluster (a cell count that is odd "
+               "and above 3 on some layer); define the grid with power-of-two cells");
     MacParams mp;
     mp.k = near_k;
     mp.spread = spread;

@@ -1093,6 +1093,9 @@ VG_API vg_status vg_tree_mlp_samples(const vg_tree *t, const vg_grid_desc *g, in
     for (int a = 0; a < 3; ++a)
         VG_REQUIRE(g->np[a] == tv.layer[0].np[a], "grid must have the tree's leaf geometry");
     VG_REQUIRE(tv.depth >= 2, "the tree has no cluster layers");
+    VG_REQUIRE(!has_orphan_cells(tv),
+               "the tree's grid leaves cells without a parent cluster; define the grid with "
+               "power-of-two cells");
     const uint32_t valid = (tv.depth - 1 >= 32) ? 0xffffffffu : ((1u << (tv.depth - 1)) - 1u);
     SampleParams sp;
     sp.layer_mask = layer_mask & valid;

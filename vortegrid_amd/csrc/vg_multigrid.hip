@@ -42,6 +42,7 @@ struct vg_multigrid {
     size_t vcap[vg::kMaxLayers] = {};
     float *lo = nullptr;       // compensation term of the finest solution
     size_t lo_cap = 0;
+    size_t lo_valid = 0;       // points of the last solve's finest grid
     double *part = nullptr;    // FP64 norm partials
 };
 
@@ -574,5 +575,15 @@ VG_API vg_status vg_mg_vcycle_solve(vg_multigrid *m, const vg_grid_desc *g, floa
     }
     if (rel_residual) *rel_residual = rel;
     if (cycles) *cycles = it;
+    m->lo_valid = cap0;
+    return VG_OK;
+}
+
+VG_API vg_status vg_mg_vcycle_compensation(const vg_multigrid *m, const float **lo,
+                                           size_t *count) {
+    VG_REQUIRE(m && lo, "null argument");
+    VG_REQUIRE(m->lo && m->lo_valid, "no V-cycle solve has run on this object");
+    *lo = m->lo;
+    if (count) *count = 3 * m->lo_valid;
     return VG_OK;
 }

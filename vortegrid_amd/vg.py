@@ -31,7 +31,8 @@ EXPORTS = (
     "vg_tree_vecpot_boundary "
     "vg_tree_splat_vorticity vg_tree_populate_velocity_grid vg_downsample_inject "
     "vg_upsample_interior vg_sor_solve vg_curl vg_residual_norm vg_mg_create vg_mg_destroy "
-    "vg_mg_cascade vg_mg_vcycle_solve vg_assign_velocity_from_field vg_evolve "
+    "vg_mg_cascade vg_mg_vcycle_solve vg_mg_vcycle_compensation vg_assign_velocity_from_field "
+    "vg_evolve "
     "vg_advect_tracers vg_sim_config_default vg_sim_create vg_sim_destroy vg_sim_set_vortons "
     "vg_sim_find_bounding_box vg_sim_update_bounding_box vg_sim_update vg_sim_velocity_grid "
     "vg_sim_vector_potential vg_sim_negative_vorticity vg_sim_grid_template vg_sim_bounds "
@@ -480,6 +481,12 @@ class Multigrid:
         call("vg_mg_vcycle_solve", self.h, C.byref(g), a0.ptr, negw.ptr, C.c_double(tol),
              C.c_uint32(max_cycles), C.byref(rel), C.byref(cyc), _s(stream))
         return rel.value, cyc.value
+
+    def compensation(self) -> np.ndarray:
+        """The FP32 compensation term lo of the last V-cycle solution a0 + lo."""
+        p, n = C.c_void_p(), C.c_size_t()
+        call("vg_mg_vcycle_compensation", self.h, C.byref(p), C.byref(n))
+        return fetch(p, n.value).reshape(-1, 3)
 
     def __del__(self):
         try:
