@@ -278,6 +278,15 @@ vg_status vg_mg_vcycle_solve(vg_multigrid *m, const vg_grid_desc *g, float *a0,
  * until the next solve on m.  No reference counterpart (the reference's
  * cascade, vortonSim.cpp:2523-2571, never converges to a tolerance). */
 vg_status vg_mg_vcycle_compensation(const vg_multigrid *m, const float **lo, size_t *count);
+/* The cascade's pyramid (NestedGrid<Vec3>, vortonSim.h:338-343): layer l >= 1
+ * of the vector potential (A) or of the negative vorticity (-w) as the last
+ * vg_mg_cascade left it (device pointer, packed Vec3, valid until the next
+ * cascade on m; layer 0 is the caller's a0 / negw0).  depth (may be NULL)
+ * receives the number of layers that cascade filled: 1 + the coarse layers
+ * it visited (those with min dimension > 2, vortonSim.cpp:2549-2561). */
+enum { VG_MG_VECTOR_POTENTIAL = 0, VG_MG_NEGATIVE_VORTICITY = 1 };
+vg_status vg_mg_layer(const vg_multigrid *m, int field, uint32_t layer, vg_grid_desc *g,
+                      const float **data, uint32_t *depth);
 
 /* ------------------------------------------------------------ particles */
 /* AssignVelocityFromField (pclOpAdvect.cpp:132-152, slice :42-81): vel =
@@ -552,6 +561,13 @@ vg_status vg_sim_stretch_tilt(vg_sim *sim, float dt, float *angvel,
 vg_status vg_sim_pse_diffuse(vg_sim *sim, float viscosity, float dt, float *angvel,
                              size_t ang_stride, const int32_t *birth,
                              size_t birth_stride, int order);
+/* GetVectorPotentialMultiGrid()[layer] / GetNegativeVorticityMultiGrid()[layer]
+ * (vortonSim.h:338-343) after a POISSON update: layer 0 is the full-size
+ * field, layers >= 1 the cascade's coarse layers (the V-cycle solver keeps
+ * only layer 0).  depth (may be NULL) receives the layer count (0 before a
+ * POISSON update).  Device pointers, valid until the next update. */
+vg_status vg_sim_multigrid(const vg_sim *sim, int field, uint32_t layer, vg_grid_desc *g,
+                           const float **data, uint32_t *depth);
 /* GetVectorPotentialMultiGrid()[0] / GetNegativeVorticityMultiGrid()[0]. */
 vg_status vg_sim_vector_potential(const vg_sim *sim, vg_grid_desc *g,
                                   const float **field);

@@ -227,11 +227,16 @@ def find_bbox(pos):
 
 
 def finalize_bbox(mn, mx, vorton_radius: float, n_vortons: int) -> Grid:
+    return finalize_bbox_corners(mn, mx, vorton_radius, n_vortons)[0]
+
+
+def finalize_bbox_corners(mn, mx, vorton_radius: float, n_vortons: int):
+    """UpdateBoundingBox(bFinal): (grid template, padded min, padded max)."""
     mn = _f32(mn).copy()
     mx = _f32(mx).copy()
     g = Grid()
     lib().vgo_finalize_bbox(mn, mx, vorton_radius, n_vortons, C.byref(g))
-    return g
+    return g, mn, mx
 
 
 def grid_for(pos, vorton_radius: float, extra_pos=None) -> Grid:

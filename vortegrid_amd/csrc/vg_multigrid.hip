@@ -33,6 +33,7 @@ struct vg_multigrid {
     float *A[vg::kMaxLayers] = {};
     float *W[vg::kMaxLayers] = {};
     size_t cap[vg::kMaxLayers] = {};
+    uint32_t cascade_layers = 0;  // layers holding values after the last cascade
     // V-cycle hierarchy
     uint32_t vdepth = 0;
     vg_grid_desc vg[vg::kMaxLayers];
@@ -510,6 +511,24 @@ VG_API vg_status vg_mg_cascade(vg_multigrid *m, const vg_grid_desc *g, float *a0
         VG_TRY(prolong_interior(ng.layer[l - 1], A[l - 1], ng.layer[l], A[l], s));
         VG_TRY(sor_solve(ng.layer[l - 1], A[l - 1], W[l - 1], steps, s));
     }
+    m->cascade_layers = max_valid + 1;
+    return VG_OK;
+}
+
+namespace vg {
+uint32_t mg_cascade_depth(const vg_multigrid *m) { return m ? m->cascade_layers : 0; }
+}  // namespace vg
+
+VG_API vg_status vg_mg_layer(const vg_multigrid *m, int field, uint32_t layer, vg_grid_desc *g,
+                             const float **data, uint32_t *depth) {
+    VG_REQUIRE(m && g && data, "null argument");
+    VG_REQUIRE(field == VG_MG_VECTOR_POTENTIAL || field == VG_MG_NEGATIVE_VORTICITY,
+               "field must be VG_MG_VECTOR_POTENTIAL or VG_MG_NEGATIVE_VORTICITY");
+    if (depth) *depth = m->cascade_layers;
+    VG_REQUIRE(layer >= 1 && layer < m->cascade_layers,
+               "layer must be a coarse layer the last cascade filled (1 <= layer < depth)");
+    *g = m->ng.layer[layer];
+    *data = field == VG_MG_VECTOR_POTENTIAL ? m->A[layer] : m->W[layer];
     return VG_OK;
 }
 

@@ -27,6 +27,7 @@ vg_status stretch_tilt_fused(const vg_grid_desc &g, const float *vel, const floa
                              size_t pos_stride, float *angvel, size_t ang_stride, size_t n,
                              float dt, const uint32_t *order, hipStream_t s);
 vg_status tree_checked_view(const vg_tree *t, TreeView *v);
+uint32_t mg_cascade_depth(const vg_multigrid *m);
 vg_status pse_diffuse(const vg_tree *t, float *angvel, size_t ang_stride, const int32_t *birth,
                       size_t birth_stride, float viscosity, float dt, int order, hipStream_t s);
 }  // namespace vg
@@ -294,6 +295,27 @@ VG_API vg_status vg_sim_vector_potential(const vg_sim *sim, vg_grid_desc *g, con
     *g = sim->tmpl;
     *field = sim->a0;
     return VG_OK;
+}
+
+VG_API vg_status vg_sim_multigrid(const vg_sim *sim, int field, uint32_t layer,
+                                  vg_grid_desc *g, const float **data, uint32_t *depth) {
+    VG_REQUIRE(sim && g && data, "null argument");
+    VG_REQUIRE(field == VG_MG_VECTOR_POTENTIAL || field == VG_MG_NEGATIVE_VORTICITY,
+               "field must be VG_MG_VECTOR_POTENTIAL or VG_MG_NEGATIVE_VORTICITY");
+    uint32_t d = 0;
+    if (sim->cfg.technique == VG_TECH_POISSON && sim->a0) {
+        d = 1;
+        if (sim->cfg.poisson_solver == 0 && mg_cascade_depth(sim->mg) > 1)
+            d = mg_cascade_depth(sim->mg);
+    }
+    if (depth) *depth = d;
+    VG_REQUIRE(layer < d, "layer beyond the pyramid the last update filled");
+    if (layer == 0) {
+        *g = sim->tmpl;
+        *data = field == VG_MG_VECTOR_POTENTIAL ? sim->a0 : sim->negw;
+        return VG_OK;
+    }
+    return vg_mg_layer(sim->mg, field, layer, g, data, nullptr);
 }
 
 VG_API vg_status vg_sim_negative_vorticity(const vg_sim *sim, vg_grid_desc *g,

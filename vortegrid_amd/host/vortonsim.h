@@ -109,36 +109,124 @@ class UniformGridGeometry {
     vg_grid_desc g_;
 };
 
-// UniformGrid<T> (uniformGrid.h:862-1785), host copy of a device grid.
+// uniformGrid.h:87-95
+struct Stats_Float {
+    float mMean = 0.0f;
+    float mStdDev = 0.0f;
+    float mMin = FLT_MAX;
+    float mMax = -FLT_MAX;
+};
+
+// UniformGrid<T> (uniformGrid.h:862-1785).  The values live either on the
+// host (contents_) or, for grids a VortonSim fills, on the device: such a
+// grid is "attached" to the device field and copies it to the host only on
+// the first host access.  A non-const host access detaches it (the host copy
+// may then be changed), so operators that read the grid see the host values.
 template <class ItemT>
 class UniformGrid : public UniformGridGeometry {
   public:
     UniformGrid() = default;
     explicit UniformGrid(const UniformGridGeometry &geom) : UniformGridGeometry(geom) {}
-    void Init() { contents_.assign(GetGridCapacity(), ItemT()); }
-    size_t Size() const { return contents_.size(); }
-    bool Empty() const { return contents_.empty(); }
-    ItemT &operator[](size_t o) { return contents_[o]; }
-    const ItemT &operator[](size_t o) const { return contents_[o]; }
-    ItemT *data() { return contents_.data(); }
-    // Copy a packed device field of this geometry into the host grid.
-    void Download(const vg_grid_desc &g, const void *device) {
-        g_ = g;
-        Init();
-        if (!contents_.empty())
-            check(vg_memcpy_dtoh(contents_.data(), device, contents_.size() * sizeof(ItemT)),
-                  "UniformGrid download");
+    // uniformGrid.h:1044-1060
+    void Init(const ItemT &initialValue = ItemT()) {
+        device_ = nullptr;
+        contents_.assign(GetGridCapacity(), initialValue);
+        host_valid_ = true;
     }
+    // uniformGrid.h:1721-1725: drop the values and the shape
+    void Clear() {
+        device_ = nullptr;
+        contents_.clear();
+        host_valid_ = true;
+        std::memset(&g_, 0, sizeof g_);
+    }
+    void CopyShape(const UniformGridGeometry &geom) { g_ = geom.desc(); }
+    size_t Size() const { return device_ ? (size_t)GetGridCapacity() : contents_.size(); }
+    bool Empty() const { return Size() == 0; }
+    const ItemT &operator[](size_t o) const {
+        materialise();
+        return contents_[o];
+    }
+    ItemT &operator[](size_t o) {
+        materialise();
+        device_ = nullptr;
+        return contents_[o];
+    }
+    const ItemT *data() const {
+        materialise();
+        return contents_.data();
+    }
+    ItemT *data() {
+        materialise();
+        device_ = nullptr;
+        return contents_.data();
+    }
+    // Copy a packed device field of this geometry into the host grid now.
+    void Download(const vg_grid_desc &g, const void *device) {
+        AttachDevice(g, device);
+        materialise();
+    }
+    // Take the geometry and refer to a device field (packed, x fastest);
+    // the host copy is made on the first host access.
+    void AttachDevice(const vg_grid_desc &g, const void *device) {
+        g_ = g;
+        device_ = device;
+        host_valid_ = device == nullptr;
+        if (!device) contents_.assign(GetGridCapacity(), ItemT());
+    }
+    // The device field this grid refers to, or null when its values are the
+    // host copy's.
+    const void *DeviceData() const { return device_; }
     // Wrap host values of the given geometry.
     void Assign(const vg_grid_desc &g, const ItemT *values) {
         g_ = g;
+        device_ = nullptr;
         contents_.assign(values, values + GetGridCapacity());
+        host_valid_ = true;
     }
     // uniformGridDiagnostics.cpp:209-314 (Vec3 grids only): OGLE brick of bytes.
     void GenerateBrickOfBytes(const char *strFilenameBase, unsigned uFrame) const;
 
   private:
-    std::vector<ItemT> contents_;
+    void materialise() const {
+        if (host_valid_) return;
+        contents_.resize(GetGridCapacity());
+        if (!contents_.empty())
+            check(vg_memcpy_dtoh(contents_.data(), device_, contents_.size() * sizeof(ItemT)),
+                  "UniformGrid download");
+        host_valid_ = true;
+    }
+
+    mutable std::vector<ItemT> contents_;
+    mutable bool host_valid_ = true;
+    const void *device_ = nullptr;
+};
+
+// NestedGrid<T> (nestedGrid.h:31-393): a pyramid of grids, layer 0 finest.
+template <class ItemT>
+class NestedGrid {
+  public:
+    size_t GetDepth() const { return layers_.size(); }
+    UniformGrid<ItemT> &operator[](size_t l) { return layers_[l]; }
+    const UniformGrid<ItemT> &operator[](size_t l) const { return layers_[l]; }
+    // nestedGrid.h:142-146: decimations of layer l relative to layer l-1
+    const unsigned *GetDecimations(size_t l) const { return dec_[l].d; }
+    void Clear() {
+        layers_.clear();
+        dec_.clear();
+    }
+    void Resize(size_t depth) {
+        layers_.resize(depth);
+        dec_.resize(depth);
+    }
+    void SetDecimations(size_t l, const unsigned d[3]) {
+        for (int a = 0; a < 3; ++a) dec_[l].d[a] = d[a];
+    }
+
+  private:
+    struct Dec { unsigned d[3] = {0, 0, 0}; };
+    std::vector<UniformGrid<ItemT>> layers_;
+    std::vector<Dec> dec_;
 };
 
 // uniformGridDiagnostics.cpp:209-314 with ENFORCE_SYMMETRIC_RANGE off, as the
@@ -152,6 +240,7 @@ class UniformGrid : public UniformGridGeometry {
 template <>
 inline void UniformGrid<Vec3>::GenerateBrickOfBytes(const char *strFilenameBase,
                                                     unsigned uFrame) const {
+    materialise();
     if (contents_.empty()) return;
     Vec3 vMin = contents_[0], vMax = vMin;
     for (const Vec3 &r : contents_) {
@@ -268,19 +357,14 @@ class VortonSim {
         vg_sim_config_default(&cfg_);
         cfg_.technique = technique;
     }
-    VortonSim(const VortonSim &that)  // copies drop the vortons (vortonSim.h:286-295)
-        : cfg_(that.cfg_), viscosity_(that.viscosity_), density_(that.density_),
-          fluidSimTechnique_(that.fluidSimTechnique_),
-          investigationTerm_(that.investigationTerm_), pseOrder_(that.pseOrder_) {}
+    // Copies keep the settings and drop the vortons and the device state
+    // (vortonSim.h:281-295).
+    VortonSim(const VortonSim &that) { copy_settings(that); }
     VortonSim &operator=(const VortonSim &that) {
         if (this != &that) {
-            cfg_ = that.cfg_;
-            viscosity_ = that.viscosity_;
-            density_ = that.density_;
-            fluidSimTechnique_ = that.fluidSimTechnique_;
-            investigationTerm_ = that.investigationTerm_;
-            pseOrder_ = that.pseOrder_;
-            mVortons = nullptr;
+            vg_sim_destroy(sim_);
+            sim_ = nullptr;
+            copy_settings(that);
         }
         return *this;
     }
@@ -299,6 +383,18 @@ class VortonSim {
     // No reference counterpart: VG_PSE_REFERENCE_ORDER (default, the
     // reference's result bit for bit) or VG_PSE_SIMULTANEOUS (parallel).
     void SetPseOrder(int order) { pseOrder_ = order; }
+
+    // vortonSim.h:346-375.  Viscosity feeds PSE diffusion; the ambient
+    // density, gravity and heat capacity belong to the baroclinic and thermal
+    // terms, which are not on this path, and are only kept.
+    void SetViscosity(float viscosity) { viscosity_ = viscosity; }
+    const float &GetViscosity() const { return viscosity_; }
+    void SetAmbientDensity(float d) { density_ = d; }
+    const float &GetAmbientDensity() const { return density_; }
+    void SetSpecificHeatCapacity(float c) { heatCapacity_ = c; }
+    const float &GetSpecificHeatCapacity() const { return heatCapacity_; }
+    void SetGravitationalAcceleration(const Vec3 &g) { gravity_ = g; }
+    const Vec3 &GetGravitationalAcceleration() { return gravity_; }
 
     void SetVortons(VECTOR<Vorton> *vortons) { mVortons = vortons; }
     VECTOR<Vorton> *GetVortons() { return mVortons; }
@@ -342,6 +438,8 @@ class VortonSim {
     // (:4208-4215), StretchAndTiltVortons then updates their angular
     // velocities from the same velocity grid, and DiffuseAndDissipateVorticityPSE
     // (:4238-4241) exchanges vorticity between neighbours with the viscosity.
+    // The velocity grid object (GetVelocityGrid) keeps its address and refers
+    // to the new device field afterwards.
     void Update(float timeStep, unsigned uFrame) {
         ensure();
         if (!mVortons || mVortons->empty()) return;
@@ -363,45 +461,102 @@ class VortonSim {
                   "DiffuseAndDissipateVorticityPSE");
         check(vg_memcpy_dtoh(mVortons->data(), dev_.get(), mVortons->size() * sizeof(Vorton)),
               "download vortons");
-        vel_stale_ = pot_stale_ = negw_stale_ = true;
+        vg_grid_desc g;
+        const float *f = nullptr;
+        check(vg_sim_velocity_grid(sim_, &g, &f), "GetVelocityGrid");
+        mVelGrid.AttachDevice(g, f);
+        multigrids_stale_ = true;
     }
 
     void Clear() {
         vg_sim_destroy(sim_);
         sim_ = nullptr;
         mVortons = nullptr;
+        mVelGrid.Clear();
+        mVectorPotentialMultiGrid.Clear();
+        mNegativeVorticityMultiGrid.Clear();
     }
 
-    // vortonSim.h:334-343 (host copies, materialised on demand)
-    const UniformGrid<Vec3> &GetVelocityGrid() {
-        if (vel_stale_ && sim_) {
-            vg_grid_desc g;
-            const float *f = nullptr;
-            check(vg_sim_velocity_grid(sim_, &g, &f), "GetVelocityGrid");
-            if (f) velGrid_.Download(g, f);
-            vel_stale_ = false;
-        }
-        return velGrid_;
+    // vortonSim.h:333-343.  The grids are members whose addresses stay valid
+    // across frames (operators may keep pointers to them); their values are
+    // the device fields of the last Update, copied to the host on first host
+    // access.
+    const UniformGrid<Vec3> &GetVelocityGrid() const { return mVelGrid; }
+    UniformGrid<Vec3> &GetVelocityGrid() { return mVelGrid; }
+    const NestedGrid<Vec3> &GetNegativeVorticityMultiGrid() const {
+        sync_multigrids();
+        return mNegativeVorticityMultiGrid;
     }
-    const UniformGrid<Vec3> &GetVectorPotentialGrid() {
-        if (pot_stale_ && sim_) {
-            vg_grid_desc g;
-            const float *f = nullptr;
-            check(vg_sim_vector_potential(sim_, &g, &f), "GetVectorPotentialMultiGrid");
-            if (f) potGrid_.Download(g, f);
-            pot_stale_ = false;
-        }
-        return potGrid_;
+    NestedGrid<Vec3> &GetNegativeVorticityMultiGrid() {
+        sync_multigrids();
+        return mNegativeVorticityMultiGrid;
     }
-    const UniformGrid<Vec3> &GetNegativeVorticityGrid() {
-        if (negw_stale_ && sim_) {
-            vg_grid_desc g;
-            const float *f = nullptr;
-            check(vg_sim_negative_vorticity(sim_, &g, &f), "GetNegativeVorticityMultiGrid");
-            if (f) negwGrid_.Download(g, f);
-            negw_stale_ = false;
+    const NestedGrid<Vec3> &GetVectorPotentialMultiGrid() const {
+        sync_multigrids();
+        return mVectorPotentialMultiGrid;
+    }
+    NestedGrid<Vec3> &GetVectorPotentialMultiGrid() {
+        sync_multigrids();
+        return mVectorPotentialMultiGrid;
+    }
+    // vortonSim.h:318: statistics of the Poisson residual.  The reference
+    // tallies the last SOR pass's update magnitudes, and only in a serial
+    // build (uniformGridMath.cpp:1173-1184; its threaded build leaves the
+    // reset values).  Here they are the statistics over the interior points
+    // of |f - L A| for the last POISSON update's vector potential A and
+    // right-hand side f = -w, the 7-point Laplacian L in FP64, reported in
+    // FP32.  Computed on the host from the two grids when called.
+    const Stats_Float &GetPoissonResidualStats() {
+        sync_multigrids();
+        residualStats_ = Stats_Float();
+        if (mVectorPotentialMultiGrid.GetDepth() == 0) return residualStats_;
+        const UniformGrid<Vec3> &A = mVectorPotentialMultiGrid[0];
+        const UniformGrid<Vec3> &F = mNegativeVorticityMultiGrid[0];
+        const unsigned nx = A.GetNumPoints(0), ny = A.GetNumPoints(1), nz = A.GetNumPoints(2);
+        const Vec3 h = A.GetCellSpacing();
+        const double r2[3] = {1.0 / ((double)h.x * h.x), 1.0 / ((double)h.y * h.y),
+                              1.0 / ((double)h.z * h.z)};
+        const Vec3 *a = A.data();
+        const Vec3 *f = F.data();
+        double sum = 0.0, sum2 = 0.0, mn = DBL_MAX, mx = -DBL_MAX;
+        size_t count = 0;
+        const size_t sx = 1, sy = nx, sz = (size_t)nx * ny;
+        for (unsigned iz = 1; iz + 1 < nz; ++iz)
+            for (unsigned iy = 1; iy + 1 < ny; ++iy)
+                for (unsigned ix = 1; ix + 1 < nx; ++ix) {
+                    const size_t o = ix + sy * iy + sz * iz;
+                    double r[3];
+                    const float *c = &a[o].x;
+                    for (int k = 0; k < 3; ++k) {
+                        auto at = [&](size_t off) { return (double)(&a[off].x)[k]; };
+                        const double lap = (at(o + sx) + at(o - sx) - 2.0 * c[k]) * r2[0] +
+                                           (at(o + sy) + at(o - sy) - 2.0 * c[k]) * r2[1] +
+                                           (at(o + sz) + at(o - sz) - 2.0 * c[k]) * r2[2];
+                        r[k] = (double)(&f[o].x)[k] - lap;
+                    }
+                    const double m = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+                    sum += m;
+                    sum2 += m * m;
+                    mn = m < mn ? m : mn;
+                    mx = m > mx ? m : mx;
+                    ++count;
+                }
+        if (count) {
+            const double mean = sum / count;
+            const double var = sum2 / count - mean * mean;
+            residualStats_.mMean = (float)mean;
+            residualStats_.mStdDev = (float)std::sqrt(var > 0.0 ? var : 0.0);
+            residualStats_.mMin = (float)mn;
+            residualStats_.mMax = (float)mx;
         }
-        return negwGrid_;
+        return residualStats_;
+    }
+    // No reference counterpart: the FP64 true relative residual
+    // ||f - L A|| / ||f|| of the last POISSON update (device reduction).
+    double GetPoissonResidual() const {
+        double r = -1.0;
+        if (sim_) check(vg_sim_poisson_residual(sim_, &r), "GetPoissonResidual");
+        return r;
     }
     // Device view of the velocity grid (no copy), for device-side operators.
     const float *GetVelocityGridDevice(vg_grid_desc *g) const {
@@ -409,16 +564,25 @@ class VortonSim {
         check(vg_sim_velocity_grid(sim_, g, &f), "GetVelocityGrid");
         return f;
     }
-    double GetPoissonResidual() const {
-        double r = -1.0;
-        if (sim_) check(vg_sim_poisson_residual(sim_, &r), "GetPoissonResidualStats");
-        return r;
-    }
+    // vortonSim.h:405-411.  The two corners are adjacent in memory, as the
+    // reference's are: PclOpPopulateVelocityGrid reads mBoundingBox[0..1]
+    // through the address of GetMinCorner() (particleSystemConfiguration.cpp:140).
     const UniformGridGeometry &GetGrid() const { return grid_; }
-    const Vec3 &GetMinCorner() const { return minCorner_; }
-    const Vec3 &GetMaxCorner() const { return maxCorner_; }
+    const Vec3 &GetMinCorner() const { return bounds_[0]; }
+    const Vec3 &GetMaxCorner() const { return bounds_[1]; }
 
   private:
+    void copy_settings(const VortonSim &that) {
+        cfg_ = that.cfg_;
+        viscosity_ = that.viscosity_;
+        density_ = that.density_;
+        heatCapacity_ = that.heatCapacity_;
+        gravity_ = that.gravity_;
+        fluidSimTechnique_ = that.fluidSimTechnique_;
+        investigationTerm_ = that.investigationTerm_;
+        pseOrder_ = that.pseOrder_;
+        mVortons = nullptr;
+    }
     void ensure() {
         if (!sim_) check(vg_sim_create(&sim_, &cfg_, nullptr), "VortonSim");
     }
@@ -442,8 +606,36 @@ class VortonSim {
     void sync_bounds() {
         float a[3], b[3];
         check(vg_sim_bounds(sim_, a, b), "bounds");
-        minCorner_ = Vec3(a[0], a[1], a[2]);
-        maxCorner_ = Vec3(b[0], b[1], b[2]);
+        bounds_[0] = Vec3(a[0], a[1], a[2]);
+        bounds_[1] = Vec3(b[0], b[1], b[2]);
+    }
+    void sync_multigrids() const {
+        if (!multigrids_stale_ || !sim_) return;
+        NestedGrid<Vec3> *grids[2] = {&mVectorPotentialMultiGrid, &mNegativeVorticityMultiGrid};
+        const int fields[2] = {VG_MG_VECTOR_POTENTIAL, VG_MG_NEGATIVE_VORTICITY};
+        for (int k = 0; k < 2; ++k) {
+            vg_grid_desc g;
+            const float *f = nullptr;
+            uint32_t depth = 0;
+            grids[k]->Clear();
+            if (vg_sim_multigrid(sim_, fields[k], 0, &g, &f, &depth) != VG_OK || depth == 0)
+                continue;
+            grids[k]->Resize(depth);
+            for (uint32_t l = 0; l < depth; ++l) {
+                check(vg_sim_multigrid(sim_, fields[k], l, &g, &f, nullptr),
+                      "GetVectorPotentialMultiGrid");
+                (*grids[k])[l].AttachDevice(g, f);
+            }
+            vg_grid_desc layers[16], g0;
+            uint32_t dec[48] = {0}, nl = 0;
+            check(vg_sim_multigrid(sim_, fields[k], 0, &g0, &f, nullptr), "multigrid");
+            check(vg_nested_layers(&g0, layers, dec, 16, &nl), "NestedGrid");
+            for (uint32_t l = 0; l < depth && l < nl; ++l) {
+                const unsigned d[3] = {dec[3 * l], dec[3 * l + 1], dec[3 * l + 2]};
+                grids[k]->SetDecimations(l, d);
+            }
+        }
+        multigrids_stale_ = false;
     }
 
     vg_sim_config cfg_;
@@ -451,20 +643,26 @@ class VortonSim {
     VECTOR<Vorton> *mVortons = nullptr;
     DeviceBuffer dev_;
     UniformGridGeometry grid_;
-    UniformGrid<Vec3> velGrid_, potGrid_, negwGrid_;
-    bool vel_stale_ = true, pot_stale_ = true, negw_stale_ = true;
-    Vec3 minCorner_, maxCorner_;
-    float viscosity_ = 0.0f, density_ = 1.0f;
+    UniformGrid<Vec3> mVelGrid;
+    mutable NestedGrid<Vec3> mVectorPotentialMultiGrid, mNegativeVorticityMultiGrid;
+    mutable bool multigrids_stale_ = true;
+    Stats_Float residualStats_;
+    Vec3 bounds_[2];
+    float viscosity_ = 0.0f, density_ = 1.0f, heatCapacity_ = 1.0f;
+    Vec3 gravity_;
     FluidSimulationTechniqueE fluidSimTechnique_ = FLUID_SIM_VPM_SPH_HYBRID;
     InvestigationTermE investigationTerm_ = INVESTIGATE_ALL;
     int pseOrder_ = VG_PSE_REFERENCE_ORDER;
 };
 
-// pclOpVortonSim.h:46-71 / pclOpVortonSim.cpp:51-71
+// pclOpVortonSim.h:46-71 / pclOpVortonSim.cpp:51-71.  mMinCorner and
+// mMaxCorner point at the tracer group's bounding box (patched in by the
+// particle system, particleSystemConfiguration.cpp:286-296).
 class PclOpVortonSim : public IParticleOperation {
   public:
     PclOpVortonSim() = default;
-    PclOpVortonSim(const PclOpVortonSim &that) : mVortonSim(that.mVortonSim) {}
+    PclOpVortonSim(const PclOpVortonSim &that)
+        : mVortonSim(that.mVortonSim), mMinCorner(that.mMinCorner), mMaxCorner(that.mMaxCorner) {}
     VGFACADE_VIRTUAL_CONSTRUCTORS(PclOpVortonSim)
     void Operate(VECTOR<Particle> &particles, float timeStep, unsigned uFrame) override {
         mVortonSim.SetVortons(reinterpret_cast<VECTOR<Vorton> *>(&particles));
@@ -482,7 +680,7 @@ class PclOpVortonSim : public IParticleOperation {
     const Vec3 *mMaxCorner = nullptr;
 };
 
-// Particle group staged on the device for the tracer operators below.
+// Particle group staged on the device for the particle operators below.
 class DeviceParticles {
   public:
     float *upload(const VECTOR<Particle> &p) {
@@ -500,29 +698,48 @@ class DeviceParticles {
     DeviceBuffer buf_;
 };
 
-// pclOpAdvect.cpp:132-163 (AssignVelocityFromField)
+// A grid's values as a device field: the field a VortonSim filled when the
+// grid is still attached to it, else the host values uploaded to `scratch`.
+inline const float *device_field(const UniformGrid<Vec3> &grid, DeviceBuffer &scratch) {
+    if (grid.DeviceData()) return static_cast<const float *>(grid.DeviceData());
+    const size_t bytes = grid.Size() * sizeof(Vec3);
+    if (!bytes) return nullptr;
+    scratch.reserve(bytes);
+    check(vg_memcpy_htod(scratch.get(), grid.data(), bytes), "upload grid");
+    return static_cast<const float *>(scratch.get());
+}
+
+// pclOpAdvect.cpp:132-152: velocity = interpolated grid velocity (gain 1)
+// or velocity += gain (interpolated - velocity).  A grid with zero extent
+// leaves the particles as they are; so does a null grid (the reference
+// would dereference it).
+inline void AssignVelocityFromField(VECTOR<Particle> &particles,
+                                    const UniformGrid<Vec3> *velocityGrid, const float gain) {
+    if (!velocityGrid || particles.empty() || velocityGrid->HasZeroExtent() ||
+        velocityGrid->Empty())
+        return;
+    static thread_local DeviceParticles dev;
+    static thread_local DeviceBuffer grid_scratch;
+    const float *f = device_field(*velocityGrid, grid_scratch);
+    float *base = dev.upload(particles);
+    check(vg_assign_velocity_from_field(&velocityGrid->desc(), f, base, base + kOffVel, kStride,
+                                        particles.size(), gain, nullptr),
+          "AssignVelocityFromField");
+    dev.download(particles);
+}
+
+// pclOpAdvect.h:19-33, pclOpAdvect.cpp:157-163
 class PclOpAssignVelocityFromField : public IParticleOperation {
   public:
     PclOpAssignVelocityFromField() = default;
     PclOpAssignVelocityFromField(const PclOpAssignVelocityFromField &o)
-        : mVortonSim(o.mVortonSim), mGain(o.mGain) {}
+        : mVelocityGrid(o.mVelocityGrid), mGridWeight(o.mGridWeight) {}
     VGFACADE_VIRTUAL_CONSTRUCTORS(PclOpAssignVelocityFromField)
     void Operate(VECTOR<Particle> &particles, float, unsigned) override {
-        if (!mVortonSim || particles.empty()) return;
-        vg_grid_desc g;
-        const float *f = mVortonSim->GetVelocityGridDevice(&g);
-        if (!f) return;
-        float *base = dev_.upload(particles);
-        check(vg_assign_velocity_from_field(&g, f, base, base + kOffVel, kStride,
-                                            particles.size(), mGain, nullptr),
-              "AssignVelocityFromField");
-        dev_.download(particles);
+        if (mVelocityGrid) AssignVelocityFromField(particles, mVelocityGrid, mGridWeight);
     }
-    const VortonSim *mVortonSim = nullptr;  // source of the velocity grid
-    float mGain = 1.0f;
-
-  private:
-    DeviceParticles dev_;
+    const UniformGrid<Vec3> *mVelocityGrid = nullptr;  // grid of velocity values
+    float mGridWeight = 1.0f;  // share of the grid velocity assigned each frame
 };
 
 // pclOpEvolve.cpp:118-123
@@ -543,27 +760,103 @@ class PclOpEvolve : public IParticleOperation {
     DeviceParticles dev_;
 };
 
-// pclOpFindBoundingBox.cpp:141-148
+// pclOpFindBoundingBox.h:16-40, pclOpFindBoundingBox.cpp:20-148
 class PclOpFindBoundingBox : public IParticleOperation {
   public:
     VGFACADE_VIRTUAL_CONSTRUCTORS(PclOpFindBoundingBox)
-    PclOpFindBoundingBox() = default;
+    PclOpFindBoundingBox() : mMinCorner(FLT_MAX, FLT_MAX, FLT_MAX), mMaxCorner(-mMinCorner) {}
     PclOpFindBoundingBox(const PclOpFindBoundingBox &o)
         : mMinCorner(o.mMinCorner), mMaxCorner(o.mMaxCorner) {}
     void Operate(VECTOR<Particle> &particles, float, unsigned) override {
-        float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-        if (!particles.empty()) {
-            float *base = dev_.upload(particles);
-            check(vg_find_bounding_box(base, kStride, particles.size(), mn, mx, nullptr),
-                  "FindBoundingBox");
-        }
-        mMinCorner = Vec3(mn[0], mn[1], mn[2]);
-        mMaxCorner = Vec3(mx[0], mx[1], mx[2]);
+        mMinCorner = Vec3(FLT_MAX, FLT_MAX, FLT_MAX);
+        mMaxCorner = -mMinCorner;
+        FindBoundingBox(particles, mMinCorner, mMaxCorner);
     }
-    Vec3 mMinCorner, mMaxCorner;
+    const Vec3 &GetMinCorner() const { return mMinCorner; }
+    const Vec3 &GetMaxCorner() const { return mMaxCorner; }
+    // Grows minCorner / maxCorner to contain every particle.
+    static void FindBoundingBox(const VECTOR<Particle> &particles, Vec3 &minCorner,
+                                Vec3 &maxCorner) {
+        if (particles.empty()) return;
+        static thread_local DeviceParticles dev;
+        float mn[3] = {minCorner.x, minCorner.y, minCorner.z};
+        float mx[3] = {maxCorner.x, maxCorner.y, maxCorner.z};
+        float *base = dev.upload(particles);
+        check(vg_find_bounding_box(base, kStride, particles.size(), mn, mx, nullptr),
+              "FindBoundingBox");
+        minCorner = Vec3(mn[0], mn[1], mn[2]);
+        maxCorner = Vec3(mx[0], mx[1], mx[2]);
+    }
 
   private:
-    DeviceParticles dev_;
+    Vec3 mMinCorner, mMaxCorner;
+};
+
+namespace Particles {
+// pclOpPopulateVelocityGrid.cpp:33-87: velocityGrid (its shape already
+// defined) becomes the trilinear-weighted average of the particles'
+// velocities.  On the device: the particles are binned into the grid's
+// cells (an influence tree over the grid, the cell lists it records) and
+// every grid point gathers from the 8 cells around it.
+inline void PopulateVelocityGrid(UniformGrid<Vec3> &velocityGrid,
+                                 const VECTOR<Particle> &particles) {
+    const size_t cap = velocityGrid.GetGridCapacity();
+    std::vector<Vec3> out(cap);
+    if (cap && !particles.empty()) {
+        static thread_local DeviceParticles dev;
+        static thread_local DeviceBuffer field;
+        float *base = dev.upload(particles);
+        vg_vortons v;
+        std::memset(&v, 0, sizeof v);
+        v.pos = base;
+        v.angvel = base + kOffVel;  // the tree's cluster sums are not used
+        v.size = base + kOffSize;
+        v.stride = kStride;
+        v.n = particles.size();
+        vg_tree *t = nullptr;
+        check(vg_tree_create(&t), "PopulateVelocityGrid");
+        field.reserve(cap * sizeof(Vec3));
+        vg_status st = vg_tree_build(t, &velocityGrid.desc(), &v, nullptr);
+        if (st == VG_OK)
+            st = vg_tree_populate_velocity_grid(t, base + kOffVel, kStride,
+                                                static_cast<float *>(field.get()), nullptr);
+        if (st == VG_OK)
+            st = vg_memcpy_dtoh(out.data(), field.get(), cap * sizeof(Vec3));
+        vg_tree_destroy(t);
+        check(st, "PopulateVelocityGrid");
+    }
+    velocityGrid.Assign(velocityGrid.desc(), out.data());
+}
+}  // namespace Particles
+
+// pclOpPopulateVelocityGrid.h:16-33, pclOpPopulateVelocityGrid.cpp:90-136
+class PclOpPopulateVelocityGrid : public IParticleOperation {
+  public:
+    VGFACADE_VIRTUAL_CONSTRUCTORS(PclOpPopulateVelocityGrid)
+    PclOpPopulateVelocityGrid() = default;
+    PclOpPopulateVelocityGrid(const PclOpPopulateVelocityGrid &o)
+        : mVelocityGrid(o.mVelocityGrid), mBoundingBox(o.mBoundingBox),
+          mUpdateBoundingBox(o.mUpdateBoundingBox) {}
+    void Operate(VECTOR<Particle> &particles, float, unsigned) override {
+        if (particles.empty() || !mVelocityGrid) return;
+        if (!mBoundingBox || mUpdateBoundingBox) {
+            // a new box holding the given one (if any) and the particles,
+            // padded by the first particle's radius
+            Vec3 minCorner = mBoundingBox ? mBoundingBox[0] : Vec3(FLT_MAX, FLT_MAX, FLT_MAX);
+            Vec3 maxCorner = mBoundingBox ? mBoundingBox[1] : -Vec3(FLT_MAX, FLT_MAX, FLT_MAX);
+            PclOpFindBoundingBox::FindBoundingBox(particles, minCorner, maxCorner);
+            const float margin = particles[0].GetRadius();
+            const Vec3 nudge(margin, margin, margin);
+            minCorner = minCorner - nudge;
+            maxCorner = maxCorner + nudge;
+            mVelocityGrid->Clear();
+            mVelocityGrid->DefineShape(particles.size(), minCorner, maxCorner, true);
+        }
+        Particles::PopulateVelocityGrid(*mVelocityGrid, particles);
+    }
+    UniformGrid<Vec3> *mVelocityGrid = nullptr;  // grid of velocity values
+    Vec3 *mBoundingBox = nullptr;                // [min, max] corners, or null
+    bool mUpdateBoundingBox = true;              // grow the box by the particles
 };
 
 }  // namespace vgfacade

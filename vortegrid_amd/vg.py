@@ -31,7 +31,8 @@ EXPORTS = (
     "vg_tree_vecpot_boundary "
     "vg_tree_splat_vorticity vg_tree_populate_velocity_grid vg_downsample_inject "
     "vg_upsample_interior vg_sor_solve vg_curl vg_residual_norm vg_mg_create vg_mg_destroy "
-    "vg_mg_cascade vg_mg_vcycle_solve vg_mg_vcycle_compensation vg_assign_velocity_from_field "
+    "vg_mg_cascade vg_mg_vcycle_solve vg_mg_vcycle_compensation vg_mg_layer vg_sim_multigrid "
+    "vg_assign_velocity_from_field "
     "vg_evolve "
     "vg_advect_tracers vg_sim_config_default vg_sim_create vg_sim_destroy vg_sim_set_vortons "
     "vg_sim_find_bounding_box vg_sim_update_bounding_box vg_sim_update vg_sim_velocity_grid "
