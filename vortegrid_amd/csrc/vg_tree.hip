@@ -658,6 +658,18 @@ __global__ void k_gather_batch(const float *__restrict__ pos, const float *__res
     sw[j] = make_float4(w[0], w[1], w[2], 0.f);
 }
 
+// Record gather for the cell sort: record j of dst is record sorted_idx[j]
+// of src (stride floats each).
+__global__ void k_gather_records(const float *__restrict__ src, size_t stride,
+                                 const uint32_t *__restrict__ sorted_idx, size_t n,
+                                 float *__restrict__ dst) {
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const float *a = src + (size_t)sorted_idx[j] * stride;
+    float *b = dst + j * stride;
+    for (size_t c = 0; c < stride; ++c) b[c] = a[c];
+}
+
 }  // namespace
 
 size_t scan_scratch_elems(size_t n) { return scan_elems(n); }
@@ -986,4 +998,54 @@ VG_API vg_status vg_tree_populate_velocity_grid(const vg_tree *t, const float *v
     const size_t cap = capacity(t->ng.layer[0]);
     return launch("vg_populate_velocity_grid", k_splat_gather<1>, dim3(ceil_div(cap, 256)),
                   dim3(256), 0, (hipStream_t)s, make_view(t), 0.0f, vel, vel_stride, out);
+}
+
+VG_API vg_status vg_sort_particles_by_cell(const vg_grid_desc *g, float *records, size_t stride,
+                                          size_t n, uint32_t *perm, vg_stream stream) {
+    VG_REQUIRE(g, "grid is null");
+    VG_REQUIRE(n == 0 || (records && stride >= 3), "bad particle records");
+    VG_REQUIRE(n < (1ull << 32), "too many particles");
+    VG_REQUIRE(g->np[0] >= 2 && g->np[1] >= 2 && g->np[2] >= 2, "grid is empty");
+    const size_t cap = capacity(*g);
+    VG_REQUIRE(cap < (1ull << 32), "grid too large");
+    if (n == 0) return VG_OK;
+    hipStream_t s = (hipStream_t)stream;
+    // one scratch block: count, start (cells), keys, slots, unsorted, sorted
+    // (particles), scan partials, then the reordered records
+    const size_t scan_n = scan_elems(cap);
+    const size_t u32s = 2 * cap + 4 * n + scan_n;
+    const size_t rec_off = (u32s * sizeof(uint32_t) + 255) & ~size_t(255);
+    unsigned char *base = nullptr;
+    VG_TRY(salloc((void **)&base, rec_off + n * stride * sizeof(float), s));
+    uint32_t *count = reinterpret_cast<uint32_t *>(base), *start = count + cap;
+    uint32_t *keys = start + cap, *slots = keys + n, *unsorted = slots + n;
+    uint32_t *sorted = unsorted + n, *tmp = sorted + n;
+    float *moved = reinterpret_cast<float *>(base + rec_off);
+    VortonIn vin{records, nullptr, nullptr, stride, 0.0f};
+    vg_status st = launch("vg_sort_fill", k_fill_u32, dim3(ceil_div(cap, 256)), dim3(256), 0, s,
+                          count, cap, 0u);
+    if (st == VG_OK)
+        st = launch("vg_sort_bin", k_bin, dim3(ceil_div(n, 256)), dim3(256), 0, s, vin, n, *g,
+                    count, keys, slots);
+    if (st == VG_OK) st = exclusive_scan(count, start, cap, tmp, s);
+    if (st == VG_OK)
+        st = launch("vg_sort_scatter", k_scatter, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                    (const uint32_t *)keys, (const uint32_t *)slots, (const uint32_t *)start, n,
+                    unsorted);
+    if (st == VG_OK)
+        st = launch("vg_sort_rank", k_rank_cells, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                    (const uint32_t *)keys, (const uint32_t *)start, (const uint32_t *)count,
+                    (const uint32_t *)unsorted, n, sorted);
+    if (st == VG_OK)
+        st = launch("vg_sort_gather", k_gather_records, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                    (const float *)records, stride, (const uint32_t *)sorted, n, moved);
+    if (st == VG_OK) {
+        hipError_t e = hipMemcpyAsync(records, moved, n * stride * sizeof(float),
+                                      hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess && perm)
+            e = hipMemcpyAsync(perm, sorted, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) st = device_error(e, "sort copy", __FILE__, __LINE__);
+    }
+    sfree(base, s);
+    return st;
 }

@@ -33,7 +33,7 @@ EXPORTS = (
     "vg_upsample_interior vg_sor_solve vg_curl vg_residual_norm vg_mg_create vg_mg_destroy "
     "vg_mg_cascade vg_mg_vcycle_solve vg_mg_vcycle_compensation vg_mg_layer vg_sim_multigrid "
     "vg_assign_velocity_from_field "
-    "vg_evolve "
+    "vg_evolve vg_sort_particles_by_cell "
     "vg_advect_tracers vg_sim_config_default vg_sim_create vg_sim_destroy vg_sim_set_vortons "
     "vg_sim_find_bounding_box vg_sim_update_bounding_box vg_sim_update vg_sim_velocity_grid "
     "vg_sim_vector_potential vg_sim_negative_vorticity vg_sim_grid_template vg_sim_bounds "
@@ -636,6 +636,12 @@ def assign_velocity_from_field(g: GridDesc, field: DeviceArray, pos, vel, n: int
 def evolve(pos, vel, n: int, dt: float, stride=3, stream=None):
     call("vg_evolve", pos.ptr if hasattr(pos, "ptr") else pos, vel.ptr if hasattr(vel, "ptr") else vel,
          C.c_size_t(stride), C.c_size_t(n), C.c_float(dt), _s(stream))
+
+
+def sort_particles_by_cell(g: GridDesc, records, n: int, stride=3, perm=None, stream=None):
+    """Reorder n particle records (stride floats, position first) in cell order."""
+    call("vg_sort_particles_by_cell", C.byref(g), _p(records), C.c_size_t(stride), C.c_size_t(n),
+         _p(perm), _s(stream))
 
 
 def advect_tracers(g: GridDesc, field, pos, vel, n: int, dt: float, gain=1.0, stride=3,
