@@ -132,6 +132,21 @@ vg_status vg_find_bounding_box(const float *pos, size_t stride, size_t n,
 vg_status vg_direct_velocity(const vg_vortons *v, const float *q, size_t q_stride,
                              size_t nq, float spread, float circ, int rsqrt_policy,
                              float *out, vg_stream s);
+/* No reference counterpart.  vg_direct_velocity for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): it calls no allocator.  Its
+ * transient buffers (the packed sources, 32 B per vorton, and the
+ * source-split partials, 12 B per query and split) live in a fixed 32 MiB
+ * device array of the library, one per GPU, held until the process ends.
+ * Sizes that need more are refused with VG_EINVAL (1,048,576 vortons at
+ * most, fewer when a small query set is split over the sources).  Every
+ * call on a GPU uses that GPU's one array and overwrites all it reads, so
+ * nothing needs clearing between calls, but calls that overlap on one GPU
+ * (other streams, other threads) race on it: serialise them.  Results equal
+ * vg_direct_velocity's bit for bit. */
+vg_status vg_direct_velocity_noalloc(const vg_vortons *v, const float *q, size_t q_stride,
+                                     size_t nq, float spread, float circ, int rsqrt_policy,
+                                     float *out, vg_stream s);
 /* VortonSim::ComputeVectorPotential_Direct (vortonSim.cpp:1113-1127). */
 vg_status vg_direct_vecpot(const vg_vortons *v, const float *q, size_t q_stride,
                            size_t nq, float spread, float circ, int rsqrt_policy,

@@ -9,7 +9,14 @@
 // plain FP32 partial per tile and folds it into a Kahan-compensated total
 // once per tile.  Small problems split the source range over blockIdx.y and
 // reduce the partials in fixed order (deterministic).
+#include <mutex>
+#include <string>
+
 #include "vg_internal.h"
+
+// Fixed device scratch of vg_direct_velocity_noalloc (32 MiB per GPU).
+constexpr size_t kFixedScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kFixedScratchBytes];
 
 namespace vg {
 namespace {
@@ -212,9 +219,18 @@ SourceView source_view(const vg_vortons *v) {
     return SourceView{v->pos, v->angvel, v->size, v->stride, v->uniform_size};
 }
 
+// Caller-provided scratch for the transient buffers (null: stream-ordered
+// allocations).
+struct Scratch {
+    unsigned char *base = nullptr;
+    size_t bytes = 0;
+};
+
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
 template <int VECPOT, int GRID>
 vg_status run_direct(const vg_vortons *v, QueryView qv, float spread, float circ, int policy,
-                     float *out, hipStream_t s, const char *name) {
+                     float *out, hipStream_t s, const char *name, Scratch fixed = Scratch()) {
     VG_TRY(check_vortons(v));
     VG_REQUIRE(out || qv.nq == 0, "out is null");
     VG_REQUIRE(policy == VG_RSQRT_REFERENCE || policy == VG_RSQRT_ACCURATE, "bad rsqrt policy");
@@ -224,11 +240,6 @@ vg_status run_direct(const vg_vortons *v, QueryView qv, float spread, float circ
         return VG_OK;
     }
     const size_t n = v->n;
-    float4 *A = nullptr, *B = nullptr;
-    VG_TRY(salloc((void **)&A, n * sizeof(float4), s));
-    VG_TRY(salloc((void **)&B, n * sizeof(float4), s));
-    VG_TRY(launch("vg_pack_sources", k_pack_sources, dim3(ceil_div(n, 256)), dim3(256), 0, s,
-                  source_view(v), n, spread, A, B));
     // Split the source range when the query count alone cannot fill 256 CUs.
     const uint32_t qblocks = ceil_div(qv.nq, (size_t)kBlock * kQPT);
     uint32_t nsplit = 1;
@@ -242,8 +253,30 @@ vg_status run_direct(const vg_vortons *v, QueryView qv, float spread, float circ
     size_t chunk = (n + nsplit - 1) / nsplit;
     chunk = (chunk + kTile - 1) / kTile * kTile;
     nsplit = (uint32_t)((n + chunk - 1) / chunk);
+    const size_t src_bytes = n * sizeof(float4);
+    const size_t part_bytes = nsplit > 1 ? (size_t)nsplit * qv.nq * 3 * sizeof(float) : 0;
+    float4 *A = nullptr, *B = nullptr;
     float *partial = nullptr;
-    if (nsplit > 1) VG_TRY(salloc((void **)&partial, (size_t)nsplit * qv.nq * 3 * sizeof(float), s));
+    if (fixed.base) {
+        const size_t need = 2 * align256(src_bytes) + align256(part_bytes);
+        if (need > fixed.bytes) {
+            set_error("direct sum needs " + std::to_string(need) + " bytes of scratch (" +
+                      std::to_string(n) + " vortons, " + std::to_string(qv.nq) + " queries, " +
+                      std::to_string(nsplit) + " source splits); the fixed scratch holds " +
+                      std::to_string(fixed.bytes));
+            return VG_EINVAL;
+        }
+        A = reinterpret_cast<float4 *>(fixed.base);
+        B = reinterpret_cast<float4 *>(fixed.base + align256(src_bytes));
+        if (part_bytes)
+            partial = reinterpret_cast<float *>(fixed.base + 2 * align256(src_bytes));
+    } else {
+        VG_TRY(salloc((void **)&A, src_bytes, s));
+        VG_TRY(salloc((void **)&B, src_bytes, s));
+        if (part_bytes) VG_TRY(salloc((void **)&partial, part_bytes, s));
+    }
+    VG_TRY(launch("vg_pack_sources", k_pack_sources, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                  source_view(v), n, spread, A, B));
     vg_status st;
     auto kern = policy == VG_RSQRT_REFERENCE ? k_direct<VG_RSQRT_REFERENCE, VECPOT, GRID>
                                              : k_direct<VG_RSQRT_ACCURATE, VECPOT, GRID>;
@@ -252,10 +285,37 @@ vg_status run_direct(const vg_vortons *v, QueryView qv, float spread, float circ
     if (st == VG_OK && nsplit > 1)
         st = launch("vg_direct_reduce", k_reduce_split, dim3(ceil_div(qv.nq, 256)), dim3(256), 0,
                     s, (const float *)partial, nsplit, qv.nq, circ, out);
-    sfree(partial, s);
-    sfree(A, s);
-    sfree(B, s);
+    if (!fixed.base) {
+        sfree(partial, s);
+        sfree(A, s);
+        sfree(B, s);
+    }
     return st;
+}
+
+// ---------------------------------------------------------------- fixed scratch
+// The transient buffers of vg_direct_velocity_noalloc: one array per GPU in
+// the code object, kept until the process ends.  In host code `scratch` only
+// names the symbol; its device address on the current GPU comes from
+// hipGetSymbolAddress, taken once per device and kept in g_scratch_ptr.
+constexpr int kMaxDevices = 64;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+vg_status fixed_scratch(Scratch *out) {
+    int dev = 0;
+    VG_HIP(hipGetDevice(&dev));
+    VG_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index beyond the fixed-scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        VG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        VG_REQUIRE(p, "no device address for the fixed scratch array");
+        g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
+    }
+    out->base = g_scratch_ptr[dev];
+    out->bytes = kFixedScratchBytes;
+    return VG_OK;
 }
 
 }  // namespace
@@ -270,6 +330,17 @@ VG_API vg_status vg_direct_velocity(const vg_vortons *v, const float *q, size_t 
     QueryView qv{q, q_stride, {}, nq};
     return run_direct<0, 0>(v, qv, spread, circ, rsqrt_policy, out, (hipStream_t)s,
                             "vg_direct_velocity");
+}
+
+VG_API vg_status vg_direct_velocity_noalloc(const vg_vortons *v, const float *q,
+                                            size_t q_stride, size_t nq, float spread, float circ,
+                                            int rsqrt_policy, float *out, vg_stream s) {
+    VG_REQUIRE(nq == 0 || (q && q_stride >= 3), "bad query array");
+    Scratch fixed;
+    VG_TRY(fixed_scratch(&fixed));
+    QueryView qv{q, q_stride, {}, nq};
+    return run_direct<0, 0>(v, qv, spread, circ, rsqrt_policy, out, (hipStream_t)s,
+                            "vg_direct_velocity", fixed);
 }
 
 VG_API vg_status vg_direct_vecpot(const vg_vortons *v, const float *q, size_t q_stride, size_t nq,

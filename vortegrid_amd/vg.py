@@ -25,6 +25,7 @@ EXPORTS = (
     "vg_profile_reset vg_profile_get vg_profile_dump vg_define_shape vg_decimate "
     "vg_nested_layers vg_finalize_bounding_box vg_auto_mollify vg_find_bounding_box "
     "vg_direct_velocity vg_direct_vecpot vg_direct_velocity_grid vg_direct_velocity_f64 "
+    "vg_direct_velocity_noalloc "
     "vg_tree_create vg_tree_destroy vg_tree_build vg_tree_depth vg_tree_layer "
     "vg_tree_layer_moments vg_tree_velocity_grid_mac vg_tree_mac_work "
     "vg_tree_velocity vg_tree_velocity_grid vg_tree_interaction_count vg_tree_vecpot "
@@ -291,6 +292,13 @@ def _s(stream):
 def direct_velocity(v: Vortons, q: DeviceArray, nq: int, out: DeviceArray, spread=1.0, circ=1.0,
                     policy=RSQRT_REFERENCE, q_stride=3, stream=None):
     call("vg_direct_velocity", C.byref(v), q.ptr, C.c_size_t(q_stride), C.c_size_t(nq),
+         C.c_float(spread), C.c_float(circ), policy, out.ptr, _s(stream))
+
+
+def direct_velocity_noalloc(v: Vortons, q: DeviceArray, nq: int, out: DeviceArray, spread=1.0,
+                            circ=1.0, policy=RSQRT_REFERENCE, q_stride=3, stream=None):
+    """direct_velocity with the library's fixed device scratch (no allocation)."""
+    call("vg_direct_velocity_noalloc", C.byref(v), q.ptr, C.c_size_t(q_stride), C.c_size_t(nq),
          C.c_float(spread), C.c_float(circ), policy, out.ptr, _s(stream))
 
 
