@@ -62,6 +62,9 @@ def main():
     ap.add_argument("--tracer-order", choices=("random", "emitted"), default="random",
                     help="tracer memory order: uniform random, or the reference emitter's "
                          "cell-by-cell jittered lattice (pclOpEmit.cpp:77-172)")
+    ap.add_argument("--sort-tracers", type=int, default=8,
+                    help="reorder the tracers into the velocity grid's cell order every K frames "
+                         "inside the timed frame (vg_sort_particles_by_cell; 0 = never)")
     ap.add_argument("--policy", type=int, default=1, help="0 reference rsqrt, 1 accurate")
     ap.add_argument("--near-k", type=int, default=3,
                     help="near-field radius of the accurate treecode; -1 = reference opening rule")
@@ -70,6 +73,10 @@ def main():
                     help="near-field treecode: first layer applied through brick-centred local "
                          "expansions (-1 = none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-correction leg of the default line (training + 3 "
+                         "solves at its near radius, against the analytic treecodes)")
+    ap.add_argument("--learned-near-k", type=int, default=1)
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--suite", action="store_true",
                     help="per-kernel records of the direct, treecode+tracer and multigrid "
@@ -136,12 +143,17 @@ def main():
         sim.set_mlp(mlp)
     grid_holder = {"g": None, "field": None}
     evals = [0]
+    frame_no = [0]
 
     def frame():
-        # tracers first, with the previous frame's grid
+        # tracers first, with the previous frame's grid; every K frames they
+        # are put back into that grid's cell order first (they drift slowly)
         if grid_holder["g"] is not None:
+            if args.sort_tracers > 0 and frame_no[0] % args.sort_tracers == 0:
+                vg.sort_particles_by_cell(grid_holder["g"], d_tr, len(tr), stream=stream)
             vg.advect_tracers(grid_holder["g"], grid_holder["field"], d_tr, None, len(tr), dt, 1.0,
                               stream=stream)
+        frame_no[0] += 1
         tmn, tmx = vg.find_bounding_box(d_tr, len(tr), stream=stream)
         sim.find_bounding_box()
         sim.update_bounding_box(tmn, tmx, True)
@@ -229,29 +241,33 @@ def main():
     if acnt:
         adv_s = ams / acnt * 1e-3
         gbs = 24.0 * len(tr) / adv_s / 1e9
-        adv = {"tracer_order": args.tracer_order, "avg_launch_ms": ams / acnt,
-               "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
-        # the same launch on the same number of tracers in the other memory
-        # order (the gathers of spatially ordered tracers hit the caches)
-        other = "random" if args.tracer_order == "emitted" else "emitted"
-        tr_o = (scenarios.tracers_emitted(len(tr), mn, mx, seed=6) if other == "emitted"
-                else scenarios.tracers_in_box(len(tr), mn, mx, seed=6))
-        d_tro = vg.DeviceArray.from_numpy(tr_o)
-        vg.profile_reset()
-        vg.profile_enable(True)
-        for _ in range(10):
-            vg.advect_tracers(g, gptr, d_tro, None, len(tr_o), dt, 1.0, stream=stream)
-        stream.sync()
-        vg.profile_enable(False)
-        ocnt, oms = vg.profile_dump().get("vg_advect_tracers", (0, 0.0))
-        if ocnt:
-            ogbs = 24.0 * len(tr_o) / (oms / ocnt * 1e-3) / 1e9
-            adv["other_order"] = {"tracer_order": other, "avg_launch_ms": oms / ocnt,
-                                  "achieved_GBs": ogbs, "frac": ogbs / HBM_PEAK_GBS}
-        del d_tro
+        sort_ms = sum(m for k, (c, m) in prof.items() if k.startswith("vg_sort_"))
+        adv = {"tracer_order": args.tracer_order + (f", cell-sorted every {args.sort_tracers} "
+                                                    f"frames" if args.sort_tracers > 0 else ""),
+               "avg_launch_ms": ams / acnt, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
+               "sort_ms_per_frame": sort_ms / args.steps}
+        # the same launch on the same number of tracers in other memory
+        # orders (the gathers of spatially ordered tracers hit the caches)
+        for other in ("random", "emitted"):
+            tr_o = (scenarios.tracers_emitted(len(tr), mn, mx, seed=6) if other == "emitted"
+                    else scenarios.tracers_in_box(len(tr), mn, mx, seed=6))
+            d_tro = vg.DeviceArray.from_numpy(tr_o)
+            vg.profile_reset()
+            vg.profile_enable(True)
+            for _ in range(10):
+                vg.advect_tracers(g, gptr, d_tro, None, len(tr_o), dt, 1.0, stream=stream)
+            stream.sync()
+            vg.profile_enable(False)
+            ocnt, oms = vg.profile_dump().get("vg_advect_tracers", (0, 0.0))
+            if ocnt:
+                ogbs = 24.0 * len(tr_o) / (oms / ocnt * 1e-3) / 1e9
+                adv[f"{other}_order_unsorted"] = {"avg_launch_ms": oms / ocnt, "achieved_GBs": ogbs,
+                                                  "frac": ogbs / HBM_PEAK_GBS}
+            del d_tro
 
     # ---- accuracy: treecode vs FP64 direct on a 1% sample of grid points
     rel = None
+    sample = None
     if not args.no_accuracy:
         cap = g.capacity
         rng = np.random.default_rng(1234)
@@ -274,6 +290,18 @@ def main():
         rel = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
         log(f"rel-L2 vs FP64 direct on {len(q)} grid points: {rel:.4e} "
             f"(fp64 direct {time.perf_counter() - tq:.2f}s)")
+        sample = (idx, ref)
+
+    # ---- learned far-field correction at its own near radius, against the
+    # analytic treecodes it sits between (dipole = its fallback, monopole),
+    # on the same state and the same FP64 sample
+    learned_rec = mlp_info
+    if not args.mlp and not args.no_learned and not args.no_accuracy and args.near_k >= 0:
+        try:
+            learned_rec = learned_leg(args, vg, v, g, tmn, tmx, sample, stream,
+                                      main_solve_ms=roofline["avg_launch_ms"], main_rel=rel)
+        except Exception as e:  # the learned leg never blocks the headline
+            log(f"learned leg skipped: {e!r}")
 
     # ---- CPU baseline: the oracle restatement of the same frame, timed on
     # bounded samples and scaled to the whole frame
@@ -315,13 +343,53 @@ def main():
                    "near_k": args.near_k, "order": args.order, "expand_from": args.expand_from,
                    "parallelism": "single GPU"},
         "rel_l2_vs_fp64_direct": rel,
-        "learned_correction": mlp_info,
+        "learned_correction": learned_rec,
         "roofline": roofline,
         "tracer_advection": adv,
         "cpu_baseline": cpu,
         "kernels_ms": {k: {"count": c, "total_ms": m} for k, (c, m) in prof.items()},
     }
     print(json.dumps(out))
+
+
+def learned_leg(args, vg, v, g, tmn, tmx, sample, stream, main_solve_ms, main_rel):
+    """Trains the far-field network on the ring, colliding-ring and plume
+    distributions (untimed), then times the velocity solve on the frame's
+    final vortons three ways at near radius K: monopole clusters, dipole
+    clusters (the network's fallback), dipole + the learned correction."""
+    from vortegrid_amd import learned
+
+    k = args.learned_near_k
+    log(f"learned leg: training the far-field correction for near_k={k}")
+    mlp, info = learned.train_far_field(near_k=k)
+    idx, ref = sample
+    out = {"near_k": k, "training": info,
+           "main_path": {"solve_ms": main_solve_ms, "rel_l2_vs_fp64_direct": main_rel}}
+    for name, order, net in (("monopole", 0, None), ("dipole", 1, None), ("learned", 1, mlp)):
+        sim = vg.Sim(vg.sim_config(technique=vg.TECH_TREE, rsqrt_policy=args.policy,
+                                   tree_near_k=k, tree_order=order, tree_expand_from=-1), stream)
+        sim.set_vortons(v)
+        if net is not None:
+            sim.set_mlp(net)
+        times = []
+        for rep in range(3):
+            sim.find_bounding_box()
+            sim.update_bounding_box(tmn, tmx, True)
+            stream.sync()
+            t0 = time.perf_counter()
+            sim.update(1.0 / 30.0, 0, None)
+            stream.sync()
+            times.append((time.perf_counter() - t0) * 1e3)
+        g2, p2 = sim.velocity_grid()
+        assert tuple(g2.np) == tuple(g.np)
+        got = vg.fetch(p2, 3 * g2.capacity).reshape(-1, 3)[idx].astype(np.float64)
+        err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        out[name] = {"update_ms": min(times), "rel_l2_vs_fp64_direct": err}
+        log(f"learned leg: {name:8s} k={k}: update {min(times):.2f} ms, rel-L2 {err:.3e}")
+    out["note"] = ("update_ms is the whole vg_sim_update (tree build + velocity grid) on the "
+                   "frame's final vortons; main_path.solve_ms is the default path's velocity "
+                   "solve launches alone")
+    return out
 
 
 def cpu_threads():
