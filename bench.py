@@ -89,6 +89,11 @@ def main():
     ap.add_argument("--effects", type=int, default=256)
     ap.add_argument("--batch-near-k", type=int, default=2)
     ap.add_argument("--batch-expand-from", type=int, default=2)
+    ap.add_argument("--autotune-json", default=None,
+                    help="write the batched autotuner's choice and trials to this file")
+    ap.add_argument("--batch-target", type=float, default=2e-2,
+                    help="accuracy target (rel-L2 vs FP64 direct) of the batched autotuner; "
+                         "<= 0 keeps --batch-near-k / --batch-expand-from")
     ap.add_argument("--graph-frame", action="store_true",
                     help="the default frame workload (1M vortons, 4M tracers) as one effect of the "
                          "batched path: bounding box, grid definition, tree, walk and Euler steps "
@@ -154,8 +159,8 @@ def main():
             vg.advect_tracers(grid_holder["g"], grid_holder["field"], d_tr, None, len(tr), dt, 1.0,
                               stream=stream)
         frame_no[0] += 1
-        tmn, tmx = vg.find_bounding_box(d_tr, len(tr), stream=stream)
-        sim.find_bounding_box()
+        # tracer box and vorton box in one pass and one wait
+        tmn, tmx = sim.find_bounding_boxes(d_tr, len(tr))
         sim.update_bounding_box(tmn, tmx, True)
         sim.update(dt, 0, d_vel)
         vg.evolve(d_pos, d_vel, len(pos), dt, stream=stream)
@@ -539,8 +544,17 @@ def batched(args):
     s = vg.Stream()
     b.step(dt, s)
     s.sync()
-    report = b.autotune(s)
-    log("autotune:\n" + report)
+    tuned = None
+    if args.batch_target > 0:
+        from vortegrid_amd import autotune
+
+        tuned = autotune.tune_batch(b, np.concatenate(vw), radii, target=args.batch_target,
+                                    order=args.order, stream=s, path=args.autotune_json)
+        nk = tuned["method"]["near_k"]
+        args.batch_expand_from = tuned["method"]["expand_from"]
+        log("autotune: " + json.dumps({k: tuned[k] for k in ("method", "launch", "chosen")}))
+    else:
+        log("autotune:\n" + b.autotune(s))
     b.capture(dt, s)
     for _ in range(args.warmup):
         b.launch(s)
@@ -652,7 +666,8 @@ def batched(args):
                                   if args.batch_expand_from >= 1 else ""),
                    "vortons": b.nv, "tracers": b.nt, "effects": args.effects},
         "rel_l2_vs_fp64_direct": rel,
-        "launch_shapes": dict(zip(("walk_waves", "tracer_block", "tracer_items"), b.get_launch())),
+        "launch_shapes": dict(zip(("walk_waves", "advect_block", "advect_items"), b.get_launch())),
+        "autotune": tuned,
         "walk_ms": wms / max(cnt, 1), "tracers_ms": tms / max(tcnt, 1),
         "tracers_GBs": 24.0 * b.nt / (tms / max(tcnt, 1) * 1e-3) / 1e9,
         "cpu_baseline": cpu,
@@ -784,8 +799,7 @@ def suite(args):
     def frame():
         if state["g"] is not None:
             vg.advect_tracers(state["g"], state["f"], dtr, None, len(tr), dt, 1.0)
-        tmn, tmx = vg.find_bounding_box(dtr, len(tr))
-        sim.find_bounding_box()
+        tmn, tmx = sim.find_bounding_boxes(dtr, len(tr))
         sim.update_bounding_box(tmn, tmx, True)
         sim.update(dt, 0, dvel)
         vg.evolve(dp, dvel, len(pos), dt)
@@ -794,14 +808,20 @@ def suite(args):
     for _ in range(3):
         frame()
     vg.synchronize()
-    vg.profile_reset()
-    vg.profile_enable(True)
-    t0 = time.perf_counter()
     steps = 100
+    # timed without per-launch events, then 20 more frames with them for the
+    # per-kernel figures
+    t0 = time.perf_counter()
     for _ in range(steps):
         frame()
     vg.synchronize()
     el = time.perf_counter() - t0
+    vg.profile_reset()
+    vg.profile_enable(True)
+    psteps = 20
+    for _ in range(psteps):
+        frame()
+    vg.synchronize()
     vg.profile_enable(False)
     prof = vg.profile_dump()
     g = state["g"]
@@ -809,10 +829,12 @@ def suite(args):
     recs.append({"workload": "treecode frame, 2 colliding rings N=65536 + 1M tracers, 100 steps",
                  "grid": list(g.shape), "ms_per_step": el * 1e3 / steps,
                  "evals_per_s": g.capacity * steps / el,
-                 "tree_grid_avg_ms": prof.get("vg_tree_velocity_grid", (1, 0.0))[1] / steps,
+                 "tree_grid_avg_ms": prof.get("vg_tree_velocity_grid", (1, 0.0))[1] / psteps,
+                 "kernel_sum_ms_per_step": sum(m for (c, m) in prof.values()) / psteps,
                  "advect_avg_ms": ms / max(c, 1),
                  "advect_GBs": gbs(24.0 * len(tr), ms / max(c, 1)),
-                 "kernels_ms": {k: {"count": c, "total_ms": m} for k, (c, m) in prof.items()}})
+                 "kernels_ms": {k: {"count": c, "total_ms": m} for k, (c, m) in prof.items()},
+                 "kernels_ms_frames": psteps})
 
     # ---- vector Poisson on 129^3: plume vorticity, treecode faces, V-cycles
     pos, om, radius = scenarios.jet_plume(262144, seed=3)

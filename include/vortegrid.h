@@ -516,15 +516,25 @@ vg_status vg_batch_launch(vg_batch *b, vg_stream s);
  * and its mollification factors.  Synchronise the device. */
 vg_status vg_batch_grid(const vg_batch *b, uint32_t effect, vg_grid_desc *g, const float **field);
 vg_status vg_batch_mollify(const vg_batch *b, uint32_t effect, float *spread, float *circ);
-/* Launch shapes: treecode waves per workgroup {1, 2, 4}; tracer workgroup
- * size (multiple of 64, <= 1024) and tracers per thread {1, 2, 4}. */
-vg_status vg_batch_set_launch(vg_batch *b, int walk_waves, int tracer_block, int tracer_items);
-vg_status vg_batch_get_launch(const vg_batch *b, int *walk_waves, int *tracer_block,
-                              int *tracer_items);
+/* The velocity grids of the current particles without moving them: bounds,
+ * geometry, trees and the walk (the middle of vg_batch_step; equal to a
+ * step with dt = 0).  For evaluating methods, e.g. by the autotuner. */
+vg_status vg_batch_solve(vg_batch *b, vg_stream s);
+/* Changes the walk's method (near-field radius, cluster order, first
+ * expanded layer) for later frames; drops a recorded frame. */
+vg_status vg_batch_set_method(vg_batch *b, int near_k, int order, int expand_from);
+/* Launch shapes: treecode waves per workgroup {1, 2, 4}; vorton-advection
+ * workgroup size (multiple of 64, <= 1024) and vortons per thread {1, 2, 4}
+ * (the tracers move in one fused advection + bounds pass). */
+vg_status vg_batch_set_launch(vg_batch *b, int walk_waves, int advect_block, int advect_items);
+vg_status vg_batch_get_launch(const vg_batch *b, int *walk_waves, int *advect_block,
+                              int *advect_items);
 /* Times every candidate launch shape on the live batch (after at least one
  * vg_batch_step) and keeps the fastest; report (may be NULL) receives one
  * line per trial.  Blocks.  Results of later frames do not depend on the
- * shapes chosen. */
+ * shapes chosen.  The method (near_k, expand_from) is chosen against an
+ * accuracy target by vortegrid_amd/autotune.py, which drives vg_batch_solve
+ * and this call. */
 vg_status vg_batch_autotune(vg_batch *b, vg_stream s, char *report, size_t cap);
 
 /* ------------------------------------------------------------ simulation */
@@ -562,6 +572,13 @@ vg_status vg_sim_set_mlp(vg_sim *sim, vg_mlp *m, uint32_t layer_mask);
 vg_status vg_sim_set_vortons(vg_sim *sim, const vg_vortons *v);
 /* VortonSim::FindBoundingBox (vortonSim.cpp:1001-1008). */
 vg_status vg_sim_find_bounding_box(vg_sim *sim);
+/* VortonSim::FindBoundingBox together with PclOpFindBoundingBox::Operate
+ * on one more particle group (the tracers, pclOpFindBoundingBox.cpp:
+ * 141-148), in one pass and one host wait: the sim's box becomes its
+ * vortons' box, and mn/mx (host, in/out) grow by the n strided positions
+ * at pos.  Equal to vg_find_bounding_box + vg_sim_find_bounding_box. */
+vg_status vg_sim_find_bounding_boxes(vg_sim *sim, const float *pos, size_t stride, size_t n,
+                                     float mn[3], float mx[3]);
 /* VortonSim::UpdateBoundingBox (vortonSim.cpp:1027-1096). */
 vg_status vg_sim_update_bounding_box(vg_sim *sim, const float mn[3],
                                      const float mx[3], int final_update);

@@ -142,3 +142,52 @@ def test_autotune_keeps_results(vg):
     vg.synchronize()
     for key in ("vpos", "vvel", "tpos"):
         assert np.array_equal(a.download(key), b.download(key)), key
+
+
+def test_solve_equals_a_still_step_and_set_method(vg):
+    """vg_batch_solve computes the grids of the current particles: equal to a
+    frame with dt = 0 (which moves nothing).  set_method switches the walk:
+    the grids then equal a batch created with that method."""
+    case("batched solve vs step(dt=0); set_method vs a batch created with the method")
+    counts = [1500, 700, 2600]
+    a, *_ = new_batch(vg, counts, seed=13, cells=16, near_k=1, order=1)
+    b, *_ = new_batch(vg, counts, seed=13, cells=16, near_k=1, order=1)
+    a.step(F32(0.0))
+    b.solve()
+    vg.synchronize()
+    for e in range(len(counts)):
+        assert np.array_equal(a.grid(e)[1], b.grid(e)[1]), e
+    for key in ("vpos", "tpos"):
+        assert np.array_equal(a.download(key), b.download(key)), key
+    c, *_ = new_batch(vg, counts, seed=13, cells=16, near_k=2, order=1, expand_from=2)
+    b.set_method(2, 1, 2)
+    b.solve()
+    c.solve()
+    vg.synchronize()
+    for e in range(len(counts)):
+        assert np.array_equal(b.grid(e)[1], c.grid(e)[1]), e
+
+
+def test_method_autotuner_meets_its_target(vg, tmp_path):
+    """The autotuner picks the fastest method within the accuracy target,
+    leaves the batch set to it, and its JSON applies to another batch."""
+    case("batched autotuner: method against an accuracy target, JSON round trip")
+    from vortegrid_amd import autotune
+
+    counts = [4096, 1200, 2800, 600, 3500, 2000]
+    b, vp, vw, tp, radii = new_batch(vg, counts, seed=17, cells=16, near_k=1, order=1)
+    b.step(DT)
+    path = tmp_path / "choice.json"
+    rec = autotune.tune_batch(b, np.concatenate(vw), radii, target=2e-2,
+                              candidates=((1, -1), (2, -1), (2, 2)), sample_points=256,
+                              path=str(path))
+    print(rec["trials"], rec["method"])
+    assert rec["met_target"] and rec["chosen"]["rel_l2"] <= 2e-2
+    fastest_ok = min((t for t in rec["trials"] if t["meets_target"]), key=lambda t: t["solve_ms"])
+    assert rec["chosen"] == fastest_ok
+    errs = {(t["near_k"], t["expand_from"]): t["rel_l2"] for t in rec["trials"]}
+    assert errs[(2, -1)] < errs[(1, -1)]
+    c, *_ = new_batch(vg, counts, seed=17, cells=16, near_k=1, order=1)
+    loaded = autotune.load_choice(c, str(path))
+    assert c.get_launch() == (loaded["launch"]["walk_waves"], loaded["launch"]["advect_block"],
+                              loaded["launch"]["advect_items"])

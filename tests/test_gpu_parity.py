@@ -299,6 +299,15 @@ def test_advection_and_bbox_bit_exact(vg, oracle, rings2):
     dpos = upload(vg, pos)
     a, b = vg.find_bounding_box(dpos, len(pos))
     assert np.array_equal(a, pos.min(0)) and np.array_equal(b, pos.max(0))
+    # vorton box and tracer box in one pass: the same as the two calls
+    dtr = upload(vg, tr)
+    sim = vg.Sim(vg.sim_config(technique=vg.TECH_TREE))
+    v = vg.vortons_view(dpos, upload(vg, om), len(pos), uniform_size=2 * radius)
+    sim.set_vortons(v)
+    tmn, tmx = sim.find_bounding_boxes(dtr, len(tr))
+    assert np.array_equal(tmn, tr.min(0)) and np.array_equal(tmx, tr.max(0))
+    smn, smx = sim.bounds()
+    assert np.array_equal(smn, pos.min(0)) and np.array_equal(smx, pos.max(0))
 
 
 def test_out_of_domain_tracers_are_clamped(vg, oracle, rings2):

@@ -159,6 +159,93 @@ __global__ __launch_bounds__(256) void k_bbox_batch(const float *__restrict__ po
     }
 }
 
+// Tracers of all effects in one pass over their positions: AssignVelocity-
+// FromField (gain 1) + Evolve with each owner's grid, and the extrema of the
+// moved positions folded into the owners' boxes (the tracer half of the
+// frame's bounding boxes).  Chunking and folding as k_bbox_batch; the
+// arithmetic is k_advect_batch's.
+__global__ __launch_bounds__(256) void k_advect_box_batch(const TreeView *__restrict__ views,
+                                                          const float *__restrict__ grid,
+                                                          size_t cap0, float *__restrict__ pos,
+                                                          const uint32_t *__restrict__ owner,
+                                                          size_t n, float dt,
+                                                          uint32_t *__restrict__ bbox) {
+#pragma clang fp contract(off)
+    const uint32_t lane = threadIdx.x & 63;
+    const size_t wave = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const size_t base = wave * 64 * kBoxItems;
+    uint32_t cur = 0xffffffffu;
+    uint32_t e[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+    auto fold = [&](uint32_t k, float x, float y, float z) {
+        if (k != cur) {
+            if (cur != 0xffffffffu) box_flush(bbox, cur, e);
+            cur = k;
+            e[0] = e[1] = e[2] = 0xffffffffu;
+            e[3] = e[4] = e[5] = 0u;
+        }
+        const uint32_t v[3] = {ord_enc(x), ord_enc(y), ord_enc(z)};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            e[a] = min(e[a], v[a]);
+            e[3 + a] = max(e[3 + a], v[a]);
+        }
+    };
+    auto move = [&](uint32_t k, float &x, float &y, float &z) {
+        const vg_grid_desc &g = views[k].layer[0];
+        const float3 v = interpolate(g, grid + 3 * cap0 * k, make_float3(x, y, z));
+        x = x + v.x * dt;
+        y = y + v.y * dt;
+        z = z + v.z * dt;
+    };
+    if (base + 64 * kBoxItems <= n) {
+#pragma unroll
+        for (int grp = 0; grp < kBoxItems / 4; ++grp) {
+            const size_t j = base + 256 * (size_t)grp + 4 * lane;
+            const uint4 k = *reinterpret_cast<const uint4 *>(owner + j);
+            float4 *p4 = reinterpret_cast<float4 *>(pos + 3 * j);
+            float4 a = p4[0], b = p4[1], c = p4[2];
+            move(k.x, a.x, a.y, a.z);
+            move(k.y, a.w, b.x, b.y);
+            move(k.z, b.z, b.w, c.x);
+            move(k.w, c.y, c.z, c.w);
+            p4[0] = a;
+            p4[1] = b;
+            p4[2] = c;
+            fold(k.x, a.x, a.y, a.z);
+            fold(k.y, a.w, b.x, b.y);
+            fold(k.z, b.z, b.w, c.x);
+            fold(k.w, c.y, c.z, c.w);
+        }
+    } else {
+        for (int it = 0; it < kBoxItems; ++it) {
+            const size_t i = base + (size_t)it * 64 + lane;
+            if (i >= n) break;
+            const uint32_t k = owner[i];
+            float x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+            move(k, x, y, z);
+            pos[3 * i] = x;
+            pos[3 * i + 1] = y;
+            pos[3 * i + 2] = z;
+            fold(k, x, y, z);
+        }
+    }
+    const uint32_t k0 = __builtin_amdgcn_readfirstlane(cur);
+    const bool uniform = __ballot(cur != 0xffffffffu && cur != k0) == 0ull;
+    if (uniform) {
+        if (k0 == 0xffffffffu) return;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                e[a] = min(e[a], (uint32_t)__shfl_xor((int)e[a], off, 64));
+                e[3 + a] = max(e[3 + a], (uint32_t)__shfl_xor((int)e[3 + a], off, 64));
+            }
+        if (lane == 0) box_flush(bbox, k0, e);
+    } else if (cur != 0xffffffffu) {
+        box_flush(bbox, cur, e);
+    }
+}
+
 struct Shape {
     uint32_t depth;
     uint32_t np[kMaxLayers][3];
@@ -281,18 +368,21 @@ __global__ void k_zero_f(float *p, size_t n) {
     if (i < n) p[i] = 0.f;
 }
 
-vg_status frame(vg_batch *b, float dt, hipStream_t s) {
-    // tracers first, with the grids of the previous frame
-    VG_TRY(advect(b, b->tpos, nullptr, b->ttree, b->nt, dt, b->adv_block, b->adv_items, s));
-    VG_TRY(launch("vg_batch_bbox", k_bbox_init, dim3(ceil_div(6 * b->effects, 256)), dim3(256), 0, s,
-                  b->bbox, b->effects));
+// Bounds, geometry, trees and the velocity grids of the current particles.
+// tracers_boxed: the tracers' extrema are already folded into b->bbox (by
+// the fused tracer pass), which then must not be cleared here.
+vg_status solve(vg_batch *b, bool tracers_boxed, hipStream_t s) {
+    if (!tracers_boxed) {
+        VG_TRY(launch("vg_batch_bbox", k_bbox_init, dim3(ceil_div(6 * b->effects, 256)), dim3(256),
+                      0, s, b->bbox, b->effects));
+        if (b->nt)
+            VG_TRY(launch("vg_batch_bbox", k_bbox_batch, dim3(ceil_div(b->nt, 256 * kBoxItems)),
+                          dim3(256), 0, s, (const float *)b->tpos, (const uint32_t *)b->ttree,
+                          b->nt, b->bbox));
+    }
     if (b->nv)
         VG_TRY(launch("vg_batch_bbox", k_bbox_batch, dim3(ceil_div(b->nv, 256 * kBoxItems)),
                       dim3(256), 0, s, (const float *)b->vpos, (const uint32_t *)b->vtree, b->nv,
-                      b->bbox));
-    if (b->nt)
-        VG_TRY(launch("vg_batch_bbox", k_bbox_batch, dim3(ceil_div(b->nt, 256 * kBoxItems)),
-                      dim3(256), 0, s, (const float *)b->tpos, (const uint32_t *)b->ttree, b->nt,
                       b->bbox));
     Shape sh;
     sh.depth = b->shape.depth;
@@ -325,10 +415,22 @@ vg_status frame(vg_batch *b, float dt, hipStream_t s) {
         ta.C[l] = b->C[l];
     }
     VG_TRY(batch_tree_build(ta, s));
-    VG_TRY(mac_velocity_grid_batch(b->views, b->mollify, b->effects, b->shape.layer[0].np,
+    return mac_velocity_grid_batch(b->views, b->mollify, b->effects, b->shape.layer[0].np,
                                    b->cfg.near_k, b->cfg.order, b->cfg.expand_from,
-                                   b->cfg.rsqrt_policy, b->walk_waves,
-                                   b->grid, 3 * b->cap0, s));
+                                   b->cfg.rsqrt_policy, b->walk_waves, b->grid, 3 * b->cap0, s);
+}
+
+vg_status frame(vg_batch *b, float dt, hipStream_t s) {
+    // tracers first, with the grids of the previous frame, folding their
+    // moved positions into the cleared boxes on the way
+    VG_TRY(launch("vg_batch_bbox", k_bbox_init, dim3(ceil_div(6 * b->effects, 256)), dim3(256), 0, s,
+                  b->bbox, b->effects));
+    if (b->nt)
+        VG_TRY(launch("vg_batch_tracers", k_advect_box_batch,
+                      dim3(ceil_div(b->nt, 256 * kBoxItems)), dim3(256), 0, s,
+                      (const TreeView *)b->views, (const float *)b->grid, b->cap0, b->tpos,
+                      (const uint32_t *)b->ttree, b->nt, dt, b->bbox));
+    VG_TRY(solve(b, true, s));
     return advect(b, b->vpos, b->vvel, b->vtree, b->nv, dt, b->adv_block, b->adv_items, s);
 }
 
@@ -502,6 +604,24 @@ VG_API vg_status vg_batch_buffers(vg_batch *b, float **vorton_pos, float **vorto
     return VG_OK;
 }
 
+VG_API vg_status vg_batch_set_method(vg_batch *b, int near_k, int order, int expand_from) {
+    VG_REQUIRE(b, "null batch");
+    VG_REQUIRE(near_k >= 0 && near_k <= 16, "near_k must be in [0, 16]");
+    VG_REQUIRE(order == 0 || order == 1, "order must be 0 or 1");
+    drop_graph(b);
+    b->cfg.near_k = near_k;
+    b->cfg.order = order;
+    b->cfg.expand_from = expand_from;
+    return VG_OK;
+}
+
+VG_API vg_status vg_batch_solve(vg_batch *b, vg_stream s) {
+    VG_REQUIRE(b, "null batch");
+    VG_TRY(solve(b, false, (hipStream_t)s));
+    b->stepped = true;
+    return VG_OK;
+}
+
 VG_API vg_status vg_batch_step(vg_batch *b, float dt, vg_stream s) {
     VG_REQUIRE(b, "null batch");
     VG_TRY(frame(b, dt, (hipStream_t)s));
@@ -635,22 +755,26 @@ VG_API vg_status vg_batch_autotune(vg_batch *b, vg_stream stream, char *report, 
         rep += line;
         if (ms < best) { best = ms; best_w = w; }
     }
-    // tracers: block size x items per thread, on a scratch copy of positions
+    // vorton advection: block size x items per thread, on scratch copies of
+    // positions and velocities (the tracers go through the fused
+    // advection + bounds pass, which has one shape)
     float *scratch = nullptr;
     int best_b = b->adv_block, best_i = b->adv_items;
-    if (st == VG_OK && b->nt) {
-        st = salloc((void **)&scratch, 3 * b->nt * sizeof(float), s);
+    if (st == VG_OK && b->nv) {
+        st = salloc((void **)&scratch, 6 * b->nv * sizeof(float), s);
         best = FLT_MAX;
         for (int blk : {128, 256, 512, 1024}) {
             for (int it : {1, 2, 4}) {
                 if (st != VG_OK) break;
                 float ms = 0.f;
-                hipError_t he = hipMemcpyAsync(scratch, b->tpos, 3 * b->nt * sizeof(float),
+                hipError_t he = hipMemcpyAsync(scratch, b->vpos, 3 * b->nv * sizeof(float),
                                                hipMemcpyDeviceToDevice, s);
                 if (he != hipSuccess) { st = device_error(he, "autotune copy", __FILE__, __LINE__); break; }
-                st = timed([&]() { return advect(b, scratch, nullptr, b->ttree, b->nt, 0.0f, blk, it, s); }, &ms);
+                st = timed([&]() {
+                    return advect(b, scratch, scratch + 3 * b->nv, b->vtree, b->nv, 0.0f, blk, it, s);
+                }, &ms);
                 char line[96];
-                snprintf(line, sizeof line, "tracers block=%d items=%d: %.3f ms\n", blk, it, ms);
+                snprintf(line, sizeof line, "vortons block=%d items=%d: %.3f ms\n", blk, it, ms);
                 rep += line;
                 if (ms < best) { best = ms; best_b = blk; best_i = it; }
             }
@@ -664,8 +788,8 @@ VG_API vg_status vg_batch_autotune(vg_batch *b, vg_stream stream, char *report, 
     b->adv_block = best_b;
     b->adv_items = best_i;
     char line[128];
-    snprintf(line, sizeof line, "chosen: walk waves_per_block=%d, tracers block=%d items=%d\n", best_w,
-             best_b, best_i);
+    snprintf(line, sizeof line, "chosen: walk waves_per_block=%d, vortons block=%d items=%d\n",
+             best_w, best_b, best_i);
     rep += line;
     if (report && cap) {
         snprintf(report, cap, "%s", rep.c_str());

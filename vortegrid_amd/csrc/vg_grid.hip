@@ -506,6 +506,54 @@ VG_API vg_status vg_advect_tracers(const vg_grid_desc *g, const float *field, fl
                   (hipStream_t)s, *g, field, pos, vel, stride, n, dt, gain, zero);
 }
 
+namespace vg {
+// Boxes of `count` particle arrays with one host wait: per array one
+// reduction pass and one fold, then a single readback of all extrema.
+vg_status find_bounding_boxes(uint32_t count, const float *const *pos, const size_t *stride,
+                              const size_t *n, float *mn, float *mx, hipStream_t s) {
+    VG_REQUIRE(count <= 8, "at most 8 arrays per call");
+    uint32_t nb[8] = {};
+    size_t words = 6 * (size_t)count;
+    for (uint32_t k = 0; k < count; ++k) {
+        if (n[k] == 0) continue;
+        VG_REQUIRE(pos[k] && stride[k] >= 3, "bad particle array");
+        nb[k] = ceil_div(n[k], 256 * 8);
+        if (nb[k] > 1024) nb[k] = 1024;
+        if (nb[k] < 1) nb[k] = 1;
+        words += 6 * (size_t)nb[k];
+    }
+    uint32_t *res = nullptr;
+    VG_TRY(salloc((void **)&res, words * sizeof(uint32_t), s));
+    vg_status st = VG_OK;
+    uint32_t *part = res + 6 * count;
+    for (uint32_t k = 0; k < count && st == VG_OK; ++k) {
+        if (n[k] == 0) continue;
+        st = launch("vg_bbox", k_bbox, dim3(nb[k]), dim3(256), 0, s, pos[k], stride[k], n[k], part);
+        if (st == VG_OK)
+            st = launch("vg_bbox_final", k_bbox_final, dim3(1), dim3(256), 0, s,
+                        (const uint32_t *)part, nb[k], res + 6 * k);
+        part += 6 * (size_t)nb[k];
+    }
+    uint32_t h[48];
+    if (st == VG_OK) {
+        hipError_t e = hipMemcpyAsync(h, res, 6 * count * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = device_error(e, "bbox copy", __FILE__, __LINE__);
+    }
+    sfree(res, s);
+    VG_TRY(st);
+    for (uint32_t k = 0; k < count; ++k) {
+        if (n[k] == 0) continue;
+        for (int a = 0; a < 3; ++a) {
+            const float lo = ord_dec(h[6 * k + a]), hi = ord_dec(h[6 * k + 3 + a]);
+            if (lo < mn[3 * k + a]) mn[3 * k + a] = lo;
+            if (hi > mx[3 * k + a]) mx[3 * k + a] = hi;
+        }
+    }
+    return VG_OK;
+}
+}  // namespace vg
+
 VG_API vg_status vg_find_bounding_box(const float *pos, size_t stride, size_t n, float mn[3],
                                       float mx[3], vg_stream stream) {
     VG_REQUIRE(mn && mx, "null bounds");

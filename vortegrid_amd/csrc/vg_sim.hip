@@ -28,6 +28,8 @@ vg_status stretch_tilt_fused(const vg_grid_desc &g, const float *vel, const floa
                              float dt, const uint32_t *order, hipStream_t s);
 vg_status tree_checked_view(const vg_tree *t, TreeView *v);
 uint32_t mg_cascade_depth(const vg_multigrid *m);
+vg_status find_bounding_boxes(uint32_t count, const float *const *pos, const size_t *stride,
+                              const size_t *n, float *mn, float *mx, hipStream_t s);
 vg_status pse_diffuse(const vg_tree *t, float *angvel, size_t ang_stride, const int32_t *birth,
                       size_t birth_stride, float viscosity, float dt, int order, hipStream_t s);
 }  // namespace vg
@@ -147,6 +149,29 @@ VG_API vg_status vg_sim_find_bounding_box(vg_sim *sim) {
     for (int a = 0; a < 3; ++a) { sim->mn[a] = FLT_MAX; sim->mx[a] = -FLT_MAX; }
     return vg_find_bounding_box(sim->v.pos, sim->v.stride, sim->v.n, sim->mn, sim->mx,
                                 sim->stream);
+}
+
+VG_API vg_status vg_sim_find_bounding_boxes(vg_sim *sim, const float *pos, size_t stride,
+                                            size_t n, float mn[3], float mx[3]) {
+    VG_REQUIRE(sim && sim->have_vortons, "no vortons set");
+    VG_REQUIRE(mn && mx && (n == 0 || (pos && stride >= 3)), "bad particle array");
+    float bmn[6], bmx[6];
+    for (int a = 0; a < 3; ++a) {
+        bmn[a] = FLT_MAX;
+        bmx[a] = -FLT_MAX;
+        bmn[3 + a] = mn[a];
+        bmx[3 + a] = mx[a];
+    }
+    const float *arrays[2] = {sim->v.pos, pos};
+    const size_t strides[2] = {sim->v.stride, stride}, counts[2] = {sim->v.n, n};
+    VG_TRY(find_bounding_boxes(2, arrays, strides, counts, bmn, bmx, sim->stream));
+    for (int a = 0; a < 3; ++a) {
+        sim->mn[a] = bmn[a];
+        sim->mx[a] = bmx[a];
+        mn[a] = bmn[3 + a];
+        mx[a] = bmx[3 + a];
+    }
+    return VG_OK;
 }
 
 VG_API vg_status vg_sim_update_bounding_box(vg_sim *sim, const float mn[3], const float mx[3],

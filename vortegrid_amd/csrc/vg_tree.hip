@@ -432,14 +432,22 @@ __global__ __launch_bounds__(256) void k_tree_points(const TreeView tv, QuerySrc
     out[3 * i + 2] = r.z * circ;
 }
 
+// One wave per 4x4x4 brick of grid points (lane -> x + 4 y + 16 z inside the
+// brick): the points of a wave share their opened cells on the upper layers,
+// so their cluster loads fall on the same addresses, and on the lower layers
+// on neighbouring ones.
 template <int POLICY>
-__global__ __launch_bounds__(256) void k_tree_grid(const TreeView tv, vg_grid_desc g, float spread,
+__global__ __launch_bounds__(256) void k_tree_grid(const TreeView tv, vg_grid_desc g, uint32_t nbx,
+                                                   uint32_t nby, uint32_t nbricks, float spread,
                                                    float circ, float *__restrict__ out) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= capacity(g)) return;
-    const uint32_t ix = (uint32_t)(i % g.np[0]);
-    const uint32_t iy = (uint32_t)((i / g.np[0]) % g.np[1]);
-    const uint32_t iz = (uint32_t)(i / ((size_t)g.np[0] * g.np[1]));
+    const uint32_t brick = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (brick >= nbricks) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t ix = 4u * (brick % nbx) + (lane & 3u);
+    const uint32_t iy = 4u * ((brick / nbx) % nby) + ((lane >> 2) & 3u);
+    const uint32_t iz = 4u * (brick / (nbx * nby)) + (lane >> 4);
+    if (ix >= g.np[0] || iy >= g.np[1] || iz >= g.np[2]) return;
+    const size_t i = point_offset(g, ix, iy, iz);
     const float3 r = eval_query<POLICY, 0>(tv, grid_query_point(g, ix, iy, iz), spread);
     out[3 * i] = r.x * circ;
     out[3 * i + 1] = r.y * circ;
@@ -931,8 +939,10 @@ VG_API vg_status vg_tree_velocity_grid(const vg_tree *t, const vg_grid_desc *g, 
     if (t->n == 0) return zero_fill(out, 3 * cap * sizeof(float), (hipStream_t)s);
     auto k = rsqrt_policy == VG_RSQRT_REFERENCE ? k_tree_grid<VG_RSQRT_REFERENCE>
                                                 : k_tree_grid<VG_RSQRT_ACCURATE>;
-    return launch("vg_tree_velocity_grid", k, dim3(ceil_div(cap, 256)), dim3(256), 0,
-                  (hipStream_t)s, make_view(t), *g, spread, circ, out);
+    const uint32_t nbx = (g->np[0] + 3) / 4, nby = (g->np[1] + 3) / 4, nbz = (g->np[2] + 3) / 4;
+    const uint32_t nbricks = nbx * nby * nbz;
+    return launch("vg_tree_velocity_grid", k, dim3(ceil_div(nbricks, 4)), dim3(256), 0,
+                  (hipStream_t)s, make_view(t), *g, nbx, nby, nbricks, spread, circ, out);
 }
 
 VG_API vg_status vg_tree_interaction_count(const vg_tree *t, const vg_grid_desc *g,

@@ -36,14 +36,16 @@ EXPORTS = (
     "vg_assign_velocity_from_field "
     "vg_evolve vg_sort_particles_by_cell "
     "vg_advect_tracers vg_sim_config_default vg_sim_create vg_sim_destroy vg_sim_set_vortons "
-    "vg_sim_find_bounding_box vg_sim_update_bounding_box vg_sim_update vg_sim_velocity_grid "
+    "vg_sim_find_bounding_box vg_sim_find_bounding_boxes vg_sim_update_bounding_box "
+    "vg_sim_update vg_sim_velocity_grid "
     "vg_sim_vector_potential vg_sim_negative_vorticity vg_sim_grid_template vg_sim_bounds "
     "vg_sim_tree vg_sim_spreading vg_sim_poisson_residual "
     "vg_mlp_create vg_mlp_destroy vg_mlp_set_params vg_mlp_get_params vg_mlp_forward "
     "vg_mlp_gradients vg_mlp_train vg_tree_mlp_samples vg_tree_velocity_grid_mlp vg_sim_set_mlp "
     "vg_batch_config_default vg_batch_create vg_batch_destroy vg_batch_buffers vg_batch_step "
     "vg_batch_capture vg_batch_launch vg_batch_grid vg_batch_mollify vg_batch_set_launch "
-    "vg_batch_get_launch vg_batch_autotune vg_jacobian vg_stretch_tilt "
+    "vg_batch_get_launch vg_batch_autotune vg_batch_solve vg_batch_set_method vg_jacobian "
+    "vg_stretch_tilt "
     "vg_stretch_tilt_from_velocity vg_sim_stretch_tilt vg_tree_partition vg_tree_pse_diffuse "
     "vg_sim_pse_diffuse"
 ).split()
@@ -553,6 +555,13 @@ class Batch:
     def step(self, dt, stream=None):
         call("vg_batch_step", self.h, C.c_float(dt), _s(stream))
 
+    def solve(self, stream=None):
+        """Velocity grids of the current particles without moving them."""
+        call("vg_batch_solve", self.h, _s(stream))
+
+    def set_method(self, near_k, order, expand_from):
+        call("vg_batch_set_method", self.h, int(near_k), int(order), int(expand_from))
+
     def capture(self, dt, stream):
         call("vg_batch_capture", self.h, C.c_float(dt), _s(stream))
 
@@ -726,6 +735,14 @@ class Sim:
 
     def find_bounding_box(self):
         call("vg_sim_find_bounding_box", self.h)
+
+    def find_bounding_boxes(self, pos, n: int, stride=3):
+        """Vorton box (kept) and the box of n other particles (returned), one wait."""
+        mn = np.full(3, np.finfo(np.float32).max, np.float32)
+        mx = -mn
+        call("vg_sim_find_bounding_boxes", self.h, _p(pos), C.c_size_t(stride), C.c_size_t(n),
+             mn.ctypes.data_as(C.c_void_p), mx.ctypes.data_as(C.c_void_p))
+        return mn, mx
 
     def update_bounding_box(self, mn, mx, final=True):
         a = (C.c_float * 3)(*[float(x) for x in mn])
