@@ -91,7 +91,7 @@ def main():
     ap.add_argument("--batch-expand-from", type=int, default=2)
     ap.add_argument("--autotune-json", default=None,
                     help="write the batched autotuner's choice and trials to this file")
-    ap.add_argument("--batch-target", type=float, default=2e-2,
+    ap.add_argument("--batch-target", type=float, default=2.5e-2,
                     help="accuracy target (rel-L2 vs FP64 direct) of the batched autotuner; "
                          "<= 0 keeps --batch-near-k / --batch-expand-from")
     ap.add_argument("--graph-frame", action="store_true",

@@ -76,8 +76,8 @@ class _Judge:
         return (num / den) ** 0.5 if den > 0 else 0.0
 
 
-def tune_batch(batch, vom_host, radii, target=2e-2, candidates=CANDIDATES, order=1,
-               sample_effects=8, sample_points=512, reps=3, seed=0, stream=None, path=None):
+def tune_batch(batch, vom_host, radii, target=2.5e-2, candidates=CANDIDATES, order=1,
+               sample_effects=16, sample_points=512, reps=3, seed=0, stream=None, path=None):
     """Pick (near_k, expand_from) and the launch shapes for `batch` (uploaded
     and stepped at least once).  vom_host: all effects' angular velocities
     (the batch's own order), radii: vorton radius per effect.  Leaves the

@@ -432,23 +432,120 @@ __global__ __launch_bounds__(256) void k_tree_points(const TreeView tv, QuerySrc
     out[3 * i + 2] = r.z * circ;
 }
 
-// One wave per 4x4x4 brick of grid points (lane -> x + 4 y + 16 z inside the
-// brick): the points of a wave share their opened cells on the upper layers,
-// so their cluster loads fall on the same addresses, and on the lower layers
-// on neighbouring ones.
+// The reference-rule walk of one query split over SPLIT lanes (lanes of one
+// query are adjacent and follow the same control flow): on every layer the
+// children of the parent's opened box are dealt round robin to the lanes,
+// which sum their share; the shares are added across the lanes (the same
+// value on each) before the layer's Kahan step.  Opening decisions are those
+// of tree_eval; only the summation order differs.  Splitting a query puts
+// SPLIT times as many waves in flight, which is what hides the cluster
+// loads' latency: the walk does ~50 dependent gathers per query and little
+// arithmetic.
+template <int POLICY, int VECPOT, int SPLIT>
+__device__ float3 tree_eval_split(const TreeView &tv, float3 q, float spread, uint32_t sub) {
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    float3 sum = make_float3(0.f, 0.f, 0.f), comp = sum;
+    const float qa[3] = {q.x, q.y, q.z};
+    for (int il = (int)tv.depth - 1; il >= 1; --il) {
+        const vg_grid_desc &gc = tv.layer[il - 1];
+        const uint32_t *dims = tv.dec[il];
+        int c0[3], c1[3], olo[3], ohi[3];
+        bool open_any = il > 1;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            c0[a] = lo[a] * (int)dims[a];
+            c1[a] = hi[a] * (int)dims[a] + (int)dims[a] - 1;
+            olo[a] = 1 << 30;
+            ohi[a] = -1;
+        }
+        if (open_any) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float cell = gc.cell[a];
+                float m;
+                {
+#pragma clang fp contract(off)
+                    m = 0.0001f * cell;
+                    if (a == 2 && 0.0f == cell) m = m + FLT_MIN;
+                }
+                for (int c = c0[a]; c <= c1[a]; ++c) {
+                    const float mn = madd_exact(gc.min[a], (float)c, cell);
+                    const float mx = madd_exact(gc.min[a], (float)(c + 1), cell);
+                    bool in;
+                    {
+#pragma clang fp contract(off)
+                        in = qa[a] >= mn - m && qa[a] < mx + m;
+                    }
+                    if (in) {
+                        olo[a] = c < olo[a] ? c : olo[a];
+                        ohi[a] = c > ohi[a] ? c : ohi[a];
+                    }
+                }
+                if (ohi[a] < olo[a]) open_any = false;
+            }
+        }
+        const float4 *__restrict__ P = tv.P[il - 1];
+        const float4 *__restrict__ W = tv.W[il - 1];
+        const int bx = c1[0] - c0[0] + 1, by = c1[1] - c0[1] + 1, bz = c1[2] - c0[2] + 1;
+        const int total = bx * by * bz;
+        float3 part = make_float3(0.f, 0.f, 0.f);
+        for (int t = (int)sub; t < total; t += SPLIT) {
+            const int rx = t % bx, r = t / bx;
+            const int cx = c0[0] + rx, cy = c0[1] + r % by, cz = c0[2] + r / by;
+            if (open_any && cx >= olo[0] && cx <= ohi[0] && cy >= olo[1] && cy <= ohi[1] &&
+                cz >= olo[2] && cz <= ohi[2])
+                continue;
+            const size_t off = point_offset(gc, cx, cy, cz);
+            const float4 p = P[off];
+            const float4 w = W[off];
+            if (p.w == 0.0f) continue;  // empty cluster: zero circulation
+            const float radius = p.w * 0.5f * spread;
+            const float r2 = radius * radius;
+            const float s = (2.0f / 3.0f) * r2 * radius;
+            const float4 A = make_float4(p.x, p.y, p.z, r2);
+            const float4 B =
+                make_float4(w.x * s, w.y * s, w.z * s, __builtin_amdgcn_rcpf(r2 * radius));
+            if (VECPOT) pair_vecpot<POLICY>(part, q, A, B);
+            else pair_velocity<POLICY>(part, q, A, B);
+        }
+#pragma unroll
+        for (int m = 1; m < SPLIT; m <<= 1) {
+            part.x += __shfl_xor(part.x, m, 64);
+            part.y += __shfl_xor(part.y, m, 64);
+            part.z += __shfl_xor(part.z, m, 64);
+        }
+        kahan_add(sum, comp, part);
+        if (!open_any) break;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { lo[a] = olo[a]; hi[a] = ohi[a]; }
+    }
+    return sum;
+}
+
+// One workgroup per 4x4x4 brick of grid points, four lanes per point (lane
+// t: point t / 4 of the brick, x fastest, share t % 4 of its walk).
+constexpr int kTreeSplit = 4;
 template <int POLICY>
 __global__ __launch_bounds__(256) void k_tree_grid(const TreeView tv, vg_grid_desc g, uint32_t nbx,
                                                    uint32_t nby, uint32_t nbricks, float spread,
                                                    float circ, float *__restrict__ out) {
-    const uint32_t brick = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const uint32_t brick = blockIdx.x;
     if (brick >= nbricks) return;
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t ix = 4u * (brick % nbx) + (lane & 3u);
-    const uint32_t iy = 4u * ((brick / nbx) % nby) + ((lane >> 2) & 3u);
-    const uint32_t iz = 4u * (brick / (nbx * nby)) + (lane >> 4);
-    if (ix >= g.np[0] || iy >= g.np[1] || iz >= g.np[2]) return;
+    const uint32_t sub = threadIdx.x % kTreeSplit, pt = threadIdx.x / kTreeSplit;
+    uint32_t ix = 4u * (brick % nbx) + (pt & 3u);
+    uint32_t iy = 4u * ((brick / nbx) % nby) + ((pt >> 2) & 3u);
+    uint32_t iz = 4u * (brick / (nbx * nby)) + (pt >> 4);
+    // points past the grid's end walk the last point (their lanes take part
+    // in the cross-lane sums) and write nothing
+    const bool valid = ix < g.np[0] && iy < g.np[1] && iz < g.np[2];
+    ix = ix < g.np[0] ? ix : g.np[0] - 1;
+    iy = iy < g.np[1] ? iy : g.np[1] - 1;
+    iz = iz < g.np[2] ? iz : g.np[2] - 1;
     const size_t i = point_offset(g, ix, iy, iz);
-    const float3 r = eval_query<POLICY, 0>(tv, grid_query_point(g, ix, iy, iz), spread);
+    const float3 q = grid_query_point(g, ix, iy, iz);
+    const float3 r = tv.depth > 1 ? tree_eval_split<POLICY, 0, kTreeSplit>(tv, q, spread, sub)
+                                  : sorted_direct<POLICY, 0>(tv, q, spread);
+    if (!valid || sub != 0) return;
     out[3 * i] = r.x * circ;
     out[3 * i + 1] = r.y * circ;
     out[3 * i + 2] = r.z * circ;
@@ -514,12 +611,18 @@ template <int POLICY>
 __global__ __launch_bounds__(256) void k_tree_faces(const TreeView tv, vg_grid_desc g,
                                                     size_t nface, int copy_odd, float spread,
                                                     float circ, float *__restrict__ out) {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // kTreeSplit lanes per face point, which leave together
+    const size_t gt = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t t = gt / kTreeSplit;
+    const uint32_t sub = (uint32_t)(gt % kTreeSplit);
     if (t >= nface) return;
     uint32_t ix, iy, iz;
     if (!face_point(g, t, ix, iy, iz)) return;
     if (copy_odd && (ix & 1u)) return;
-    const float3 r = eval_query<POLICY, 1>(tv, grid_query_point(g, ix, iy, iz), spread);
+    const float3 q = grid_query_point(g, ix, iy, iz);
+    const float3 r = tv.depth > 1 ? tree_eval_split<POLICY, 1, kTreeSplit>(tv, q, spread, sub)
+                                  : sorted_direct<POLICY, 1>(tv, q, spread);
+    if (sub != 0) return;
     const size_t o = point_offset(g, ix, iy, iz);
     out[3 * o] = r.x * circ;
     out[3 * o + 1] = r.y * circ;
@@ -941,7 +1044,7 @@ VG_API vg_status vg_tree_velocity_grid(const vg_tree *t, const vg_grid_desc *g, 
                                                 : k_tree_grid<VG_RSQRT_ACCURATE>;
     const uint32_t nbx = (g->np[0] + 3) / 4, nby = (g->np[1] + 3) / 4, nbz = (g->np[2] + 3) / 4;
     const uint32_t nbricks = nbx * nby * nbz;
-    return launch("vg_tree_velocity_grid", k, dim3(ceil_div(nbricks, 4)), dim3(256), 0,
+    return launch("vg_tree_velocity_grid", k, dim3(nbricks), dim3(64 * kTreeSplit), 0,
                   (hipStream_t)s, make_view(t), *g, nbx, nby, nbricks, spread, circ, out);
 }
 
@@ -975,9 +1078,9 @@ VG_API vg_status vg_tree_vecpot_boundary(const vg_tree *t, const vg_grid_desc *g
     hipStream_t s = (hipStream_t)stream;
     const size_t nf = face_count(*g);
     // With no vortons every cluster is empty and the faces evaluate to zero.
-    auto k =rsqrt_policy == VG_RSQRT_REFERENCE ? k_tree_faces<VG_RSQRT_REFERENCE>
+    auto k = rsqrt_policy == VG_RSQRT_REFERENCE ? k_tree_faces<VG_RSQRT_REFERENCE>
                                                 : k_tree_faces<VG_RSQRT_ACCURATE>;
-    VG_TRY(launch("vg_tree_vecpot_faces", k, dim3(ceil_div(nf, 256)), dim3(256), 0, s,
+    VG_TRY(launch("vg_tree_vecpot_faces", k, dim3(ceil_div(nf * kTreeSplit, 256)), dim3(256), 0, s,
                   make_view(t), *g, nf, copy_odd, spread, circ, out));
     if (copy_odd)
         VG_TRY(launch("vg_tree_faces_copy_odd", k_faces_copy_odd, dim3(ceil_div(nf, 256)),
