@@ -161,12 +161,12 @@ __global__ __launch_bounds__(256) void k_bbox_batch(const float *__restrict__ po
 
 // Tracers of all effects in one pass over their positions: AssignVelocity-
 // FromField (gain 1) + Evolve with each owner's grid (k_advect_batch's
-// arithmetic, one tracer per thread so the grid gathers of many tracers are
-// in flight at once), and the extrema of the moved positions folded into
-// the owners' boxes (the tracer half of the frame's bounding boxes).
-// Owners are contiguous, so a wave nearly always holds one owner: it then
-// reduces across its lanes before one lane's 6 atomics; a wave that spans
-// two owners lets each lane flush its own.
+// arithmetic), and the extrema of the moved positions folded into the
+// owners' boxes (the tracer half of the frame's bounding boxes).  A wave
+// takes kBoxItems x 64 consecutive tracers, 64 at a time (coalesced), four
+// rounds of gathers in flight per lane; its lanes fold their extrema while
+// the owner stays the same and the wave reduces before its atomics, as in
+// k_bbox_batch, so the atomics stay few (owners are contiguous).
 __global__ __launch_bounds__(256) void k_advect_box_batch(const TreeView *__restrict__ views,
                                                           const float *__restrict__ grid,
                                                           size_t cap0, float *__restrict__ pos,
@@ -174,26 +174,55 @@ __global__ __launch_bounds__(256) void k_advect_box_batch(const TreeView *__rest
                                                           size_t n, float dt,
                                                           uint32_t *__restrict__ bbox) {
 #pragma clang fp contract(off)
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < n;
-    uint32_t k = 0xffffffffu;
-    uint32_t e[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
-    if (live) {
-        k = owner[i];
-        const vg_grid_desc &g = views[k].layer[0];
-        float *p = pos + 3 * i;
-        const float3 v = interpolate(g, grid + 3 * cap0 * k, make_float3(p[0], p[1], p[2]));
-        const float x = p[0] + v.x * dt, y = p[1] + v.y * dt, z = p[2] + v.z * dt;
-        p[0] = x;
-        p[1] = y;
-        p[2] = z;
-        e[0] = e[3] = ord_enc(x);
-        e[1] = e[4] = ord_enc(y);
-        e[2] = e[5] = ord_enc(z);
-    }
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t k0 = __builtin_amdgcn_readfirstlane(k);
-    const bool uniform = __ballot(live && k != k0) == 0ull;
+    const size_t wave = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const size_t base = wave * 64 * kBoxItems;
+    uint32_t cur = 0xffffffffu;
+    uint32_t e[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+    auto fold = [&](uint32_t k, float x, float y, float z) {
+        if (k != cur) {
+            if (cur != 0xffffffffu) box_flush(bbox, cur, e);
+            cur = k;
+            e[0] = e[1] = e[2] = 0xffffffffu;
+            e[3] = e[4] = e[5] = 0u;
+        }
+        const uint32_t v[3] = {ord_enc(x), ord_enc(y), ord_enc(z)};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            e[a] = min(e[a], v[a]);
+            e[3 + a] = max(e[3 + a], v[a]);
+        }
+    };
+    constexpr int kRound = 4;
+    for (int it0 = 0; it0 < kBoxItems; it0 += kRound) {
+        uint32_t k[kRound];
+        float3 p[kRound], v[kRound];
+        bool live[kRound];
+#pragma unroll
+        for (int r = 0; r < kRound; ++r) {
+            const size_t i = base + (size_t)(it0 + r) * 64 + lane;
+            live[r] = i < n;
+            k[r] = live[r] ? owner[i] : 0u;
+            p[r] = live[r] ? make_float3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2])
+                           : make_float3(0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int r = 0; r < kRound; ++r)
+            if (live[r]) v[r] = interpolate(views[k[r]].layer[0], grid + 3 * cap0 * k[r], p[r]);
+#pragma unroll
+        for (int r = 0; r < kRound; ++r) {
+            if (!live[r]) continue;
+            const size_t i = base + (size_t)(it0 + r) * 64 + lane;
+            const float x = p[r].x + v[r].x * dt, y = p[r].y + v[r].y * dt,
+                        z = p[r].z + v[r].z * dt;
+            pos[3 * i] = x;
+            pos[3 * i + 1] = y;
+            pos[3 * i + 2] = z;
+            fold(k[r], x, y, z);
+        }
+    }
+    const uint32_t k0 = __builtin_amdgcn_readfirstlane(cur);
+    const bool uniform = __ballot(cur != 0xffffffffu && cur != k0) == 0ull;
     if (uniform) {
         if (k0 == 0xffffffffu) return;
 #pragma unroll
@@ -204,8 +233,8 @@ __global__ __launch_bounds__(256) void k_advect_box_batch(const TreeView *__rest
                 e[3 + a] = max(e[3 + a], (uint32_t)__shfl_xor((int)e[3 + a], off, 64));
             }
         if (lane == 0) box_flush(bbox, k0, e);
-    } else if (live) {
-        box_flush(bbox, k, e);
+    } else if (cur != 0xffffffffu) {
+        box_flush(bbox, cur, e);
     }
 }
 
@@ -389,8 +418,8 @@ vg_status frame(vg_batch *b, float dt, hipStream_t s) {
     VG_TRY(launch("vg_batch_bbox", k_bbox_init, dim3(ceil_div(6 * b->effects, 256)), dim3(256), 0, s,
                   b->bbox, b->effects));
     if (b->nt)
-        VG_TRY(launch("vg_batch_tracers", k_advect_box_batch, dim3(ceil_div(b->nt, 256)),
-                      dim3(256), 0, s,
+        VG_TRY(launch("vg_batch_tracers", k_advect_box_batch,
+                      dim3(ceil_div(b->nt, 256 * kBoxItems)), dim3(256), 0, s,
                       (const TreeView *)b->views, (const float *)b->grid, b->cap0, b->tpos,
                       (const uint32_t *)b->ttree, b->nt, dt, b->bbox));
     VG_TRY(solve(b, true, s));
