@@ -259,3 +259,100 @@ def test_default_frame_accuracy_1m(vg):
             assert g.shape == (129, 129, 129)
         vg.evolve(dp, dv, n, float(DT), stream=s)
     assert max(errs) <= 1e-3, errs
+
+
+# ------------------------------------------------------------ direct sum (4096-vorton ring)
+@pytest.mark.parametrize("policy", [0, 1])
+def test_ring_4096_direct_at_vortons_and_grid(vg, oracle, policy):
+    """The 4096-vorton ring (torus R=1, r=0.25, |w| = 1 + N(0, 0.05), vorton
+    radius 0.05): direct velocity at every vorton and at every point of a
+    32^3-cell grid over the padded bounding box, against the oracle (same
+    rsqrt policy) and against FP64 (within the policy's own error)."""
+    pos, om, radius = scenarios.vortex_ring(4096, seed=1)
+    n = len(pos)
+    case(f"ring N={n}: direct at vortons + 32^3-cell grid, policy {policy}")
+    dp, dw = vg.DeviceArray.from_numpy(pos), vg.DeviceArray.from_numpy(om)
+    v = vg.vortons_view(dp, dw, n, uniform_size=2 * radius)
+    mn, mx = vg.find_bounding_box(dp, n)
+    g0 = vg.finalize_bounding_box(mn, mx, radius, n)
+    g = vg.define_shape(32 ** 3, g0.min, [g0.min[a] + g0.extent[a] for a in range(3)])
+    og = to_oracle_grid(oracle, g)
+    assert (g.np[0] - 1) * (g.np[1] - 1) * (g.np[2] - 1) == 32 ** 3
+    out = vg.DeviceArray.zeros((n, 3))
+    vg.direct_velocity(v, dp, n, out, policy=policy)
+    gout = vg.DeviceArray.zeros((g.capacity, 3))
+    vg.direct_velocity_grid(v, g, gout, policy=policy)
+    o64 = vg.DeviceArray.zeros((n, 3), np.float64)
+    vg.direct_velocity_f64(v, dp, n, o64)
+    vg.synchronize()
+    want = oracle.direct_velocity(pos, om, 2 * radius, pos, policy=policy, threads=8)
+    assert rel_l2(out.numpy(), want) < 1e-5
+    wgrid = oracle.velocity_grid(None, og, pos, om, 2 * radius, technique=0, policy=policy,
+                                 threads=8)
+    assert rel_l2(gout.numpy(), wgrid) < 1e-5
+    w64 = oracle.direct_velocity_f64(pos, om, 2 * radius, pos, threads=8)
+    assert rel_l2(o64.numpy(), w64) < 1e-12
+    # the FP32 sum against FP64: the reference rsqrt's one Newton step is good
+    # to 1.75e-3, the hardware one to float rounding
+    assert rel_l2(out.numpy(), w64) < (1.8e-3 if policy == 0 else 1e-5)
+
+
+# ------------------------------------------------------------ treecode frame (65,536 vortons)
+def test_colliding_rings_65536_frames(vg, oracle):
+    """Two coaxial colliding rings (N=65,536) with 1M tracers uniform in their
+    box, the reference-rule treecode on the 64^3-cell grid, 100 frames of
+    dt = 1/30 in the reference order.  Frames 0, 1, 50 and 99 are each checked
+    against the oracle from the GPU's own state before the frame; every frame
+    stays finite and leaves the vorticity unchanged (the default hybrid
+    technique moves vortons only)."""
+    pos, om, radius = scenarios.colliding_rings(65536, seed=2)
+    mn, mx = pos.min(0), pos.max(0)
+    tr = scenarios.tracers_in_box(1 << 20, mn, mx, seed=3)
+    case(f"colliding rings N={len(pos)}, {len(tr)} tracers, 100 frames, reference-rule treecode")
+    dp, dw = vg.DeviceArray.from_numpy(pos), vg.DeviceArray.from_numpy(om)
+    dvel, dtr = vg.DeviceArray.zeros(pos.shape), vg.DeviceArray.from_numpy(tr)
+    v = vg.vortons_view(dp, dw, len(pos), uniform_size=2 * radius)
+    sim = vg.Sim(vg.sim_config(technique=vg.TECH_TREE, rsqrt_policy=0))
+    sim.set_vortons(v)
+    dt = F32(1.0 / 30.0)
+    grid = None
+    checked = []
+    for frame in range(100):
+        check = frame in (0, 1, 50, 99)
+        if check:
+            vg.synchronize()
+            p0, t0 = dp.numpy(), dtr.numpy()
+            g_prev = None if grid is None else (to_oracle_grid(oracle, grid[0]),
+                                                vg.fetch(grid[1], 3 * grid[0].capacity).reshape(-1, 3))
+        if grid is not None:
+            vg.advect_tracers(grid[0], grid[1], dtr, None, len(tr), float(dt), 1.0)
+        tmn, tmx = sim.find_bounding_boxes(dtr, len(tr))
+        sim.update_bounding_box(tmn, tmx, True)
+        sim.update(float(dt), frame, dvel)
+        vg.evolve(dp, dvel, len(pos), float(dt))
+        grid = sim.velocity_grid()
+        if not check:
+            continue
+        vg.synchronize()
+        g, field = grid[0], vg.fetch(grid[1], 3 * grid[0].capacity).reshape(-1, 3)
+        assert g.shape == (65, 65, 65) and np.all(np.isfinite(field))
+        # tracers moved with the previous grid, bit for bit
+        want_t = t0 if g_prev is None else oracle.advect(g_prev[0], g_prev[1], t0,
+                                                         np.zeros_like(t0), dt, 1.0)[0]
+        assert np.array_equal(dtr.numpy(), want_t), frame
+        # the frame's grid from the vortons before the frame
+        og = oracle.grid_for(p0, radius, extra_pos=want_t)
+        assert list(og.np) == list(g.np) and np.array_equal(np.float32(og.min), np.float32(g.min))
+        ot = oracle.InfluenceTree(og, p0, om, 2 * radius)
+        s, c = oracle.auto_mollify(og, radius)
+        want = oracle.velocity_grid(ot, og, p0, om, 2 * radius, 1, s, c, 0, threads=8)
+        err = rel_l2(field, want)
+        assert err < 1e-5, (frame, err)
+        wp, wv = oracle.advect(og, want, p0, np.zeros_like(p0), dt, 1.0)
+        assert rel_l2(dvel.numpy(), wv) < 1e-5
+        assert rel_l2(dp.numpy(), wp) < 1e-6
+        checked.append((frame, err))
+    vg.synchronize()
+    assert np.all(np.isfinite(dp.numpy())) and np.all(np.isfinite(dtr.numpy()))
+    assert np.array_equal(dw.numpy(), om)
+    print(f"[result] frames checked against the oracle (frame, rel-L2): {checked}", flush=True)
