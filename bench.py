@@ -905,6 +905,43 @@ def suite(args):
         },
         "vcycle_kernels_ms": {k: {"count": c, "total_ms": m} for k, (c, m) in mgprof.items()},
     })
+    # ---- the reference's default technique: a whole POISSON update (tree,
+    # vorticity splat, treecode faces, the reference cascade with 16 SOR
+    # steps per level visit, curl, vortons' grid velocity) on the plume
+    psim = vg.Sim(vg.sim_config(technique=vg.TECH_POISSON, rsqrt_policy=args.policy))
+    pvel = vg.DeviceArray.zeros((n, 3))
+    psim.set_vortons(v)
+
+    def poisson_update():
+        psim.find_bounding_box()
+        psim.update_bounding_box(mn, mx, True)
+        psim.update(1.0 / 30.0, 0, pvel)
+
+    poisson_update()
+    vg.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        poisson_update()
+    vg.synchronize()
+    upd_ms = (time.perf_counter() - t0) * 1e3 / reps
+    vg.profile_reset()
+    vg.profile_enable(True)
+    poisson_update()
+    vg.synchronize()
+    vg.profile_enable(False)
+    pprof = vg.profile_dump()
+    pg = psim.grid_template()
+    sc, sms = pprof.get("vg_sor_rows", (0, 0.0))
+    interior0 = (pg.np[0] - 2) * (pg.np[1] - 2) * (pg.np[2] - 2)
+    recs.append({
+        "workload": "POISSON technique update (the reference default): plume N=262144, reference "
+                    "cascade (injection, overwrite prolongation, 16 SOR steps per level visit)",
+        "grid": list(pg.shape), "update_ms": upd_ms, "sor_rows_launches": sc,
+        "sor_rows_ms_total": sms,
+        "kernels_ms": {k: {"count": c, "total_ms": m} for k, (c, m) in pprof.items()},
+        "note": f"one SOR colour pass at the finest level moves 18 B per interior point "
+                f"({interior0} points)"})
     # ---- stretching and tilting: 1M vortons (64 random rings) on their 129^3-class grid
     pos, om, radius = scenarios.random_rings(1 << 20, seed=5)
     n = len(pos)

@@ -275,9 +275,17 @@ def test_ring_4096_direct_at_vortons_and_grid(vg, oracle, policy):
     v = vg.vortons_view(dp, dw, n, uniform_size=2 * radius)
     mn, mx = vg.find_bounding_box(dp, n)
     g0 = vg.finalize_bounding_box(mn, mx, radius, n)
-    g = vg.define_shape(32 ** 3, g0.min, [g0.min[a] + g0.extent[a] for a in range(3)])
+    # 32 cells per axis over the padded box (UniformGridGeometry's spacing
+    # rules, uniformGrid.h:794-809, in float32)
+    g = vg.GridDesc()
+    for a in range(3):
+        g.min[a] = g0.min[a]
+        g.extent[a] = g0.extent[a]
+        g.np[a] = 33
+        g.cell[a] = F32(g0.extent[a]) / F32(32)
+        g.cpe[a] = F32(32) / F32(g0.extent[a])
     og = to_oracle_grid(oracle, g)
-    assert (g.np[0] - 1) * (g.np[1] - 1) * (g.np[2] - 1) == 32 ** 3
+    assert g.shape == (33, 33, 33)
     out = vg.DeviceArray.zeros((n, 3))
     vg.direct_velocity(v, dp, n, out, policy=policy)
     gout = vg.DeviceArray.zeros((g.capacity, 3))

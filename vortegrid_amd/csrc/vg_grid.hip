@@ -80,7 +80,15 @@ SorCoef sor_coef(const vg_grid_desc &g) {
 
 // One colour pass of StepTowardVectorPoissonSolution: one thread per row
 // (iy, iz) of the colour, x swept sequentially in place so each point sees
-// its already-updated x-1 neighbour exactly as the reference does.
+// its already-updated x-1 neighbour exactly as the reference does.  Only
+// that neighbour chains the steps: the other inputs of a step (the old x+1
+// value, the four rows of the other colour, f) are loaded kSorAhead steps at
+// a time before those steps run, so the row's loads are in flight together
+// instead of one round trip per point.  Loading x+1..x+kSorAhead before
+// writing x..x+kSorAhead-1 reads the same old values the sequential loop
+// does, so the result is unchanged bit for bit.
+constexpr int kSorAhead = 8;
+
 __global__ __launch_bounds__(64) void k_sor_rows(vg_grid_desc g, float *__restrict__ s,
                                                  const float *__restrict__ lap, int color,
                                                  SorCoef c) {
@@ -94,24 +102,44 @@ __global__ __launch_bounds__(64) void k_sor_rows(vg_grid_desc g, float *__restri
     const size_t row = (size_t)iy * nx + (size_t)iz * nxy;
     float lx = s[3 * row], ly = s[3 * row + 1], lz = s[3 * row + 2];          // x-1 (new)
     float cx = s[3 * (row + 1)], cy = s[3 * (row + 1) + 1], cz = s[3 * (row + 1) + 2];  // x
-    for (size_t ix = 1; ix + 1 < nx; ++ix) {
-        const size_t o = row + ix;
-        const float *xp = s + 3 * (o + 1);
-        const float *ym = s + 3 * (o - nx), *yp = s + 3 * (o + nx);
-        const float *zm = s + 3 * (o - nxy), *zp = s + 3 * (o + nxy);
-        const float *f = lap + 3 * o;
-        const float rx = xp[0], ry = xp[1], rz = xp[2];
-        const float vx = ((rx + lx) * c.r2x + (yp[0] + ym[0]) * c.r2y + (zp[0] + zm[0]) * c.r2z - f[0]) * c.h;
-        const float vy = ((ry + ly) * c.r2x + (yp[1] + ym[1]) * c.r2y + (zp[1] + zm[1]) * c.r2z - f[1]) * c.h;
-        const float vz = ((rz + lz) * c.r2x + (yp[2] + ym[2]) * c.r2y + (zp[2] + zm[2]) * c.r2z - f[2]) * c.h;
-        const float nxv = c.omr * cx + c.relax * vx;
-        const float nyv = c.omr * cy + c.relax * vy;
-        const float nzv = c.omr * cz + c.relax * vz;
-        s[3 * o] = nxv;
-        s[3 * o + 1] = nyv;
-        s[3 * o + 2] = nzv;
-        lx = nxv; ly = nyv; lz = nzv;
-        cx = rx; cy = ry; cz = rz;
+    for (size_t ix0 = 1; ix0 + 1 < nx; ix0 += kSorAhead) {
+        // per step: old x+1 (3), y+1 + y-1 (3), z+1 + z-1 (3) as the sums the
+        // update uses, f (3)
+        float xp[kSorAhead][3], ys[kSorAhead][3], zs[kSorAhead][3], ff[kSorAhead][3];
+#pragma unroll
+        for (int u = 0; u < kSorAhead; ++u) {
+            const size_t ix = ix0 + u;
+            if (ix + 1 >= nx) break;
+            const size_t o = row + ix;
+            const float *a = s + 3 * (o + 1);
+            const float *ym = s + 3 * (o - nx), *yp = s + 3 * (o + nx);
+            const float *zm = s + 3 * (o - nxy), *zp = s + 3 * (o + nxy);
+            const float *f = lap + 3 * o;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                xp[u][k] = a[k];
+                ys[u][k] = yp[k] + ym[k];
+                zs[u][k] = zp[k] + zm[k];
+                ff[u][k] = f[k];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kSorAhead; ++u) {
+            const size_t ix = ix0 + u;
+            if (ix + 1 >= nx) break;
+            const size_t o = row + ix;
+            const float vx = ((xp[u][0] + lx) * c.r2x + ys[u][0] * c.r2y + zs[u][0] * c.r2z - ff[u][0]) * c.h;
+            const float vy = ((xp[u][1] + ly) * c.r2x + ys[u][1] * c.r2y + zs[u][1] * c.r2z - ff[u][1]) * c.h;
+            const float vz = ((xp[u][2] + lz) * c.r2x + ys[u][2] * c.r2y + zs[u][2] * c.r2z - ff[u][2]) * c.h;
+            const float nxv = c.omr * cx + c.relax * vx;
+            const float nyv = c.omr * cy + c.relax * vy;
+            const float nzv = c.omr * cz + c.relax * vz;
+            s[3 * o] = nxv;
+            s[3 * o + 1] = nyv;
+            s[3 * o + 2] = nzv;
+            lx = nxv; ly = nyv; lz = nzv;
+            cx = xp[u][0]; cy = xp[u][1]; cz = xp[u][2];
+        }
     }
 }
 
