@@ -64,7 +64,8 @@ def main():
                          "cell-by-cell jittered lattice (pclOpEmit.cpp:77-172)")
     ap.add_argument("--sort-tracers", type=int, default=8,
                     help="reorder the tracers into the velocity grid's cell order every K frames "
-                         "inside the timed frame (vg_sort_particles_by_cell; 0 = never)")
+                         "inside the timed frame (vg_sort_particles_by_cell, with --batched "
+                         "vg_batch_sort_tracers; 0 = never)")
     ap.add_argument("--policy", type=int, default=1, help="0 reference rsqrt, 1 accurate")
     ap.add_argument("--near-k", type=int, default=3,
                     help="near-field radius of the accurate treecode; -1 = reference opening rule")
@@ -555,25 +556,43 @@ def batched(args):
         log("autotune: " + json.dumps({k: tuned[k] for k in ("method", "launch", "chosen")}))
     else:
         log("autotune:\n" + b.autotune(s))
+    # every K frames each effect's tracers go back into its grid's cell order
+    # (inside the timed loop; the first call allocates, so it comes first)
+    K = args.sort_tracers
+    if K > 0:
+        b.sort_tracers(s)
     b.capture(dt, s)
-    for _ in range(args.warmup):
+    for i in range(args.warmup):
+        if K > 0 and i % K == 0:
+            b.sort_tracers(s)
         b.launch(s)
     s.sync()
     vg.profile_reset()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for i in range(args.steps):
+        if K > 0 and i % K == 0:
+            b.sort_tracers(s)
         b.launch(s)
     s.sync()
     el = time.perf_counter() - t0
     ms = el * 1e3 / args.steps
     pts = args.effects * 33 ** 3
     log(f"{ms:.3f} ms/frame for {args.effects} effects ({b.nv} vortons, {b.nt} tracers)")
-    # per-kernel times of one eager frame
+    # per-kernel times of one eager frame, and of one tracer sort
     vg.profile_enable(True)
     b.step(dt, s)
     s.sync()
     vg.profile_enable(False)
     prof = vg.profile_dump()
+    sort_ms = None
+    if K > 0:
+        vg.profile_reset()
+        vg.profile_enable(True)
+        b.sort_tracers(s)
+        s.sync()
+        vg.profile_enable(False)
+        sort_ms = sum(m for k, (c, m) in vg.profile_dump().items() if k.startswith("vg_batch_sort"))
+        log(f"tracer sort {sort_ms:.3f} ms (every {K} frames)")
     # accuracy on 8 effects: 1% of their grid points vs the FP64 direct sum
     # over the effect's current vortons (the grid of the last frame was
     # computed before its Evolve, so re-run one frame with dt = 0 first)
@@ -660,7 +679,9 @@ def batched(args):
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{args.effects} effects, N log-uniform [512, 32768], rings or "
-                               f"plumes, 16 tracers per vorton, 32^3-cell grid each, near-field "
+                               f"plumes, 16 tracers per vorton (uniform in each effect's box"
+                               + (f", cell-sorted every {K} frames" if K > 0 else "") +
+                               f"), 32^3-cell grid each, near-field "
                                f"treecode near_k={nk} {'dipole' if args.order else 'monopole'}"
                                + (f", local expansions from layer {args.batch_expand_from}"
                                   if args.batch_expand_from >= 1 else ""),
@@ -669,6 +690,7 @@ def batched(args):
         "launch_shapes": dict(zip(("walk_waves", "advect_block", "advect_items"), b.get_launch())),
         "autotune": tuned,
         "walk_ms": wms / max(cnt, 1), "tracers_ms": tms / max(tcnt, 1),
+        "tracer_sort": ({"every_frames": K, "ms": sort_ms} if K > 0 else None),
         "tracers_GBs": 24.0 * b.nt / (tms / max(tcnt, 1) * 1e-3) / 1e9,
         "cpu_baseline": cpu,
         "kernels_ms_one_frame": {k: {"count": c, "total_ms": m} for k, (c, m) in prof.items()},

@@ -520,6 +520,16 @@ vg_status vg_batch_mollify(const vg_batch *b, uint32_t effect, float *spread, fl
  * geometry, trees and the walk (the middle of vg_batch_step; equal to a
  * step with dt = 0).  For evaluating methods, e.g. by the autotuner. */
 vg_status vg_batch_solve(vg_batch *b, vg_stream s);
+/* Reorders each effect's tracers into the cell order of its current velocity
+ * grid (the grid the next frame's tracer pass interpolates), by a stable
+ * counting sort; every tracer stays in its effect's range.  The tracer
+ * order carries no meaning in the reference (Particles::KillParticle moves
+ * the last particle into the hole, particle.cpp:103), and cell order makes
+ * the tracer pass's grid gathers fall on shared cache lines.  Needs a
+ * computed frame (VG_ESTATE otherwise).  The first call allocates the sort
+ * buffers (~28 B per tracer + 8 B per grid point): make it outside graph
+ * capture. */
+vg_status vg_batch_sort_tracers(vg_batch *b, vg_stream s);
 /* Changes the walk's method (near-field radius, cluster order, first
  * expanded layer) for later frames; drops a recorded frame. */
 vg_status vg_batch_set_method(vg_batch *b, int near_k, int order, int expand_from);

@@ -191,3 +191,41 @@ def test_method_autotuner_meets_its_target(vg, tmp_path):
     loaded = autotune.load_choice(c, str(path))
     assert c.get_launch() == (loaded["launch"]["walk_waves"], loaded["launch"]["advect_block"],
                               loaded["launch"]["advect_items"])
+
+
+def test_sort_tracers_is_per_effect_cell_order(vg):
+    """vg_batch_sort_tracers equals the single-grid stable cell sort of each
+    effect's tracers on that effect's grid, keeps every tracer in its effect,
+    and the next frame equals the frame of an unsorted twin up to that
+    permutation, bit for bit (grids included)."""
+    case(f"batched tracer cell sort vs per-effect single-grid sort, effects {COUNTS}, cells 16")
+    a, vp, vw, tp, radii = new_batch(vg, COUNTS, seed=19, cells=16, near_k=1, order=1)
+    b, *_ = new_batch(vg, COUNTS, seed=19, cells=16, near_k=1, order=1)
+    with pytest.raises(vg.VGError):
+        b.sort_tracers()  # no grids yet
+    a.step(DT)
+    b.step(DT)
+    vg.synchronize()
+    before = b.download("tpos")
+    b.sort_tracers()
+    vg.synchronize()
+    after = b.download("tpos")
+    to = np.cumsum([0] + [len(t) for t in tp])
+    perm = np.empty(len(before), np.int64)
+    for e in range(len(COUNTS)):
+        g, _ = b.grid(e)
+        seg = vg.DeviceArray.from_numpy(before[to[e]:to[e + 1]])
+        p = vg.DeviceArray.zeros((len(tp[e]),), np.uint32)
+        vg.sort_particles_by_cell(g, seg, len(tp[e]), perm=p)
+        vg.synchronize()
+        assert np.array_equal(seg.numpy(), after[to[e]:to[e + 1]]), e
+        perm[to[e]:to[e + 1]] = to[e] + p.numpy().astype(np.int64)
+    assert np.array_equal(before[perm], after)
+    a.step(DT)
+    b.step(DT)
+    vg.synchronize()
+    assert np.array_equal(a.download("tpos")[perm], b.download("tpos"))
+    for key in ("vpos", "vvel"):
+        assert np.array_equal(a.download(key), b.download(key)), key
+    for e in range(len(COUNTS)):
+        assert np.array_equal(a.grid(e)[1], b.grid(e)[1]), e

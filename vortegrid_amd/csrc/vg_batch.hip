@@ -55,6 +55,10 @@ struct vg_batch {
     float2 *mollify = nullptr;
     uint32_t *bbox = nullptr;  // 6 ordered-encoded extrema per effect
     float *grid = nullptr;     // effects * cap0 * 3
+    // tracer cell sort (allocated by the first vg_batch_sort_tracers)
+    uint32_t *ts_count = nullptr, *ts_start = nullptr, *ts_keys = nullptr, *ts_slots = nullptr,
+             *ts_unsorted = nullptr, *ts_sorted = nullptr, *ts_scan = nullptr;
+    float *ts_moved = nullptr;
     // launch shapes
     int walk_waves = 4, adv_block = 256, adv_items = 1;
     // recorded frame
@@ -162,11 +166,14 @@ __global__ __launch_bounds__(256) void k_bbox_batch(const float *__restrict__ po
 // Tracers of all effects in one pass over their positions: AssignVelocity-
 // FromField (gain 1) + Evolve with each owner's grid (k_advect_batch's
 // arithmetic), and the extrema of the moved positions folded into the
-// owners' boxes (the tracer half of the frame's bounding boxes).  A wave
-// takes kBoxItems x 64 consecutive tracers, 64 at a time (coalesced), four
-// rounds of gathers in flight per lane; its lanes fold their extrema while
-// the owner stays the same and the wave reduces before its atomics, as in
-// k_bbox_batch, so the atomics stay few (owners are contiguous).
+// owners' boxes (the tracer half of the frame's bounding boxes).  A
+// workgroup takes kAdvRound x 256 consecutive tracers, kAdvRound per thread
+// with all their gathers in flight together.  Owners are contiguous, so a
+// workgroup nearly always holds one owner: it then reduces its extrema across
+// lanes and waves before one set of ordered-integer atomics; otherwise every
+// lane flushes its own running extrema when its owner changes.
+constexpr int kAdvRound = 4;
+
 __global__ __launch_bounds__(256) void k_advect_box_batch(const TreeView *__restrict__ views,
                                                           const float *__restrict__ grid,
                                                           size_t cap0, float *__restrict__ pos,
@@ -174,57 +181,51 @@ __global__ __launch_bounds__(256) void k_advect_box_batch(const TreeView *__rest
                                                           size_t n, float dt,
                                                           uint32_t *__restrict__ bbox) {
 #pragma clang fp contract(off)
-    const uint32_t lane = threadIdx.x & 63;
-    const size_t wave = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const size_t base = wave * 64 * kBoxItems;
+    __shared__ uint32_t wave_box[4][7];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const size_t base = (size_t)blockIdx.x * (256 * kAdvRound) + threadIdx.x;
     uint32_t cur = 0xffffffffu;
     uint32_t e[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
-    auto fold = [&](uint32_t k, float x, float y, float z) {
-        if (k != cur) {
+    uint32_t k[kAdvRound];
+    float3 p[kAdvRound], v[kAdvRound];
+    bool live[kAdvRound];
+#pragma unroll
+    for (int r = 0; r < kAdvRound; ++r) {
+        const size_t i = base + (size_t)r * 256;
+        live[r] = i < n;
+        k[r] = live[r] ? owner[i] : 0u;
+        p[r] = live[r] ? make_float3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2])
+                       : make_float3(0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int r = 0; r < kAdvRound; ++r)
+        v[r] = live[r] ? interpolate(views[k[r]].layer[0], grid + 3 * cap0 * k[r], p[r])
+                       : make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int r = 0; r < kAdvRound; ++r) {
+        if (!live[r]) continue;
+        const size_t i = base + (size_t)r * 256;
+        const float x = p[r].x + v[r].x * dt, y = p[r].y + v[r].y * dt, z = p[r].z + v[r].z * dt;
+        pos[3 * i] = x;
+        pos[3 * i + 1] = y;
+        pos[3 * i + 2] = z;
+        if (k[r] != cur) {
             if (cur != 0xffffffffu) box_flush(bbox, cur, e);
-            cur = k;
+            cur = k[r];
             e[0] = e[1] = e[2] = 0xffffffffu;
             e[3] = e[4] = e[5] = 0u;
         }
-        const uint32_t v[3] = {ord_enc(x), ord_enc(y), ord_enc(z)};
+        const uint32_t c[3] = {ord_enc(x), ord_enc(y), ord_enc(z)};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            e[a] = min(e[a], v[a]);
-            e[3 + a] = max(e[3 + a], v[a]);
-        }
-    };
-    constexpr int kRound = 4;
-    for (int it0 = 0; it0 < kBoxItems; it0 += kRound) {
-        uint32_t k[kRound];
-        float3 p[kRound], v[kRound];
-        bool live[kRound];
-#pragma unroll
-        for (int r = 0; r < kRound; ++r) {
-            const size_t i = base + (size_t)(it0 + r) * 64 + lane;
-            live[r] = i < n;
-            k[r] = live[r] ? owner[i] : 0u;
-            p[r] = live[r] ? make_float3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2])
-                           : make_float3(0.f, 0.f, 0.f);
-        }
-#pragma unroll
-        for (int r = 0; r < kRound; ++r)
-            if (live[r]) v[r] = interpolate(views[k[r]].layer[0], grid + 3 * cap0 * k[r], p[r]);
-#pragma unroll
-        for (int r = 0; r < kRound; ++r) {
-            if (!live[r]) continue;
-            const size_t i = base + (size_t)(it0 + r) * 64 + lane;
-            const float x = p[r].x + v[r].x * dt, y = p[r].y + v[r].y * dt,
-                        z = p[r].z + v[r].z * dt;
-            pos[3 * i] = x;
-            pos[3 * i + 1] = y;
-            pos[3 * i + 2] = z;
-            fold(k[r], x, y, z);
+            e[a] = min(e[a], c[a]);
+            e[3 + a] = max(e[3 + a], c[a]);
         }
     }
+    // one owner in the wave? then reduce it to lane 0
     const uint32_t k0 = __builtin_amdgcn_readfirstlane(cur);
     const bool uniform = __ballot(cur != 0xffffffffu && cur != k0) == 0ull;
     if (uniform) {
-        if (k0 == 0xffffffffu) return;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
@@ -232,9 +233,34 @@ __global__ __launch_bounds__(256) void k_advect_box_batch(const TreeView *__rest
                 e[a] = min(e[a], (uint32_t)__shfl_xor((int)e[a], off, 64));
                 e[3 + a] = max(e[3 + a], (uint32_t)__shfl_xor((int)e[3 + a], off, 64));
             }
-        if (lane == 0) box_flush(bbox, k0, e);
     } else if (cur != 0xffffffffu) {
-        box_flush(bbox, cur, e);
+        box_flush(bbox, cur, e);  // mixed wave: every lane flushes its own
+    }
+    if (lane == 0) {
+        wave_box[wave][6] = uniform ? k0 : 0xfffffffeu;  // 0xffffffff: nothing
+#pragma unroll
+        for (int a = 0; a < 6; ++a) wave_box[wave][a] = e[a];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // fold the uniform waves' extrema, one set of atomics per owner run
+        uint32_t ko = 0xffffffffu, acc[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+        for (int w = 0; w < 4; ++w) {
+            const uint32_t kw = wave_box[w][6];
+            if (kw >= 0xfffffffeu) continue;
+            if (kw != ko) {
+                if (ko != 0xffffffffu) box_flush(bbox, ko, acc);
+                ko = kw;
+                acc[0] = acc[1] = acc[2] = 0xffffffffu;
+                acc[3] = acc[4] = acc[5] = 0u;
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                acc[a] = min(acc[a], wave_box[w][a]);
+                acc[3 + a] = max(acc[3 + a], wave_box[w][3 + a]);
+            }
+        }
+        if (ko != 0xffffffffu) box_flush(bbox, ko, acc);
     }
 }
 
@@ -419,7 +445,7 @@ vg_status frame(vg_batch *b, float dt, hipStream_t s) {
                   b->bbox, b->effects));
     if (b->nt)
         VG_TRY(launch("vg_batch_tracers", k_advect_box_batch,
-                      dim3(ceil_div(b->nt, 256 * kBoxItems)), dim3(256), 0, s,
+                      dim3(ceil_div(b->nt, 256 * kAdvRound)), dim3(256), 0, s,
                       (const TreeView *)b->views, (const float *)b->grid, b->cap0, b->tpos,
                       (const uint32_t *)b->ttree, b->nt, dt, b->bbox));
     VG_TRY(solve(b, true, s));
@@ -612,6 +638,47 @@ VG_API vg_status vg_batch_solve(vg_batch *b, vg_stream s) {
     VG_TRY(solve(b, false, (hipStream_t)s));
     b->stepped = true;
     return VG_OK;
+}
+
+VG_API vg_status vg_batch_sort_tracers(vg_batch *b, vg_stream stream) {
+    VG_REQUIRE(b, "null batch");
+    if (!b->stepped) {
+        set_error("no velocity grids yet (step or solve the batch first)");
+        return VG_ESTATE;
+    }
+    if (b->nt == 0) return VG_OK;
+    const size_t cells = (size_t)b->effects * b->cap0;
+    if (!b->ts_count) {
+        vg_status st = VG_OK;
+        if (st == VG_OK) st = dalloc(b, &b->ts_count, cells);
+        if (st == VG_OK) st = dalloc(b, &b->ts_start, cells);
+        if (st == VG_OK) st = dalloc(b, &b->ts_keys, b->nt);
+        if (st == VG_OK) st = dalloc(b, &b->ts_slots, b->nt);
+        if (st == VG_OK) st = dalloc(b, &b->ts_unsorted, b->nt);
+        if (st == VG_OK) st = dalloc(b, &b->ts_sorted, b->nt);
+        if (st == VG_OK) st = dalloc(b, &b->ts_scan, scan_scratch_elems(cells));
+        if (st == VG_OK) st = dalloc(b, &b->ts_moved, 3 * b->nt);
+        if (st != VG_OK) {
+            b->ts_count = nullptr;  // the partial buffers stay in allocs until destroy
+            return st;
+        }
+    }
+    BatchSortArgs a;
+    a.n = b->nt;
+    a.cap0 = b->cap0;
+    a.cells = cells;
+    a.pos = b->tpos;
+    a.tree_of = b->ttree;
+    a.leaf_desc = b->leaf_desc;
+    a.count = b->ts_count;
+    a.start = b->ts_start;
+    a.keys = b->ts_keys;
+    a.slots = b->ts_slots;
+    a.unsorted = b->ts_unsorted;
+    a.sorted = b->ts_sorted;
+    a.scan_tmp = b->ts_scan;
+    a.moved = b->ts_moved;
+    return batch_sort_particles(a, (hipStream_t)stream);
 }
 
 VG_API vg_status vg_batch_step(vg_batch *b, float dt, vg_stream s) {

@@ -149,6 +149,21 @@ size_t scan_scratch_elems(size_t n);
 vg_status exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp,
                              hipStream_t s);
 vg_status batch_tree_build(const BatchTreeArgs &a, hipStream_t s);
+// Stable counting sort of particles owned by batched trees into (owner,
+// leaf cell of the owner's grid) order.  Owners are contiguous and ascending,
+// so every particle stays in its owner's range: tree_of does not change.
+struct BatchSortArgs {
+    size_t n = 0, cap0 = 0, cells = 0;  // particles, points per tree, trees * cap0
+    float *pos = nullptr;               // packed xyz, reordered in place
+    const uint32_t *tree_of = nullptr;
+    const vg_grid_desc *leaf_desc = nullptr;
+    uint32_t *count = nullptr, *start = nullptr;                         // cells
+    uint32_t *keys = nullptr, *slots = nullptr, *unsorted = nullptr;     // n
+    uint32_t *sorted = nullptr;                                          // n
+    uint32_t *scan_tmp = nullptr;  // scan_scratch_elems(cells)
+    float *moved = nullptr;        // 3 n
+};
+vg_status batch_sort_particles(const BatchSortArgs &a, hipStream_t s);
 
 // ------------------------------------------------------------------ device helpers
 #ifdef __HIPCC__

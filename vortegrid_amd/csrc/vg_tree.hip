@@ -825,6 +825,28 @@ vg_status batch_tree_build(const BatchTreeArgs &a, hipStream_t s) {
     return VG_OK;
 }
 
+vg_status batch_sort_particles(const BatchSortArgs &a, hipStream_t s) {
+    VG_REQUIRE(a.cells < (1ull << 32) && a.n < (1ull << 32), "batched sort too large");
+    const size_t n = a.n;
+    if (n == 0) return VG_OK;
+    VG_TRY(launch("vg_batch_sort_fill", k_fill_u32, dim3(ceil_div(a.cells, 256)), dim3(256), 0, s,
+                  a.count, a.cells, 0u));
+    VG_TRY(launch("vg_batch_sort_bin", k_bin_batch, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                  (const float *)a.pos, n, a.tree_of, a.leaf_desc, a.cap0, a.count, a.keys,
+                  a.slots));
+    VG_TRY(exclusive_scan(a.count, a.start, a.cells, a.scan_tmp, s));
+    VG_TRY(launch("vg_batch_sort_scatter", k_scatter, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                  (const uint32_t *)a.keys, (const uint32_t *)a.slots, (const uint32_t *)a.start,
+                  n, a.unsorted));
+    VG_TRY(launch("vg_batch_sort_rank", k_rank_cells, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                  (const uint32_t *)a.keys, (const uint32_t *)a.start, (const uint32_t *)a.count,
+                  (const uint32_t *)a.unsorted, n, a.sorted));
+    VG_TRY(launch("vg_batch_sort_gather", k_gather_records, dim3(ceil_div(n, 256)), dim3(256), 0,
+                  s, (const float *)a.pos, (size_t)3, (const uint32_t *)a.sorted, n, a.moved));
+    VG_HIP(hipMemcpyAsync(a.pos, a.moved, 3 * n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    return VG_OK;
+}
+
 // Used by the simulation driver.
 TreeView tree_view(const vg_tree *t) { return make_view(t); }
 

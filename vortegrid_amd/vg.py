@@ -44,7 +44,8 @@ EXPORTS = (
     "vg_mlp_gradients vg_mlp_train vg_tree_mlp_samples vg_tree_velocity_grid_mlp vg_sim_set_mlp "
     "vg_batch_config_default vg_batch_create vg_batch_destroy vg_batch_buffers vg_batch_step "
     "vg_batch_capture vg_batch_launch vg_batch_grid vg_batch_mollify vg_batch_set_launch "
-    "vg_batch_get_launch vg_batch_autotune vg_batch_solve vg_batch_set_method vg_jacobian "
+    "vg_batch_get_launch vg_batch_autotune vg_batch_solve vg_batch_set_method "
+    "vg_batch_sort_tracers vg_jacobian "
     "vg_stretch_tilt "
     "vg_stretch_tilt_from_velocity vg_sim_stretch_tilt vg_tree_partition vg_tree_pse_diffuse "
     "vg_sim_pse_diffuse"
@@ -558,6 +559,10 @@ class Batch:
     def solve(self, stream=None):
         """Velocity grids of the current particles without moving them."""
         call("vg_batch_solve", self.h, _s(stream))
+
+    def sort_tracers(self, stream=None):
+        """Reorder each effect's tracers into its current grid's cell order."""
+        call("vg_batch_sort_tracers", self.h, _s(stream))
 
     def set_method(self, near_k, order, expand_from):
         call("vg_batch_set_method", self.h, int(near_k), int(order), int(expand_from))
