@@ -269,15 +269,6 @@ struct Shape {
     uint32_t np[kMaxLayers][3];
 };
 
-// UniformGridGeometry spacing (uniformGrid.h:794-809), as the host code.
-__device__ __forceinline__ void spacing(vg_grid_desc &g) {
-#pragma clang fp contract(off)
-    for (int a = 0; a < 3; ++a) g.cell[a] = g.extent[a] / (float)(g.np[a] - 1);
-    g.cpe[0] = (float)(g.np[0] - 1) / g.extent[0];
-    g.cpe[1] = (float)(g.np[1] - 1) / g.extent[1];
-    g.cpe[2] = (0.0f == g.extent[2]) ? 1.0f / FLT_MIN : (float)(g.np[2] - 1) / g.extent[2];
-}
-
 // Per effect: UpdateBoundingBox(final) padding (vortonSim.cpp:1027-1096) with
 // the fixed grid shape, every layer's descriptor, auto-mollification
 // (vortonSim.cpp:2458-2477).  An effect with no items keeps a unit box.
@@ -313,7 +304,7 @@ __global__ void k_geometry(const uint32_t *__restrict__ bbox, const float *__res
             g.extent[a] = mx[a] - mn[a];
             g.np[a] = shape.np[l][a];
         }
-        spacing(g);
+        precompute_spacing(&g);
         tv.layer[l] = g;
         if (l == 0) leaf_desc[e] = g;
     }

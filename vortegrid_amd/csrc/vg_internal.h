@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <string>
@@ -74,9 +75,9 @@ vg_status sfree(void *p, hipStream_t s);
 // a memset node).
 vg_status zero_fill(void *p, size_t bytes, hipStream_t s);
 
-// ------------------------------------------------------------------ geometry (host)
-void define_shape(vg_grid_desc *g, size_t n, const float vmin[3], const float vmax[3], int pow2);
-void decimate(vg_grid_desc *dst, const vg_grid_desc *src, uint32_t dec);
+// ------------------------------------------------------------------ geometry
+// Shared by the host and the device (a batch defines its grids on the device
+// inside a recorded frame), so both run the same float operations.
 __host__ __device__ inline size_t capacity(const vg_grid_desc &g) {
     return (size_t)g.np[0] * g.np[1] * g.np[2];
 }
@@ -86,7 +87,109 @@ struct Nested {
     vg_grid_desc layer[kMaxLayers];
     uint32_t dec[kMaxLayers][3];
 };
-void nested_init(Nested *ng, const vg_grid_desc &src);
+
+// uniformGrid.h:794-809 PrecomputeSpacing
+__host__ __device__ inline void precompute_spacing(vg_grid_desc *g) {
+#pragma clang fp contract(off)
+    for (int a = 0; a < 3; ++a) g->cell[a] = g->extent[a] / (float)(g->np[a] - 1);
+    g->cpe[0] = (float)(g->np[0] - 1) / g->extent[0];
+    g->cpe[1] = (float)(g->np[1] - 1) / g->extent[1];
+    if (0.0f == g->extent[2])
+        g->cpe[2] = 1.0f / FLT_MIN;
+    else
+        g->cpe[2] = (float)(g->np[2] - 1) / g->extent[2];
+}
+
+// Core/Math/math.h:149-175 NearestPowerOfTwo
+__host__ __device__ inline size_t nearest_pow2(size_t v) {
+    if (v == 0) return 1;
+    --v;
+    unsigned shift = 0;
+    while ((v >> shift) != 0) ++shift;
+    return (size_t)1 << shift;
+}
+
+__host__ __device__ inline void clear_desc(vg_grid_desc *g) {
+    for (int a = 0; a < 3; ++a) {
+        g->min[a] = 0.f;
+        g->extent[a] = 0.f;
+        g->cell[a] = 0.f;
+        g->cpe[a] = 0.f;
+        g->np[a] = 0u;
+    }
+}
+
+// uniformGrid.h:259-322 DefineShape.  The cell counts go through powf: the
+// host's and the device's powf may differ in the last bit, which changes a
+// count only when extent * cells-per-extent + 1/2 lies within that bit of an
+// integer (and, with pow2, of a power of two plus one).
+__host__ __device__ inline void define_shape(vg_grid_desc *g, size_t n, const float vmin[3],
+                                             const float vmax[3], int pow2) {
+#pragma clang fp contract(off)
+    clear_desc(g);
+    if (n == 0) return;
+    const float nudge = 1.0f + FLT_EPSILON;
+    float eff[3];
+    int ndims = 3;
+    for (int a = 0; a < 3; ++a) {
+        g->min[a] = vmin[a];
+        g->extent[a] = (vmax[a] - vmin[a]) * nudge;
+        eff[a] = g->extent[a];
+    }
+    for (int a = 0; a < 3; ++a)
+        if (0.0f == eff[a]) {
+            eff[a] = 1.0f;
+            g->extent[a] = 0.0f;
+            --ndims;
+        }
+    const float volume = eff[0] * eff[1] * eff[2];
+    const float icvcr = powf(volume / (float)n, -1.0f / (float)ndims);
+    size_t nc[3];
+    for (int a = 0; a < 3; ++a) {
+        const size_t v = (size_t)(g->extent[a] * icvcr + 0.5f);
+        nc[a] = v > 1 ? v : 1;
+    }
+    if (pow2)
+        for (int a = 0; a < 3; ++a) nc[a] = nearest_pow2(nc[a]);
+    while (nc[0] * nc[1] * nc[2] >= n * 8)
+        for (int a = 0; a < 3; ++a) nc[a] = nc[a] / 2 > 1 ? nc[a] / 2 : 1;
+    for (int a = 0; a < 3; ++a) g->np[a] = (uint32_t)(nc[a] + 1);
+    precompute_spacing(g);
+}
+
+// uniformGrid.h:361-376 Decimate
+__host__ __device__ inline void decimate(vg_grid_desc *dst, const vg_grid_desc *src,
+                                         uint32_t dec) {
+    vg_grid_desc d;
+    clear_desc(&d);
+    for (int a = 0; a < 3; ++a) {
+        d.extent[a] = src->extent[a];
+        d.min[a] = src->min[a];
+        d.np[a] = (src->np[a] - 1) / dec + 1;
+        if (dec > 1 && d.np[a] < 2) d.np[a] = 2;
+    }
+    precompute_spacing(&d);
+    *dst = d;
+}
+
+// nestedGrid.h:69-85 Initialize, :371-391 PrecomputeDecimations
+__host__ __device__ inline void nested_init(Nested *ng, const vg_grid_desc &src) {
+    ng->depth = 0;
+    for (uint32_t l = 0; l < kMaxLayers; ++l) {
+        clear_desc(&ng->layer[l]);
+        for (int a = 0; a < 3; ++a) ng->dec[l][a] = 0u;
+    }
+    decimate(&ng->layer[0], &src, 1);
+    uint32_t idx = 1;
+    while (capacity(ng->layer[idx - 1]) > 8 && idx < kMaxLayers) {
+        decimate(&ng->layer[idx], &ng->layer[idx - 1], 2);
+        ++idx;
+    }
+    ng->depth = idx;
+    for (uint32_t l = 1; l < idx; ++l)
+        for (int a = 0; a < 3; ++a)
+            ng->dec[l][a] = (ng->layer[l - 1].np[a] - 1) / (ng->layer[l].np[a] - 1);
+}
 
 // True when a layer leaves a cell without a parent: on some axis the child
 // layer has more cells than decimation x parent cells (an odd count above 3
@@ -97,7 +200,7 @@ void nested_init(Nested *ng, const vg_grid_desc &src);
 // bPowerOf2 = true, vortonSim.cpp:1089), where this cannot happen; the
 // near-field walks refuse such trees instead of dropping vorticity.
 template <class NG>
-inline bool has_orphan_cells(const NG &ng) {
+__host__ __device__ inline bool has_orphan_cells(const NG &ng) {
     for (uint32_t l = 1; l < ng.depth; ++l)
         for (int a = 0; a < 3; ++a)
             if (ng.layer[l - 1].np[a] - 1 != ng.dec[l][a] * (ng.layer[l].np[a] - 1)) return true;

@@ -85,88 +85,6 @@ vg_status sfree(void *p, hipStream_t s) {
     return VG_OK;
 }
 
-// ------------------------------------------------------------------ geometry
-// uniformGrid.h:794-809 PrecomputeSpacing
-static void precompute_spacing(vg_grid_desc *g) {
-#pragma clang fp contract(off)
-    for (int a = 0; a < 3; ++a) g->cell[a] = g->extent[a] / (float)(g->np[a] - 1);
-    g->cpe[0] = (float)(g->np[0] - 1) / g->extent[0];
-    g->cpe[1] = (float)(g->np[1] - 1) / g->extent[1];
-    if (0.0f == g->extent[2])
-        g->cpe[2] = 1.0f / FLT_MIN;
-    else
-        g->cpe[2] = (float)(g->np[2] - 1) / g->extent[2];
-}
-
-// Core/Math/math.h:149-175 NearestPowerOfTwo
-static size_t nearest_pow2(size_t v) {
-    if (v == 0) return 1;
-    --v;
-    unsigned shift = 0;
-    while ((v >> shift) != 0) ++shift;
-    return (size_t)1 << shift;
-}
-
-// uniformGrid.h:259-322 DefineShape
-void define_shape(vg_grid_desc *g, size_t n, const float vmin[3], const float vmax[3],
-                  int pow2) {
-#pragma clang fp contract(off)
-    memset(g, 0, sizeof(*g));
-    if (n == 0) return;
-    const float nudge = 1.0f + FLT_EPSILON;
-    float eff[3];
-    int ndims = 3;
-    for (int a = 0; a < 3; ++a) {
-        g->min[a] = vmin[a];
-        g->extent[a] = (vmax[a] - vmin[a]) * nudge;
-        eff[a] = g->extent[a];
-    }
-    for (int a = 0; a < 3; ++a)
-        if (0.0f == eff[a]) { eff[a] = 1.0f; g->extent[a] = 0.0f; --ndims; }
-    const float volume = eff[0] * eff[1] * eff[2];
-    const float icvcr = powf(volume / (float)n, -1.0f / (float)ndims);
-    size_t nc[3];
-    for (int a = 0; a < 3; ++a) {
-        const size_t v = (size_t)(g->extent[a] * icvcr + 0.5f);
-        nc[a] = v > 1 ? v : 1;
-    }
-    if (pow2)
-        for (int a = 0; a < 3; ++a) nc[a] = nearest_pow2(nc[a]);
-    while (nc[0] * nc[1] * nc[2] >= n * 8)
-        for (int a = 0; a < 3; ++a) nc[a] = nc[a] / 2 > 1 ? nc[a] / 2 : 1;
-    for (int a = 0; a < 3; ++a) g->np[a] = (uint32_t)(nc[a] + 1);
-    precompute_spacing(g);
-}
-
-// uniformGrid.h:361-376 Decimate
-void decimate(vg_grid_desc *dst, const vg_grid_desc *src, uint32_t dec) {
-    vg_grid_desc d;
-    memset(&d, 0, sizeof d);
-    for (int a = 0; a < 3; ++a) {
-        d.extent[a] = src->extent[a];
-        d.min[a] = src->min[a];
-        d.np[a] = (src->np[a] - 1) / dec + 1;
-        if (dec > 1 && d.np[a] < 2) d.np[a] = 2;
-    }
-    precompute_spacing(&d);
-    *dst = d;
-}
-
-// nestedGrid.h:69-85 Initialize, :371-391 PrecomputeDecimations
-void nested_init(Nested *ng, const vg_grid_desc &src) {
-    memset(ng, 0, sizeof(*ng));
-    decimate(&ng->layer[0], &src, 1);
-    uint32_t idx = 1;
-    while (capacity(ng->layer[idx - 1]) > 8 && idx < kMaxLayers) {
-        decimate(&ng->layer[idx], &ng->layer[idx - 1], 2);
-        ++idx;
-    }
-    ng->depth = idx;
-    for (uint32_t l = 1; l < idx; ++l)
-        for (int a = 0; a < 3; ++a)
-            ng->dec[l][a] = (ng->layer[l - 1].np[a] - 1) / (ng->layer[l].np[a] - 1);
-}
-
 }  // namespace vg
 
 using namespace vg;

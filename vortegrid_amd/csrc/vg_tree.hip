@@ -7,7 +7,8 @@
 //
 // Build (all deterministic, no float atomics):
 //   bin     vorton -> leaf cell key (truncation, as OffsetOfPosition),
-//           slot = atomicAdd(count[key])
+//           slot = atomicAdd(count[key]), one atomic per run of equal keys
+//           in a wave
 //   scan    count -> cell start (two-level exclusive scan)
 //   scatter start + slot = i into a cell-grouped list, then each vorton
 //           moves to its rank among its cell's indices (k_rank_cells), so
@@ -129,18 +130,46 @@ __global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
     if (i < n) p[i] = v;
 }
 
+// A slot in cell `key` for each live lane of the wave.  Lanes holding the
+// same key next to each other (particles already in or near cell order) form
+// a run that takes its slots with ONE atomic by the run's first lane; the
+// others add their distance from it.  Slots inside a cell are then unique
+// but arbitrary, as with one atomic per particle: the cell sort puts them in
+// index order afterwards.  Every lane of the wave must call this (dead lanes
+// with live = false, after all live lanes).
+__device__ __forceinline__ uint32_t bin_slot(uint32_t *__restrict__ count, uint32_t key,
+                                             bool live) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t k = live ? key : 0xffffffffu;
+    const uint32_t prev = (uint32_t)__shfl_up((int)k, 1, 64);
+    const bool head = lane == 0 || prev != k;
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const uint32_t first = 63u - (uint32_t)__clzll(heads & upto);
+    const uint64_t after = heads & ~upto;
+    const uint32_t next = after ? (uint32_t)__ffsll((long long)after) - 1u : 64u;
+    uint32_t base = 0;
+    if (head && live) base = atomicAdd(&count[key], next - lane);
+    return (uint32_t)__shfl((int)base, (int)first, 64) + (lane - first);
+}
+
 // MakeBaseVortonGrid binning: OffsetOfPosition (uniformGrid.h:645-651).
 __global__ void k_bin(VortonIn vin, size_t n, vg_grid_desc g, uint32_t *__restrict__ count,
                       uint32_t *__restrict__ keys, uint32_t *__restrict__ slots) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float *p = vin.pos + i * vin.stride;
-    const int ix = cell_of(p[0], g.min[0], g.cpe[0], (int)g.np[0] - 1);
-    const int iy = cell_of(p[1], g.min[1], g.cpe[1], (int)g.np[1] - 1);
-    const int iz = cell_of(p[2], g.min[2], g.cpe[2], (int)g.np[2] - 1);
-    const uint32_t key = (uint32_t)point_offset(g, ix, iy, iz);
+    const bool live = i < n;
+    uint32_t key = 0;
+    if (live) {
+        const float *p = vin.pos + i * vin.stride;
+        const int ix = cell_of(p[0], g.min[0], g.cpe[0], (int)g.np[0] - 1);
+        const int iy = cell_of(p[1], g.min[1], g.cpe[1], (int)g.np[1] - 1);
+        const int iz = cell_of(p[2], g.min[2], g.cpe[2], (int)g.np[2] - 1);
+        key = (uint32_t)point_offset(g, ix, iy, iz);
+    }
+    const uint32_t slot = bin_slot(count, key, live);
+    if (!live) return;
     keys[i] = key;
-    slots[i] = atomicAdd(&count[key], 1u);
+    slots[i] = slot;
 }
 
 __global__ void k_scatter(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ slots,
@@ -744,16 +773,21 @@ __global__ void k_bin_batch(const float *__restrict__ pos, size_t n,
                             uint32_t *__restrict__ count, uint32_t *__restrict__ keys,
                             uint32_t *__restrict__ slots) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t k = tree_of[i];
-    const vg_grid_desc g = leaf_desc[k];
-    const float *p = pos + 3 * i;
-    const int ix = cell_of(p[0], g.min[0], g.cpe[0], (int)g.np[0] - 1);
-    const int iy = cell_of(p[1], g.min[1], g.cpe[1], (int)g.np[1] - 1);
-    const int iz = cell_of(p[2], g.min[2], g.cpe[2], (int)g.np[2] - 1);
-    const uint32_t key = (uint32_t)((size_t)k * cap0 + point_offset(g, ix, iy, iz));
+    const bool live = i < n;
+    uint32_t key = 0;
+    if (live) {
+        const uint32_t k = tree_of[i];
+        const vg_grid_desc g = leaf_desc[k];
+        const float *p = pos + 3 * i;
+        const int ix = cell_of(p[0], g.min[0], g.cpe[0], (int)g.np[0] - 1);
+        const int iy = cell_of(p[1], g.min[1], g.cpe[1], (int)g.np[1] - 1);
+        const int iz = cell_of(p[2], g.min[2], g.cpe[2], (int)g.np[2] - 1);
+        key = (uint32_t)((size_t)k * cap0 + point_offset(g, ix, iy, iz));
+    }
+    const uint32_t slot = bin_slot(count, key, live);
+    if (!live) return;
     keys[i] = key;
-    slots[i] = atomicAdd(&count[key], 1u);
+    slots[i] = slot;
 }
 
 __global__ void k_gather_batch(const float *__restrict__ pos, const float *__restrict__ angvel,
