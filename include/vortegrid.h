@@ -487,11 +487,22 @@ typedef struct vg_batch vg_batch;
 typedef struct vg_batch_config {
     uint32_t cells;    /* cells per axis of every effect's grid (default 32): a power
                         * of two or 3 * 2^k; counts that leave a tree cell without
-                        * a parent (e.g. 20, 36) are refused with VG_EINVAL */
+                        * a parent (e.g. 20, 36) are refused with VG_EINVAL.
+                        * 0: every frame, each effect's grid is
+                        * UniformGridGeometry::DefineShape(vorton count, padded box,
+                        * power-of-two cells) as UpdateBoundingBox calls it
+                        * (uniformGrid.h:259-322, vortonSim.cpp:1089), computed on
+                        * the device; the walk then takes (effect, brick) items
+                        * from a device work queue, largest effects first */
     int near_k;        /* near-field treecode radius (default 2) */
     int order;         /* cluster order: 0 monopole, 1 dipole (default 1) */
     int rsqrt_policy;  /* VG_RSQRT_* (default ACCURATE) */
     int expand_from;   /* local expansions from this layer (default -1: off) */
+    uint32_t max_cells; /* cells == 0: at most this many cells per axis, a power of two
+                         * in [4, 256] (default 64); storage is sized for it.  A
+                         * shape beyond it on an axis is limited to it there, and a
+                         * single-cell shape gets 2 cells per axis; both are counted
+                         * (vg_batch_clamped) */
 } vg_batch_config;
 vg_status vg_batch_config_default(vg_batch_config *cfg);
 /* Host arrays of `effects` entries; tracer_counts may be NULL (none). */
@@ -520,6 +531,9 @@ vg_status vg_batch_mollify(const vg_batch *b, uint32_t effect, float *spread, fl
  * geometry, trees and the walk (the middle of vg_batch_step; equal to a
  * step with dt = 0).  For evaluating methods, e.g. by the autotuner. */
 vg_status vg_batch_solve(vg_batch *b, vg_stream s);
+/* Number of (effect, frame) grid definitions limited by max_cells (or
+ * widened from a single cell) since creation.  Blocks. */
+vg_status vg_batch_clamped(const vg_batch *b, uint32_t *count);
 /* Reorders each effect's tracers into the cell order of its current velocity
  * grid (the grid the next frame's tracer pass interpolates), by a stable
  * counting sort; every tracer stays in its effect's range.  The tracer

@@ -208,6 +208,13 @@ static void precompute_spacing(vgo_grid *g) {
         g->cpe[2] = (float)(g->np[2] - 1) / g->extent[2];
 }
 
+/* The reference's powf calls of the grid geometry (DefineShape's cells per
+ * extent, the auto-mollifier's cell size; uniformGrid.h:291,
+ * vortonSim.cpp:2471) as pow in double rounded once to float: within half an
+ * ulp of the exact power up to double rounding.  The reference's own result
+ * is its platform powf's, which may differ from this in the last bit. */
+static float pow_rounded(float x, float y) { return (float)pow((double)x, (double)y); }
+
 /* uniformGrid.h:259-322 DefineShape */
 void vgo_define_shape(vgo_grid *g, size_t n, const float vmin[3],
                       const float vmax[3], int pow2) {
@@ -225,7 +232,7 @@ void vgo_define_shape(vgo_grid *g, size_t n, const float vmin[3],
         if (0.0f == eff[a]) { eff[a] = 1.0f; g->extent[a] = 0.0f; --ndims; }
     }
     const float volume = eff[0] * eff[1] * eff[2];
-    const float icvcr = powf(volume / (float)n, -1.0f / (float)ndims);
+    const float icvcr = pow_rounded(volume / (float)n, -1.0f / (float)ndims);
     size_t nc[3];
     for (int a = 0; a < 3; ++a) {
         size_t v = (size_t)(g->extent[a] * icvcr + 0.5f);
@@ -876,7 +883,7 @@ void vgo_auto_mollify(const vgo_grid *g, float vorton_radius, float *spread,
                       float *circ) {
     const float one_third = 1.0f / 3.0f;
     const float cell_volume = g->cell[0] * g->cell[1] * g->cell[2];
-    const float cgl = 1.0f * powf(cell_volume, one_third);
+    const float cgl = 1.0f * pow_rounded(cell_volume, one_third);
     float s = cgl / vorton_radius;
     if (s < 1.0f) s = 1.0f;
     *spread = s;

@@ -18,6 +18,7 @@
 // device), so a frame can be recorded once into a hipGraph and replayed.
 // Launch shapes of the walk and the tracer kernel are chosen by
 // vg_batch_autotune from timed trials on the live batch.
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -31,13 +32,14 @@ namespace vg {
 vg_status mac_velocity_grid_batch(const TreeView *views, const float2 *mollify, uint32_t trees,
                                   const uint32_t np[3], int near_k, int order, int expand_from,
                                   int policy, int waves_per_block, float *out, size_t out_stride,
-                                  hipStream_t s);
+                                  hipStream_t s, const BatchQueue *queue);
 }
 
 struct vg_batch {
     uint32_t effects = 0;
     vg_batch_config cfg{};
-    vg::Nested shape{};
+    vg::Nested shape{};  // the fixed shape, or with adaptive shapes the largest one
+    bool adaptive = false;
     size_t cap[vg::kMaxLayers] = {};
     size_t cap0 = 0, nv = 0, nt = 0;
     // particles (packed xyz, effects back to back) and their owners
@@ -55,6 +57,10 @@ struct vg_batch {
     float2 *mollify = nullptr;
     uint32_t *bbox = nullptr;  // 6 ordered-encoded extrema per effect
     float *grid = nullptr;     // effects * cap0 * 3
+    // adaptive shapes: vorton count per effect, the walk's work queue (effects
+    // by descending vorton count, first brick of each), and counters (walk,
+    // expansions, effect-frames whose shape was clamped)
+    uint32_t *vcount = nullptr, *qorder = nullptr, *brick_start = nullptr, *counters = nullptr;
     // tracer cell sort (allocated by the first vg_batch_sort_tracers)
     uint32_t *ts_count = nullptr, *ts_start = nullptr, *ts_keys = nullptr, *ts_slots = nullptr,
              *ts_unsorted = nullptr, *ts_sorted = nullptr, *ts_scan = nullptr;
@@ -267,14 +273,23 @@ __global__ __launch_bounds__(256) void k_advect_box_batch(const TreeView *__rest
 struct Shape {
     uint32_t depth;
     uint32_t np[kMaxLayers][3];
+    uint32_t adaptive;   // 1: each effect's shape from DefineShape
+    uint32_t max_np;     // adaptive: at most this many points per axis
 };
 
-// Per effect: UpdateBoundingBox(final) padding (vortonSim.cpp:1027-1096) with
-// the fixed grid shape, every layer's descriptor, auto-mollification
-// (vortonSim.cpp:2458-2477).  An effect with no items keeps a unit box.
+// Per effect: UpdateBoundingBox(final) padding (vortonSim.cpp:1027-1096),
+// then either the fixed grid shape over the padded box, or (adaptive)
+// DefineShape(vorton count, padded box, power-of-two cells) as
+// UpdateBoundingBox calls it (vortonSim.cpp:1089) and the NestedGrid layers
+// of that shape; every layer's descriptor; auto-mollification
+// (vortonSim.cpp:2458-2477).  An effect with no items keeps a unit box (4
+// cells per axis when adaptive).  Adaptive shapes beyond max_np points on an
+// axis, and single-cell shapes, are limited to max_np and to 2 cells per
+// axis; each such effect adds one to *clamped.
 __global__ void k_geometry(const uint32_t *__restrict__ bbox, const float *__restrict__ radius,
-                           uint32_t effects, Shape shape, TreeView *__restrict__ views,
-                           vg_grid_desc *__restrict__ leaf_desc, float2 *__restrict__ mollify) {
+                           const uint32_t *__restrict__ vcount, uint32_t effects, Shape shape,
+                           TreeView *__restrict__ views, vg_grid_desc *__restrict__ leaf_desc,
+                           float2 *__restrict__ mollify, uint32_t *__restrict__ clamped) {
 #pragma clang fp contract(off)
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= effects) return;
@@ -297,22 +312,93 @@ __global__ void k_geometry(const uint32_t *__restrict__ bbox, const float *__res
         }
     }
     TreeView &tv = views[e];
-    for (uint32_t l = 0; l < shape.depth; ++l) {
-        vg_grid_desc g;
-        for (int a = 0; a < 3; ++a) {
-            g.min[a] = mn[a];
-            g.extent[a] = mx[a] - mn[a];
-            g.np[a] = shape.np[l][a];
+    if (shape.adaptive) {
+        vg_grid_desc g0;
+        bool clamp = false;
+        if (empty || vcount[e] == 0) {
+            for (int a = 0; a < 3; ++a) {
+                g0.min[a] = mn[a];
+                g0.extent[a] = mx[a] - mn[a];
+                g0.np[a] = 5;
+            }
+            precompute_spacing(&g0);
+        } else {
+            define_shape(&g0, vcount[e], mn, mx, 1);
+            for (int a = 0; a < 3; ++a)
+                if (g0.np[a] > shape.max_np) {
+                    g0.np[a] = shape.max_np;
+                    clamp = true;
+                }
+            if (capacity(g0) <= 8) {  // one cell: a tree of one layer has no walk
+                for (int a = 0; a < 3; ++a) g0.np[a] = 3;
+                clamp = true;
+            }
+            if (clamp) precompute_spacing(&g0);
         }
-        precompute_spacing(&g);
-        tv.layer[l] = g;
-        if (l == 0) leaf_desc[e] = g;
+        if (clamp) atomicAdd(clamped, 1u);
+        Nested ng;
+        nested_init(&ng, g0);
+        tv.depth = ng.depth;
+        for (uint32_t l = 0; l < kMaxLayers; ++l) {
+            tv.layer[l] = ng.layer[l];
+            for (int a = 0; a < 3; ++a) tv.dec[l][a] = ng.dec[l][a];
+        }
+        leaf_desc[e] = ng.layer[0];
+    } else {
+        for (uint32_t l = 0; l < shape.depth; ++l) {
+            vg_grid_desc g;
+            for (int a = 0; a < 3; ++a) {
+                g.min[a] = mn[a];
+                g.extent[a] = mx[a] - mn[a];
+                g.np[a] = shape.np[l][a];
+            }
+            precompute_spacing(&g);
+            tv.layer[l] = g;
+            if (l == 0) leaf_desc[e] = g;
+        }
     }
     const vg_grid_desc &g0 = tv.layer[0];
     const float cell_volume = g0.cell[0] * g0.cell[1] * g0.cell[2];
-    float s = powf(cell_volume, 1.0f / 3.0f) / r;
+    float s = pow_rounded(cell_volume, 1.0f / 3.0f) / r;
     if (s < 1.0f) s = 1.0f;
     mollify[e] = make_float2(s, 1.0f / (s * s * s));
+}
+
+// First (tree, brick) item of every effect in queue order, and the total:
+// one workgroup, each thread summing a contiguous run of effects.
+__global__ __launch_bounds__(256) void k_brick_starts(const TreeView *__restrict__ views,
+                                                      const uint32_t *__restrict__ order,
+                                                      uint32_t effects,
+                                                      uint32_t *__restrict__ brick_start) {
+    __shared__ uint32_t part[256];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (effects + 255) / 256;
+    const uint32_t q0 = t * per < effects ? t * per : effects;
+    const uint32_t q1 = q0 + per < effects ? q0 + per : effects;
+    auto bricks = [&](uint32_t q) {
+        const vg_grid_desc &g = views[order[q]].layer[0];
+        return ((g.np[0] + 3) / 4) * ((g.np[1] + 3) / 4) * ((g.np[2] + 3) / 4);
+    };
+    uint32_t sum = 0;
+    for (uint32_t q = q0; q < q1; ++q) sum += bricks(q);
+    part[t] = sum;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t run = 0;
+        for (int i = 0; i < 256; ++i) {
+            const uint32_t v = part[i];
+            part[i] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    uint32_t run = part[t];
+    for (uint32_t q = q0; q < q1; ++q) {
+        brick_start[q] = run;
+        run += bricks(q);
+    }
+    if (q1 == effects && q0 < q1) brick_start[effects] = run;
+    if (effects == 0 && t == 0) brick_start[0] = 0;
 }
 
 // AssignVelocityFromField (gain 1) + Evolve for items owned by effects, each
@@ -377,6 +463,19 @@ __global__ void k_zero_f(float *p, size_t n) {
     if (i < n) p[i] = 0.f;
 }
 
+// The near-field walk of every effect: one launch per stage over the fixed
+// shape's bricks, or work-queue launches over each effect's own bricks.
+vg_status walk(vg_batch *b, int waves, hipStream_t s) {
+    BatchQueue q;
+    q.order = b->qorder;
+    q.brick_start = b->brick_start;
+    q.counters = b->counters;
+    return mac_velocity_grid_batch(b->views, b->mollify, b->effects, b->shape.layer[0].np,
+                                   b->cfg.near_k, b->cfg.order, b->cfg.expand_from,
+                                   b->cfg.rsqrt_policy, waves, b->grid, 3 * b->cap0, s,
+                                   b->adaptive ? &q : nullptr);
+}
+
 // Bounds, geometry, trees and the velocity grids of the current particles.
 // tracers_boxed: the tracers' extrema are already folded into b->bbox (by
 // the fused tracer pass), which then must not be cleared here.
@@ -397,9 +496,16 @@ vg_status solve(vg_batch *b, bool tracers_boxed, hipStream_t s) {
     sh.depth = b->shape.depth;
     for (uint32_t l = 0; l < kMaxLayers; ++l)
         for (int a = 0; a < 3; ++a) sh.np[l][a] = l < sh.depth ? b->shape.layer[l].np[a] : 0;
+    sh.adaptive = b->adaptive ? 1u : 0u;
+    sh.max_np = b->shape.layer[0].np[0];
     VG_TRY(launch("vg_batch_geometry", k_geometry, dim3(ceil_div(b->effects, 64)), dim3(64), 0, s,
-                  (const uint32_t *)b->bbox, (const float *)b->radius, b->effects, sh, b->views,
-                  b->leaf_desc, b->mollify));
+                  (const uint32_t *)b->bbox, (const float *)b->radius,
+                  (const uint32_t *)b->vcount, b->effects, sh, b->views, b->leaf_desc, b->mollify,
+                  b->counters + 2));
+    if (b->adaptive)
+        VG_TRY(launch("vg_batch_geometry", k_brick_starts, dim3(1), dim3(256), 0, s,
+                      (const TreeView *)b->views, (const uint32_t *)b->qorder, b->effects,
+                      b->brick_start));
     BatchTreeArgs ta;
     ta.trees = b->effects;
     ta.shape = &b->shape;
@@ -423,10 +529,9 @@ vg_status solve(vg_batch *b, bool tracers_boxed, hipStream_t s) {
         ta.M[l] = b->M[l];
         ta.C[l] = b->C[l];
     }
+    if (b->adaptive) ta.views = b->views;
     VG_TRY(batch_tree_build(ta, s));
-    return mac_velocity_grid_batch(b->views, b->mollify, b->effects, b->shape.layer[0].np,
-                                   b->cfg.near_k, b->cfg.order, b->cfg.expand_from,
-                                   b->cfg.rsqrt_policy, b->walk_waves, b->grid, 3 * b->cap0, s);
+    return walk(b, b->walk_waves, s);
 }
 
 vg_status frame(vg_batch *b, float dt, hipStream_t s) {
@@ -461,6 +566,7 @@ VG_API vg_status vg_batch_config_default(vg_batch_config *cfg) {
     cfg->order = 1;
     cfg->rsqrt_policy = VG_RSQRT_ACCURATE;
     cfg->expand_from = -1;
+    cfg->max_cells = 64;
     return VG_OK;
 }
 
@@ -472,7 +578,11 @@ VG_API vg_status vg_batch_create(vg_batch **out, uint32_t effects, const uint32_
     vg_batch_config c;
     if (cfg) c = *cfg;
     else vg_batch_config_default(&c);
-    VG_REQUIRE(c.cells >= 4 && c.cells <= 256, "cells must be in [4, 256]");
+    const bool adaptive = c.cells == 0;
+    VG_REQUIRE(adaptive || (c.cells >= 4 && c.cells <= 256), "cells must be 0 or in [4, 256]");
+    VG_REQUIRE(!adaptive || (c.max_cells >= 4 && c.max_cells <= 256 &&
+                             (c.max_cells & (c.max_cells - 1)) == 0),
+               "max_cells must be a power of two in [4, 256]");
     VG_REQUIRE(c.near_k >= 0 && c.near_k <= 16, "near_k must be in [0, 16]");
     VG_REQUIRE(c.order == 0 || c.order == 1, "order must be 0 or 1");
     VG_REQUIRE(c.rsqrt_policy == VG_RSQRT_REFERENCE || c.rsqrt_policy == VG_RSQRT_ACCURATE,
@@ -485,14 +595,17 @@ VG_API vg_status vg_batch_create(vg_batch **out, uint32_t effects, const uint32_
         vg_batch_destroy(b);
         return st;
     };
-    // the fixed shape: (cells + 1)^3 points over a unit box
+    // the fixed shape, or the largest adaptive one: (cells + 1)^3 points
+    // over a unit box
+    b->adaptive = adaptive;
+    const uint32_t cells_axis = adaptive ? c.max_cells : c.cells;
     vg_grid_desc tmpl;
     for (int a = 0; a < 3; ++a) {
         tmpl.min[a] = 0.f;
         tmpl.extent[a] = 1.f;
-        tmpl.np[a] = c.cells + 1;
-        tmpl.cell[a] = 1.f / (float)c.cells;
-        tmpl.cpe[a] = (float)c.cells;
+        tmpl.np[a] = cells_axis + 1;
+        tmpl.cell[a] = 1.f / (float)cells_axis;
+        tmpl.cpe[a] = (float)cells_axis;
     }
     nested_init(&b->shape, tmpl);
     if (has_orphan_cells(b->shape))
@@ -551,6 +664,12 @@ VG_API vg_status vg_batch_create(vg_batch **out, uint32_t effects, const uint32_
     VG_B(dalloc(b, &b->mollify, effects));
     VG_B(dalloc(b, &b->bbox, 6 * (size_t)effects));
     VG_B(dalloc(b, &b->grid, 3 * cells));
+    VG_B(dalloc(b, &b->counters, 4));
+    if (adaptive) {
+        VG_B(dalloc(b, &b->vcount, effects));
+        VG_B(dalloc(b, &b->qorder, effects));
+        VG_B(dalloc(b, &b->brick_start, (size_t)effects + 1));
+    }
 #undef VG_B
     auto up = [&](void *dst, const void *src, size_t bytes) -> vg_status {
         if (bytes) VG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
@@ -560,6 +679,15 @@ VG_API vg_status vg_batch_create(vg_batch **out, uint32_t effects, const uint32_
     if ((st = up(b->ttree, tt.data(), tt.size() * 4)) != VG_OK) return fail(st);
     if ((st = up(b->radius, rad.data(), effects * 4)) != VG_OK) return fail(st);
     if ((st = up(b->size, sz.data(), effects * 4)) != VG_OK) return fail(st);
+    if (adaptive) {
+        // the walk's queue takes the effects with the most vortons first
+        std::vector<uint32_t> vc(vorton_counts, vorton_counts + effects), order(effects);
+        for (uint32_t e = 0; e < effects; ++e) order[e] = e;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint32_t x, uint32_t y) { return vc[x] > vc[y]; });
+        if ((st = up(b->vcount, vc.data(), effects * 4)) != VG_OK) return fail(st);
+        if ((st = up(b->qorder, order.data(), effects * 4)) != VG_OK) return fail(st);
+    }
     // per-effect views: fixed pointers and decimations, unit-box geometry
     // until the first frame computes the real one
     std::vector<TreeView> views(effects);
@@ -585,6 +713,7 @@ VG_API vg_status vg_batch_create(vg_batch **out, uint32_t effects, const uint32_
     }
     if ((st = up(b->views, views.data(), effects * sizeof(TreeView))) != VG_OK) return fail(st);
     hipError_t e = hipMemset(b->grid, 0, 3 * cells * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(b->counters, 0, 4 * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(b->vvel, 0, 3 * (b->nv ? b->nv : 1) * sizeof(float));
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) return fail(device_error(e, "batch init", __FILE__, __LINE__));
@@ -726,6 +855,13 @@ VG_API vg_status vg_batch_grid(const vg_batch *b, uint32_t effect, vg_grid_desc 
     return VG_OK;
 }
 
+VG_API vg_status vg_batch_clamped(const vg_batch *b, uint32_t *count) {
+    VG_REQUIRE(b && count, "null argument");
+    VG_HIP(hipDeviceSynchronize());
+    VG_HIP(hipMemcpy(count, b->counters + 2, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return VG_OK;
+}
+
 VG_API vg_status vg_batch_mollify(const vg_batch *b, uint32_t effect, float *spread, float *circ) {
     VG_REQUIRE(b && spread && circ, "null argument");
     VG_REQUIRE(effect < b->effects, "effect out of range");
@@ -793,12 +929,7 @@ VG_API vg_status vg_batch_autotune(vg_batch *b, vg_stream stream, char *report, 
     int best_w = b->walk_waves;
     for (int w : {1, 2, 4}) {
         float ms = 0.f;
-        st = timed([&]() {
-            return mac_velocity_grid_batch(b->views, b->mollify, b->effects, b->shape.layer[0].np,
-                                           b->cfg.near_k, b->cfg.order, b->cfg.expand_from,
-                                           b->cfg.rsqrt_policy, w,
-                                           b->grid, 3 * b->cap0, s);
-        }, &ms);
+        st = timed([&]() { return walk(b, w, s); }, &ms);
         if (st != VG_OK) break;
         char line[96];
         snprintf(line, sizeof line, "walk waves_per_block=%d: %.3f ms\n", w, ms);

@@ -119,10 +119,16 @@ __host__ __device__ inline void clear_desc(vg_grid_desc *g) {
     }
 }
 
-// uniformGrid.h:259-322 DefineShape.  The cell counts go through powf: the
-// host's and the device's powf may differ in the last bit, which changes a
-// count only when extent * cells-per-extent + 1/2 lies within that bit of an
-// integer (and, with pow2, of a power of two plus one).
+// The reference's powf calls of the grid geometry (DefineShape's cells per
+// extent, the auto-mollifier's cell size) as pow in double rounded once to
+// float: within half an ulp of the exact power (up to double rounding), so
+// the host's and the device's results agree bit for bit, which two powf
+// implementations do not (each may be off by an ulp in different cases).
+__host__ __device__ inline float pow_rounded(float x, float y) {
+    return (float)pow((double)x, (double)y);
+}
+
+// uniformGrid.h:259-322 DefineShape
 __host__ __device__ inline void define_shape(vg_grid_desc *g, size_t n, const float vmin[3],
                                              const float vmax[3], int pow2) {
 #pragma clang fp contract(off)
@@ -143,7 +149,7 @@ __host__ __device__ inline void define_shape(vg_grid_desc *g, size_t n, const fl
             --ndims;
         }
     const float volume = eff[0] * eff[1] * eff[2];
-    const float icvcr = powf(volume / (float)n, -1.0f / (float)ndims);
+    const float icvcr = pow_rounded(volume / (float)n, -1.0f / (float)ndims);
     size_t nc[3];
     for (int a = 0; a < 3; ++a) {
         const size_t v = (size_t)(g->extent[a] * icvcr + 0.5f);
@@ -246,12 +252,23 @@ struct BatchTreeArgs {
     uint32_t *scan_tmp = nullptr;  // scan_scratch_elems(trees * cap0)
     float4 *P[kMaxLayers] = {}, *W[kMaxLayers] = {}, *M[kMaxLayers] = {};
     uint32_t *C[kMaxLayers] = {};  // child masks, layers >= 1
+    // non-null: the trees have their own shapes (device views), each within
+    // `shape`, whose capacities are the storage strides
+    const TreeView *views = nullptr;
 };
 size_t scan_scratch_elems(size_t n);
 // Exclusive scan in -> out using tmp (scan_scratch_elems(n) elements).
 vg_status exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp,
                              hipStream_t s);
 vg_status batch_tree_build(const BatchTreeArgs &a, hipStream_t s);
+// Work queue of a batched walk over trees of different shapes (see
+// k_mac_grid_queue): trees in queue order, the first (tree, brick) item of
+// each (trees + 1 entries), two counters (walk, expansions).
+struct BatchQueue {
+    const uint32_t *order = nullptr;
+    const uint32_t *brick_start = nullptr;
+    uint32_t *counters = nullptr;
+};
 // Stable counting sort of particles owned by batched trees into (owner,
 // leaf cell of the owner's grid) order.  Owners are contiguous and ascending,
 // so every particle stays in its owner's range: tree_of does not change.

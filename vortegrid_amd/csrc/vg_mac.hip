@@ -813,6 +813,91 @@ __global__ __launch_bounds__(64 * WPB) void k_mac_grid_batch(const TreeView *__r
                                 nullptr);
 }
 
+// Work-queue forms for trees of different shapes: a grid of resident
+// workgroups whose waves take (tree, brick) items from a device counter until
+// none is left, so every wave reaches the exit and uneven trees balance
+// across the chip.  Items are numbered tree by tree in queue order (the
+// largest trees first): brick_start[q] is the first item of the q-th tree
+// (tree id order[q]), brick_start[trees] the total; each tree's bricks come
+// from its own leaf layer.
+__device__ __forceinline__ bool take_brick(const uint32_t *__restrict__ order,
+                                           const uint32_t *__restrict__ brick_start,
+                                           uint32_t trees, uint32_t *__restrict__ counter,
+                                           uint32_t &tree, uint32_t &brick) {
+    uint32_t item = 0;
+    if ((threadIdx.x & 63) == 0) item = atomicAdd(counter, 1u);
+    item = uni((uint32_t)__shfl((int)item, 0, 64));
+    if (item >= brick_start[trees]) return false;
+    uint32_t lo = 0, hi = trees;  // brick_start[lo] <= item < brick_start[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (brick_start[mid] <= item) lo = mid;
+        else hi = mid;
+    }
+    tree = uni(order[lo]);
+    brick = item - brick_start[lo];
+    return true;
+}
+
+__device__ __forceinline__ void tree_bricks(const vg_grid_desc &g, MacParams &p) {
+    for (int a = 0; a < 3; ++a) p.nb[a] = (g.np[a] + 3) / 4;
+    p.nbricks = p.nb[0] * p.nb[1] * p.nb[2];
+}
+
+template <int ORDER, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_mac_expand_queue(
+    const TreeView *__restrict__ views, const float2 *__restrict__ mollify, MacParams mp,
+    float *__restrict__ out, size_t out_stride, const uint32_t *__restrict__ order,
+    const uint32_t *__restrict__ brick_start, uint32_t trees, uint32_t *__restrict__ counter) {
+    __shared__ uint32_t queue[WPB][128];
+    const uint32_t wave = uni(threadIdx.x >> 6);
+    uint32_t tree, brick;
+    while (take_brick(order, brick_start, trees, counter, tree, brick)) {
+        const TreeView &tv = views[tree];
+        MacParams p = mp;
+        p.spread = mollify[tree].x;
+        tree_bricks(tv.layer[0], p);
+        expand_brick<ORDER>(tv, tv.layer[0], p, brick, queue[wave], out + tree * out_stride);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+template <int POLICY, int ORDER, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_mac_grid_queue(
+    const TreeView *__restrict__ views, const float2 *__restrict__ mollify, MacParams mp,
+    float *__restrict__ out, size_t out_stride, const uint32_t *__restrict__ order,
+    const uint32_t *__restrict__ brick_start, uint32_t trees, uint32_t *__restrict__ counter) {
+    __shared__ WaveStage stage[WPB];
+    const uint32_t wave = uni(threadIdx.x >> 6);
+    uint32_t tree, brick;
+    while (take_brick(order, brick_start, trees, counter, tree, brick)) {
+        const TreeView &tv = views[tree];
+        const float2 sc = mollify[tree];
+        MacParams p = mp;
+        p.spread = sc.x;
+        p.circ = sc.y;
+        tree_bricks(tv.layer[0], p);
+        mac_brick<POLICY, ORDER, 0>(tv, tv.layer[0], p, brick, stage[wave],
+                                    out + tree * out_stride, nullptr);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+__global__ void k_clear_counters(uint32_t *__restrict__ c) {
+    if (threadIdx.x < 2) c[threadIdx.x] = 0u;
+}
+
+// Workgroups of `block` threads of kernel k that fit on the whole chip at once.
+template <typename K>
+vg_status resident_blocks(K k, int block, uint32_t *out) {
+    int dev = 0, cus = 0, per_cu = 0;
+    VG_HIP(hipGetDevice(&dev));
+    VG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    VG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0));
+    *out = (uint32_t)((per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1));
+    return VG_OK;
+}
+
 template <int COUNT>
 vg_status run_mac(const vg_tree *t, const vg_grid_desc *g, int near_k, int order, int expand_from,
                   float spread, float circ, int policy, float *out, unsigned long long *counts,
@@ -867,7 +952,7 @@ vg_status mac_velocity_grid(const vg_tree *t, const vg_grid_desc *g, int near_k,
 vg_status mac_velocity_grid_batch(const TreeView *views, const float2 *mollify, uint32_t trees,
                                   const uint32_t np[3], int near_k, int order, int expand_from,
                                   int policy, int waves_per_block, float *out, size_t out_stride,
-                                  hipStream_t s) {
+                                  hipStream_t s, const BatchQueue *queue) {
     VG_REQUIRE(near_k >= 0 && near_k <= 16, "near_k must be in [0, 16]");
     VG_REQUIRE(order == 0 || order == 1, "order must be 0 or 1");
     VG_REQUIRE(policy == VG_RSQRT_REFERENCE || policy == VG_RSQRT_ACCURATE, "bad rsqrt policy");
@@ -887,6 +972,36 @@ vg_status mac_velocity_grid_batch(const TreeView *views, const float2 *mollify, 
         {{k_mac_grid_batch<1, 0, 1>, k_mac_grid_batch<1, 0, 2>, k_mac_grid_batch<1, 0, 4>},
          {k_mac_grid_batch<1, 1, 1>, k_mac_grid_batch<1, 1, 2>, k_mac_grid_batch<1, 1, 4>}}};
     const int wi = waves_per_block == 1 ? 0 : waves_per_block == 2 ? 1 : 2;
+    if (queue) {
+        // trees of different shapes, each within np: work-queue kernels
+        using KQ = void (*)(const TreeView *, const float2 *, MacParams, float *, size_t,
+                            const uint32_t *, const uint32_t *, uint32_t, uint32_t *);
+        VG_TRY(launch("vg_batch_queue_clear", k_clear_counters, dim3(1), dim3(64), 0, s,
+                      queue->counters));
+        if (expand_from >= 1) {
+            VG_REQUIRE((size_t)np[0] * np[1] * np[2] < (1u << 28),
+                       "grid too large for local expansions");
+            KQ te[2][3] = {{k_mac_expand_queue<0, 1>, k_mac_expand_queue<0, 2>,
+                            k_mac_expand_queue<0, 4>},
+                           {k_mac_expand_queue<1, 1>, k_mac_expand_queue<1, 2>,
+                            k_mac_expand_queue<1, 4>}};
+            uint32_t nb = 0;
+            VG_TRY(resident_blocks(te[order][wi], 64 * waves_per_block, &nb));
+            VG_TRY(launch("vg_batch_expand", te[order][wi], dim3(nb), dim3(64 * waves_per_block),
+                          0, s, views, mollify, mp, out, out_stride, queue->order,
+                          queue->brick_start, trees, queue->counters + 1));
+        }
+        KQ tq[2][2][3] = {
+            {{k_mac_grid_queue<0, 0, 1>, k_mac_grid_queue<0, 0, 2>, k_mac_grid_queue<0, 0, 4>},
+             {k_mac_grid_queue<0, 1, 1>, k_mac_grid_queue<0, 1, 2>, k_mac_grid_queue<0, 1, 4>}},
+            {{k_mac_grid_queue<1, 0, 1>, k_mac_grid_queue<1, 0, 2>, k_mac_grid_queue<1, 0, 4>},
+             {k_mac_grid_queue<1, 1, 1>, k_mac_grid_queue<1, 1, 2>, k_mac_grid_queue<1, 1, 4>}}};
+        uint32_t nb = 0;
+        VG_TRY(resident_blocks(tq[policy][order][wi], 64 * waves_per_block, &nb));
+        return launch("vg_batch_velocity_grid", tq[policy][order][wi], dim3(nb),
+                      dim3(64 * waves_per_block), 0, s, views, mollify, mp, out, out_stride,
+                      queue->order, queue->brick_start, trees, queue->counters);
+    }
     if (expand_from >= 1) {
         // trees of the batch share one shape; the walk checks the depth.
         // The expansion queue packs (layer << 28 | offset).

@@ -246,7 +246,7 @@ VG_API vg_status vg_auto_mollify(const vg_grid_desc *g, float vorton_radius, flo
     VG_REQUIRE(g && spread && circ && vorton_radius > 0.f, "bad argument");
     const float one_third = 1.0f / 3.0f;
     const float cell_volume = g->cell[0] * g->cell[1] * g->cell[2];
-    const float cgl = 1.0f * powf(cell_volume, one_third);
+    const float cgl = 1.0f * pow_rounded(cell_volume, one_third);
     float s = cgl / vorton_radius;
     if (s < 1.0f) s = 1.0f;
     *spread = s;

@@ -252,22 +252,17 @@ __global__ void k_leaf(const uint32_t *__restrict__ start, const uint32_t *__res
     M[3 * c + 2] = make_float4(D[6], D[7], D[8], D[1] - D[3]);
 }
 
-// AggregateClusters (vortonSim.cpp:1337-1569) for one parent layer.
-__global__ void k_aggregate(vg_grid_desc gp, vg_grid_desc gc, uint32_t dx, uint32_t dy,
-                            uint32_t dz, const float4 *__restrict__ Pc,
-                            const float4 *__restrict__ Wc, const float4 *__restrict__ Mc,
-                            float4 *__restrict__ Pp, float4 *__restrict__ Wp,
-                            float4 *__restrict__ Mp, uint32_t *__restrict__ Cp) {
+// AggregateClusters (vortonSim.cpp:1337-1569): parent point o of layer gp
+// from its dx x dy x dz children in layer gc.
+__device__ __forceinline__ void aggregate_point(const vg_grid_desc &gp, const vg_grid_desc &gc,
+                                                uint32_t dx, uint32_t dy, uint32_t dz, size_t o,
+                                                const float4 *__restrict__ Pc,
+                                                const float4 *__restrict__ Wc,
+                                                const float4 *__restrict__ Mc,
+                                                float4 *__restrict__ Pp, float4 *__restrict__ Wp,
+                                                float4 *__restrict__ Mp,
+                                                uint32_t *__restrict__ Cp) {
 #pragma clang fp contract(off)
-    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t cap = capacity(gp);
-    if (o >= cap) return;
-    // blockIdx.y selects one of several trees of this shape stored back to back
-    {
-        const size_t tp = (size_t)blockIdx.y * cap, tc = (size_t)blockIdx.y * capacity(gc);
-        Pc += tc; Wc += tc; Mc += 3 * tc;
-        Pp += tp; Wp += tp; Mp += 3 * tp; Cp += tp;
-    }
     const uint32_t ix = (uint32_t)(o % gp.np[0]);
     const uint32_t iy = (uint32_t)((o / gp.np[0]) % gp.np[1]);
     const uint32_t iz = (uint32_t)(o / ((size_t)gp.np[0] * gp.np[1]));
@@ -318,6 +313,38 @@ __global__ void k_aggregate(vg_grid_desc gp, vg_grid_desc gc, uint32_t dx, uint3
     Mp[3 * o] = make_float4(D[0], D[1], D[2], D[5] - D[7]);
     Mp[3 * o + 1] = make_float4(D[3], D[4], D[5], D[6] - D[2]);
     Mp[3 * o + 2] = make_float4(D[6], D[7], D[8], D[1] - D[3]);
+}
+
+// One parent layer of one tree, or (blockIdx.y) of several trees of this
+// shape stored back to back.
+__global__ void k_aggregate(vg_grid_desc gp, vg_grid_desc gc, uint32_t dx, uint32_t dy,
+                            uint32_t dz, const float4 *__restrict__ Pc,
+                            const float4 *__restrict__ Wc, const float4 *__restrict__ Mc,
+                            float4 *__restrict__ Pp, float4 *__restrict__ Wp,
+                            float4 *__restrict__ Mp, uint32_t *__restrict__ Cp) {
+    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t cap = capacity(gp);
+    if (o >= cap) return;
+    const size_t tp = (size_t)blockIdx.y * cap, tc = (size_t)blockIdx.y * capacity(gc);
+    aggregate_point(gp, gc, dx, dy, dz, o, Pc + tc, Wc + tc, Mc + 3 * tc, Pp + tp, Wp + tp,
+                    Mp + 3 * tp, Cp + tp);
+}
+
+// Parent layer l of trees of different shapes (blockIdx.y = tree), each
+// read from its view; storage strides are the per-layer capacities cp, cc.
+__global__ void k_aggregate_views(const TreeView *__restrict__ views, uint32_t l, size_t cp,
+                                  size_t cc, const float4 *__restrict__ Pc,
+                                  const float4 *__restrict__ Wc, const float4 *__restrict__ Mc,
+                                  float4 *__restrict__ Pp, float4 *__restrict__ Wp,
+                                  float4 *__restrict__ Mp, uint32_t *__restrict__ Cp) {
+    const TreeView &tv = views[blockIdx.y];
+    if (l >= tv.depth) return;
+    const vg_grid_desc &gp = tv.layer[l];
+    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= capacity(gp)) return;
+    const size_t tp = (size_t)blockIdx.y * cp, tc = (size_t)blockIdx.y * cc;
+    aggregate_point(gp, tv.layer[l - 1], tv.dec[l][0], tv.dec[l][1], tv.dec[l][2], o, Pc + tc,
+                    Wc + tc, Mc + 3 * tc, Pp + tp, Wp + tp, Mp + 3 * tp, Cp + tp);
 }
 
 template <typename T>
@@ -851,6 +878,15 @@ vg_status batch_tree_build(const BatchTreeArgs &a, hipStream_t s) {
                   a.M[0]));
     for (uint32_t l = 1; l < ng.depth; ++l) {
         const size_t cap = capacity(ng.layer[l]);
+        if (a.views) {
+            // per-tree shapes within ng (the largest shape): storage strides
+            // are ng's capacities, geometry comes from each view
+            VG_TRY(launch("vg_batch_aggregate", k_aggregate_views, dim3(ceil_div(cap, 256), a.trees),
+                          dim3(256), 0, s, a.views, l, cap, capacity(ng.layer[l - 1]),
+                          (const float4 *)a.P[l - 1], (const float4 *)a.W[l - 1],
+                          (const float4 *)a.M[l - 1], a.P[l], a.W[l], a.M[l], a.C[l]));
+            continue;
+        }
         VG_TRY(launch("vg_batch_aggregate", k_aggregate, dim3(ceil_div(cap, 256), a.trees), dim3(256),
                       0, s, ng.layer[l], ng.layer[l - 1], ng.dec[l][0], ng.dec[l][1], ng.dec[l][2],
                       (const float4 *)a.P[l - 1], (const float4 *)a.W[l - 1],

@@ -45,7 +45,7 @@ EXPORTS = (
     "vg_batch_config_default vg_batch_create vg_batch_destroy vg_batch_buffers vg_batch_step "
     "vg_batch_capture vg_batch_launch vg_batch_grid vg_batch_mollify vg_batch_set_launch "
     "vg_batch_get_launch vg_batch_autotune vg_batch_solve vg_batch_set_method "
-    "vg_batch_sort_tracers vg_jacobian "
+    "vg_batch_sort_tracers vg_batch_clamped vg_jacobian "
     "vg_stretch_tilt "
     "vg_stretch_tilt_from_velocity vg_sim_stretch_tilt vg_tree_partition vg_tree_pse_diffuse "
     "vg_sim_pse_diffuse"
@@ -510,18 +510,20 @@ class Multigrid:
 
 class BatchConfig(C.Structure):
     _fields_ = [("cells", C.c_uint32), ("near_k", C.c_int), ("order", C.c_int),
-                ("rsqrt_policy", C.c_int), ("expand_from", C.c_int)]
+                ("rsqrt_policy", C.c_int), ("expand_from", C.c_int), ("max_cells", C.c_uint32)]
 
 
 class Batch:
     """Many small effects advanced together (vg_batch)."""
 
     def __init__(self, vorton_counts, tracer_counts, radii, cells=32, near_k=2, order=1,
-                 policy=1, expand_from=-1):
+                 policy=1, expand_from=-1, max_cells=64):
+        """cells=0: each effect's grid from DefineShape every frame (at most
+        max_cells per axis)."""
         cfg = BatchConfig()
         call("vg_batch_config_default", C.byref(cfg))
         cfg.cells, cfg.near_k, cfg.order, cfg.rsqrt_policy = cells, near_k, order, policy
-        cfg.expand_from = expand_from
+        cfg.expand_from, cfg.max_cells = expand_from, max_cells
         vc = np.ascontiguousarray(vorton_counts, np.uint32)
         tc = np.ascontiguousarray(tracer_counts, np.uint32)
         rr = np.ascontiguousarray(radii, np.float32)
@@ -559,6 +561,12 @@ class Batch:
     def solve(self, stream=None):
         """Velocity grids of the current particles without moving them."""
         call("vg_batch_solve", self.h, _s(stream))
+
+    def clamped(self):
+        """(effect, frame) grid definitions limited by max_cells so far."""
+        n = C.c_uint32()
+        call("vg_batch_clamped", self.h, C.byref(n))
+        return n.value
 
     def sort_tracers(self, stream=None):
         """Reorder each effect's tracers into its current grid's cell order."""
