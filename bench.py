@@ -88,6 +88,11 @@ def main():
     ap.add_argument("--batched", action="store_true",
                     help="batched effects: 256 independent effects, one recorded frame launch")
     ap.add_argument("--effects", type=int, default=256)
+    ap.add_argument("--batch-cells", type=int, default=32,
+                    help="cells per axis of every effect's grid; 0 = DefineShape per effect on "
+                         "the device every frame")
+    ap.add_argument("--batch-max-cells", type=int, default=64,
+                    help="with --batch-cells 0: at most this many cells per axis")
     ap.add_argument("--batch-near-k", type=int, default=2)
     ap.add_argument("--batch-expand-from", type=int, default=2)
     ap.add_argument("--autotune-json", default=None,
@@ -539,8 +544,9 @@ def batched(args):
     log(f"generating {args.effects} effects")
     vp, vw, tp, radii = effect_fields(args.effects)
     nk = args.batch_near_k
-    b = vg.Batch([len(p) for p in vp], [len(t) for t in tp], radii, cells=32, near_k=nk,
-                 order=args.order, policy=args.policy, expand_from=args.batch_expand_from)
+    b = vg.Batch([len(p) for p in vp], [len(t) for t in tp], radii, cells=args.batch_cells,
+                 near_k=nk, order=args.order, policy=args.policy,
+                 expand_from=args.batch_expand_from, max_cells=args.batch_max_cells)
     b.upload(np.concatenate(vp), np.concatenate(vw), np.concatenate(tp))
     s = vg.Stream()
     b.step(dt, s)
@@ -576,8 +582,9 @@ def batched(args):
     s.sync()
     el = time.perf_counter() - t0
     ms = el * 1e3 / args.steps
-    pts = args.effects * 33 ** 3
-    log(f"{ms:.3f} ms/frame for {args.effects} effects ({b.nv} vortons, {b.nt} tracers)")
+    pts = sum(b.desc(e).capacity for e in range(args.effects))
+    log(f"{ms:.3f} ms/frame for {args.effects} effects ({b.nv} vortons, {b.nt} tracers, "
+        f"{pts} grid points)")
     # per-kernel times of one eager frame, and of one tracer sort
     vg.profile_enable(True)
     b.step(dt, s)
@@ -680,12 +687,16 @@ def batched(args):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{args.effects} effects, N log-uniform [512, 32768], rings or "
                                f"plumes, 16 tracers per vorton (uniform in each effect's box"
-                               + (f", cell-sorted every {K} frames" if K > 0 else "") +
-                               f"), 32^3-cell grid each, near-field "
+                               + (f", cell-sorted every {K} frames" if K > 0 else "") + "), "
+                               + (f"{args.batch_cells}^3-cell grid each" if args.batch_cells else
+                                  f"grids from DefineShape on the device (at most "
+                                  f"{args.batch_max_cells} cells per axis)") +
+                               f", near-field "
                                f"treecode near_k={nk} {'dipole' if args.order else 'monopole'}"
                                + (f", local expansions from layer {args.batch_expand_from}"
                                   if args.batch_expand_from >= 1 else ""),
-                   "vortons": b.nv, "tracers": b.nt, "effects": args.effects},
+                   "vortons": b.nv, "tracers": b.nt, "effects": args.effects,
+                   "grid_points": pts, "clamped_grids": b.clamped()},
         "rel_l2_vs_fp64_direct": rel,
         "launch_shapes": dict(zip(("walk_waves", "advect_block", "advect_items"), b.get_launch())),
         "autotune": tuned,
@@ -730,39 +741,59 @@ def _timed(vg, name, fn, reps):
 
 def graph_frame(args):
     """The default frame's workload with no host round trip: one effect of the
-    batched path (device-side bounding box and grid definition at a fixed
-    128^3-cell shape, which is what DefineShape picks for this scene), eager
-    and as a captured hipGraph."""
+    batched path whose grid DefineShape defines on the device every frame
+    (the same frame as the default one, bit for bit: tested), eager and as a
+    captured hipGraph."""
     from vortegrid_amd import scenarios, vg
 
     vg.require_gpu()
     pos, om, radius = scenarios.random_rings(args.vortons, seed=5)
     tr = scenarios.tracers_in_box(args.tracers, pos.min(0), pos.max(0), seed=6)
-    b = vg.Batch([len(pos)], [len(tr)], [radius], cells=128, near_k=args.near_k, order=args.order,
-                 policy=args.policy, expand_from=args.expand_from)
+    b = vg.Batch([len(pos)], [len(tr)], [radius], cells=0, max_cells=256, near_k=args.near_k,
+                 order=args.order, policy=args.policy, expand_from=args.expand_from)
     b.upload(pos, om, tr)
     dt = 1.0 / 30.0
     s = vg.Stream()
+    K = args.sort_tracers
+    b.step(dt, s)  # a first frame: grids to sort the tracers by
+    if K > 0:
+        b.sort_tracers(s)  # the first call allocates
     res = {}
     for mode in ("eager", "graph"):
         if mode == "graph":
             b.capture(dt, s)
         run = (lambda: b.step(dt, s)) if mode == "eager" else (lambda: b.launch(s))
-        for _ in range(args.warmup):
+        for i in range(args.warmup):
+            if K > 0 and i % K == 0:
+                b.sort_tracers(s)
             run()
         s.sync()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for i in range(args.steps):
+            if K > 0 and i % K == 0:
+                b.sort_tracers(s)
             run()
         s.sync()
         res[mode] = (time.perf_counter() - t0) * 1e3 / args.steps
-    g, _ = b.grid(0)
+    vg.profile_reset()
+    vg.profile_enable(True)
+    b.step(dt, s)
+    s.sync()
+    vg.profile_enable(False)
+    prof = vg.profile_dump()
+    g = b.desc(0)
+    log("one eager frame: " + ", ".join(f"{k} {m:.3f}" for k, (c, m) in
+                                        sorted(prof.items(), key=lambda kv: -kv[1][1])))
     print(json.dumps({
-        "workload": "default frame workload as one batched effect (device geometry, no host "
-                    "round trip): 1M vortons (64 random rings), 4M tracers, near_k=%d, order=%d, "
-                    "expand_from=%d" % (args.near_k, args.order, args.expand_from),
+        "workload": "default frame workload as one batched effect (grid defined on the device by "
+                    "DefineShape, no host round trip): 1M vortons (64 random rings), 4M tracers"
+                    "%s, near_k=%d, order=%d, expand_from=%d" % (
+                        f" cell-sorted every {K} frames" if K > 0 else "", args.near_k,
+                        args.order, args.expand_from),
         "grid": list(g.np), "eager_ms_per_frame": res["eager"],
-        "graph_ms_per_frame": res["graph"], "steps": args.steps}), flush=True)
+        "graph_ms_per_frame": res["graph"], "steps": args.steps, "clamped_grids": b.clamped(),
+        "kernels_ms_one_frame": {k: {"count": c, "total_ms": m} for k, (c, m) in prof.items()}}),
+        flush=True)
 
 
 def suite(args):

@@ -55,7 +55,7 @@ struct vg_batch {
     vg::TreeView *views = nullptr;  // per effect, device
     vg_grid_desc *leaf_desc = nullptr;
     float2 *mollify = nullptr;
-    uint32_t *bbox = nullptr;  // 6 ordered-encoded extrema per effect
+    uint32_t *bbox = nullptr;  // 6 ordered-encoded extrema per effect and slot
     float *grid = nullptr;     // effects * cap0 * 3
     // adaptive shapes: vorton count per effect, the walk's work queue (effects
     // by descending vorton count, first brick of each), and counters (walk,
@@ -86,9 +86,14 @@ __device__ __forceinline__ float ord_dec(uint32_t u) {
     return __uint_as_float(u);
 }
 
+// Each effect's extrema are folded into kBoxSlots copies, the copy chosen by
+// the workgroup, so that the many workgroups of one large effect do not all
+// contend for the same six words; k_geometry reduces the copies.
+constexpr uint32_t kBoxSlots = 32;
+
 __global__ void k_bbox_init(uint32_t *__restrict__ bbox, uint32_t effects) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 6 * effects) return;
+    if (i >= 6 * kBoxSlots * effects) return;
     bbox[i] = (i % 6) < 3 ? 0xffffffffu : 0u;
 }
 
@@ -101,10 +106,11 @@ __global__ void k_bbox_init(uint32_t *__restrict__ bbox, uint32_t effects) {
 constexpr int kBoxItems = 16;  // items per lane
 
 __device__ __forceinline__ void box_flush(uint32_t *bbox, uint32_t k, const uint32_t *e) {
+    uint32_t *box = bbox + 6 * ((size_t)k * kBoxSlots + (blockIdx.x & (kBoxSlots - 1)));
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        atomicMin(&bbox[6 * k + a], e[a]);
-        atomicMax(&bbox[6 * k + 3 + a], e[3 + a]);
+        atomicMin(&box[a], e[a]);
+        atomicMax(&box[3 + a], e[3 + a]);
     }
 }
 
@@ -296,8 +302,13 @@ __global__ void k_geometry(const uint32_t *__restrict__ bbox, const float *__res
     float mn[3], mx[3];
     bool empty = false;
     for (int a = 0; a < 3; ++a) {
-        mn[a] = ord_dec(bbox[6 * e + a]);
-        mx[a] = ord_dec(bbox[6 * e + 3 + a]);
+        uint32_t lo = 0xffffffffu, hi = 0u;  // ordered encodings: min / max commute
+        for (uint32_t c = 0; c < kBoxSlots; ++c) {
+            lo = min(lo, bbox[6 * ((size_t)e * kBoxSlots + c) + a]);
+            hi = max(hi, bbox[6 * ((size_t)e * kBoxSlots + c) + 3 + a]);
+        }
+        mn[a] = ord_dec(lo);
+        mx[a] = ord_dec(hi);
         empty = empty || !(mn[a] <= mx[a]);
     }
     const float r = radius[e];
@@ -481,7 +492,8 @@ vg_status walk(vg_batch *b, int waves, hipStream_t s) {
 // the fused tracer pass), which then must not be cleared here.
 vg_status solve(vg_batch *b, bool tracers_boxed, hipStream_t s) {
     if (!tracers_boxed) {
-        VG_TRY(launch("vg_batch_bbox", k_bbox_init, dim3(ceil_div(6 * b->effects, 256)), dim3(256),
+        VG_TRY(launch("vg_batch_bbox", k_bbox_init,
+                      dim3(ceil_div(6 * kBoxSlots * b->effects, 256)), dim3(256),
                       0, s, b->bbox, b->effects));
         if (b->nt)
             VG_TRY(launch("vg_batch_bbox", k_bbox_batch, dim3(ceil_div(b->nt, 256 * kBoxItems)),
@@ -537,7 +549,8 @@ vg_status solve(vg_batch *b, bool tracers_boxed, hipStream_t s) {
 vg_status frame(vg_batch *b, float dt, hipStream_t s) {
     // tracers first, with the grids of the previous frame, folding their
     // moved positions into the cleared boxes on the way
-    VG_TRY(launch("vg_batch_bbox", k_bbox_init, dim3(ceil_div(6 * b->effects, 256)), dim3(256), 0, s,
+    VG_TRY(launch("vg_batch_bbox", k_bbox_init, dim3(ceil_div(6 * kBoxSlots * b->effects, 256)),
+                  dim3(256), 0, s,
                   b->bbox, b->effects));
     if (b->nt)
         VG_TRY(launch("vg_batch_tracers", k_advect_box_batch,
@@ -662,7 +675,7 @@ VG_API vg_status vg_batch_create(vg_batch **out, uint32_t effects, const uint32_
     VG_B(dalloc(b, &b->views, effects));
     VG_B(dalloc(b, &b->leaf_desc, effects));
     VG_B(dalloc(b, &b->mollify, effects));
-    VG_B(dalloc(b, &b->bbox, 6 * (size_t)effects));
+    VG_B(dalloc(b, &b->bbox, 6 * (size_t)kBoxSlots * effects));
     VG_B(dalloc(b, &b->grid, 3 * cells));
     VG_B(dalloc(b, &b->counters, 4));
     if (adaptive) {

@@ -814,20 +814,40 @@ __global__ __launch_bounds__(64 * WPB) void k_mac_grid_batch(const TreeView *__r
 }
 
 // Work-queue forms for trees of different shapes: a grid of resident
-// workgroups whose waves take (tree, brick) items from a device counter until
-// none is left, so every wave reaches the exit and uneven trees balance
-// across the chip.  Items are numbered tree by tree in queue order (the
+// workgroups that take (tree, brick) items from a device counter until none
+// is left, so every wave reaches the exit and uneven trees balance across
+// the chip.  Items are numbered tree by tree in queue order (the
 // largest trees first): brick_start[q] is the first item of the q-th tree
 // (tree id order[q]), brick_start[trees] the total; each tree's bricks come
 // from its own leaf layer.
-__device__ __forceinline__ bool take_brick(const uint32_t *__restrict__ order,
-                                           const uint32_t *__restrict__ brick_start,
-                                           uint32_t trees, uint32_t *__restrict__ counter,
-                                           uint32_t &tree, uint32_t &brick) {
+// The walk's waves take one item at a time: its bricks are long and uneven,
+// so a wave that finishes early goes on at once.
+__device__ __forceinline__ uint32_t take_item(uint32_t *__restrict__ counter) {
     uint32_t item = 0;
     if ((threadIdx.x & 63) == 0) item = atomicAdd(counter, 1u);
-    item = uni((uint32_t)__shfl((int)item, 0, 64));
-    if (item >= brick_start[trees]) return false;
+    return uni((uint32_t)__shfl((int)item, 0, 64));
+}
+
+// The expansion's workgroups take WPB consecutive items at a time: its bricks
+// are short and even, and adjacent bricks share their clusters in the CU's
+// caches, as with a plain launch of WPB bricks per workgroup.  grab_items
+// returns the first item, the same in every wave; every wave of the
+// workgroup calls it (it holds two barriers).
+template <int WPB>
+__device__ __forceinline__ uint32_t grab_items(uint32_t *__restrict__ counter) {
+    __shared__ uint32_t first;
+    if (threadIdx.x == 0) first = atomicAdd(counter, (uint32_t)WPB);
+    __syncthreads();
+    const uint32_t f = uni(first);
+    __syncthreads();  // read by every wave before the next grab overwrites it
+    return f;
+}
+
+// Tree and brick of work item `item` (< brick_start[trees]).
+__device__ __forceinline__ void item_brick(const uint32_t *__restrict__ order,
+                                           const uint32_t *__restrict__ brick_start,
+                                           uint32_t trees, uint32_t item, uint32_t &tree,
+                                           uint32_t &brick) {
     uint32_t lo = 0, hi = trees;  // brick_start[lo] <= item < brick_start[hi]
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
@@ -836,7 +856,6 @@ __device__ __forceinline__ bool take_brick(const uint32_t *__restrict__ order,
     }
     tree = uni(order[lo]);
     brick = item - brick_start[lo];
-    return true;
 }
 
 __device__ __forceinline__ void tree_bricks(const vg_grid_desc &g, MacParams &p) {
@@ -851,14 +870,20 @@ __global__ __launch_bounds__(64 * WPB) void k_mac_expand_queue(
     const uint32_t *__restrict__ brick_start, uint32_t trees, uint32_t *__restrict__ counter) {
     __shared__ uint32_t queue[WPB][128];
     const uint32_t wave = uni(threadIdx.x >> 6);
-    uint32_t tree, brick;
-    while (take_brick(order, brick_start, trees, counter, tree, brick)) {
-        const TreeView &tv = views[tree];
-        MacParams p = mp;
-        p.spread = mollify[tree].x;
-        tree_bricks(tv.layer[0], p);
-        expand_brick<ORDER>(tv, tv.layer[0], p, brick, queue[wave], out + tree * out_stride);
-        __builtin_amdgcn_wave_barrier();
+    const uint32_t total = brick_start[trees];
+    for (;;) {
+        const uint32_t first = grab_items<WPB>(counter);
+        if (first >= total) break;  // the same in every wave of the workgroup
+        const uint32_t item = first + wave;
+        if (item < total) {
+            uint32_t tree, brick;
+            item_brick(order, brick_start, trees, item, tree, brick);
+            const TreeView &tv = views[tree];
+            MacParams p = mp;
+            p.spread = mollify[tree].x;
+            tree_bricks(tv.layer[0], p);
+            expand_brick<ORDER>(tv, tv.layer[0], p, brick, queue[wave], out + tree * out_stride);
+        }
     }
 }
 
@@ -869,8 +894,12 @@ __global__ __launch_bounds__(64 * WPB) void k_mac_grid_queue(
     const uint32_t *__restrict__ brick_start, uint32_t trees, uint32_t *__restrict__ counter) {
     __shared__ WaveStage stage[WPB];
     const uint32_t wave = uni(threadIdx.x >> 6);
-    uint32_t tree, brick;
-    while (take_brick(order, brick_start, trees, counter, tree, brick)) {
+    const uint32_t total = brick_start[trees];
+    for (;;) {
+        const uint32_t item = take_item(counter);
+        if (item >= total) break;
+        uint32_t tree, brick;
+        item_brick(order, brick_start, trees, item, tree, brick);
         const TreeView &tv = views[tree];
         const float2 sc = mollify[tree];
         MacParams p = mp;

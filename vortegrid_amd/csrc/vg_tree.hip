@@ -210,12 +210,20 @@ __global__ void k_gather_sorted(VortonIn vin, const uint32_t *__restrict__ sorte
 }
 
 // MakeBaseVortonGrid accumulation + normalisation (vortonSim.cpp:1237-1310).
+// views (may be null): trees of their own shapes, blockIdx.y = tree, cap
+// the per-tree stride; only each tree's own leaf points are written.
 __global__ void k_leaf(const uint32_t *__restrict__ start, const uint32_t *__restrict__ count,
                        const float4 *__restrict__ sp, const float4 *__restrict__ sw, size_t cap,
-                       float4 *__restrict__ P, float4 *__restrict__ W, float4 *__restrict__ M) {
+                       float4 *__restrict__ P, float4 *__restrict__ W, float4 *__restrict__ M,
+                       const TreeView *__restrict__ views) {
 #pragma clang fp contract(off)
-    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cap) return;
+    size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (views) {
+        if (c >= capacity(views[blockIdx.y].layer[0])) return;
+        c += (size_t)blockIdx.y * cap;
+    } else if (c >= cap) {
+        return;
+    }
     const uint32_t cnt = count[c];
     float px = 0.f, py = 0.f, pz = 0.f, wx = 0.f, wy = 0.f, wz = 0.f, sz = 0.f;
     float aux = FLT_MIN;
@@ -872,10 +880,16 @@ vg_status batch_tree_build(const BatchTreeArgs &a, hipStream_t s) {
     VG_TRY(launch("vg_batch_gather", k_gather_batch, dim3(ceil_div(n, 256)), dim3(256), 0, s, a.pos,
                   a.angvel, a.tree_of, a.size, (const uint32_t *)a.sorted_idx, n, a.sorted_p,
                   a.sorted_w));
-    VG_TRY(launch("vg_batch_leaf", k_leaf, dim3(ceil_div(cells, 256)), dim3(256), 0, s,
-                  (const uint32_t *)a.cell_start, (const uint32_t *)a.cell_count,
-                  (const float4 *)a.sorted_p, (const float4 *)a.sorted_w, cells, a.P[0], a.W[0],
-                  a.M[0]));
+    if (a.views)
+        VG_TRY(launch("vg_batch_leaf", k_leaf, dim3(ceil_div(cap0, 256), a.trees), dim3(256), 0, s,
+                      (const uint32_t *)a.cell_start, (const uint32_t *)a.cell_count,
+                      (const float4 *)a.sorted_p, (const float4 *)a.sorted_w, cap0, a.P[0],
+                      a.W[0], a.M[0], a.views));
+    else
+        VG_TRY(launch("vg_batch_leaf", k_leaf, dim3(ceil_div(cells, 256)), dim3(256), 0, s,
+                      (const uint32_t *)a.cell_start, (const uint32_t *)a.cell_count,
+                      (const float4 *)a.sorted_p, (const float4 *)a.sorted_w, cells, a.P[0],
+                      a.W[0], a.M[0], (const TreeView *)nullptr));
     for (uint32_t l = 1; l < ng.depth; ++l) {
         const size_t cap = capacity(ng.layer[l]);
         if (a.views) {
@@ -1022,7 +1036,7 @@ VG_API vg_status vg_tree_build(vg_tree *t, const vg_grid_desc *tmpl, const vg_vo
     VG_TRY(launch("vg_tree_leaf", k_leaf, dim3(ceil_div(cap0, 256)), dim3(256), 0, s,
                   (const uint32_t *)t->cell_start, (const uint32_t *)t->cell_count,
                   (const float4 *)t->sorted_p, (const float4 *)t->sorted_w, cap0, t->P[0],
-                  t->W[0], t->M[0]));
+                  t->W[0], t->M[0], (const TreeView *)nullptr));
     for (uint32_t l = 1; l < t->ng.depth; ++l) {
         const size_t cap = capacity(t->ng.layer[l]);
         VG_TRY(launch("vg_tree_aggregate", k_aggregate, dim3(ceil_div(cap, 256)), dim3(256), 0, s,

@@ -587,6 +587,12 @@ class Batch:
         call("vg_batch_grid", self.h, C.c_uint32(effect), C.byref(g), C.byref(f))
         return g, fetch(f, 3 * g.capacity).reshape(-1, 3)
 
+    def desc(self, effect) -> GridDesc:
+        """The effect's current grid geometry (no field copy)."""
+        g = GridDesc()
+        call("vg_batch_grid", self.h, C.c_uint32(effect), C.byref(g), None)
+        return g
+
     def mollify(self, effect):
         s, c = C.c_float(), C.c_float()
         call("vg_batch_mollify", self.h, C.c_uint32(effect), C.byref(s), C.byref(c))
