@@ -233,6 +233,27 @@ def test_restrict_prolong_sor_curl_bit_exact(vg, oracle, mg_case):
     assert np.array_equal(cu.numpy(), oracle.curl(og, f))
 
 
+@pytest.mark.parametrize("shape", [(9, 129, 131), (12, 33, 20)])
+def test_sor_row_kernels_bit_exact(vg, oracle, shape):
+    """The SOR colour pass has two kernels, chosen by the number of rows per
+    colour: three components per lane below 8192 rows, one lane per
+    component from there (9x129x131 has 65 x 129 = 8385 rows per colour,
+    12x33x20 has 17 x 18 = 306).  Both equal the oracle's sequential
+    SolveVectorPoisson bit for bit."""
+    case(f"sor row kernels, grid {shape}")
+    rng = np.random.default_rng(15)
+    og = oracle.define_shape(int(np.prod([c - 1 for c in shape])), np.zeros(3),
+                             np.array([c - 1 for c in shape], np.float64) * 0.05, pow2=False)
+    assert og.shape == shape
+    g = vg.GridDesc.from_fields(og)
+    f = rng.normal(size=(og.capacity, 3)).astype(F32)
+    u0 = rng.normal(size=(og.capacity, 3)).astype(F32)
+    sol = upload(vg, u0)
+    vg.sor_solve(g, sol, upload(vg, f), 3)
+    vg.synchronize()
+    assert np.array_equal(sol.numpy(), oracle.solve_vector_poisson(og, u0, f, 3))
+
+
 def test_mg_cascade(vg, oracle, mg_case):
     pos, om, radius, og, negw = mg_case
     case(f"mg cascade grid={og.shape}")
