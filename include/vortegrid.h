@@ -500,9 +500,8 @@ typedef struct vg_batch_config {
     int expand_from;   /* local expansions from this layer (default -1: off) */
     uint32_t max_cells; /* cells == 0: at most this many cells per axis, a power of two
                          * in [4, 256] (default 64); storage is sized for it.  A
-                         * shape beyond it on an axis is limited to it there, and a
-                         * single-cell shape gets 2 cells per axis; both are counted
-                         * (vg_batch_clamped) */
+                         * shape beyond it on an axis is limited to it there and
+                         * counted (vg_batch_clamped) */
 } vg_batch_config;
 vg_status vg_batch_config_default(vg_batch_config *cfg);
 /* Host arrays of `effects` entries; tracer_counts may be NULL (none). */
@@ -531,8 +530,8 @@ vg_status vg_batch_mollify(const vg_batch *b, uint32_t effect, float *spread, fl
  * geometry, trees and the walk (the middle of vg_batch_step; equal to a
  * step with dt = 0).  For evaluating methods, e.g. by the autotuner. */
 vg_status vg_batch_solve(vg_batch *b, vg_stream s);
-/* Number of (effect, frame) grid definitions limited by max_cells (or
- * widened from a single cell) since creation.  Blocks. */
+/* Number of (effect, frame) grid definitions limited by max_cells since
+ * creation.  Blocks. */
 vg_status vg_batch_clamped(const vg_batch *b, uint32_t *count);
 /* Reorders each effect's tracers into the cell order of its current velocity
  * grid (the grid the next frame's tracer pass interpolates), by a stable

@@ -5,8 +5,8 @@ walk takes (effect, brick) items from a device work queue.
 
 Checked here: the device's grid definition equals the host's (and so the
 oracle's, which tests/test_abi.py pins) bit for bit; every effect's grid equals
-the single-tree walk on that geometry; the max_cells limit and the
-single-cell widening are counted; graph replays equal eager frames; and a
+the single-tree walk on that geometry (a one-vorton effect's single-cell
+grid included); the max_cells limit is counted; graph replays equal eager frames; and a
 one-effect batch runs the reference frame of vg_sim (tracers with the
 previous grid, vorton and tracer bounds, UpdateBoundingBox + DefineShape,
 tree, walk, vorton advection) bit for bit with no host round trip."""
@@ -78,16 +78,13 @@ def test_adaptive_shapes_equal_host_define_shape_and_single_trees(vg, expand_fro
             pv = prev_v[e] if frame else vp[e]
             pt = tpos[to[e]:to[e + 1]]
             hg, _ = host_shape(vg, np.concatenate([pv, pt]), n, radii[e], 64)
+            assert same_desc(g, hg), (frame, e, list(g.np), list(hg.np))
             if n == 1:
-                # DefineShape gives one cell; the batch widens it to 2 per axis
-                assert list(hg.np) == [2, 2, 2] and list(g.np) == [3, 3, 3]
-            else:
-                assert same_desc(g, hg), (frame, e, list(g.np), list(hg.np))
+                assert list(g.np) == [2, 2, 2]  # one cell: a tree of one layer
             ref = single_tree_grid(vg, g, pv, vw[e], radii[e], spread, circ, 2, 1, expand_from)
             assert np.array_equal(field, ref), (frame, e)
         prev_v = [vpos[vo[e]:vo[e + 1]] for e in range(len(counts))]
-    # the single-cell effect was widened on both frames
-    assert b.clamped() == 2
+    assert b.clamped() == 0
 
 
 def test_adaptive_max_cells_limit(vg):

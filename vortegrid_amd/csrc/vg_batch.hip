@@ -289,9 +289,8 @@ struct Shape {
 // UpdateBoundingBox calls it (vortonSim.cpp:1089) and the NestedGrid layers
 // of that shape; every layer's descriptor; auto-mollification
 // (vortonSim.cpp:2458-2477).  An effect with no items keeps a unit box (4
-// cells per axis when adaptive).  Adaptive shapes beyond max_np points on an
-// axis, and single-cell shapes, are limited to max_np and to 2 cells per
-// axis; each such effect adds one to *clamped.
+// cells per axis when adaptive).  An adaptive shape beyond max_np points on
+// an axis is limited to max_np there, and adds one to *clamped.
 __global__ void k_geometry(const uint32_t *__restrict__ bbox, const float *__restrict__ radius,
                            const uint32_t *__restrict__ vcount, uint32_t effects, Shape shape,
                            TreeView *__restrict__ views, vg_grid_desc *__restrict__ leaf_desc,
@@ -340,10 +339,6 @@ __global__ void k_geometry(const uint32_t *__restrict__ bbox, const float *__res
                     g0.np[a] = shape.max_np;
                     clamp = true;
                 }
-            if (capacity(g0) <= 8) {  // one cell: a tree of one layer has no walk
-                for (int a = 0; a < 3; ++a) g0.np[a] = 3;
-                clamp = true;
-            }
             if (clamp) precompute_spacing(&g0);
         }
         if (clamp) atomicAdd(clamped, 1u);
