@@ -523,29 +523,67 @@ __device__ float3 tree_eval_split(const TreeView &tv, float3 q, float spread, ui
             ohi[a] = -1;
         }
         if (open_any) {
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const float cell = gc.cell[a];
+            if (SPLIT >= 4) {
+                // lane `sub` of the point's group decides axis `sub` (the
+                // fourth lane idles) and the group shares the box
+                const int a = (int)sub < 3 ? (int)sub : 2;
+                const float cell = a == 0 ? gc.cell[0] : a == 1 ? gc.cell[1] : gc.cell[2];
+                const float gmin = a == 0 ? gc.min[0] : a == 1 ? gc.min[1] : gc.min[2];
+                const float qv = a == 0 ? qa[0] : a == 1 ? qa[1] : qa[2];
+                const int ca = a == 0 ? c0[0] : a == 1 ? c0[1] : c0[2];
+                const int cb = a == 0 ? c1[0] : a == 1 ? c1[1] : c1[2];
                 float m;
                 {
 #pragma clang fp contract(off)
                     m = 0.0001f * cell;
                     if (a == 2 && 0.0f == cell) m = m + FLT_MIN;
                 }
-                for (int c = c0[a]; c <= c1[a]; ++c) {
-                    const float mn = madd_exact(gc.min[a], (float)c, cell);
-                    const float mx = madd_exact(gc.min[a], (float)(c + 1), cell);
+                int mylo = 1 << 30, myhi = -1;
+                for (int c = ca; c <= cb; ++c) {
+                    const float mn = madd_exact(gmin, (float)c, cell);
+                    const float mx = madd_exact(gmin, (float)(c + 1), cell);
                     bool in;
                     {
 #pragma clang fp contract(off)
-                        in = qa[a] >= mn - m && qa[a] < mx + m;
+                        in = qv >= mn - m && qv < mx + m;
                     }
                     if (in) {
-                        olo[a] = c < olo[a] ? c : olo[a];
-                        ohi[a] = c > ohi[a] ? c : ohi[a];
+                        mylo = c < mylo ? c : mylo;
+                        myhi = c > myhi ? c : myhi;
                     }
                 }
-                if (ohi[a] < olo[a]) open_any = false;
+                const int base = (int)(threadIdx.x & 63u) & ~(SPLIT - 1);
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    olo[b] = __shfl(mylo, base + b, 64);
+                    ohi[b] = __shfl(myhi, base + b, 64);
+                    if (ohi[b] < olo[b]) open_any = false;
+                }
+            } else {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const float cell = gc.cell[a];
+                    float m;
+                    {
+#pragma clang fp contract(off)
+                        m = 0.0001f * cell;
+                        if (a == 2 && 0.0f == cell) m = m + FLT_MIN;
+                    }
+                    for (int c = c0[a]; c <= c1[a]; ++c) {
+                        const float mn = madd_exact(gc.min[a], (float)c, cell);
+                        const float mx = madd_exact(gc.min[a], (float)(c + 1), cell);
+                        bool in;
+                        {
+#pragma clang fp contract(off)
+                            in = qa[a] >= mn - m && qa[a] < mx + m;
+                        }
+                        if (in) {
+                            olo[a] = c < olo[a] ? c : olo[a];
+                            ohi[a] = c > ohi[a] ? c : ohi[a];
+                        }
+                    }
+                    if (ohi[a] < olo[a]) open_any = false;
+                }
             }
         }
         const float4 *__restrict__ P = tv.P[il - 1];
