@@ -2,11 +2,16 @@
 
 Workload (per step = one frame, reference frame order: tracers first,
 particleSystemConfiguration.cpp:282-296):
-  1. tracer AABB + vorton AABB -> UpdateBoundingBox(final) -> grid template
-  2. tracers: AssignVelocityFromField(previous grid) + Evolve (fused)
+  1. tracers: AssignVelocityFromField(previous grid) + Evolve (fused)
+  2. tracer AABB + vorton AABB -> UpdateBoundingBox(final) -> DefineShape
   3. VortonSim::Update, TREE technique: influence tree, auto-mollified
      treecode velocity at every grid point, vortons get grid velocity
   4. vortons: Evolve
+By default every stage runs on the device with nothing read back (a
+one-effect vg_batch whose grid DefineShape defines on the device); the same
+frames through vg_sim, which reads the two boxes back and defines the grid
+on the host, are re-run on the same input, timed and compared bit for bit
+(--host-geometry makes that form the headline).
 on N = 1,048,576 vortons (64 random rings) and 4,194,304 tracers, inputs
 resident in HBM.  value = treecode velocity evaluations (grid points) per
 second over the whole frame; rel-L2 of the treecode against an FP64 direct
@@ -52,6 +57,107 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+class HostFrame:
+    """The frame through vg_sim, as the reference wires it: tracers, the two
+    bounding boxes read back to the host, UpdateBoundingBox + DefineShape on
+    the host, VortonSim::Update (tree, near-field walk), vortons Evolve."""
+
+    walk_kernels = ("vg_tree_velocity_grid_mac", "vg_tree_mac_expand")
+    tracer_kernel = "vg_advect_tracers"
+    sort_prefix = "vg_sort_"
+
+    def __init__(self, vg, args, pos, om, radius, tr, stream):
+        self.vg, self.args, self.stream, self.dt = vg, args, stream, 1.0 / 30.0
+        self.n, self.nt = len(pos), len(tr)
+        self.d_pos = vg.DeviceArray.from_numpy(pos)
+        self.d_om = vg.DeviceArray.from_numpy(om)
+        self.d_vel = vg.DeviceArray.zeros(pos.shape)
+        self.d_tr = vg.DeviceArray.from_numpy(tr)
+        self.v = vg.vortons_view(self.d_pos, self.d_om, len(pos), uniform_size=2 * radius)
+        self.sim = vg.Sim(vg.sim_config(technique=vg.TECH_TREE, rsqrt_policy=args.policy,
+                                        tree_near_k=args.near_k, tree_order=args.order,
+                                        tree_expand_from=args.expand_from), stream)
+        self.sim.set_vortons(self.v)
+        self.grid = None
+        self.frames = 0
+        self.evals = 0
+
+    def frame(self):
+        vg, a = self.vg, self.args
+        # tracers first, with the previous frame's grid; every K frames they
+        # are put back into that grid's cell order first (they drift slowly)
+        if self.grid is not None:
+            g, p = self.grid
+            if a.sort_tracers > 0 and self.frames % a.sort_tracers == 0:
+                vg.sort_particles_by_cell(g, self.d_tr, self.nt, stream=self.stream)
+            vg.advect_tracers(g, p, self.d_tr, None, self.nt, self.dt, 1.0, stream=self.stream)
+        self.frames += 1
+        # tracer box and vorton box in one pass and one wait
+        tmn, tmx = self.sim.find_bounding_boxes(self.d_tr, self.nt)
+        self.sim.update_bounding_box(tmn, tmx, True)
+        self.sim.update(self.dt, 0, self.d_vel)
+        vg.evolve(self.d_pos, self.d_vel, self.n, self.dt, stream=self.stream)
+        self.grid = self.sim.velocity_grid()
+        self.evals += self.grid[0].capacity
+
+    def solve(self):
+        """One more velocity solve on the current particles (nothing moves);
+        returns (grid, field pointer, (spread, circ))."""
+        vg = self.vg
+        tmn, tmx = vg.find_bounding_box(self.d_tr, self.nt, stream=self.stream)
+        self.sim.find_bounding_box()
+        self.sim.update_bounding_box(tmn, tmx, True)
+        self.sim.update(self.dt, 0, None)
+        self.stream.sync()
+        g, p = self.sim.velocity_grid()
+        return g, p, self.sim.spreading()
+
+    def vortons(self):
+        return self.d_pos.numpy()
+
+    def tracers(self):
+        return self.d_tr.numpy()
+
+
+class DeviceFrame:
+    """The same frame with nothing read back to the host: a one-effect
+    vg_batch whose grid DefineShape defines on the device every frame (tracer
+    pass with the tracer box, vorton box, UpdateBoundingBox padding +
+    DefineShape + NestedGrid layers, tree, near-field walk, vortons
+    AssignVelocityFromField + Evolve), bit for bit the host-geometry frame."""
+
+    walk_kernels = ("vg_batch_velocity_grid", "vg_batch_expand")
+    tracer_kernel = "vg_batch_tracers"
+    sort_prefix = "vg_batch_sort"
+
+    def __init__(self, vg, args, pos, om, radius, tr, stream):
+        self.vg, self.args, self.stream, self.dt = vg, args, stream, 1.0 / 30.0
+        self.b = vg.Batch([len(pos)], [len(tr)], [radius], cells=0,
+                          max_cells=args.graph_max_cells, near_k=args.near_k, order=args.order,
+                          policy=args.policy, expand_from=args.expand_from)
+        self.b.upload(pos, om, tr)
+        self.frames = 0
+
+    def frame(self):
+        a = self.args
+        if self.frames > 0 and a.sort_tracers > 0 and self.frames % a.sort_tracers == 0:
+            self.b.sort_tracers(self.stream)
+        self.b.step(self.dt, self.stream)
+        self.frames += 1
+
+    def solve(self):
+        self.b.solve(self.stream)
+        self.stream.sync()
+        g, p = self.b.grid_ptr(0)
+        return g, p, self.b.mollify(0)
+
+    def vortons(self):
+        return self.b.download("vpos")
+
+    def tracers(self):
+        return self.b.download("tpos")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +199,9 @@ def main():
                          "the device every frame")
     ap.add_argument("--batch-max-cells", type=int, default=64,
                     help="with --batch-cells 0: at most this many cells per axis")
+    ap.add_argument("--graph-max-cells", type=int, default=128,
+                    help="--graph-frame: storage bound of the device-defined grid, cells per "
+                         "axis (DefineShape gives 128^3 for the default scene)")
     ap.add_argument("--batch-near-k", type=int, default=2)
     ap.add_argument("--batch-expand-from", type=int, default=2)
     ap.add_argument("--autotune-json", default=None,
@@ -100,6 +209,11 @@ def main():
     ap.add_argument("--batch-target", type=float, default=2.5e-2,
                     help="accuracy target (rel-L2 vs FP64 direct) of the batched autotuner; "
                          "<= 0 keeps --batch-near-k / --batch-expand-from")
+    ap.add_argument("--host-geometry", action="store_true",
+                    help="run the frame through vg_sim (bounding boxes read back, DefineShape on "
+                         "the host) instead of the device-geometry frame")
+    ap.add_argument("--no-host-compare", action="store_true",
+                    help="skip re-running the frames with host geometry for the comparison record")
     ap.add_argument("--graph-frame", action="store_true",
                     help="the default frame workload (1M vortons, 4M tracers) as one effect of the "
                          "batched path: bounding box, grid definition, tree, walk and Euler steps "
@@ -133,15 +247,10 @@ def main():
     tr = (scenarios.tracers_emitted(args.tracers, mn, mx, seed=6)
           if args.tracer_order == "emitted" else scenarios.tracers_in_box(args.tracers, mn, mx, seed=6))
     stream = vg.Stream()
-    d_pos = vg.DeviceArray.from_numpy(pos)
-    d_om = vg.DeviceArray.from_numpy(om)
-    d_vel = vg.DeviceArray.zeros(pos.shape)
-    d_tr = vg.DeviceArray.from_numpy(tr)
-    v = vg.vortons_view(d_pos, d_om, len(pos), uniform_size=2 * radius)
-    sim = vg.Sim(vg.sim_config(technique=vg.TECH_TREE, rsqrt_policy=args.policy,
-                               tree_near_k=args.near_k, tree_order=args.order,
-                               tree_expand_from=args.expand_from), stream)
-    sim.set_vortons(v)
+    # the device-geometry frame runs the near-field treecode; the reference
+    # rule and the learned correction go through vg_sim
+    device = not (args.host_geometry or args.mlp or args.near_k < 0)
+    eng = (DeviceFrame if device else HostFrame)(vg, args, pos, om, radius, tr, stream)
     mlp_info = None
     if args.mlp:
         if args.near_k < 0:
@@ -151,61 +260,73 @@ def main():
         log("training the far-field correction (untimed)")
         mlp, mlp_info = learned.train_far_field(near_k=args.near_k)
         log(f"trained: {mlp_info}")
-        sim.set_mlp(mlp)
-    grid_holder = {"g": None, "field": None}
-    evals = [0]
-    frame_no = [0]
+        eng.sim.set_mlp(mlp)
 
-    def frame():
-        # tracers first, with the previous frame's grid; every K frames they
-        # are put back into that grid's cell order first (they drift slowly)
-        if grid_holder["g"] is not None:
-            if args.sort_tracers > 0 and frame_no[0] % args.sort_tracers == 0:
-                vg.sort_particles_by_cell(grid_holder["g"], d_tr, len(tr), stream=stream)
-            vg.advect_tracers(grid_holder["g"], grid_holder["field"], d_tr, None, len(tr), dt, 1.0,
-                              stream=stream)
-        frame_no[0] += 1
-        # tracer box and vorton box in one pass and one wait
-        tmn, tmx = sim.find_bounding_boxes(d_tr, len(tr))
-        sim.update_bounding_box(tmn, tmx, True)
-        sim.update(dt, 0, d_vel)
-        vg.evolve(d_pos, d_vel, len(pos), dt, stream=stream)
-        g, p = sim.velocity_grid()
-        grid_holder["g"], grid_holder["field"] = g, p
-        evals[0] += g.capacity
-
-    log("warmup")
+    log(f"warmup ({'device' if device else 'host'} geometry)")
     for _ in range(args.warmup):
-        frame()
+        eng.frame()
     stream.sync()
-    evals[0] = 0
+    if device:
+        g_start = eng.b.desc(0) if eng.frames else None
+    else:
+        eng.evals = 0
+    # HIP events bracket only the launches the record reports (the solve's
+    # walk and expansions, the tracer pass, the tracer sort): an event pair
+    # around every launch of the frame costs ~0.25 ms per frame
+    timed = list(eng.walk_kernels) + [eng.tracer_kernel, eng.sort_prefix]
+    if args.mlp:
+        timed.append("vg_tree_velocity_grid_mlp")
+    elif args.near_k < 0:
+        timed.append("vg_tree_velocity_grid")
+    vg.profile_select(timed)
     vg.profile_reset()
     vg.profile_enable(True)
     log(f"timing {args.steps} frames")
     stream.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        frame()
+        eng.frame()
     stream.sync()
     t1 = time.perf_counter()
     vg.profile_enable(False)
     prof = vg.profile_dump()
+    vg.profile_select(None)
     elapsed = t1 - t0
     ms_per_frame = 1e3 * elapsed / args.steps
-    value = evals[0] / elapsed
-    g = grid_holder["g"]
-    log(f"{ms_per_frame:.3f} ms/frame, grid {g.shape}, {value:.4g} evals/s")
+    shape_note = None
+    if device:
+        g_end = eng.b.desc(0)
+        # the grid is defined on the device each frame; this scene keeps one
+        # shape, which the first and last timed frames confirm
+        steady = g_start is not None and tuple(g_start.np) == tuple(g_end.np)
+        value = g_end.capacity * args.steps / elapsed
+        shape_note = {"before_timed_frames": list(g_start.np) if g_start is not None else None,
+                      "last_timed_frame": list(g_end.np),
+                      "steady": steady, "clamped_grids": eng.b.clamped()}
+        snapshot = (eng.vortons(), eng.tracers(), eng.frames)
+    else:
+        value = eng.evals / elapsed
+    log(f"{ms_per_frame:.3f} ms/frame, {value:.4g} evals/s")
+    # every kernel of a few more frames, each launch bracketed by events (a
+    # breakdown, not the timed frames)
+    vg.profile_reset()
+    vg.profile_enable(True)
+    for _ in range(4):
+        eng.frame()
+    stream.sync()
+    vg.profile_enable(False)
+    breakdown = vg.profile_dump()
 
     # ---- one more velocity solve on the current vortons (no Evolve), so the
     # grid, the interaction count, the FP64 reference and the CPU leg all see
     # the same state
-    tmn, tmx = vg.find_bounding_box(d_tr, len(tr), stream=stream)
-    sim.find_bounding_box()
-    sim.update_bounding_box(tmn, tmx, True)
-    sim.update(dt, 0, None)
-    stream.sync()
-    g, gptr = sim.velocity_grid()
-    spread, circ = sim.spreading()
+    g, gptr, (spread, circ) = eng.solve()
+    cur = eng.vortons()
+    d_cur = vg.DeviceArray.from_numpy(cur)
+    d_om = vg.DeviceArray.from_numpy(om)
+    v = vg.vortons_view(d_cur, d_om, len(cur), uniform_size=2 * radius)
+    tr_now = eng.tracers()
+    tmn, tmx = tr_now.min(0), tr_now.max(0)
 
     # ---- roofline of the dominant kernel (treecode at grid points): kernel
     # time averaged over the timed launches, interactions counted on the final
@@ -224,7 +345,7 @@ def main():
         clusters, pairs, expansions = t.interaction_count(g, stream), 0, 0
         flop_cluster = FLOP_PER_PAIR
     else:
-        kname = "vg_tree_velocity_grid_mac"
+        kname = eng.walk_kernels[0]
         clusters, pairs, expansions = t.mac_work(g, args.near_k, args.expand_from, stream)
         flop_cluster = FLOP_PER_PAIR + (FLOP_DIPOLE if args.order else 0.0)
     flop_expand = FLOP_EXPAND_MONO + (FLOP_EXPAND_DIPOLE if args.order else 0.0)
@@ -232,10 +353,10 @@ def main():
     if expansions:
         flops += FLOP_EXPAND_EVAL * g.capacity
     cnt, ms = prof.get(kname, (0, 0.0))
-    if kname == "vg_tree_velocity_grid_mac" and expansions:
+    if kname == eng.walk_kernels[0] and expansions:
         # the expansion launch belongs to the same solve
-        ms += prof.get("vg_tree_mac_expand", (0, 0.0))[1]
-        kname = "vg_tree_velocity_grid_mac + vg_tree_mac_expand"
+        ms += prof.get(eng.walk_kernels[1], (0, 0.0))[1]
+        kname = f"{eng.walk_kernels[0]} + {eng.walk_kernels[1]}"
     avg_s = (ms / max(cnt, 1)) * 1e-3
     achieved = flops / avg_s / 1e12 if avg_s > 0 else 0.0
     roofline = {"kernel": kname, "bound": "mfma" if args.mlp else "fp32", "achieved": achieved,
@@ -247,13 +368,15 @@ def main():
                 "clusters_expanded_per_launch": expansions, "flop_per_expansion": flop_expand,
                 "flop_per_pair": FLOP_PER_PAIR, "note": note}
     # tracer advection (HBM bound): 24 B per tracer (pos read + write)
-    acnt, ams = prof.get("vg_advect_tracers", (0, 0.0))
+    acnt, ams = prof.get(eng.tracer_kernel, (0, 0.0))
     adv = None
     if acnt:
         adv_s = ams / acnt * 1e-3
         gbs = 24.0 * len(tr) / adv_s / 1e9
-        sort_ms = sum(m for k, (c, m) in prof.items() if k.startswith("vg_sort_"))
-        adv = {"tracer_order": args.tracer_order + (f", cell-sorted every {args.sort_tracers} "
+        sort_ms = sum(m for k, (c, m) in prof.items() if k.startswith(eng.sort_prefix))
+        adv = {"kernel": eng.tracer_kernel + (" (advection + the tracer bounding box)"
+                                              if device else ""),
+               "tracer_order": args.tracer_order + (f", cell-sorted every {args.sort_tracers} "
                                                     f"frames" if args.sort_tracers > 0 else ""),
                "avg_launch_ms": ams / acnt, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
                "sort_ms_per_frame": sort_ms / args.steps}
@@ -319,13 +442,43 @@ def main():
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(args, g, sim.spreading(), d_pos.numpy(), om, radius, tr, dt,
-                               vg.fetch(sim.velocity_grid()[1], 3 * g.capacity))
+            cpu = cpu_baseline(args, g, (spread, circ), cur, om, radius, tr, dt,
+                               vg.fetch(gptr, 3 * g.capacity))
             log(f"cpu baseline {cpu['ms_per_frame']:.1f} ms/frame on {cpu['cores']} threads, "
                 f"{cpu['one_thread']['ms_per_frame']:.1f} ms/frame on 1; reference rule "
                 f"{cpu['reference_rule']['ms_per_frame']:.1f} ms/frame")
         except Exception as e:  # the CPU leg never blocks the GPU number
             log(f"cpu baseline skipped: {e!r}")
+
+    # ---- the same frames with the host defining the grid (vg_sim, the
+    # bounding boxes read back every frame): timed, and checked bit for bit
+    # against the device-geometry frames on the same input
+    host_rec = None
+    if device and not args.no_host_compare:
+        try:
+            del d_cur
+            h = HostFrame(vg, args, pos, om, radius, tr, stream)
+            vs, ts, nframes = snapshot
+            for _ in range(nframes - args.steps):
+                h.frame()
+            stream.sync()
+            th = time.perf_counter()
+            for _ in range(args.steps):
+                h.frame()
+            stream.sync()
+            hms = (time.perf_counter() - th) * 1e3 / args.steps
+            same_v = bool(np.array_equal(h.vortons(), vs))
+            same_t = bool(np.array_equal(h.tracers(), ts))
+            host_rec = {"ms_per_step": hms, "frames": nframes,
+                        "vortons_bit_identical": same_v, "tracers_bit_identical": same_t,
+                        "note": "vg_sim frames (host UpdateBoundingBox + DefineShape after a "
+                                "readback of the two bounding boxes) on the same input and "
+                                "sort schedule, timed without per-kernel events"}
+            log(f"host-geometry frame {hms:.3f} ms/frame; after {nframes} frames vortons "
+                f"{'equal' if same_v else 'DIFFER'}, tracers {'equal' if same_t else 'DIFFER'}")
+            del h
+        except Exception as e:  # the comparison never blocks the headline
+            log(f"host-geometry comparison skipped: {e!r}")
 
     out = {
         "metric": "velocity evals/s + ms/frame at 1M vortons; rel-L2 vs FP64 direct",
@@ -348,17 +501,25 @@ def main():
                        f"{'dipole' if args.order else 'monopole'} clusters" +
                        (f", local expansions from layer {args.expand_from}"
                         if args.expand_from >= 1 else "")) +
-                       ", auto-mollified) + 4M-tracer advection"),
+                       ", auto-mollified) + 4M-tracer advection" +
+                       ("; bounding boxes, DefineShape and every stage on the device, one "
+                        "stream, no host round trip" if device else
+                        "; bounding boxes read back, DefineShape on the host")),
+                   "frame_geometry": "device" if device else "host",
                    "vortons": len(pos), "tracers": len(tr), "grid": list(g.shape),
                    "scenario": "64 random rings seed 5", "rsqrt_policy": args.policy,
                    "near_k": args.near_k, "order": args.order, "expand_from": args.expand_from,
                    "parallelism": "single GPU"},
         "rel_l2_vs_fp64_direct": rel,
+        "grid_shape": shape_note,
+        "host_geometry_frame": host_rec,
         "learned_correction": learned_rec,
         "roofline": roofline,
         "tracer_advection": adv,
         "cpu_baseline": cpu,
         "kernels_ms": {k: {"count": c, "total_ms": m} for k, (c, m) in prof.items()},
+        "kernels_ms_4_frames_all_timed": {k: {"count": c, "total_ms": m}
+                                          for k, (c, m) in breakdown.items()},
     }
     print(json.dumps(out))
 
@@ -749,8 +910,9 @@ def graph_frame(args):
     vg.require_gpu()
     pos, om, radius = scenarios.random_rings(args.vortons, seed=5)
     tr = scenarios.tracers_in_box(args.tracers, pos.min(0), pos.max(0), seed=6)
-    b = vg.Batch([len(pos)], [len(tr)], [radius], cells=0, max_cells=256, near_k=args.near_k,
-                 order=args.order, policy=args.policy, expand_from=args.expand_from)
+    b = vg.Batch([len(pos)], [len(tr)], [radius], cells=0, max_cells=args.graph_max_cells,
+                 near_k=args.near_k, order=args.order, policy=args.policy,
+                 expand_from=args.expand_from)
     b.upload(pos, om, tr)
     dt = 1.0 / 30.0
     s = vg.Stream()

@@ -93,6 +93,10 @@ vg_status vg_stream_synchronize(vg_stream s);
  * bracketed by HIP events on its own stream and its duration is tallied
  * by kernel name (read with vg_profile_get after a synchronize). */
 vg_status vg_profile_enable(int on);
+/* Times only the kernels whose names start with one of the comma-separated
+ * prefixes (NULL or "": every kernel).  Each timed launch costs two event
+ * records on its stream, so a timed frame selects the kernels it reports. */
+vg_status vg_profile_select(const char *prefixes);
 vg_status vg_profile_reset(void);
 vg_status vg_profile_get(const char *kernel, double *total_ms, uint64_t *count);
 /* Writes "name count total_ms\n" lines into buf (NUL-terminated). */

@@ -32,7 +32,7 @@ namespace vg {
 vg_status mac_velocity_grid_batch(const TreeView *views, const float2 *mollify, uint32_t trees,
                                   const uint32_t np[3], int near_k, int order, int expand_from,
                                   int policy, int waves_per_block, float *out, size_t out_stride,
-                                  hipStream_t s, const BatchQueue *queue);
+                                  hipStream_t s, const BatchQueue *queue, bool own_shapes);
 }
 
 struct vg_batch {
@@ -469,17 +469,21 @@ __global__ void k_zero_f(float *p, size_t n) {
     if (i < n) p[i] = 0.f;
 }
 
-// The near-field walk of every effect: one launch per stage over the fixed
-// shape's bricks, or work-queue launches over each effect's own bricks.
+// The near-field walk of every effect: plain launches over the fixed
+// shape's bricks; with device-defined shapes, work-queue launches over each
+// effect's own bricks, except for a single effect, whose plain launch over
+// the largest shape's bricks wastes nothing when max_cells fits its grid
+// (the workgroups past its own bricks leave at once).
 vg_status walk(vg_batch *b, int waves, hipStream_t s) {
     BatchQueue q;
     q.order = b->qorder;
     q.brick_start = b->brick_start;
     q.counters = b->counters;
+    const bool queue = b->adaptive && b->effects > 1;
     return mac_velocity_grid_batch(b->views, b->mollify, b->effects, b->shape.layer[0].np,
                                    b->cfg.near_k, b->cfg.order, b->cfg.expand_from,
                                    b->cfg.rsqrt_policy, waves, b->grid, 3 * b->cap0, s,
-                                   b->adaptive ? &q : nullptr);
+                                   queue ? &q : nullptr, b->adaptive);
 }
 
 // Bounds, geometry, trees and the velocity grids of the current particles.

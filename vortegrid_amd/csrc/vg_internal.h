@@ -45,9 +45,11 @@ vg_status device_error(hipError_t e, const char *what, const char *file, int lin
     } while (0)
 
 // ------------------------------------------------------------------ profiling
-// Kernel launches go through launch(); with profiling enabled each launch is
-// bracketed by two events on its stream and tallied under `name`.
+// Kernel launches go through launch(); with profiling enabled each launch
+// whose name is selected (vg_profile_select; all by default) is bracketed by
+// two events on its stream and tallied under `name`.
 bool profiling_enabled();
+bool profile_wanted(const char *name);
 void profile_begin(const char *name, hipStream_t s, void **token);
 void profile_end(void *token, hipStream_t s);
 
@@ -56,7 +58,7 @@ vg_status launch(const char *name, Kernel k, dim3 grid, dim3 block, size_t shmem
                  hipStream_t s, Args... args) {
     if (grid.x == 0 || grid.y == 0 || grid.z == 0) return VG_OK;
     void *tok = nullptr;
-    const bool prof = profiling_enabled();
+    const bool prof = profile_wanted(name);
     if (prof) profile_begin(name, s, &tok);
     hipLaunchKernelGGL(k, grid, block, shmem, s, args...);
     hipError_t e = hipGetLastError();

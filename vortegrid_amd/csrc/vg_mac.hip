@@ -764,6 +764,12 @@ __global__ __launch_bounds__(256) void k_mac_expand(const TreeView tv, vg_grid_d
     expand_brick<ORDER>(tv, g, mp, brick, queue[wave], out);
 }
 
+// Brick counts of a tree's own leaf layer.
+__device__ __forceinline__ void tree_bricks(const vg_grid_desc &g, MacParams &p) {
+    for (int a = 0; a < 3; ++a) p.nb[a] = (g.np[a] + 3) / 4;
+    p.nbricks = p.nb[0] * p.nb[1] * p.nb[2];
+}
+
 template <int ORDER, int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_mac_expand_batch(const TreeView *__restrict__ views,
                                                                const float2 *__restrict__ mollify,
@@ -773,10 +779,11 @@ __global__ __launch_bounds__(64 * WPB) void k_mac_expand_batch(const TreeView *_
     __shared__ uint32_t queue[WPB][128];
     const uint32_t wave = uni(threadIdx.x >> 6);
     const uint32_t brick = uni(blockIdx.x * WPB + wave);
-    if (brick >= mp.nbricks) return;
     const uint32_t tree = blockIdx.y;
     const TreeView &tv = views[tree];
     MacParams p = mp;
+    if (p.nbricks == 0) tree_bricks(tv.layer[0], p);  // trees of their own shapes
+    if (brick >= p.nbricks) return;
     p.spread = mollify[tree].x;
     expand_brick<ORDER>(tv, tv.layer[0], p, brick, queue[wave], out + tree * out_stride);
 }
@@ -802,11 +809,12 @@ __global__ __launch_bounds__(64 * WPB) void k_mac_grid_batch(const TreeView *__r
     __shared__ WaveStage stage[WPB];
     const uint32_t wave = uni(threadIdx.x >> 6);
     const uint32_t brick = uni(blockIdx.x * WPB + wave);
-    if (brick >= mp.nbricks) return;
     const uint32_t tree = blockIdx.y;
     const TreeView &tv = views[tree];
-    const float2 sc = mollify[tree];
     MacParams p = mp;
+    if (p.nbricks == 0) tree_bricks(tv.layer[0], p);  // trees of their own shapes
+    if (brick >= p.nbricks) return;
+    const float2 sc = mollify[tree];
     p.spread = sc.x;
     p.circ = sc.y;
     mac_brick<POLICY, ORDER, 0>(tv, tv.layer[0], p, brick, stage[wave], out + tree * out_stride,
@@ -856,11 +864,6 @@ __device__ __forceinline__ void item_brick(const uint32_t *__restrict__ order,
     }
     tree = uni(order[lo]);
     brick = item - brick_start[lo];
-}
-
-__device__ __forceinline__ void tree_bricks(const vg_grid_desc &g, MacParams &p) {
-    for (int a = 0; a < 3; ++a) p.nb[a] = (g.np[a] + 3) / 4;
-    p.nbricks = p.nb[0] * p.nb[1] * p.nb[2];
 }
 
 template <int ORDER, int WPB>
@@ -981,7 +984,7 @@ vg_status mac_velocity_grid(const vg_tree *t, const vg_grid_desc *g, int near_k,
 vg_status mac_velocity_grid_batch(const TreeView *views, const float2 *mollify, uint32_t trees,
                                   const uint32_t np[3], int near_k, int order, int expand_from,
                                   int policy, int waves_per_block, float *out, size_t out_stride,
-                                  hipStream_t s, const BatchQueue *queue) {
+                                  hipStream_t s, const BatchQueue *queue, bool own_shapes) {
     VG_REQUIRE(near_k >= 0 && near_k <= 16, "near_k must be in [0, 16]");
     VG_REQUIRE(order == 0 || order == 1, "order must be 0 or 1");
     VG_REQUIRE(policy == VG_RSQRT_REFERENCE || policy == VG_RSQRT_ACCURATE, "bad rsqrt policy");
@@ -994,6 +997,10 @@ vg_status mac_velocity_grid_batch(const TreeView *views, const float2 *mollify, 
     for (int a = 0; a < 3; ++a) mp.nb[a] = (np[a] + 3) / 4;
     mp.nbricks = mp.nb[0] * mp.nb[1] * mp.nb[2];
     mp.expand_from = expand_from;
+    // the plain launch covers np's bricks; with own_shapes each tree's
+    // workgroups past its own brick count leave at once
+    const uint32_t launch_bricks = mp.nbricks;
+    if (own_shapes) mp.nbricks = 0;
     using K = void (*)(const TreeView *, const float2 *, MacParams, float *, size_t);
     K tab[2][2][3] = {
         {{k_mac_grid_batch<0, 0, 1>, k_mac_grid_batch<0, 0, 2>, k_mac_grid_batch<0, 0, 4>},
@@ -1040,12 +1047,12 @@ vg_status mac_velocity_grid_batch(const TreeView *views, const float2 *mollify, 
         KE te[2][3] = {{k_mac_expand_batch<0, 1>, k_mac_expand_batch<0, 2>, k_mac_expand_batch<0, 4>},
                        {k_mac_expand_batch<1, 1>, k_mac_expand_batch<1, 2>, k_mac_expand_batch<1, 4>}};
         VG_TRY(launch("vg_batch_expand", te[order][wi],
-                      dim3(ceil_div(mp.nbricks, waves_per_block), trees), dim3(64 * waves_per_block),
-                      0, s, views, mollify, mp, out, out_stride));
+                      dim3(ceil_div(launch_bricks, waves_per_block), trees),
+                      dim3(64 * waves_per_block), 0, s, views, mollify, mp, out, out_stride));
     }
     return launch("vg_batch_velocity_grid", tab[policy][order][wi],
-                  dim3(ceil_div(mp.nbricks, waves_per_block), trees), dim3(64 * waves_per_block), 0,
-                  s, views, mollify, mp, out, out_stride);
+                  dim3(ceil_div(launch_bricks, waves_per_block), trees), dim3(64 * waves_per_block),
+                  0, s, views, mollify, mp, out, out_stride);
 }
 }  // namespace vg
 

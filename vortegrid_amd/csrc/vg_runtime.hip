@@ -35,8 +35,21 @@ struct Tally {
 };
 std::mutex g_prof_mu;
 bool g_prof_on = false;
+std::vector<std::string> g_prof_only;  // name prefixes to time; empty: every kernel
 std::vector<PendingLaunch *> g_pending;
+std::vector<hipEvent_t> g_free_events;  // recorded and read events, reused
 std::map<std::string, Tally> g_tally;
+
+hipEvent_t take_event_locked() {
+    if (!g_free_events.empty()) {
+        hipEvent_t e = g_free_events.back();
+        g_free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    hipEventCreate(&e);
+    return e;
+}
 
 void drain_locked() {
     for (PendingLaunch *p : g_pending) {
@@ -47,8 +60,8 @@ void drain_locked() {
             t.ms += ms;
             t.count += 1;
         }
-        hipEventDestroy(p->start);
-        hipEventDestroy(p->stop);
+        g_free_events.push_back(p->start);
+        g_free_events.push_back(p->stop);
         delete p;
     }
     g_pending.clear();
@@ -57,10 +70,22 @@ void drain_locked() {
 
 bool profiling_enabled() { return g_prof_on; }
 
+bool profile_wanted(const char *name) {
+    if (!g_prof_on) return false;
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    if (g_prof_only.empty()) return true;
+    for (const std::string &p : g_prof_only)
+        if (strncmp(name, p.c_str(), p.size()) == 0) return true;
+    return false;
+}
+
 void profile_begin(const char *name, hipStream_t s, void **token) {
     auto *p = new PendingLaunch{name, nullptr, nullptr};
-    hipEventCreate(&p->start);
-    hipEventCreate(&p->stop);
+    {
+        std::lock_guard<std::mutex> lk(g_prof_mu);
+        p->start = take_event_locked();
+        p->stop = take_event_locked();
+    }
     hipEventRecord(p->start, s);
     *token = p;
 }
@@ -157,6 +182,23 @@ VG_API vg_status vg_stream_synchronize(vg_stream s) {
 VG_API vg_status vg_profile_enable(int on) {
     std::lock_guard<std::mutex> lk(g_prof_mu);
     g_prof_on = on != 0;
+    return VG_OK;
+}
+
+VG_API vg_status vg_profile_select(const char *prefixes) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof_only.clear();
+    if (!prefixes) return VG_OK;
+    std::string cur;
+    for (const char *c = prefixes;; ++c) {
+        if (*c == ',' || *c == '\0') {
+            if (!cur.empty()) g_prof_only.push_back(cur);
+            cur.clear();
+            if (*c == '\0') break;
+        } else if (*c != ' ') {
+            cur += *c;
+        }
+    }
     return VG_OK;
 }
 

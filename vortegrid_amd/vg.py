@@ -22,7 +22,7 @@ TECH_DIRECT, TECH_TREE, TECH_POISSON = 0, 1, 2
 EXPORTS = (
     "vg_last_error vg_version vg_device_count vg_malloc vg_free vg_memcpy_htod vg_memcpy_dtoh "
     "vg_memset vg_stream_create vg_stream_destroy vg_stream_synchronize vg_profile_enable "
-    "vg_profile_reset vg_profile_get vg_profile_dump vg_define_shape vg_decimate "
+    "vg_profile_reset vg_profile_select vg_profile_get vg_profile_dump vg_define_shape vg_decimate "
     "vg_nested_layers vg_finalize_bounding_box vg_auto_mollify vg_find_bounding_box "
     "vg_direct_velocity vg_direct_vecpot vg_direct_velocity_grid vg_direct_velocity_f64 "
     "vg_direct_velocity_noalloc "
@@ -587,6 +587,13 @@ class Batch:
         call("vg_batch_grid", self.h, C.c_uint32(effect), C.byref(g), C.byref(f))
         return g, fetch(f, 3 * g.capacity).reshape(-1, 3)
 
+    def grid_ptr(self, effect):
+        """The effect's grid geometry and its device field pointer."""
+        g = GridDesc()
+        f = C.c_void_p()
+        call("vg_batch_grid", self.h, C.c_uint32(effect), C.byref(g), C.byref(f))
+        return g, f
+
     def desc(self, effect) -> GridDesc:
         """The effect's current grid geometry (no field copy)."""
         g = GridDesc()
@@ -840,6 +847,11 @@ def fetch(ptr: C.c_void_p, count: int, dtype=np.float32) -> np.ndarray:
 # ---------------------------------------------------------------- profiling
 def profile_enable(on=True):
     call("vg_profile_enable", int(on))
+
+
+def profile_select(prefixes=None):
+    """Time only kernels whose names start with one of the prefixes (None: all)."""
+    call("vg_profile_select", ",".join(prefixes).encode() if prefixes else None)
 
 
 def profile_reset():
