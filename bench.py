@@ -88,7 +88,9 @@ class HostFrame:
         # are put back into that grid's cell order first (they drift slowly)
         if self.grid is not None:
             g, p = self.grid
-            if a.sort_tracers > 0 and self.frames % a.sort_tracers == 0:
+            # sorts before frames 1, 1 + K, 1 + 2K, ... (the first as soon as
+            # a grid exists, so no timed frame runs on never-sorted tracers)
+            if a.sort_tracers > 0 and self.frames % a.sort_tracers == 1 % a.sort_tracers:
                 vg.sort_particles_by_cell(g, self.d_tr, self.nt, stream=self.stream)
             vg.advect_tracers(g, p, self.d_tr, None, self.nt, self.dt, 1.0, stream=self.stream)
         self.frames += 1
@@ -140,7 +142,8 @@ class DeviceFrame:
 
     def frame(self):
         a = self.args
-        if self.frames > 0 and a.sort_tracers > 0 and self.frames % a.sort_tracers == 0:
+        if (self.frames > 0 and a.sort_tracers > 0
+                and self.frames % a.sort_tracers == 1 % a.sort_tracers):
             self.b.sort_tracers(self.stream)
         self.b.step(self.dt, self.stream)
         self.frames += 1
