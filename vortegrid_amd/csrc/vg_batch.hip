@@ -342,14 +342,20 @@ __global__ void k_geometry(const uint32_t *__restrict__ bbox, const float *__res
             if (clamp) precompute_spacing(&g0);
         }
         if (clamp) atomicAdd(clamped, 1u);
-        Nested ng;
-        nested_init(&ng, g0);
-        tv.depth = ng.depth;
-        for (uint32_t l = 0; l < kMaxLayers; ++l) {
-            tv.layer[l] = ng.layer[l];
-            for (int a = 0; a < 3; ++a) tv.dec[l][a] = ng.dec[l][a];
+        // NestedGrid layers of g0 (nested_init's steps), written straight
+        // into the view: layers past the depth keep stale descriptors, which
+        // nothing reads
+        decimate(&tv.layer[0], &g0, 1);
+        uint32_t depth = 1;
+        while (capacity(tv.layer[depth - 1]) > 8 && depth < kMaxLayers) {
+            decimate(&tv.layer[depth], &tv.layer[depth - 1], 2);
+            ++depth;
         }
-        leaf_desc[e] = ng.layer[0];
+        tv.depth = depth;
+        for (uint32_t l = 1; l < depth; ++l)
+            for (int a = 0; a < 3; ++a)
+                tv.dec[l][a] = (tv.layer[l - 1].np[a] - 1) / (tv.layer[l].np[a] - 1);
+        leaf_desc[e] = tv.layer[0];
     } else {
         for (uint32_t l = 0; l < shape.depth; ++l) {
             vg_grid_desc g;

@@ -212,10 +212,13 @@ __global__ void k_gather_sorted(VortonIn vin, const uint32_t *__restrict__ sorte
 // MakeBaseVortonGrid accumulation + normalisation (vortonSim.cpp:1237-1310).
 // views (may be null): trees of their own shapes, blockIdx.y = tree, cap
 // the per-tree stride; only each tree's own leaf points are written.
+// skip_empty: cells without vortons are not written (their previous values
+// stay); only for trees whose readers reach leaf clusters through the cell
+// counts or the parents' child masks (the batched walk), never by scanning.
 __global__ void k_leaf(const uint32_t *__restrict__ start, const uint32_t *__restrict__ count,
                        const float4 *__restrict__ sp, const float4 *__restrict__ sw, size_t cap,
                        float4 *__restrict__ P, float4 *__restrict__ W, float4 *__restrict__ M,
-                       const TreeView *__restrict__ views) {
+                       const TreeView *__restrict__ views, int skip_empty) {
 #pragma clang fp contract(off)
     size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (views) {
@@ -225,6 +228,7 @@ __global__ void k_leaf(const uint32_t *__restrict__ start, const uint32_t *__res
         return;
     }
     const uint32_t cnt = count[c];
+    if (skip_empty && cnt == 0) return;
     float px = 0.f, py = 0.f, pz = 0.f, wx = 0.f, wy = 0.f, wz = 0.f, sz = 0.f;
     float aux = FLT_MIN;
     const uint32_t b = cnt ? start[c] : 0u;
@@ -262,11 +266,15 @@ __global__ void k_leaf(const uint32_t *__restrict__ start, const uint32_t *__res
 
 // AggregateClusters (vortonSim.cpp:1337-1569): parent point o of layer gp
 // from its dx x dy x dz children in layer gc.
+// cc (may be null): vorton counts of the child cells; children with none are
+// skipped without reading their clusters (they add exact zeros), so layer-0
+// clusters of empty cells need not be written (k_leaf skip_empty).
 __device__ __forceinline__ void aggregate_point(const vg_grid_desc &gp, const vg_grid_desc &gc,
                                                 uint32_t dx, uint32_t dy, uint32_t dz, size_t o,
                                                 const float4 *__restrict__ Pc,
                                                 const float4 *__restrict__ Wc,
                                                 const float4 *__restrict__ Mc,
+                                                const uint32_t *__restrict__ cc,
                                                 float4 *__restrict__ Pp, float4 *__restrict__ Wp,
                                                 float4 *__restrict__ Mp,
                                                 uint32_t *__restrict__ Cp) {
@@ -283,6 +291,7 @@ __device__ __forceinline__ void aggregate_point(const vg_grid_desc &gp, const vg
             for (uint32_t cy = 0; cy < dy; ++cy)
                 for (uint32_t cx = 0; cx < dx; ++cx) {
                     const size_t co = point_offset(gc, ix * dx + cx, iy * dy + cy, iz * dz + cz);
+                    if (cc && cc[co] == 0) continue;
                     const float4 p = Pc[co], w = Wc[co];
                     const float vx = w.x * 2.0f, vy = w.y * 2.0f, vz = w.z * 2.0f;
                     const float mag = sqrtf(vx * vx + vy * vy + vz * vz);
@@ -302,6 +311,7 @@ __device__ __forceinline__ void aggregate_point(const vg_grid_desc &gp, const vg
             for (uint32_t cy = 0; cy < dy; ++cy)
                 for (uint32_t cx = 0; cx < dx; ++cx) {
                     const size_t co = point_offset(gc, ix * dx + cx, iy * dy + cy, iz * dz + cz);
+                    if (cc && cc[co] == 0) continue;
                     const float4 p = Pc[co], w = Wc[co];
                     const float4 m0 = Mc[3 * co], m1 = Mc[3 * co + 1], m2 = Mc[3 * co + 2];
                     const float cd[9] = {m0.x, m0.y, m0.z, m1.x, m1.y, m1.z, m2.x, m2.y, m2.z};
@@ -328,31 +338,34 @@ __device__ __forceinline__ void aggregate_point(const vg_grid_desc &gp, const vg
 __global__ void k_aggregate(vg_grid_desc gp, vg_grid_desc gc, uint32_t dx, uint32_t dy,
                             uint32_t dz, const float4 *__restrict__ Pc,
                             const float4 *__restrict__ Wc, const float4 *__restrict__ Mc,
-                            float4 *__restrict__ Pp, float4 *__restrict__ Wp,
-                            float4 *__restrict__ Mp, uint32_t *__restrict__ Cp) {
+                            const uint32_t *__restrict__ cc, float4 *__restrict__ Pp,
+                            float4 *__restrict__ Wp, float4 *__restrict__ Mp,
+                            uint32_t *__restrict__ Cp) {
     const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t cap = capacity(gp);
     if (o >= cap) return;
     const size_t tp = (size_t)blockIdx.y * cap, tc = (size_t)blockIdx.y * capacity(gc);
-    aggregate_point(gp, gc, dx, dy, dz, o, Pc + tc, Wc + tc, Mc + 3 * tc, Pp + tp, Wp + tp,
-                    Mp + 3 * tp, Cp + tp);
+    aggregate_point(gp, gc, dx, dy, dz, o, Pc + tc, Wc + tc, Mc + 3 * tc, cc ? cc + tc : nullptr,
+                    Pp + tp, Wp + tp, Mp + 3 * tp, Cp + tp);
 }
 
 // Parent layer l of trees of different shapes (blockIdx.y = tree), each
 // read from its view; storage strides are the per-layer capacities cp, cc.
 __global__ void k_aggregate_views(const TreeView *__restrict__ views, uint32_t l, size_t cp,
-                                  size_t cc, const float4 *__restrict__ Pc,
+                                  size_t ccap, const float4 *__restrict__ Pc,
                                   const float4 *__restrict__ Wc, const float4 *__restrict__ Mc,
-                                  float4 *__restrict__ Pp, float4 *__restrict__ Wp,
-                                  float4 *__restrict__ Mp, uint32_t *__restrict__ Cp) {
+                                  const uint32_t *__restrict__ cc, float4 *__restrict__ Pp,
+                                  float4 *__restrict__ Wp, float4 *__restrict__ Mp,
+                                  uint32_t *__restrict__ Cp) {
     const TreeView &tv = views[blockIdx.y];
     if (l >= tv.depth) return;
     const vg_grid_desc &gp = tv.layer[l];
     const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= capacity(gp)) return;
-    const size_t tp = (size_t)blockIdx.y * cp, tc = (size_t)blockIdx.y * cc;
+    const size_t tp = (size_t)blockIdx.y * cp, tc = (size_t)blockIdx.y * ccap;
     aggregate_point(gp, tv.layer[l - 1], tv.dec[l][0], tv.dec[l][1], tv.dec[l][2], o, Pc + tc,
-                    Wc + tc, Mc + 3 * tc, Pp + tp, Wp + tp, Mp + 3 * tp, Cp + tp);
+                    Wc + tc, Mc + 3 * tc, cc ? cc + tc : nullptr, Pp + tp, Wp + tp, Mp + 3 * tp,
+                    Cp + tp);
 }
 
 template <typename T>
@@ -918,31 +931,35 @@ vg_status batch_tree_build(const BatchTreeArgs &a, hipStream_t s) {
     VG_TRY(launch("vg_batch_gather", k_gather_batch, dim3(ceil_div(n, 256)), dim3(256), 0, s, a.pos,
                   a.angvel, a.tree_of, a.size, (const uint32_t *)a.sorted_idx, n, a.sorted_p,
                   a.sorted_w));
+    // The batched walk reaches leaf clusters only through the layer-1 child
+    // masks and the cell lists, so empty leaf cells are neither written nor,
+    // in layer 1's aggregation, read.
     if (a.views)
         VG_TRY(launch("vg_batch_leaf", k_leaf, dim3(ceil_div(cap0, 256), a.trees), dim3(256), 0, s,
                       (const uint32_t *)a.cell_start, (const uint32_t *)a.cell_count,
                       (const float4 *)a.sorted_p, (const float4 *)a.sorted_w, cap0, a.P[0],
-                      a.W[0], a.M[0], a.views));
+                      a.W[0], a.M[0], a.views, 1));
     else
         VG_TRY(launch("vg_batch_leaf", k_leaf, dim3(ceil_div(cells, 256)), dim3(256), 0, s,
                       (const uint32_t *)a.cell_start, (const uint32_t *)a.cell_count,
                       (const float4 *)a.sorted_p, (const float4 *)a.sorted_w, cells, a.P[0],
-                      a.W[0], a.M[0], (const TreeView *)nullptr));
+                      a.W[0], a.M[0], (const TreeView *)nullptr, 1));
     for (uint32_t l = 1; l < ng.depth; ++l) {
         const size_t cap = capacity(ng.layer[l]);
+        const uint32_t *cc = l == 1 ? (const uint32_t *)a.cell_count : nullptr;
         if (a.views) {
             // per-tree shapes within ng (the largest shape): storage strides
             // are ng's capacities, geometry comes from each view
             VG_TRY(launch("vg_batch_aggregate", k_aggregate_views, dim3(ceil_div(cap, 256), a.trees),
                           dim3(256), 0, s, a.views, l, cap, capacity(ng.layer[l - 1]),
                           (const float4 *)a.P[l - 1], (const float4 *)a.W[l - 1],
-                          (const float4 *)a.M[l - 1], a.P[l], a.W[l], a.M[l], a.C[l]));
+                          (const float4 *)a.M[l - 1], cc, a.P[l], a.W[l], a.M[l], a.C[l]));
             continue;
         }
         VG_TRY(launch("vg_batch_aggregate", k_aggregate, dim3(ceil_div(cap, 256), a.trees), dim3(256),
                       0, s, ng.layer[l], ng.layer[l - 1], ng.dec[l][0], ng.dec[l][1], ng.dec[l][2],
                       (const float4 *)a.P[l - 1], (const float4 *)a.W[l - 1],
-                      (const float4 *)a.M[l - 1], a.P[l], a.W[l], a.M[l], a.C[l]));
+                      (const float4 *)a.M[l - 1], cc, a.P[l], a.W[l], a.M[l], a.C[l]));
     }
     return VG_OK;
 }
@@ -1074,13 +1091,14 @@ VG_API vg_status vg_tree_build(vg_tree *t, const vg_grid_desc *tmpl, const vg_vo
     VG_TRY(launch("vg_tree_leaf", k_leaf, dim3(ceil_div(cap0, 256)), dim3(256), 0, s,
                   (const uint32_t *)t->cell_start, (const uint32_t *)t->cell_count,
                   (const float4 *)t->sorted_p, (const float4 *)t->sorted_w, cap0, t->P[0],
-                  t->W[0], t->M[0], (const TreeView *)nullptr));
+                  t->W[0], t->M[0], (const TreeView *)nullptr, 0));
     for (uint32_t l = 1; l < t->ng.depth; ++l) {
         const size_t cap = capacity(t->ng.layer[l]);
         VG_TRY(launch("vg_tree_aggregate", k_aggregate, dim3(ceil_div(cap, 256)), dim3(256), 0, s,
                       t->ng.layer[l], t->ng.layer[l - 1], t->ng.dec[l][0], t->ng.dec[l][1],
                       t->ng.dec[l][2], (const float4 *)t->P[l - 1], (const float4 *)t->W[l - 1],
-                      (const float4 *)t->M[l - 1], t->P[l], t->W[l], t->M[l], t->C[l]));
+                      (const float4 *)t->M[l - 1], (const uint32_t *)nullptr, t->P[l], t->W[l],
+                      t->M[l], t->C[l]));
     }
     t->built = true;
     return VG_OK;
