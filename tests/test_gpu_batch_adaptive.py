@@ -59,7 +59,7 @@ def single_tree_grid(vg, g, pos, om, radius, spread, circ, near_k, order, expand
 
 @pytest.mark.parametrize("expand_from", [-1, 2])
 def test_adaptive_shapes_equal_host_define_shape_and_single_trees(vg, expand_from):
-    counts = [4096, 700, 3, 2500, 1]
+    counts = [4096, 700, 0, 3, 2500, 1]
     case(f"adaptive batch: device DefineShape vs host, grids vs single trees, effects {counts}, "
          f"expand_from={expand_from}")
     b, vp, vw, tp, radii = adaptive_batch(vg, counts, seed=23, near_k=2, order=1,
@@ -77,6 +77,15 @@ def test_adaptive_shapes_equal_host_define_shape_and_single_trees(vg, expand_fro
             # positions at the start of the frame, after the tracers moved
             pv = prev_v[e] if frame else vp[e]
             pt = tpos[to[e]:to[e + 1]]
+            if n == 0:
+                # no vortons: 4 cells per axis over the padded tracer box, no field
+                pad = F32(1.0 + 2.0) * F32(radii[e])
+                mn, mx = (pt.min(0) - pad).astype(F32), (pt.max(0) + pad).astype(F32)
+                assert list(g.np) == [5, 5, 5]
+                assert np.array_equal(np.float32(g.min), mn)
+                assert np.array_equal(np.float32(g.extent), (mx - mn).astype(F32))
+                assert not np.any(field)
+                continue
             hg, _ = host_shape(vg, np.concatenate([pv, pt]), n, radii[e], 64)
             assert same_desc(g, hg), (frame, e, list(g.np), list(hg.np))
             if n == 1:
