@@ -158,6 +158,68 @@ def test_tree_velocity_and_vecpot_at_points(vg, oracle, rings2):
     assert rel_l2(vp.numpy(), wp) < 1e-5
 
 
+@pytest.fixture(scope="module")
+def slab(oracle):
+    """6000 vortons in a slab 2 x 2 x 0.08: a 64 x 64 x 4-cell tree whose
+    layers 1-2 halve every axis and layers 3-6 keep z (decimation 2, 2, 1)."""
+    rng = np.random.default_rng(41)
+    n = 6000
+    pos = np.column_stack([rng.uniform(-1, 1, n), rng.uniform(-1, 1, n),
+                           rng.uniform(-0.04, 0.04, n)]).astype(F32)
+    om = rng.normal(size=(n, 3)).astype(F32)
+    radius = 0.02
+    og = oracle.grid_for(pos, radius)
+    ot = oracle.InfluenceTree(og, pos, om, 2 * radius)
+    assert [tuple(ot.decimations(l)) for l in range(1, ot.depth)] == \
+        [(2, 2, 2)] * 2 + [(2, 2, 1)] * (ot.depth - 3)
+    return pos, om, radius, og, ot
+
+
+@pytest.mark.parametrize("policy", [0, 1])
+def test_tree_walk_mixed_decimation_and_outside_queries(vg, oracle, slab, policy):
+    """The walk visits the non-empty children of a layer through the parents'
+    child masks where the layer halves every axis and enumerates the children
+    elsewhere: grids, faces and point queries on a tree with both kinds of
+    layer, with query grids that cover, undershoot and overreach the tree."""
+    pos, om, radius, og, ot = slab
+    case(f"tree walk on a slab tree {list(og.np)}, mixed decimation, policy={policy}")
+    v, keep = vortons(vg, pos, om, radius)
+    t = vg.Tree()
+    t.build(vg.GridDesc.from_fields(og), v)
+    spread, circ = vg.auto_mollify(vg.GridDesc.from_fields(og), radius)
+    mn, mx = np.float32(og.min), np.float32(og.min) + np.float32(og.extent)
+    wide = oracle.define_shape(len(pos), mn - (mx - mn), mx + (mx - mn))
+    coarse = oracle.define_shape(64, mn, mx)
+    for qg in (og, wide, coarse):
+        g = vg.GridDesc.from_fields(qg)
+        out = vg.DeviceArray.zeros((g.capacity, 3))
+        t.velocity_grid(g, out, spread, circ, policy)
+        vg.synchronize()
+        want = oracle.velocity_grid(ot, qg, pos, om, 2 * radius, technique=1, spread=spread,
+                                    circ=circ, policy=policy, threads=8)
+        assert rel_l2(out.numpy(), want) < 1e-5, list(qg.np)
+    g = vg.GridDesc.from_fields(og)
+    for copy_odd in (True, False):
+        out = vg.DeviceArray.zeros((g.capacity, 3))
+        t.vecpot_boundary(g, out, 1.0, 1.0, policy, copy_odd)
+        vg.synchronize()
+        want = oracle.vecpot_boundary(ot, og, pos, om, 2 * radius, 1.0, 1.0, policy, copy_odd,
+                                      threads=8)
+        assert rel_l2(out.numpy(), want) < 1e-5, copy_odd
+    rng = np.random.default_rng(43)
+    q = np.concatenate([pos[::29], rng.uniform(mn - 0.5, mx + 0.5, (300, 3))]).astype(F32)
+    dq = upload(vg, q)
+    vel = vg.DeviceArray.zeros((len(q), 3))
+    vp = vg.DeviceArray.zeros((len(q), 3))
+    t.velocity(dq, len(q), vel, spread, circ, policy)
+    t.vecpot(dq, len(q), vp, spread, circ, policy)
+    vg.synchronize()
+    wv = np.array([ot.velocity(x, spread, circ, policy) for x in q])
+    wp = np.array([ot.vecpot(x, spread, circ, policy) for x in q])
+    assert rel_l2(vel.numpy(), wv) < 1e-5
+    assert rel_l2(vp.numpy(), wp) < 1e-5
+
+
 @pytest.mark.parametrize("copy_odd", [True, False])
 def test_boundary_vecpot(vg, oracle, rings2, copy_odd):
     pos, om, radius, og = rings2

@@ -491,9 +491,13 @@ __device__ float3 sorted_direct(const TreeView &tv, float3 q, float spread) {
     return sum;
 }
 
+template <int POLICY, int VECPOT, int SPLIT>
+__device__ float3 tree_eval_split(const TreeView &tv, float3 q, float spread, uint32_t sub);
+
+// One lane per query point: the walk of tree_eval_split with SPLIT = 1.
 template <int POLICY, int VECPOT>
 __device__ __forceinline__ float3 eval_query(const TreeView &tv, float3 q, float spread) {
-    return tv.depth > 1 ? tree_eval<POLICY, VECPOT>(tv, q, spread)
+    return tv.depth > 1 ? tree_eval_split<POLICY, VECPOT, 1>(tv, q, spread, 0u)
                         : sorted_direct<POLICY, VECPOT>(tv, q, spread);
 }
 
@@ -509,15 +513,103 @@ __global__ __launch_bounds__(256) void k_tree_points(const TreeView tv, QuerySrc
     out[3 * i + 2] = r.z * circ;
 }
 
+// One cluster applied to the query as a supervorton (vortonSim.cpp:1875-1890).
+template <int POLICY, int VECPOT>
+__device__ __forceinline__ void apply_cluster(float3 &part, float3 q, float4 p, float4 w,
+                                              float spread) {
+    const float radius = p.w * 0.5f * spread;
+    const float r2 = radius * radius;
+    const float s = (2.0f / 3.0f) * r2 * radius;
+    const float4 A = make_float4(p.x, p.y, p.z, r2);
+    const float4 B = make_float4(w.x * s, w.y * s, w.z * s, __builtin_amdgcn_rcpf(r2 * radius));
+    if (VECPOT) pair_vecpot<POLICY>(part, q, A, B);
+    else pair_velocity<POLICY>(part, q, A, B);
+}
+
+// A layer's candidates by enumeration: every child in [c0, c1] (x fastest)
+// outside the opened box, dealt round robin to the SPLIT lanes; empty
+// clusters are read and skipped.  Any decimation.
+template <int POLICY, int VECPOT, int SPLIT>
+__device__ __forceinline__ void layer_enumerated(float3 &part, const TreeView &tv, int il, float3 q,
+                                                 float spread, const int *c0, const int *c1,
+                                                 bool open_any, const int *olo, const int *ohi,
+                                                 uint32_t sub) {
+    const vg_grid_desc &gc = tv.layer[il - 1];
+    const float4 *__restrict__ P = tv.P[il - 1];
+    const float4 *__restrict__ W = tv.W[il - 1];
+    const int bx = c1[0] - c0[0] + 1, by = c1[1] - c0[1] + 1, bz = c1[2] - c0[2] + 1;
+    const int total = bx * by * bz;
+    for (int t = (int)sub; t < total; t += SPLIT) {
+        const int rx = t % bx, r = t / bx;
+        const int cx = c0[0] + rx, cy = c0[1] + r % by, cz = c0[2] + r / by;
+        if (open_any && cx >= olo[0] && cx <= ohi[0] && cy >= olo[1] && cy <= ohi[1] &&
+            cz >= olo[2] && cz <= ohi[2])
+            continue;
+        const size_t off = point_offset(gc, cx, cy, cz);
+        const float4 p = P[off];
+        const float4 w = W[off];
+        if (p.w == 0.0f) continue;  // empty cluster: zero circulation
+        apply_cluster<POLICY, VECPOT>(part, q, p, w, spread);
+    }
+}
+
+// A layer's candidates through the parents' masks of non-empty children
+// (bit cx + 2 (cy + 2 cz), decimation 2 on every axis, at most 2 parents per
+// axis).  The candidates form a mask over the (at most) 4x4x4 children box,
+// bit x + 4y + 16z standing for child c0 + (x, y, z), less the opened box;
+// only its set bits are visited, in bit order (z, y, x: the order of the
+// enumeration), lane `sub` taking the bits of rank sub, sub + SPLIT, ...
+// Empty clusters cost nothing here, and most candidates are empty (about 70 %
+// on a 64-cell grid around two rings).
+template <int POLICY, int VECPOT, int SPLIT>
+__device__ __forceinline__ void layer_masked(float3 &part, const TreeView &tv, int il, float3 q,
+                                             float spread, const int *lo, const int *hi,
+                                             const int *c0, bool open_any, const int *olo,
+                                             const int *ohi, uint32_t sub) {
+    const vg_grid_desc &gp = tv.layer[il];
+    const vg_grid_desc &gc = tv.layer[il - 1];
+    const uint32_t *__restrict__ Cm = tv.C[il];
+    const float4 *__restrict__ P = tv.P[il - 1];
+    const float4 *__restrict__ W = tv.W[il - 1];
+    uint64_t cand = 0;
+    for (int pz = lo[2]; pz <= hi[2]; ++pz)
+        for (int py = lo[1]; py <= hi[1]; ++py)
+            for (int px = lo[0]; px <= hi[0]; ++px) {
+                const uint64_t cm = Cm[point_offset(gp, px, py, pz)];
+                const uint64_t e = (cm & 3u) | ((cm & 0xCu) << 2) | ((cm & 0x30u) << 12) |
+                                   ((cm & 0xC0u) << 14);
+                cand |= e << (2 * (px - lo[0]) + 8 * (py - lo[1]) + 32 * (pz - lo[2]));
+            }
+    if (open_any) {
+        // opened box as bit rows (x), row sets (y) and planes (z)
+        const uint32_t xm = ((1u << (ohi[0] - olo[0] + 1)) - 1u) << (olo[0] - c0[0]);
+        const uint32_t ym = ((1u << (ohi[1] - olo[1] + 1)) - 1u) << (olo[1] - c0[1]);
+        const uint32_t zm = ((1u << (ohi[2] - olo[2] + 1)) - 1u) << (olo[2] - c0[2]);
+        const uint32_t sy = (ym & 1u) | ((ym & 2u) << 3) | ((ym & 4u) << 6) | ((ym & 8u) << 9);
+        const uint64_t sz = (uint64_t)(zm & 1u) | ((uint64_t)(zm & 2u) << 15) |
+                            ((uint64_t)(zm & 4u) << 30) | ((uint64_t)(zm & 8u) << 45);
+        cand &= ~((uint64_t)(xm * sy) * sz);
+    }
+    for (uint32_t k = 0; k < sub; ++k) cand &= cand - 1;
+    while (cand) {
+        const int b = __builtin_ctzll(cand);
+#pragma unroll
+        for (int k = 0; k < SPLIT; ++k) cand &= cand - 1;
+        const size_t off =
+            point_offset(gc, c0[0] + (b & 3), c0[1] + ((b >> 2) & 3), c0[2] + (b >> 4));
+        apply_cluster<POLICY, VECPOT>(part, q, P[off], W[off], spread);
+    }
+}
+
 // The reference-rule walk of one query split over SPLIT lanes (lanes of one
 // query are adjacent and follow the same control flow): on every layer the
-// children of the parent's opened box are dealt round robin to the lanes,
-// which sum their share; the shares are added across the lanes (the same
-// value on each) before the layer's Kahan step.  Opening decisions are those
-// of tree_eval; only the summation order differs.  Splitting a query puts
-// SPLIT times as many waves in flight, which is what hides the cluster
-// loads' latency: the walk does ~50 dependent gathers per query and little
-// arithmetic.
+// candidates (children of the parents' box outside the opened box) are dealt
+// to the lanes, which sum their share; the shares are added across the lanes
+// (the same value on each) before the layer's Kahan step.  Opening decisions
+// are those of tree_eval; only the summation order differs.  The walk is
+// bound by instruction issue, not by its loads (the loads hit the caches), so
+// what counts is the work per candidate: layers of decimation 2 visit only the
+// non-empty children (layer_masked), others enumerate (layer_enumerated).
 template <int POLICY, int VECPOT, int SPLIT>
 __device__ float3 tree_eval_split(const TreeView &tv, float3 q, float spread, uint32_t sub) {
     int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
@@ -599,30 +691,14 @@ __device__ float3 tree_eval_split(const TreeView &tv, float3 q, float spread, ui
                 }
             }
         }
-        const float4 *__restrict__ P = tv.P[il - 1];
-        const float4 *__restrict__ W = tv.W[il - 1];
-        const int bx = c1[0] - c0[0] + 1, by = c1[1] - c0[1] + 1, bz = c1[2] - c0[2] + 1;
-        const int total = bx * by * bz;
         float3 part = make_float3(0.f, 0.f, 0.f);
-        for (int t = (int)sub; t < total; t += SPLIT) {
-            const int rx = t % bx, r = t / bx;
-            const int cx = c0[0] + rx, cy = c0[1] + r % by, cz = c0[2] + r / by;
-            if (open_any && cx >= olo[0] && cx <= ohi[0] && cy >= olo[1] && cy <= ohi[1] &&
-                cz >= olo[2] && cz <= ohi[2])
-                continue;
-            const size_t off = point_offset(gc, cx, cy, cz);
-            const float4 p = P[off];
-            const float4 w = W[off];
-            if (p.w == 0.0f) continue;  // empty cluster: zero circulation
-            const float radius = p.w * 0.5f * spread;
-            const float r2 = radius * radius;
-            const float s = (2.0f / 3.0f) * r2 * radius;
-            const float4 A = make_float4(p.x, p.y, p.z, r2);
-            const float4 B =
-                make_float4(w.x * s, w.y * s, w.z * s, __builtin_amdgcn_rcpf(r2 * radius));
-            if (VECPOT) pair_vecpot<POLICY>(part, q, A, B);
-            else pair_velocity<POLICY>(part, q, A, B);
-        }
+        if (dims[0] == 2 && dims[1] == 2 && dims[2] == 2 && hi[0] - lo[0] <= 1 &&
+            hi[1] - lo[1] <= 1 && hi[2] - lo[2] <= 1)
+            layer_masked<POLICY, VECPOT, SPLIT>(part, tv, il, q, spread, lo, hi, c0, open_any, olo,
+                                                ohi, sub);
+        else
+            layer_enumerated<POLICY, VECPOT, SPLIT>(part, tv, il, q, spread, c0, c1, open_any, olo,
+                                                    ohi, sub);
 #pragma unroll
         for (int m = 1; m < SPLIT; m <<= 1) {
             part.x += __shfl_xor(part.x, m, 64);
@@ -637,16 +713,21 @@ __device__ float3 tree_eval_split(const TreeView &tv, float3 q, float spread, ui
     return sum;
 }
 
-// One workgroup per 4x4x4 brick of grid points, four lanes per point (lane
-// t: point t / 4 of the brick, x fastest, share t % 4 of its walk).
-constexpr int kTreeSplit = 4;
+// One workgroup per 4x4x4 brick of grid points, kTreeGridSplit lanes per
+// point (lane t: point t / kTreeGridSplit of the brick, x fastest, share
+// t % kTreeGridSplit of its walk).  Measured at 65^3 around two rings: 2
+// lanes per point 0.055 ms, 4 lanes 0.060 ms, 1 lane 0.059 ms.
+constexpr int kTreeGridSplit = 2;
+// Face points: 4 lanes each (0.0195 ms for the faces of a 65^3 grid, against
+// 0.023 ms with 2 lanes and 0.032 ms with 1).
+constexpr int kTreeFaceSplit = 4;
 template <int POLICY>
-__global__ __launch_bounds__(256) void k_tree_grid(const TreeView tv, vg_grid_desc g, uint32_t nbx,
+__global__ __launch_bounds__(64 * kTreeGridSplit) void k_tree_grid(const TreeView tv, vg_grid_desc g, uint32_t nbx,
                                                    uint32_t nby, uint32_t nbricks, float spread,
                                                    float circ, float *__restrict__ out) {
     const uint32_t brick = blockIdx.x;
     if (brick >= nbricks) return;
-    const uint32_t sub = threadIdx.x % kTreeSplit, pt = threadIdx.x / kTreeSplit;
+    const uint32_t sub = threadIdx.x % kTreeGridSplit, pt = threadIdx.x / kTreeGridSplit;
     uint32_t ix = 4u * (brick % nbx) + (pt & 3u);
     uint32_t iy = 4u * ((brick / nbx) % nby) + ((pt >> 2) & 3u);
     uint32_t iz = 4u * (brick / (nbx * nby)) + (pt >> 4);
@@ -658,7 +739,7 @@ __global__ __launch_bounds__(256) void k_tree_grid(const TreeView tv, vg_grid_de
     iz = iz < g.np[2] ? iz : g.np[2] - 1;
     const size_t i = point_offset(g, ix, iy, iz);
     const float3 q = grid_query_point(g, ix, iy, iz);
-    const float3 r = tv.depth > 1 ? tree_eval_split<POLICY, 0, kTreeSplit>(tv, q, spread, sub)
+    const float3 r = tv.depth > 1 ? tree_eval_split<POLICY, 0, kTreeGridSplit>(tv, q, spread, sub)
                                   : sorted_direct<POLICY, 0>(tv, q, spread);
     if (!valid || sub != 0) return;
     out[3 * i] = r.x * circ;
@@ -726,16 +807,16 @@ template <int POLICY>
 __global__ __launch_bounds__(256) void k_tree_faces(const TreeView tv, vg_grid_desc g,
                                                     size_t nface, int copy_odd, float spread,
                                                     float circ, float *__restrict__ out) {
-    // kTreeSplit lanes per face point, which leave together
+    // kTreeFaceSplit lanes per face point, which leave together
     const size_t gt = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t t = gt / kTreeSplit;
-    const uint32_t sub = (uint32_t)(gt % kTreeSplit);
+    const size_t t = gt / kTreeFaceSplit;
+    const uint32_t sub = (uint32_t)(gt % kTreeFaceSplit);
     if (t >= nface) return;
     uint32_t ix, iy, iz;
     if (!face_point(g, t, ix, iy, iz)) return;
     if (copy_odd && (ix & 1u)) return;
     const float3 q = grid_query_point(g, ix, iy, iz);
-    const float3 r = tv.depth > 1 ? tree_eval_split<POLICY, 1, kTreeSplit>(tv, q, spread, sub)
+    const float3 r = tv.depth > 1 ? tree_eval_split<POLICY, 1, kTreeFaceSplit>(tv, q, spread, sub)
                                   : sorted_direct<POLICY, 1>(tv, q, spread);
     if (sub != 0) return;
     const size_t o = point_offset(g, ix, iy, iz);
@@ -1206,7 +1287,7 @@ VG_API vg_status vg_tree_velocity_grid(const vg_tree *t, const vg_grid_desc *g, 
                                                 : k_tree_grid<VG_RSQRT_ACCURATE>;
     const uint32_t nbx = (g->np[0] + 3) / 4, nby = (g->np[1] + 3) / 4, nbz = (g->np[2] + 3) / 4;
     const uint32_t nbricks = nbx * nby * nbz;
-    return launch("vg_tree_velocity_grid", k, dim3(nbricks), dim3(64 * kTreeSplit), 0,
+    return launch("vg_tree_velocity_grid", k, dim3(nbricks), dim3(64 * kTreeGridSplit), 0,
                   (hipStream_t)s, make_view(t), *g, nbx, nby, nbricks, spread, circ, out);
 }
 
@@ -1242,7 +1323,7 @@ VG_API vg_status vg_tree_vecpot_boundary(const vg_tree *t, const vg_grid_desc *g
     // With no vortons every cluster is empty and the faces evaluate to zero.
     auto k = rsqrt_policy == VG_RSQRT_REFERENCE ? k_tree_faces<VG_RSQRT_REFERENCE>
                                                 : k_tree_faces<VG_RSQRT_ACCURATE>;
-    VG_TRY(launch("vg_tree_vecpot_faces", k, dim3(ceil_div(nf * kTreeSplit, 256)), dim3(256), 0, s,
+    VG_TRY(launch("vg_tree_vecpot_faces", k, dim3(ceil_div(nf * kTreeFaceSplit, 256)), dim3(256), 0, s,
                   make_view(t), *g, nf, copy_odd, spread, circ, out));
     if (copy_odd)
         VG_TRY(launch("vg_tree_faces_copy_odd", k_faces_copy_odd, dim3(ceil_div(nf, 256)),
