@@ -455,14 +455,14 @@ VG_API vg_status vg_mg_create(vg_multigrid **m) {
 VG_API vg_status vg_mg_destroy(vg_multigrid *m) {
     if (!m) return VG_OK;
     for (uint32_t l = 0; l < kMaxLayers; ++l) {
-        hipFree(m->A[l]);
-        hipFree(m->W[l]);
-        hipFree(m->u[l]);
-        hipFree(m->f[l]);
-        hipFree(m->r[l]);
+        (void)hipFree(m->A[l]);
+        (void)hipFree(m->W[l]);
+        (void)hipFree(m->u[l]);
+        (void)hipFree(m->f[l]);
+        (void)hipFree(m->r[l]);
     }
-    hipFree(m->lo);
-    hipFree(m->part);
+    (void)hipFree(m->lo);
+    (void)hipFree(m->part);
     delete m;
     return VG_OK;
 }

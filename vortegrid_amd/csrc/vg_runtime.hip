@@ -47,15 +47,15 @@ hipEvent_t take_event_locked() {
         return e;
     }
     hipEvent_t e = nullptr;
-    hipEventCreate(&e);
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
     return e;
 }
 
 void drain_locked() {
     for (PendingLaunch *p : g_pending) {
-        hipEventSynchronize(p->stop);
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, p->start, p->stop) == hipSuccess) {
+        if (hipEventSynchronize(p->stop) == hipSuccess &&
+            hipEventElapsedTime(&ms, p->start, p->stop) == hipSuccess) {
             Tally &t = g_tally[p->name];
             t.ms += ms;
             t.count += 1;
@@ -79,22 +79,35 @@ bool profile_wanted(const char *name) {
     return false;
 }
 
+// A launch whose events cannot be created or recorded is left untimed (the
+// profiler is a diagnostic: it never fails the launch it brackets).
 void profile_begin(const char *name, hipStream_t s, void **token) {
+    *token = nullptr;
     auto *p = new PendingLaunch{name, nullptr, nullptr};
-    {
-        std::lock_guard<std::mutex> lk(g_prof_mu);
-        p->start = take_event_locked();
-        p->stop = take_event_locked();
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    p->start = take_event_locked();
+    p->stop = take_event_locked();
+    if (p->start && p->stop && hipEventRecord(p->start, s) == hipSuccess) {
+        *token = p;
+        return;
     }
-    hipEventRecord(p->start, s);
-    *token = p;
+    if (p->start) g_free_events.push_back(p->start);
+    if (p->stop) g_free_events.push_back(p->stop);
+    delete p;
 }
 
 void profile_end(void *token, hipStream_t s) {
     auto *p = static_cast<PendingLaunch *>(token);
-    hipEventRecord(p->stop, s);
+    if (!p) return;
+    const bool recorded = hipEventRecord(p->stop, s) == hipSuccess;
     std::lock_guard<std::mutex> lk(g_prof_mu);
-    g_pending.push_back(p);
+    if (recorded) {
+        g_pending.push_back(p);
+        return;
+    }
+    g_free_events.push_back(p->start);
+    g_free_events.push_back(p->stop);
+    delete p;
 }
 
 // ------------------------------------------------------------------ scratch

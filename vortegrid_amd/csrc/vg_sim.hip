@@ -108,10 +108,10 @@ VG_API vg_status vg_sim_destroy(vg_sim *sim) {
     if (!sim) return VG_OK;
     vg_tree_destroy(sim->tree);
     vg_mg_destroy(sim->mg);
-    hipFree(sim->vel_grid);
-    hipFree(sim->a0);
-    hipFree(sim->negw);
-    hipFree(sim->vort_vel);
+    (void)hipFree(sim->vel_grid);
+    (void)hipFree(sim->a0);
+    (void)hipFree(sim->negw);
+    (void)hipFree(sim->vort_vel);
     delete sim;
     return VG_OK;
 }

@@ -1093,19 +1093,19 @@ VG_API vg_status vg_tree_create(vg_tree **t) {
 VG_API vg_status vg_tree_destroy(vg_tree *t) {
     if (!t) return VG_OK;
     for (uint32_t l = 0; l < kMaxLayers; ++l) {
-        hipFree(t->P[l]);
-        hipFree(t->W[l]);
-        hipFree(t->M[l]);
-        hipFree(t->C[l]);
+        (void)hipFree(t->P[l]);
+        (void)hipFree(t->W[l]);
+        (void)hipFree(t->M[l]);
+        (void)hipFree(t->C[l]);
     }
-    hipFree(t->cell_start);
-    hipFree(t->cell_count);
-    hipFree(t->keys);
-    hipFree(t->slots);
-    hipFree(t->sorted_idx);
-    hipFree(t->sorted_p);
-    hipFree(t->sorted_w);
-    hipFree(t->scan_tmp);
+    (void)hipFree(t->cell_start);
+    (void)hipFree(t->cell_count);
+    (void)hipFree(t->keys);
+    (void)hipFree(t->slots);
+    (void)hipFree(t->sorted_idx);
+    (void)hipFree(t->sorted_p);
+    (void)hipFree(t->sorted_w);
+    (void)hipFree(t->scan_tmp);
     delete t;
     return VG_OK;
 }
