@@ -114,16 +114,19 @@ __device__ __forceinline__ void set_size(SizeFactors &f, float size, float sprea
 }
 
 // Monopole (+ first-moment) contribution of one cluster, unscaled by s.
+// m0..m2 carry the first moment's rows times 3 (.xyz, staged so by the
+// walk) and its antisymmetric part (.w, unscaled).
 // FAR: the caller knows the cluster lies outside the core radius (the law
-// inside the core never applies); d2 is kept away from 0 so that a cluster
-// of zero vorticity (whose centroid defaults to the origin) adds 0 even at
-// a query point on the origin.
+// inside the core never applies); d2 carries 1e-24 so that a cluster of
+// zero vorticity (whose centroid defaults to the origin) adds 0 even at a
+// query point on the origin (1e-24 is below half an ulp of any squared
+// distance above 1e-16, so elsewhere it can only settle a rounding tie).
 template <int POLICY, int ORDER, bool FAR = false>
 __device__ __forceinline__ void cluster_term(float3 &part, float3 q, float4 p, float4 w, float4 m0,
                                              float4 m1, float4 m2, const SizeFactors &f) {
     const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
-    float d2 = dx * dx + dy * dy + dz * dz;
-    if (FAR) d2 = fmaxf(d2, 1e-24f);
+    const float d2 = FAR ? fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, 1e-24f)))
+                         : dx * dx + dy * dy + dz * dz;
     float law, rs2;
     if (POLICY == VG_RSQRT_REFERENCE) {
         rs2 = __builtin_amdgcn_rcpf(d2);
@@ -137,12 +140,12 @@ __device__ __forceinline__ void cluster_term(float3 &part, float3 q, float4 p, f
     const float lm = inside ? f.ir3 : law;
     if (ORDER >= 1) {
         // (w x d) lm - (a - 3 rs2 (t x d)) ld with t = D d folds into
-        // ((w + c3 t) x d) lm - a ld, c3 = 3 rs2 outside the core, 0 inside
-        // (where ld = 0): one cross product instead of two
+        // ((w + c3 t3) x d) lm - a ld with t3 = 3 D d, c3 = rs2 outside the
+        // core, 0 inside (where ld = 0): one cross product instead of two
         const float tx = m0.x * dx + m0.y * dy + m0.z * dz;
         const float ty = m1.x * dx + m1.y * dy + m1.z * dz;
         const float tz = m2.x * dx + m2.y * dy + m2.z * dz;
-        const float c3 = inside ? 0.0f : 3.0f * rs2;
+        const float c3 = inside ? 0.0f : rs2;
         const float vx = w.x + c3 * tx, vy = w.y + c3 * ty, vz = w.z + c3 * tz;
         if (FAR) {
             // lm = ld = law: ((v x d) - a) law, three fused operations a row
@@ -472,9 +475,10 @@ __device__ __forceinline__ void mac_brick(const TreeView &tv, const vg_grid_desc
                 S.P[lane] = p;
                 S.W[lane] = w;
                 if (ORDER) {
-                    S.M0[lane] = m0;
-                    S.M1[lane] = m1;
-                    S.M2[lane] = m2;
+                    // rows times 3 (cluster_term's convention), once per cluster
+                    S.M0[lane] = make_float4(3.0f * m0.x, 3.0f * m0.y, 3.0f * m0.z, m0.w);
+                    S.M1[lane] = make_float4(3.0f * m1.x, 3.0f * m1.y, 3.0f * m1.z, m1.w);
+                    S.M2[lane] = make_float4(3.0f * m2.x, 3.0f * m2.y, 3.0f * m2.z, m2.w);
                 }
                 // every cluster of the batch has the current size (the usual
                 // case: one vorton size per tree)?
