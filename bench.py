@@ -43,10 +43,11 @@ FLOP_PER_PAIR = 26.0   # monopole / direct pair (FMA = 2, rsqrt = 1)
 FLOP_DIPOLE = 42.0     # first-moment correction of one cluster interaction
 # brick-centred local expansion of one cluster (FMA = 2): monopole to second
 # order (offset and distance powers 20, Omega x d 9, value 6, gradient 42,
-# Hessian 150) and first moment to first order (moments and t x d 51,
-# gradient 99); per grid point, the 30-coefficient wave sum (180) and the
-# polynomial (60)
-FLOP_EXPAND_MONO = 227.0
+# Hessian 98: the 10 distinct components of the symmetric second-derivative
+# tensor 26, Omega x its six columns 72) and first moment to first order
+# (moments and t x d 51, gradient 99); per grid point, the 30-coefficient
+# wave sum (180) and the polynomial (60)
+FLOP_EXPAND_MONO = 175.0
 FLOP_EXPAND_DIPOLE = 150.0
 FLOP_EXPAND_EVAL = 240.0
 BF16_PEAK_TFLOPS = 2500.0

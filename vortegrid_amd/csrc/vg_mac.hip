@@ -217,16 +217,27 @@ __device__ __forceinline__ void expansion_add(Expansion &E, float3 cb, float4 p,
         E.c1[1][j] += oe[j].y * ir3 + od.y * f;
         E.c1[2][j] += oe[j].z * ir3 + od.z * f;
     }
-    // T[:, j, k] = -3 ir5 (e_j d_k + e_k d_j + delta_jk d) + 15 ir7 d_j d_k d
-    const int pj[6] = {0, 1, 2, 0, 0, 1}, pk[6] = {0, 1, 2, 1, 2, 2};
+    // c2[:, jk] = Om x T_jk. with T_jkb = d_j d_k K_b = 15 ir7 d_j d_k d_b
+    // - 3 ir5 (delta_jb d_k + delta_kb d_j + delta_jk d_b), fully symmetric:
+    // its 10 distinct components once, then one cross product per jk
+    {
+        const float a = -3.0f * ir5, g = 15.0f * ir7;
+        const float gx = g * d.x, gy = g * d.y;
+        const float xx = gx * d.x, yy = gy * d.y, zz = g * d.z * d.z, xy = gx * d.y;
+        const float xxa = xx + a, yya = yy + a, zza = zz + a;
+        const float txxx = d.x * (xxa + 2.0f * a), tyyy = d.y * (yya + 2.0f * a),
+                    tzzz = d.z * (zza + 2.0f * a);
+        const float txxy = d.y * xxa, txxz = d.z * xxa, tyyx = d.x * yya, tyyz = d.z * yya,
+                    tzzx = d.x * zza, tzzy = d.y * zza, txyz = xy * d.z;
+        const float3 T[6] = {make_float3(txxx, txxy, txxz), make_float3(tyyx, tyyy, tyyz),
+                             make_float3(tzzx, tzzy, tzzz), make_float3(txxy, tyyx, txyz),
+                             make_float3(txxz, txyz, tzzx), make_float3(txyz, tyyz, tzzy)};
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        const int j = pj[q], k = pk[q];
-        const float a = -3.0f * ir5;
-        const float g = 15.0f * ir7 * dv[j] * dv[k] + (j == k ? a : 0.f);
-        E.c2[0][q] += a * (dv[k] * oe[j].x + dv[j] * oe[k].x) + g * od.x;
-        E.c2[1][q] += a * (dv[k] * oe[j].y + dv[j] * oe[k].y) + g * od.y;
-        E.c2[2][q] += a * (dv[k] * oe[j].z + dv[j] * oe[k].z) + g * od.z;
+        for (int q = 0; q < 6; ++q) {
+            E.c2[0][q] = fmaf(om.y, T[q].z, fmaf(-om.z, T[q].y, E.c2[0][q]));
+            E.c2[1][q] = fmaf(om.z, T[q].x, fmaf(-om.x, T[q].z, E.c2[1][q]));
+            E.c2[2][q] = fmaf(om.x, T[q].y, fmaf(-om.y, T[q].x, E.c2[2][q]));
+        }
     }
     if (ORDER >= 1) {
         // D rows (scaled), antisymmetric part a, t_b = D_b . d
