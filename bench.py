@@ -1045,7 +1045,43 @@ def suite(args):
     prof = vg.profile_dump()
     g = state["g"]
     c, ms = prof.get("vg_advect_tracers", (1, 0.0))
+    frames_sim = 3 + steps + psteps
+    # The same frames on the device: a one-effect batch with the reference
+    # opening rule (near_k = -1) defines the grid on the GPU (no readback),
+    # eagerly and as a replayed hipGraph; after as many eager frames as the
+    # vg_sim run above, vortons and tracers must be bit-identical to it.
+    b = vg.Batch([len(pos)], [len(tr)], [radius], cells=0, max_cells=64, near_k=-1,
+                 policy=args.policy)
+    b.upload(pos, om, tr)
+    for _ in range(3):
+        b.step(dt)
+    vg.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.step(dt)
+    vg.synchronize()
+    dev_el = time.perf_counter() - t0
+    for _ in range(psteps):
+        b.step(dt)
+    vg.synchronize()
+    same = (b.clamped() == 0 and np.array_equal(dp.numpy(), b.download("vpos")) and
+            np.array_equal(dtr.numpy(), b.download("tpos")))
+    gs = vg.Stream()
+    b.capture(dt, gs)
+    for _ in range(3):
+        b.launch(gs)
+    gs.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.launch(gs)
+    gs.sync()
+    graph_el = time.perf_counter() - t0
+    device_frame = {"ms_per_step": dev_el * 1e3 / steps, "graph_ms_per_step": graph_el * 1e3 / steps,
+                    "bit_identical_to_vg_sim_after_frames": frames_sim if same else False,
+                    "note": "one-effect vg_batch, near_k = -1 (reference opening rule), DefineShape "
+                            "on the device; graph: the frame recorded once and replayed"}
     recs.append({"workload": "treecode frame, 2 colliding rings N=65536 + 1M tracers, 100 steps",
+                 "device_frame": device_frame,
                  "grid": list(g.shape), "ms_per_step": el * 1e3 / steps,
                  "evals_per_s": g.capacity * steps / el,
                  "tree_grid_avg_ms": prof.get("vg_tree_velocity_grid", (1, 0.0))[1] / psteps,

@@ -498,7 +498,11 @@ typedef struct vg_batch_config {
                         * (uniformGrid.h:259-322, vortonSim.cpp:1089), computed on
                         * the device; the walk then takes (effect, brick) items
                         * from a device work queue, largest effects first */
-    int near_k;        /* near-field treecode radius (default 2) */
+    int near_k;        /* near-field treecode radius (default 2); -1: the reference
+                        * opening rule instead (ComputeVelocityAtGridpoints of the
+                        * TREE technique, vortonSim.cpp:1787-1905, 2223-2293), as
+                        * vg_sim with tree_near_k < 0; order and expand_from are
+                        * then unused */
     int order;         /* cluster order: 0 monopole, 1 dipole (default 1) */
     int rsqrt_policy;  /* VG_RSQRT_* (default ACCURATE) */
     int expand_from;   /* local expansions from this layer (default -1: off) */

@@ -6,10 +6,12 @@ walk takes (effect, brick) items from a device work queue.
 Checked here: the device's grid definition equals the host's (and so the
 oracle's, which tests/test_abi.py pins) bit for bit; every effect's grid equals
 the single-tree walk on that geometry (a one-vorton effect's single-cell
-grid included); the max_cells limit is counted; graph replays equal eager frames; and a
-one-effect batch runs the reference frame of vg_sim (tracers with the
-previous grid, vorton and tracer bounds, UpdateBoundingBox + DefineShape,
-tree, walk, vorton advection) bit for bit with no host round trip."""
+grid included), and so does every grid of the reference opening rule
+(near_k = -1) against the single-tree reference walk; the max_cells limit is
+counted; graph replays equal eager frames; and a one-effect batch runs the
+reference frame of vg_sim (tracers with the previous grid, vorton and tracer
+bounds, UpdateBoundingBox + DefineShape, tree, walk, vorton advection) bit for
+bit with no host round trip, with either walk."""
 import numpy as np
 import pytest
 
@@ -96,6 +98,39 @@ def test_adaptive_shapes_equal_host_define_shape_and_single_trees(vg, expand_fro
     assert b.clamped() == 0
 
 
+def test_adaptive_reference_rule_equals_single_trees(vg):
+    """near_k = -1: every effect's grid by the reference opening rule
+    (k_tree_grid on the batched trees, whose empty leaf cells are then
+    written too) equals the single-tree reference walk on the effect's
+    geometry bit for bit, including an effect without vortons and a
+    one-vorton effect (a one-layer tree: the direct sum over its cell)."""
+    counts = [4096, 700, 0, 3, 2500, 1]
+    case(f"adaptive batch, reference opening rule vs single trees, effects {counts}")
+    b, vp, vw, tp, radii = adaptive_batch(vg, counts, seed=41, near_k=-1)
+    vo = np.cumsum([0] + counts)
+    for frame in range(2):
+        b.step(DT)
+        vg.synchronize()
+        vpos = b.download("vpos")
+        for e, n in enumerate(counts):
+            g, field = b.grid(e)
+            if n == 0:
+                assert not np.any(field)
+                continue
+            spread, circ = b.mollify(e)
+            pv = prev_v[e] if frame else vp[e]
+            dp, dw = vg.DeviceArray.from_numpy(pv), vg.DeviceArray.from_numpy(vw[e])
+            v = vg.vortons_view(dp, dw, n, uniform_size=2 * radii[e])
+            t = vg.Tree()
+            t.build(g, v)
+            ref = vg.DeviceArray.zeros((g.capacity, 3))
+            t.velocity_grid(g, ref, spread, circ, 1)
+            vg.synchronize()
+            assert np.array_equal(field, ref.numpy()), (frame, e, list(g.np))
+        prev_v = [vpos[vo[e]:vo[e + 1]] for e in range(len(counts))]
+    assert b.clamped() == 0
+
+
 def test_adaptive_max_cells_limit(vg):
     """A shape beyond max_cells on an axis is limited there and counted; its
     grid still equals the single-tree walk on the limited geometry."""
@@ -138,16 +173,19 @@ def test_adaptive_graph_replay_matches_eager(vg):
             assert same_desc(ga, gb) and np.array_equal(fa, fb), (f, e)
 
 
-def test_one_effect_batch_is_the_sim_frame(vg):
+@pytest.mark.parametrize("near_k", [2, -1])
+def test_one_effect_batch_is_the_sim_frame(vg, near_k):
     """The reference frame with host geometry (vg_sim: tracers with the
     previous grid, tracer + vorton bounds, UpdateBoundingBox + DefineShape on
-    the host, tree, near-field walk, vortons AssignVelocityFromField +
-    Evolve) equals a one-effect adaptive batch frame, which defines the grid
-    on the device, bit for bit over 3 frames."""
-    case("one-effect adaptive batch vs vg_sim TREE near-field frame, colliding rings 16384, 3 frames")
+    the host, tree, near-field walk or, with near_k = -1, the reference
+    opening rule, vortons AssignVelocityFromField + Evolve) equals a
+    one-effect adaptive batch frame, which defines the grid on the device,
+    bit for bit over 3 frames."""
+    rule = "near-field" if near_k >= 0 else "reference-rule"
+    case(f"one-effect adaptive batch vs vg_sim TREE {rule} frame, colliding rings 16384, 3 frames")
     pos, om, radius = scenarios.colliding_rings(16384, seed=37)
     tr = scenarios.tracers_in_box(65536, pos.min(0), pos.max(0), seed=38)
-    near_k, order, expand_from = 2, 1, 2
+    order, expand_from = 1, 2
     s = vg.Stream()
     d_pos, d_om = vg.DeviceArray.from_numpy(pos), vg.DeviceArray.from_numpy(om)
     d_vel = vg.DeviceArray.zeros(pos.shape)

@@ -485,6 +485,9 @@ vg_status walk(vg_batch *b, int waves, hipStream_t s) {
     q.order = b->qorder;
     q.brick_start = b->brick_start;
     q.counters = b->counters;
+    if (b->cfg.near_k < 0)
+        return tree_velocity_grid_batch(b->views, b->mollify, b->effects, b->shape.layer[0].np,
+                                        b->cfg.rsqrt_policy, b->grid, 3 * b->cap0, s);
     const bool queue = b->adaptive && b->effects > 1;
     return mac_velocity_grid_batch(b->views, b->mollify, b->effects, b->shape.layer[0].np,
                                    b->cfg.near_k, b->cfg.order, b->cfg.expand_from,
@@ -547,6 +550,7 @@ vg_status solve(vg_batch *b, bool tracers_boxed, hipStream_t s) {
         ta.C[l] = b->C[l];
     }
     if (b->adaptive) ta.views = b->views;
+    ta.all_leaves = b->cfg.near_k < 0;
     VG_TRY(batch_tree_build(ta, s));
     return walk(b, b->walk_waves, s);
 }
@@ -601,7 +605,7 @@ VG_API vg_status vg_batch_create(vg_batch **out, uint32_t effects, const uint32_
     VG_REQUIRE(!adaptive || (c.max_cells >= 4 && c.max_cells <= 256 &&
                              (c.max_cells & (c.max_cells - 1)) == 0),
                "max_cells must be a power of two in [4, 256]");
-    VG_REQUIRE(c.near_k >= 0 && c.near_k <= 16, "near_k must be in [0, 16]");
+    VG_REQUIRE(c.near_k >= -1 && c.near_k <= 16, "near_k must be in [-1, 16]");
     VG_REQUIRE(c.order == 0 || c.order == 1, "order must be 0 or 1");
     VG_REQUIRE(c.rsqrt_policy == VG_RSQRT_REFERENCE || c.rsqrt_policy == VG_RSQRT_ACCURATE,
                "bad rsqrt policy");
@@ -762,7 +766,7 @@ VG_API vg_status vg_batch_buffers(vg_batch *b, float **vorton_pos, float **vorto
 
 VG_API vg_status vg_batch_set_method(vg_batch *b, int near_k, int order, int expand_from) {
     VG_REQUIRE(b, "null batch");
-    VG_REQUIRE(near_k >= 0 && near_k <= 16, "near_k must be in [0, 16]");
+    VG_REQUIRE(near_k >= -1 && near_k <= 16, "near_k must be in [-1, 16]");
     VG_REQUIRE(order == 0 || order == 1, "order must be 0 or 1");
     drop_graph(b);
     b->cfg.near_k = near_k;

@@ -257,12 +257,21 @@ struct BatchTreeArgs {
     // non-null: the trees have their own shapes (device views), each within
     // `shape`, whose capacities are the storage strides
     const TreeView *views = nullptr;
+    // write the clusters of empty leaf cells too (readers that scan leaf
+    // clusters by position, such as the reference-rule walk)
+    bool all_leaves = false;
 };
 size_t scan_scratch_elems(size_t n);
 // Exclusive scan in -> out using tmp (scan_scratch_elems(n) elements).
 vg_status exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp,
                              hipStream_t s);
 vg_status batch_tree_build(const BatchTreeArgs &a, hipStream_t s);
+// Reference-rule velocity grids of batched trees (device views, each on its
+// own leaf grid within np; mollify[tree] = (spread, circ)), tree k's grid at
+// out + k * out_stride.
+vg_status tree_velocity_grid_batch(const TreeView *views, const float2 *mollify, uint32_t trees,
+                                   const uint32_t np[3], int rsqrt_policy, float *out,
+                                   size_t out_stride, hipStream_t s);
 // Work queue of a batched walk over trees of different shapes (see
 // k_mac_grid_queue): trees in queue order, the first (tree, brick) item of
 // each (trees + 1 entries), two counters (walk, expansions).

@@ -472,23 +472,30 @@ __device__ float3 tree_eval(const TreeView &tv, float3 q, float spread, uint32_t
     return sum;
 }
 
-// Direct fallback when the tree has a single layer (vortonSim.cpp:2276-2279).
+// Direct fallback when the tree has a single layer (vortonSim.cpp:2276-2279),
+// over the cell-sorted vortons [begin, end).
 template <int POLICY, int VECPOT>
-__device__ float3 sorted_direct(const TreeView &tv, float3 q, float spread) {
+__device__ float3 sorted_direct_range(const TreeView &tv, uint32_t begin, uint32_t end, float3 q,
+                                      float spread) {
     float3 sum = make_float3(0.f, 0.f, 0.f), comp = sum;
     float3 part = sum;
-    for (uint32_t j = 0; j < tv.n; ++j) {
+    for (uint32_t j = begin; j < end; ++j) {
         float4 A, B;
         prepare_source(tv.sorted_p[j], tv.sorted_w[j], spread, A, B);
         if (VECPOT) pair_vecpot<POLICY>(part, q, A, B);
         else pair_velocity<POLICY>(part, q, A, B);
-        if ((j & 255u) == 255u) {
+        if (((j - begin) & 255u) == 255u) {
             kahan_add(sum, comp, part);
             part = make_float3(0.f, 0.f, 0.f);
         }
     }
     kahan_add(sum, comp, part);
     return sum;
+}
+
+template <int POLICY, int VECPOT>
+__device__ float3 sorted_direct(const TreeView &tv, float3 q, float spread) {
+    return sorted_direct_range<POLICY, VECPOT>(tv, 0u, tv.n, q, spread);
 }
 
 template <int POLICY, int VECPOT, int SPLIT>
@@ -721,12 +728,14 @@ constexpr int kTreeGridSplit = 2;
 // Face points: 4 lanes each (0.0195 ms for the faces of a 65^3 grid, against
 // 0.023 ms with 2 lanes and 0.032 ms with 1).
 constexpr int kTreeFaceSplit = 4;
+// Brick `brick` of grid g (bricks nbx x nby x ..., x fastest) from tree tv.
+// own_cells: a single-layer tree sums the vortons of its one leaf cell
+// (trees of a batch share the sorted arrays) instead of all tv.n.
 template <int POLICY>
-__global__ __launch_bounds__(64 * kTreeGridSplit) void k_tree_grid(const TreeView tv, vg_grid_desc g, uint32_t nbx,
-                                                   uint32_t nby, uint32_t nbricks, float spread,
-                                                   float circ, float *__restrict__ out) {
-    const uint32_t brick = blockIdx.x;
-    if (brick >= nbricks) return;
+__device__ __forceinline__ void tree_grid_brick(const TreeView &tv, const vg_grid_desc &g,
+                                                uint32_t nbx, uint32_t nby, uint32_t brick,
+                                                float spread, float circ, bool own_cells,
+                                                float *__restrict__ out) {
     const uint32_t sub = threadIdx.x % kTreeGridSplit, pt = threadIdx.x / kTreeGridSplit;
     uint32_t ix = 4u * (brick % nbx) + (pt & 3u);
     uint32_t iy = 4u * ((brick / nbx) % nby) + ((pt >> 2) & 3u);
@@ -739,12 +748,44 @@ __global__ __launch_bounds__(64 * kTreeGridSplit) void k_tree_grid(const TreeVie
     iz = iz < g.np[2] ? iz : g.np[2] - 1;
     const size_t i = point_offset(g, ix, iy, iz);
     const float3 q = grid_query_point(g, ix, iy, iz);
-    const float3 r = tv.depth > 1 ? tree_eval_split<POLICY, 0, kTreeGridSplit>(tv, q, spread, sub)
-                                  : sorted_direct<POLICY, 0>(tv, q, spread);
+    float3 r;
+    if (tv.depth > 1)
+        r = tree_eval_split<POLICY, 0, kTreeGridSplit>(tv, q, spread, sub);
+    else if (own_cells)
+        r = sorted_direct_range<POLICY, 0>(tv, tv.cell_start[0],
+                                           tv.cell_start[0] + tv.cell_count[0], q, spread);
+    else
+        r = sorted_direct<POLICY, 0>(tv, q, spread);
     if (!valid || sub != 0) return;
     out[3 * i] = r.x * circ;
     out[3 * i + 1] = r.y * circ;
     out[3 * i + 2] = r.z * circ;
+}
+
+template <int POLICY>
+__global__ __launch_bounds__(64 * kTreeGridSplit) void k_tree_grid(const TreeView tv, vg_grid_desc g,
+                                                                   uint32_t nbx, uint32_t nby,
+                                                                   uint32_t nbricks, float spread,
+                                                                   float circ,
+                                                                   float *__restrict__ out) {
+    const uint32_t brick = blockIdx.x;
+    if (brick >= nbricks) return;
+    tree_grid_brick<POLICY>(tv, g, nbx, nby, brick, spread, circ, false, out);
+}
+
+// Trees of a batch (blockIdx.y = tree), each on its own leaf grid with its
+// own mollification; workgroups past a tree's brick count leave at once.
+template <int POLICY>
+__global__ __launch_bounds__(64 * kTreeGridSplit) void k_tree_grid_views(
+    const TreeView *__restrict__ views, const float2 *__restrict__ mollify,
+    float *__restrict__ out, size_t out_stride) {
+    const uint32_t tree = blockIdx.y, brick = blockIdx.x;
+    const TreeView &tv = views[tree];
+    const vg_grid_desc &g = tv.layer[0];
+    const uint32_t nbx = (g.np[0] + 3) / 4, nby = (g.np[1] + 3) / 4, nbz = (g.np[2] + 3) / 4;
+    if (brick >= nbx * nby * nbz) return;
+    const float2 sc = mollify[tree];
+    tree_grid_brick<POLICY>(tv, g, nbx, nby, brick, sc.x, sc.y, true, out + tree * out_stride);
 }
 
 __global__ __launch_bounds__(256) void k_tree_count(const TreeView tv, vg_grid_desc g,
@@ -1012,19 +1053,20 @@ vg_status batch_tree_build(const BatchTreeArgs &a, hipStream_t s) {
     VG_TRY(launch("vg_batch_gather", k_gather_batch, dim3(ceil_div(n, 256)), dim3(256), 0, s, a.pos,
                   a.angvel, a.tree_of, a.size, (const uint32_t *)a.sorted_idx, n, a.sorted_p,
                   a.sorted_w));
-    // The batched walk reaches leaf clusters only through the layer-1 child
-    // masks and the cell lists, so empty leaf cells are neither written nor,
-    // in layer 1's aggregation, read.
+    // The near-field walk reaches leaf clusters only through the layer-1
+    // child masks and the cell lists, so empty leaf cells are neither
+    // written nor, in layer 1's aggregation, read; the reference-rule walk
+    // reads leaf clusters by position on layers it enumerates (all_leaves).
     if (a.views)
         VG_TRY(launch("vg_batch_leaf", k_leaf, dim3(ceil_div(cap0, 256), a.trees), dim3(256), 0, s,
                       (const uint32_t *)a.cell_start, (const uint32_t *)a.cell_count,
                       (const float4 *)a.sorted_p, (const float4 *)a.sorted_w, cap0, a.P[0],
-                      a.W[0], a.M[0], a.views, 1));
+                      a.W[0], a.M[0], a.views, a.all_leaves ? 0 : 1));
     else
         VG_TRY(launch("vg_batch_leaf", k_leaf, dim3(ceil_div(cells, 256)), dim3(256), 0, s,
                       (const uint32_t *)a.cell_start, (const uint32_t *)a.cell_count,
                       (const float4 *)a.sorted_p, (const float4 *)a.sorted_w, cells, a.P[0],
-                      a.W[0], a.M[0], (const TreeView *)nullptr, 1));
+                      a.W[0], a.M[0], (const TreeView *)nullptr, a.all_leaves ? 0 : 1));
     for (uint32_t l = 1; l < ng.depth; ++l) {
         const size_t cap = capacity(ng.layer[l]);
         const uint32_t *cc = l == 1 ? (const uint32_t *)a.cell_count : nullptr;
@@ -1078,6 +1120,18 @@ vg_status tree_checked_view(const vg_tree *t, TreeView *v) {
     }
     *v = make_view(t);
     return VG_OK;
+}
+
+vg_status tree_velocity_grid_batch(const TreeView *views, const float2 *mollify, uint32_t trees,
+                                   const uint32_t np[3], int rsqrt_policy, float *out,
+                                   size_t out_stride, hipStream_t s) {
+    VG_REQUIRE(rsqrt_policy == VG_RSQRT_REFERENCE || rsqrt_policy == VG_RSQRT_ACCURATE,
+               "bad rsqrt policy");
+    auto k = rsqrt_policy == VG_RSQRT_REFERENCE ? k_tree_grid_views<VG_RSQRT_REFERENCE>
+                                                : k_tree_grid_views<VG_RSQRT_ACCURATE>;
+    const uint32_t nbricks = ((np[0] + 3) / 4) * ((np[1] + 3) / 4) * ((np[2] + 3) / 4);
+    return launch("vg_batch_velocity_grid", k, dim3(nbricks, trees), dim3(64 * kTreeGridSplit), 0,
+                  s, views, mollify, out, out_stride);
 }
 
 }  // namespace vg
