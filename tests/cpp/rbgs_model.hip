@@ -61,7 +61,9 @@ int main() {
             const int gx = (nx - 2 + kSX - 1) / kSX, gy = (ny - 2 + kSY - 1) / kSY, gz = (nz - 2 + zc - 1) / zc;
             float *ring[kSlots];
             for (int k = 0; k < kSlots; ++k) ring[k] = (float *)malloc(kSlotF * 4);
-            struct Th { int off[kStageN]; float sv[kStageN], fr[3 * kRedN], fr2[3 * kRedN], fb[3], fb2[3]; };
+            // two register sets alternate between plane steps, as in the kernel:
+            // set s = (z - z0) & 1 serves step z and is refilled for step z + 2
+            struct Th { int off[kStageN]; float sv[2][kStageN], fr[2][3 * kRedN], fb[2][3]; };
             std::vector<Th> th(kThreads);
             for (int bz = 0; bz < gz; ++bz) for (int by = 0; by < gy; ++by) for (int bx = 0; bx < gx; ++bx) {
                 const Tile t = at(base, bx, by);
@@ -70,35 +72,37 @@ int main() {
                 for (int k = 0; k < kSlots; ++k) for (int e = 0; e < kSlotF; ++e) ring[k][e] = 7e30f;
                 each([&](int tid, Th &r) {
                     stage_map(t, tid, r.off);
-                    for (int q = z0 - 2; q <= z0 + 1; ++q) { stage_load(t, ui, r.off, q, r.sv); stage_store(r.sv, ring[slot(q)], tid); }
-                    red_load(t, f, z0 - 1, tid, r.fr);
-                    red_load(t, f, z0, tid, r.fr2);
+                    for (int q = z0 - 2; q <= z0 + 1; ++q) { stage_load(t, ui, r.off, q, r.sv[0]); stage_store(r.sv[0], ring[slot(q)], tid); }
+                    red_load(t, f, z0 - 1, tid, r.fr[0]);
+                    red_load(t, f, z0, tid, r.fr[1]);
                 });
                 // barrier
                 if (rev) {  // the store of plane z0+2 racing ahead of the colour-0 passes
                     each([&](int tid, Th &r) { float v[kStageN]; stage_load(t, ui, r.off, z0 + 2, v); stage_store(v, ring[slot(z0 + 2)], tid); });
                 }
                 each([&](int tid, Th &r) {
-                    red(t, r.fr, ring[slot(z0 - 2)], ring[slot(z0 - 1)], ring[slot(z0)], z0 - 1, tid);
-                    red(t, r.fr2, ring[slot(z0 - 1)], ring[slot(z0)], ring[slot(z0 + 1)], z0, tid);
-                    stage_load(t, ui, r.off, z0 + 2, r.sv);
-                    red_load(t, f, z0 + 1, tid, r.fr);
-                    black_load(t, f, z0, tid, r.fb);
+                    red(t, r.fr[0], ring[slot(z0 - 2)], ring[slot(z0 - 1)], ring[slot(z0)], z0 - 1, tid);
+                    red(t, r.fr[1], ring[slot(z0 - 1)], ring[slot(z0)], ring[slot(z0 + 1)], z0, tid);
+                    stage_load(t, ui, r.off, z0 + 2, r.sv[0]);
+                    red_load(t, f, z0 + 1, tid, r.fr[0]);
+                    black_load(t, f, z0, tid, r.fb[0]);
+                    stage_load(t, ui, r.off, z0 + 3, r.sv[1]);
+                    red_load(t, f, z0 + 2, tid, r.fr[1]);
+                    black_load(t, f, z0 + 1, tid, r.fb[1]);
                 });
                 for (int z = z0; z < z1; ++z) {
+                    const int st = (z - z0) & 1;
                     each([&](int tid, Th &r) {
-                        if (!rev || z == z0) stage_store(r.sv, ring[slot(z + 2)], tid);
-                        stage_load(t, ui, r.off, z + 3, r.sv);
-                        red_load(t, f, z + 2, tid, r.fr2);
-                        black_load(t, f, z + 1, tid, r.fb2);
+                        if (!rev || z == z0) stage_store(r.sv[st], ring[slot(z + 2)], tid);
+                        stage_load(t, ui, r.off, z + 4, r.sv[st]);
                     });
-                    each([&](int tid, Th &r) { red(t, r.fr, ring[slot(z)], ring[slot(z + 1)], ring[slot(z + 2)], z + 1, tid); });
+                    each([&](int tid, Th &r) { red(t, r.fr[st], ring[slot(z)], ring[slot(z + 1)], ring[slot(z + 2)], z + 1, tid); });
                     if (rev && z + 1 < z1)  // the next plane's store racing ahead of this colour-1 pass
-                        each([&](int tid, Th &r) { stage_store(r.sv, ring[slot(z + 3)], tid); });
+                        each([&](int tid, Th &r) { stage_store(r.sv[st ^ 1], ring[slot(z + 3)], tid); });
                     each([&](int tid, Th &r) {
-                        black(t, r.fb, uo, ring[slot(z - 1)], ring[slot(z)], ring[slot(z + 1)], z, tid);
-                        for (int j = 0; j < 3 * kRedN; ++j) r.fr[j] = r.fr2[j];
-                        for (int c = 0; c < 3; ++c) r.fb[c] = r.fb2[c];
+                        black(t, r.fb[st], uo, ring[slot(z - 1)], ring[slot(z)], ring[slot(z + 1)], z, tid);
+                        red_load(t, f, z + 3, tid, r.fr[st]);
+                        black_load(t, f, z + 2, tid, r.fb[st]);
                     });
                 }
             }

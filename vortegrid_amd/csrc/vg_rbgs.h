@@ -16,9 +16,10 @@
 // another's output, and one launch does what two colour launches do with
 // half the traffic: ui, f read once and uo written once per point.
 //
-// Global loads run one plane ahead in registers (the staged plane and the f
-// values both passes of the next plane need), so each plane costs one
-// memory round trip, overlapped with the current plane's arithmetic.
+// Global loads run two planes ahead in registers (the staged plane and the f
+// values both passes of a plane need), in two sets that alternate between
+// planes: a workgroup's march is a chain of dependent plane steps, and with
+// two workgroups per CU a one-plane lead left each step waiting on memory.
 //
 // The phases are __host__ __device__ functions of the thread index so the
 // same code runs in a serial host model of the workgroup.
@@ -175,12 +176,31 @@ __host__ __device__ inline Tile at(Tile t, int bx, int by) {
     return t;
 }
 
+// One plane step of the march: store the prefetched plane z+2 into its
+// ring slot (that of z-3, which no pass still in flight reads: the colour-1
+// pass of z-1 reads z-2 .. z), barrier, colour 0 of plane z+1 (reads
+// z .. z+2), barrier, colour 1 of plane z (reads z-1 .. z+1).  The register
+// set it uses is refilled for the step two planes on: plane z+4 to stage,
+// f of plane z+3 for its colour-0 pass and of plane z+2 for its colour-1
+// pass, so every load has two steps to arrive.
+__device__ __forceinline__ void march_step(const Tile &t, const float *__restrict__ ui,
+                                           float *__restrict__ uo, const float *__restrict__ f,
+                                           float (*ring)[kSlotF], const int (&off)[kStageN], int z,
+                                           int tid, float (&sv)[kStageN], float (&fr)[3 * kRedN],
+                                           float (&fb)[3]) {
+    stage_store(sv, ring[slot(z + 2)], tid);
+    stage_load(t, ui, off, z + 4, sv);
+    __syncthreads();
+    red(t, fr, ring[slot(z)], ring[slot(z + 1)], ring[slot(z + 2)], z + 1, tid);
+    __syncthreads();
+    black(t, fb, uo, ring[slot(z - 1)], ring[slot(z)], ring[slot(z + 1)], z, tid);
+    red_load(t, f, z + 3, tid, fr);
+    black_load(t, f, z + 2, tid, fb);
+}
+
 // The march of one workgroup over planes z0 .. z1-1 (z1 exclusive); ring
-// slot of plane q: slot(q).  Per plane z: store the prefetched plane z+2
-// (its slot, that of z-3, is read by no pass still in flight: the colour-1
-// pass of z-1 reads z-2 .. z), issue the loads of the next plane, barrier,
-// colour 0 of plane z+1 (reads z .. z+2), barrier, colour 1 of plane z
-// (reads z-1 .. z+1).
+// slot of plane q: slot(q).  Two register sets alternate between the steps
+// (the loop is unrolled by two so that each step names its set).
 __global__ __launch_bounds__(kThreads) void k_rbgs_sweep(Tile base, const float *__restrict__ ui,
                                                          float *__restrict__ uo,
                                                          const float *__restrict__ f, int zc) {
@@ -190,31 +210,27 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_sweep(Tile base, const float 
     const int z0 = 1 + (int)blockIdx.z * zc;
     const int z1 = min(z0 + zc, t.nz - 1);
     int off[kStageN];
-    float sv[kStageN], fr[3 * kRedN], fr2[3 * kRedN], fb[3], fb2[3];
+    float sv0[kStageN], sv1[kStageN], fr0[3 * kRedN], fr1[3 * kRedN], fb0[3], fb1[3];
     stage_map(t, tid, off);
     for (int q = z0 - 2; q <= z0 + 1; ++q) {
-        stage_load(t, ui, off, q, sv);
-        stage_store(sv, ring[slot(q)], tid);
+        stage_load(t, ui, off, q, sv0);
+        stage_store(sv0, ring[slot(q)], tid);
     }
-    red_load(t, f, z0 - 1, tid, fr);
-    red_load(t, f, z0, tid, fr2);
+    red_load(t, f, z0 - 1, tid, fr0);
+    red_load(t, f, z0, tid, fr1);
     __syncthreads();
-    red(t, fr, ring[slot(z0 - 2)], ring[slot(z0 - 1)], ring[slot(z0)], z0 - 1, tid);
-    red(t, fr2, ring[slot(z0 - 1)], ring[slot(z0)], ring[slot(z0 + 1)], z0, tid);
-    stage_load(t, ui, off, z0 + 2, sv);
-    red_load(t, f, z0 + 1, tid, fr);
-    black_load(t, f, z0, tid, fb);
-    for (int z = z0; z < z1; ++z) {
-        stage_store(sv, ring[slot(z + 2)], tid);
-        stage_load(t, ui, off, z + 3, sv);
-        red_load(t, f, z + 2, tid, fr2);
-        black_load(t, f, z + 1, tid, fb2);
-        __syncthreads();
-        red(t, fr, ring[slot(z)], ring[slot(z + 1)], ring[slot(z + 2)], z + 1, tid);
-        __syncthreads();
-        black(t, fb, uo, ring[slot(z - 1)], ring[slot(z)], ring[slot(z + 1)], z, tid);
-        for (int j = 0; j < 3 * kRedN; ++j) fr[j] = fr2[j];
-        for (int c = 0; c < 3; ++c) fb[c] = fb2[c];
+    red(t, fr0, ring[slot(z0 - 2)], ring[slot(z0 - 1)], ring[slot(z0)], z0 - 1, tid);
+    red(t, fr1, ring[slot(z0 - 1)], ring[slot(z0)], ring[slot(z0 + 1)], z0, tid);
+    // set 0 serves steps z0, z0 + 2, ...; set 1 steps z0 + 1, z0 + 3, ...
+    stage_load(t, ui, off, z0 + 2, sv0);
+    red_load(t, f, z0 + 1, tid, fr0);
+    black_load(t, f, z0, tid, fb0);
+    stage_load(t, ui, off, z0 + 3, sv1);
+    red_load(t, f, z0 + 2, tid, fr1);
+    black_load(t, f, z0 + 1, tid, fb1);
+    for (int z = z0; z < z1; z += 2) {
+        march_step(t, ui, uo, f, ring, off, z, tid, sv0, fr0, fb0);
+        if (z + 1 < z1) march_step(t, ui, uo, f, ring, off, z + 1, tid, sv1, fr1, fb1);
     }
 }
 
