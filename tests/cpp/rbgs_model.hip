@@ -40,7 +40,7 @@ static void plain(int nx, int ny, int nz, std::vector<float> &u, const std::vect
 int main() {
     const int sizes[][3] = {{5, 5, 5}, {9, 9, 17}, {17, 17, 33}, {33, 33, 65}, {67, 19, 23},
                             {66, 10, 12}, {3, 3, 3}, {130, 11, 7}, {65, 65, 129}};
-    const int zcs[] = {1, 2, 3, 8, 16};
+    const int zcs[] = {1, 2, 3, 8, 11, 16};
     int bad = 0;
     for (auto &sz3 : sizes) {
         const int nx = sz3[0], ny = sz3[1], nz = sz3[2];

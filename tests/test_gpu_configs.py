@@ -95,7 +95,9 @@ def test_plume_vcycles_to_1e6(vg, oracle, plume):
     """Splat (against the oracle), treecode faces, then V-cycles to 1e-6;
     the oracle recomputes the FP64 true relative residual ||f - L u|| / ||f||
     of the delivered solution u = a0 + lo on the CPU.  The finest level
-    (129 x 129 x 257) runs the fused LDS sweep at its full 16-plane depth."""
+    (129 x 129 x 257) runs the fused LDS sweep at an 11-plane depth (768
+    workgroups, each marching its planes with two register sets of loads in
+    flight)."""
     pos, om, radius, g, og, t, ot, keep = plume
     case(f"plume N=262144 grid {g.shape}: splat, faces, V-cycles to 1e-6")
     negw = vg.DeviceArray.zeros((g.capacity, 3))

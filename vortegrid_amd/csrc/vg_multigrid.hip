@@ -360,13 +360,22 @@ size_t sweep_blocks(const vg_grid_desc &g, int zc) {
            ceil_div(g.np[2] - 2, zc);
 }
 
-// Tile depth of the fused sweep: 16 planes, halved while fewer than 512
-// workgroups would result (measured on MI355X: 129x129x257 sweeps in 58 us
-// at 16 planes and 512 workgroups, 70 us at 8 planes; 65x65x129 in 14 us at
+// Workgroups of the fused sweep resident at once: three per CU (48,960 B of
+// LDS each) on the 256 CUs of an MI355X.
+constexpr size_t kSweepResident = 3 * 256;
+
+// Tile depth of the fused sweep.  A workgroup marches its planes for the
+// whole launch, so the deepest tile that still gives every resident slot a
+// workgroup is best (129x129x257: 11 planes, 768 workgroups, 49 us; 16
+// planes and 512 workgroups 54 us), down to 8 planes (a tile stages four
+// halo planes besides its own).  Grids too small for that: 16 planes,
+// halved while fewer than 512 workgroups would result (65x65x129 in 15 us at
 // 2 planes).  0 = one launch per colour is faster (grids too small to fill
 // the chip with tiles: 33x33x65 sweeps in 6.8 us that way, 12 us fused).
 int sweep_depth(const vg_grid_desc &g) {
     if (g.np[0] < 33 || sweep_blocks(g, 2) < 256) return 0;
+    for (int zc = 16; zc >= 8; --zc)
+        if (sweep_blocks(g, zc) >= kSweepResident) return zc;
     int zc = 16;
     while (zc > 2 && sweep_blocks(g, zc) < 512) zc /= 2;
     return zc;
