@@ -44,7 +44,8 @@ struct vg_multigrid {
     float *lo = nullptr;       // compensation term of the finest solution
     size_t lo_cap = 0;
     size_t lo_valid = 0;       // points of the last solve's finest grid
-    double *part = nullptr;    // FP64 norm partials
+    double *part = nullptr;    // FP64 norm partials, one pair per workgroup, then the sums
+    size_t part_cap = 0;
 };
 
 namespace vg {
@@ -268,7 +269,9 @@ __global__ __launch_bounds__(256) void k_coarse_f64(vg_grid_desc g, float *__res
 }
 
 // FP64 residual of the compensated solution (hi + lo): writes the FP32
-// residual for the next correction and block partials of ||r||^2, ||f||^2.
+// residual for the next correction and per-workgroup partials of ||r||^2,
+// ||f||^2.  One point per thread: a grid-stride loop left each wave walking
+// eight points one after another, each waiting on its fourteen stencil loads.
 __global__ __launch_bounds__(256) void k_residual_pair(vg_grid_desc g, const float *__restrict__ hi,
                                                        const float *__restrict__ lo,
                                                        const float *__restrict__ f,
@@ -279,11 +282,11 @@ __global__ __launch_bounds__(256) void k_residual_pair(vg_grid_desc g, const flo
     const size_t nx = g.np[0], nxy = nx * g.np[1];
     const size_t total = capacity(g);
     double num = 0.0, den = 0.0;
-    for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < total;
-         o += (size_t)gridDim.x * blockDim.x) {
+    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o < total) {
         if (on_boundary(split((uint32_t)o, g.np[0], g.np[1]), g)) {
             r32[3 * o] = r32[3 * o + 1] = r32[3 * o + 2] = 0.f;
-            continue;
+            goto reduce;
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -300,6 +303,7 @@ __global__ __launch_bounds__(256) void k_residual_pair(vg_grid_desc g, const flo
             den += fc * fc;
         }
     }
+reduce:
     sn[threadIdx.x] = num;
     sd[threadIdx.x] = den;
     __syncthreads();
@@ -332,25 +336,59 @@ __global__ void k_twosum_add(float *__restrict__ hi, float *__restrict__ lo,
     e[i] = 0.f;
 }
 
-// Workgroups of the FP64 residual: enough to keep every CU's loads in
-// flight (512 left it latency-bound at 110 us on a 129x129x257 grid).
-constexpr uint32_t kNormBlocks = 2048;
+// The workgroups' partial pairs summed in a fixed order by one workgroup of
+// 1024 threads: thread t takes pairs t + 1024 j into accumulator j % 4 (four
+// loads in flight), adds the four in order, then a fixed tree; the sums go
+// to part[2 * nb], part[2 * nb + 1].
+constexpr int kFinishThreads = 1024;
+__global__ __launch_bounds__(kFinishThreads) void k_norm_finish(double *__restrict__ part,
+                                                                uint32_t nb) {
+    __shared__ double sn[kFinishThreads], sd[kFinishThreads];
+    double n4[4] = {0.0, 0.0, 0.0, 0.0}, d4[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t b = threadIdx.x; b < nb; b += 4 * kFinishThreads) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t i = b + (uint32_t)j * kFinishThreads;
+            if (i < nb) {
+                n4[j] += part[2 * i];
+                d4[j] += part[2 * i + 1];
+            }
+        }
+    }
+    sn[threadIdx.x] = (n4[0] + n4[1]) + (n4[2] + n4[3]);
+    sd[threadIdx.x] = (d4[0] + d4[1]) + (d4[2] + d4[3]);
+    __syncthreads();
+    for (int w = kFinishThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sn[threadIdx.x] += sn[threadIdx.x + w];
+            sd[threadIdx.x] += sd[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * nb] = sn[0];
+        part[2 * nb + 1] = sd[0];
+    }
+}
+
+// Pairs of doubles m->part must hold for grid g: one per workgroup of the
+// residual, plus the sums.
+size_t norm_part_pairs(const vg_grid_desc &g) { return ceil_div(capacity(g), 256) + 1; }
 
 vg_status fp64_residual(vg_multigrid *m, const vg_grid_desc &g, const float *hi, const float *lo,
                         const float *f, float *r32, double *rel, hipStream_t s) {
     const double rx = 1.0 / (double)g.cell[0], ry = 1.0 / (double)g.cell[1],
                  rz = 1.0 / (double)g.cell[2];
-    VG_TRY(launch("vg_mg_residual_f64", k_residual_pair, dim3(kNormBlocks), dim3(256), 0, s, g, hi,
-                  lo, f, r32, rx * rx, ry * ry, rz * rz, m->part));
-    std::vector<double> h(2 * kNormBlocks);
-    VG_HIP(hipMemcpyAsync(h.data(), m->part, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    const uint32_t nb = ceil_div(capacity(g), 256);
+    VG_REQUIRE(m->part && m->part_cap >= 2 * norm_part_pairs(g), "norm partials not allocated");
+    VG_TRY(launch("vg_mg_residual_f64", k_residual_pair, dim3(nb), dim3(256), 0, s, g, hi, lo, f,
+                  r32, rx * rx, ry * ry, rz * rz, m->part));
+    VG_TRY(launch("vg_mg_residual_f64", k_norm_finish, dim3(1), dim3(kFinishThreads), 0, s, m->part,
+                  nb));
+    double h[2] = {0.0, 0.0};
+    VG_HIP(hipMemcpyAsync(h, m->part + 2 * (size_t)nb, sizeof h, hipMemcpyDeviceToHost, s));
     VG_HIP(hipStreamSynchronize(s));
-    double num = 0.0, den = 0.0;
-    for (uint32_t b = 0; b < kNormBlocks; ++b) {
-        num += h[2 * b];
-        den += h[2 * b + 1];
-    }
-    *rel = den > 0.0 ? sqrt(num / den) : sqrt(num);
+    *rel = h[1] > 0.0 ? sqrt(h[0] / h[1]) : sqrt(h[0]);
     return VG_OK;
 }
 
@@ -584,7 +622,7 @@ VG_API vg_status vg_mg_vcycle_solve(vg_multigrid *m, const vg_grid_desc *g, floa
         VG_TRY(launch("vg_mg_zero", k_zero_all, dim3(ceil_div(c, 256)), dim3(256), 0, s, m->r[l], c));
     }
     VG_TRY(grow(&m->lo, &m->lo_cap, 3 * cap0));
-    if (!m->part) VG_HIP(hipMalloc((void **)&m->part, 2 * kNormBlocks * sizeof(double)));
+    VG_TRY(grow(&m->part, &m->part_cap, 2 * norm_part_pairs(*g)));
     VG_TRY(launch("vg_mg_zero", k_zero_all, dim3(ceil_div(3 * cap0, 256)), dim3(256), 0, s, m->lo,
                   3 * cap0));
     double rel = 0.0;
