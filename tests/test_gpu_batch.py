@@ -47,11 +47,15 @@ def new_batch(vg, counts, seed=1, **kw):
 COUNTS = [700, 2500, 0, 1800, 300, 4096]
 
 
-@pytest.mark.parametrize("expand_from", [-1, 2])
-def test_batch_frame_matches_single_trees(vg, expand_from):
-    case(f"batched frame vs single trees, effects {COUNTS}, cells 16, k=1 dipole, "
-         f"expand_from={expand_from}")
-    b, vp, vw, tp, radii = new_batch(vg, COUNTS, cells=16, near_k=1, order=1,
+@pytest.mark.parametrize("near_k,expand_from", [(1, -1), (1, 2), (-1, -1)])
+def test_batch_frame_matches_single_trees(vg, near_k, expand_from):
+    """Every effect's grid (the near-field walk, or with near_k = -1 the
+    reference opening rule), the vortons' velocities and positions after the
+    frame and the tracers moved with the zero initial grids equal the same
+    steps on each effect alone, bit for bit."""
+    method = "reference rule" if near_k < 0 else f"k={near_k} dipole, expand_from={expand_from}"
+    case(f"batched frame vs single trees, effects {COUNTS}, cells 16, {method}")
+    b, vp, vw, tp, radii = new_batch(vg, COUNTS, cells=16, near_k=near_k, order=1,
                                      expand_from=expand_from)
     b.step(DT)
     vg.synchronize()
@@ -78,7 +82,11 @@ def test_batch_frame_matches_single_trees(vg, expand_from):
             t = vg.Tree()
             t.build(g, v)
             out = vg.DeviceArray.zeros((g.capacity, 3))
-            t.velocity_grid_mac(g, out, 1, 1, spread, circ, policy=1, expand_from=expand_from)
+            if near_k < 0:
+                t.velocity_grid(g, out, spread, circ, 1)
+            else:
+                t.velocity_grid_mac(g, out, near_k, 1, spread, circ, policy=1,
+                                    expand_from=expand_from)
             vg.synchronize()
             assert np.array_equal(out.numpy(), field), e
             # vortons: AssignVelocityFromField + Evolve with that grid
