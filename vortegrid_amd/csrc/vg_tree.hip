@@ -472,7 +472,7 @@ __device__ float3 tree_eval(const TreeView &tv, float3 q, float spread, uint32_t
     return sum;
 }
 
-// Direct fallback when the tree has a single layer (vortonSim.cpp:2276-2279),
+// Direct fallback when the tree has a single layer (vortonSim.cpp:2265-2277),
 // over the cell-sorted vortons [begin, end).
 template <int POLICY, int VECPOT>
 __device__ float3 sorted_direct_range(const TreeView &tv, uint32_t begin, uint32_t end, float3 q,
@@ -520,7 +520,8 @@ __global__ __launch_bounds__(256) void k_tree_points(const TreeView tv, QuerySrc
     out[3 * i + 2] = r.z * circ;
 }
 
-// One cluster applied to the query as a supervorton (vortonSim.cpp:1875-1890).
+// One cluster applied to the query as a supervorton (vortonSim.cpp:1888-1892,
+// VORTON_ACCUMULATE_VELOCITY on a child of the influence tree).
 template <int POLICY, int VECPOT>
 __device__ __forceinline__ void apply_cluster(float3 &part, float3 q, float4 p, float4 w,
                                               float spread) {
