@@ -264,6 +264,43 @@ __global__ void k_leaf(const uint32_t *__restrict__ start, const uint32_t *__res
     M[3 * c + 2] = make_float4(D[6], D[7], D[8], D[1] - D[3]);
 }
 
+// Running sums of one parent cluster.  Both steps are shared by the general
+// child loop and the 2x2x2 form below, so the two give identical bits.
+struct ParentSums {
+    float px = 0.f, py = 0.f, pz = 0.f, wx = 0.f, wy = 0.f, wz = 0.f, sz = 0.f;
+    float aux = FLT_MIN;
+    uint32_t cmask = 0;
+    float D[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+};
+
+// Child with position p and vorticity w adds to the magnitude-weighted
+// centroid, the summed vorticity and the child mask (bit = child number).
+__device__ __forceinline__ void parent_add_child(ParentSums &a, float4 p, float4 w, uint32_t bit) {
+#pragma clang fp contract(off)
+    const float vx = w.x * 2.0f, vy = w.y * 2.0f, vz = w.z * 2.0f;
+    const float mag = sqrtf(vx * vx + vy * vy + vz * vz);
+    a.px += p.x * mag; a.py += p.y * mag; a.pz += p.z * mag;
+    a.wx += w.x; a.wy += w.y; a.wz += w.z;
+    a.aux += mag;
+    if (p.w != 0.0f) {
+        a.sz = p.w;
+        a.cmask |= 1u << bit;
+    }
+}
+
+// First moment about the parent centroid cp: D_p += D_c + Omega_c (c_c - c_p).
+__device__ __forceinline__ void parent_add_moment(ParentSums &a, float4 p, float4 w, float4 m0,
+                                                  float4 m1, float4 m2, const float cp[3]) {
+#pragma clang fp contract(off)
+    const float cd[9] = {m0.x, m0.y, m0.z, m1.x, m1.y, m1.z, m2.x, m2.y, m2.z};
+    const float cw[3] = {w.x, w.y, w.z}, cc[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+        for (int a3 = 0; a3 < 3; ++a3)
+            a.D[3 * bb + a3] += cd[3 * bb + a3] + cw[bb] * (cc[a3] - cp[a3]);
+}
+
 // AggregateClusters (vortonSim.cpp:1337-1569): parent point o of layer gp
 // from its dx x dy x dz children in layer gc.
 // cc (may be null): vorton counts of the child cells; children with none are
@@ -282,52 +319,65 @@ __device__ __forceinline__ void aggregate_point(const vg_grid_desc &gp, const vg
     const uint32_t ix = (uint32_t)(o % gp.np[0]);
     const uint32_t iy = (uint32_t)((o / gp.np[0]) % gp.np[1]);
     const uint32_t iz = (uint32_t)(o / ((size_t)gp.np[0] * gp.np[1]));
-    float px = 0.f, py = 0.f, pz = 0.f, wx = 0.f, wy = 0.f, wz = 0.f, sz = 0.f;
-    float aux = FLT_MIN;
-    uint32_t cmask = 0;
-    float D[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    ParentSums a;
     if (ix < gp.np[0] - 1 && iy < gp.np[1] - 1 && iz < gp.np[2] - 1) {
-        for (uint32_t cz = 0; cz < dz; ++cz)
-            for (uint32_t cy = 0; cy < dy; ++cy)
-                for (uint32_t cx = 0; cx < dx; ++cx) {
-                    const size_t co = point_offset(gc, ix * dx + cx, iy * dy + cy, iz * dz + cz);
-                    if (cc && cc[co] == 0) continue;
-                    const float4 p = Pc[co], w = Wc[co];
-                    const float vx = w.x * 2.0f, vy = w.y * 2.0f, vz = w.z * 2.0f;
-                    const float mag = sqrtf(vx * vx + vy * vy + vz * vz);
-                    px += p.x * mag; py += p.y * mag; pz += p.z * mag;
-                    wx += w.x; wy += w.y; wz += w.z;
-                    aux += mag;
-                    if (p.w != 0.0f) {
-                        sz = p.w;
-                        cmask |= 1u << (cx + dx * (cy + dy * cz));
+        if (dx == 2 && dy == 2 && dz == 2) {
+            // Eight children: their loads are all issued before the sums
+            // start (the general loop waits on one child's loads per trip,
+            // which sets the time of every small layer).  Children of an
+            // empty cell are loaded (in bounds, possibly never written) but
+            // not added.
+            const size_t base = point_offset(gc, ix * 2, iy * 2, iz * 2);
+            const size_t sy = gc.np[0], sz2 = (size_t)gc.np[0] * gc.np[1];
+            size_t co[8];
+            bool live[8];
+            float4 p[8], w[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) co[k] = base + (k & 1) + ((k >> 1) & 1) * sy + (k >> 2) * sz2;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                live[k] = !cc || cc[co[k]] != 0;
+                p[k] = Pc[co[k]];
+                w[k] = Wc[co[k]];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (live[k]) parent_add_child(a, p[k], w[k], (uint32_t)k);
+            const float inv = 1.0f / a.aux;
+            a.px *= inv; a.py *= inv; a.pz *= inv;
+            const float cp[3] = {a.px, a.py, a.pz};
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const float4 m0 = Mc[3 * co[k]], m1 = Mc[3 * co[k] + 1], m2 = Mc[3 * co[k] + 2];
+                if (live[k]) parent_add_moment(a, p[k], w[k], m0, m1, m2, cp);
+            }
+        } else {
+            for (uint32_t cz = 0; cz < dz; ++cz)
+                for (uint32_t cy = 0; cy < dy; ++cy)
+                    for (uint32_t cx = 0; cx < dx; ++cx) {
+                        const size_t co = point_offset(gc, ix * dx + cx, iy * dy + cy, iz * dz + cz);
+                        if (cc && cc[co] == 0) continue;
+                        parent_add_child(a, Pc[co], Wc[co], cx + dx * (cy + dy * cz));
                     }
-                }
-        const float inv = 1.0f / aux;
-        px *= inv; py *= inv; pz *= inv;
-        // first moment about the new centroid: D_p += D_c + Omega_c (c_c - c_p)
-        const float cp[3] = {px, py, pz};
-        for (uint32_t cz = 0; cz < dz; ++cz)
-            for (uint32_t cy = 0; cy < dy; ++cy)
-                for (uint32_t cx = 0; cx < dx; ++cx) {
-                    const size_t co = point_offset(gc, ix * dx + cx, iy * dy + cy, iz * dz + cz);
-                    if (cc && cc[co] == 0) continue;
-                    const float4 p = Pc[co], w = Wc[co];
-                    const float4 m0 = Mc[3 * co], m1 = Mc[3 * co + 1], m2 = Mc[3 * co + 2];
-                    const float cd[9] = {m0.x, m0.y, m0.z, m1.x, m1.y, m1.z, m2.x, m2.y, m2.z};
-                    const float cw[3] = {w.x, w.y, w.z}, cc[3] = {p.x, p.y, p.z};
-#pragma unroll
-                    for (int bb = 0; bb < 3; ++bb)
-#pragma unroll
-                        for (int a = 0; a < 3; ++a)
-                            D[3 * bb + a] += cd[3 * bb + a] + cw[bb] * (cc[a] - cp[a]);
-                }
+            const float inv = 1.0f / a.aux;
+            a.px *= inv; a.py *= inv; a.pz *= inv;
+            const float cp[3] = {a.px, a.py, a.pz};
+            for (uint32_t cz = 0; cz < dz; ++cz)
+                for (uint32_t cy = 0; cy < dy; ++cy)
+                    for (uint32_t cx = 0; cx < dx; ++cx) {
+                        const size_t co = point_offset(gc, ix * dx + cx, iy * dy + cy, iz * dz + cz);
+                        if (cc && cc[co] == 0) continue;
+                        parent_add_moment(a, Pc[co], Wc[co], Mc[3 * co], Mc[3 * co + 1],
+                                          Mc[3 * co + 2], cp);
+                    }
+        }
     } else {
-        aux = 0.f;
+        a.aux = 0.f;
     }
-    Pp[o] = make_float4(px, py, pz, sz);
-    Wp[o] = make_float4(wx, wy, wz, aux);
-    Cp[o] = cmask;
+    const float *D = a.D;
+    Pp[o] = make_float4(a.px, a.py, a.pz, a.sz);
+    Wp[o] = make_float4(a.wx, a.wy, a.wz, a.aux);
+    Cp[o] = a.cmask;
     Mp[3 * o] = make_float4(D[0], D[1], D[2], D[5] - D[7]);
     Mp[3 * o + 1] = make_float4(D[3], D[4], D[5], D[6] - D[2]);
     Mp[3 * o + 2] = make_float4(D[6], D[7], D[8], D[1] - D[3]);
