@@ -322,6 +322,7 @@ __global__ void k_geometry(const uint32_t *__restrict__ bbox, const float *__res
         }
     }
     TreeView &tv = views[e];
+    vg_grid_desc leaf;  // the leaf layer, for the mollifier
     if (shape.adaptive) {
         vg_grid_desc g0;
         bool clamp = false;
@@ -342,20 +343,24 @@ __global__ void k_geometry(const uint32_t *__restrict__ bbox, const float *__res
             if (clamp) precompute_spacing(&g0);
         }
         if (clamp) atomicAdd(clamped, 1u);
-        // NestedGrid layers of g0 (nested_init's steps), written straight
-        // into the view: layers past the depth keep stale descriptors, which
-        // nothing reads
-        decimate(&tv.layer[0], &g0, 1);
+        // NestedGrid layers of g0 (nested_init's steps), each decimated from
+        // the previous one held in registers and written into the view:
+        // layers past the depth keep stale descriptors, which nothing reads
+        vg_grid_desc cur;
+        decimate(&cur, &g0, 1);
+        tv.layer[0] = cur;
+        leaf_desc[e] = cur;
+        leaf = cur;
         uint32_t depth = 1;
-        while (capacity(tv.layer[depth - 1]) > 8 && depth < kMaxLayers) {
-            decimate(&tv.layer[depth], &tv.layer[depth - 1], 2);
+        while (capacity(cur) > 8 && depth < kMaxLayers) {
+            vg_grid_desc next;
+            decimate(&next, &cur, 2);
+            tv.layer[depth] = next;
+            for (int a = 0; a < 3; ++a) tv.dec[depth][a] = (cur.np[a] - 1) / (next.np[a] - 1);
+            cur = next;
             ++depth;
         }
         tv.depth = depth;
-        for (uint32_t l = 1; l < depth; ++l)
-            for (int a = 0; a < 3; ++a)
-                tv.dec[l][a] = (tv.layer[l - 1].np[a] - 1) / (tv.layer[l].np[a] - 1);
-        leaf_desc[e] = tv.layer[0];
     } else {
         for (uint32_t l = 0; l < shape.depth; ++l) {
             vg_grid_desc g;
@@ -366,11 +371,13 @@ __global__ void k_geometry(const uint32_t *__restrict__ bbox, const float *__res
             }
             precompute_spacing(&g);
             tv.layer[l] = g;
-            if (l == 0) leaf_desc[e] = g;
+            if (l == 0) {
+                leaf_desc[e] = g;
+                leaf = g;
+            }
         }
     }
-    const vg_grid_desc &g0 = tv.layer[0];
-    const float cell_volume = g0.cell[0] * g0.cell[1] * g0.cell[2];
+    const float cell_volume = leaf.cell[0] * leaf.cell[1] * leaf.cell[2];
     float s = pow_rounded(cell_volume, 1.0f / 3.0f) / r;
     if (s < 1.0f) s = 1.0f;
     mollify[e] = make_float2(s, 1.0f / (s * s * s));
