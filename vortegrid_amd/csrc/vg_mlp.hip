@@ -158,7 +158,7 @@ __global__ void k_reduce_split(const float *__restrict__ part, int nsplit, int M
     C[(size_t)m * ldc + n] = s;
 }
 
-constexpr int kSplitChunk = 256;  // K per block when the product is split
+constexpr int kSplitChunk = 64;  // K per block when the product is split
 constexpr int kMaxSplit = 64;
 
 // Products with a small output and a long K (the weight gradients, K = the
@@ -260,20 +260,51 @@ __global__ __launch_bounds__(256) void k_loss_grad(const float *__restrict__ Y,
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// Column sums of D [rows][cols] (cols divides 256), fixed order.
-__global__ __launch_bounds__(256) void k_colsum(const float *__restrict__ D, uint32_t rows, int cols,
-                                                float *__restrict__ out) {
+// Column sums of D [rows][cols] (cols divides 256) in a fixed order, in two
+// stages: workgroup b sums rows [b chunk, (b + 1) chunk) into part[b][cols],
+// then k_colsum_final adds the partials in workgroup order.
+constexpr uint32_t kColRows = 128;  // rows per workgroup unless the scratch needs more
+__global__ __launch_bounds__(256) void k_colsum_part(const float *__restrict__ D, uint32_t rows,
+                                                     int cols, uint32_t chunk,
+                                                     float *__restrict__ part) {
     __shared__ float red[256];
     const int t = threadIdx.x, c = t % cols, g = t / cols, ng = 256 / cols;
+    const uint32_t r0 = blockIdx.x * chunk;
+    const uint32_t r1 = rows - r0 < chunk ? rows : r0 + chunk;
     float s = 0.f;
-    for (uint32_t r = g; r < rows; r += ng) s += D[(size_t)r * cols + c];
+    for (uint32_t r = r0 + (uint32_t)g; r < r1; r += (uint32_t)ng) s += D[(size_t)r * cols + c];
     red[t] = s;
     __syncthreads();
     if (t < cols) {
         float a = 0.f;
         for (int j = 0; j < ng; ++j) a += red[j * cols + t];
-        out[t] = a;
+        part[(size_t)blockIdx.x * cols + t] = a;
     }
+}
+
+__global__ void k_colsum_final(const float *__restrict__ part, uint32_t nparts, int cols,
+                               float *__restrict__ out) {
+    const int t = threadIdx.x;
+    if (t >= cols) return;
+    float a = 0.f;
+    for (uint32_t b = 0; b < nparts; ++b) a += part[(size_t)b * cols + t];
+    out[t] = a;
+}
+
+// Column sums of D [rows][cols] into out, with `part` (kMaxSplit * 64 * 64
+// floats, the split products' scratch, free between products) holding the
+// per-workgroup partials.
+vg_status colsum(const float *D, uint32_t rows, int cols, float *out, float *part,
+                 hipStream_t s) {
+    VG_REQUIRE(cols > 0 && cols <= 64 && 256 % cols == 0, "column sum: cols must divide 256");
+    const size_t max_parts = (size_t)kMaxSplit * 64 * 64 / cols;
+    uint32_t chunk = kColRows;
+    if (vg::ceil_div(rows, chunk) > max_parts) chunk = vg::ceil_div(rows, max_parts);
+    const uint32_t nparts = vg::ceil_div(rows, chunk);
+    VG_TRY(vg::launch("vg_mlp_colsum", k_colsum_part, dim3(nparts), dim3(256), 0, s, D, rows, cols,
+                      chunk, part));
+    return vg::launch("vg_mlp_colsum", k_colsum_final, dim3(1), dim3(64), 0, s, (const float *)part,
+                      nparts, cols, out);
 }
 
 __global__ void k_sum_partials(const float *__restrict__ partial, uint32_t n, float *__restrict__ out) {
@@ -895,8 +926,7 @@ vg_status backward(vg_mlp *m, const float *X, const float *Bm, const float *T, u
                   nparts, a.loss));
     // output layer
     VG_TRY(gemm_tn(kHid, kOutPad, rows, a.A3, kHid, a.dY, kOutPad, G + kOffW4, kOutPad, s, a.split));
-    VG_TRY(launch("vg_mlp_colsum", k_colsum, dim3(1), dim3(256), 0, s, (const float *)a.dY, rows,
-                  kOutPad, G + kOffB4));
+    VG_TRY(colsum(a.dY, rows, kOutPad, G + kOffB4, a.split, s));
     VG_TRY(gemm_nt(rows, kHid, kOutPad, a.dY, kOutPad, P + kOffW4, kOutPad, a.dA, kHid, s));
     // hidden layers 3, 2, 1: dZ = dA * gelu'(Z), dW = A_prev^T dZ, dA_prev = dZ W^T
     struct L { float *Z; const float *Aprev; int lda_prev, kprev, offW, offB; };
@@ -909,8 +939,7 @@ vg_status backward(vg_mlp *m, const float *X, const float *Bm, const float *T, u
                       (const float *)L_.Z, h));
         VG_TRY(gemm_tn(L_.kprev, kHid, rows, L_.Aprev, L_.lda_prev, a.dA, kHid,
                        G + L_.offW, kHid, s, a.split));
-        VG_TRY(launch("vg_mlp_colsum", k_colsum, dim3(1), dim3(256), 0, s, (const float *)a.dA,
-                      rows, kHid, G + L_.offB));
+        VG_TRY(colsum(a.dA, rows, kHid, G + L_.offB, a.split, s));
         if (li < 2) {
             // dA_prev = dZ W^T, written over the previous layer's Z (its
             // pre-activation is still needed: use A_prev's slot instead)
