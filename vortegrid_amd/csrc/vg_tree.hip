@@ -215,6 +215,20 @@ __global__ void k_gather_sorted(VortonIn vin, const uint32_t *__restrict__ sorte
 // skip_empty: cells without vortons are not written (their previous values
 // stay); only for trees whose readers reach leaf clusters through the cell
 // counts or the parents' child masks (the batched walk), never by scanning.
+// kLeafSkipUnlessRead: per tree of `views`, skip them unless the
+// reference-rule walk of that tree scans leaf clusters (reads_empty_leaves).
+constexpr int kLeafSkipUnlessRead = 2;
+
+// Whether the reference-rule walk (tree_eval_split) reads the clusters of
+// empty leaf cells.  It reads layer 0 only as the children of layer-1
+// parents: through the parents' child masks when layer 1 decimates by 2 on
+// every axis (at most 2 parents per axis are ever open, so layer_masked
+// applies and visits non-empty children only), by enumeration otherwise.  A
+// single-layer tree sums the sorted vortons and reads no clusters.
+__host__ __device__ inline bool reads_empty_leaves(uint32_t depth, const uint32_t *dec1) {
+    return depth >= 2 && !(dec1[0] == 2 && dec1[1] == 2 && dec1[2] == 2);
+}
+
 __global__ void k_leaf(const uint32_t *__restrict__ start, const uint32_t *__restrict__ count,
                        const float4 *__restrict__ sp, const float4 *__restrict__ sw, size_t cap,
                        float4 *__restrict__ P, float4 *__restrict__ W, float4 *__restrict__ M,
@@ -222,7 +236,10 @@ __global__ void k_leaf(const uint32_t *__restrict__ start, const uint32_t *__res
 #pragma clang fp contract(off)
     size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (views) {
-        if (c >= capacity(views[blockIdx.y].layer[0])) return;
+        const TreeView &tv = views[blockIdx.y];
+        if (c >= capacity(tv.layer[0])) return;
+        if (skip_empty == kLeafSkipUnlessRead)
+            skip_empty = reads_empty_leaves(tv.depth, tv.dec[1]) ? 0 : 1;
         c += (size_t)blockIdx.y * cap;
     } else if (c >= cap) {
         return;
@@ -1112,12 +1129,13 @@ vg_status batch_tree_build(const BatchTreeArgs &a, hipStream_t s) {
         VG_TRY(launch("vg_batch_leaf", k_leaf, dim3(ceil_div(cap0, 256), a.trees), dim3(256), 0, s,
                       (const uint32_t *)a.cell_start, (const uint32_t *)a.cell_count,
                       (const float4 *)a.sorted_p, (const float4 *)a.sorted_w, cap0, a.P[0],
-                      a.W[0], a.M[0], a.views, a.all_leaves ? 0 : 1));
+                      a.W[0], a.M[0], a.views, a.all_leaves ? kLeafSkipUnlessRead : 1));
     else
         VG_TRY(launch("vg_batch_leaf", k_leaf, dim3(ceil_div(cells, 256)), dim3(256), 0, s,
                       (const uint32_t *)a.cell_start, (const uint32_t *)a.cell_count,
                       (const float4 *)a.sorted_p, (const float4 *)a.sorted_w, cells, a.P[0],
-                      a.W[0], a.M[0], (const TreeView *)nullptr, a.all_leaves ? 0 : 1));
+                      a.W[0], a.M[0], (const TreeView *)nullptr,
+                      a.all_leaves && reads_empty_leaves(ng.depth, ng.dec[1]) ? 0 : 1));
     for (uint32_t l = 1; l < ng.depth; ++l) {
         const size_t cap = capacity(ng.layer[l]);
         const uint32_t *cc = l == 1 ? (const uint32_t *)a.cell_count : nullptr;
