@@ -90,6 +90,7 @@ __device__ __forceinline__ float ord_dec(uint32_t u) {
 // the workgroup, so that the many workgroups of one large effect do not all
 // contend for the same six words; k_geometry reduces the copies.
 constexpr uint32_t kBoxSlots = 32;
+static_assert(kBoxSlots <= 64, "k_geometry folds the copies across one wave");
 
 __global__ void k_bbox_init(uint32_t *__restrict__ bbox, uint32_t effects) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -296,18 +297,31 @@ __global__ void k_geometry(const uint32_t *__restrict__ bbox, const float *__res
                            TreeView *__restrict__ views, vg_grid_desc *__restrict__ leaf_desc,
                            float2 *__restrict__ mollify, uint32_t *__restrict__ clamped) {
 #pragma clang fp contract(off)
-    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= effects) return;
+    // one wave per effect: lane c < kBoxSlots reads copy c of the extrema and
+    // the wave folds them (ordered encodings: min / max commute, so the
+    // result does not depend on the order); lane 0 then defines the grid
+    const uint32_t e = blockIdx.x, lane = threadIdx.x;
+    uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
+    if (lane < kBoxSlots) {
+        const uint32_t *box = bbox + 6 * ((size_t)e * kBoxSlots + lane);
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = box[a];
+            hi[a] = box[3 + a];
+        }
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = min(lo[a], (uint32_t)__shfl_xor((int)lo[a], m, 64));
+            hi[a] = max(hi[a], (uint32_t)__shfl_xor((int)hi[a], m, 64));
+        }
+    if (lane != 0) return;
     float mn[3], mx[3];
     bool empty = false;
     for (int a = 0; a < 3; ++a) {
-        uint32_t lo = 0xffffffffu, hi = 0u;  // ordered encodings: min / max commute
-        for (uint32_t c = 0; c < kBoxSlots; ++c) {
-            lo = min(lo, bbox[6 * ((size_t)e * kBoxSlots + c) + a]);
-            hi = max(hi, bbox[6 * ((size_t)e * kBoxSlots + c) + 3 + a]);
-        }
-        mn[a] = ord_dec(lo);
-        mx[a] = ord_dec(hi);
+        mn[a] = ord_dec(lo[a]);
+        mx[a] = ord_dec(hi[a]);
         empty = empty || !(mn[a] <= mx[a]);
     }
     const float r = radius[e];
@@ -525,7 +539,7 @@ vg_status solve(vg_batch *b, bool tracers_boxed, hipStream_t s) {
         for (int a = 0; a < 3; ++a) sh.np[l][a] = l < sh.depth ? b->shape.layer[l].np[a] : 0;
     sh.adaptive = b->adaptive ? 1u : 0u;
     sh.max_np = b->shape.layer[0].np[0];
-    VG_TRY(launch("vg_batch_geometry", k_geometry, dim3(ceil_div(b->effects, 64)), dim3(64), 0, s,
+    VG_TRY(launch("vg_batch_geometry", k_geometry, dim3(b->effects), dim3(64), 0, s,
                   (const uint32_t *)b->bbox, (const float *)b->radius,
                   (const uint32_t *)b->vcount, b->effects, sh, b->views, b->leaf_desc, b->mollify,
                   b->counters + 2));
