@@ -321,8 +321,9 @@ __device__ __forceinline__ void parent_add_moment(ParentSums &a, float4 p, float
 // AggregateClusters (vortonSim.cpp:1337-1569): parent point o of layer gp
 // from its dx x dy x dz children in layer gc.
 // cc (may be null): vorton counts of the child cells; children with none are
-// skipped without reading their clusters (they add exact zeros), so layer-0
-// clusters of empty cells need not be written (k_leaf skip_empty).
+// not added (they would add exact zeros; the 2x2x2 form loads their clusters
+// with the others' and drops them), so layer-0 clusters of empty cells need
+// not be written (k_leaf skip_empty).
 __device__ __forceinline__ void aggregate_point(const vg_grid_desc &gp, const vg_grid_desc &gc,
                                                 uint32_t dx, uint32_t dy, uint32_t dz, size_t o,
                                                 const float4 *__restrict__ Pc,
