@@ -73,9 +73,17 @@ __device__ __forceinline__ float tanh_fast(float u) {
     return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
 }
 
+// GELU(z) = z (1 + tanh u) / 2 with u = kGeluC (z + kGeluA z^3), written as
+// z - z / (1 + e^(2u)) and e^(2u) = 2^(z (g1 + g3 z^2)): six VALU operations
+// besides the exp and the reciprocal (against thirteen for the tanh form);
+// the limits stay exact (z for large z, where e^(2u) overflows; 0 for very
+// negative z).
+constexpr float kGeluG1 = 2.0f * kGeluC * 1.4426950408889634f;  // 2 sqrt(2/pi) log2(e)
+constexpr float kGeluG3 = kGeluG1 * kGeluA;
 __device__ __forceinline__ float gelu(float z) {
-    const float t = tanh_fast(kGeluC * (z + kGeluA * z * z * z));
-    return 0.5f * z * (1.0f + t);
+    const float q = z * fmaf(kGeluG3, z * z, kGeluG1);
+    const float r = __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(q) + 1.0f);
+    return fmaf(-z, r, z);
 }
 
 __device__ __forceinline__ float gelu_grad(float z) {
