@@ -521,7 +521,7 @@ __device__ __forceinline__ bf16x8 pack8(const float *v) {
 
 // One wave = one brick; for every accepted cluster of the masked layers the
 // brick's 64 points form one batch of 64 pairs (two 32-column tiles).
-__global__ __launch_bounds__(256) void k_mlp_grid(const vg::TreeView tv, vg_grid_desc g,
+__global__ __launch_bounds__(256, 2) void k_mlp_grid(const vg::TreeView tv, vg_grid_desc g,
                                                   MlpGridParams mp,
                                                   const __bf16 *__restrict__ packed,
                                                   const float *__restrict__ params,
@@ -595,6 +595,13 @@ __global__ __launch_bounds__(256) void k_mlp_grid(const vg::TreeView tv, vg_grid
                 S.M2[lane] = m2;
                 __builtin_amdgcn_wave_barrier();
                 while (mask) {
+                    // The weights and biases are re-read from LDS for every
+                    // cluster: hoisted out of this loop they hold ~190
+                    // registers and leave one wave per SIMD to hide the
+                    // MFMA, LDS and exp latencies alone.  The empty asm
+                    // (no instruction) tells the compiler that LDS may
+                    // change here, so it does not hoist them.
+                    __asm__ volatile("" ::: "memory");
                     const int j = __builtin_ctzll(mask);
                     mask &= mask - 1;
                     const float4 cp = S.P[j], cw = S.W[j];
