@@ -528,6 +528,11 @@ __global__ __launch_bounds__(256, 2) void k_mlp_grid(const vg::TreeView tv, vg_g
                                                   float *__restrict__ out) {
     __shared__ __attribute__((aligned(16))) __bf16 wts[kPackedElems];
     __shared__ float bias[3 * kHid + kOutPad];
+    // hidden-layer biases in accumulator order: binit[layer][r][grp] holds
+    // the 16 rows 32 r + 8 (v >> 2) + 4 grp + (v & 3) that element v of a
+    // 32x32 tile's accumulator covers on lane group grp; the products start
+    // from them instead of adding them afterwards
+    __shared__ f32x4 binit[3][2][2][4];
     __shared__ ClusterStage stage[4];
     {
         const uint4 *src = (const uint4 *)packed;
@@ -539,6 +544,12 @@ __global__ __launch_bounds__(256, 2) void k_mlp_grid(const vg::TreeView tv, vg_g
                     : layer == 1 ? params[kOffB2 + i - kHid]
                     : layer == 2 ? params[kOffB3 + i - 2 * kHid]
                                  : params[kOffB4 + i - 3 * kHid];
+        }
+        for (int i = threadIdx.x; i < 3 * 2 * 2 * 16; i += 256) {
+            const int v = i & 15, grp_i = (i >> 4) & 1, r = (i >> 5) & 1, layer = i >> 6;
+            const int row = 32 * r + 8 * (v >> 2) + 4 * grp_i + (v & 3);
+            const int off = layer == 0 ? kOffB1 : layer == 1 ? kOffB2 : kOffB3;
+            binit[layer][r][grp_i][v >> 2][v & 3] = params[off + row];
         }
     }
     __syncthreads();
@@ -611,18 +622,30 @@ __global__ __launch_bounds__(256, 2) void k_mlp_grid(const vg::TreeView tv, vg_g
                     cluster_features(fa, nullptr, qa, cp, cw, a0, a1, a2, inv_h);
                     cluster_features(fb, nullptr, qb, cp, cw, a0, a1, a2, inv_h);
                     const bf16x8 xa = pack8(fa + 8 * grp), xb = pack8(fb + 8 * grp);
+                    auto bias16 = [&](int layer, int r) {
+                        f32x16 b;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const f32x4 t = binit[layer][r][grp][q];
+                            b[4 * q] = t[0];
+                            b[4 * q + 1] = t[1];
+                            b[4 * q + 2] = t[2];
+                            b[4 * q + 3] = t[3];
+                        }
+                        return b;
+                    };
                     f32x16 acc[2][2];
 #pragma unroll
                     for (int r = 0; r < 2; ++r) {
                         const bf16x8 wv = W8[(kBlkL1 + r) * 64 + lane];
-                        const f32x16 z = {};
-                        acc[r][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv, xa, z, 0, 0, 0);
-                        acc[r][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv, xb, z, 0, 0, 0);
+                        const f32x16 b0 = bias16(0, r);
+                        acc[r][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv, xa, b0, 0, 0, 0);
+                        acc[r][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv, xb, b0, 0, 0, 0);
                     }
                     bf16x8 h[2][4];
 #pragma unroll
                     for (int layer = 0; layer < 3; ++layer) {
-                        // bias + GELU on the accumulators, repacked as the next inputs
+                        // GELU on the accumulators (biases included), repacked as the next inputs
 #pragma unroll
                         for (int c = 0; c < 2; ++c)
 #pragma unroll
@@ -630,9 +653,7 @@ __global__ __launch_bounds__(256, 2) void k_mlp_grid(const vg::TreeView tv, vg_g
                                 float v8[8];
 #pragma unroll
                                 for (int t8 = 0; t8 < 8; ++t8) {
-                                    const int v = 8 * (b & 1) + t8;
-                                    const int row = 32 * (b >> 1) + 8 * (v >> 2) + 4 * (int)grp + (v & 3);
-                                    v8[t8] = gelu(acc[b >> 1][c][v] + bias[layer * kHid + row]);
+                                    v8[t8] = gelu(acc[b >> 1][c][8 * (b & 1) + t8]);
                                 }
                                 h[c][b] = pack8(v8);
                             }
@@ -640,7 +661,7 @@ __global__ __launch_bounds__(256, 2) void k_mlp_grid(const vg::TreeView tv, vg_g
                         const int blk0 = layer == 0 ? kBlkL2 : kBlkL3;
 #pragma unroll
                         for (int r = 0; r < 2; ++r) {
-                            f32x16 s0 = {}, s1 = {};
+                            f32x16 s0 = bias16(layer + 1, r), s1 = s0;
 #pragma unroll
                             for (int b = 0; b < 4; ++b) {
                                 const bf16x8 wv = W8[(blk0 + r * 4 + b) * 64 + lane];
